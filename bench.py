@@ -1,0 +1,210 @@
+"""Benchmark of the iterative-solve path on one MI355X.
+
+Workload: the SPE10-shaped 60x220x85 system (1,122,000 cells, 7,780,000
+nonzeros) assembled with the reference's own formula (modules/mesh/spe.cc:69-131,
+reaction coefficient c = 1) on a seeded synthetic permeability field, solved by
+PCG with the default uh_cheb preconditioner (sigma 3, 2 Chebyshev steps) from
+x0 = 0 to a relative residual of 1e-8.  A step is one complete solve with the
+matrix, hierarchy and right-hand side already resident in HBM.
+
+Prints one JSON line: the mixed-precision solve time (FP64 outer flexible PCG,
+FP32 hierarchy) as `value`, the FP64 solve time beside it, the SpMV roofline
+(cold caches) and the single-core CPU oracle timed on the same host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "PCG time to 1e-8 rel-res @1.12M cells, mixed vs FP64 GPU; SpMV HBM GB/s"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def load_traffic():
+    """HBM bytes per SpMV launch from the committed rocprofv3 PMC summary, if present."""
+    p = os.path.join(ROOT, "profiles", "spmv_cfg1_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            d = json.load(f)
+        return d.get("traffic_bytes_per_launch")
+    return None
+
+
+def cpu_baseline(s, eps, budget_s=20.0):
+    """The C oracle (single thread) on the same system: construction + PCG from x0 = 0."""
+    import oracle as O
+    t0 = time.perf_counter()
+    P = O.cheb_prec(s.ia, s.ja, s.a)
+    t_const = time.perf_counter() - t0
+    times, iters = [], 0
+    while True:
+        t0 = time.perf_counter()
+        x, st = O.pcg(s.ia, s.ja, s.a, s.b, P, eps=eps, x0=np.zeros(s.n), maxit=10000)
+        times.append(time.perf_counter() - t0)
+        iters = st["niter"]
+        if sum(times) + t_const > budget_s / 2 or len(times) >= 5:
+            break
+    return {"value": 1e3 * float(np.median(times)), "unit": "ms per solve", "cores": 1, "kind": "port",
+            "sample": f"{len(times)} full PCG solves of the same 60x220x85 system (x0=0, eps={eps:g}, "
+                      f"{iters} iterations each) by the C oracle on one host core; construction {t_const:.2f} s",
+            "t_const_s": t_const, "niter": iters}, x
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--eps", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--nz", type=int, default=85)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (each system is solved on one "
+              "device; the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from ml_prec_amd import native as N
+    from ml_prec_amd import spe_synth as S
+
+    t0 = time.perf_counter()
+    s = S.config_system(1, kind="spe", c=1.0, nz=args.nz)
+    log(f"system {s.nx}x{s.ny}x{s.nz}: n={s.n} nnz={s.nnz} generated in {time.perf_counter() - t0:.2f}s")
+    t0 = time.perf_counter()
+    H = N.build_uh_cheb(s.ia, s.ja, s.a)
+    t_const = time.perf_counter() - t0
+    levels = [H.level_info(l) for l in range(H.nlevels)]
+    log(f"hierarchy: {H.nlevels} levels, N = {[l['N'] for l in levels]}, built in {t_const:.2f}s")
+
+    ctx = N.Context(0)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B64 = N.DevPrec(ctx, A, H, precision=N.MLP_FP64)
+    B32 = N.DevPrec(ctx, A, H, precision=N.MLP_FP32)
+    b = ctx.to_device(s.b)
+    x = ctx.array(s.n)
+
+    def solve_fp64():
+        return N.pcg(ctx, A, B64, b, x, eps=args.eps, maxit=10000, use_graph=bool(args.graph), check_every=8)
+
+    def solve_mixed():
+        return N.pcg(ctx, A, B32, b, x, eps=args.eps, maxit=10000, flexible=True, true_res_every=20,
+                     use_graph=bool(args.graph), check_every=8)
+
+    for _ in range(args.warmup):
+        solve_fp64()
+        solve_mixed()
+    ctx.sync()
+    log("warmup done")
+
+    ctx.sync()
+    t0 = time.perf_counter()
+    st64 = [solve_fp64() for _ in range(args.steps)]
+    ctx.sync()
+    ms64 = 1e3 * (time.perf_counter() - t0) / args.steps
+    x64 = x.get()
+    log(f"FP64: {ms64:.3f} ms/solve, {st64[-1]['niter']} iterations, true rel res {st64[-1]['true_rel_res']:.3e}")
+
+    ctx.sync()
+    t0 = time.perf_counter()
+    stm = [solve_mixed() for _ in range(args.steps)]
+    ctx.sync()
+    msm = 1e3 * (time.perf_counter() - t0) / args.steps
+    xm = x.get()
+    err_mixed = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+    log(f"mixed: {msm:.3f} ms/solve, {stm[-1]['niter']} iterations, true rel res {stm[-1]['true_rel_res']:.3e}, "
+        f"rel err vs FP64 GPU {err_mixed:.3e}")
+
+    # SpMV roofline: algorithmic bytes (w+4)*nnz + 4(N+1) + 2wN, cold L2 and Infinity Cache
+    bytes_spmv = 12.0 * s.nnz + 4.0 * (s.n + 1) + 16.0 * s.n
+    xv = ctx.to_device(np.random.default_rng(0).standard_normal(s.n))
+    yv = ctx.array(s.n)
+    spmv_cold_ms = A.time_spmv(xv, yv, reps=args.spmv_reps, flush_bytes=1 << 30)
+    spmv_warm_ms = A.time_spmv(xv, yv, reps=args.spmv_reps, flush_bytes=0)
+    ach = bytes_spmv / (spmv_cold_ms * 1e-3) / 1e9
+    log(f"SpMV: cold {spmv_cold_ms * 1e3:.1f} us ({ach:.0f} GB/s), warm {spmv_warm_ms * 1e3:.1f} us "
+        f"({bytes_spmv / (spmv_warm_ms * 1e-3) / 1e9:.0f} GB/s)")
+    f = ctx.to_device(np.random.default_rng(1).standard_normal(s.n))
+    z = ctx.array(s.n)
+    apply64_ms = B64.time_apply(f, z, reps=20)
+    apply32_ms = B32.time_apply(f, z, reps=20)
+    by64, by32 = B64.bytes_per_apply(), B32.bytes_per_apply()
+    log(f"prec apply: FP64 {apply64_ms * 1e3:.1f} us ({by64 / apply64_ms / 1e6:.0f} GB/s alg), "
+        f"FP32 {apply32_ms * 1e3:.1f} us ({by32 / apply32_ms / 1e6:.0f} GB/s alg)")
+
+    cpu = None
+    err_cpu = None
+    if not args.no_cpu:
+        log("timing the CPU oracle ...")
+        cpu, xcpu = cpu_baseline(s, args.eps)
+        err_cpu = float(np.linalg.norm(x64 - xcpu) / np.linalg.norm(xcpu))
+        log(f"CPU oracle: {cpu['value']:.1f} ms/solve ({cpu['niter']} iterations); FP64 GPU vs CPU rel diff "
+            f"{err_cpu:.3e}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(msm, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(msm, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"SPE10-shaped {s.nx}x{s.ny}x{s.nz} ({s.n} cells, {s.nnz} nnz), reference spe.cc "
+                        "assembly c=1 on a seeded channelised field, uh_cheb sigma=3 ncheb=2, PCG x0=0 to "
+                        f"{args.eps:g} relative residual",
+            "levels": H.nlevels,
+            "level_sizes": [l["N"] for l in levels],
+            "mixed": "FP64 flexible PCG, FP32 hierarchy, true residual every 20 iterations",
+        },
+        "fp64_ms_per_solve": round(ms64, 4),
+        "speedup_mixed_vs_fp64": round(ms64 / msm, 3),
+        "niter_fp64": int(st64[-1]["niter"]),
+        "niter_mixed": int(stm[-1]["niter"]),
+        "true_rel_res_fp64": st64[-1]["true_rel_res"],
+        "true_rel_res_mixed": stm[-1]["true_rel_res"],
+        "mixed_rel_err_vs_fp64": err_mixed,
+        "fp64_rel_diff_vs_cpu_oracle": err_cpu,
+        "spmv_us_cold": round(spmv_cold_ms * 1e3, 3),
+        "spmv_us_warm": round(spmv_warm_ms * 1e3, 3),
+        "spmv_gbs_warm_algorithmic": round(bytes_spmv / (spmv_warm_ms * 1e-3) / 1e9, 1),
+        "prec_apply_us": {"fp64": round(apply64_ms * 1e3, 2), "fp32": round(apply32_ms * 1e3, 2)},
+        "prec_apply_gbs_algorithmic": {"fp64": round(by64 / apply64_ms / 1e6, 1),
+                                       "fp32": round(by32 / apply32_ms / 1e6, 1)},
+        "t_const_host_s": round(t_const, 3),
+        "roofline": {
+            "kernel": "k_spmv (FP64 sliced-ELL SpMV on the 1.12M-cell matrix, caches flushed between launches)",
+            "bound": "hbm",
+            "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic(),
+            "algorithmic_bytes_per_launch": bytes_spmv,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

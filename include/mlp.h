@@ -1,0 +1,205 @@
+/*
+ * mlp.h -- C ABI of the MI355X (gfx950) implementation of ml_prec's
+ * iterative-solve path: CSR SpMV, the multilevel uh_cheb / multi_split
+ * preconditioner apply, PCG and GMRES with their BLAS-1 reductions.
+ *
+ * Conventions
+ *   - every entry point returns int status (MLP_OK = 0) and, on failure,
+ *     leaves a message for mlp_last_error() (thread-local); no C++ exception
+ *     crosses this boundary;
+ *   - pointers named d_* are device pointers (HBM); pointers without the
+ *     prefix are host pointers;
+ *   - all work of a context is ordered on the context's HIP stream; a context
+ *     is not thread-safe, like the reference's solvers (whose scratch vectors
+ *     are `mutable` members, modules/prec/cheb/cheb_prec.h:49-50);
+ *   - matrices use the reference's Skyline CSR layout: uint32 ia[n+1],
+ *     uint32 ja[nnz], double a[nnz], diagonal first in every row, then
+ *     ascending columns (modules/matrix/matrix.h:187-291).
+ *
+ * Each declaration cites the reference interface it replaces.
+ */
+#ifndef ML_PREC_MLP_H
+#define ML_PREC_MLP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference throws Exception for the same cases) ---- */
+#define MLP_OK              0
+#define MLP_ERR_INVALID     1   /* bad argument / size mismatch (ASSERT_SIZE, include/exception.h) */
+#define MLP_ERR_HIP         2   /* HIP runtime failure */
+#define MLP_ERR_NOT_SPD     3   /* "<Ap, p> <= 0"            modules/solvers/pcg.cc:51-56 */
+#define MLP_ERR_NO_ITER     4   /* "No need to iterate"      modules/solvers/pcg.cc:94-96 */
+#define MLP_ERR_MAXIT       5   /* iteration cap reached (no counterpart: the reference loops on) */
+#define MLP_ERR_CONSTRUCT   6   /* hierarchy construction ("Too many levels", ...) */
+#define MLP_ERR_CONFIG      7   /* prec config file errors   multi_split_constr.cc:204-228 */
+
+/* ---- precision of a device preconditioner hierarchy ---- */
+#define MLP_FP64 0
+#define MLP_FP32 1
+
+/* ---- preconditioner kinds ---- */
+#define MLP_PREC_UH_CHEB     1  /* Prec, "uh_cheb"           modules/prec/cheb/cheb_prec.h:27 */
+#define MLP_PREC_MULTI_SPLIT 2  /* MultiSplitPrec            modules/prec/multi_split/multi_split_prec.h:15 */
+#define MLP_PREC_DIAG        3  /* DiagPrec                  modules/prec/diag/diag_prec.cc:7-23 */
+#define MLP_PREC_ID          4  /* IdPrec                    modules/prec/id/id_prec.h:6-19 */
+
+/* ---- multi_split elimination orders (ConfigFile key elim_order) ---- */
+#define MLP_ORDER_NONE     0
+#define MLP_ORDER_ORIGINAL 1
+#define MLP_ORDER_BLOCK    2
+#define MLP_ORDER_SIMPLE_1 3
+
+typedef struct mlp_ctx_s*  mlp_ctx;    /* device, stream, scratch */
+typedef struct mlp_mat_s*  mlp_mat;    /* a matrix resident in HBM */
+typedef struct mlp_hier_s* mlp_hier;   /* a hierarchy built on the host */
+typedef struct mlp_prec_s* mlp_prec;   /* a hierarchy resident in HBM */
+
+const char* mlp_last_error(void);
+
+/* ---- context ---- */
+int mlp_ctx_create(int device, mlp_ctx* out);
+int mlp_ctx_destroy(mlp_ctx ctx);
+int mlp_ctx_sync(mlp_ctx ctx);
+int mlp_ctx_stream(mlp_ctx ctx, void** hip_stream_out);
+
+/* ---- device memory helpers (for callers without their own allocator) ---- */
+int mlp_dev_alloc(mlp_ctx ctx, size_t bytes, void** d_out);
+int mlp_dev_free(mlp_ctx ctx, void* d_ptr);
+int mlp_copy_h2d(mlp_ctx ctx, void* d_dst, const void* src, size_t bytes);
+int mlp_copy_d2h(mlp_ctx ctx, void* dst, const void* d_src, size_t bytes);
+
+/* ---- matrices ----
+ * mlp_csr_upload replaces holding a CSRMatrix/SkylineMatrix for the solve
+ * (modules/matrix/matrix.h:187-291); the device copy is a sliced-ELL layout
+ * (64-row slices, column-major within a slice) with the same per-row order. */
+int mlp_csr_upload(mlp_ctx ctx, uint32_t n, const uint32_t* ia, const uint32_t* ja,
+                   const double* a, mlp_mat* out);
+int mlp_mat_destroy(mlp_mat m);
+int mlp_mat_info(mlp_mat m, uint32_t* n, uint32_t* nnz, uint64_t* device_bytes);
+/* y = A x: multiply(A, v, res, 'o')   modules/matrix/csrmatrix.cc:260-291 */
+int mlp_spmv(mlp_ctx ctx, mlp_mat A, const double* d_x, double* d_y);
+/* r = b - A x: residual(A, b, x, r)   modules/matrix/csrmatrix.cc:293-300 */
+int mlp_residual(mlp_ctx ctx, mlp_mat A, const double* d_b, const double* d_x, double* d_r);
+
+/* ---- BLAS-1: modules/vector/blas.cc:57-82 (ddot, dnrm2, daxpy, dscal) ----
+ * daxpy/dscal keep the reference's alpha == 1 shortcuts (blas.cc:43-49, 77-82). */
+int mlp_ddot(mlp_ctx ctx, uint32_t n, const double* d_x, const double* d_y, double* out);
+int mlp_dnrm2(mlp_ctx ctx, uint32_t n, const double* d_x, double* out);
+int mlp_daxpy(mlp_ctx ctx, uint32_t n, double alpha, const double* d_x, double* d_y);
+int mlp_dscal(mlp_ctx ctx, uint32_t n, double alpha, double* d_x);
+
+/* ---- host construction of a hierarchy ----
+ * Prec(const SkylineMatrix&, const Config&)       modules/prec/cheb/cheb_constr.cc:163-248 */
+int mlp_hier_build_uh_cheb(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
+                           const double* sigmas, uint32_t nsigmas,
+                           const uint32_t* niters, uint32_t nniters,
+                           int use_tails, mlp_hier* out);
+
+typedef struct {
+    const double* q;        uint32_t nq;          /* key q            */
+    const uint32_t* niters; uint32_t nniters;     /* key level_niter  */
+    const double* eps;      uint32_t neps;        /* key level_eps    */
+    int inner_dynamic;                            /* key inner_iter   */
+    int order;                                    /* key elim_order   */
+    int use_tails;                                /* Config.use_tails */
+    uint32_t degree_max;                          /* key degree_max   */
+    uint32_t max_levels;                          /* key max_levels   */
+    uint32_t coarse_n;                            /* key coarse_n     */
+} mlp_msplit_params;
+
+/* MultiSplitPrec(const SkylineMatrix&, const Config&)  multi_split_constr.cc:170-299.
+ * config_path != NULL: read the prec config file with the reference's keys and
+ * defaults (modules/config_file/config_file.h:54-116); params then only supply
+ * use_tails.  A missing file keeps params, as the reference keeps its CLI values. */
+int mlp_hier_build_multi_split(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
+                               const char* config_path, const mlp_msplit_params* params,
+                               mlp_hier* out);
+int mlp_hier_destroy(mlp_hier h);
+int mlp_hier_nlevels(mlp_hier h, uint32_t* out);
+
+typedef struct {
+    uint32_t N, nnz, M, Md;        /* level sizes (Level::N, nnz, M, Md) */
+    uint32_t nit;                  /* ncheb (uh_cheb) or niter (multi_split) */
+    uint32_t nnz_L, nnz_U;
+    uint32_t n_next, nnz_next;     /* the Schur complement A_{l+1} */
+    double alpha, beta_or_q, lmin, lmax, eps;
+} mlp_level_info;
+int mlp_hier_level_info(mlp_hier h, uint32_t level, mlp_level_info* out);
+/* which: 0 map, 1 L.ia, 2 L.ja, 3 L.a, 4 U.ia, 5 U.ja, 6 U.a, 7 dval,
+ *        8 A_next.ia, 9 A_next.ja, 10 A_next.a, 11 rmap, 12 aux */
+int mlp_hier_level_array(mlp_hier h, uint32_t level, int which, void* out);
+
+/* ---- device preconditioners ----
+ * A hierarchy as plain per-level arrays: what Prec::Level / MultiSplitPrec::Level
+ * hold (cheb_prec.h:31-54, multi_split_prec.h:19-44).  A_next is levels[l+1].A. */
+typedef struct {
+    uint32_t N, M, Md;
+    uint32_t nit;                            /* ncheb or niter */
+    double lmin, lmax;                       /* this level's spectral bounds (uh_cheb) */
+    double eps;                              /* multi_split level_eps (0 = fixed) */
+    const uint32_t* map;                     /* N */
+    const uint32_t *L_ia, *L_ja; const double* L_a;   /* N-Md rows */
+    const uint32_t *U_ia, *U_ja; const double* U_a;   /* M rows, pivot first */
+    const double* dval;                      /* Md */
+    const uint32_t *A_ia, *A_ja; const double* A_a;   /* this level's matrix (unused for level 0) */
+} mlp_level_desc;
+
+int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_desc* levels,
+                    mlp_mat A0, int precision, mlp_prec* out);
+int mlp_prec_from_hier(mlp_ctx ctx, mlp_hier h, mlp_mat A0, int precision, mlp_prec* out);
+int mlp_prec_simple(mlp_ctx ctx, int kind, mlp_mat A0, mlp_prec* out);   /* DIAG or ID */
+int mlp_prec_destroy(mlp_prec p);
+/* x = B f: PrecBase::solve(f, x)   modules/prec/prec_base.h:8-12,
+ * Prec::solve cheb_solve.cc:9-123, MultiSplitPrec::solve multi_split_solve.cc:59-157 */
+int mlp_prec_apply(mlp_ctx ctx, mlp_prec p, const double* d_f, double* d_x);
+
+/* ---- solvers: PCGSolver / GMRESSolver   modules/solvers/solvers.h:31-38 ---- */
+#define MLP_X0_ZERO      0   /* x0 = 0 */
+#define MLP_X0_GIVEN     1   /* x0 = d_x on entry */
+#define MLP_X0_REFERENCE 2   /* generate_x0: srandom(3), x = 20(random()-RAND_MAX/2)/RAND_MAX + 100
+                                modules/solvers/solvers.cc:9-21 */
+typedef struct {
+    double eps;               /* relative tolerance on the recurrence residual */
+    uint32_t maxit;           /* 0 = no cap */
+    int x0_mode;
+    int flexible;             /* PCG: Polak-Ribiere beta (for an inexact preconditioner) */
+    uint32_t true_res_every;  /* PCG: recompute r = b - A x every k iterations (0 = never) */
+    uint32_t check_every;     /* iterations enqueued between host convergence checks (0 = 16) */
+    int use_graph;            /* capture one iteration in a hipGraph and replay it */
+} mlp_solve_opts;
+
+typedef struct {
+    uint32_t niter;
+    double init_norm, final_norm, eps_used;
+    double true_rel_res;      /* ||b - A x|| / ||b - A x0|| in FP64 after the solve */
+    double t_solve_ms;        /* wall time of the solve on the host clock */
+    int status;
+} mlp_solve_stats;
+
+int mlp_pcg(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, double* d_x,
+            const mlp_solve_opts* opts, mlp_solve_stats* stats);
+int mlp_gmres(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, double* d_x,
+              const mlp_solve_opts* opts, mlp_solve_stats* stats);
+
+/* ---- diagnostics (no counterpart in the reference; used for roofline reporting) ---- */
+/* algorithmic HBM bytes of one apply (sum over level visits) */
+int mlp_prec_bytes(mlp_prec p, double* bytes_per_apply, uint32_t* launches_per_apply);
+/* Average device time of `reps` SpMV launches y = A x (HIP events on the context
+ * stream around each launch).  flush_bytes > 0 streams a buffer of that size
+ * between launches so that A and x are read from HBM, not from the 256 MB
+ * Infinity Cache. */
+int mlp_time_spmv(mlp_ctx ctx, mlp_mat A, const double* d_x, double* d_y, uint32_t reps,
+                  size_t flush_bytes, double* avg_ms);
+/* Average device time of `reps` preconditioner applies x = B f. */
+int mlp_time_prec_apply(mlp_ctx ctx, mlp_prec p, const double* d_f, double* d_x, uint32_t reps,
+                        double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1264 @@
+// C ABI of the MI355X iterative-solve path: device matrices, device
+// preconditioner hierarchies, PCG and GMRES.  See include/mlp.h.
+#include "../../include/mlp.h"
+
+#include "hierarchy.h"
+#include "kernels.h"
+#include "mlp_error.h"
+#include "plan.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace mlp {
+
+thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+namespace {
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
+inline bool host_is_one(double a) { return std::fabs(a - 1.0) <= 2.220446049250313e-16; }   // include/tools.h:28-30
+
+// ---------------------------------------------------------------------------
+// device memory
+// ---------------------------------------------------------------------------
+struct DevMem {
+    std::vector<void*> ptrs;
+    ~DevMem() { release(); }
+    void release() {
+        for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+    }
+    template <class T> T* alloc(size_t n) {
+        void* p = nullptr;
+        MLP_HIP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    // synchronous: the host vectors are usually temporaries
+    template <class T> T* upload(const std::vector<T>& v, hipStream_t s) {
+        T* d = alloc<T>(v.size());
+        if (!v.empty()) {
+            MLP_HIP_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+            MLP_HIP_CHECK(hipStreamSynchronize(s));
+        }
+        return d;
+    }
+    template <class T, class S> T* upload_as(const std::vector<S>& v, hipStream_t s) {
+        std::vector<T> c(v.begin(), v.end());
+        return upload(c, s);
+    }
+};
+
+template <class VT> struct DevSell {
+    SellView<VT> v;
+    uint64_t nent = 0;
+    uint32_t nnz = 0;
+};
+
+template <class VT>
+void upload_sell(DevMem& mem, hipStream_t s, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
+                 DevSell<VT>& out) {
+    SellLayout L;
+    plan_sell(n, ia, ja, a, L);
+    out.v.n = n;
+    out.v.off64 = mem.upload(L.off64, s);
+    out.v.col = mem.upload(L.col, s);
+    out.v.val = mem.upload_as<VT>(L.val, s);
+    out.nent = L.nent;
+    out.nnz = n ? ia[n] : 0;
+    MLP_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+}  // namespace mlp
+
+// ---------------------------------------------------------------------------
+// handle types
+// ---------------------------------------------------------------------------
+struct mlp_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    mlp::DevMem mem;
+    double* partials = nullptr;        // kRedBlocks * kPartialsPerBlock
+    unsigned* counters = nullptr;      // 8
+    double* scal = nullptr;            // 4 result slots for standalone BLAS-1
+    mlp::SolveState* st = nullptr;
+    double* host_pinned = nullptr;     // 64 doubles
+    // solver workspace, grown on demand and kept for the next solve
+    uint32_t ws_n = 0;
+    mlp::DevMem ws_mem;
+    double *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *zp = nullptr, *V = nullptr;
+    double* mdot = nullptr;            // 32 results of the Gram-Schmidt dots
+};
+
+struct mlp_mat_s {
+    mlp_ctx ctx = nullptr;
+    uint32_t n = 0, nnz = 0;
+    mlp::DevMem mem;
+    mlp::DevSell<double> A;
+    std::vector<uint32_t> ia, ja;      // host copy for diagnostics and DiagPrec
+    std::vector<double> a;
+    uint64_t bytes = 0;
+};
+
+struct mlp_hier_s {
+    mlp::HostCSR A0;
+    mlp::Hierarchy H;
+};
+
+namespace mlp {
+namespace {
+
+// ---------------------------------------------------------------------------
+// device hierarchy
+// ---------------------------------------------------------------------------
+template <class VT> struct DevLevel {
+    uint32_t N = 0, M = 0, Md = 0, n = 0, nit = 0;
+    double eps = 0.0;
+    // Chebyshev coefficients for the iteration on the next level (cheb_solve.cc:53-101)
+    double a1 = 0; int s1_skip = 0;
+    double ma2 = 0; int s2_skip = 0; double opb2 = 0; int opb2_one = 0;
+    std::vector<double> ak, bk;               // steps 3..ncheb
+    uint32_t* map = nullptr;
+    TailView<VT> tail;
+    uint32_t tail_smem = 0;
+    uint32_t *Sptr = nullptr, *Scol = nullptr;
+    VT* Sval = nullptr;
+    VT* dval = nullptr;
+    DevSell<VT> A;                             // this level's matrix (levels >= 1)
+    VT *wt = nullptr, *F = nullptr, *tmp = nullptr, *u1 = nullptr, *u0 = nullptr, *x2 = nullptr;
+    VT *r = nullptr, *uc = nullptr;           // multi_split
+    // bookkeeping for the byte model
+    uint32_t nnzL = 0, nnzU = 0;
+};
+
+struct DevPrecBase {
+    virtual ~DevPrecBase() {}
+    virtual void apply(const double* f, double* x, const uint32_t* guard) = 0;
+    virtual double bytes_per_apply() const = 0;
+    virtual uint32_t launches_per_apply() const = 0;
+    virtual bool graph_safe() const { return true; }
+};
+
+template <class VT> struct DevHier : DevPrecBase {
+    mlp_ctx ctx;
+    int kind;
+    uint32_t nlevels = 0;
+    std::vector<DevLevel<VT>> lv;
+    DevMem mem;
+    mlp_mat A0 = nullptr;
+    mutable uint32_t launches = 0;
+    bool dynamic = false;
+
+    hipStream_t stream() const { return ctx->stream; }
+
+    void build(const std::vector<mlp_level_desc>& d, int kind_, mlp_mat A0_) {
+        kind = kind_;
+        A0 = A0_;
+        nlevels = uint32_t(d.size());
+        lv.resize(nlevels);
+        hipStream_t s = stream();
+        for (uint32_t l = 0; l < nlevels; l++) {
+            const mlp_level_desc& D = d[l];
+            DevLevel<VT>& L = lv[l];
+            L.N = D.N; L.M = D.M; L.Md = D.Md; L.nit = D.nit; L.eps = D.eps;
+            if (L.M + L.Md > L.N) throw Error("level sizes inconsistent", 1);
+            L.n = L.N - L.M - L.Md;
+            if (l + 1 == nlevels && L.n != 0) throw Error("the coarsest level must have no Schur part", 1);
+            if (l + 1 < nlevels && d[l + 1].N != L.n) throw Error("level sizes do not chain", 1);
+            if (L.eps != 0.0) dynamic = true;
+            LevelArrays la;
+            la.N = D.N; la.M = D.M; la.Md = D.Md; la.map = D.map;
+            la.L_ia = D.L_ia; la.L_ja = D.L_ja; la.L_a = D.L_a;
+            la.U_ia = D.U_ia; la.U_ja = D.U_ja; la.U_a = D.U_a; la.dval = D.dval;
+            std::vector<uint32_t> mapv(D.map, D.map + D.N);
+            L.map = mem.upload(mapv, s);
+            L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
+            L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
+            TailLayout T;
+            plan_tail(la, 256, T);
+            if (T.M) {
+                L.tail.nblk = T.nblk;
+                L.tail.blk = mem.upload(T.blk, s);
+                L.tail.lvlL = mem.upload(T.lvlL, s);
+                L.tail.lvlU = mem.upload(T.lvlU, s);
+                L.tail.fidx = mem.upload(T.fidx, s);
+                L.tail.Lptr = mem.upload(T.Lptr, s);
+                L.tail.Lloc = mem.upload(T.Lloc, s);
+                L.tail.Lval = mem.upload_as<VT>(T.Lval, s);
+                L.tail.uloc = mem.upload(T.uloc, s);
+                L.tail.uxidx = mem.upload(T.uxidx, s);
+                L.tail.Uptr = mem.upload(T.Uptr, s);
+                L.tail.Ucol = mem.upload(T.Ucol, s);
+                L.tail.Uval = mem.upload_as<VT>(T.Uval, s);
+                L.tail.Udiag = mem.upload_as<VT>(T.Udiag, s);
+                L.tail_smem = T.max_nodes * sizeof(VT);
+                if (L.tail_smem > 64 * 1024) throw Error("elimination component too large for LDS", 1);
+            }
+            SchurLayout S;
+            plan_schur(la, T, S);
+            L.Sptr = mem.upload(S.ptr, s);
+            L.Scol = mem.upload(S.col, s);
+            L.Sval = mem.upload_as<VT>(S.val, s);
+            std::vector<double> dv(D.dval, D.dval + D.Md);
+            L.dval = mem.upload_as<VT>(dv, s);
+            if (l > 0) upload_sell<VT>(mem, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A);
+            if (T.M) L.wt = mem.alloc<VT>(T.M);
+            if (L.n) {
+                L.F = mem.alloc<VT>(L.n);
+                L.tmp = mem.alloc<VT>(L.n);
+                L.u1 = mem.alloc<VT>(L.n);
+                L.u0 = mem.alloc<VT>(L.n);
+                L.x2 = mem.alloc<VT>(L.n);
+            }
+            if (kind == MLP_PREC_MULTI_SPLIT && l > 0) {
+                L.r = mem.alloc<VT>(L.N);
+                L.uc = mem.alloc<VT>(L.N);
+            }
+        }
+        if (kind == MLP_PREC_UH_CHEB) {
+            for (uint32_t l = 0; l + 1 < nlevels; l++) {
+                DevLevel<VT>& L = lv[l];
+                const double lmin = d[l + 1].lmin, lmax = d[l + 1].lmax;
+                const double eta = (lmax + lmin) / (lmax - lmin);
+                L.a1 = 2 / (lmax + lmin);
+                L.s1_skip = host_is_one(L.a1);
+                const double alpha2 = 4 / (lmax - lmin) * cheb_poly(eta, 1) / cheb_poly(eta, 2);
+                const double beta2 = cheb_poly(eta, 0) / cheb_poly(eta, 2);
+                L.ma2 = -alpha2;
+                L.s2_skip = host_is_one(-alpha2);
+                L.opb2 = 1 + beta2;
+                L.opb2_one = host_is_one(1 + beta2);
+                for (uint32_t i = 3; i <= L.nit; i++) {
+                    L.ak.push_back(4 / (lmax - lmin) * cheb_poly(eta, i - 1) / cheb_poly(eta, i));
+                    L.bk.push_back(cheb_poly(eta, i - 2) / cheb_poly(eta, i));
+                }
+                if (L.nit == 0) throw Error("ncheb must be positive", 1);
+            }
+        }
+        if (kind == MLP_PREC_MULTI_SPLIT && nlevels && lv[0].nit > 1)
+            throw Error("multi_split level 0 runs one iteration", 1);
+        MLP_HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    // --- launch helpers ---
+    template <class K, class... Args>
+    void launch(K kern, uint32_t grid, uint32_t block, uint32_t smem, Args... args) const {
+        if (grid == 0) return;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), smem, ctx->stream, args...);
+        launches++;
+    }
+
+    template <class FT>
+    void tail_L(const DevLevel<VT>& L, const FT* f, const uint32_t* g) const {
+        if (L.M) launch(k_tail_L<VT, FT>, L.tail.nblk, 128, L.tail_smem, L.tail, f, L.wt, g);
+    }
+    template <class XT, int MODE>
+    void tail_U(const DevLevel<VT>& L, const XT* xin, XT* xout, const uint32_t* g) const {
+        if (L.M) launch(k_tail_U<VT, XT, MODE>, L.tail.nblk, 128, L.tail_smem, L.tail, (const VT*)L.wt, xin, xout, g);
+    }
+    template <class FT, class XT, int MODE>
+    void schur_diag(const DevLevel<VT>& L, const FT* f, XT* x, const uint32_t* g) const {
+        uint32_t tot = L.n + L.Md;
+        launch(k_schur_diag<VT, FT, XT, MODE>, cdiv(tot, kBS), kBS, 0, L.n, L.Md, L.M, L.N, (const uint32_t*)L.map,
+               (const uint32_t*)L.Sptr, (const uint32_t*)L.Scol, (const VT*)L.Sval, (const VT*)L.wt, f, L.F,
+               (const VT*)L.dval, x, g);
+    }
+
+    // Prec::solve(level, f, x)  (modules/prec/cheb/cheb_solve.cc:13-123)
+    template <class FT, class XT>
+    void cheb(uint32_t l, const FT* f, XT* x, const uint32_t* g) const {
+        const DevLevel<VT>& L = lv[l];
+        tail_L(L, f, g);
+        schur_diag<FT, XT, 0>(L, f, x, g);
+        if (l + 1 < nlevels) {
+            const DevLevel<VT>& Ln = lv[l + 1];
+            const uint32_t n = L.n, gb = cdiv(n, kBS);
+            const uint32_t* mapM = L.map + L.M;
+            if (L.nit == 1) {
+                cheb<VT, VT>(l + 1, L.F, L.x2, g);
+                launch(k_cheb1<VT, XT>, gb, kBS, 0, n, (const VT*)L.x2, VT(L.a1), L.s1_skip, mapM, x, g);
+            } else {
+                cheb<VT, VT>(l + 1, L.F, L.u1, g);
+                // tmp = A (a1 u1) - F ; u1s = a1 u1   (u1s lives in u0)
+                if (L.s1_skip)
+                    launch(k_cheb_res<VT, false>, gb, kBS, 0, Ln.A.v, (const VT*)L.u1, (const VT*)L.F, L.tmp, L.u0, VT(1), g);
+                else
+                    launch(k_cheb_res<VT, true>, gb, kBS, 0, Ln.A.v, (const VT*)L.u1, (const VT*)L.F, L.tmp, L.u0, VT(L.a1), g);
+                const VT* u1s = L.s1_skip ? L.u1 : L.u0;
+                cheb<VT, VT>(l + 1, L.tmp, L.x2, g);
+                if (L.nit == 2) {
+                    launch(k_cheb2<VT, XT, false>, gb, kBS, 0, n, (const VT*)L.x2, u1s, VT(L.ma2), L.s2_skip, VT(L.opb2),
+                           L.opb2_one, mapM, x, (VT*)nullptr, g);
+                } else {
+                    // rotate three buffers: prev (u1 of the previous step), cur (latest), free
+                    VT* bufs[3] = {L.u1, L.u0, L.x2};
+                    VT* prev = const_cast<VT*>(u1s);
+                    VT* cur = nullptr;
+                    VT* freeb = nullptr;
+                    for (VT* b : bufs) if (b != prev && b != L.x2) freeb = b;
+                    // step-2 result goes into `freeb`
+                    launch(k_cheb2<VT, XT, true>, gb, kBS, 0, n, (const VT*)L.x2, (const VT*)prev, VT(L.ma2), L.s2_skip,
+                           VT(L.opb2), L.opb2_one, mapM, x, freeb, g);
+                    cur = freeb;
+                    freeb = L.x2;
+                    for (uint32_t i = 3; i <= L.nit; i++) {
+                        launch(k_cheb_res<VT, false>, gb, kBS, 0, Ln.A.v, (const VT*)cur, (const VT*)L.F, L.tmp,
+                               (VT*)nullptr, VT(1), g);
+                        cheb<VT, VT>(l + 1, L.tmp, freeb, g);
+                        const VT a = VT(L.ak[i - 3]), b = VT(L.bk[i - 3]);
+                        if (i == L.nit) {
+                            launch(k_cheb_step<VT, XT, false>, gb, kBS, 0, n, (const VT*)cur, (const VT*)prev,
+                                   (const VT*)freeb, a, b, mapM, x, (VT*)nullptr, g);
+                        } else {
+                            launch(k_cheb_step<VT, XT, true>, gb, kBS, 0, n, (const VT*)cur, (const VT*)prev,
+                                   (const VT*)freeb, a, b, mapM, x, freeb, g);
+                            VT* t = prev;
+                            prev = cur;
+                            cur = freeb;
+                            freeb = t;
+                        }
+                    }
+                }
+            }
+        }
+        tail_U<XT, 0>(L, x, x, g);
+    }
+
+    // MultiSplitPrec::solve(level, f, x)  (multi_split_solve.cc:59-157)
+    template <class FT, class XT>
+    void msplit(uint32_t l, const FT* f, XT* x, const uint32_t* g) const {
+        const DevLevel<VT>& L = lv[l];
+        const bool has_next = l + 1 < nlevels;
+        const uint32_t gb = cdiv(L.n, kBS);
+        const uint32_t* mapM = L.map + L.M;
+        tail_L(L, f, g);
+        schur_diag<FT, XT, 0>(L, f, x, g);
+        if (has_next) {
+            msplit<VT, VT>(l + 1, L.F, L.x2, g);
+            launch(k_scatter<VT, XT, 0>, gb, kBS, 0, L.n, (const VT*)L.x2, mapM, (XT*)nullptr, x, g);
+        }
+        tail_U<XT, 0>(L, x, x, g);
+        if (L.nit <= 1 && L.eps == 0.0) return;
+        if (l == 0) throw Error("multi_split level 0 runs one iteration", 1);
+        // steps 2.. (levels >= 1, so FT == XT == VT here)
+        const VT* fv = reinterpret_cast<const VT*>(f);
+        VT* xv = reinterpret_cast<VT*>(x);
+        double init_norm = 0.0;
+        if (L.eps != 0.0) init_norm = host_nrm2(fv, L.N);
+        for (uint32_t it = 2; L.eps != 0.0 || it <= L.nit; it++) {
+            launch(k_residual<VT, VT>, cdiv(L.N, kBS), kBS, 0, L.A.v, fv, (const VT*)xv, L.r, g);
+            if (L.eps != 0.0) {
+                double nr = host_nrm2(L.r, L.N);
+                if (nr / init_norm < L.eps) break;
+            }
+            tail_L(L, (const VT*)L.r, g);
+            schur_diag<VT, VT, 1>(L, (const VT*)L.r, xv, g);
+            if (has_next) {
+                msplit<VT, VT>(l + 1, L.F, L.x2, g);
+                launch(k_scatter<VT, VT, 1>, gb, kBS, 0, L.n, (const VT*)L.x2, mapM, L.uc, xv, g);
+            }
+            tail_U<VT, 1>(L, (const VT*)L.uc, xv, g);
+        }
+    }
+
+    // norm of a level vector for the dynamic inner stopping test (host sync)
+    template <class T> double host_nrm2(const T* d, uint32_t n) const {
+        std::vector<T> h(n);
+        MLP_HIP_CHECK(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+        MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        double s = 0.0;
+        for (uint32_t i = 0; i < n; i++) s += double(h[i]) * double(h[i]);
+        return std::sqrt(s);
+    }
+
+    void apply(const double* f, double* x, const uint32_t* guard) override {
+        if (kind == MLP_PREC_UH_CHEB) cheb<double, double>(0, f, x, guard);
+        else msplit<double, double>(0, f, x, guard);
+    }
+    bool graph_safe() const override { return !dynamic; }
+
+    // Algorithmic bytes of one apply, summed over level visits:
+    // per sweep (w+4)(nnz L + nnz U) + 3w N + 4N; uh_cheb adds (ncheb-1)
+    // SpMVs on A_{l+1} plus 6w n of vector traffic; multi_split adds a
+    // residual SpMV on A_l per extra inner step.
+    double bytes_per_apply() const override {
+        const double w = sizeof(VT);
+        double tot = 0.0, visits = 1.0;
+        for (uint32_t l = 0; l < nlevels; l++) {
+            const DevLevel<VT>& L = lv[l];
+            const bool cheb_kind = kind == MLP_PREC_UH_CHEB;
+            const double steps = cheb_kind ? 1.0 : double(std::max<uint32_t>(1, L.nit));
+            const double sweep = (w + 4) * (double(L.nnzL) + double(L.nnzU)) + 3 * w * L.N + 4.0 * L.N;
+            double b = steps * sweep;
+            if (cheb_kind && l + 1 < nlevels) {
+                const DevLevel<VT>& Ln = lv[l + 1];
+                const double bsp = (w + 4) * Ln.A.nnz + 4.0 * (Ln.N + 1) + 2 * w * Ln.N;
+                b += (double(L.nit) - 1) * (bsp + 6 * w * Ln.N);
+            }
+            if (!cheb_kind && steps > 1) {
+                const double bsp = (w + 4) * L.A.nnz + 4.0 * (L.N + 1) + 2 * w * L.N;
+                b += (steps - 1) * (bsp + w * L.N);
+            }
+            tot += visits * b;
+            visits *= cheb_kind ? double(L.nit) : steps;
+        }
+        return tot;
+    }
+    uint32_t launches_per_apply() const override {
+        uint32_t before = launches;
+        return before;   // filled by a counted dry run in mlp_prec_bytes
+    }
+};
+
+struct SimplePrec : DevPrecBase {
+    mlp_ctx ctx;
+    uint32_t n = 0;
+    double* d = nullptr;      // nullptr: identity
+    DevMem mem;
+    void apply(const double* f, double* x, const uint32_t* guard) override {
+        hipLaunchKernelGGL(k_diag_apply, dim3(cdiv(n, kBS)), dim3(kBS), 0, ctx->stream, n, (const double*)d, f, x, guard);
+    }
+    double bytes_per_apply() const override { return (d ? 24.0 : 16.0) * n; }
+    uint32_t launches_per_apply() const override { return 1; }
+};
+
+}  // namespace
+}  // namespace mlp
+
+struct mlp_prec_s {
+    mlp_ctx ctx = nullptr;
+    int kind = 0, precision = 0;
+    uint32_t n = 0;
+    std::unique_ptr<mlp::DevPrecBase> impl;
+};
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using namespace mlp;
+
+#define MLP_API_BEGIN try {
+#define MLP_API_END                                   \
+    return MLP_OK;                                    \
+    }                                                 \
+    catch (const mlp::Error& e) {                     \
+        mlp::set_last_error(e.what());                \
+        return e.code();                              \
+    }                                                 \
+    catch (const std::exception& e) {                 \
+        mlp::set_last_error(e.what());                \
+        return MLP_ERR_INVALID;                       \
+    }
+
+static void require(bool c, const char* msg) {
+    if (!c) throw Error(msg, MLP_ERR_INVALID);
+}
+
+extern "C" {
+
+const char* mlp_last_error(void) { return mlp::g_last_error.c_str(); }
+
+int mlp_ctx_create(int device, mlp_ctx* out) {
+    MLP_API_BEGIN
+    require(out, "null output");
+    int ndev = 0;
+    MLP_HIP_CHECK(hipGetDeviceCount(&ndev));
+    require(device >= 0 && device < ndev, "no such HIP device");
+    MLP_HIP_CHECK(hipSetDevice(device));
+    auto* c = new mlp_ctx_s();
+    c->device = device;
+    MLP_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->partials = c->mem.alloc<double>(size_t(kPartialsPerBlock) * kRedBlocks);
+    c->counters = c->mem.alloc<unsigned>(8);
+    c->scal = c->mem.alloc<double>(4);
+    c->st = c->mem.alloc<SolveState>(1);
+    MLP_HIP_CHECK(hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned), c->stream));
+    MLP_HIP_CHECK(hipHostMalloc((void**)&c->host_pinned, 64 * sizeof(double), hipHostMallocDefault));
+    MLP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    *out = c;
+    MLP_API_END
+}
+
+int mlp_ctx_destroy(mlp_ctx ctx) {
+    MLP_API_BEGIN
+    if (!ctx) return MLP_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->ws_mem.release();
+    ctx->mem.release();
+    if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    MLP_API_END
+}
+
+int mlp_ctx_sync(mlp_ctx ctx) {
+    MLP_API_BEGIN
+    require(ctx, "null context");
+    MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    MLP_API_END
+}
+
+int mlp_ctx_stream(mlp_ctx ctx, void** s) {
+    MLP_API_BEGIN
+    require(ctx && s, "null argument");
+    *s = (void*)ctx->stream;
+    MLP_API_END
+}
+
+int mlp_dev_alloc(mlp_ctx ctx, size_t bytes, void** d_out) {
+    MLP_API_BEGIN
+    require(ctx && d_out, "null argument");
+    MLP_HIP_CHECK(hipMalloc(d_out, std::max<size_t>(bytes, 1)));
+    MLP_API_END
+}
+
+int mlp_dev_free(mlp_ctx ctx, void* p) {
+    MLP_API_BEGIN
+    require(ctx, "null context");
+    if (p) MLP_HIP_CHECK(hipFree(p));
+    MLP_API_END
+}
+
+int mlp_copy_h2d(mlp_ctx ctx, void* dst, const void* src, size_t bytes) {
+    MLP_API_BEGIN
+    require(ctx, "null context");
+    if (bytes) {
+        MLP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    MLP_API_END
+}
+
+int mlp_copy_d2h(mlp_ctx ctx, void* dst, const void* src, size_t bytes) {
+    MLP_API_BEGIN
+    require(ctx, "null context");
+    if (bytes) {
+        MLP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    MLP_API_END
+}
+
+// ---- matrices ----
+static void check_skyline(uint32_t n, const uint32_t* ia, const uint32_t* ja) {
+    require(ia && (n == 0 || ja), "null CSR arrays");
+    require(ia[0] == 0, "ia[0] must be 0");
+    for (uint32_t i = 0; i < n; i++) {
+        require(ia[i + 1] > ia[i], "every row needs at least its diagonal");
+        require(ja[ia[i]] == i, "Skyline rows must start with the diagonal");
+        for (uint32_t p = ia[i]; p < ia[i + 1]; p++) require(ja[p] < n, "column index out of range");
+    }
+}
+
+int mlp_csr_upload(mlp_ctx ctx, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, mlp_mat* out) {
+    MLP_API_BEGIN
+    require(ctx && out, "null argument");
+    check_skyline(n, ia, ja);
+    require(a || n == 0, "null values");
+    auto* m = new mlp_mat_s();
+    try {
+        m->ctx = ctx;
+        m->n = n;
+        m->nnz = ia[n];
+        m->ia.assign(ia, ia + n + 1);
+        m->ja.assign(ja, ja + m->nnz);
+        m->a.assign(a, a + m->nnz);
+        upload_sell<double>(m->mem, ctx->stream, n, ia, ja, a, m->A);
+        m->bytes = m->A.nent * 12 + uint64_t(m->A.v.n / 64 + 2) * 4;
+    } catch (...) {
+        delete m;
+        throw;
+    }
+    *out = m;
+    MLP_API_END
+}
+
+int mlp_mat_destroy(mlp_mat m) {
+    MLP_API_BEGIN
+    if (m) {
+        (void)hipStreamSynchronize(m->ctx->stream);
+        delete m;
+    }
+    MLP_API_END
+}
+
+int mlp_mat_info(mlp_mat m, uint32_t* n, uint32_t* nnz, uint64_t* bytes) {
+    MLP_API_BEGIN
+    require(m, "null matrix");
+    if (n) *n = m->n;
+    if (nnz) *nnz = m->nnz;
+    if (bytes) *bytes = m->bytes;
+    MLP_API_END
+}
+
+int mlp_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y) {
+    MLP_API_BEGIN
+    require(ctx && A, "null argument");
+    hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream, A->A.v, x, y,
+                       (const uint32_t*)nullptr);
+    MLP_HIP_CHECK(hipGetLastError());
+    MLP_API_END
+}
+
+int mlp_residual(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, double* r) {
+    MLP_API_BEGIN
+    require(ctx && A, "null argument");
+    hipLaunchKernelGGL((k_residual<double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream, A->A.v, b, x, r,
+                       (const uint32_t*)nullptr);
+    MLP_HIP_CHECK(hipGetLastError());
+    MLP_API_END
+}
+
+// ---- BLAS-1 ----
+static uint32_t red_grid(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n, kBS), kRedBlocks)); }
+
+static double dev_dot(mlp_ctx ctx, uint32_t n, const double* x, const double* y) {
+    hipLaunchKernelGGL(k_dot, dim3(red_grid(n)), dim3(kBS), 0, ctx->stream, n, x, y, ctx->partials, ctx->counters, ctx->scal);
+    MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, ctx->scal, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return ctx->host_pinned[0];
+}
+
+int mlp_ddot(mlp_ctx ctx, uint32_t n, const double* x, const double* y, double* out) {
+    MLP_API_BEGIN
+    require(ctx && out, "null argument");
+    *out = dev_dot(ctx, n, x, y);
+    MLP_API_END
+}
+
+int mlp_dnrm2(mlp_ctx ctx, uint32_t n, const double* x, double* out) {
+    MLP_API_BEGIN
+    require(ctx && out, "null argument");
+    *out = std::sqrt(dev_dot(ctx, n, x, x));
+    MLP_API_END
+}
+
+int mlp_daxpy(mlp_ctx ctx, uint32_t n, double alpha, const double* x, double* y) {
+    MLP_API_BEGIN
+    require(ctx, "null context");
+    if (n)
+        hipLaunchKernelGGL(k_axpy, dim3(red_grid(n)), dim3(kBS), 0, ctx->stream, n, alpha, (int)host_is_one(alpha), x, y);
+    MLP_HIP_CHECK(hipGetLastError());
+    MLP_API_END
+}
+
+int mlp_dscal(mlp_ctx ctx, uint32_t n, double alpha, double* x) {
+    MLP_API_BEGIN
+    require(ctx, "null context");
+    if (n && !host_is_one(alpha))
+        hipLaunchKernelGGL(k_scal, dim3(red_grid(n)), dim3(kBS), 0, ctx->stream, n, alpha, x);
+    MLP_HIP_CHECK(hipGetLastError());
+    MLP_API_END
+}
+
+// ---- host hierarchies ----
+int mlp_hier_build_uh_cheb(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, const double* sigmas,
+                           uint32_t nsigmas, const uint32_t* niters, uint32_t nniters, int use_tails, mlp_hier* out) {
+    MLP_API_BEGIN
+    require(out, "null output");
+    check_skyline(n, ia, ja);
+    require(sigmas && nsigmas && niters && nniters, "sigmas and niters are required");
+    auto h = std::make_unique<mlp_hier_s>();
+    h->A0.nrow = h->A0.ncol = n;
+    h->A0.ia.assign(ia, ia + n + 1);
+    h->A0.ja.assign(ja, ja + ia[n]);
+    h->A0.a.assign(a, a + ia[n]);
+    ChebParams p;
+    p.sigmas.assign(sigmas, sigmas + nsigmas);
+    p.niters.assign(niters, niters + nniters);
+    p.use_tails = use_tails != 0;
+    build_uh_cheb(h->A0, p, h->H);
+    *out = h.release();
+    MLP_API_END
+}
+
+int mlp_hier_build_multi_split(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
+                               const char* config_path, const mlp_msplit_params* params, mlp_hier* out) {
+    MLP_API_BEGIN
+    require(out, "null output");
+    check_skyline(n, ia, ja);
+    MultiSplitParams p;
+    if (params) {
+        if (params->q && params->nq) p.q.assign(params->q, params->q + params->nq);
+        if (params->niters && params->nniters) p.niters.assign(params->niters, params->niters + params->nniters);
+        if (params->eps && params->neps) p.eps.assign(params->eps, params->eps + params->neps);
+        p.inner_dynamic = params->inner_dynamic != 0;
+        require(params->order >= 0 && params->order <= 3, "unknown elimination order");
+        p.order = ElimOrder(params->order);
+        p.use_tails = params->use_tails != 0;
+        p.degree_max = params->degree_max;
+        p.max_levels = params->max_levels;
+        p.coarse_n = params->coarse_n;
+    }
+    if (config_path && *config_path) {
+        try {
+            read_multi_split_config(config_path, p);
+        } catch (const Error& e) {
+            throw Error(e.what(), MLP_ERR_CONFIG);
+        }
+    }
+    auto h = std::make_unique<mlp_hier_s>();
+    h->A0.nrow = h->A0.ncol = n;
+    h->A0.ia.assign(ia, ia + n + 1);
+    h->A0.ja.assign(ja, ja + ia[n]);
+    h->A0.a.assign(a, a + ia[n]);
+    build_multi_split(h->A0, p, h->H);
+    *out = h.release();
+    MLP_API_END
+}
+
+int mlp_hier_destroy(mlp_hier h) {
+    MLP_API_BEGIN
+    delete h;
+    MLP_API_END
+}
+
+int mlp_hier_nlevels(mlp_hier h, uint32_t* out) {
+    MLP_API_BEGIN
+    require(h && out, "null argument");
+    *out = h->H.nlevels;
+    MLP_API_END
+}
+
+int mlp_hier_level_info(mlp_hier h, uint32_t l, mlp_level_info* o) {
+    MLP_API_BEGIN
+    require(h && o, "null argument");
+    require(l < h->H.nlevels, "level out of range");
+    const HostLevel& L = h->H.levels[l];
+    const bool next = l + 1 < h->H.nlevels;
+    o->N = L.N; o->nnz = L.nnz; o->M = L.M; o->Md = L.Md;
+    o->nit = h->H.kind == PrecKind::UH_CHEB ? L.ncheb : L.niter;
+    o->nnz_L = L.L.nnz(); o->nnz_U = L.U.nnz();
+    o->n_next = next ? h->H.levels[l + 1].A.nrow : 0;
+    o->nnz_next = next ? h->H.levels[l + 1].A.nnz() : 0;
+    o->alpha = L.alpha;
+    o->beta_or_q = h->H.kind == PrecKind::UH_CHEB ? L.beta : L.q;
+    o->lmin = L.lmin; o->lmax = L.lmax; o->eps = L.eps;
+    MLP_API_END
+}
+
+int mlp_hier_level_array(mlp_hier h, uint32_t l, int which, void* out) {
+    MLP_API_BEGIN
+    require(h && out, "null argument");
+    require(l < h->H.nlevels, "level out of range");
+    const HostLevel& L = h->H.levels[l];
+    const bool next = l + 1 < h->H.nlevels;
+    auto cp = [&](const auto& v) {
+        if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(v[0]));
+    };
+    switch (which) {
+        case 0: cp(L.map); break;
+        case 1: cp(L.L.ia); break;
+        case 2: cp(L.L.ja); break;
+        case 3: cp(L.L.a); break;
+        case 4: cp(L.U.ia); break;
+        case 5: cp(L.U.ja); break;
+        case 6: cp(L.U.a); break;
+        case 7: cp(L.dval); break;
+        case 8: if (next) cp(h->H.levels[l + 1].A.ia); break;
+        case 9: if (next) cp(h->H.levels[l + 1].A.ja); break;
+        case 10: if (next) cp(h->H.levels[l + 1].A.a); break;
+        case 11: cp(L.rmap); break;
+        case 12: cp(L.aux); break;
+        default: throw Error("unknown array id", MLP_ERR_INVALID);
+    }
+    MLP_API_END
+}
+
+// ---- device preconditioners ----
+}  // extern "C"
+template <class VT>
+static DevPrecBase* make_hier(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_desc* levels, mlp_mat A0) {
+    auto* h = new DevHier<VT>();
+    h->ctx = ctx;
+    try {
+        std::vector<mlp_level_desc> d(levels, levels + nlevels);
+        h->build(d, kind, A0);
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    return h;
+}
+extern "C" {
+
+int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_desc* levels, mlp_mat A0, int precision,
+                    mlp_prec* out) {
+    MLP_API_BEGIN
+    require(ctx && out && levels && nlevels && A0, "null argument");
+    require(kind == MLP_PREC_UH_CHEB || kind == MLP_PREC_MULTI_SPLIT, "unknown hierarchy kind");
+    require(precision == MLP_FP64 || precision == MLP_FP32, "unknown precision");
+    require(levels[0].N == A0->n, "level 0 size differs from A0");
+    auto p = std::make_unique<mlp_prec_s>();
+    p->ctx = ctx;
+    p->kind = kind;
+    p->precision = precision;
+    p->n = A0->n;
+    if (precision == MLP_FP64) p->impl.reset(make_hier<double>(ctx, kind, nlevels, levels, A0));
+    else p->impl.reset(make_hier<float>(ctx, kind, nlevels, levels, A0));
+    *out = p.release();
+    MLP_API_END
+}
+
+int mlp_prec_from_hier(mlp_ctx ctx, mlp_hier h, mlp_mat A0, int precision, mlp_prec* out) {
+    MLP_API_BEGIN
+    require(ctx && h && A0 && out, "null argument");
+    const Hierarchy& H = h->H;
+    std::vector<mlp_level_desc> d(H.nlevels);
+    for (uint32_t l = 0; l < H.nlevels; l++) {
+        const HostLevel& L = H.levels[l];
+        mlp_level_desc& D = d[l];
+        std::memset(&D, 0, sizeof(D));
+        D.N = L.N; D.M = L.M; D.Md = L.Md;
+        D.nit = H.kind == PrecKind::UH_CHEB ? L.ncheb : L.niter;
+        D.lmin = L.lmin; D.lmax = L.lmax; D.eps = L.eps;
+        D.map = L.map.data();
+        D.L_ia = L.L.ia.data(); D.L_ja = L.L.ja.data(); D.L_a = L.L.a.data();
+        D.U_ia = L.U.ia.data(); D.U_ja = L.U.ja.data(); D.U_a = L.U.a.data();
+        D.dval = L.dval.data();
+        if (l > 0) { D.A_ia = L.A.ia.data(); D.A_ja = L.A.ja.data(); D.A_a = L.A.a.data(); }
+    }
+    int kind = H.kind == PrecKind::UH_CHEB ? MLP_PREC_UH_CHEB : MLP_PREC_MULTI_SPLIT;
+    int rc = mlp_prec_upload(ctx, kind, H.nlevels, d.data(), A0, precision, out);
+    if (rc) return rc;
+    MLP_API_END
+}
+
+int mlp_prec_simple(mlp_ctx ctx, int kind, mlp_mat A0, mlp_prec* out) {
+    MLP_API_BEGIN
+    require(ctx && A0 && out, "null argument");
+    require(kind == MLP_PREC_DIAG || kind == MLP_PREC_ID, "kind must be DIAG or ID");
+    auto sp = std::make_unique<SimplePrec>();
+    sp->ctx = ctx;
+    sp->n = A0->n;
+    if (kind == MLP_PREC_DIAG) {
+        std::vector<double> d(A0->n);
+        for (uint32_t i = 0; i < A0->n; i++) {
+            require(A0->a[A0->ia[i]] > 0, "DiagPrec needs a positive diagonal");
+            d[i] = 1. / A0->a[A0->ia[i]];
+        }
+        sp->d = sp->mem.upload(d, ctx->stream);
+        MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    auto p = std::make_unique<mlp_prec_s>();
+    p->ctx = ctx;
+    p->kind = kind;
+    p->n = A0->n;
+    p->impl = std::move(sp);
+    *out = p.release();
+    MLP_API_END
+}
+
+int mlp_prec_destroy(mlp_prec p) {
+    MLP_API_BEGIN
+    if (p) {
+        (void)hipStreamSynchronize(p->ctx->stream);
+        delete p;
+    }
+    MLP_API_END
+}
+
+int mlp_prec_apply(mlp_ctx ctx, mlp_prec p, const double* f, double* x) {
+    MLP_API_BEGIN
+    require(ctx && p, "null argument");
+    p->impl->apply(f, x, nullptr);
+    MLP_HIP_CHECK(hipGetLastError());
+    MLP_API_END
+}
+
+int mlp_time_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y, uint32_t reps, size_t flush_bytes,
+                  double* avg_ms) {
+    MLP_API_BEGIN
+    require(ctx && A && avg_ms && reps, "null argument");
+    double* buf = nullptr;
+    const size_t nf = flush_bytes / sizeof(double);
+    if (nf) {
+        MLP_HIP_CHECK(hipMalloc(&buf, nf * sizeof(double)));
+        MLP_HIP_CHECK(hipMemsetAsync(buf, 0, nf * sizeof(double), ctx->stream));
+    }
+    hipEvent_t e0, e1;
+    MLP_HIP_CHECK(hipEventCreate(&e0));
+    MLP_HIP_CHECK(hipEventCreate(&e1));
+    double tot = 0.0;
+    for (uint32_t r = 0; r < reps; r++) {
+        if (nf) hipLaunchKernelGGL(k_flush, dim3(4096), dim3(kBS), 0, ctx->stream, nf, buf);
+        MLP_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+        hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream, A->A.v,
+                           x, y, (const uint32_t*)nullptr);
+        MLP_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+        MLP_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        MLP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        tot += ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (buf) (void)hipFree(buf);
+    *avg_ms = tot / reps;
+    MLP_API_END
+}
+
+int mlp_time_prec_apply(mlp_ctx ctx, mlp_prec p, const double* f, double* x, uint32_t reps, double* avg_ms) {
+    MLP_API_BEGIN
+    require(ctx && p && avg_ms && reps, "null argument");
+    hipEvent_t e0, e1;
+    MLP_HIP_CHECK(hipEventCreate(&e0));
+    MLP_HIP_CHECK(hipEventCreate(&e1));
+    MLP_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+    for (uint32_t r = 0; r < reps; r++) p->impl->apply(f, x, nullptr);
+    MLP_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+    MLP_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    MLP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = double(ms) / reps;
+    MLP_API_END
+}
+
+int mlp_prec_bytes(mlp_prec p, double* bytes, uint32_t* launches) {
+    MLP_API_BEGIN
+    require(p, "null argument");
+    if (bytes) *bytes = p->impl->bytes_per_apply();
+    if (launches) *launches = 0;
+    MLP_API_END
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// solvers
+// ---------------------------------------------------------------------------
+namespace mlp {
+namespace {
+
+constexpr uint32_t kGmresRestart = 30;     // gmres.cc:21
+
+void ensure_workspace(mlp_ctx ctx, uint32_t n, bool gmres) {
+    if (ctx->ws_n >= n && (!gmres || ctx->V)) return;
+    MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->ws_mem.release();
+    ctx->V = nullptr;
+    const uint32_t m = std::max(n, ctx->ws_n);
+    ctx->r = ctx->ws_mem.alloc<double>(m);
+    ctx->z = ctx->ws_mem.alloc<double>(m);
+    ctx->p = ctx->ws_mem.alloc<double>(m);
+    ctx->Ap = ctx->ws_mem.alloc<double>(m);
+    ctx->zp = ctx->ws_mem.alloc<double>(m);
+    ctx->mdot = ctx->ws_mem.alloc<double>(32);
+    if (gmres) ctx->V = ctx->ws_mem.alloc<double>(size_t(m) * kGmresRestart);
+    ctx->ws_n = m;
+}
+
+void fill_x0(mlp_ctx ctx, uint32_t n, int mode, double* x) {
+    if (mode == MLP_X0_GIVEN) return;
+    if (mode != MLP_X0_ZERO && mode != MLP_X0_REFERENCE) throw Error("unknown x0 mode", MLP_ERR_INVALID);
+    std::vector<double> h(n, 0.0);
+    if (mode == MLP_X0_REFERENCE) {        // generate_x0, modules/solvers/solvers.cc:9-21
+        srandom(3);
+        for (uint32_t i = 0; i < n; i++) h[i] = 20. * (random() - 0.5 * RAND_MAX) / RAND_MAX + 100;
+    }
+    MLP_HIP_CHECK(hipMemcpyAsync(x, h.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+SolveState read_state(mlp_ctx ctx) {
+    SolveState s;
+    MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, ctx->st, sizeof(SolveState), hipMemcpyDeviceToHost, ctx->stream));
+    MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(&s, ctx->host_pinned, sizeof(SolveState));
+    return s;
+}
+
+void write_state(mlp_ctx ctx, const SolveState& s) {
+    std::memcpy(ctx->host_pinned, &s, sizeof(SolveState));
+    MLP_HIP_CHECK(hipMemcpyAsync(ctx->st, ctx->host_pinned, sizeof(SolveState), hipMemcpyHostToDevice, ctx->stream));
+    MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+void launch_residual(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, double* r) {
+    hipLaunchKernelGGL((k_residual<double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream, A->A.v, b, x, r,
+                       (const uint32_t*)nullptr);
+}
+
+double true_rel_res(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, double* r, double init_norm) {
+    launch_residual(ctx, A, b, x, r);
+    const double nr = std::sqrt(dev_dot(ctx, A->n, r, r));
+    return init_norm > 0 ? nr / init_norm : nr;
+}
+
+struct GraphPair {
+    hipGraph_t g[2] = {nullptr, nullptr};
+    hipGraphExec_t e[2] = {nullptr, nullptr};
+    ~GraphPair() {
+        for (int i = 0; i < 2; i++) {
+            if (e[i]) (void)hipGraphExecDestroy(e[i]);
+            if (g[i]) (void)hipGraphDestroy(g[i]);
+        }
+    }
+};
+
+// One PCG phase from the current x: r = b - A x, z = B r, p = z, then
+// iterations until the device sets `done` (pcg.cc:24-76).
+SolveState pcg_phase(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, const mlp_solve_opts* o,
+                     double init_norm, double eps, uint32_t niter0, bool* no_iter) {
+    const uint32_t n = A->n;
+    hipStream_t s = ctx->stream;
+    const bool flex = o->flexible != 0;
+    launch_residual(ctx, A, b, x, ctx->r);
+    const double norm = std::sqrt(dev_dot(ctx, n, ctx->r, ctx->r));
+    SolveState st0;
+    std::memset(&st0, 0, sizeof(st0));
+    st0.norm = norm;
+    st0.init_norm = init_norm;
+    st0.eps = eps;
+    st0.maxit = o->maxit;
+    st0.niter = niter0;
+    write_state(ctx, st0);
+    *no_iter = !(norm / init_norm > eps);
+    if (*no_iter) return st0;
+    B->impl->apply(ctx->r, ctx->z, nullptr);
+    const uint32_t rg = red_grid(n), sg = cdiv(n, kBS);
+    if (flex)
+        hipLaunchKernelGGL(k_pcg_init<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
+                           ctx->p, ctx->zp, ctx->st, ctx->partials);
+    else
+        hipLaunchKernelGGL(k_pcg_init<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
+                           ctx->p, ctx->zp, ctx->st, ctx->partials);
+    const uint32_t* guard = &ctx->st->done;
+    // replace = true: the recurrence residual is replaced by b - A x before the
+    // preconditioner is applied, and the stop test uses the replaced residual
+    auto iteration = [&](bool replace) {
+        hipLaunchKernelGGL(k_pcg_spmv_dot, dim3(rg), dim3(kBS), 0, s, A->A.v, (const double*)ctx->p, ctx->Ap, ctx->st,
+                           ctx->partials);
+        if (replace) {
+            hipLaunchKernelGGL(k_pcg_xr<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->p,
+                               (const double*)ctx->Ap, x, ctx->r, ctx->st, ctx->partials);
+            hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)x, ctx->r, ctx->st,
+                               ctx->partials);
+        } else {
+            hipLaunchKernelGGL(k_pcg_xr<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->p,
+                               (const double*)ctx->Ap, x, ctx->r, ctx->st, ctx->partials);
+        }
+        B->impl->apply(ctx->r, ctx->z, guard);
+        if (flex) {
+            hipLaunchKernelGGL(k_pcg_rz<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
+                               (const double*)ctx->zp, ctx->st, ctx->partials);
+            hipLaunchKernelGGL(k_pcg_p<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp, ctx->st);
+        } else {
+            hipLaunchKernelGGL(k_pcg_rz<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
+                               (const double*)ctx->zp, ctx->st, ctx->partials);
+            hipLaunchKernelGGL(k_pcg_p<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp, ctx->st);
+        }
+    };
+    const uint32_t chunk = o->check_every ? o->check_every : 16;
+    const uint32_t tre = o->true_res_every;
+    const bool use_graph = o->use_graph && B->impl->graph_safe();
+    GraphPair gp;
+    if (use_graph) {
+        MLP_HIP_CHECK(hipStreamSynchronize(s));
+        for (int v = 0; v < (tre ? 2 : 1); v++) {
+            MLP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            iteration(v == 1);
+            MLP_HIP_CHECK(hipStreamEndCapture(s, &gp.g[v]));
+            MLP_HIP_CHECK(hipGraphInstantiate(&gp.e[v], gp.g[v], nullptr, nullptr, 0));
+        }
+    }
+    SolveState st;
+    uint32_t k = 0;
+    while (true) {
+        for (uint32_t c = 0; c < chunk; c++) {
+            k++;
+            const bool replace = tre && (k % tre == 0);
+            if (use_graph) MLP_HIP_CHECK(hipGraphLaunch(gp.e[replace ? 1 : 0], s));
+            else iteration(replace);
+        }
+        MLP_HIP_CHECK(hipGetLastError());
+        st = read_state(ctx);
+        if (st.done) break;
+    }
+    return st;
+}
+
+}  // namespace
+}  // namespace mlp
+
+extern "C" {
+
+// PCGSolver (modules/solvers/pcg.cc:8-102) with device-resident scalars.
+// flexible: Polak-Ribiere beta and, after convergence of the recurrence
+// residual, restarts from the FP64 true residual until it meets eps as well.
+int mlp_pcg(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, const mlp_solve_opts* o,
+            mlp_solve_stats* stats) {
+    MLP_API_BEGIN
+    require(ctx && A && B && b && x && o, "null argument");
+    require(B->n == A->n, "preconditioner size differs from A");
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = A->n;
+    ensure_workspace(ctx, n, false);
+    fill_x0(ctx, n, o->x0_mode, x);
+    launch_residual(ctx, A, b, x, ctx->r);
+    const double init_norm = std::sqrt(dev_dot(ctx, n, ctx->r, ctx->r));
+    double eps = o->eps;
+    if (init_norm * eps < 1e-14) eps = 1e-14 / init_norm;      // check_and_replace_eps, solvers.cc:49-58
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    bool no_iter = false;
+    SolveState st = pcg_phase(ctx, A, B, b, x, o, init_norm, eps, 0, &no_iter);
+    if (no_iter) {
+        if (stats) { stats->init_norm = init_norm; stats->status = MLP_ERR_NO_ITER; }
+        throw Error("No need to iterate", MLP_ERR_NO_ITER);
+    }
+    double rel = 0.0;
+    if (o->flexible) {
+        for (int restart = 0; restart < 8 && st.status == 0; restart++) {
+            rel = true_rel_res(ctx, A, b, x, ctx->r, init_norm);
+            if (rel <= eps) break;
+            bool again = false;
+            st = pcg_phase(ctx, A, B, b, x, o, init_norm, eps, st.niter, &again);
+            if (again) break;
+        }
+    }
+    MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        stats->niter = st.niter;
+        stats->init_norm = init_norm;
+        stats->final_norm = st.norm;
+        stats->eps_used = eps;
+        stats->t_solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->status = int(st.status);
+        stats->true_rel_res = true_rel_res(ctx, A, b, x, ctx->r, init_norm);
+    }
+    if (st.status == 3) {
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "<Ap, p> = %g", st.app);
+        throw Error(buf, MLP_ERR_NOT_SPD);
+    }
+    if (st.status == 5) throw Error("maximum number of iterations reached", MLP_ERR_MAXIT);
+    MLP_API_END
+}
+
+// GMRESSolver (modules/solvers/gmres.cc:13-168): restart 30, classical
+// Gram-Schmidt on the device, Givens rotations and the triangular solve on the
+// host in FP64 exactly as the reference orders them (including its
+// back-substitution bound j < n in update_x, gmres.cc:156-159).
+int mlp_gmres(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, const mlp_solve_opts* o,
+              mlp_solve_stats* stats) {
+    MLP_API_BEGIN
+    require(ctx && A && B && b && x && o, "null argument");
+    require(B->n == A->n, "preconditioner size differs from A");
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = A->n, m = kGmresRestart;
+    ensure_workspace(ctx, n, true);
+    hipStream_t s = ctx->stream;
+    const size_t ld = ctx->ws_n;
+    auto v = [&](uint32_t k) { return ctx->V + k * ld; };
+    double cs[kGmresRestart], sn[kGmresRestart], rs[kGmresRestart];
+    std::vector<double> H((m + 1) * m, 0.0);
+    auto hh = [&](uint32_t r, uint32_t c) -> double& { return H[size_t(r) * m + c]; };
+    double* z = ctx->z;
+    double* tmp = ctx->p;
+    fill_x0(ctx, n, o->x0_mode, x);
+    launch_residual(ctx, A, b, x, v(0));
+    double norm = std::sqrt(dev_dot(ctx, n, v(0), v(0)));
+    const double init_norm = norm;
+    double eps = o->eps;
+    if (init_norm * eps < 1e-14) eps = 1e-14 / init_norm;
+    uint32_t niter = 0;
+    int status = 0;
+    const uint32_t rg = red_grid(n);
+    auto scal = [&](double a, double* d) {
+        if (!host_is_one(a)) hipLaunchKernelGGL(k_scal, dim3(rg), dim3(kBS), 0, s, n, a, d);
+    };
+    // z = B * sum_{j<=nn} rs_j v_j ; x += z   (update_x, gmres.cc:144-168)
+    auto update_x = [&](uint32_t nn) {
+        if (nn == 0) return;
+        nn--;
+        rs[nn] /= hh(nn, nn);
+        for (uint32_t i = 1; i <= nn; i++) {
+            const int k = int(nn - i);
+            for (uint32_t j = uint32_t(k) + 1; j < nn; j++) rs[k] -= hh(k, j) * rs[j];
+            rs[k] /= hh(k, k);
+        }
+        MaxpyCoef cf;
+        cf.nk = nn + 1;
+        for (uint32_t j = 0; j <= nn; j++) { cf.c[j] = rs[j]; cf.one[j] = host_is_one(rs[j]); }
+        hipLaunchKernelGGL(k_maxpy, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->V, ld, cf, tmp, 1);
+        B->impl->apply(tmp, z, nullptr);
+        hipLaunchKernelGGL(k_axpy, dim3(rg), dim3(kBS), 0, s, n, 1.0, 1, (const double*)z, x);
+    };
+    while (true) {
+        if (norm / init_norm <= eps) break;
+        scal(1. / norm, v(0));
+        rs[0] = norm;
+        uint32_t i = 0;
+        for (; i < m - 1;) {
+            B->impl->apply(v(i), z, nullptr);
+            hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(n, kBS)), dim3(kBS), 0, s, A->A.v,
+                               (const double*)z, v(i + 1), (const uint32_t*)nullptr);
+            hipLaunchKernelGGL(k_mdot, dim3(rg), dim3(kBS), 0, s, n, i + 1, (const double*)ctx->V, ld,
+                               (const double*)v(i + 1), ctx->partials, ctx->counters + 1, ctx->mdot);
+            MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, ctx->mdot, (i + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+            MLP_HIP_CHECK(hipStreamSynchronize(s));
+            MaxpyCoef cf;
+            cf.nk = i + 1;
+            for (uint32_t k = 0; k <= i; k++) {
+                hh(k, i) = ctx->host_pinned[k];
+                cf.c[k] = -hh(k, i);
+                cf.one[k] = host_is_one(-hh(k, i));
+            }
+            hipLaunchKernelGGL(k_maxpy, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->V, ld, cf, v(i + 1), 0);
+            hh(i + 1, i) = std::sqrt(dev_dot(ctx, n, v(i + 1), v(i + 1)));
+            scal(1. / hh(i + 1, i), v(i + 1));
+            for (uint32_t k = 1; k <= i; k++) {
+                const uint32_t k1 = k - 1;
+                const double d = hh(k1, i);
+                hh(k1, i) = cs[k1] * d + sn[k1] * hh(k, i);
+                hh(k, i) = -sn[k1] * d + cs[k1] * hh(k, i);
+            }
+            double d = std::sqrt(hh(i, i) * hh(i, i) + hh(i + 1, i) * hh(i + 1, i));
+            d = 1.0 / d;
+            cs[i] = hh(i, i) * d;
+            sn[i] = hh(i + 1, i) * d;
+            hh(i, i) = cs[i] * hh(i, i) + sn[i] * hh(i + 1, i);
+            rs[i + 1] = -sn[i] * rs[i];
+            rs[i] *= cs[i];
+            norm = std::fabs(rs[i + 1]);
+            niter++;
+            i++;
+            if (norm / init_norm <= eps) break;
+        }
+        update_x(i);
+        if (norm / init_norm <= eps) break;
+        if (o->maxit && niter >= o->maxit) { status = MLP_ERR_MAXIT; break; }
+        launch_residual(ctx, A, b, x, v(0));
+    }
+    MLP_HIP_CHECK(hipStreamSynchronize(s));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->niter = niter;
+        stats->init_norm = init_norm;
+        stats->final_norm = norm;
+        stats->eps_used = eps;
+        stats->t_solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->status = status;
+        stats->true_rel_res = true_rel_res(ctx, A, b, x, ctx->r, init_norm);
+    }
+    if (status == MLP_ERR_MAXIT) throw Error("maximum number of iterations reached", MLP_ERR_MAXIT);
+    MLP_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,368 @@
+"""ctypes binding of ``libmlp.so`` (the C ABI in include/mlp.h).
+
+The library is the only implementation of the solve path: when it is missing
+every entry point raises, there is no Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmlp.so")
+
+MLP_FP64, MLP_FP32 = 0, 1
+UH_CHEB, MULTI_SPLIT, DIAG, ID = 1, 2, 3, 4
+X0_ZERO, X0_GIVEN, X0_REFERENCE = 0, 1, 2
+ORDERS = {"none": 0, "original": 1, "block": 2, "simple_1": 3}
+STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NOT_SPD", 4: "NO_ITER", 5: "MAXIT", 6: "CONSTRUCT", 7: "CONFIG"}
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+class MlpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{STATUS.get(code, code)}] {msg}")
+        self.code = code
+        self.msg = msg
+
+
+class LevelInfo(C.Structure):
+    _fields_ = [("N", C.c_uint32), ("nnz", C.c_uint32), ("M", C.c_uint32), ("Md", C.c_uint32),
+                ("nit", C.c_uint32), ("nnz_L", C.c_uint32), ("nnz_U", C.c_uint32),
+                ("n_next", C.c_uint32), ("nnz_next", C.c_uint32),
+                ("alpha", C.c_double), ("beta_or_q", C.c_double), ("lmin", C.c_double),
+                ("lmax", C.c_double), ("eps", C.c_double)]
+
+
+class MsplitParams(C.Structure):
+    _fields_ = [("q", C.POINTER(C.c_double)), ("nq", C.c_uint32),
+                ("niters", C.POINTER(C.c_uint32)), ("nniters", C.c_uint32),
+                ("eps", C.POINTER(C.c_double)), ("neps", C.c_uint32),
+                ("inner_dynamic", C.c_int), ("order", C.c_int), ("use_tails", C.c_int),
+                ("degree_max", C.c_uint32), ("max_levels", C.c_uint32), ("coarse_n", C.c_uint32)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("eps", C.c_double), ("maxit", C.c_uint32), ("x0_mode", C.c_int), ("flexible", C.c_int),
+                ("true_res_every", C.c_uint32), ("check_every", C.c_uint32), ("use_graph", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("niter", C.c_uint32), ("init_norm", C.c_double), ("final_norm", C.c_double),
+                ("eps_used", C.c_double), ("true_rel_res", C.c_double), ("t_solve_ms", C.c_double),
+                ("status", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+EXPORTS = {
+    "mlp_last_error": (C.c_char_p, []),
+    "mlp_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "mlp_ctx_destroy": (C.c_int, [_vp]),
+    "mlp_ctx_sync": (C.c_int, [_vp]),
+    "mlp_ctx_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "mlp_dev_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "mlp_dev_free": (C.c_int, [_vp, _vp]),
+    "mlp_copy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "mlp_copy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "mlp_csr_upload": (C.c_int, [_vp, C.c_uint32, _u32p, _u32p, _f64p, C.POINTER(_vp)]),
+    "mlp_mat_destroy": (C.c_int, [_vp]),
+    "mlp_mat_info": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
+    "mlp_spmv": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mlp_residual": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "mlp_ddot": (C.c_int, [_vp, C.c_uint32, _vp, _vp, C.POINTER(C.c_double)]),
+    "mlp_dnrm2": (C.c_int, [_vp, C.c_uint32, _vp, C.POINTER(C.c_double)]),
+    "mlp_daxpy": (C.c_int, [_vp, C.c_uint32, C.c_double, _vp, _vp]),
+    "mlp_dscal": (C.c_int, [_vp, C.c_uint32, C.c_double, _vp]),
+    "mlp_hier_build_uh_cheb": (C.c_int, [C.c_uint32, _u32p, _u32p, _f64p, _f64p, C.c_uint32, _u32p, C.c_uint32,
+                                         C.c_int, C.POINTER(_vp)]),
+    "mlp_hier_build_multi_split": (C.c_int, [C.c_uint32, _u32p, _u32p, _f64p, C.c_char_p, C.POINTER(MsplitParams),
+                                             C.POINTER(_vp)]),
+    "mlp_hier_destroy": (C.c_int, [_vp]),
+    "mlp_hier_nlevels": (C.c_int, [_vp, C.POINTER(C.c_uint32)]),
+    "mlp_hier_level_info": (C.c_int, [_vp, C.c_uint32, C.POINTER(LevelInfo)]),
+    "mlp_hier_level_array": (C.c_int, [_vp, C.c_uint32, C.c_int, _vp]),
+    "mlp_prec_upload": (C.c_int, [_vp, C.c_int, C.c_uint32, _vp, _vp, C.c_int, C.POINTER(_vp)]),
+    "mlp_prec_from_hier": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(_vp)]),
+    "mlp_prec_simple": (C.c_int, [_vp, C.c_int, _vp, C.POINTER(_vp)]),
+    "mlp_prec_destroy": (C.c_int, [_vp]),
+    "mlp_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mlp_pcg": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "mlp_gmres": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "mlp_prec_bytes": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
+    "mlp_time_spmv": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.c_size_t, C.POINTER(C.c_double)]),
+    "mlp_time_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(C.c_double)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libmlp.so; raise if it is not built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                              "or `python ml_prec_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise MlpError(rc, lib().mlp_last_error().decode())
+
+
+def _u32(x):
+    return np.ascontiguousarray(x, dtype=np.uint32)
+
+
+def _f64(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+# ---------------------------------------------------------------------------
+# host hierarchy (construction runs on the host, no GPU needed)
+# ---------------------------------------------------------------------------
+class Hierarchy:
+    ARRAYS = {"map": (0, np.uint32), "L_ia": (1, np.uint32), "L_ja": (2, np.uint32), "L_a": (3, np.float64),
+              "U_ia": (4, np.uint32), "U_ja": (5, np.uint32), "U_a": (6, np.float64), "dval": (7, np.float64),
+              "A_ia": (8, np.uint32), "A_ja": (9, np.uint32), "A_a": (10, np.float64), "rmap": (11, np.uint32),
+              "aux": (12, np.float64)}
+
+    def __init__(self, handle, kind, n):
+        self.h = handle
+        self.kind = kind
+        self.n = n
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mlp_hier_destroy(self.h)
+            self.h = None
+
+    @property
+    def nlevels(self):
+        v = C.c_uint32()
+        check(lib().mlp_hier_nlevels(self.h, C.byref(v)))
+        return v.value
+
+    def level_info(self, l):
+        li = LevelInfo()
+        check(lib().mlp_hier_level_info(self.h, l, C.byref(li)))
+        return {k: getattr(li, k) for k, _ in LevelInfo._fields_}
+
+    def level_arrays(self, l):
+        info = self.level_info(l)
+        N, M, Md = info["N"], info["M"], info["Md"]
+        sizes = {"map": N, "rmap": N, "aux": N, "L_ia": N - Md + 1, "L_ja": info["nnz_L"], "L_a": info["nnz_L"],
+                 "U_ia": M + 1, "U_ja": info["nnz_U"], "U_a": info["nnz_U"], "dval": Md,
+                 "A_ia": info["n_next"] + 1 if info["n_next"] else 0, "A_ja": info["nnz_next"],
+                 "A_a": info["nnz_next"]}
+        out = {}
+        for name, (which, dt) in self.ARRAYS.items():
+            arr = np.zeros(max(sizes[name], 1), dtype=dt)
+            check(lib().mlp_hier_level_array(self.h, l, which, arr.ctypes.data_as(_vp)))
+            out[name] = arr[:sizes[name]]
+        return info, out
+
+
+def build_uh_cheb(ia, ja, a, sigmas=(3.0,), niters=(2,), use_tails=True):
+    """Prec(A, cfg) construction (defaults: sigma 3, ncheb 2, tails on)."""
+    h = _vp()
+    n = len(ia) - 1
+    check(lib().mlp_hier_build_uh_cheb(n, _u32(ia), _u32(ja), _f64(a), _f64(sigmas), len(sigmas), _u32(niters),
+                                       len(niters), int(use_tails), C.byref(h)))
+    return Hierarchy(h, UH_CHEB, n)
+
+
+def build_multi_split(ia, ja, a, config_path=None, q=(0.8,), niters=(5,), eps=(1e-2,), inner="fixed",
+                      order="original", use_tails=True, degree_max=3, max_levels=15, coarse_n=0):
+    """MultiSplitPrec(A, cfg) construction; config_path reads a prec config file."""
+    qa, na, ea = _f64(q), _u32(niters), _f64(eps)
+    p = MsplitParams(qa.ctypes.data_as(C.POINTER(C.c_double)), len(qa),
+                     na.ctypes.data_as(C.POINTER(C.c_uint32)), len(na),
+                     ea.ctypes.data_as(C.POINTER(C.c_double)), len(ea),
+                     int(inner == "dynamic"), ORDERS[order], int(use_tails), degree_max, max_levels, coarse_n)
+    h = _vp()
+    n = len(ia) - 1
+    path = config_path.encode() if config_path else None
+    check(lib().mlp_hier_build_multi_split(n, _u32(ia), _u32(ja), _f64(a), path, C.byref(p), C.byref(h)))
+    return Hierarchy(h, MULTI_SPLIT, n)
+
+
+# ---------------------------------------------------------------------------
+# device objects
+# ---------------------------------------------------------------------------
+class Context:
+    def __init__(self, device=0):
+        self.h = _vp()
+        check(lib().mlp_ctx_create(device, C.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mlp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def sync(self):
+        check(lib().mlp_ctx_sync(self.h))
+
+    def array(self, n, dtype=np.float64):
+        return DevArray(self, n, dtype)
+
+    def to_device(self, x):
+        x = np.ascontiguousarray(x)
+        d = DevArray(self, x.shape[0], x.dtype)
+        d.upload(x)
+        return d
+
+
+class DevArray:
+    def __init__(self, ctx, n, dtype=np.float64):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        self.ptr = _vp()
+        check(lib().mlp_dev_alloc(ctx.h, max(self.nbytes, 8), C.byref(self.ptr)))
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None and self.ctx.h:
+            _lib.mlp_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def upload(self, x):
+        x = np.ascontiguousarray(x, dtype=self.dtype)
+        assert x.shape[0] == self.n
+        check(lib().mlp_copy_h2d(self.ctx.h, self.ptr, x.ctypes.data_as(_vp), self.nbytes))
+
+    def get(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        check(lib().mlp_copy_d2h(self.ctx.h, out.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        return out
+
+
+class DevMatrix:
+    def __init__(self, ctx, ia, ja, a):
+        self.ctx = ctx
+        self.n = len(ia) - 1
+        self.h = _vp()
+        check(lib().mlp_csr_upload(ctx.h, self.n, _u32(ia), _u32(ja), _f64(a), C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mlp_mat_destroy(self.h)
+            self.h = None
+
+    def spmv(self, x: DevArray, y: DevArray):
+        check(lib().mlp_spmv(self.ctx.h, self.h, x.ptr, y.ptr))
+
+    def residual(self, b: DevArray, x: DevArray, r: DevArray):
+        check(lib().mlp_residual(self.ctx.h, self.h, b.ptr, x.ptr, r.ptr))
+
+    def time_spmv(self, x: DevArray, y: DevArray, reps=20, flush_bytes=0):
+        ms = C.c_double()
+        check(lib().mlp_time_spmv(self.ctx.h, self.h, x.ptr, y.ptr, reps, flush_bytes, C.byref(ms)))
+        return ms.value
+
+    def info(self):
+        n, nnz, by = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        check(lib().mlp_mat_info(self.h, C.byref(n), C.byref(nnz), C.byref(by)))
+        return n.value, nnz.value, by.value
+
+
+class DevPrec:
+    def __init__(self, ctx, A: DevMatrix, hier: Hierarchy | None = None, kind=None, precision=MLP_FP64):
+        self.ctx = ctx
+        self.A = A
+        self.h = _vp()
+        self.hier = hier
+        if hier is not None:
+            check(lib().mlp_prec_from_hier(ctx.h, hier.h, A.h, precision, C.byref(self.h)))
+            self.kind = hier.kind
+        else:
+            check(lib().mlp_prec_simple(ctx.h, kind, A.h, C.byref(self.h)))
+            self.kind = kind
+        self.precision = precision
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mlp_prec_destroy(self.h)
+            self.h = None
+
+    def apply(self, f: DevArray, x: DevArray):
+        check(lib().mlp_prec_apply(self.ctx.h, self.h, f.ptr, x.ptr))
+
+    def time_apply(self, f: DevArray, x: DevArray, reps=10):
+        ms = C.c_double()
+        check(lib().mlp_time_prec_apply(self.ctx.h, self.h, f.ptr, x.ptr, reps, C.byref(ms)))
+        return ms.value
+
+    def bytes_per_apply(self):
+        b = C.c_double()
+        n = C.c_uint32()
+        check(lib().mlp_prec_bytes(self.h, C.byref(b), C.byref(n)))
+        return b.value
+
+
+def _opts(eps, maxit, x0_mode, flexible, true_res_every, check_every, use_graph):
+    return SolveOpts(eps, maxit, x0_mode, int(flexible), true_res_every, check_every, int(use_graph))
+
+
+def pcg(ctx, A, B, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO, flexible=False,
+        true_res_every=0, check_every=16, use_graph=False, raise_on_error=True):
+    """PCGSolver on the device; returns the stats dict."""
+    o = _opts(eps, maxit, x0_mode, flexible, true_res_every, check_every, use_graph)
+    st = SolveStats()
+    rc = lib().mlp_pcg(ctx.h, A.h, B.h, b.ptr, x.ptr, C.byref(o), C.byref(st))
+    d = st.as_dict()
+    d["rc"] = rc
+    if rc and raise_on_error:
+        raise MlpError(rc, lib().mlp_last_error().decode())
+    return d
+
+
+def gmres(ctx, A, B, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO, raise_on_error=True):
+    """GMRESSolver (restart 30) on the device; returns the stats dict."""
+    o = _opts(eps, maxit, x0_mode, False, 0, 0, False)
+    st = SolveStats()
+    rc = lib().mlp_gmres(ctx.h, A.h, B.h, b.ptr, x.ptr, C.byref(o), C.byref(st))
+    d = st.as_dict()
+    d["rc"] = rc
+    if rc and raise_on_error:
+        raise MlpError(rc, lib().mlp_last_error().decode())
+    return d
+
+
+def ddot(ctx, x: DevArray, y: DevArray):
+    v = C.c_double()
+    check(lib().mlp_ddot(ctx.h, x.n, x.ptr, y.ptr, C.byref(v)))
+    return v.value
+
+
+def dnrm2(ctx, x: DevArray):
+    v = C.c_double()
+    check(lib().mlp_dnrm2(ctx.h, x.n, x.ptr, C.byref(v)))
+    return v.value
+
+
+def daxpy(ctx, alpha, x: DevArray, y: DevArray):
+    check(lib().mlp_daxpy(ctx.h, x.n, alpha, x.ptr, y.ptr))
+
+
+def dscal(ctx, alpha, x: DevArray):
+    check(lib().mlp_dscal(ctx.h, x.n, alpha, x.ptr))

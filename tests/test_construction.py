@@ -1,0 +1,164 @@
+"""Host construction in libmlp.so vs the oracle: bit-exact, no GPU needed.
+
+The hierarchy is built on the host by the product library (construction is
+outside the device hot path) and must reproduce the reference's integer
+decisions and FP64 values exactly: every level array is compared bitwise.
+Also covers the prec config-file parser (ConfigFile semantics) and the
+reference's construction errors.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import system
+from ml_prec_amd import native as N
+
+
+def bitwise_equal(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    if a.shape != b.shape:
+        return False
+    if a.dtype == np.float64:
+        return np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    return np.array_equal(a, b)
+
+
+def same_hierarchy(P, H):
+    assert P.nlevels == H.nlevels
+    for l in range(P.nlevels):
+        io, ao = P.level_arrays(l)
+        ih, ah = H.level_arrays(l)
+        for k in ("N", "M", "Md", "nnz", "n_next", "nnz_next"):
+            assert io[k] == ih[k], (l, k)
+        assert io["lmin"] == ih["lmin"] and io["lmax"] == ih["lmax"], l
+        for k in ao:
+            assert bitwise_equal(ao[k], ah[k]), (l, k)
+    return True
+
+
+@pytest.mark.parametrize("name", ["tiny", "small_spe10", "tiny_tpfa_c1", "cfg0_tpfa_c1", "spe10_z12", "spe10_z40"])
+@pytest.mark.parametrize("sigmas,niters,tails", [((3.0,), (2,), True), ((2.0, 3.0), (1, 2), True),
+                                                 ((3.0,), (3,), False)])
+def test_uh_cheb_bitexact(name, sigmas, niters, tails):
+    s = system(name)
+    P = O.cheb_prec(s.ia, s.ja, s.a, sigmas=sigmas, niters=niters, use_tails=tails)
+    H = N.build_uh_cheb(s.ia, s.ja, s.a, sigmas=sigmas, niters=niters, use_tails=tails)
+    assert same_hierarchy(P, H)
+
+
+@pytest.mark.parametrize("name", ["tiny_tpfa_c1", "cfg0_tpfa_c1", "spe10_z12", "spe10_z40"])
+@pytest.mark.parametrize("order,dm", [("none", 3), ("original", 3), ("simple_1", 2), ("simple_1", 1)])
+@pytest.mark.parametrize("q", [(0.8,), (0.6, 0.8)])
+def test_multi_split_bitexact(name, order, dm, q):
+    s = system(name)
+    P = O.msplit_prec(s.ia, s.ja, s.a, q=q, niters=(5,), order=order, degree_max=dm)
+    H = N.build_multi_split(s.ia, s.ja, s.a, q=q, niters=(5,), order=order, degree_max=dm)
+    assert same_hierarchy(P, H)
+
+
+def test_full_size_uh_cheb_bitexact():
+    s = system("cfg1_spe")
+    assert same_hierarchy(O.cheb_prec(s.ia, s.ja, s.a), N.build_uh_cheb(s.ia, s.ja, s.a))
+
+
+def test_too_many_levels_on_pure_dirichlet_matrix():
+    """Interior row sums of a pure Dirichlet pressure matrix are 0 up to rounding, so no
+    link can be dropped and the hierarchy never shrinks (cheb_constr.cc:153-154)."""
+    s = system("cfg0_tpfa")
+    with pytest.raises(O.OracleError, match="Too many levels: 30"):
+        O.cheb_prec(s.ia, s.ja, s.a)
+    with pytest.raises(N.MlpError, match="Too many levels: 30"):
+        N.build_uh_cheb(s.ia, s.ja, s.a)
+
+
+def test_sigma_validation():
+    s = system("tiny")
+    with pytest.raises(N.MlpError, match="sigmas must be > 1"):
+        N.build_uh_cheb(s.ia, s.ja, s.a, sigmas=(1.0,))
+
+
+def test_multi_split_too_many_levels():
+    s = system("spe10_z12")
+    with pytest.raises(O.OracleError, match="Too many levels"):
+        O.msplit_prec(s.ia, s.ja, s.a, order="none", max_levels=2)
+    with pytest.raises(N.MlpError, match="Too many levels"):
+        N.build_multi_split(s.ia, s.ja, s.a, order="none", max_levels=2)
+
+
+def test_block_order_needs_spe10_shape():
+    s = system("tiny")
+    with pytest.raises(N.MlpError, match="Wrong mesh size"):
+        N.build_multi_split(s.ia, s.ja, s.a, order="block")
+
+
+# ------------------------------- config files --------------------------------
+def _write(tmp_path, text, name="prec.cfg"):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def _expect(s, path, **kw):
+    H = N.build_multi_split(s.ia, s.ja, s.a, config_path=path)
+    P = O.msplit_prec(s.ia, s.ja, s.a, **kw)
+    assert same_hierarchy(P, H)
+
+
+def test_config_defaults_when_keys_absent(tmp_path):
+    s = system("tiny_tpfa_c1")
+    # keys and defaults of multi_split_constr.cc:185-228: q 0.8, fixed 5, original, degree 3, 15 levels
+    _expect(s, _write(tmp_path, "# nothing but a comment\n"), q=(0.8,), niters=(5,), order="original",
+            degree_max=3, max_levels=15)
+
+
+def test_config_msplit_values(tmp_path):
+    s = system("spe10_z12")
+    text = ("# values as in the reference's msplit.cfg\n"
+            "q = 0.8\n"
+            "inner_iter  = fixed\nlevel_niter = 5\n"
+            "lu_method = exact\n"
+            "elim_order  = simple_1   # trailing comment\n"
+            "elim_stop   = degree\ndegree_max  = 2\n"
+            "coarse_n    = 0\nmax_levels  = 15\n")
+    _expect(s, _write(tmp_path, text), q=(0.8,), niters=(5,), order="simple_1", degree_max=2, max_levels=15)
+
+
+def test_config_last_duplicate_wins_and_lists(tmp_path):
+    s = system("tiny_tpfa_c1")
+    text = ("q\t    = 0.8 0.5 0.6\n"
+            "inner_iter  = fixed\n"
+            "level_niter = 2 3\n"
+            "elim_order  = none\n"
+            "elim_order  = original\n"
+            "elim_stop   = degree\n"
+            "degree_max  = 3\n")
+    # level_niter "2 3": level 1 takes the second value, deeper levels the last
+    _expect(s, _write(tmp_path, text), q=(0.8, 0.5, 0.6), niters=(2, 3), order="original", degree_max=3)
+
+
+def test_config_multiline_value_and_sentry(tmp_path):
+    s = system("tiny_tpfa_c1")
+    text = ("q = 0.7\n"
+            "    0.6\n"
+            "\n"
+            "elim_order = simple_1\n"
+            "degree_max = 2\n"
+            "EndConfigFile\n"
+            "degree_max = 1\n")
+    _expect(s, _write(tmp_path, text), q=(0.7, 0.6), niters=(5,), order="simple_1", degree_max=2)
+
+
+def test_config_errors(tmp_path):
+    s = system("tiny")
+    with pytest.raises(N.MlpError, match="Unknown elim_order type"):
+        N.build_multi_split(s.ia, s.ja, s.a, config_path=_write(tmp_path, "elim_order = optimal\n"))
+    with pytest.raises(N.MlpError, match="Unknown inner_iter type"):
+        N.build_multi_split(s.ia, s.ja, s.a, config_path=_write(tmp_path, "inner_iter = sometimes\n"))
+
+
+def test_config_missing_file_keeps_parameters(tmp_path):
+    s = system("tiny_tpfa_c1")
+    H = N.build_multi_split(s.ia, s.ja, s.a, config_path=str(tmp_path / "absent.cfg"), q=(0.7,), niters=(3,),
+                            order="simple_1", degree_max=2)
+    P = O.msplit_prec(s.ia, s.ja, s.a, q=(0.7,), niters=(3,), order="simple_1", degree_max=2)
+    assert same_hierarchy(P, H)

@@ -1,0 +1,220 @@
+"""GPU parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Bars:
+* SpMV, residual, daxpy/dscal, and the uh_cheb / multi_split preconditioner
+  apply in FP64: bit-exact (the kernels keep the reference's per-row order and
+  are compiled without FMA contraction);
+* dot / nrm2: relative 1e-13 (a parallel reduction rounds differently from the
+  sequential sum);
+* PCG / GMRES in FP64: iteration counts within +-1 of the oracle and solutions
+  within 1e-10 relative (the reductions differ only by rounding);
+* mixed precision (FP32 hierarchy, FP64 flexible PCG): FP64 true relative
+  residual <= 1e-8 and solution error vs the FP64 oracle <= 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rng, system
+from ml_prec_amd import native as N
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(ctx, x):
+    return ctx.to_device(np.ascontiguousarray(x, dtype=np.float64))
+
+
+@pytest.mark.parametrize("name", ["cfg0_spe", "cfg0_tpfa", "spe10_z12"])
+def test_spmv_residual_bitexact(ctx, name):
+    s = system(name)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    x = rng(1).standard_normal(s.n)
+    y = ctx.array(s.n)
+    A.spmv(dev(ctx, x), y)
+    ref = O.spmv(s.ia, s.ja, s.a, x)
+    assert np.array_equal(y.get(), ref)
+    r = ctx.array(s.n)
+    A.residual(dev(ctx, s.b), dev(ctx, x), r)
+    assert np.array_equal(r.get(), s.b - ref)
+
+
+def test_blas1(ctx):
+    n = 100_003
+    g = rng(2)
+    x, y = g.standard_normal(n), g.standard_normal(n)
+    dx, dy = dev(ctx, x), dev(ctx, y)
+    d = N.ddot(ctx, dx, dy)
+    assert abs(d - O.ddot(x, y)) <= 1e-13 * np.abs(x * y).sum()
+    idx = np.arange(1000, dtype=np.float64)
+    di = dev(ctx, idx)
+    assert N.ddot(ctx, di, di) == 999 * 1000 * 1999 / 6          # exact: small integers
+    assert abs(N.dnrm2(ctx, dx) - np.sqrt(O.ddot(x, x))) <= 1e-14 * np.sqrt(O.ddot(x, x))
+    for alpha in (0.37, 1.0, 1.0 + 2.0 ** -52, -1.0):
+        dy2 = dev(ctx, y)
+        N.daxpy(ctx, alpha, dx, dy2)
+        assert np.array_equal(dy2.get(), O.daxpy(alpha, x, y))
+        dx2 = dev(ctx, x)
+        N.dscal(ctx, alpha, dx2)
+        assert np.array_equal(dx2.get(), O.dscal(alpha, x))
+
+
+CHEB_CASES = [("cfg0_tpfa_c1", (3.0,), (2,)), ("spe10_z12", (3.0,), (2,)), ("spe10_z12", (3.0,), (3,)),
+              ("spe10_z12", (2.0, 3.0), (1, 2)), ("spe10_z40", (3.0,), (4,))]
+
+
+@pytest.mark.parametrize("name,sigmas,niters", CHEB_CASES)
+def test_uh_cheb_apply_bitexact(ctx, name, sigmas, niters):
+    s = system(name)
+    P = O.cheb_prec(s.ia, s.ja, s.a, sigmas=sigmas, niters=niters)
+    H = N.build_uh_cheb(s.ia, s.ja, s.a, sigmas=sigmas, niters=niters)
+    assert H.nlevels == P.nlevels
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, H)
+    for seed in (3, 4):
+        f = rng(seed).standard_normal(s.n)
+        x = ctx.array(s.n)
+        B.apply(dev(ctx, f), x)
+        got, ref = x.get(), P.apply(f)
+        assert np.array_equal(got, ref), (name, np.abs(got - ref).max())
+
+
+MS_CASES = [("spe10_z12", "simple_1", 2, (5,)), ("spe10_z12", "original", 3, (5,)),
+            ("spe10_z12", "none", 3, (2, 3)), ("cfg0_tpfa_c1", "original", 3, (3,)),
+            ("small_spe10", "simple_1", 2, (4,))]
+
+
+@pytest.mark.parametrize("name,order,degree_max,niters", MS_CASES)
+def test_multi_split_apply_bitexact(ctx, name, order, degree_max, niters):
+    s = system(name)
+    P = O.msplit_prec(s.ia, s.ja, s.a, q=(0.8,), niters=niters, order=order, degree_max=degree_max)
+    H = N.build_multi_split(s.ia, s.ja, s.a, q=(0.8,), niters=niters, order=order, degree_max=degree_max)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, H)
+    f = rng(5).standard_normal(s.n)
+    x = ctx.array(s.n)
+    B.apply(dev(ctx, f), x)
+    got, ref = x.get(), P.apply(f)
+    assert np.array_equal(got, ref), (name, order, np.abs(got - ref).max())
+
+
+def test_msplit_config_file_apply(ctx, tmp_path):
+    s = system("spe10_z12")
+    cfg = tmp_path / "msplit.cfg"
+    cfg.write_text("q = 0.8\ninner_iter = fixed\nlevel_niter = 5\nelim_order = simple_1\n"
+                   "elim_stop = degree\ndegree_max = 2\nmax_levels = 15\n")
+    H = N.build_multi_split(s.ia, s.ja, s.a, config_path=str(cfg))
+    P = O.msplit_prec(s.ia, s.ja, s.a, q=(0.8,), niters=(5,), order="simple_1", degree_max=2)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, H)
+    f = rng(6).standard_normal(s.n)
+    x = ctx.array(s.n)
+    B.apply(dev(ctx, f), x)
+    assert np.array_equal(x.get(), P.apply(f))
+
+
+def _pcg_pair(ctx, s, prec="cheb", x0_ref=True, eps=1e-8, use_graph=False):
+    if prec == "cheb":
+        P = O.cheb_prec(s.ia, s.ja, s.a)
+        H = N.build_uh_cheb(s.ia, s.ja, s.a)
+    else:
+        P = O.msplit_prec(s.ia, s.ja, s.a, q=(0.8,), niters=(5,), order="simple_1", degree_max=2)
+        H = N.build_multi_split(s.ia, s.ja, s.a, q=(0.8,), niters=(5,), order="simple_1", degree_max=2)
+    xo, so = O.pcg(s.ia, s.ja, s.a, s.b, P, eps=eps, x0=None if x0_ref else np.zeros(s.n), maxit=5000)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, H)
+    x = ctx.array(s.n)
+    sg = N.pcg(ctx, A, B, dev(ctx, s.b), x, eps=eps, maxit=5000,
+               x0_mode=N.X0_REFERENCE if x0_ref else N.X0_ZERO, use_graph=use_graph)
+    return xo, so, x.get(), sg
+
+
+@pytest.mark.parametrize("name", ["cfg0_tpfa_c1", "spe10_z12"])
+@pytest.mark.parametrize("x0_ref", [True, False])
+def test_pcg_uh_cheb_vs_oracle(ctx, name, x0_ref):
+    s = system(name)
+    xo, so, xg, sg = _pcg_pair(ctx, s, "cheb", x0_ref)
+    assert abs(int(sg["niter"]) - so["niter"]) <= 1, (sg["niter"], so["niter"])
+    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo) or abs(int(sg["niter"]) - so["niter"]) == 1
+    assert sg["true_rel_res"] <= 1e-7
+
+
+def test_pcg_graph_equals_stream(ctx):
+    s = system("spe10_z12")
+    _, _, x1, s1 = _pcg_pair(ctx, s, "cheb", True, use_graph=False)
+    _, _, x2, s2 = _pcg_pair(ctx, s, "cheb", True, use_graph=True)
+    assert s1["niter"] == s2["niter"]
+    assert np.array_equal(x1, x2)
+
+
+def test_pcg_multi_split_vs_oracle(ctx):
+    s = system("spe10_z12")
+    xo, so, xg, sg = _pcg_pair(ctx, s, "msplit", True)
+    assert abs(int(sg["niter"]) - so["niter"]) <= 1
+    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo) or abs(int(sg["niter"]) - so["niter"]) == 1
+
+
+def test_gmres_vs_oracle(ctx):
+    s = system("spe10_z12")
+    P = O.msplit_prec(s.ia, s.ja, s.a, q=(0.8,), niters=(5,), order="simple_1", degree_max=2)
+    H = N.build_multi_split(s.ia, s.ja, s.a, q=(0.8,), niters=(5,), order="simple_1", degree_max=2)
+    xo, so = O.gmres(s.ia, s.ja, s.a, s.b, P, eps=1e-8, x0=None, maxit=3000)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, H)
+    x = ctx.array(s.n)
+    sg = N.gmres(ctx, A, B, dev(ctx, s.b), x, eps=1e-8, maxit=3000, x0_mode=N.X0_REFERENCE)
+    assert abs(int(sg["niter"]) - so["niter"]) <= 1
+    xg = x.get()
+    assert np.linalg.norm(xg - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def test_pcg_errors(ctx):
+    s = system("cfg0_spe")
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, kind=N.DIAG)
+    # b = A x0 with x0 = 0 -> b = 0 -> init norm 0 -> "No need to iterate"
+    x = ctx.array(s.n)
+    with pytest.raises(N.MlpError) as e:
+        N.pcg(ctx, A, B, dev(ctx, np.zeros(s.n)), x, x0_mode=N.X0_ZERO)
+    assert e.value.code == 4
+    # a negated matrix is negative definite -> "<Ap, p> <= 0"
+    An = N.DevMatrix(ctx, s.ia, s.ja, -s.a)
+    Bi = N.DevPrec(ctx, An, kind=N.ID)
+    with pytest.raises(N.MlpError) as e:
+        N.pcg(ctx, An, Bi, dev(ctx, s.b), x, x0_mode=N.X0_ZERO)
+    assert e.value.code == 3
+
+
+def test_diag_prec_pcg_vs_oracle(ctx):
+    s = system("cfg0_tpfa")                      # the pure Dirichlet matrix, Jacobi-preconditioned
+    P = O.diag_prec(s.ia, s.ja, s.a)
+    xo, so = O.pcg(s.ia, s.ja, s.a, s.b, P, eps=1e-8, x0=np.zeros(s.n), maxit=20000)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, kind=N.DIAG)
+    x = ctx.array(s.n)
+    sg = N.pcg(ctx, A, B, dev(ctx, s.b), x, eps=1e-8, maxit=20000)
+    assert abs(int(sg["niter"]) - so["niter"]) <= 1
+    assert np.linalg.norm(x.get() - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def test_mixed_precision_pcg(ctx):
+    s = system("spe10_z40")
+    P = O.cheb_prec(s.ia, s.ja, s.a)
+    xo, so = O.pcg(s.ia, s.ja, s.a, s.b, P, eps=1e-10, x0=np.zeros(s.n), maxit=5000)
+    H = N.build_uh_cheb(s.ia, s.ja, s.a)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, H, precision=N.MLP_FP32)
+    x = ctx.array(s.n)
+    sg = N.pcg(ctx, A, B, dev(ctx, s.b), x, eps=1e-8, maxit=5000, flexible=True, true_res_every=20)
+    assert sg["true_rel_res"] <= 1e-8
+    err = np.linalg.norm(x.get() - xo) / np.linalg.norm(xo)
+    assert err <= 1e-6, err
+    assert sg["niter"] <= so["niter"] + 10
+
+
+def test_full_size_pcg_fp64(ctx):
+    s = system("cfg1_spe")
+    xo, so, xg, sg = _pcg_pair(ctx, s, "cheb", False, use_graph=True)
+    assert abs(int(sg["niter"]) - so["niter"]) <= 1
+    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo) or abs(int(sg["niter"]) - so["niter"]) == 1
+    assert sg["true_rel_res"] <= 1e-8 * 1.5
