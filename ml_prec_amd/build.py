@@ -16,6 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 CPP = os.path.join(HERE, "cpp")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmlp.so")
+DEMO = os.path.join(BUILD, "dropin_demo")
 ARCH = os.environ.get("MLP_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -57,6 +58,11 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # the C++ drop-in driver, linked against libmlp.so
+    demo = os.path.join(CPP, "dropin_demo.cc")
+    if os.path.exists(demo):
+        _run([HIPCC] + COMMON + ["-x", "c++", demo, "-o", DEMO, "-L" + HERE, "-lmlp",
+                                 "-Wl,-rpath,$ORIGIN/.."])
     if verbose:
         print(f"built {LIB} in {time.time() - t0:.1f}s")
     return LIB

@@ -57,6 +57,11 @@ def lib():
             f = getattr(L, nm)
             f.argtypes = [C.c_uint32, _u32p, _u32p, _f64p, _f64p, C.c_void_p, _f64p, _f64p,
                           C.c_double, C.c_int, C.c_uint32, C.c_char_p, C.c_size_t]
+        L.or_cheb_solver_csr.argtypes = [C.c_uint32, _u32p, _u32p, _f64p, C.c_double, C.c_double, _f64p,
+                                         C.c_void_p, _f64p, _f64p, C.c_double, C.c_int, C.c_uint32,
+                                         C.c_char_p, C.c_size_t]
+        L.or_simple_solver_csr.argtypes = [C.c_uint32, _u32p, _u32p, _f64p, _f64p, C.c_void_p, _f64p, _f64p,
+                                           C.c_double, C.c_int, C.c_uint32, C.c_char_p, C.c_size_t]
         for nm in ("or_spmv_csr", "or_sym_spmv_csr"):
             f = getattr(L, nm)
             f.argtypes = [C.c_uint32, _u32p, _u32p, _f64p, _f64p, _f64p]
@@ -200,6 +205,29 @@ def pcg(ia, ja, a, b, prec, eps=1e-8, x0=None, maxit=0, check=True):
 
 def gmres(ia, ja, a, b, prec, eps=1e-8, x0=None, maxit=0, check=True):
     x, st = _solve(lib().or_gmres_csr, ia, ja, a, b, prec, eps, x0, maxit)
+    if check and st["rc"]:
+        raise OracleError(st["err"])
+    return x, st
+
+
+def cheb_solver(ia, ja, a, b, prec, lmin, lmax, eps=1e-8, x0=None, maxit=0, check=True):
+    """ChebSolver (needs the uh_cheb bounds Prec::lmin()/lmax())."""
+    n = len(ia) - 1
+    x = np.zeros(n) if x0 is None else _f64(x0).copy()
+    st = np.zeros(6)
+    err = C.create_string_buffer(512)
+    rc = lib().or_cheb_solver_csr(n, _u32(ia), _u32(ja), _f64(a), lmin, lmax, _f64(b), prec.h, x, st, eps,
+                                  0 if x0 is None else 1, maxit, err, 512)
+    stats = dict(niter=int(st[0]), t_sol=st[1], init_norm=st[2], final_norm=st[3], eps=st[4], rc=rc,
+                 err=err.value.decode())
+    if check and rc:
+        raise OracleError(stats["err"])
+    return x, stats
+
+
+def simple_solver(ia, ja, a, b, prec, eps=1e-8, x0=None, maxit=0, check=True):
+    """SimpleSolver (preconditioned Richardson)."""
+    x, st = _solve(lib().or_simple_solver_csr, ia, ja, a, b, prec, eps, x0, maxit)
     if check and st["rc"]:
         raise OracleError(st["err"])
     return x, st

@@ -1192,6 +1192,95 @@ int or_gmres(const or_mat* A, const double* b, const or_prec* B, double* x, or_s
     return rc;
 }
 
+/* ChebSolver (modules/solvers/cheb.cc:9-100), NORM_L2; lmin/lmax are Prec::lmin()/lmax() */
+int or_cheb_solver(const or_mat* A, double lmin, double lmax, const double* b, const or_prec* B, double* x,
+                   or_stats* st, double eps, int x0_mode, uint32_t maxit, char* err, size_t errlen) {
+    double gtime = now_s();
+    uint32_t n = A->nrow;
+    double* r = xmalloc((size_t)n * 8 + 8);
+    double* u0 = xcalloc(n, 8);
+    double* u1 = xcalloc(n, 8);
+    double* xc = x;                                   /* the buffers rotate like the reference's swaps */
+    double alpha, beta, norm, init_norm;
+    double eta = (lmax + lmin) / (lmax - lmin);
+    int rc = 0;
+    if (x0_mode == 0) or_generate_x0(x, n);
+    or_residual(A, b, x, r);
+    norm = init_norm = or_dnrm2(r, n);
+    uint32_t niter = 1;
+    check_and_replace_eps(init_norm, &eps);
+    alpha = 2 / (lmax + lmin);                        /* :37-47 */
+    memcpy(u1, x, (size_t)n * 8);
+    or_prec_apply(B, r, x);
+    or_dscal(alpha, x, n);
+    or_daxpy(1, u1, x, n);
+    or_residual(A, b, x, r);
+    norm = or_dnrm2(r, n);
+    niter++;
+    double *pu0 = u0, *pu1 = u1, *px = x;
+    while (norm / init_norm > eps) {                  /* :50-78 */
+        if (maxit && niter - 1 >= maxit) { set_err(err, errlen, "maxit %u reached", maxit); rc = 3; break; }
+        double* t0 = pu0; pu0 = px; px = t0;          /* u0.swap(x) */
+        double* t1 = pu1; pu1 = pu0; pu0 = t1;        /* u1.swap(u0) */
+        alpha = 4 / (lmax - lmin) * or_cheb(eta, niter - 1) / or_cheb(eta, niter);
+        beta = or_cheb(eta, niter - 2) / or_cheb(eta, niter);
+        or_prec_apply(B, r, px);
+        for (uint32_t k = 0; k < n; k++) px[k] = pu1[k] + alpha * px[k] + beta * (pu1[k] - pu0[k]);
+        or_residual(A, b, px, r);
+        norm = or_dnrm2(r, n);
+        niter++;
+    }
+    niter--;
+    if (px != xc) memcpy(xc, px, (size_t)n * 8);
+    if (st) {
+        st->niter = niter;
+        st->t_sol = now_s() - gtime;
+        st->init_norm = init_norm;
+        st->final_norm = norm;
+        st->eps_used = eps;
+    }
+    /* free the two buffers that are not the caller's x */
+    double* bufs[3] = {u0, u1, xc};
+    for (int i = 0; i < 3; i++) if (bufs[i] != xc) free(bufs[i]);
+    free(r);
+    return rc;
+}
+
+/* SimpleSolver (modules/solvers/simple_solver.cc:8-88), NORM_L2 */
+int or_simple_solver(const or_mat* A, const double* b, const or_prec* B, double* x, or_stats* st,
+                     double eps, int x0_mode, uint32_t maxit, char* err, size_t errlen) {
+    double gtime = now_s();
+    uint32_t n = A->nrow;
+    double* r = xmalloc((size_t)n * 8 + 8);
+    double* z = xmalloc((size_t)n * 8 + 8);
+    double norm, init_norm;
+    int rc = 0;
+    if (x0_mode == 0) or_generate_x0(x, n);
+    or_residual(A, b, x, r);
+    norm = init_norm = or_dnrm2(r, n);
+    uint32_t niter = 0, ninv = 0;
+    check_and_replace_eps(init_norm, &eps);
+    while (norm / init_norm > eps) {
+        if (maxit && niter >= maxit) { set_err(err, errlen, "maxit %u reached", maxit); rc = 3; break; }
+        or_prec_apply(B, r, z);
+        if (niter) ninv++;                            /* the first apply counts as construction (:45-52) */
+        or_daxpy(1., z, x, n);
+        or_residual(A, b, x, r);
+        norm = or_dnrm2(r, n);
+        niter++;
+    }
+    if (rc == 0 && !ninv) { set_err(err, errlen, "No need to iterate"); rc = 2; }   /* :65-82 */
+    if (st) {
+        st->niter = niter;
+        st->t_sol = now_s() - gtime;
+        st->init_norm = init_norm;
+        st->final_norm = norm;
+        st->eps_used = eps;
+    }
+    free(r); free(z);
+    return rc;
+}
+
 /* ------------------------------------------------------------------ */
 /* raw-array wrappers                                                  */
 /* ------------------------------------------------------------------ */
@@ -1223,6 +1312,26 @@ int or_gmres_csr(uint32_t n, const uint32_t* ia, const uint32_t* ja, const doubl
     or_mat A; or_stats st = {0};
     mat_view(&A, n, ia, ja, a);
     int rc = or_gmres(&A, b, B, x, &st, eps, x0_mode, maxit, err, errlen);
+    stats_out(&st, stats6);
+    return rc;
+}
+
+int or_cheb_solver_csr(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, double lmin,
+                       double lmax, const double* b, const or_prec* B, double* x, double* stats6, double eps,
+                       int x0_mode, uint32_t maxit, char* err, size_t errlen) {
+    or_mat A; or_stats st = {0};
+    mat_view(&A, n, ia, ja, a);
+    int rc = or_cheb_solver(&A, lmin, lmax, b, B, x, &st, eps, x0_mode, maxit, err, errlen);
+    stats_out(&st, stats6);
+    return rc;
+}
+
+int or_simple_solver_csr(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, const double* b,
+                         const or_prec* B, double* x, double* stats6, double eps, int x0_mode, uint32_t maxit,
+                         char* err, size_t errlen) {
+    or_mat A; or_stats st = {0};
+    mat_view(&A, n, ia, ja, a);
+    int rc = or_simple_solver(&A, b, B, x, &st, eps, x0_mode, maxit, err, errlen);
     stats_out(&st, stats6);
     return rc;
 }

@@ -95,7 +95,18 @@ int or_pcg(const or_mat* A, const double* b, const or_prec* B, double* x, or_sta
 int or_gmres(const or_mat* A, const double* b, const or_prec* B, double* x, or_stats* st,
              double eps, int x0_mode, uint32_t maxit, char* err, size_t errlen);
 
+int or_cheb_solver(const or_mat* A, double lmin, double lmax, const double* b, const or_prec* B, double* x,
+                   or_stats* st, double eps, int x0_mode, uint32_t maxit, char* err, size_t errlen);
+int or_simple_solver(const or_mat* A, const double* b, const or_prec* B, double* x, or_stats* st,
+                     double eps, int x0_mode, uint32_t maxit, char* err, size_t errlen);
+
 /* convenience wrappers taking raw CSR arrays (used through ctypes) */
+int or_cheb_solver_csr(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, double lmin,
+                       double lmax, const double* b, const or_prec* B, double* x, double* stats_out6, double eps,
+                       int x0_mode, uint32_t maxit, char* err, size_t errlen);
+int or_simple_solver_csr(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, const double* b,
+                         const or_prec* B, double* x, double* stats_out6, double eps, int x0_mode,
+                         uint32_t maxit, char* err, size_t errlen);
 int or_pcg_csr(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
                const double* b, const or_prec* B, double* x, double* stats_out6,
                double eps, int x0_mode, uint32_t maxit, char* err, size_t errlen);
