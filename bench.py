@@ -203,6 +203,9 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    for o in (B32, B64, A):
+        o.close()
+    del xv, yv, f, z, b, x
     ctx.close()
 
 

@@ -89,7 +89,7 @@ struct mlp_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
     mlp::DevMem mem;
-    double* partials = nullptr;        // kRedBlocks * kPartialsPerBlock
+    double* partials = nullptr;        // kPartials doubles
     unsigned* counters = nullptr;      // 8
     double* scal = nullptr;            // 4 result slots for standalone BLAS-1
     mlp::SolveState* st = nullptr;
@@ -99,10 +99,36 @@ struct mlp_ctx_s {
     mlp::DevMem ws_mem;
     double *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *zp = nullptr, *V = nullptr;
     double* mdot = nullptr;            // 32 results of the Gram-Schmidt dots
+    uint64_t next_id = 1;
+    // instantiated PCG iteration graphs, reused while the same matrix,
+    // preconditioner and vectors are solved again
+    struct GraphEntry {
+        uint64_t A, B;
+        const void *b, *x;
+        int flex;
+        uint32_t tre, ws_n;
+        hipGraph_t g[2] = {nullptr, nullptr};
+        hipGraphExec_t e[2] = {nullptr, nullptr};
+    };
+    std::vector<GraphEntry> graphs;
+    void drop_graphs(uint64_t id) {
+        for (size_t i = 0; i < graphs.size();) {
+            if (id == 0 || graphs[i].A == id || graphs[i].B == id) {
+                for (int k = 0; k < 2; k++) {
+                    if (graphs[i].e[k]) (void)hipGraphExecDestroy(graphs[i].e[k]);
+                    if (graphs[i].g[k]) (void)hipGraphDestroy(graphs[i].g[k]);
+                }
+                graphs.erase(graphs.begin() + i);
+            } else {
+                i++;
+            }
+        }
+    }
 };
 
 struct mlp_mat_s {
     mlp_ctx ctx = nullptr;
+    uint64_t id = 0;                   // unique per context lifetime (graph cache key)
     uint32_t n = 0, nnz = 0;
     mlp::DevMem mem;
     mlp::DevSell<double> A;
@@ -130,14 +156,12 @@ template <class VT> struct DevLevel {
     double ma2 = 0; int s2_skip = 0; double opb2 = 0; int opb2_one = 0;
     std::vector<double> ak, bk;               // steps 3..ncheb
     uint32_t* map = nullptr;
-    TailView<VT> tail;
-    uint32_t tail_smem = 0;
-    uint32_t *Sptr = nullptr, *Scol = nullptr;
-    VT* Sval = nullptr;
+    ProgView<VT> prog;
+    uint32_t lds_down = 0, lds_up = 0, lds_solve = 0;
     VT* dval = nullptr;
     DevSell<VT> A;                             // this level's matrix (levels >= 1)
     VT *wt = nullptr, *F = nullptr, *tmp = nullptr, *u1 = nullptr, *u0 = nullptr, *x2 = nullptr;
-    VT *r = nullptr, *uc = nullptr;           // multi_split
+    VT* r = nullptr;                           // multi_split inner residual
     // bookkeeping for the byte model
     uint32_t nnzL = 0, nnzU = 0;
 };
@@ -149,6 +173,8 @@ struct DevPrecBase {
     virtual uint32_t launches_per_apply() const = 0;
     virtual bool graph_safe() const { return true; }
 };
+
+constexpr uint32_t kLdsBudget = 60 * 1024;
 
 template <class VT> struct DevHier : DevPrecBase {
     mlp_ctx ctx;
@@ -162,11 +188,41 @@ template <class VT> struct DevHier : DevPrecBase {
 
     hipStream_t stream() const { return ctx->stream; }
 
+    // Launch shape of the component blocks: `lbs` threads per block and at
+    // most `cap` nodes per block (a larger component gets a block of its own).
+    // MLP_LEVEL_BLOCK / MLP_LEVEL_CAP override the defaults for tuning.
+    uint32_t lbs = kLBS, cap0 = 256;
+
+    void read_launch_shape() {
+        if (const char* e = getenv("MLP_LEVEL_BLOCK")) {
+            const uint32_t v = uint32_t(atoi(e));
+            if (v != 64 && v != 128 && v != 256) throw Error("MLP_LEVEL_BLOCK must be 64, 128 or 256", 1);
+            lbs = v;
+        }
+        if (const char* e = getenv("MLP_LEVEL_CAP")) {
+            const uint32_t v = uint32_t(atoi(e));
+            if (v < 8 || v > 4096) throw Error("MLP_LEVEL_CAP must lie in [8, 4096]", 1);
+            cap0 = v;
+        }
+    }
+
+    // plan a level with the largest block size whose LDS needs fit the budget
+    void plan_fitting(const LevelArrays& la, LevelProgram& P) const {
+        for (uint32_t cap = cap0;; cap /= 2) {
+            plan_level(la, cap, P);
+            const LdsPlan s = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, true);
+            if (s.total <= kLdsBudget || cap <= 8) break;
+        }
+        const LdsPlan s = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, true);
+        if (s.total > 64 * 1024) throw Error("an elimination component is too large for LDS", 1);
+    }
+
     void build(const std::vector<mlp_level_desc>& d, int kind_, mlp_mat A0_) {
         kind = kind_;
         A0 = A0_;
         nlevels = uint32_t(d.size());
         lv.resize(nlevels);
+        read_launch_shape();
         hipStream_t s = stream();
         for (uint32_t l = 0; l < nlevels; l++) {
             const mlp_level_desc& D = d[l];
@@ -174,7 +230,7 @@ template <class VT> struct DevHier : DevPrecBase {
             L.N = D.N; L.M = D.M; L.Md = D.Md; L.nit = D.nit; L.eps = D.eps;
             if (L.M + L.Md > L.N) throw Error("level sizes inconsistent", 1);
             L.n = L.N - L.M - L.Md;
-            if (l + 1 == nlevels && L.n != 0) throw Error("the coarsest level must have no Schur part", 1);
+            if (l + 1 == nlevels && L.n != 0) throw Error("the coarsest level must have no kept rows", 1);
             if (l + 1 < nlevels && d[l + 1].N != L.n) throw Error("level sizes do not chain", 1);
             if (L.eps != 0.0) dynamic = true;
             LevelArrays la;
@@ -185,35 +241,37 @@ template <class VT> struct DevHier : DevPrecBase {
             L.map = mem.upload(mapv, s);
             L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
             L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
-            TailLayout T;
-            plan_tail(la, 256, T);
-            if (T.M) {
-                L.tail.nblk = T.nblk;
-                L.tail.blk = mem.upload(T.blk, s);
-                L.tail.lvlL = mem.upload(T.lvlL, s);
-                L.tail.lvlU = mem.upload(T.lvlU, s);
-                L.tail.fidx = mem.upload(T.fidx, s);
-                L.tail.Lptr = mem.upload(T.Lptr, s);
-                L.tail.Lloc = mem.upload(T.Lloc, s);
-                L.tail.Lval = mem.upload_as<VT>(T.Lval, s);
-                L.tail.uloc = mem.upload(T.uloc, s);
-                L.tail.uxidx = mem.upload(T.uxidx, s);
-                L.tail.Uptr = mem.upload(T.Uptr, s);
-                L.tail.Ucol = mem.upload(T.Ucol, s);
-                L.tail.Uval = mem.upload_as<VT>(T.Uval, s);
-                L.tail.Udiag = mem.upload_as<VT>(T.Udiag, s);
-                L.tail_smem = T.max_nodes * sizeof(VT);
-                if (L.tail_smem > 64 * 1024) throw Error("elimination component too large for LDS", 1);
+            LevelProgram P;
+            plan_fitting(la, P);
+            ProgView<VT>& G = L.prog;
+            G.nblk = P.nblk;
+            G.nflat = uint32_t(P.flat.size());
+            G.max_nodes = P.max_nodes; G.max_Lent = P.max_Lent; G.max_Uent = P.max_Uent; G.max_Unodes = P.max_Unodes;
+            G.flat = mem.upload(P.flat, s);
+            if (P.nblk) {
+                G.blk = mem.upload(P.blk, s);
+                G.ublk = mem.upload(P.ublk, s);
+                G.lvlL = mem.upload(P.lvlL, s);
+                G.lvlU = mem.upload(P.lvlU, s);
+                G.fidx = mem.upload(P.fidx, s);
+                G.out = mem.upload(P.out, s);
+                G.Lptr = mem.upload(P.Lptr, s);
+                G.Lloc = mem.upload(P.Lloc, s);
+                G.Lval = mem.upload_as<VT>(P.Lval, s);
+                G.uloc = mem.upload(P.uloc, s);
+                G.uxidx = mem.upload(P.uxidx, s);
+                G.Uptr = mem.upload(P.Uptr, s);
+                G.Ucol = mem.upload(P.Ucol, s);
+                G.Uval = mem.upload_as<VT>(P.Uval, s);
+                G.Udiag = mem.upload_as<VT>(P.Udiag, s);
+                L.wt = mem.alloc<VT>(P.nodes);
             }
-            SchurLayout S;
-            plan_schur(la, T, S);
-            L.Sptr = mem.upload(S.ptr, s);
-            L.Scol = mem.upload(S.col, s);
-            L.Sval = mem.upload_as<VT>(S.val, s);
+            L.lds_down = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, false).total;
+            L.lds_up = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, false, true).total;
+            L.lds_solve = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, true).total;
             std::vector<double> dv(D.dval, D.dval + D.Md);
             L.dval = mem.upload_as<VT>(dv, s);
             if (l > 0) upload_sell<VT>(mem, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A);
-            if (T.M) L.wt = mem.alloc<VT>(T.M);
             if (L.n) {
                 L.F = mem.alloc<VT>(L.n);
                 L.tmp = mem.alloc<VT>(L.n);
@@ -221,14 +279,12 @@ template <class VT> struct DevHier : DevPrecBase {
                 L.u0 = mem.alloc<VT>(L.n);
                 L.x2 = mem.alloc<VT>(L.n);
             }
-            if (kind == MLP_PREC_MULTI_SPLIT && l > 0) {
-                L.r = mem.alloc<VT>(L.N);
-                L.uc = mem.alloc<VT>(L.N);
-            }
+            if (kind == MLP_PREC_MULTI_SPLIT && l > 0) L.r = mem.alloc<VT>(L.N);
         }
         if (kind == MLP_PREC_UH_CHEB) {
             for (uint32_t l = 0; l + 1 < nlevels; l++) {
                 DevLevel<VT>& L = lv[l];
+                if (L.nit == 0) throw Error("ncheb must be positive", 1);
                 const double lmin = d[l + 1].lmin, lmax = d[l + 1].lmax;
                 const double eta = (lmax + lmin) / (lmax - lmin);
                 L.a1 = 2 / (lmax + lmin);
@@ -243,7 +299,6 @@ template <class VT> struct DevHier : DevPrecBase {
                     L.ak.push_back(4 / (lmax - lmin) * cheb_poly(eta, i - 1) / cheb_poly(eta, i));
                     L.bk.push_back(cheb_poly(eta, i - 2) / cheb_poly(eta, i));
                 }
-                if (L.nit == 0) throw Error("ncheb must be positive", 1);
             }
         }
         if (kind == MLP_PREC_MULTI_SPLIT && nlevels && lv[0].nit > 1)
@@ -259,19 +314,22 @@ template <class VT> struct DevHier : DevPrecBase {
         launches++;
     }
 
-    template <class FT>
-    void tail_L(const DevLevel<VT>& L, const FT* f, const uint32_t* g) const {
-        if (L.M) launch(k_tail_L<VT, FT>, L.tail.nblk, 128, L.tail_smem, L.tail, f, L.wt, g);
+    template <class FT, class XT, int DMODE>
+    void down(const DevLevel<VT>& L, const FT* f, XT* x, const uint32_t* g) const {
+        const uint32_t grid = L.prog.nblk + cdiv(uint64_t(L.prog.nflat) + L.Md, lbs);
+        launch(k_level_down<VT, FT, XT, DMODE>, grid, lbs, L.lds_down, L.prog, f, L.wt, L.F, (const uint32_t*)L.map,
+               L.M, L.N, L.Md, (const VT*)L.dval, x, g);
     }
-    template <class XT, int MODE>
-    void tail_U(const DevLevel<VT>& L, const XT* xin, XT* xout, const uint32_t* g) const {
-        if (L.M) launch(k_tail_U<VT, XT, MODE>, L.tail.nblk, 128, L.tail_smem, L.tail, (const VT*)L.wt, xin, xout, g);
+    template <class XT, int OP, int UMODE>
+    void up(const DevLevel<VT>& L, const Combine<VT>& C, XT* x, const uint32_t* g) const {
+        const uint32_t grid = L.prog.nblk + cdiv(L.n, lbs);
+        launch(k_level_up<VT, XT, OP, UMODE>, grid, lbs, L.lds_up, L.prog, (const VT*)L.wt, C,
+               (const uint32_t*)(L.map + L.M), L.n, x, g);
     }
     template <class FT, class XT, int MODE>
-    void schur_diag(const DevLevel<VT>& L, const FT* f, XT* x, const uint32_t* g) const {
-        uint32_t tot = L.n + L.Md;
-        launch(k_schur_diag<VT, FT, XT, MODE>, cdiv(tot, kBS), kBS, 0, L.n, L.Md, L.M, L.N, (const uint32_t*)L.map,
-               (const uint32_t*)L.Sptr, (const uint32_t*)L.Scol, (const VT*)L.Sval, (const VT*)L.wt, f, L.F,
+    void solve_last(const DevLevel<VT>& L, const FT* f, XT* x, const uint32_t* g) const {
+        const uint32_t grid = L.prog.nblk + cdiv(L.Md, lbs);
+        launch(k_level_solve<VT, FT, XT, MODE>, grid, lbs, L.lds_solve, L.prog, f, (const uint32_t*)L.map, L.N, L.Md,
                (const VT*)L.dval, x, g);
     }
 
@@ -279,79 +337,77 @@ template <class VT> struct DevHier : DevPrecBase {
     template <class FT, class XT>
     void cheb(uint32_t l, const FT* f, XT* x, const uint32_t* g) const {
         const DevLevel<VT>& L = lv[l];
-        tail_L(L, f, g);
-        schur_diag<FT, XT, 0>(L, f, x, g);
-        if (l + 1 < nlevels) {
-            const DevLevel<VT>& Ln = lv[l + 1];
-            const uint32_t n = L.n, gb = cdiv(n, kBS);
-            const uint32_t* mapM = L.map + L.M;
-            if (L.nit == 1) {
-                cheb<VT, VT>(l + 1, L.F, L.x2, g);
-                launch(k_cheb1<VT, XT>, gb, kBS, 0, n, (const VT*)L.x2, VT(L.a1), L.s1_skip, mapM, x, g);
+        if (l + 1 == nlevels) {
+            solve_last<FT, XT, 0>(L, f, x, g);
+            return;
+        }
+        down<FT, XT, 0>(L, f, x, g);
+        const DevLevel<VT>& Ln = lv[l + 1];
+        const uint32_t n = L.n, gb = cdiv(n, kBS);
+        Combine<VT> C;
+        if (L.nit == 1) {
+            cheb<VT, VT>(l + 1, L.F, L.x2, g);
+            C.x2 = L.x2; C.a = VT(L.a1); C.skip = L.s1_skip;
+            up<XT, OP_SCALE, 0>(L, C, x, g);
+            return;
+        }
+        cheb<VT, VT>(l + 1, L.F, L.u1, g);
+        // tmp = A (a1 u1) - F ; u1s = a1 u1 (kept in u0)
+        if (L.s1_skip)
+            launch(k_cheb_res<VT, false>, gb, kBS, 0, Ln.A.v, (const VT*)L.u1, (const VT*)L.F, L.tmp, L.u0, VT(1), g);
+        else
+            launch(k_cheb_res<VT, true>, gb, kBS, 0, Ln.A.v, (const VT*)L.u1, (const VT*)L.F, L.tmp, L.u0, VT(L.a1), g);
+        VT* u1s = L.s1_skip ? L.u1 : L.u0;
+        cheb<VT, VT>(l + 1, L.tmp, L.x2, g);
+        if (L.nit == 2) {
+            C.x2 = L.x2; C.u1 = u1s; C.a = VT(L.ma2); C.skip = L.s2_skip; C.b = VT(L.opb2); C.one = L.opb2_one;
+            up<XT, OP_CHEB2, 0>(L, C, x, g);
+            return;
+        }
+        // three rotating buffers: prev (previous iterate), cur (latest), spare
+        VT* prev = u1s;
+        VT* spare = nullptr;
+        for (VT* b : {L.u1, L.u0}) if (b != prev) spare = b;
+        launch(k_cheb2<VT, XT, true>, gb, kBS, 0, n, (const VT*)L.x2, (const VT*)prev, VT(L.ma2), L.s2_skip, VT(L.opb2),
+               L.opb2_one, (const uint32_t*)nullptr, (XT*)nullptr, spare, g);
+        VT* cur = spare;
+        spare = L.x2;
+        for (uint32_t i = 3; i <= L.nit; i++) {
+            launch(k_cheb_res<VT, false>, gb, kBS, 0, Ln.A.v, (const VT*)cur, (const VT*)L.F, L.tmp, (VT*)nullptr, VT(1),
+                   g);
+            cheb<VT, VT>(l + 1, L.tmp, spare, g);
+            const VT a = VT(L.ak[i - 3]), b = VT(L.bk[i - 3]);
+            if (i == L.nit) {
+                C.x2 = spare; C.u1 = cur; C.u0 = prev; C.a = a; C.b = b;
+                up<XT, OP_CHEBK, 0>(L, C, x, g);
             } else {
-                cheb<VT, VT>(l + 1, L.F, L.u1, g);
-                // tmp = A (a1 u1) - F ; u1s = a1 u1   (u1s lives in u0)
-                if (L.s1_skip)
-                    launch(k_cheb_res<VT, false>, gb, kBS, 0, Ln.A.v, (const VT*)L.u1, (const VT*)L.F, L.tmp, L.u0, VT(1), g);
-                else
-                    launch(k_cheb_res<VT, true>, gb, kBS, 0, Ln.A.v, (const VT*)L.u1, (const VT*)L.F, L.tmp, L.u0, VT(L.a1), g);
-                const VT* u1s = L.s1_skip ? L.u1 : L.u0;
-                cheb<VT, VT>(l + 1, L.tmp, L.x2, g);
-                if (L.nit == 2) {
-                    launch(k_cheb2<VT, XT, false>, gb, kBS, 0, n, (const VT*)L.x2, u1s, VT(L.ma2), L.s2_skip, VT(L.opb2),
-                           L.opb2_one, mapM, x, (VT*)nullptr, g);
-                } else {
-                    // rotate three buffers: prev (u1 of the previous step), cur (latest), free
-                    VT* bufs[3] = {L.u1, L.u0, L.x2};
-                    VT* prev = const_cast<VT*>(u1s);
-                    VT* cur = nullptr;
-                    VT* freeb = nullptr;
-                    for (VT* b : bufs) if (b != prev && b != L.x2) freeb = b;
-                    // step-2 result goes into `freeb`
-                    launch(k_cheb2<VT, XT, true>, gb, kBS, 0, n, (const VT*)L.x2, (const VT*)prev, VT(L.ma2), L.s2_skip,
-                           VT(L.opb2), L.opb2_one, mapM, x, freeb, g);
-                    cur = freeb;
-                    freeb = L.x2;
-                    for (uint32_t i = 3; i <= L.nit; i++) {
-                        launch(k_cheb_res<VT, false>, gb, kBS, 0, Ln.A.v, (const VT*)cur, (const VT*)L.F, L.tmp,
-                               (VT*)nullptr, VT(1), g);
-                        cheb<VT, VT>(l + 1, L.tmp, freeb, g);
-                        const VT a = VT(L.ak[i - 3]), b = VT(L.bk[i - 3]);
-                        if (i == L.nit) {
-                            launch(k_cheb_step<VT, XT, false>, gb, kBS, 0, n, (const VT*)cur, (const VT*)prev,
-                                   (const VT*)freeb, a, b, mapM, x, (VT*)nullptr, g);
-                        } else {
-                            launch(k_cheb_step<VT, XT, true>, gb, kBS, 0, n, (const VT*)cur, (const VT*)prev,
-                                   (const VT*)freeb, a, b, mapM, x, freeb, g);
-                            VT* t = prev;
-                            prev = cur;
-                            cur = freeb;
-                            freeb = t;
-                        }
-                    }
-                }
+                launch(k_cheb_step<VT, XT, true>, gb, kBS, 0, n, (const VT*)cur, (const VT*)prev, (const VT*)spare, a, b,
+                       (const uint32_t*)nullptr, (XT*)nullptr, spare, g);
+                VT* t = prev;
+                prev = cur;
+                cur = spare;
+                spare = t;
             }
         }
-        tail_U<XT, 0>(L, x, x, g);
     }
 
     // MultiSplitPrec::solve(level, f, x)  (multi_split_solve.cc:59-157)
     template <class FT, class XT>
     void msplit(uint32_t l, const FT* f, XT* x, const uint32_t* g) const {
         const DevLevel<VT>& L = lv[l];
-        const bool has_next = l + 1 < nlevels;
-        const uint32_t gb = cdiv(L.n, kBS);
-        const uint32_t* mapM = L.map + L.M;
-        tail_L(L, f, g);
-        schur_diag<FT, XT, 0>(L, f, x, g);
-        if (has_next) {
+        const bool last = l + 1 == nlevels;
+        Combine<VT> C;
+        C.x2 = L.x2;
+        if (last) {
+            solve_last<FT, XT, 0>(L, f, x, g);
+        } else {
+            down<FT, XT, 0>(L, f, x, g);
             msplit<VT, VT>(l + 1, L.F, L.x2, g);
-            launch(k_scatter<VT, XT, 0>, gb, kBS, 0, L.n, (const VT*)L.x2, mapM, (XT*)nullptr, x, g);
+            up<XT, OP_COPY, 0>(L, C, x, g);
         }
-        tail_U<XT, 0>(L, x, x, g);
         if (L.nit <= 1 && L.eps == 0.0) return;
         if (l == 0) throw Error("multi_split level 0 runs one iteration", 1);
-        // steps 2.. (levels >= 1, so FT == XT == VT here)
+        // steps 2.. (levels >= 1, so FT == XT == VT here): x += solve(f - A x)
         const VT* fv = reinterpret_cast<const VT*>(f);
         VT* xv = reinterpret_cast<VT*>(x);
         double init_norm = 0.0;
@@ -362,13 +418,13 @@ template <class VT> struct DevHier : DevPrecBase {
                 double nr = host_nrm2(L.r, L.N);
                 if (nr / init_norm < L.eps) break;
             }
-            tail_L(L, (const VT*)L.r, g);
-            schur_diag<VT, VT, 1>(L, (const VT*)L.r, xv, g);
-            if (has_next) {
+            if (last) {
+                solve_last<VT, VT, 1>(L, (const VT*)L.r, xv, g);
+            } else {
+                down<VT, VT, 1>(L, (const VT*)L.r, xv, g);
                 msplit<VT, VT>(l + 1, L.F, L.x2, g);
-                launch(k_scatter<VT, VT, 1>, gb, kBS, 0, L.n, (const VT*)L.x2, mapM, L.uc, xv, g);
+                up<VT, OP_COPY, 1>(L, C, xv, g);
             }
-            tail_U<VT, 1>(L, (const VT*)L.uc, xv, g);
         }
     }
 
@@ -417,7 +473,7 @@ template <class VT> struct DevHier : DevPrecBase {
     }
     uint32_t launches_per_apply() const override {
         uint32_t before = launches;
-        return before;   // filled by a counted dry run in mlp_prec_bytes
+        return before;
     }
 };
 
@@ -438,6 +494,7 @@ struct SimplePrec : DevPrecBase {
 
 struct mlp_prec_s {
     mlp_ctx ctx = nullptr;
+    uint64_t id = 0;
     int kind = 0, precision = 0;
     uint32_t n = 0;
     std::unique_ptr<mlp::DevPrecBase> impl;
@@ -479,7 +536,7 @@ int mlp_ctx_create(int device, mlp_ctx* out) {
     auto* c = new mlp_ctx_s();
     c->device = device;
     MLP_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    c->partials = c->mem.alloc<double>(size_t(kPartialsPerBlock) * kRedBlocks);
+    c->partials = c->mem.alloc<double>(kPartials);
     c->counters = c->mem.alloc<unsigned>(8);
     c->scal = c->mem.alloc<double>(4);
     c->st = c->mem.alloc<SolveState>(1);
@@ -495,6 +552,7 @@ int mlp_ctx_destroy(mlp_ctx ctx) {
     if (!ctx) return MLP_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    ctx->drop_graphs(0);
     ctx->ws_mem.release();
     ctx->mem.release();
     if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
@@ -570,6 +628,7 @@ int mlp_csr_upload(mlp_ctx ctx, uint32_t n, const uint32_t* ia, const uint32_t* 
     auto* m = new mlp_mat_s();
     try {
         m->ctx = ctx;
+        m->id = ctx->next_id++;
         m->n = n;
         m->nnz = ia[n];
         m->ia.assign(ia, ia + n + 1);
@@ -589,6 +648,7 @@ int mlp_mat_destroy(mlp_mat m) {
     MLP_API_BEGIN
     if (m) {
         (void)hipStreamSynchronize(m->ctx->stream);
+        m->ctx->drop_graphs(m->id);
         delete m;
     }
     MLP_API_END
@@ -622,10 +682,18 @@ int mlp_residual(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, doubl
 }
 
 // ---- BLAS-1 ----
-static uint32_t red_grid(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n, kBS), kRedBlocks)); }
+// Streaming reductions run a grid-stride loop over at most kRedGrid blocks
+// (8 per CU), which keeps the one-block finalize short.
+constexpr uint32_t kRedGrid = 2048;
+static_assert(kRedGrid <= kRedBlocks, "partials buffer too small");
+static uint32_t red_grid(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n, kBS), kRedGrid)); }
+static uint32_t mdot_grid(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n, kBS), kMdotBlocks)); }
 
 static double dev_dot(mlp_ctx ctx, uint32_t n, const double* x, const double* y) {
-    hipLaunchKernelGGL(k_dot, dim3(red_grid(n)), dim3(kBS), 0, ctx->stream, n, x, y, ctx->partials, ctx->counters, ctx->scal);
+    const uint32_t g = red_grid(n);
+    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBS), 0, ctx->stream, n, x, y, ctx->partials);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, ctx->stream, int(FIN_DOT), g, (const double*)ctx->partials,
+                       ctx->st, ctx->scal);
     MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, ctx->scal, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return ctx->host_pinned[0];
@@ -803,6 +871,7 @@ int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_des
     require(levels[0].N == A0->n, "level 0 size differs from A0");
     auto p = std::make_unique<mlp_prec_s>();
     p->ctx = ctx;
+    p->id = ctx->next_id++;
     p->kind = kind;
     p->precision = precision;
     p->n = A0->n;
@@ -854,6 +923,7 @@ int mlp_prec_simple(mlp_ctx ctx, int kind, mlp_mat A0, mlp_prec* out) {
     }
     auto p = std::make_unique<mlp_prec_s>();
     p->ctx = ctx;
+    p->id = ctx->next_id++;
     p->kind = kind;
     p->n = A0->n;
     p->impl = std::move(sp);
@@ -865,6 +935,7 @@ int mlp_prec_destroy(mlp_prec p) {
     MLP_API_BEGIN
     if (p) {
         (void)hipStreamSynchronize(p->ctx->stream);
+        p->ctx->drop_graphs(p->id);
         delete p;
     }
     MLP_API_END
@@ -885,15 +956,15 @@ int mlp_time_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y, uint32_t r
     double* buf = nullptr;
     const size_t nf = flush_bytes / sizeof(double);
     if (nf) {
-        MLP_HIP_CHECK(hipMalloc(&buf, nf * sizeof(double)));
-        MLP_HIP_CHECK(hipMemsetAsync(buf, 0, nf * sizeof(double), ctx->stream));
+        MLP_HIP_CHECK(hipMalloc(&buf, (nf + 1) * sizeof(double)));
+        MLP_HIP_CHECK(hipMemsetAsync(buf, 0, (nf + 1) * sizeof(double), ctx->stream));
     }
     hipEvent_t e0, e1;
     MLP_HIP_CHECK(hipEventCreate(&e0));
     MLP_HIP_CHECK(hipEventCreate(&e1));
     double tot = 0.0;
     for (uint32_t r = 0; r < reps; r++) {
-        if (nf) hipLaunchKernelGGL(k_flush, dim3(4096), dim3(kBS), 0, ctx->stream, nf, buf);
+        if (nf) hipLaunchKernelGGL(k_flush, dim3(4096), dim3(kBS), 0, ctx->stream, nf, (const double*)buf, buf + nf);
         MLP_HIP_CHECK(hipEventRecord(e0, ctx->stream));
         hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream, A->A.v,
                            x, y, (const uint32_t*)nullptr);
@@ -949,6 +1020,7 @@ constexpr uint32_t kGmresRestart = 30;     // gmres.cc:21
 void ensure_workspace(mlp_ctx ctx, uint32_t n, bool gmres) {
     if (ctx->ws_n >= n && (!gmres || ctx->V)) return;
     MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->drop_graphs(0);
     ctx->ws_mem.release();
     ctx->V = nullptr;
     const uint32_t m = std::max(n, ctx->ws_n);
@@ -965,6 +1037,10 @@ void ensure_workspace(mlp_ctx ctx, uint32_t n, bool gmres) {
 void fill_x0(mlp_ctx ctx, uint32_t n, int mode, double* x) {
     if (mode == MLP_X0_GIVEN) return;
     if (mode != MLP_X0_ZERO && mode != MLP_X0_REFERENCE) throw Error("unknown x0 mode", MLP_ERR_INVALID);
+    if (mode == MLP_X0_ZERO) {
+        MLP_HIP_CHECK(hipMemsetAsync(x, 0, size_t(n) * sizeof(double), ctx->stream));
+        return;
+    }
     std::vector<double> h(n, 0.0);
     if (mode == MLP_X0_REFERENCE) {        // generate_x0, modules/solvers/solvers.cc:9-21
         srandom(3);
@@ -1031,49 +1107,71 @@ SolveState pcg_phase(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double
     if (*no_iter) return st0;
     B->impl->apply(ctx->r, ctx->z, nullptr);
     const uint32_t rg = red_grid(n), sg = cdiv(n, kBS);
+    auto fin = [&](int op) {
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, op, rg, (const double*)ctx->partials, ctx->st,
+                           (double*)nullptr);
+    };
     if (flex)
         hipLaunchKernelGGL(k_pcg_init<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                           ctx->p, ctx->zp, ctx->st, ctx->partials);
+                           ctx->p, ctx->zp, ctx->partials);
     else
         hipLaunchKernelGGL(k_pcg_init<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                           ctx->p, ctx->zp, ctx->st, ctx->partials);
+                           ctx->p, ctx->zp, ctx->partials);
+    fin(FIN_INIT);
     const uint32_t* guard = &ctx->st->done;
     // replace = true: the recurrence residual is replaced by b - A x before the
     // preconditioner is applied, and the stop test uses the replaced residual
     auto iteration = [&](bool replace) {
-        hipLaunchKernelGGL(k_pcg_spmv_dot, dim3(rg), dim3(kBS), 0, s, A->A.v, (const double*)ctx->p, ctx->Ap, ctx->st,
-                           ctx->partials);
+        hipLaunchKernelGGL(k_pcg_spmv_dot, dim3(rg), dim3(kBS), 0, s, A->A.v, (const double*)ctx->p, ctx->Ap,
+                           (const SolveState*)ctx->st, ctx->partials);
+        fin(FIN_APP);
+        hipLaunchKernelGGL(k_pcg_xr, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->p, (const double*)ctx->Ap, x,
+                           ctx->r, (const SolveState*)ctx->st, ctx->partials);
         if (replace) {
-            hipLaunchKernelGGL(k_pcg_xr<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->p,
-                               (const double*)ctx->Ap, x, ctx->r, ctx->st, ctx->partials);
-            hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)x, ctx->r, ctx->st,
-                               ctx->partials);
+            fin(FIN_XR_NOTEST);
+            hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)x, ctx->r,
+                               (const SolveState*)ctx->st, ctx->partials);
+            fin(FIN_TRUE_RES);
         } else {
-            hipLaunchKernelGGL(k_pcg_xr<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->p,
-                               (const double*)ctx->Ap, x, ctx->r, ctx->st, ctx->partials);
+            fin(FIN_XR);
         }
         B->impl->apply(ctx->r, ctx->z, guard);
         if (flex) {
             hipLaunchKernelGGL(k_pcg_rz<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                               (const double*)ctx->zp, ctx->st, ctx->partials);
-            hipLaunchKernelGGL(k_pcg_p<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp, ctx->st);
+                               (const double*)ctx->zp, (const SolveState*)ctx->st, ctx->partials);
+            fin(FIN_RZ_FLEX);
+            hipLaunchKernelGGL(k_pcg_p<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp,
+                               (const SolveState*)ctx->st);
         } else {
             hipLaunchKernelGGL(k_pcg_rz<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                               (const double*)ctx->zp, ctx->st, ctx->partials);
-            hipLaunchKernelGGL(k_pcg_p<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp, ctx->st);
+                               (const double*)ctx->zp, (const SolveState*)ctx->st, ctx->partials);
+            fin(FIN_RZ);
+            hipLaunchKernelGGL(k_pcg_p<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp,
+                               (const SolveState*)ctx->st);
         }
     };
     const uint32_t chunk = o->check_every ? o->check_every : 16;
     const uint32_t tre = o->true_res_every;
     const bool use_graph = o->use_graph && B->impl->graph_safe();
-    GraphPair gp;
+    mlp_ctx_s::GraphEntry* ge = nullptr;
     if (use_graph) {
-        MLP_HIP_CHECK(hipStreamSynchronize(s));
-        for (int v = 0; v < (tre ? 2 : 1); v++) {
-            MLP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            iteration(v == 1);
-            MLP_HIP_CHECK(hipStreamEndCapture(s, &gp.g[v]));
-            MLP_HIP_CHECK(hipGraphInstantiate(&gp.e[v], gp.g[v], nullptr, nullptr, 0));
+        for (auto& e : ctx->graphs)
+            if (e.A == A->id && e.B == B->id && e.b == b && e.x == x && e.flex == int(flex) && e.tre == tre &&
+                e.ws_n == ctx->ws_n)
+                ge = &e;
+        if (!ge) {
+            MLP_HIP_CHECK(hipStreamSynchronize(s));
+            mlp_ctx_s::GraphEntry e;
+            e.A = A->id; e.B = B->id; e.b = b; e.x = x; e.flex = int(flex); e.tre = tre; e.ws_n = ctx->ws_n;
+            for (int v = 0; v < (tre ? 2 : 1); v++) {
+                MLP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                iteration(v == 1);
+                MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[v]));
+                MLP_HIP_CHECK(hipGraphInstantiate(&e.e[v], e.g[v], nullptr, nullptr, 0));
+            }
+            if (ctx->graphs.size() >= 16) ctx->drop_graphs(ctx->graphs.front().A);
+            ctx->graphs.push_back(e);
+            ge = &ctx->graphs.back();
         }
     }
     SolveState st;
@@ -1082,7 +1180,7 @@ SolveState pcg_phase(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double
         for (uint32_t c = 0; c < chunk; c++) {
             k++;
             const bool replace = tre && (k % tre == 0);
-            if (use_graph) MLP_HIP_CHECK(hipGraphLaunch(gp.e[replace ? 1 : 0], s));
+            if (use_graph) MLP_HIP_CHECK(hipGraphLaunch(ge->e[replace ? 1 : 0], s));
             else iteration(replace);
         }
         MLP_HIP_CHECK(hipGetLastError());
@@ -1208,8 +1306,10 @@ int mlp_gmres(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, co
             B->impl->apply(v(i), z, nullptr);
             hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(n, kBS)), dim3(kBS), 0, s, A->A.v,
                                (const double*)z, v(i + 1), (const uint32_t*)nullptr);
-            hipLaunchKernelGGL(k_mdot, dim3(rg), dim3(kBS), 0, s, n, i + 1, (const double*)ctx->V, ld,
-                               (const double*)v(i + 1), ctx->partials, ctx->counters + 1, ctx->mdot);
+            hipLaunchKernelGGL(k_mdot, dim3(mdot_grid(n)), dim3(kBS), 0, s, n, i + 1, (const double*)ctx->V, ld,
+                               (const double*)v(i + 1), ctx->partials);
+            hipLaunchKernelGGL(k_mdot_finalize, dim3(1), dim3(kBS), 0, s, mdot_grid(n), i + 1,
+                               (const double*)ctx->partials, ctx->mdot);
             MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, ctx->mdot, (i + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
             MLP_HIP_CHECK(hipStreamSynchronize(s));
             MaxpyCoef cf;

@@ -23,36 +23,48 @@ struct DSU {
     }
 };
 
+constexpr uint32_t kNoNode = 0xffffffffu;
+
 }  // namespace
 
-void plan_tail(const LevelArrays& lv, uint32_t cap, TailLayout& T) {
-    const uint32_t M = lv.M;
-    T = TailLayout();
-    T.M = M;
-    T.pos_of_row.assign(M, 0);
-    T.Lptr.assign(M + 1, 0);
-    T.Uptr.assign(M + 1, 0);
-    if (M == 0) return;
+void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
+    const uint32_t N = lv.N, M = lv.M, Md = lv.Md, K = N - Md, n = N - M - Md;
+    P = LevelProgram();
+    P.M = M; P.n = n; P.Md = Md; P.N = N;
 
-    // validate the structure the kernels rely on
+    // the structure the kernels rely on
+    for (uint32_t i = 0; i < K; i++)
+        for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
+            uint32_t k = lv.L_ja[p];
+            if (k >= i || k >= M)
+                throw Error("L row " + std::to_string(i) + " has a column outside the eliminated block", 1);
+        }
     for (uint32_t i = 0; i < M; i++) {
-        for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++)
-            if (lv.L_ja[p] >= i) throw Error("L row " + std::to_string(i) + " is not strictly lower", 1);
         if (lv.U_ia[i + 1] <= lv.U_ia[i] || lv.U_ja[lv.U_ia[i]] != i)
             throw Error("U row " + std::to_string(i) + " does not start with its pivot", 1);
         for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++)
-            if (lv.U_ja[p] <= i || lv.U_ja[p] >= lv.N - lv.Md)
+            if (lv.U_ja[p] <= i || lv.U_ja[p] >= K)
                 throw Error("U row " + std::to_string(i) + " has a column outside (i, N-Md)", 1);
     }
 
-    // dependency components among the eliminated rows
-    DSU dsu(M);
-    std::vector<uint32_t> dL(M, 0), dU(M, 0);
-    for (uint32_t i = 0; i < M; i++)
+    // nodes: eliminated rows and kept rows fed by L entries
+    std::vector<char> is_node(K, 0);
+    for (uint32_t i = 0; i < M; i++) is_node[i] = 1;
+    for (uint32_t i = M; i < K; i++) {
+        if (lv.L_ia[i + 1] > lv.L_ia[i]) is_node[i] = 1;
+        else P.flat.push_back(i - M);
+    }
+
+    // dependency components and depths
+    DSU dsu(std::max<uint32_t>(K, 1));
+    std::vector<uint32_t> dL(K, 0), dU(M, 0);
+    for (uint32_t i = 0; i < K; i++) {
+        if (!is_node[i]) continue;
         for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
             dsu.unite(i, lv.L_ja[p]);
             dL[i] = std::max(dL[i], dL[lv.L_ja[p]] + 1);
         }
+    }
     for (uint32_t i = M; i-- > 0;)
         for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) {
             uint32_t j = lv.U_ja[p];
@@ -61,19 +73,22 @@ void plan_tail(const LevelArrays& lv, uint32_t cap, TailLayout& T) {
                 dU[i] = std::max(dU[i], dU[j] + 1);
             }
         }
-    // components in order of their first row
-    std::vector<uint32_t> comp_id(M), root_to_comp(M, 0xffffffffu);
+    for (uint32_t i = 0; i < K; i++) if (is_node[i]) P.max_depth = std::max(P.max_depth, dL[i]);
+    for (uint32_t i = 0; i < M; i++) P.max_depth = std::max(P.max_depth, dU[i]);
+
+    std::vector<uint32_t> comp_of(K, kNoNode), root_comp(std::max<uint32_t>(K, 1), kNoNode);
     uint32_t ncomp = 0;
-    for (uint32_t i = 0; i < M; i++) {
+    for (uint32_t i = 0; i < K; i++) {
+        if (!is_node[i]) continue;
         uint32_t r = dsu.find(i);
-        if (root_to_comp[r] == 0xffffffffu) root_to_comp[r] = ncomp++;
-        comp_id[i] = root_to_comp[r];
+        if (root_comp[r] == kNoNode) root_comp[r] = ncomp++;
+        comp_of[i] = root_comp[r];
     }
+    if (ncomp == 0) return;
     std::vector<uint32_t> csize(ncomp, 0);
-    for (uint32_t i = 0; i < M; i++) csize[comp_id[i]]++;
-    uint32_t maxc = *std::max_element(csize.begin(), csize.end());
-    if (maxc > cap) cap = maxc;
-    // greedy packing of consecutive components into blocks of at most cap rows
+    for (uint32_t i = 0; i < K; i++) if (is_node[i]) csize[comp_of[i]]++;
+    P.max_comp_size = *std::max_element(csize.begin(), csize.end());
+    cap = std::max(cap, P.max_comp_size);
     std::vector<uint32_t> comp_blk(ncomp);
     uint32_t nblk = 0, fill = 0;
     for (uint32_t c = 0; c < ncomp; c++) {
@@ -82,80 +97,78 @@ void plan_tail(const LevelArrays& lv, uint32_t cap, TailLayout& T) {
         fill += csize[c];
     }
     std::vector<std::vector<uint32_t>> rows(nblk);
-    for (uint32_t i = 0; i < M; i++) rows[comp_blk[comp_id[i]]].push_back(i);
+    for (uint32_t i = 0; i < K; i++) if (is_node[i]) rows[comp_blk[comp_of[i]]].push_back(i);
 
-    T.nblk = nblk;
-    T.blk.resize(6 * size_t(nblk));
-    T.fidx.resize(M);
-    T.uloc.resize(M);
-    T.uxidx.resize(M);
-    T.Udiag.resize(M);
-    std::vector<uint32_t> local(M);
+    P.nblk = nblk;
+    P.blk.resize(kBlkFields * size_t(nblk));
+    P.ublk.resize(2 * size_t(nblk));
+    uint32_t total = 0;
+    for (auto& R : rows) total += uint32_t(R.size());
+    P.nodes = total;
+    P.fidx.resize(total);
+    P.out.resize(total);
+    P.Lptr.assign(total + 1, 0);
+    P.uloc.resize(M);
+    P.uxidx.resize(M);
+    P.Udiag.resize(M);
+    P.Uptr.assign(M + 1, 0);
+    std::vector<uint32_t> local(K, kNoNode);
     uint32_t pos = 0, upos = 0;
     for (uint32_t b = 0; b < nblk; b++) {
         auto& R = rows[b];
         const uint32_t base = pos;
-        std::vector<uint32_t> lo = R, uo = R;
+        std::vector<uint32_t> lo = R;
         std::stable_sort(lo.begin(), lo.end(), [&](uint32_t x, uint32_t y) { return dL[x] < dL[y]; });
+        std::vector<uint32_t> uo;
+        for (uint32_t i : R) if (i < M) uo.push_back(i);
         std::stable_sort(uo.begin(), uo.end(), [&](uint32_t x, uint32_t y) { return dU[x] < dU[y]; });
+        for (uint32_t k = 0; k < lo.size(); k++) local[lo[k]] = k;
+        uint32_t* B = &P.blk[kBlkFields * b];
+        B[0] = base;
+        B[1] = uint32_t(lo.size());
+        B[2] = uint32_t(P.lvlL.size());
+        B[6] = uint32_t(P.Lloc.size());
         for (uint32_t k = 0; k < lo.size(); k++) {
-            local[lo[k]] = k;
-            T.pos_of_row[lo[k]] = base + k;
-        }
-        T.blk[6 * b + 0] = base;
-        T.blk[6 * b + 2] = uint32_t(T.lvlL.size());
-        for (uint32_t k = 0; k < lo.size(); k++) {
-            uint32_t i = lo[k];
-            if (k + 1 == lo.size() || dL[lo[k + 1]] != dL[i]) T.lvlL.push_back(base + k + 1);
-            T.fidx[base + k] = lv.map[i];
+            const uint32_t i = lo[k];
+            if (k + 1 == lo.size() || dL[lo[k + 1]] != dL[i]) P.lvlL.push_back(k + 1);
+            P.fidx[base + k] = lv.map[i];
+            P.out[base + k] = i >= M ? (kGlobalRef | (i - M)) : k;
             for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
-                T.Lloc.push_back(local[lv.L_ja[p]]);
-                T.Lval.push_back(lv.L_a[p]);
+                P.Lloc.push_back(local[lv.L_ja[p]]);
+                P.Lval.push_back(lv.L_a[p]);
             }
-            T.Lptr[base + k + 1] = uint32_t(T.Lloc.size());
+            P.Lptr[base + k + 1] = uint32_t(P.Lloc.size());
         }
-        T.blk[6 * b + 3] = uint32_t(T.lvlL.size());
-        T.blk[6 * b + 4] = uint32_t(T.lvlU.size());
+        B[3] = uint32_t(P.lvlL.size());
+        B[4] = uint32_t(P.lvlU.size());
+        B[7] = uint32_t(P.Ucol.size());
+        P.ublk[2 * b] = upos;
+        P.ublk[2 * b + 1] = uint32_t(uo.size());
         for (uint32_t k = 0; k < uo.size(); k++) {
-            uint32_t i = uo[k];
-            if (k + 1 == uo.size() || dU[uo[k + 1]] != dU[i]) T.lvlU.push_back(upos + k + 1);
-            T.uloc[upos + k] = local[i];
-            T.uxidx[upos + k] = lv.map[i];
-            T.Udiag[upos + k] = lv.U_a[lv.U_ia[i]];
+            const uint32_t i = uo[k];
+            if (k + 1 == uo.size() || dU[uo[k + 1]] != dU[i]) P.lvlU.push_back(k + 1);
+            P.uloc[upos + k] = local[i];
+            P.uxidx[upos + k] = lv.map[i];
+            P.Udiag[upos + k] = lv.U_a[lv.U_ia[i]];
             for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) {
                 uint32_t j = lv.U_ja[p];
-                T.Ucol.push_back(j < M ? local[j] : (kGlobalRef | lv.map[j]));
-                T.Uval.push_back(lv.U_a[p]);
+                P.Ucol.push_back(j < M ? local[j] : (kGlobalRef | (j - M)));
+                P.Uval.push_back(lv.U_a[p]);
             }
-            T.Uptr[upos + k + 1] = uint32_t(T.Ucol.size());
+            P.Uptr[upos + k + 1] = uint32_t(P.Ucol.size());
         }
-        T.blk[6 * b + 5] = uint32_t(T.lvlU.size());
+        B[5] = uint32_t(P.lvlU.size());
+        P.max_nodes = std::max<uint32_t>(P.max_nodes, uint32_t(lo.size()));
+        P.max_Unodes = std::max<uint32_t>(P.max_Unodes, uint32_t(uo.size()));
+        P.max_Lent = std::max<uint32_t>(P.max_Lent, uint32_t(P.Lloc.size()) - B[6]);
+        P.max_Uent = std::max<uint32_t>(P.max_Uent, uint32_t(P.Ucol.size()) - B[7]);
+        P.max_levels = std::max<uint32_t>(P.max_levels, std::max(B[3] - B[2], B[5] - B[4]));
         pos += uint32_t(lo.size());
         upos += uint32_t(uo.size());
-        T.blk[6 * b + 1] = pos;
-        T.max_nodes = std::max<uint32_t>(T.max_nodes, uint32_t(lo.size()));
-        T.max_levels = std::max<uint32_t>(T.max_levels, std::max(T.blk[6 * b + 3] - T.blk[6 * b + 2],
-                                                                  T.blk[6 * b + 5] - T.blk[6 * b + 4]));
     }
-    if (T.Lloc.empty()) { T.Lloc.push_back(0); T.Lval.push_back(0.0); }
-    if (T.Ucol.empty()) { T.Ucol.push_back(0); T.Uval.push_back(0.0); }
-}
-
-void plan_schur(const LevelArrays& lv, const TailLayout& T, SchurLayout& S) {
-    const uint32_t n = lv.N - lv.M - lv.Md;
-    S = SchurLayout();
-    S.ptr.assign(n + 1, 0);
-    for (uint32_t r = 0; r < n; r++) {
-        uint32_t i = lv.M + r;
-        for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
-            uint32_t k = lv.L_ja[p];
-            if (k >= lv.M) throw Error("Schur row " + std::to_string(i) + " references a kept row", 1);
-            S.col.push_back(T.pos_of_row[k]);
-            S.val.push_back(lv.L_a[p]);
-        }
-        S.ptr[r + 1] = uint32_t(S.col.size());
-    }
-    if (S.col.empty()) { S.col.push_back(0); S.val.push_back(0.0); }
+    if (P.Lloc.empty()) { P.Lloc.push_back(0); P.Lval.push_back(0.0); }
+    if (P.Ucol.empty()) { P.Ucol.push_back(0); P.Uval.push_back(0.0); }
+    if (P.lvlU.empty()) P.lvlU.push_back(0);
 }
 
 void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, SellLayout& S) {

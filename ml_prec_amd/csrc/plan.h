@@ -3,10 +3,12 @@
 // The L forward solve and U back solve of a level (cheb_solve.cc:33-38 and
 // :108-115; multi_split_solve.cc:25-53) only couple the eliminated rows
 // 0..M-1 within small connected components.  The planner packs whole
-// components into workgroup-sized blocks and orders each block's rows by
-// dependency depth, so one kernel launch runs every depth level of a block
-// with a workgroup barrier between levels.  The arithmetic per row keeps the
-// reference's operand order, so results are unchanged by the schedule.
+// components, together with the kept rows they feed (their L children), into
+// workgroup-sized blocks, and orders each block's rows by dependency depth.
+// One kernel launch then stages a block's inputs in LDS and runs every depth
+// level with a workgroup barrier between levels.  The arithmetic per row
+// keeps the reference's operand order, so results are unchanged by the
+// schedule.
 #pragma once
 
 #include <cstdint>
@@ -23,35 +25,44 @@ struct LevelArrays {
     const double* dval = nullptr;                                       // Md
 };
 
-constexpr uint32_t kGlobalRef = 0x80000000u;   // U column refers to the level output x (global)
+constexpr uint32_t kGlobalRef = 0x80000000u;   // refers to a kept row (index into the next level)
+constexpr uint32_t kBlkFields = 8;
 
-// Blocked program for the eliminated rows.  Block b owns positions
-// [blk[6b], blk[6b+1]) of the L-ordered node list; its L depth levels are
-// lvlL[blk[6b+2] .. blk[6b+3]] (absolute node positions, one past each level
-// end) and its U levels lvlU[blk[6b+4] .. blk[6b+5]] (positions in U order).
-struct TailLayout {
-    uint32_t M = 0, nblk = 0, max_nodes = 0, max_levels = 0;
-    std::vector<uint32_t> blk;
+// Component-blocked program of one level.
+//
+// Nodes are the eliminated rows (i < M) and the kept rows with L entries
+// ("roots", M <= i < N-Md).  Rows that depend on each other form small
+// connected components; whole components are packed into blocks of at most
+// `cap` nodes.  Inside a block the nodes are ordered by L depth (a node's L
+// columns lie in earlier depth levels), and the eliminated rows are listed a
+// second time ordered by U depth.  A block runs its depth levels with a
+// barrier between levels.
+// Block b: blk[8b] = first node, blk[8b+1] = nodes, blk[8b+2..3] = its L
+// depth levels in lvlL (local end offsets), blk[8b+4..5] = its U levels in
+// lvlU (local end offsets into its U list), blk[8b+6] / blk[8b+7] = first L /
+// U entry; ublk[2b] / ublk[2b+1] = first U-list position and length.
+struct LevelProgram {
+    uint32_t M = 0, n = 0, Md = 0, N = 0;
+    uint32_t nblk = 0, nodes = 0;
+    uint32_t max_nodes = 0, max_Lent = 0, max_Uent = 0, max_Unodes = 0, max_levels = 0;
+    uint32_t max_depth = 0, max_comp_size = 0;
+    std::vector<uint32_t> blk;           // kBlkFields per block
+    std::vector<uint32_t> ublk;          // 2 per block
     std::vector<uint32_t> lvlL, lvlU;
-    // L order (size M)
+    // L order (size nodes)
     std::vector<uint32_t> fidx;          // map[i]: where the row reads the level input
-    std::vector<uint32_t> Lptr;          // M+1
-    std::vector<uint32_t> Lloc;          // local position of the child within the block
+    std::vector<uint32_t> out;           // eliminated: local position; kept row r = i-M: kGlobalRef | r
+    std::vector<uint32_t> Lptr;          // nodes+1 (global entry offsets)
+    std::vector<uint32_t> Lloc;          // local position of the L column within the block
     std::vector<double> Lval;
-    // U order (size M)
+    // U order (eliminated rows only, size M)
     std::vector<uint32_t> uloc;          // local position (L order) of the row
     std::vector<uint32_t> uxidx;         // map[i]: where the row writes the level output
     std::vector<uint32_t> Uptr;          // M+1
-    std::vector<uint32_t> Ucol;          // local position, or kGlobalRef | map[j] for rows >= M
+    std::vector<uint32_t> Ucol;          // local position, or kGlobalRef | (j - M) for kept columns
     std::vector<double> Uval, Udiag;
-    // row i (< M) -> global L-order position
-    std::vector<uint32_t> pos_of_row;
-};
-
-// Rows M..N-Md-1 (the next level's right-hand side) as a gather + short dot.
-struct SchurLayout {
-    std::vector<uint32_t> ptr, col;      // col = L-order position of the eliminated row
-    std::vector<double> val;
+    // kept rows without L entries (their right-hand side is a plain gather)
+    std::vector<uint32_t> flat;          // r = i - M
 };
 
 // 64-row slices, column-major inside a slice: entry k of row 64s+t sits at
@@ -64,8 +75,7 @@ struct SellLayout {
     std::vector<double> val;
 };
 
-void plan_tail(const LevelArrays& lv, uint32_t cap, TailLayout& out);
-void plan_schur(const LevelArrays& lv, const TailLayout& tl, SchurLayout& out);
+void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& out);
 void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, SellLayout& out);
 
 }  // namespace mlp

@@ -210,6 +210,8 @@ class Context:
         check(lib().mlp_ctx_create(device, C.byref(self.h)))
 
     def close(self):
+        """Destroy the context; objects created on it must be closed first (their
+        device memory is released with the context otherwise)."""
         if getattr(self, "h", None):
             lib().mlp_ctx_destroy(self.h)
             self.h = None
@@ -263,10 +265,14 @@ class DevMatrix:
         self.h = _vp()
         check(lib().mlp_csr_upload(ctx.h, self.n, _u32(ia), _u32(ja), _f64(a), C.byref(self.h)))
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
+    def close(self):
+        # a matrix outlives nothing: destroy it only while its context is alive
+        if getattr(self, "h", None) and _lib is not None and self.ctx.h:
             _lib.mlp_mat_destroy(self.h)
-            self.h = None
+        self.h = None
+
+    def __del__(self):
+        self.close()
 
     def spmv(self, x: DevArray, y: DevArray):
         check(lib().mlp_spmv(self.ctx.h, self.h, x.ptr, y.ptr))
@@ -299,10 +305,13 @@ class DevPrec:
             self.kind = kind
         self.precision = precision
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None and self.ctx.h:
             _lib.mlp_prec_destroy(self.h)
-            self.h = None
+        self.h = None
+
+    def __del__(self):
+        self.close()
 
     def apply(self, f: DevArray, x: DevArray):
         check(lib().mlp_prec_apply(self.ctx.h, self.h, f.ptr, x.ptr))
