@@ -189,10 +189,11 @@ int mlp_gmres(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, double* d_x
 /* ---- diagnostics (no counterpart in the reference; used for roofline reporting) ---- */
 /* algorithmic HBM bytes of one apply (sum over level visits) */
 int mlp_prec_bytes(mlp_prec p, double* bytes_per_apply, uint32_t* launches_per_apply);
-/* Average device time of `reps` SpMV launches y = A x (HIP events on the context
- * stream around each launch).  flush_bytes > 0 streams a buffer of that size
- * between launches so that A and x are read from HBM, not from the 256 MB
- * Infinity Cache. */
+/* Average device time of `reps` back-to-back SpMV launches y = A x (two HIP
+ * events on the context stream around the whole run).  flush_bytes > 0 makes
+ * the launches rotate over device copies of A totalling at least flush_bytes,
+ * so that each launch reads its matrix from HBM rather than from L2 or the
+ * 256 MB Infinity Cache; 0 reuses A (cache-warm). */
 int mlp_time_spmv(mlp_ctx ctx, mlp_mat A, const double* d_x, double* d_y, uint32_t reps,
                   size_t flush_bytes, double* avg_ms);
 /* Average device time of `reps` preconditioner applies x = B f. */

@@ -67,9 +67,9 @@ template <class VT> struct DevSell {
 
 template <class VT>
 void upload_sell(DevMem& mem, hipStream_t s, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
-                 DevSell<VT>& out) {
+                 DevSell<VT>& out, const uint32_t* rows = nullptr) {
     SellLayout L;
-    plan_sell(n, ia, ja, a, L);
+    plan_sell(n, ia, ja, a, L, rows);
     out.v.n = n;
     out.v.off64 = mem.upload(L.off64, s);
     out.v.col = mem.upload(L.col, s);
@@ -159,8 +159,9 @@ template <class VT> struct DevLevel {
     ProgView<VT> prog;
     uint32_t lds_down = 0, lds_up = 0, lds_solve = 0;
     VT* dval = nullptr;
-    DevSell<VT> A;                             // this level's matrix (levels >= 1)
-    VT *wt = nullptr, *F = nullptr, *tmp = nullptr, *u1 = nullptr, *u0 = nullptr, *x2 = nullptr;
+    DevSell<VT> A;                             // this level's matrix (levels >= 1); uh_cheb stores its rows
+                                               // in the order the level kernels read them
+    VT *wt = nullptr, *F = nullptr, *u1 = nullptr, *u0 = nullptr, *x2 = nullptr;
     VT* r = nullptr;                           // multi_split inner residual
     // bookkeeping for the byte model
     uint32_t nnzL = 0, nnzU = 0;
@@ -245,6 +246,7 @@ template <class VT> struct DevHier : DevPrecBase {
             plan_fitting(la, P);
             ProgView<VT>& G = L.prog;
             G.nblk = P.nblk;
+            G.nodes = P.nodes;
             G.nflat = uint32_t(P.flat.size());
             G.max_nodes = P.max_nodes; G.max_Lent = P.max_Lent; G.max_Uent = P.max_Uent; G.max_Unodes = P.max_Unodes;
             G.flat = mem.upload(P.flat, s);
@@ -271,10 +273,16 @@ template <class VT> struct DevHier : DevPrecBase {
             L.lds_solve = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, true).total;
             std::vector<double> dv(D.dval, D.dval + D.Md);
             L.dval = mem.upload_as<VT>(dv, s);
-            if (l > 0) upload_sell<VT>(mem, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A);
+            if (l > 0) {
+                if (kind == MLP_PREC_UH_CHEB) {
+                    const std::vector<uint32_t> order = consumption_order(P, D.map);
+                    upload_sell<VT>(mem, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A, order.data());
+                } else {
+                    upload_sell<VT>(mem, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A);
+                }
+            }
             if (L.n) {
                 L.F = mem.alloc<VT>(L.n);
-                L.tmp = mem.alloc<VT>(L.n);
                 L.u1 = mem.alloc<VT>(L.n);
                 L.u0 = mem.alloc<VT>(L.n);
                 L.x2 = mem.alloc<VT>(L.n);
@@ -314,10 +322,10 @@ template <class VT> struct DevHier : DevPrecBase {
         launches++;
     }
 
-    template <class FT, class XT, int DMODE>
-    void down(const DevLevel<VT>& L, const FT* f, XT* x, const uint32_t* g) const {
+    template <class FT, int IN, class XT, int DMODE>
+    void down(const DevLevel<VT>& L, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
         const uint32_t grid = L.prog.nblk + cdiv(uint64_t(L.prog.nflat) + L.Md, lbs);
-        launch(k_level_down<VT, FT, XT, DMODE>, grid, lbs, L.lds_down, L.prog, f, L.wt, L.F, (const uint32_t*)L.map,
+        launch(k_level_down<VT, FT, IN, XT, DMODE>, grid, lbs, L.lds_down, L.prog, f, L.wt, L.F, (const uint32_t*)L.map,
                L.M, L.N, L.Md, (const VT*)L.dval, x, g);
     }
     template <class XT, int OP, int UMODE>
@@ -326,39 +334,53 @@ template <class VT> struct DevHier : DevPrecBase {
         launch(k_level_up<VT, XT, OP, UMODE>, grid, lbs, L.lds_up, L.prog, (const VT*)L.wt, C,
                (const uint32_t*)(L.map + L.M), L.n, x, g);
     }
-    template <class FT, class XT, int MODE>
-    void solve_last(const DevLevel<VT>& L, const FT* f, XT* x, const uint32_t* g) const {
+    template <class FT, int IN, class XT, int MODE>
+    void solve_last(const DevLevel<VT>& L, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
         const uint32_t grid = L.prog.nblk + cdiv(L.Md, lbs);
-        launch(k_level_solve<VT, FT, XT, MODE>, grid, lbs, L.lds_solve, L.prog, f, (const uint32_t*)L.map, L.N, L.Md,
+        launch(k_level_solve<VT, FT, IN, XT, MODE>, grid, lbs, L.lds_solve, L.prog, f, (const uint32_t*)L.map, L.N, L.Md,
                (const VT*)L.dval, x, g);
     }
 
-    // Prec::solve(level, f, x)  (modules/prec/cheb/cheb_solve.cc:13-123)
-    template <class FT, class XT>
-    void cheb(uint32_t l, const FT* f, XT* x, const uint32_t* g) const {
+    static InSrc<VT, VT, IN_PLAIN> plain(const VT* f) {
+        InSrc<VT, VT, IN_PLAIN> S;
+        S.f = f;
+        return S;
+    }
+    // f = A_{l+1} (s u) - F_l, read row by row by the visit of level l+1
+    template <int IN> InSrc<VT, VT, IN> residual_src(uint32_t l, const VT* u, VT s, VT* us) const {
+        InSrc<VT, VT, IN> S;
+        S.A = lv[l + 1].A.v;
+        S.u = u;
+        S.F = lv[l].F;
+        S.us = us;
+        S.s = s;
+        return S;
+    }
+
+    // Prec::solve(level, f, x)  (modules/prec/cheb/cheb_solve.cc:13-123).  The
+    // Chebyshev residual tmp = A_{l+1} u - F of steps >= 2 is not stored: the
+    // visit of level l+1 computes each row where it reads it.
+    template <class FT, int IN, class XT>
+    void cheb(uint32_t l, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
         const DevLevel<VT>& L = lv[l];
         if (l + 1 == nlevels) {
-            solve_last<FT, XT, 0>(L, f, x, g);
+            solve_last<FT, IN, XT, 0>(L, f, x, g);
             return;
         }
-        down<FT, XT, 0>(L, f, x, g);
-        const DevLevel<VT>& Ln = lv[l + 1];
+        down<FT, IN, XT, 0>(L, f, x, g);
         const uint32_t n = L.n, gb = cdiv(n, kBS);
         Combine<VT> C;
         if (L.nit == 1) {
-            cheb<VT, VT>(l + 1, L.F, L.x2, g);
+            cheb(l + 1, plain(L.F), L.x2, g);
             C.x2 = L.x2; C.a = VT(L.a1); C.skip = L.s1_skip;
             up<XT, OP_SCALE, 0>(L, C, x, g);
             return;
         }
-        cheb<VT, VT>(l + 1, L.F, L.u1, g);
-        // tmp = A (a1 u1) - F ; u1s = a1 u1 (kept in u0)
-        if (L.s1_skip)
-            launch(k_cheb_res<VT, false>, gb, kBS, 0, Ln.A.v, (const VT*)L.u1, (const VT*)L.F, L.tmp, L.u0, VT(1), g);
-        else
-            launch(k_cheb_res<VT, true>, gb, kBS, 0, Ln.A.v, (const VT*)L.u1, (const VT*)L.F, L.tmp, L.u0, VT(L.a1), g);
+        cheb(l + 1, plain(L.F), L.u1, g);
+        // step 2 on tmp = A (a1 u1) - F, with u1s = a1 u1 kept in u0
+        if (L.s1_skip) cheb(l + 1, residual_src<IN_RES>(l, L.u1, VT(1), nullptr), L.x2, g);
+        else cheb(l + 1, residual_src<IN_RES_SCALE>(l, L.u1, VT(L.a1), L.u0), L.x2, g);
         VT* u1s = L.s1_skip ? L.u1 : L.u0;
-        cheb<VT, VT>(l + 1, L.tmp, L.x2, g);
         if (L.nit == 2) {
             C.x2 = L.x2; C.u1 = u1s; C.a = VT(L.ma2); C.skip = L.s2_skip; C.b = VT(L.opb2); C.one = L.opb2_one;
             up<XT, OP_CHEB2, 0>(L, C, x, g);
@@ -373,9 +395,7 @@ template <class VT> struct DevHier : DevPrecBase {
         VT* cur = spare;
         spare = L.x2;
         for (uint32_t i = 3; i <= L.nit; i++) {
-            launch(k_cheb_res<VT, false>, gb, kBS, 0, Ln.A.v, (const VT*)cur, (const VT*)L.F, L.tmp, (VT*)nullptr, VT(1),
-                   g);
-            cheb<VT, VT>(l + 1, L.tmp, spare, g);
+            cheb(l + 1, residual_src<IN_RES>(l, cur, VT(1), nullptr), spare, g);
             const VT a = VT(L.ak[i - 3]), b = VT(L.bk[i - 3]);
             if (i == L.nit) {
                 C.x2 = spare; C.u1 = cur; C.u0 = prev; C.a = a; C.b = b;
@@ -398,10 +418,12 @@ template <class VT> struct DevHier : DevPrecBase {
         const bool last = l + 1 == nlevels;
         Combine<VT> C;
         C.x2 = L.x2;
+        InSrc<VT, FT, IN_PLAIN> fs;
+        fs.f = f;
         if (last) {
-            solve_last<FT, XT, 0>(L, f, x, g);
+            solve_last<FT, IN_PLAIN, XT, 0>(L, fs, x, g);
         } else {
-            down<FT, XT, 0>(L, f, x, g);
+            down<FT, IN_PLAIN, XT, 0>(L, fs, x, g);
             msplit<VT, VT>(l + 1, L.F, L.x2, g);
             up<XT, OP_COPY, 0>(L, C, x, g);
         }
@@ -419,9 +441,9 @@ template <class VT> struct DevHier : DevPrecBase {
                 if (nr / init_norm < L.eps) break;
             }
             if (last) {
-                solve_last<VT, VT, 1>(L, (const VT*)L.r, xv, g);
+                solve_last<VT, IN_PLAIN, VT, 1>(L, plain(L.r), xv, g);
             } else {
-                down<VT, VT, 1>(L, (const VT*)L.r, xv, g);
+                down<VT, IN_PLAIN, VT, 1>(L, plain(L.r), xv, g);
                 msplit<VT, VT>(l + 1, L.F, L.x2, g);
                 up<VT, OP_COPY, 1>(L, C, xv, g);
             }
@@ -439,8 +461,13 @@ template <class VT> struct DevHier : DevPrecBase {
     }
 
     void apply(const double* f, double* x, const uint32_t* guard) override {
-        if (kind == MLP_PREC_UH_CHEB) cheb<double, double>(0, f, x, guard);
-        else msplit<double, double>(0, f, x, guard);
+        if (kind == MLP_PREC_UH_CHEB) {
+            InSrc<VT, double, IN_PLAIN> fs;
+            fs.f = f;
+            cheb(0, fs, x, guard);
+        } else {
+            msplit<double, double>(0, f, x, guard);
+        }
     }
     bool graph_safe() const override { return !dynamic; }
 
@@ -953,31 +980,51 @@ int mlp_time_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y, uint32_t r
                   double* avg_ms) {
     MLP_API_BEGIN
     require(ctx && A && avg_ms && reps, "null argument");
-    double* buf = nullptr;
-    const size_t nf = flush_bytes / sizeof(double);
-    if (nf) {
-        MLP_HIP_CHECK(hipMalloc(&buf, (nf + 1) * sizeof(double)));
-        MLP_HIP_CHECK(hipMemsetAsync(buf, 0, (nf + 1) * sizeof(double), ctx->stream));
+    // flush_bytes > 0: launches rotate over copies of the matrix totalling at
+    // least flush_bytes, so each launch finds its matrix evicted from L2 and
+    // the Infinity Cache by the launches before it; 0: the same matrix every
+    // launch (cache-warm).  Launches run back to back between two events.
+    std::vector<SellView<double>> views{A->A.v};
+    std::vector<void*> extra;
+    const size_t mbytes = size_t(A->A.nent) * (sizeof(uint32_t) + sizeof(double));
+    if (flush_bytes && mbytes) {
+        const size_t nc = std::max<size_t>(2, (flush_bytes + mbytes - 1) / mbytes);
+        for (size_t c = 1; c < nc; c++) {
+            SellView<double> v = A->A.v;
+            uint32_t* col = nullptr;
+            double* val = nullptr;
+            MLP_HIP_CHECK(hipMalloc(&col, A->A.nent * sizeof(uint32_t)));
+            extra.push_back(col);
+            MLP_HIP_CHECK(hipMalloc(&val, A->A.nent * sizeof(double)));
+            extra.push_back(val);
+            MLP_HIP_CHECK(hipMemcpyAsync(col, A->A.v.col, A->A.nent * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                         ctx->stream));
+            MLP_HIP_CHECK(hipMemcpyAsync(val, A->A.v.val, A->A.nent * sizeof(double), hipMemcpyDeviceToDevice,
+                                         ctx->stream));
+            v.col = col;
+            v.val = val;
+            views.push_back(v);
+        }
     }
+    auto run = [&](uint32_t r) {
+        hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream,
+                           views[r % views.size()], x, y, (const uint32_t*)nullptr);
+    };
+    for (uint32_t r = 0; r < views.size(); r++) run(r);   // untimed pass over every copy
     hipEvent_t e0, e1;
     MLP_HIP_CHECK(hipEventCreate(&e0));
     MLP_HIP_CHECK(hipEventCreate(&e1));
-    double tot = 0.0;
-    for (uint32_t r = 0; r < reps; r++) {
-        if (nf) hipLaunchKernelGGL(k_flush, dim3(4096), dim3(kBS), 0, ctx->stream, nf, (const double*)buf, buf + nf);
-        MLP_HIP_CHECK(hipEventRecord(e0, ctx->stream));
-        hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream, A->A.v,
-                           x, y, (const uint32_t*)nullptr);
-        MLP_HIP_CHECK(hipEventRecord(e1, ctx->stream));
-        MLP_HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        MLP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        tot += ms;
-    }
+    MLP_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+    for (uint32_t r = 0; r < reps; r++) run(r);
+    MLP_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+    MLP_HIP_CHECK(hipEventSynchronize(e1));
+    MLP_HIP_CHECK(hipGetLastError());
+    float ms = 0.f;
+    MLP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (buf) (void)hipFree(buf);
-    *avg_ms = tot / reps;
+    for (void* p : extra) (void)hipFree(p);
+    *avg_ms = double(ms) / reps;
     MLP_API_END
 }
 

@@ -134,20 +134,6 @@ __global__ void __launch_bounds__(kBS) k_residual(SellView<VT> A, const XT* __re
     rr[r] = XT(VT(b[r]) - acc);
 }
 
-// tmp = A (s u) - F ; us = s u     (Chebyshev step 2, cheb_solve.cc:61-78)
-// tmp = A u - F                    (steps >= 3, cheb_solve.cc:96-97)
-template <class VT, bool SCALE>
-__global__ void __launch_bounds__(kBS) k_cheb_res(SellView<VT> A, const VT* __restrict__ u, const VT* __restrict__ F,
-                                                  VT* __restrict__ tmp, VT* __restrict__ us, VT s,
-                                                  const uint32_t* guard) {
-    if (guarded(guard)) return;
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= A.n) return;
-    VT acc = sell_row<VT, VT, SCALE>(A, u, r, s);
-    tmp[r] = acc - F[r];
-    if (SCALE) us[r] = u[r] * s;
-}
-
 // ---------------------------------------------------------------------------
 // level programs: triangular sweeps over the eliminated rows
 // (cheb_solve.cc:33-38, 108-122; multi_split_solve.cc:12-53)
@@ -155,7 +141,7 @@ __global__ void __launch_bounds__(kBS) k_cheb_res(SellView<VT> A, const VT* __re
 constexpr int kLBS = 256;              // threads per block of the level kernels
 
 template <class VT> struct ProgView {
-    uint32_t nblk = 0, nflat = 0;
+    uint32_t nblk = 0, nflat = 0, nodes = 0;
     uint32_t max_nodes = 0, max_Lent = 0, max_Uent = 0, max_Unodes = 0;
     const uint32_t* blk = nullptr;     // 8 per block (see plan.h)
     const uint32_t* ublk = nullptr;    // 2 per block
@@ -199,6 +185,31 @@ __device__ __forceinline__ VT combine(const Combine<VT>& C, uint32_t r) {
     return VT(0);
 }
 
+// Where a level reads its right-hand side f.  IN_PLAIN: f[j].  IN_RES and
+// IN_RES_SCALE fuse the Chebyshev residual of the parent level into the
+// read: f[j] = (A (s u))_j - F[j], the row taken from a copy of A whose rows
+// are stored in the order the level kernels consume them (position q), and
+// IN_RES_SCALE also stores us[j] = s u[j] (cheb_solve.cc:61-78, 96-97).  The
+// kernels read every row of f exactly once, so each residual row is computed
+// once, with the same operations as a separate SpMV.
+enum { IN_PLAIN = 0, IN_RES = 1, IN_RES_SCALE = 2 };
+template <class VT, class FT, int IN> struct InSrc {
+    const FT* f = nullptr;
+    SellView<VT> A{};
+    const VT* u = nullptr;
+    const VT* F = nullptr;
+    VT* us = nullptr;
+    VT s = VT(1);
+};
+
+template <class VT, class FT, int IN>
+__device__ __forceinline__ VT fetch(const InSrc<VT, FT, IN>& S, uint32_t q, uint32_t j) {
+    if (IN == IN_PLAIN) return VT(S.f[j]);
+    const VT acc = sell_row<VT, VT, IN == IN_RES_SCALE>(S.A, S.u, q, S.s);
+    if (IN == IN_RES_SCALE) S.us[j] = S.u[j] * S.s;
+    return acc - S.F[j];
+}
+
 // LDS carve-up shared by the host (sizes) and the kernels (offsets)
 struct LdsPlan {
     uint32_t sw, sx, lval, uval, uref, sz, sdiag, lptr, lloc, uptr, ucol, uloc, total;
@@ -227,8 +238,8 @@ __host__ __device__ inline LdsPlan lds_plan(uint32_t nodes, uint32_t lent, uint3
 
 // Forward sweep of one block: stage the gathered right-hand side and the L
 // rows in LDS, then one depth level per barrier.  Leaves w in sw.
-template <class VT, class FT>
-__device__ __forceinline__ void block_down(const ProgView<VT>& P, const FT* __restrict__ f, unsigned char* lds,
+template <class VT, class FT, int IN>
+__device__ __forceinline__ void block_down(const ProgView<VT>& P, const InSrc<VT, FT, IN>& f, unsigned char* lds,
                                            const LdsPlan& lp) {
     VT* sw = reinterpret_cast<VT*>(lds + lp.sw);
     VT* sval = reinterpret_cast<VT*>(lds + lp.lval);
@@ -238,7 +249,7 @@ __device__ __forceinline__ void block_down(const ProgView<VT>& P, const FT* __re
     const uint32_t base = B[0], S = B[1], e0 = B[6];
     const uint32_t ne = P.Lptr[base + S] - e0;
     for (uint32_t p = threadIdx.x; p < S; p += blockDim.x) {
-        sw[p] = VT(f[P.fidx[base + p]]);
+        sw[p] = fetch(f, base + p, P.fidx[base + p]);
         sptr[p] = P.Lptr[base + p] - e0;
     }
     if (threadIdx.x == 0) sptr[S] = ne;
@@ -323,8 +334,8 @@ __device__ __forceinline__ void put(XT* x, uint32_t j, XT v) {
 // the eliminated rows (wt) and the right-hand side F of the kept rows they
 // feed; the remaining blocks gather F for the other kept rows and apply the
 // isolated-node diagonal x[j] = dval * f[j] (DMODE 1 adds it to x).
-template <class VT, class FT, class XT, int DMODE>
-__global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, const FT* __restrict__ f, VT* __restrict__ wt,
+template <class VT, class FT, int IN, class XT, int DMODE>
+__global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, InSrc<VT, FT, IN> f, VT* __restrict__ wt,
                                                      VT* __restrict__ F, const uint32_t* __restrict__ map, uint32_t M,
                                                      uint32_t N, uint32_t Md, const VT* __restrict__ dval,
                                                      XT* __restrict__ x, const uint32_t* guard) {
@@ -332,7 +343,7 @@ __global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, const FT* _
     if (blockIdx.x < P.nblk) {
         extern __shared__ __align__(16) unsigned char lds_raw[];
         const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, false);
-        block_down<VT, FT>(P, f, lds_raw, lp);
+        block_down(P, f, lds_raw, lp);
         const VT* sw = reinterpret_cast<const VT*>(lds_raw + lp.sw);
         const uint32_t* B = P.blk + 8 * blockIdx.x;
         const uint32_t base = B[0], S = B[1];
@@ -345,11 +356,11 @@ __global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, const FT* _
         const uint32_t i = (blockIdx.x - P.nblk) * blockDim.x + threadIdx.x;
         if (i < P.nflat) {
             const uint32_t r = P.flat[i];
-            F[r] = VT(f[map[M + r]]);
+            F[r] = fetch(f, P.nodes + i, map[M + r]);
         } else if (i < P.nflat + Md) {
             const uint32_t k = i - P.nflat;
             const uint32_t j = map[N - Md + k];
-            put<XT, DMODE>(x, j, XT(dval[k] * VT(f[j])));
+            put<XT, DMODE>(x, j, XT(dval[k] * fetch(f, P.nodes + i, j)));
         }
     }
 }
@@ -377,8 +388,8 @@ __global__ void __launch_bounds__(kLBS) k_level_up(ProgView<VT> P, const VT* __r
 
 // A whole level without kept rows (the coarsest level): L sweep, U sweep and
 // diagonal in one launch.  MODE 1 adds the result into x.
-template <class VT, class FT, class XT, int MODE>
-__global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, const FT* __restrict__ f,
+template <class VT, class FT, int IN, class XT, int MODE>
+__global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, InSrc<VT, FT, IN> f,
                                                       const uint32_t* __restrict__ map, uint32_t N, uint32_t Md,
                                                       const VT* __restrict__ dval, XT* __restrict__ x,
                                                       const uint32_t* guard) {
@@ -386,7 +397,7 @@ __global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, const FT* 
     if (blockIdx.x < P.nblk) {
         extern __shared__ __align__(16) unsigned char lds_raw[];
         const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, true);
-        block_down<VT, FT>(P, f, lds_raw, lp);
+        block_down(P, f, lds_raw, lp);
         Combine<VT> none;
         block_up<VT, OP_NONE, true>(P, nullptr, none, lds_raw, lp);
         const VT* sz = reinterpret_cast<const VT*>(lds_raw + lp.sz);
@@ -396,7 +407,7 @@ __global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, const FT* 
         const uint32_t k = (blockIdx.x - P.nblk) * blockDim.x + threadIdx.x;
         if (k < Md) {
             const uint32_t j = map[N - Md + k];
-            put<XT, MODE>(x, j, XT(dval[k] * VT(f[j])));
+            put<XT, MODE>(x, j, XT(dval[k] * fetch(f, P.nodes + P.nflat + k, j)));
         }
     }
 }
@@ -639,17 +650,6 @@ __global__ void __launch_bounds__(kBS) k_mdot_finalize(uint32_t nb, uint32_t nv,
         const double a = sum_partials<kBS>(partials, nb, 32, int(k), sh);
         if (threadIdx.x == 0) out[k] = a;
     }
-}
-
-// Reads through a zero-filled buffer to evict the SpMV operands from L2 and
-// the Infinity Cache.  It only reads, so no dirty lines are left behind to be
-// written back during the next timed launch; the store never happens (the
-// sum of zeros is zero) and only keeps the loads alive.
-__global__ void __launch_bounds__(kBS) k_flush(size_t n, const double* __restrict__ buf, double* __restrict__ sink) {
-    double acc = 0.0;
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
-        acc += buf[i];
-    if (acc != 0.0) sink[0] = acc;
 }
 
 struct MaxpyCoef {

@@ -171,21 +171,37 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
     if (P.lvlU.empty()) P.lvlU.push_back(0);
 }
 
-void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, SellLayout& S) {
+std::vector<uint32_t> consumption_order(const LevelProgram& P, const uint32_t* map) {
+    std::vector<uint32_t> q(P.fidx.begin(), P.fidx.end());
+    q.reserve(P.N);
+    for (uint32_t r : P.flat) q.push_back(map[P.M + r]);
+    for (uint32_t k = 0; k < P.Md; k++) q.push_back(map[P.N - P.Md + k]);
+    if (q.size() != P.N) throw Error("level rows are not covered exactly once", 1);
+    std::vector<char> seen(P.N, 0);
+    for (uint32_t j : q) {
+        if (j >= P.N || seen[j]) throw Error("level rows are not covered exactly once", 1);
+        seen[j] = 1;
+    }
+    return q;
+}
+
+void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, SellLayout& S,
+               const uint32_t* rows) {
     S = SellLayout();
     S.n = n;
     S.nslices = (n + 63) / 64;
     S.off64.assign(S.nslices + 1, 0);
+    auto row = [&](uint32_t q) { return rows ? rows[q] : q; };
     for (uint32_t s = 0; s < S.nslices; s++) {
         uint32_t w = 0;
-        for (uint32_t r = 64 * s; r < std::min(n, 64 * s + 64); r++) w = std::max(w, ia[r + 1] - ia[r]);
+        for (uint32_t q = 64 * s; q < std::min(n, 64 * s + 64); q++) w = std::max(w, ia[row(q) + 1] - ia[row(q)]);
         S.off64[s + 1] = S.off64[s] + w;
     }
     S.nent = uint64_t(S.off64[S.nslices]) * 64;
     S.col.assign(S.nent ? S.nent : 1, 0xffffffffu);
     S.val.assign(S.nent ? S.nent : 1, 0.0);
-    for (uint32_t r = 0; r < n; r++) {
-        uint32_t s = r / 64, t = r % 64;
+    for (uint32_t q = 0; q < n; q++) {
+        const uint32_t r = row(q), s = q / 64, t = q % 64;
         for (uint32_t k = 0; k < ia[r + 1] - ia[r]; k++) {
             uint64_t e = (uint64_t(S.off64[s]) + k) * 64 + t;
             S.col[e] = ja[ia[r] + k];

@@ -67,6 +67,8 @@ struct LevelProgram {
 
 // 64-row slices, column-major inside a slice: entry k of row 64s+t sits at
 // (off64[s] + k) * 64 + t.  Padding has col = 0xffffffff and val = 0.
+// With a row order `rows`, slot q holds matrix row rows[q] (entries and their
+// order unchanged, columns in the original numbering).
 struct SellLayout {
     uint32_t n = 0, nslices = 0;
     uint64_t nent = 0;
@@ -76,6 +78,10 @@ struct SellLayout {
 };
 
 void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& out);
-void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, SellLayout& out);
+void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, SellLayout& out,
+               const uint32_t* rows = nullptr);
+// The order in which the level kernels read the rows of a level's input:
+// block nodes (fidx), kept rows without L entries, then the isolated rows.
+std::vector<uint32_t> consumption_order(const LevelProgram& P, const uint32_t* map);
 
 }  // namespace mlp
