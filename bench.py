@@ -62,6 +62,29 @@ def cpu_baseline(s, eps, budget_s=20.0):
             "t_const_s": t_const, "niter": iters}, x
 
 
+def spmv_traffic_probe(N, s, reps):
+    """Cold SpMV launches plus a calibration kernel of known traffic: x = 0.5 x
+    over 2^27 doubles reads and writes exactly 1 GiB each.  Used under
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to measure HBM bytes per launch."""
+    ctx = N.Context(0)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    xv = ctx.to_device(np.random.default_rng(0).standard_normal(s.n))
+    yv = ctx.array(s.n)
+    ms = A.time_spmv(xv, yv, reps=reps, flush_bytes=1 << 30)
+    log(f"SpMV cold {ms * 1e3:.2f} us over {reps} launches")
+    big = ctx.array(1 << 27)
+    for _ in range(3):
+        N.dscal(ctx, 0.5, big)
+    ctx.sync()
+    log("calibration: 3 launches of x = 0.5 x over 2^27 doubles (1 GiB read + 1 GiB written each)")
+    _, _, dev_bytes = A.info()
+    print(json.dumps({"spmv_traffic_probe": True, "n": s.n, "nnz": s.nnz, "device_bytes": dev_bytes,
+                      "spmv_us_cold": round(ms * 1e3, 3)}), flush=True)
+    A.close()
+    del xv, yv, big
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +96,8 @@ def main():
     ap.add_argument("--spmv-warm-reps", type=int, default=None, help="0 skips the cache-warm SpMV timing")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--nz", type=int, default=85)
+    ap.add_argument("--spmv-traffic", action="store_true",
+                    help="only run cold SpMV launches and a 1 GiB scale kernel (for HBM counter runs), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (each system is solved on one "
@@ -85,6 +110,10 @@ def main():
     t0 = time.perf_counter()
     s = S.config_system(1, kind="spe", c=1.0, nz=args.nz)
     log(f"system {s.nx}x{s.ny}x{s.nz}: n={s.n} nnz={s.nnz} generated in {time.perf_counter() - t0:.2f}s")
+    if args.spmv_traffic:
+        spmv_traffic_probe(N, s, args.spmv_reps)
+        return
+
     t0 = time.perf_counter()
     H = N.build_uh_cheb(s.ia, s.ja, s.a)
     t_const = time.perf_counter() - t0

@@ -211,10 +211,10 @@ template <class VT> struct DevHier : DevPrecBase {
     void plan_fitting(const LevelArrays& la, LevelProgram& P) const {
         for (uint32_t cap = cap0;; cap /= 2) {
             plan_level(la, cap, P);
-            const LdsPlan s = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, true);
+            const LdsPlan s = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true);
             if (s.total <= kLdsBudget || cap <= 8) break;
         }
-        const LdsPlan s = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, true);
+        const LdsPlan s = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true);
         if (s.total > 64 * 1024) throw Error("an elimination component is too large for LDS", 1);
     }
 
@@ -225,37 +225,46 @@ template <class VT> struct DevHier : DevPrecBase {
         lv.resize(nlevels);
         read_launch_shape();
         hipStream_t s = stream();
+        // plan every level first: uh_cheb links each level's F to the next
+        // level's consumption order
+        std::vector<LevelProgram> progs(nlevels);
         for (uint32_t l = 0; l < nlevels; l++) {
             const mlp_level_desc& D = d[l];
-            DevLevel<VT>& L = lv[l];
-            L.N = D.N; L.M = D.M; L.Md = D.Md; L.nit = D.nit; L.eps = D.eps;
-            if (L.M + L.Md > L.N) throw Error("level sizes inconsistent", 1);
-            L.n = L.N - L.M - L.Md;
-            if (l + 1 == nlevels && L.n != 0) throw Error("the coarsest level must have no kept rows", 1);
-            if (l + 1 < nlevels && d[l + 1].N != L.n) throw Error("level sizes do not chain", 1);
-            if (L.eps != 0.0) dynamic = true;
+            if (D.M + D.Md > D.N) throw Error("level sizes inconsistent", 1);
+            const uint32_t n = D.N - D.M - D.Md;
+            if (l + 1 == nlevels && n != 0) throw Error("the coarsest level must have no kept rows", 1);
+            if (l + 1 < nlevels && d[l + 1].N != n) throw Error("level sizes do not chain", 1);
             LevelArrays la;
             la.N = D.N; la.M = D.M; la.Md = D.Md; la.map = D.map;
             la.L_ia = D.L_ia; la.L_ja = D.L_ja; la.L_a = D.L_a;
             la.U_ia = D.U_ia; la.U_ja = D.U_ja; la.U_a = D.U_a; la.dval = D.dval;
+            plan_fitting(la, progs[l]);
+        }
+        if (kind == MLP_PREC_UH_CHEB)
+            for (uint32_t l = 0; l + 1 < nlevels; l++) link_levels(progs[l], progs[l + 1]);
+        for (uint32_t l = 0; l < nlevels; l++) {
+            const mlp_level_desc& D = d[l];
+            const LevelProgram& P = progs[l];
+            DevLevel<VT>& L = lv[l];
+            L.N = D.N; L.M = D.M; L.Md = D.Md; L.nit = D.nit; L.eps = D.eps;
+            L.n = L.N - L.M - L.Md;
+            if (L.eps != 0.0) dynamic = true;
             std::vector<uint32_t> mapv(D.map, D.map + D.N);
             L.map = mem.upload(mapv, s);
             L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
             L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
-            LevelProgram P;
-            plan_fitting(la, P);
             ProgView<VT>& G = L.prog;
             G.nblk = P.nblk;
             G.nodes = P.nodes;
             G.nflat = uint32_t(P.flat.size());
             G.max_nodes = P.max_nodes; G.max_Lent = P.max_Lent; G.max_Uent = P.max_Uent; G.max_Unodes = P.max_Unodes;
-            G.flat = mem.upload(P.flat, s);
+            G.max_levels = P.max_levels;
+            G.qrow = mem.upload(P.qrow, s);
+            G.flatq = mem.upload(P.flatq, s);
             if (P.nblk) {
                 G.blk = mem.upload(P.blk, s);
-                G.ublk = mem.upload(P.ublk, s);
                 G.lvlL = mem.upload(P.lvlL, s);
                 G.lvlU = mem.upload(P.lvlU, s);
-                G.fidx = mem.upload(P.fidx, s);
                 G.out = mem.upload(P.out, s);
                 G.Lptr = mem.upload(P.Lptr, s);
                 G.Lloc = mem.upload(P.Lloc, s);
@@ -268,18 +277,17 @@ template <class VT> struct DevHier : DevPrecBase {
                 G.Udiag = mem.upload_as<VT>(P.Udiag, s);
                 L.wt = mem.alloc<VT>(P.nodes);
             }
-            L.lds_down = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, false).total;
-            L.lds_up = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, false, true).total;
-            L.lds_solve = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, true).total;
+            L.lds_down = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, false).total;
+            L.lds_up = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, false, true).total;
+            L.lds_solve = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true).total;
             std::vector<double> dv(D.dval, D.dval + D.Md);
             L.dval = mem.upload_as<VT>(dv, s);
             if (l > 0) {
-                if (kind == MLP_PREC_UH_CHEB) {
-                    const std::vector<uint32_t> order = consumption_order(P, D.map);
-                    upload_sell<VT>(mem, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A, order.data());
-                } else {
+                // uh_cheb reads A_l only through the fused Chebyshev residual, row by row in consumption order
+                if (kind == MLP_PREC_UH_CHEB)
+                    upload_sell<VT>(mem, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A, P.qrow.data());
+                else
                     upload_sell<VT>(mem, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A);
-                }
             }
             if (L.n) {
                 L.F = mem.alloc<VT>(L.n);
@@ -325,8 +333,8 @@ template <class VT> struct DevHier : DevPrecBase {
     template <class FT, int IN, class XT, int DMODE>
     void down(const DevLevel<VT>& L, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
         const uint32_t grid = L.prog.nblk + cdiv(uint64_t(L.prog.nflat) + L.Md, lbs);
-        launch(k_level_down<VT, FT, IN, XT, DMODE>, grid, lbs, L.lds_down, L.prog, f, L.wt, L.F, (const uint32_t*)L.map,
-               L.M, L.N, L.Md, (const VT*)L.dval, x, g);
+        launch(k_level_down<VT, FT, IN, XT, DMODE>, grid, lbs, L.lds_down, L.prog, f, L.wt, L.F, L.Md,
+               (const VT*)L.dval, x, g);
     }
     template <class XT, int OP, int UMODE>
     void up(const DevLevel<VT>& L, const Combine<VT>& C, XT* x, const uint32_t* g) const {
@@ -337,12 +345,17 @@ template <class VT> struct DevHier : DevPrecBase {
     template <class FT, int IN, class XT, int MODE>
     void solve_last(const DevLevel<VT>& L, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
         const uint32_t grid = L.prog.nblk + cdiv(L.Md, lbs);
-        launch(k_level_solve<VT, FT, IN, XT, MODE>, grid, lbs, L.lds_solve, L.prog, f, (const uint32_t*)L.map, L.N, L.Md,
-               (const VT*)L.dval, x, g);
+        launch(k_level_solve<VT, FT, IN, XT, MODE>, grid, lbs, L.lds_solve, L.prog, f, L.Md, (const VT*)L.dval, x, g);
     }
 
     static InSrc<VT, VT, IN_PLAIN> plain(const VT* f) {
         InSrc<VT, VT, IN_PLAIN> S;
+        S.f = f;
+        return S;
+    }
+    // F of the parent level, stored in this level's consumption order
+    static InSrc<VT, VT, IN_PERM> perm(const VT* f) {
+        InSrc<VT, VT, IN_PERM> S;
         S.f = f;
         return S;
     }
@@ -371,12 +384,12 @@ template <class VT> struct DevHier : DevPrecBase {
         const uint32_t n = L.n, gb = cdiv(n, kBS);
         Combine<VT> C;
         if (L.nit == 1) {
-            cheb(l + 1, plain(L.F), L.x2, g);
+            cheb(l + 1, perm(L.F), L.x2, g);
             C.x2 = L.x2; C.a = VT(L.a1); C.skip = L.s1_skip;
             up<XT, OP_SCALE, 0>(L, C, x, g);
             return;
         }
-        cheb(l + 1, plain(L.F), L.u1, g);
+        cheb(l + 1, perm(L.F), L.u1, g);
         // step 2 on tmp = A (a1 u1) - F, with u1s = a1 u1 kept in u0
         if (L.s1_skip) cheb(l + 1, residual_src<IN_RES>(l, L.u1, VT(1), nullptr), L.x2, g);
         else cheb(l + 1, residual_src<IN_RES_SCALE>(l, L.u1, VT(L.a1), L.u0), L.x2, g);

@@ -79,10 +79,22 @@ __device__ __forceinline__ double sum_partials(const double* partials, uint32_t 
     return block_sum<BS>(s, sh);
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
+// b % 8), and each XCD has its own L2.  xcd_block maps b to a logical block so
+// that every XCD gets one contiguous run of logical blocks: the neighbouring
+// rows an SpMV gathers from x are then mostly in that XCD's L2.  A bijection
+// on [0, G) for any G.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
+    constexpr uint32_t X = 8;
+    const uint32_t q = G / X, rem = G % X, x = b % X;
+    return x * q + (x < rem ? x : rem) + b / X;
+}
+
 // y_r = sum_k a_rk * (S ? s*x_k : x_k), in storage order, from 0.
-// Entries are fetched in groups of 8 (column indices, then values, then the
-// x gathers) so that a row's loads are all in flight before the first
-// product; the sum itself still runs entry by entry.
+// Entries are fetched in groups of 8: column indices and values together
+// (padding slots hold a valid value 0), then the x gathers, so that a row's
+// loads are in flight before the first product; the sum itself still runs
+// entry by entry and skips padding.
 template <class VT, class XT, bool SCALE>
 __device__ __forceinline__ VT sell_row(const SellView<VT>& A, const XT* __restrict__ x, uint32_t r, VT s) {
     constexpr uint32_t G = 8;
@@ -96,7 +108,7 @@ __device__ __forceinline__ VT sell_row(const SellView<VT>& A, const XT* __restri
 #pragma unroll
         for (uint32_t j = 0; j < G; j++) c[j] = k + j < e ? A.col[size_t(k + j) * 64 + lane] : kPad;
 #pragma unroll
-        for (uint32_t j = 0; j < G; j++) v[j] = c[j] != kPad ? A.val[size_t(k + j) * 64 + lane] : VT(0);
+        for (uint32_t j = 0; j < G; j++) v[j] = k + j < e ? A.val[size_t(k + j) * 64 + lane] : VT(0);   // not gated on c
 #pragma unroll
         for (uint32_t j = 0; j < G; j++) xv[j] = c[j] != kPad ? x[c[j]] : XT(0);
 #pragma unroll
@@ -118,7 +130,7 @@ template <class VT, class XT, class YT>
 __global__ void __launch_bounds__(kBS) k_spmv(SellView<VT> A, const XT* __restrict__ x, YT* __restrict__ y,
                                               const uint32_t* guard) {
     if (guarded(guard)) return;
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t r = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (r >= A.n) return;
     y[r] = YT(sell_row<VT, XT, false>(A, x, r, VT(1)));
 }
@@ -128,7 +140,7 @@ template <class VT, class XT>
 __global__ void __launch_bounds__(kBS) k_residual(SellView<VT> A, const XT* __restrict__ b, const XT* __restrict__ x,
                                                   XT* __restrict__ rr, const uint32_t* guard) {
     if (guarded(guard)) return;
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t r = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (r >= A.n) return;
     VT acc = sell_row<VT, XT, false>(A, x, r, VT(1));
     rr[r] = XT(VT(b[r]) - acc);
@@ -142,12 +154,11 @@ constexpr int kLBS = 256;              // threads per block of the level kernels
 
 template <class VT> struct ProgView {
     uint32_t nblk = 0, nflat = 0, nodes = 0;
-    uint32_t max_nodes = 0, max_Lent = 0, max_Uent = 0, max_Unodes = 0;
-    const uint32_t* blk = nullptr;     // 8 per block (see plan.h)
-    const uint32_t* ublk = nullptr;    // 2 per block
+    uint32_t max_nodes = 0, max_Lent = 0, max_Uent = 0, max_Unodes = 0, max_levels = 0;
+    const uint32_t* blk = nullptr;     // 12 per block (see plan.h)
     const uint32_t* lvlL = nullptr;
     const uint32_t* lvlU = nullptr;
-    const uint32_t* fidx = nullptr;
+    const uint32_t* qrow = nullptr;    // level row read at consumption position q
     const uint32_t* out = nullptr;
     const uint32_t* Lptr = nullptr;
     const uint32_t* Lloc = nullptr;
@@ -158,7 +169,7 @@ template <class VT> struct ProgView {
     const uint32_t* Ucol = nullptr;
     const VT* Uval = nullptr;
     const VT* Udiag = nullptr;
-    const uint32_t* flat = nullptr;
+    const uint32_t* flatq = nullptr;   // F slot written by flat row i
 };
 
 // Value of kept row r of the level output, produced by the iteration on the
@@ -185,15 +196,20 @@ __device__ __forceinline__ VT combine(const Combine<VT>& C, uint32_t r) {
     return VT(0);
 }
 
-// Where a level reads its right-hand side f.  IN_PLAIN: f[j].  IN_RES and
-// IN_RES_SCALE fuse the Chebyshev residual of the parent level into the
-// read: f[j] = (A (s u))_j - F[j], the row taken from a copy of A whose rows
-// are stored in the order the level kernels consume them (position q), and
-// IN_RES_SCALE also stores us[j] = s u[j] (cheb_solve.cc:61-78, 96-97).  The
-// kernels read every row of f exactly once, so each residual row is computed
-// once, with the same operations as a separate SpMV.
-enum { IN_PLAIN = 0, IN_RES = 1, IN_RES_SCALE = 2 };
+// Where a level reads its right-hand side; q is the consumption position and
+// j = qrow[q] the level row.
+//  IN_PLAIN: f[j] (a vector in level order).
+//  IN_PERM:  f[q] (a vector already stored in consumption order: the kept-row
+//            right-hand side F written by the parent level).
+//  IN_RES, IN_RES_SCALE: the Chebyshev residual of the parent level fused into
+//            the read, f = (A (s u))_j - F[q], the row taken from a copy of A
+//            stored in consumption order; IN_RES_SCALE also stores
+//            us[j] = s u[j] (cheb_solve.cc:61-78, 96-97).
+// The kernels read every position exactly once, so each residual row is
+// computed once, with the same operations as a separate SpMV.
+enum { IN_PLAIN = 0, IN_PERM = 1, IN_RES = 2, IN_RES_SCALE = 3 };
 template <class VT, class FT, int IN> struct InSrc {
+    static constexpr bool kNeedRow = IN == IN_PLAIN || IN == IN_RES_SCALE;
     const FT* f = nullptr;
     SellView<VT> A{};
     const VT* u = nullptr;
@@ -205,18 +221,25 @@ template <class VT, class FT, int IN> struct InSrc {
 template <class VT, class FT, int IN>
 __device__ __forceinline__ VT fetch(const InSrc<VT, FT, IN>& S, uint32_t q, uint32_t j) {
     if (IN == IN_PLAIN) return VT(S.f[j]);
+    if (IN == IN_PERM) return VT(S.f[q]);
     const VT acc = sell_row<VT, VT, IN == IN_RES_SCALE>(S.A, S.u, q, S.s);
     if (IN == IN_RES_SCALE) S.us[j] = S.u[j] * S.s;
-    return acc - S.F[j];
+    return acc - S.F[q];
+}
+
+template <class VT, class FT, int IN>
+__device__ __forceinline__ VT fetch_at(const ProgView<VT>& P, const InSrc<VT, FT, IN>& S, uint32_t q) {
+    const uint32_t j = InSrc<VT, FT, IN>::kNeedRow ? P.qrow[q] : 0u;
+    return fetch(S, q, j);
 }
 
 // LDS carve-up shared by the host (sizes) and the kernels (offsets)
 struct LdsPlan {
-    uint32_t sw, sx, lval, uval, uref, sz, sdiag, lptr, lloc, uptr, ucol, uloc, total;
+    uint32_t sw, sx, lval, uval, uref, sz, sdiag, lptr, lloc, uptr, ucol, uloc, lvl, total;
 };
 template <class VT>
-__host__ __device__ inline LdsPlan lds_plan(uint32_t nodes, uint32_t lent, uint32_t uent, uint32_t unodes, bool down,
-                                            bool up) {
+__host__ __device__ inline LdsPlan lds_plan(uint32_t nodes, uint32_t lent, uint32_t uent, uint32_t unodes,
+                                            uint32_t levels, bool down, bool up) {
     LdsPlan p{};
     uint32_t o = 0;
     auto take = [&](uint32_t bytes) { uint32_t r = o; o += (bytes + 7u) & ~7u; return r; };
@@ -232,24 +255,24 @@ __host__ __device__ inline LdsPlan lds_plan(uint32_t nodes, uint32_t lent, uint3
     p.uptr = up ? take((unodes + 1) * 4) : 0;
     p.ucol = up ? take(uent * 4) : 0;
     p.uloc = up ? take(unodes * 4) : 0;
+    p.lvl = take(levels * 4);            // depth-level end offsets of the sweep being run
     p.total = o;
     return p;
 }
 
-// Forward sweep of one block: stage the gathered right-hand side and the L
-// rows in LDS, then one depth level per barrier.  Leaves w in sw.
+// Forward sweep of one block: stage the right-hand side and the L rows in
+// LDS, then one depth level per barrier.  Leaves w in sw.
 template <class VT, class FT, int IN>
-__device__ __forceinline__ void block_down(const ProgView<VT>& P, const InSrc<VT, FT, IN>& f, unsigned char* lds,
-                                           const LdsPlan& lp) {
+__device__ __forceinline__ void block_down(const ProgView<VT>& P, const uint32_t* B, const InSrc<VT, FT, IN>& f,
+                                           unsigned char* lds, const LdsPlan& lp) {
     VT* sw = reinterpret_cast<VT*>(lds + lp.sw);
     VT* sval = reinterpret_cast<VT*>(lds + lp.lval);
     uint32_t* sptr = reinterpret_cast<uint32_t*>(lds + lp.lptr);
     uint32_t* sloc = reinterpret_cast<uint32_t*>(lds + lp.lloc);
-    const uint32_t* B = P.blk + 8 * blockIdx.x;
-    const uint32_t base = B[0], S = B[1], e0 = B[6];
-    const uint32_t ne = P.Lptr[base + S] - e0;
+    uint32_t* slv = reinterpret_cast<uint32_t*>(lds + lp.lvl);
+    const uint32_t base = B[0], S = B[1], e0 = B[6], ne = B[7] - B[6], nl = B[3] - B[2];
     for (uint32_t p = threadIdx.x; p < S; p += blockDim.x) {
-        sw[p] = fetch(f, base + p, P.fidx[base + p]);
+        sw[p] = fetch_at(P, f, base + p);
         sptr[p] = P.Lptr[base + p] - e0;
     }
     if (threadIdx.x == 0) sptr[S] = ne;
@@ -257,10 +280,11 @@ __device__ __forceinline__ void block_down(const ProgView<VT>& P, const InSrc<VT
         sloc[q] = P.Lloc[e0 + q];
         sval[q] = P.Lval[e0 + q];
     }
+    for (uint32_t li = threadIdx.x; li < nl; li += blockDim.x) slv[li] = P.lvlL[B[2] + li];
     __syncthreads();
     uint32_t s = 0;
-    for (uint32_t li = B[2]; li < B[3]; li++) {
-        const uint32_t e = P.lvlL[li];
+    for (uint32_t li = 0; li < nl; li++) {
+        const uint32_t e = slv[li];
         for (uint32_t p = s + threadIdx.x; p < e; p += blockDim.x) {
             VT acc = sw[p];
             for (uint32_t q = sptr[p]; q < sptr[p + 1]; q++) acc -= sval[q] * sw[sloc[q]];
@@ -275,8 +299,8 @@ __device__ __forceinline__ void block_down(const ProgView<VT>& P, const InSrc<VT
 // comes from LDS (after block_down) or from wt; kept columns take their value
 // from the combine op.  Results land in sz.
 template <class VT, int OP, bool W_IN_LDS>
-__device__ __forceinline__ void block_up(const ProgView<VT>& P, const VT* __restrict__ wt, const Combine<VT>& C,
-                                         unsigned char* lds, const LdsPlan& lp) {
+__device__ __forceinline__ void block_up(const ProgView<VT>& P, const uint32_t* B, const VT* __restrict__ wt,
+                                         const Combine<VT>& C, unsigned char* lds, const LdsPlan& lp) {
     VT* sw = reinterpret_cast<VT*>(lds + lp.sw);
     VT* sx = reinterpret_cast<VT*>(lds + lp.sx);
     VT* sval = reinterpret_cast<VT*>(lds + lp.uval);
@@ -286,10 +310,8 @@ __device__ __forceinline__ void block_up(const ProgView<VT>& P, const VT* __rest
     uint32_t* sptr = reinterpret_cast<uint32_t*>(lds + lp.uptr);
     uint32_t* scol = reinterpret_cast<uint32_t*>(lds + lp.ucol);
     uint32_t* sloc = reinterpret_cast<uint32_t*>(lds + lp.uloc);
-    const uint32_t* B = P.blk + 8 * blockIdx.x;
-    const uint32_t base = B[0], e0 = B[7];
-    const uint32_t ub = P.ublk[2 * blockIdx.x], nu = P.ublk[2 * blockIdx.x + 1];
-    const uint32_t ne = P.Uptr[ub + nu] - e0;
+    uint32_t* slv = reinterpret_cast<uint32_t*>(lds + lp.lvl);
+    const uint32_t base = B[0], e0 = B[8], ne = B[9] - B[8], ub = B[10], nu = B[11], nl = B[5] - B[4];
     for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x) {
         const uint32_t loc = P.uloc[ub + u];
         sloc[u] = loc;
@@ -304,10 +326,11 @@ __device__ __forceinline__ void block_up(const ProgView<VT>& P, const VT* __rest
         sval[q] = P.Uval[e0 + q];
         if (OP != OP_NONE && (c & kGlobalBit)) sref[q] = combine<VT, OP>(C, c & ~kGlobalBit);
     }
+    for (uint32_t li = threadIdx.x; li < nl; li += blockDim.x) slv[li] = P.lvlU[B[4] + li];
     __syncthreads();
     uint32_t s = 0;
-    for (uint32_t li = B[4]; li < B[5]; li++) {
-        const uint32_t e = P.lvlU[li];
+    for (uint32_t li = 0; li < nl; li++) {
+        const uint32_t e = slv[li];
         for (uint32_t u = s + threadIdx.x; u < e; u += blockDim.x) {
             VT z = sz[u];
             for (uint32_t q = sptr[u]; q < sptr[u + 1]; q++) {
@@ -330,37 +353,50 @@ __device__ __forceinline__ void put(XT* x, uint32_t j, XT v) {
     else x[j] = x[j] + v;
 }
 
+// Loads the block header (kBlkFields words) into registers.
+struct BlkHdr {
+    uint32_t w[12];
+};
+__device__ __forceinline__ BlkHdr load_hdr(const uint32_t* blk) {
+    BlkHdr h;
+    const uint32_t* B = blk + 12 * blockIdx.x;
+#pragma unroll
+    for (int k = 0; k < 12; k++) h.w[k] = B[k];
+    return h;
+}
+
 // Forward part of a level: component blocks run the L sweep and write w of
 // the eliminated rows (wt) and the right-hand side F of the kept rows they
-// feed; the remaining blocks gather F for the other kept rows and apply the
-// isolated-node diagonal x[j] = dval * f[j] (DMODE 1 adds it to x).
+// feed; the remaining blocks move the right-hand side of the other kept rows
+// into F and apply the isolated-node diagonal x[j] = dval * f[j] (DMODE 1
+// adds it to x).
 template <class VT, class FT, int IN, class XT, int DMODE>
 __global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, InSrc<VT, FT, IN> f, VT* __restrict__ wt,
-                                                     VT* __restrict__ F, const uint32_t* __restrict__ map, uint32_t M,
-                                                     uint32_t N, uint32_t Md, const VT* __restrict__ dval,
+                                                     VT* __restrict__ F, uint32_t Md, const VT* __restrict__ dval,
                                                      XT* __restrict__ x, const uint32_t* guard) {
     if (guarded(guard)) return;
     if (blockIdx.x < P.nblk) {
         extern __shared__ __align__(16) unsigned char lds_raw[];
-        const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, false);
-        block_down(P, f, lds_raw, lp);
+        const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, false);
+        const BlkHdr h = load_hdr(P.blk);
+        const uint32_t base = h.w[0], S = h.w[1];
+        // the store targets of this thread's first node, fetched ahead of the sweep
+        const uint32_t o0 = threadIdx.x < S ? P.out[base + threadIdx.x] : 0u;
+        block_down(P, h.w, f, lds_raw, lp);
         const VT* sw = reinterpret_cast<const VT*>(lds_raw + lp.sw);
-        const uint32_t* B = P.blk + 8 * blockIdx.x;
-        const uint32_t base = B[0], S = B[1];
         for (uint32_t p = threadIdx.x; p < S; p += blockDim.x) {
-            const uint32_t o = P.out[base + p];
+            const uint32_t o = p == threadIdx.x ? o0 : P.out[base + p];
             if (o & kGlobalBit) F[o & ~kGlobalBit] = sw[p];
             else wt[base + p] = sw[p];
         }
     } else {
         const uint32_t i = (blockIdx.x - P.nblk) * blockDim.x + threadIdx.x;
+        const uint32_t q = P.nodes + i;
         if (i < P.nflat) {
-            const uint32_t r = P.flat[i];
-            F[r] = fetch(f, P.nodes + i, map[M + r]);
+            F[P.flatq[i]] = fetch_at(P, f, q);
         } else if (i < P.nflat + Md) {
-            const uint32_t k = i - P.nflat;
-            const uint32_t j = map[N - Md + k];
-            put<XT, DMODE>(x, j, XT(dval[k] * fetch(f, P.nodes + i, j)));
+            const uint32_t j = P.qrow[q];
+            put<XT, DMODE>(x, j, XT(dval[i - P.nflat] * fetch(f, q, j)));
         }
     }
 }
@@ -375,11 +411,14 @@ __global__ void __launch_bounds__(kLBS) k_level_up(ProgView<VT> P, const VT* __r
     if (guarded(guard)) return;
     if (blockIdx.x < P.nblk) {
         extern __shared__ __align__(16) unsigned char lds_raw[];
-        const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, false, true);
-        block_up<VT, OP, false>(P, wt, C, lds_raw, lp);
+        const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, false, true);
+        const BlkHdr h = load_hdr(P.blk);
+        const uint32_t ub = h.w[10], nu = h.w[11];
+        const uint32_t x0 = threadIdx.x < nu ? P.uxidx[ub + threadIdx.x] : 0u;
+        block_up<VT, OP, false>(P, h.w, wt, C, lds_raw, lp);
         const VT* sz = reinterpret_cast<const VT*>(lds_raw + lp.sz);
-        const uint32_t ub = P.ublk[2 * blockIdx.x], nu = P.ublk[2 * blockIdx.x + 1];
-        for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x) put<XT, UMODE>(x, P.uxidx[ub + u], XT(sz[u]));
+        for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x)
+            put<XT, UMODE>(x, u == threadIdx.x ? x0 : P.uxidx[ub + u], XT(sz[u]));
     } else {
         const uint32_t r = (blockIdx.x - P.nblk) * blockDim.x + threadIdx.x;
         if (r < n) put<XT, UMODE>(x, mapM[r], XT(combine<VT, OP>(C, r)));
@@ -389,25 +428,28 @@ __global__ void __launch_bounds__(kLBS) k_level_up(ProgView<VT> P, const VT* __r
 // A whole level without kept rows (the coarsest level): L sweep, U sweep and
 // diagonal in one launch.  MODE 1 adds the result into x.
 template <class VT, class FT, int IN, class XT, int MODE>
-__global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, InSrc<VT, FT, IN> f,
-                                                      const uint32_t* __restrict__ map, uint32_t N, uint32_t Md,
+__global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, InSrc<VT, FT, IN> f, uint32_t Md,
                                                       const VT* __restrict__ dval, XT* __restrict__ x,
                                                       const uint32_t* guard) {
     if (guarded(guard)) return;
     if (blockIdx.x < P.nblk) {
         extern __shared__ __align__(16) unsigned char lds_raw[];
-        const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, true, true);
-        block_down(P, f, lds_raw, lp);
+        const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true);
+        const BlkHdr h = load_hdr(P.blk);
+        const uint32_t ub = h.w[10], nu = h.w[11];
+        const uint32_t x0 = threadIdx.x < nu ? P.uxidx[ub + threadIdx.x] : 0u;
+        block_down(P, h.w, f, lds_raw, lp);
         Combine<VT> none;
-        block_up<VT, OP_NONE, true>(P, nullptr, none, lds_raw, lp);
+        block_up<VT, OP_NONE, true>(P, h.w, nullptr, none, lds_raw, lp);
         const VT* sz = reinterpret_cast<const VT*>(lds_raw + lp.sz);
-        const uint32_t ub = P.ublk[2 * blockIdx.x], nu = P.ublk[2 * blockIdx.x + 1];
-        for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x) put<XT, MODE>(x, P.uxidx[ub + u], XT(sz[u]));
+        for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x)
+            put<XT, MODE>(x, u == threadIdx.x ? x0 : P.uxidx[ub + u], XT(sz[u]));
     } else {
         const uint32_t k = (blockIdx.x - P.nblk) * blockDim.x + threadIdx.x;
         if (k < Md) {
-            const uint32_t j = map[N - Md + k];
-            put<XT, MODE>(x, j, XT(dval[k] * fetch(f, P.nodes + P.nflat + k, j)));
+            const uint32_t q = P.nodes + P.nflat + k;
+            const uint32_t j = P.qrow[q];
+            put<XT, MODE>(x, j, XT(dval[k] * fetch(f, q, j)));
         }
     }
 }
@@ -537,11 +579,15 @@ __global__ void __launch_bounds__(kBS) k_pcg_spmv_dot(SellView<double> A, const 
                                                       double* __restrict__ Ap, const SolveState* st, double* partials) {
     if (st->done) return;
     double s = 0.0;
-    // grid-stride over rows: the stride is a multiple of 64, so lane == row % 64
-    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < A.n; r += gridDim.x * blockDim.x) {
-        const double v = sell_row<double, double, false>(A, p, r, 1.0);
-        Ap[r] = v;
-        s += v * p[r];
+    // grid-stride over 256-row tiles, XCD-grouped within each sweep of the grid
+    const uint32_t tiles = (A.n + blockDim.x - 1) / blockDim.x;
+    for (uint32_t t = xcd_block(blockIdx.x, gridDim.x); t < tiles; t += gridDim.x) {
+        const uint32_t r = t * blockDim.x + threadIdx.x;
+        if (r < A.n) {
+            const double v = sell_row<double, double, false>(A, p, r, 1.0);
+            Ap[r] = v;
+            s += v * p[r];
+        }
     }
     double part[1] = {s};
     block_partials<kBS, 1>(part, partials);
@@ -618,10 +664,14 @@ __global__ void __launch_bounds__(kBS) k_pcg_true_res(SellView<double> A, const 
                                                       const SolveState* st, double* partials) {
     if (st->done) return;
     double s = 0.0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
-        const double v = b[i] - sell_row<double, double, false>(A, x, i, 1.0);
-        r[i] = v;
-        s += v * v;
+    const uint32_t tiles = (A.n + blockDim.x - 1) / blockDim.x;
+    for (uint32_t t = xcd_block(blockIdx.x, gridDim.x); t < tiles; t += gridDim.x) {
+        const uint32_t i = t * blockDim.x + threadIdx.x;
+        if (i < A.n) {
+            const double v = b[i] - sell_row<double, double, false>(A, x, i, 1.0);
+            r[i] = v;
+            s += v * v;
+        }
     }
     double part[1] = {s};
     block_partials<kBS, 1>(part, partials);

@@ -25,6 +25,20 @@ struct DSU {
 
 constexpr uint32_t kNoNode = 0xffffffffu;
 
+// appends the flat and isolated rows to the consumption order and checks that
+// it covers every level row once; flat rows write F slot r until linked
+void finish_order(const LevelArrays& lv, LevelProgram& P) {
+    for (uint32_t r : P.flat) P.qrow.push_back(lv.map[P.M + r]);
+    for (uint32_t k = 0; k < P.Md; k++) P.qrow.push_back(lv.map[P.N - P.Md + k]);
+    if (P.qrow.size() != P.N) throw Error("level rows are not covered exactly once", 1);
+    std::vector<char> seen(P.N, 0);
+    for (uint32_t j : P.qrow) {
+        if (j >= P.N || seen[j]) throw Error("level rows are not covered exactly once", 1);
+        seen[j] = 1;
+    }
+    P.flatq = P.flat;
+}
+
 }  // namespace
 
 void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
@@ -84,7 +98,10 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
         if (root_comp[r] == kNoNode) root_comp[r] = ncomp++;
         comp_of[i] = root_comp[r];
     }
-    if (ncomp == 0) return;
+    if (ncomp == 0) {
+        finish_order(lv, P);
+        return;
+    }
     std::vector<uint32_t> csize(ncomp, 0);
     for (uint32_t i = 0; i < K; i++) if (is_node[i]) csize[comp_of[i]]++;
     P.max_comp_size = *std::max_element(csize.begin(), csize.end());
@@ -101,11 +118,10 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
 
     P.nblk = nblk;
     P.blk.resize(kBlkFields * size_t(nblk));
-    P.ublk.resize(2 * size_t(nblk));
     uint32_t total = 0;
     for (auto& R : rows) total += uint32_t(R.size());
     P.nodes = total;
-    P.fidx.resize(total);
+    P.qrow.resize(total);
     P.out.resize(total);
     P.Lptr.assign(total + 1, 0);
     P.uloc.resize(M);
@@ -131,7 +147,7 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
         for (uint32_t k = 0; k < lo.size(); k++) {
             const uint32_t i = lo[k];
             if (k + 1 == lo.size() || dL[lo[k + 1]] != dL[i]) P.lvlL.push_back(k + 1);
-            P.fidx[base + k] = lv.map[i];
+            P.qrow[base + k] = lv.map[i];
             P.out[base + k] = i >= M ? (kGlobalRef | (i - M)) : k;
             for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
                 P.Lloc.push_back(local[lv.L_ja[p]]);
@@ -140,10 +156,11 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
             P.Lptr[base + k + 1] = uint32_t(P.Lloc.size());
         }
         B[3] = uint32_t(P.lvlL.size());
+        B[7] = uint32_t(P.Lloc.size());
         B[4] = uint32_t(P.lvlU.size());
-        B[7] = uint32_t(P.Ucol.size());
-        P.ublk[2 * b] = upos;
-        P.ublk[2 * b + 1] = uint32_t(uo.size());
+        B[8] = uint32_t(P.Ucol.size());
+        B[10] = upos;
+        B[11] = uint32_t(uo.size());
         for (uint32_t k = 0; k < uo.size(); k++) {
             const uint32_t i = uo[k];
             if (k + 1 == uo.size() || dU[uo[k + 1]] != dU[i]) P.lvlU.push_back(k + 1);
@@ -158,31 +175,28 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
             P.Uptr[upos + k + 1] = uint32_t(P.Ucol.size());
         }
         B[5] = uint32_t(P.lvlU.size());
+        B[9] = uint32_t(P.Ucol.size());
         P.max_nodes = std::max<uint32_t>(P.max_nodes, uint32_t(lo.size()));
         P.max_Unodes = std::max<uint32_t>(P.max_Unodes, uint32_t(uo.size()));
-        P.max_Lent = std::max<uint32_t>(P.max_Lent, uint32_t(P.Lloc.size()) - B[6]);
-        P.max_Uent = std::max<uint32_t>(P.max_Uent, uint32_t(P.Ucol.size()) - B[7]);
+        P.max_Lent = std::max<uint32_t>(P.max_Lent, B[7] - B[6]);
+        P.max_Uent = std::max<uint32_t>(P.max_Uent, B[9] - B[8]);
         P.max_levels = std::max<uint32_t>(P.max_levels, std::max(B[3] - B[2], B[5] - B[4]));
         pos += uint32_t(lo.size());
         upos += uint32_t(uo.size());
     }
+    finish_order(lv, P);
     if (P.Lloc.empty()) { P.Lloc.push_back(0); P.Lval.push_back(0.0); }
     if (P.Ucol.empty()) { P.Ucol.push_back(0); P.Uval.push_back(0.0); }
     if (P.lvlU.empty()) P.lvlU.push_back(0);
 }
 
-std::vector<uint32_t> consumption_order(const LevelProgram& P, const uint32_t* map) {
-    std::vector<uint32_t> q(P.fidx.begin(), P.fidx.end());
-    q.reserve(P.N);
-    for (uint32_t r : P.flat) q.push_back(map[P.M + r]);
-    for (uint32_t k = 0; k < P.Md; k++) q.push_back(map[P.N - P.Md + k]);
-    if (q.size() != P.N) throw Error("level rows are not covered exactly once", 1);
-    std::vector<char> seen(P.N, 0);
-    for (uint32_t j : q) {
-        if (j >= P.N || seen[j]) throw Error("level rows are not covered exactly once", 1);
-        seen[j] = 1;
-    }
-    return q;
+void link_levels(LevelProgram& P, const LevelProgram& next) {
+    if (next.N != P.n) throw Error("level sizes do not chain", 1);
+    std::vector<uint32_t> qpos(P.n);
+    for (uint32_t q = 0; q < next.N; q++) qpos[next.qrow[q]] = q;
+    for (uint32_t& o : P.out)
+        if (o & kGlobalRef) o = kGlobalRef | qpos[o & ~kGlobalRef];
+    for (size_t i = 0; i < P.flat.size(); i++) P.flatq[i] = qpos[P.flat[i]];
 }
 
 void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, SellLayout& S,

@@ -26,7 +26,7 @@ struct LevelArrays {
 };
 
 constexpr uint32_t kGlobalRef = 0x80000000u;   // refers to a kept row (index into the next level)
-constexpr uint32_t kBlkFields = 8;
+constexpr uint32_t kBlkFields = 12;
 
 // Component-blocked program of one level.
 //
@@ -37,21 +37,25 @@ constexpr uint32_t kBlkFields = 8;
 // columns lie in earlier depth levels), and the eliminated rows are listed a
 // second time ordered by U depth.  A block runs its depth levels with a
 // barrier between levels.
-// Block b: blk[8b] = first node, blk[8b+1] = nodes, blk[8b+2..3] = its L
-// depth levels in lvlL (local end offsets), blk[8b+4..5] = its U levels in
-// lvlU (local end offsets into its U list), blk[8b+6] / blk[8b+7] = first L /
-// U entry; ublk[2b] / ublk[2b+1] = first U-list position and length.
+//
+// The kernels read the level input in "consumption order": position q runs
+// over the block nodes, then the kept rows without L entries (flat), then
+// the isolated rows; qrow[q] is the level row read at q.
+//
+// Block b header blk[12b ..]: [0] first node, [1] nodes, [2..3] its L depth
+// levels in lvlL (local end offsets), [4..5] its U levels in lvlU, [6..7] its
+// L entry range, [8..9] its U entry range, [10] first U-list position,
+// [11] U-list length.
 struct LevelProgram {
     uint32_t M = 0, n = 0, Md = 0, N = 0;
     uint32_t nblk = 0, nodes = 0;
     uint32_t max_nodes = 0, max_Lent = 0, max_Uent = 0, max_Unodes = 0, max_levels = 0;
     uint32_t max_depth = 0, max_comp_size = 0;
     std::vector<uint32_t> blk;           // kBlkFields per block
-    std::vector<uint32_t> ublk;          // 2 per block
     std::vector<uint32_t> lvlL, lvlU;
+    std::vector<uint32_t> qrow;          // N: level row read at consumption position q
     // L order (size nodes)
-    std::vector<uint32_t> fidx;          // map[i]: where the row reads the level input
-    std::vector<uint32_t> out;           // eliminated: local position; kept row r = i-M: kGlobalRef | r
+    std::vector<uint32_t> out;           // eliminated: local position; root: kGlobalRef | (F slot of kept row i-M)
     std::vector<uint32_t> Lptr;          // nodes+1 (global entry offsets)
     std::vector<uint32_t> Lloc;          // local position of the L column within the block
     std::vector<double> Lval;
@@ -61,8 +65,9 @@ struct LevelProgram {
     std::vector<uint32_t> Uptr;          // M+1
     std::vector<uint32_t> Ucol;          // local position, or kGlobalRef | (j - M) for kept columns
     std::vector<double> Uval, Udiag;
-    // kept rows without L entries (their right-hand side is a plain gather)
+    // kept rows without L entries (their right-hand side is a plain read)
     std::vector<uint32_t> flat;          // r = i - M
+    std::vector<uint32_t> flatq;         // F slot the flat row writes (r, or its position in the next level)
 };
 
 // 64-row slices, column-major inside a slice: entry k of row 64s+t sits at
@@ -80,8 +85,9 @@ struct SellLayout {
 void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& out);
 void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, SellLayout& out,
                const uint32_t* rows = nullptr);
-// The order in which the level kernels read the rows of a level's input:
-// block nodes (fidx), kept rows without L entries, then the isolated rows.
-std::vector<uint32_t> consumption_order(const LevelProgram& P, const uint32_t* map);
+// Makes level l write its kept-row right-hand side F in the consumption order
+// of level l+1 (F slot of kept row r = position of row r in next.qrow), so
+// that level l+1 reads it with plain coalesced loads.
+void link_levels(LevelProgram& P, const LevelProgram& next);
 
 }  // namespace mlp
