@@ -158,7 +158,7 @@ int mlp_prec_destroy(mlp_prec p);
  * Prec::solve cheb_solve.cc:9-123, MultiSplitPrec::solve multi_split_solve.cc:59-157 */
 int mlp_prec_apply(mlp_ctx ctx, mlp_prec p, const double* d_f, double* d_x);
 
-/* ---- solvers: PCGSolver / GMRESSolver   modules/solvers/solvers.h:31-38 ---- */
+/* ---- solvers: PCGSolver / GMRESSolver / ChebSolver / SimpleSolver   modules/solvers/solvers.h:31-45 ---- */
 #define MLP_X0_ZERO      0   /* x0 = 0 */
 #define MLP_X0_GIVEN     1   /* x0 = d_x on entry */
 #define MLP_X0_REFERENCE 2   /* generate_x0: srandom(3), x = 20(random()-RAND_MAX/2)/RAND_MAX + 100
@@ -185,6 +185,17 @@ int mlp_pcg(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, double* d_x,
             const mlp_solve_opts* opts, mlp_solve_stats* stats);
 int mlp_gmres(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, double* d_x,
               const mlp_solve_opts* opts, mlp_solve_stats* stats);
+/* ChebSolver(A, lmin, lmax, b, B, x, eps, norm, silent)  modules/solvers/cheb.cc:9-100 (NORM_L2).
+ * [lmin, lmax] bound the spectrum of B A, e.g. Prec::lmin()/lmax() (cheb_prec.h:122-123) = level 0 of
+ * the hierarchy.  stats->niter follows the reference's count (cheb.cc:81).  opts->flexible and
+ * opts->true_res_every are ignored; the residual is recomputed every iteration as in the reference. */
+int mlp_cheb_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, double lmin, double lmax, const double* d_b, double* d_x,
+                    const mlp_solve_opts* opts, mlp_solve_stats* stats);
+/* SimpleSolver(A, b, B, x, stats, eps, norm, silent)  modules/solvers/simple_solver.cc:8-88 (NORM_L2):
+ * x += B (b - A x).  Returns MLP_ERR_NO_ITER ("No need to iterate", simple_solver.cc:80-82) when fewer
+ * than two iterations ran; x holds the result all the same. */
+int mlp_simple_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, double* d_x,
+                      const mlp_solve_opts* opts, mlp_solve_stats* stats);
 
 /* ---- diagnostics (no counterpart in the reference; used for roofline reporting) ---- */
 /* algorithmic HBM bytes of one apply (sum over level visits) */

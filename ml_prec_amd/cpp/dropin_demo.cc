@@ -51,7 +51,12 @@ int main(int argc, char** argv) {
         SolverStats stats;
         if (solver == "pcg") PCGSolver(A, b, *B, x, stats, eps, NORM_L2, true);
         else if (solver == "gmres") GMRESSolver(A, b, *B, x, stats, eps, NORM_L2, true);
-        else THROW_EXCEPTION("Unknown solver type \"" << solver << "\"");
+        else if (solver == "simple") SimpleSolver(A, b, *B, x, stats, eps, NORM_L2, true);
+        else if (solver == "cheb") {
+            // ChebSolver needs the spectral bounds of the Chebyshev preconditioner (project/main.cc:142-149)
+            Prec& Bcheb = dynamic_cast<Prec&>(*B);
+            ChebSolver(A, Bcheb.lmin(), Bcheb.lmax(), b, Bcheb, x, eps, NORM_L2, false);
+        } else THROW_EXCEPTION("Unknown solver type \"" << solver << "\"");
 
         Vector z(A.size());
         B->solve(b, z);                    // one standalone preconditioner apply, written after x

@@ -303,8 +303,10 @@ std::ostream& operator<<(std::ostream& os, const SolverStats& stats) {
     return os;
 }
 
-static void run_solver(bool gmres, const CSRMatrix& A, const Vector& b, const PrecBase& B, Vector& x,
-                       SolverStats& stats, double eps, NormType norm_type) {
+enum class SolverKind { PCG, GMRES, CHEB, SIMPLE };
+
+static void run_solver(SolverKind kind, const CSRMatrix& A, const Vector& b, const PrecBase& B, Vector& x,
+                       SolverStats& stats, double eps, NormType norm_type, double lmin = 0, double lmax = 0) {
     if (norm_type != NORM_L2) THROW_EXCEPTION("Only the L2 stopping norm runs on the device");
     if (b.size() != A.size() || x.size() != A.size()) THROW_EXCEPTION("Wrong sizes");
     DevVec db(b), dx(A.size());
@@ -314,14 +316,23 @@ static void run_solver(bool gmres, const CSRMatrix& A, const Vector& b, const Pr
     o.x0_mode = MLP_X0_REFERENCE;        // the reference always starts from generate_x0 (pcg.cc:24)
     o.use_graph = 1;
     o.check_every = 8;
-    if (B.precision() == MLP_FP32) {
+    if (kind == SolverKind::PCG && B.precision() == MLP_FP32) {
         o.flexible = 1;
         o.true_res_every = 20;
     }
     mlp_solve_stats st;
     std::memset(&st, 0, sizeof(st));
-    int rc = gmres ? mlp_gmres(context(), A.device(), B.device(), db.d, dx.d, &o, &st)
-                   : mlp_pcg(context(), A.device(), B.device(), db.d, dx.d, &o, &st);
+    int rc = 0;
+    switch (kind) {
+        case SolverKind::PCG: rc = mlp_pcg(context(), A.device(), B.device(), db.d, dx.d, &o, &st); break;
+        case SolverKind::GMRES: rc = mlp_gmres(context(), A.device(), B.device(), db.d, dx.d, &o, &st); break;
+        case SolverKind::CHEB:
+            rc = mlp_cheb_solver(context(), A.device(), B.device(), lmin, lmax, db.d, dx.d, &o, &st);
+            break;
+        case SolverKind::SIMPLE: rc = mlp_simple_solver(context(), A.device(), B.device(), db.d, dx.d, &o, &st); break;
+    }
+    // SimpleSolver computes x before it reports "No need to iterate" (simple_solver.cc:52-82)
+    if (rc == MLP_ERR_NO_ITER && kind == SolverKind::SIMPLE) dx.get(x);
     if (rc) THROW_EXCEPTION(mlp_last_error());
     dx.get(x);
     stats.niter = st.niter;
@@ -331,12 +342,25 @@ static void run_solver(bool gmres, const CSRMatrix& A, const Vector& b, const Pr
 
 void PCGSolver(const CSRMatrix& A, const Vector& b, const PrecBase& B, Vector& x, SolverStats& stats, double eps,
                NormType norm_type, bool silent) {
-    run_solver(false, A, b, B, x, stats, eps, norm_type);
+    run_solver(SolverKind::PCG, A, b, B, x, stats, eps, norm_type);
     if (!silent) std::cout << "#" << stats.niter << ": relative -> " << std::scientific << stats.true_rel_res << std::endl;
 }
 
 void GMRESSolver(const CSRMatrix& A, const Vector& b, const PrecBase& B, Vector& x, SolverStats& stats, double eps,
                  NormType norm_type, bool silent) {
-    run_solver(true, A, b, B, x, stats, eps, norm_type);
+    run_solver(SolverKind::GMRES, A, b, B, x, stats, eps, norm_type);
+    if (!silent) std::cout << "#" << stats.niter << ": relative -> " << std::scientific << stats.true_rel_res << std::endl;
+}
+
+void ChebSolver(const CSRMatrix& A, double lmin, double lmax, const Vector& b, const PrecBase& B, Vector& x,
+                double eps, NormType norm_type, bool silent) {
+    SolverStats stats;
+    run_solver(SolverKind::CHEB, A, b, B, x, stats, eps, norm_type, lmin, lmax);
+    if (!silent) std::cout << "#" << stats.niter << ": relative -> " << std::scientific << stats.true_rel_res << std::endl;
+}
+
+void SimpleSolver(const CSRMatrix& A, const Vector& b, const PrecBase& B, Vector& x, SolverStats& stats, double eps,
+                  NormType norm_type, bool silent) {
+    run_solver(SolverKind::SIMPLE, A, b, B, x, stats, eps, norm_type);
     if (!silent) std::cout << "#" << stats.niter << ": relative -> " << std::scientific << stats.true_rel_res << std::endl;
 }

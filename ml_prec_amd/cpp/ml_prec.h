@@ -12,11 +12,11 @@
 //   MultiSplitPrec(A, cfg)                     modules/prec/multi_split/multi_split_prec.h:105
 //   DiagPrec / IdPrec                          modules/prec/diag/diag_prec.h, id/id_prec.h
 //   Config                                     project/config.h:54-86
-//   SolverStats, PCGSolver, GMRESSolver        modules/solvers/solvers.h:15-38
+//   SolverStats, PCG/GMRES/Cheb/SimpleSolver   modules/solvers/solvers.h:15-38
 //   ConfigFile                                 modules/config_file/config_file.h:54-116
 //
 // Matrices and preconditioners upload themselves to device 0 on first use and
-// keep the device copy; PCGSolver/GMRESSolver run entirely on the device and
+// keep the device copy; the solvers run entirely on the device and
 // copy x back.  Errors surface as Exception, like the reference's
 // THROW_EXCEPTION.
 #pragma once
@@ -245,3 +245,7 @@ void PCGSolver(const CSRMatrix& A, const Vector& b, const PrecBase& B, Vector& x
                double eps = 1e-10, NormType norm_type = NORM_L2, bool silent = false);
 void GMRESSolver(const CSRMatrix& A, const Vector& b, const PrecBase& B, Vector& x, SolverStats& stats,
                  double eps = 1e-10, NormType norm_type = NORM_L2, bool silent = false);
+void ChebSolver(const CSRMatrix& A, double lmin, double lmax, const Vector& b, const PrecBase& B, Vector& x,
+                double eps = 1e-10, NormType norm_type = NORM_L2, bool silent = false);
+void SimpleSolver(const CSRMatrix& A, const Vector& b, const PrecBase& B, Vector& x, SolverStats& stats,
+                  double eps = 1e-10, NormType norm_type = NORM_L2, bool silent = false);

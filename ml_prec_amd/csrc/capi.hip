@@ -103,18 +103,24 @@ struct mlp_ctx_s {
     // instantiated PCG iteration graphs, reused while the same matrix,
     // preconditioner and vectors are solved again
     struct GraphEntry {
+        int solver = 0;                // 0 PCG, 1 Chebyshev, 2 simple
         uint64_t A, B;
         const void *b, *x;
         int flex;
         uint32_t tre, ws_n;
-        hipGraph_t g[2] = {nullptr, nullptr};
-        hipGraphExec_t e[2] = {nullptr, nullptr};
+        hipGraph_t g[3] = {nullptr, nullptr, nullptr};
+        hipGraphExec_t e[3] = {nullptr, nullptr, nullptr};
     };
     std::vector<GraphEntry> graphs;
+    // Chebyshev outer-iteration coefficients alpha_k, beta_k (k = iteration
+    // count) for the bounds (coef_lmin, coef_lmax), filled up to coef_filled
+    double* coef = nullptr;
+    uint32_t coef_filled = 0;
+    double coef_lmin = 0.0, coef_lmax = 0.0;
     void drop_graphs(uint64_t id) {
         for (size_t i = 0; i < graphs.size();) {
             if (id == 0 || graphs[i].A == id || graphs[i].B == id) {
-                for (int k = 0; k < 2; k++) {
+                for (int k = 0; k < 3; k++) {
                     if (graphs[i].e[k]) (void)hipGraphExecDestroy(graphs[i].e[k]);
                     if (graphs[i].g[k]) (void)hipGraphDestroy(graphs[i].g[k]);
                 }
@@ -1216,8 +1222,8 @@ SolveState pcg_phase(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double
     mlp_ctx_s::GraphEntry* ge = nullptr;
     if (use_graph) {
         for (auto& e : ctx->graphs)
-            if (e.A == A->id && e.B == B->id && e.b == b && e.x == x && e.flex == int(flex) && e.tre == tre &&
-                e.ws_n == ctx->ws_n)
+            if (e.solver == 0 && e.A == A->id && e.B == B->id && e.b == b && e.x == x && e.flex == int(flex) &&
+                e.tre == tre && e.ws_n == ctx->ws_n)
                 ge = &e;
         if (!ge) {
             MLP_HIP_CHECK(hipStreamSynchronize(s));
@@ -1248,6 +1254,92 @@ SolveState pcg_phase(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double
         if (st.done) break;
     }
     return st;
+}
+
+constexpr uint32_t kCoefCap = 1u << 20;   // Chebyshev outer iterations covered by the coefficient table
+
+// alpha_k, beta_k of ChebSolver steps 2+ (cheb.cc:59-60), k = iteration count,
+// computed with the reference's cheb() and operation order
+void ensure_coef(mlp_ctx ctx, double lmin, double lmax, uint32_t upto) {
+    if (upto > kCoefCap) throw Error("Chebyshev iteration count beyond the coefficient table", MLP_ERR_MAXIT);
+    if (!ctx->coef) ctx->coef = ctx->mem.alloc<double>(2 * size_t(kCoefCap));
+    if (lmin != ctx->coef_lmin || lmax != ctx->coef_lmax) {
+        ctx->coef_filled = 0;
+        ctx->coef_lmin = lmin;
+        ctx->coef_lmax = lmax;
+    }
+    if (upto <= ctx->coef_filled) return;
+    const uint32_t from = ctx->coef_filled, to = std::min(kCoefCap, std::max(upto, from + 4096));
+    const double eta = (lmax + lmin) / (lmax - lmin);
+    std::vector<double> h(2 * size_t(to - from), 0.0);
+    for (uint32_t k = std::max(from, 2u); k < to; k++) {
+        h[2 * size_t(k - from)] = 4 / (lmax - lmin) * cheb_poly(eta, k - 1) / cheb_poly(eta, k);
+        h[2 * size_t(k - from) + 1] = cheb_poly(eta, k - 2) / cheb_poly(eta, k);
+    }
+    MLP_HIP_CHECK(hipMemcpyAsync(ctx->coef + 2 * size_t(from), h.data(), h.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->coef_filled = to;
+}
+
+// Replays one outer iteration (variant v = pick(k) for the k-th iteration,
+// 1-based) in chunks until the device sets `done`.  Variants are captured as
+// hipGraphs and cached on the context.  before_chunk(k_last) runs on the host
+// before a chunk whose last iteration is k_last.
+template <class Iter, class Pick, class Before>
+SolveState run_outer(mlp_ctx ctx, int solver, mlp_mat A, mlp_prec B, const void* b, const void* x, int nvar,
+                     const mlp_solve_opts* o, Iter iteration, Pick pick, Before before_chunk) {
+    hipStream_t s = ctx->stream;
+    const uint32_t chunk = o->check_every ? o->check_every : 16;
+    const bool use_graph = o->use_graph && B->impl->graph_safe();
+    mlp_ctx_s::GraphEntry* ge = nullptr;
+    if (use_graph) {
+        for (auto& e : ctx->graphs)
+            if (e.solver == solver && e.A == A->id && e.B == B->id && e.b == b && e.x == x && e.ws_n == ctx->ws_n)
+                ge = &e;
+        if (!ge) {
+            MLP_HIP_CHECK(hipStreamSynchronize(s));
+            mlp_ctx_s::GraphEntry e;
+            e.solver = solver; e.A = A->id; e.B = B->id; e.b = b; e.x = x; e.flex = 0; e.tre = 0; e.ws_n = ctx->ws_n;
+            for (int v = 0; v < nvar; v++) {
+                MLP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                iteration(v);
+                MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[v]));
+                MLP_HIP_CHECK(hipGraphInstantiate(&e.e[v], e.g[v], nullptr, nullptr, 0));
+            }
+            if (ctx->graphs.size() >= 16) ctx->drop_graphs(ctx->graphs.front().A);
+            ctx->graphs.push_back(e);
+            ge = &ctx->graphs.back();
+        }
+    }
+    SolveState st;
+    uint32_t k = 0;
+    while (true) {
+        before_chunk(k + chunk);
+        for (uint32_t c = 0; c < chunk; c++) {
+            k++;
+            const int v = pick(k);
+            if (use_graph) MLP_HIP_CHECK(hipGraphLaunch(ge->e[v], s));
+            else iteration(v);
+        }
+        MLP_HIP_CHECK(hipGetLastError());
+        st = read_state(ctx);
+        if (st.done) break;
+    }
+    return st;
+}
+
+void init_state(mlp_ctx ctx, double norm, double init_norm, double eps, uint32_t maxit, uint32_t niter,
+                uint32_t maxit_offset) {
+    SolveState st0;
+    std::memset(&st0, 0, sizeof(st0));
+    st0.norm = norm;
+    st0.init_norm = init_norm;
+    st0.eps = eps;
+    st0.maxit = maxit;
+    st0.niter = niter;
+    st0.pad = maxit_offset;
+    write_state(ctx, st0);
 }
 
 }  // namespace
@@ -1418,6 +1510,137 @@ int mlp_gmres(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, co
         stats->true_rel_res = true_rel_res(ctx, A, b, x, ctx->r, init_norm);
     }
     if (status == MLP_ERR_MAXIT) throw Error("maximum number of iterations reached", MLP_ERR_MAXIT);
+    MLP_API_END
+}
+
+
+// ChebSolver (modules/solvers/cheb.cc:9-100): Chebyshev outer iteration on
+// [lmin, lmax] (Prec::lmin()/lmax(), cheb_prec.h:122-123) with the
+// preconditioner; stops on ||r|| / ||r0|| <= eps.  The reported niter is the
+// reference's (loop count + 1, cheb.cc:28, 47, 77, 81).
+int mlp_cheb_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, double lmin, double lmax, const double* b, double* x,
+                    const mlp_solve_opts* o, mlp_solve_stats* stats) {
+    MLP_API_BEGIN
+    require(ctx && A && B && b && x && o, "null argument");
+    require(B->n == A->n, "preconditioner size differs from A");
+    require(lmax > lmin && lmin > 0, "Chebyshev bounds need 0 < lmin < lmax");
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = A->n;
+    hipStream_t s = ctx->stream;
+    ensure_workspace(ctx, n, false);
+    fill_x0(ctx, n, o->x0_mode, x);
+    launch_residual(ctx, A, b, x, ctx->r);
+    const double init_norm = std::sqrt(dev_dot(ctx, n, ctx->r, ctx->r));
+    double eps = o->eps;
+    if (init_norm * eps < 1e-14) eps = 1e-14 / init_norm;      // check_and_replace_eps, solvers.cc:49-58
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    // step 1 (cheb.cc:37-47): u1 = x; x = alpha B r + u1; r = b - A x
+    double* bufs[3] = {x, ctx->z, ctx->p};   // x, then the two rotating work vectors
+    const double alpha = 2 / (lmax + lmin);
+    const uint32_t rg = red_grid(n);
+    MLP_HIP_CHECK(hipMemcpyAsync(bufs[1], x, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    B->impl->apply(ctx->r, x, nullptr);
+    hipLaunchKernelGGL(k_cheb_first, dim3(rg), dim3(kBS), 0, s, n, x, (const double*)bufs[1], alpha,
+                       int(host_is_one(alpha)));
+    launch_residual(ctx, A, b, x, ctx->r);
+    const double norm1 = std::sqrt(dev_dot(ctx, n, ctx->r, ctx->r));
+    // SolveState.niter holds the reference's niter (2 after step 1); maxit counts niter - 1
+    init_state(ctx, norm1, init_norm, eps, o->maxit, 2, 1);
+    SolveState st;
+    if (!(norm1 / init_norm > eps)) {
+        st = read_state(ctx);
+        st.done = 1;
+    } else if (o->maxit && 1 >= o->maxit) {
+        st = read_state(ctx);
+        st.status = 5;
+    } else {
+        // loop iteration k (1-based) uses roles (X, U0, U1) = rotation (k-1) mod 3 of (bufs[2], bufs[1], bufs[0])
+        auto iteration = [&](int v) {
+            double* X = bufs[(2 + 2 * v) % 3];
+            double* U0 = bufs[(1 + 2 * v) % 3];
+            double* U1 = bufs[(0 + 2 * v) % 3];
+            B->impl->apply(ctx->r, X, &ctx->st->done);
+            hipLaunchKernelGGL(k_cheb_outer, dim3(rg), dim3(kBS), 0, s, n, X, (const double*)U1, (const double*)U0,
+                               (const double*)ctx->coef, (const SolveState*)ctx->st);
+            hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)X, ctx->r,
+                               (const SolveState*)ctx->st, ctx->partials);
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, int(FIN_STEP), rg, (const double*)ctx->partials,
+                               ctx->st, (double*)nullptr);
+        };
+        ensure_coef(ctx, lmin, lmax, 2);
+        st = run_outer(ctx, 1, A, B, b, x, 3, o, iteration, [](uint32_t k) { return int((k - 1) % 3); },
+                       [&](uint32_t k_last) { ensure_coef(ctx, lmin, lmax, k_last + 3); });
+        const uint32_t loops = st.niter - 2;
+        if (loops) {
+            double* X = bufs[(2 + 2 * ((loops - 1) % 3)) % 3];
+            if (X != x) MLP_HIP_CHECK(hipMemcpyAsync(x, X, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+    }
+    MLP_HIP_CHECK(hipStreamSynchronize(s));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        stats->niter = st.niter - 1;                  // cheb.cc:81
+        stats->init_norm = init_norm;
+        stats->final_norm = st.norm;
+        stats->eps_used = eps;
+        stats->t_solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->status = int(st.status);
+        stats->true_rel_res = true_rel_res(ctx, A, b, x, ctx->r, init_norm);
+    }
+    if (st.status == 5) throw Error("maximum number of iterations reached", MLP_ERR_MAXIT);
+    MLP_API_END
+}
+
+// SimpleSolver (modules/solvers/simple_solver.cc:8-88): x += B (b - A x)
+// until ||r|| / ||r0|| <= eps; like the reference it reports "No need to
+// iterate" when fewer than two iterations ran (x is updated all the same).
+int mlp_simple_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, const mlp_solve_opts* o,
+                      mlp_solve_stats* stats) {
+    MLP_API_BEGIN
+    require(ctx && A && B && b && x && o, "null argument");
+    require(B->n == A->n, "preconditioner size differs from A");
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = A->n;
+    hipStream_t s = ctx->stream;
+    ensure_workspace(ctx, n, false);
+    fill_x0(ctx, n, o->x0_mode, x);
+    launch_residual(ctx, A, b, x, ctx->r);
+    const double init_norm = std::sqrt(dev_dot(ctx, n, ctx->r, ctx->r));
+    double eps = o->eps;
+    if (init_norm * eps < 1e-14) eps = 1e-14 / init_norm;      // check_and_replace_eps, solvers.cc:49-58
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    init_state(ctx, init_norm, init_norm, eps, o->maxit, 0, 0);
+    SolveState st;
+    if (!(init_norm / init_norm > eps)) {
+        st = read_state(ctx);
+    } else {
+        const uint32_t rg = red_grid(n);
+        auto iteration = [&](int) {
+            B->impl->apply(ctx->r, ctx->z, &ctx->st->done);
+            hipLaunchKernelGGL(k_add, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, x, (const SolveState*)ctx->st);
+            hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)x, ctx->r,
+                               (const SolveState*)ctx->st, ctx->partials);
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, int(FIN_STEP), rg, (const double*)ctx->partials,
+                               ctx->st, (double*)nullptr);
+        };
+        st = run_outer(ctx, 2, A, B, b, x, 1, o, iteration, [](uint32_t) { return 0; }, [](uint32_t) {});
+    }
+    MLP_HIP_CHECK(hipStreamSynchronize(s));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        stats->niter = st.niter;
+        stats->init_norm = init_norm;
+        stats->final_norm = st.norm;
+        stats->eps_used = eps;
+        stats->t_solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->status = int(st.status);
+        stats->true_rel_res = true_rel_res(ctx, A, b, x, ctx->r, init_norm);
+    }
+    if (st.status == 5) throw Error("maximum number of iterations reached", MLP_ERR_MAXIT);
+    if (st.niter < 2) {
+        if (stats) stats->status = MLP_ERR_NO_ITER;
+        throw Error("No need to iterate", MLP_ERR_NO_ITER);
+    }
     MLP_API_END
 }
 

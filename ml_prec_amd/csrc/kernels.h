@@ -33,7 +33,8 @@ struct SolveState {
     double rz, app, alpha, beta, rr, norm, init_norm, eps;
     double rz_prev;
     uint32_t niter, done, status, maxit;
-    uint32_t alpha_one, malpha_one, beta_one, pad;
+    uint32_t alpha_one, malpha_one, beta_one;
+    uint32_t pad;                        // Chebyshev/simple: niter offset of the maxit test
     unsigned counter[8];                 // last-block counters of the fused reductions
 };
 
@@ -501,7 +502,7 @@ __global__ void __launch_bounds__(kBS) k_diag_apply(uint32_t n, const double* __
 // and updates the scalar recurrences of the device-resident SolveState.
 // ---------------------------------------------------------------------------
 enum { FIN_DOT = 0, FIN_APP = 1, FIN_XR = 2, FIN_XR_NOTEST = 3, FIN_RZ = 4, FIN_RZ_FLEX = 5, FIN_INIT = 6,
-       FIN_TRUE_RES = 7 };
+       FIN_TRUE_RES = 7, FIN_STEP = 8 };
 
 __global__ void __launch_bounds__(kBS) k_finalize(int op, uint32_t nb, const double* __restrict__ partials,
                                                   SolveState* st, double* out) {
@@ -546,6 +547,13 @@ __global__ void __launch_bounds__(kBS) k_finalize(int op, uint32_t nb, const dou
             st->rr = t0;
             st->norm = sqrt(t0);
             if (!(st->norm / st->init_norm > st->eps)) st->done = 1;
+            break;
+        case FIN_STEP:                                       // Chebyshev / simple iteration: norm, count, test
+            st->rr = t0;
+            st->norm = sqrt(t0);
+            st->niter += 1;
+            if (!(st->norm / st->init_norm > st->eps)) st->done = 1;
+            else if (st->maxit && st->niter - st->pad >= st->maxit) { st->status = 5; st->done = 1; }
             break;
     }
 }
@@ -675,6 +683,35 @@ __global__ void __launch_bounds__(kBS) k_pcg_true_res(SellView<double> A, const 
     }
     double part[1] = {s};
     block_partials<kBS, 1>(part, partials);
+}
+
+// ChebSolver step 1 (cheb.cc:37-44): x = B r scaled by alpha (dscal, skipped
+// when alpha == 1), then x += u1 (daxpy with alpha 1)
+__global__ void __launch_bounds__(kBS) k_cheb_first(uint32_t n, double* __restrict__ x, const double* __restrict__ u1,
+                                                    double alpha, int alpha_one) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double v = alpha_one ? x[i] : x[i] * alpha;
+        x[i] = v + u1[i];
+    }
+}
+
+// ChebSolver steps 2+ (cheb.cc:59-69): x = u1 + alpha x + beta (u1 - u0) with
+// alpha, beta of the current iteration count taken from a table
+// coef[2k], coef[2k+1] computed on the host
+__global__ void __launch_bounds__(kBS) k_cheb_outer(uint32_t n, double* __restrict__ x, const double* __restrict__ u1,
+                                                    const double* __restrict__ u0, const double* __restrict__ coef,
+                                                    const SolveState* st) {
+    if (st->done) return;
+    const double a = coef[2 * st->niter], b = coef[2 * st->niter + 1];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        x[i] = u1[i] + a * x[i] + b * (u1[i] - u0[i]);
+}
+
+// SimpleSolver update x += z (daxpy with alpha 1, simple_solver.cc:52)
+__global__ void __launch_bounds__(kBS) k_add(uint32_t n, const double* __restrict__ z, double* __restrict__ x,
+                                             const SolveState* st) {
+    if (st->done) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = x[i] + z[i];
 }
 
 // GMRES: the i+1 classical Gram-Schmidt dots (v_{i+1}, v_k), k = 0..i (gmres.cc:80-82)

@@ -95,6 +95,9 @@ EXPORTS = {
     "mlp_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mlp_pcg": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "mlp_gmres": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "mlp_cheb_solver": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, C.POINTER(SolveOpts),
+                                  C.POINTER(SolveStats)]),
+    "mlp_simple_solver": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "mlp_prec_bytes": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
     "mlp_time_spmv": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.c_size_t, C.POINTER(C.c_double)]),
     "mlp_time_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(C.c_double)]),
@@ -350,6 +353,32 @@ def gmres(ctx, A, B, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZER
     o = _opts(eps, maxit, x0_mode, False, 0, 0, False)
     st = SolveStats()
     rc = lib().mlp_gmres(ctx.h, A.h, B.h, b.ptr, x.ptr, C.byref(o), C.byref(st))
+    d = st.as_dict()
+    d["rc"] = rc
+    if rc and raise_on_error:
+        raise MlpError(rc, lib().mlp_last_error().decode())
+    return d
+
+
+def cheb_solver(ctx, A, B, lmin, lmax, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO,
+                check_every=0, use_graph=True, raise_on_error=True):
+    """ChebSolver on the device over [lmin, lmax]; returns the stats dict."""
+    o = _opts(eps, maxit, x0_mode, False, 0, check_every, use_graph)
+    st = SolveStats()
+    rc = lib().mlp_cheb_solver(ctx.h, A.h, B.h, float(lmin), float(lmax), b.ptr, x.ptr, C.byref(o), C.byref(st))
+    d = st.as_dict()
+    d["rc"] = rc
+    if rc and raise_on_error:
+        raise MlpError(rc, lib().mlp_last_error().decode())
+    return d
+
+
+def simple_solver(ctx, A, B, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO, check_every=0,
+                  use_graph=True, raise_on_error=True):
+    """SimpleSolver (x += B(b - A x)) on the device; returns the stats dict."""
+    o = _opts(eps, maxit, x0_mode, False, 0, check_every, use_graph)
+    st = SolveStats()
+    rc = lib().mlp_simple_solver(ctx.h, A.h, B.h, b.ptr, x.ptr, C.byref(o), C.byref(st))
     d = st.as_dict()
     d["rc"] = rc
     if rc and raise_on_error:

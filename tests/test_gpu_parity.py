@@ -218,3 +218,95 @@ def test_full_size_pcg_fp64(ctx):
     assert abs(int(sg["niter"]) - so["niter"]) <= 1
     assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo) or abs(int(sg["niter"]) - so["niter"]) == 1
     assert sg["true_rel_res"] <= 1e-8 * 1.5
+
+
+def _cheb_solver_pair(ctx, s, eps=1e-8, use_graph=True, check_every=0):
+    P = O.cheb_prec(s.ia, s.ja, s.a)
+    H = N.build_uh_cheb(s.ia, s.ja, s.a)
+    li = H.level_info(0)
+    assert (li["lmin"], li["lmax"]) == (P.level_info(0)["lmin"], P.level_info(0)["lmax"])
+    xo, so = O.cheb_solver(s.ia, s.ja, s.a, s.b, P, li["lmin"], li["lmax"], eps=eps, x0=None, maxit=20000)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, H)
+    x = ctx.array(s.n)
+    sg = N.cheb_solver(ctx, A, B, li["lmin"], li["lmax"], dev(ctx, s.b), x, eps=eps, maxit=20000,
+                       x0_mode=N.X0_REFERENCE, use_graph=use_graph, check_every=check_every)
+    return xo, so, x.get(), sg
+
+
+@pytest.mark.parametrize("name", ["cfg0_tpfa_c1", "spe10_z12"])
+def test_cheb_solver_vs_oracle(ctx, name):
+    """ChebSolver: the coefficients depend only on the iteration count, so the
+    iterates are bit-exact; only the stop test (a reduction) may differ by one."""
+    s = system(name)
+    xo, so, xg, sg = _cheb_solver_pair(ctx, s)
+    assert abs(int(sg["niter"]) - so["niter"]) <= 1, (sg["niter"], so["niter"])
+    if sg["niter"] == so["niter"]:
+        assert np.array_equal(xg, xo)
+    assert sg["true_rel_res"] <= 1e-8 * 1.0001
+
+
+def test_cheb_solver_graph_equals_stream(ctx):
+    s = system("spe10_z12")
+    _, _, x1, s1 = _cheb_solver_pair(ctx, s, use_graph=False, check_every=5)
+    _, _, x2, s2 = _cheb_solver_pair(ctx, s, use_graph=True, check_every=7)
+    assert s1["niter"] == s2["niter"]
+    assert np.array_equal(x1, x2)
+
+
+def test_cheb_solver_maxit(ctx):
+    s = system("spe10_z12")
+    H = N.build_uh_cheb(s.ia, s.ja, s.a)
+    li = H.level_info(0)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, H)
+    x = ctx.array(s.n)
+    with pytest.raises(N.MlpError) as e:
+        N.cheb_solver(ctx, A, B, li["lmin"], li["lmax"], dev(ctx, s.b), x, eps=1e-12, maxit=3)
+    assert e.value.code == 5
+    P = O.cheb_prec(s.ia, s.ja, s.a)
+    xo, so = O.cheb_solver(s.ia, s.ja, s.a, s.b, P, li["lmin"], li["lmax"], eps=1e-12, x0=np.zeros(s.n), maxit=3,
+                           check=False)
+    assert so["rc"] != 0
+    assert np.array_equal(x.get(), xo)
+
+
+def _msplit(s):
+    P = O.msplit_prec(s.ia, s.ja, s.a, q=(0.8,), niters=(5,), order="simple_1", degree_max=2)
+    H = N.build_multi_split(s.ia, s.ja, s.a, q=(0.8,), niters=(5,), order="simple_1", degree_max=2)
+    return P, H
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_simple_solver_vs_oracle(ctx, use_graph):
+    """SimpleSolver with the multi_split preconditioner: iterates bit-exact,
+    stop test within one iteration."""
+    s = system("spe10_z12")
+    P, H = _msplit(s)
+    xo, so = O.simple_solver(s.ia, s.ja, s.a, s.b, P, eps=1e-8, x0=None, maxit=5000)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, H)
+    x = ctx.array(s.n)
+    sg = N.simple_solver(ctx, A, B, dev(ctx, s.b), x, eps=1e-8, maxit=5000, x0_mode=N.X0_REFERENCE,
+                         use_graph=use_graph)
+    assert abs(int(sg["niter"]) - so["niter"]) <= 1, (sg["niter"], so["niter"])
+    if sg["niter"] == so["niter"]:
+        assert np.array_equal(x.get(), xo)
+    assert sg["true_rel_res"] <= 1e-8 * 1.0001
+
+
+def test_simple_solver_errors(ctx):
+    s = system("cfg0_spe")
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, kind=N.DIAG)
+    x = ctx.array(s.n)
+    # one iteration is enough -> "No need to iterate", x = B b all the same
+    with pytest.raises(N.MlpError) as e:
+        N.simple_solver(ctx, A, B, dev(ctx, s.b), x, eps=0.999999, x0_mode=N.X0_ZERO)
+    assert e.value.code == 4
+    P = O.diag_prec(s.ia, s.ja, s.a)
+    xo, so = O.simple_solver(s.ia, s.ja, s.a, s.b, P, eps=0.999999, x0=np.zeros(s.n), check=False)
+    assert so["niter"] == 1 and np.array_equal(x.get(), xo)
+    with pytest.raises(N.MlpError) as e:
+        N.simple_solver(ctx, A, B, dev(ctx, s.b), x, eps=1e-12, maxit=4, x0_mode=N.X0_ZERO)
+    assert e.value.code == 5

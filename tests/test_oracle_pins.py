@@ -390,3 +390,79 @@ def test_gmres_solution_and_update_bound():
     x, st = O.gmres(ia, ja, d, b, O.id_prec(n), eps=1e-12, x0=np.zeros(n), maxit=2)
     assert st["niter"] == 2
     assert np.linalg.norm(d * x - b) > 1e-3 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("name", ["tiny", "tiny_tpfa_c1"])
+def test_cheb_solver_identity_is_textbook_chebyshev(name):
+    """ChebSolver with B = I and the exact extreme eigenvalues is Chebyshev
+    acceleration; Saad, Iterative Methods, Alg. 12.1 (an algebraically
+    equivalent two-term recurrence written independently here) gives the same
+    iterates and stops at the same count."""
+    s = system(name)
+    A = dense(s)
+    w = np.linalg.eigvalsh(A)
+    lmin, lmax = w[0], w[-1]
+    xc, sc = O.cheb_solver(s.ia, s.ja, s.a, s.b, O.id_prec(s.n), lmin, lmax, eps=1e-8, x0=np.zeros(s.n),
+                           maxit=100000)
+    theta, delta = (lmax + lmin) / 2, (lmax - lmin) / 2
+    sig = theta / delta
+    rho = 1 / sig
+    x = np.zeros(s.n)
+    r = s.b - A @ x
+    i0 = np.linalg.norm(r)
+    d = r / theta
+    k = 1
+    while True:
+        x = x + d
+        r = r - A @ d
+        if np.linalg.norm(s.b - A @ x) / i0 <= 1e-8:
+            break
+        rho1 = 1 / (2 * sig - rho)
+        d = rho1 * rho * d + 2 * rho1 / delta * r
+        rho = rho1
+        k += 1
+    assert sc["niter"] == k
+    assert np.linalg.norm(x - xc) <= 1e-12 * np.linalg.norm(xc)
+
+
+def test_cheb_solver_uh_cheb_solves():
+    s = system("tiny_tpfa_c1")
+    P = O.cheb_prec(s.ia, s.ja, s.a)
+    li = P.level_info(0)
+    x, st = O.cheb_solver(s.ia, s.ja, s.a, s.b, P, li["lmin"], li["lmax"], eps=1e-10, x0=np.zeros(s.n),
+                          maxit=10000)
+    A = dense(s)
+    # the stop test is on the true residual, recomputed every iteration (cheb.cc:72-75)
+    assert np.linalg.norm(s.b - A @ x) <= 1e-10 * np.linalg.norm(s.b) * 1.0001
+    assert st["final_norm"] / st["init_norm"] <= 1e-10
+
+
+def test_simple_solver_diag_is_jacobi():
+    """SimpleSolver with B = diag(A)^-1 is the Jacobi iteration."""
+    s = system("tiny")
+    A = dense(s)
+    x, st = O.simple_solver(s.ia, s.ja, s.a, s.b, O.diag_prec(s.ia, s.ja, s.a), eps=1e-8, x0=np.zeros(s.n),
+                            maxit=100000)
+    d = np.diag(A).copy()
+    xj = np.zeros(s.n)
+    r = s.b - A @ xj
+    i0 = np.linalg.norm(r)
+    k = 0
+    while np.linalg.norm(r) / i0 > 1e-8:
+        xj = xj + r / d
+        r = s.b - A @ xj
+        k += 1
+    assert st["niter"] == k
+    assert np.linalg.norm(x - xj) <= 1e-13 * np.linalg.norm(xj)
+
+
+def test_simple_solver_no_need_to_iterate():
+    """One iteration is enough -> the reference throws "No need to iterate"
+    (simple_solver.cc:65-82) after updating x."""
+    s = system("tiny")
+    A = dense(s)
+    x, st = O.simple_solver(s.ia, s.ja, s.a, s.b, O.diag_prec(s.ia, s.ja, s.a), eps=0.99, x0=np.zeros(s.n),
+                            check=False)
+    assert st["rc"] != 0 and "No need to iterate" in st["err"]
+    assert st["niter"] == 1
+    assert np.array_equal(x, s.b * (1.0 / np.diag(A)))     # DiagPrec multiplies by 1/a_ii (diag_prec.cc)
