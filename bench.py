@@ -34,7 +34,7 @@ def log(msg):
 
 def load_traffic():
     """HBM bytes per SpMV launch from the committed rocprofv3 PMC summary, if present."""
-    p = os.path.join(ROOT, "profiles", "spmv_cfg1_traffic.json")
+    p = os.path.join(ROOT, "profiles", "k_spmv_spe10_60x220x85_traffic.json")
     if os.path.exists(p):
         with open(p) as f:
             d = json.load(f)
