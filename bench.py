@@ -62,6 +62,79 @@ def cpu_baseline(s, eps, budget_s=20.0):
             "t_const_s": t_const, "niter": iters}, x
 
 
+def ensemble(N, S, args):
+    """Config 3: COUNT SPE10-shaped systems of 60x220xnz cells, nz drawn from 5/10/20/40/85, each with its own
+    field, uh_cheb hierarchy and context (stream).  A step solves all of them to 1e-8 from x0 = 0, concurrently
+    through mlp_pcg_batch; the same systems solved one after another are timed beside it."""
+    from concurrent.futures import ThreadPoolExecutor
+    spec = S.ensemble_spec(args.ensemble)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(8) as ex:
+        systems = list(ex.map(lambda p: S.ensemble_system(*p), spec))
+        hiers = list(ex.map(lambda s: N.build_uh_cheb(s.ia, s.ja, s.a), systems))
+    cells = sum(s.n for s in systems)
+    log(f"ensemble: {len(systems)} systems, {cells} cells, generated and built in {time.perf_counter() - t0:.1f}s")
+    jobs = []
+    for s, H in zip(systems, hiers):
+        ctx = N.Context(0)
+        A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+        B = N.DevPrec(ctx, A, H, precision=N.MLP_FP64)
+        jobs.append((ctx, A, B, ctx.to_device(s.b), ctx.array(s.n)))
+    kw = dict(eps=args.eps, maxit=10000, x0_mode=N.X0_ZERO, check_every=8, use_graph=bool(args.graph))
+    for _ in range(max(1, args.warmup)):
+        rc, st = N.pcg_batch(jobs, **kw)
+        if rc:
+            raise RuntimeError(f"ensemble solve failed: {rc}")
+    for j in jobs:
+        j[0].sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rc, st = N.pcg_batch(jobs, **kw)
+    for j in jobs:
+        j[0].sync()
+    ms_batch = 1e3 * (time.perf_counter() - t0) / args.steps
+    iters = [d["niter"] for d in st]
+    worst = max(d["true_rel_res"] for d in st)
+    log(f"concurrent: {ms_batch:.2f} ms for {len(jobs)} solves ({len(jobs) / ms_batch * 1e3:.0f} solves/s), "
+        f"iterations {min(iters)}..{max(iters)}, worst true rel res {worst:.2e}")
+    t0 = time.perf_counter()
+    for ctx, A, B, b, x in jobs:
+        N.pcg(ctx, A, B, b, x, **kw)
+    for j in jobs:
+        j[0].sync()
+    ms_seq = 1e3 * (time.perf_counter() - t0)
+    log(f"one after another: {ms_seq:.2f} ms ({len(jobs) / ms_seq * 1e3:.0f} solves/s)")
+    out = {
+        "metric": "ensemble PCG solves per second (config 3, 1e-8 rel-res, FP64)",
+        "value": round(len(jobs) / ms_batch * 1e3, 2),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_batch, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(jobs)} SPE10-shaped 60x220xnz systems (nz from 5/10/20/40/85, {cells} cells "
+                               "in total), reference spe.cc assembly c=1, uh_cheb sigma=3 ncheb=2, PCG x0=0 to "
+                               f"{args.eps:g}, all solved concurrently (one stream per system)",
+                   "nz": [nz for nz, _ in spec]},
+        "ms_one_after_another": round(ms_seq, 3),
+        "speedup_concurrent": round(ms_seq / ms_batch, 3),
+        "iterations": iters,
+        "worst_true_rel_res": worst,
+    }
+    print(json.dumps(out), flush=True)
+    for ctx, A, B, b, x in jobs:
+        B.close()
+        A.close()
+    del b, x
+    for j in jobs:
+        j[0].close()
+
+
 def spmv_traffic_probe(N, s, reps):
     """Cold SpMV launches plus a calibration kernel of known traffic: x = 0.5 x
     over 2^27 doubles reads and writes exactly 1 GiB each.  Used under
@@ -96,6 +169,9 @@ def main():
     ap.add_argument("--spmv-warm-reps", type=int, default=None, help="0 skips the cache-warm SpMV timing")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--nz", type=int, default=85)
+    ap.add_argument("--ensemble", type=int, default=0, metavar="COUNT",
+                    help="config 3 instead: COUNT independent systems (nz drawn from 5/10/20/40/85) solved "
+                         "concurrently; prints its own JSON line")
     ap.add_argument("--spmv-traffic", action="store_true",
                     help="only run cold SpMV launches and a 1 GiB scale kernel (for HBM counter runs), then exit")
     args = ap.parse_args()
@@ -106,6 +182,10 @@ def main():
 
     from ml_prec_amd import native as N
     from ml_prec_amd import spe_synth as S
+
+    if args.ensemble:
+        ensemble(N, S, args)
+        return
 
     t0 = time.perf_counter()
     s = S.config_system(1, kind="spe", c=1.0, nz=args.nz)

@@ -185,6 +185,21 @@ int mlp_pcg(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, double* d_x,
             const mlp_solve_opts* opts, mlp_solve_stats* stats);
 int mlp_gmres(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, double* d_x,
               const mlp_solve_opts* opts, mlp_solve_stats* stats);
+/* Ensembles (no counterpart in the reference, which solves one system per run): `count` independent
+ * PCGSolver calls (pcg.cc:8-102) run concurrently.  Every job needs its own context (stream, workspace,
+ * device state) holding its matrix and preconditioner.  Each job's x and stats are bitwise those of a
+ * single mlp_pcg call with the same opts.  Returns MLP_OK, or the first failing job's code (every job's
+ * own code is in job.status; the jobs after a failing one still run). */
+typedef struct {
+    mlp_ctx ctx;
+    mlp_mat A;
+    mlp_prec B;
+    const double* d_b;
+    double* d_x;
+    mlp_solve_stats stats;    /* out */
+    int status;               /* out */
+} mlp_pcg_job;
+int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* opts);
 /* ChebSolver(A, lmin, lmax, b, B, x, eps, norm, silent)  modules/solvers/cheb.cc:9-100 (NORM_L2).
  * [lmin, lmax] bound the spectrum of B A, e.g. Prec::lmin()/lmax() (cheb_prec.h:122-123) = level 0 of
  * the hierarchy.  stats->niter follows the reference's count (cheb.cc:81).  opts->flexible and

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1152,12 +1153,100 @@ struct GraphPair {
     }
 };
 
+// One PCG iteration (pcg.cc:46-76) on the context's stream; every kernel is a
+// no-op once the device state says `done`.  replace = true: the recurrence
+// residual is replaced by b - A x before the preconditioner is applied, and
+// the stop test uses the replaced residual.
+void enqueue_pcg_iteration(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, bool flex, bool replace) {
+    const uint32_t n = A->n, rg = red_grid(n);
+    hipStream_t s = ctx->stream;
+    auto fin = [&](int op) {
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, op, rg, (const double*)ctx->partials, ctx->st,
+                           (double*)nullptr);
+    };
+    hipLaunchKernelGGL(k_pcg_spmv_dot, dim3(rg), dim3(kBS), 0, s, A->A.v, (const double*)ctx->p, ctx->Ap,
+                       (const SolveState*)ctx->st, ctx->partials);
+    fin(FIN_APP);
+    hipLaunchKernelGGL(k_pcg_xr, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->p, (const double*)ctx->Ap, x,
+                       ctx->r, (const SolveState*)ctx->st, ctx->partials);
+    if (replace) {
+        fin(FIN_XR_NOTEST);
+        hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)x, ctx->r,
+                           (const SolveState*)ctx->st, ctx->partials);
+        fin(FIN_TRUE_RES);
+    } else {
+        fin(FIN_XR);
+    }
+    B->impl->apply(ctx->r, ctx->z, &ctx->st->done);
+    if (flex) {
+        hipLaunchKernelGGL(k_pcg_rz<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
+                           (const double*)ctx->zp, (const SolveState*)ctx->st, ctx->partials);
+        fin(FIN_RZ_FLEX);
+        hipLaunchKernelGGL(k_pcg_p<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp,
+                           (const SolveState*)ctx->st);
+    } else {
+        hipLaunchKernelGGL(k_pcg_rz<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
+                           (const double*)ctx->zp, (const SolveState*)ctx->st, ctx->partials);
+        fin(FIN_RZ);
+        hipLaunchKernelGGL(k_pcg_p<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp,
+                           (const SolveState*)ctx->st);
+    }
+}
+
+// p = z, rz = (r, z) after z = B r (pcg.cc:32-33); no-ops when `done`
+void enqueue_pcg_start(mlp_ctx ctx, mlp_prec B, uint32_t n, bool flex) {
+    const uint32_t rg = red_grid(n);
+    hipStream_t s = ctx->stream;
+    B->impl->apply(ctx->r, ctx->z, &ctx->st->done);
+    if (flex)
+        hipLaunchKernelGGL(k_pcg_init<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
+                           ctx->p, ctx->zp, (const SolveState*)ctx->st, ctx->partials);
+    else
+        hipLaunchKernelGGL(k_pcg_init<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
+                           ctx->p, ctx->zp, (const SolveState*)ctx->st, ctx->partials);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, int(FIN_INIT), rg, (const double*)ctx->partials, ctx->st,
+                       (double*)nullptr);
+}
+
+// The cached graphs of one PCG iteration (plain and, with true_res_every,
+// residual-replacing) for these operands; nullptr when graphs are off.
+mlp_ctx_s::GraphEntry* pcg_graphs(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, bool flex,
+                                  uint32_t tre, bool use_graph) {
+    if (!use_graph) return nullptr;
+    for (auto& e : ctx->graphs)
+        if (e.solver == 0 && e.A == A->id && e.B == B->id && e.b == b && e.x == x && e.flex == int(flex) &&
+            e.tre == tre && e.ws_n == ctx->ws_n)
+            return &e;
+    hipStream_t s = ctx->stream;
+    MLP_HIP_CHECK(hipStreamSynchronize(s));
+    mlp_ctx_s::GraphEntry e;
+    e.A = A->id; e.B = B->id; e.b = b; e.x = x; e.flex = int(flex); e.tre = tre; e.ws_n = ctx->ws_n;
+    for (int v = 0; v < (tre ? 2 : 1); v++) {
+        MLP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        enqueue_pcg_iteration(ctx, A, B, b, x, flex, v == 1);
+        MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[v]));
+        MLP_HIP_CHECK(hipGraphInstantiate(&e.e[v], e.g[v], nullptr, nullptr, 0));
+    }
+    if (ctx->graphs.size() >= 16) ctx->drop_graphs(ctx->graphs.front().A);
+    ctx->graphs.push_back(e);
+    return &ctx->graphs.back();
+}
+
+// iterations k0+1 .. k0+chunk (graph replays or direct launches)
+void enqueue_pcg_chunk(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, bool flex, uint32_t tre,
+                       mlp_ctx_s::GraphEntry* ge, uint32_t k0, uint32_t chunk) {
+    for (uint32_t k = k0 + 1; k <= k0 + chunk; k++) {
+        const bool replace = tre && (k % tre == 0);
+        if (ge) MLP_HIP_CHECK(hipGraphLaunch(ge->e[replace ? 1 : 0], ctx->stream));
+        else enqueue_pcg_iteration(ctx, A, B, b, x, flex, replace);
+    }
+}
+
 // One PCG phase from the current x: r = b - A x, z = B r, p = z, then
 // iterations until the device sets `done` (pcg.cc:24-76).
 SolveState pcg_phase(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, const mlp_solve_opts* o,
                      double init_norm, double eps, uint32_t niter0, bool* no_iter) {
     const uint32_t n = A->n;
-    hipStream_t s = ctx->stream;
     const bool flex = o->flexible != 0;
     launch_residual(ctx, A, b, x, ctx->r);
     const double norm = std::sqrt(dev_dot(ctx, n, ctx->r, ctx->r));
@@ -1171,84 +1260,15 @@ SolveState pcg_phase(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double
     write_state(ctx, st0);
     *no_iter = !(norm / init_norm > eps);
     if (*no_iter) return st0;
-    B->impl->apply(ctx->r, ctx->z, nullptr);
-    const uint32_t rg = red_grid(n), sg = cdiv(n, kBS);
-    auto fin = [&](int op) {
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, op, rg, (const double*)ctx->partials, ctx->st,
-                           (double*)nullptr);
-    };
-    if (flex)
-        hipLaunchKernelGGL(k_pcg_init<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                           ctx->p, ctx->zp, ctx->partials);
-    else
-        hipLaunchKernelGGL(k_pcg_init<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                           ctx->p, ctx->zp, ctx->partials);
-    fin(FIN_INIT);
-    const uint32_t* guard = &ctx->st->done;
-    // replace = true: the recurrence residual is replaced by b - A x before the
-    // preconditioner is applied, and the stop test uses the replaced residual
-    auto iteration = [&](bool replace) {
-        hipLaunchKernelGGL(k_pcg_spmv_dot, dim3(rg), dim3(kBS), 0, s, A->A.v, (const double*)ctx->p, ctx->Ap,
-                           (const SolveState*)ctx->st, ctx->partials);
-        fin(FIN_APP);
-        hipLaunchKernelGGL(k_pcg_xr, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->p, (const double*)ctx->Ap, x,
-                           ctx->r, (const SolveState*)ctx->st, ctx->partials);
-        if (replace) {
-            fin(FIN_XR_NOTEST);
-            hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)x, ctx->r,
-                               (const SolveState*)ctx->st, ctx->partials);
-            fin(FIN_TRUE_RES);
-        } else {
-            fin(FIN_XR);
-        }
-        B->impl->apply(ctx->r, ctx->z, guard);
-        if (flex) {
-            hipLaunchKernelGGL(k_pcg_rz<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                               (const double*)ctx->zp, (const SolveState*)ctx->st, ctx->partials);
-            fin(FIN_RZ_FLEX);
-            hipLaunchKernelGGL(k_pcg_p<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp,
-                               (const SolveState*)ctx->st);
-        } else {
-            hipLaunchKernelGGL(k_pcg_rz<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                               (const double*)ctx->zp, (const SolveState*)ctx->st, ctx->partials);
-            fin(FIN_RZ);
-            hipLaunchKernelGGL(k_pcg_p<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp,
-                               (const SolveState*)ctx->st);
-        }
-    };
+    enqueue_pcg_start(ctx, B, n, flex);
     const uint32_t chunk = o->check_every ? o->check_every : 16;
     const uint32_t tre = o->true_res_every;
-    const bool use_graph = o->use_graph && B->impl->graph_safe();
-    mlp_ctx_s::GraphEntry* ge = nullptr;
-    if (use_graph) {
-        for (auto& e : ctx->graphs)
-            if (e.solver == 0 && e.A == A->id && e.B == B->id && e.b == b && e.x == x && e.flex == int(flex) &&
-                e.tre == tre && e.ws_n == ctx->ws_n)
-                ge = &e;
-        if (!ge) {
-            MLP_HIP_CHECK(hipStreamSynchronize(s));
-            mlp_ctx_s::GraphEntry e;
-            e.A = A->id; e.B = B->id; e.b = b; e.x = x; e.flex = int(flex); e.tre = tre; e.ws_n = ctx->ws_n;
-            for (int v = 0; v < (tre ? 2 : 1); v++) {
-                MLP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                iteration(v == 1);
-                MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[v]));
-                MLP_HIP_CHECK(hipGraphInstantiate(&e.e[v], e.g[v], nullptr, nullptr, 0));
-            }
-            if (ctx->graphs.size() >= 16) ctx->drop_graphs(ctx->graphs.front().A);
-            ctx->graphs.push_back(e);
-            ge = &ctx->graphs.back();
-        }
-    }
+    mlp_ctx_s::GraphEntry* ge = pcg_graphs(ctx, A, B, b, x, flex, tre, o->use_graph && B->impl->graph_safe());
     SolveState st;
     uint32_t k = 0;
     while (true) {
-        for (uint32_t c = 0; c < chunk; c++) {
-            k++;
-            const bool replace = tre && (k % tre == 0);
-            if (use_graph) MLP_HIP_CHECK(hipGraphLaunch(ge->e[replace ? 1 : 0], s));
-            else iteration(replace);
-        }
+        enqueue_pcg_chunk(ctx, A, B, b, x, flex, tre, ge, k, chunk);
+        k += chunk;
         MLP_HIP_CHECK(hipGetLastError());
         st = read_state(ctx);
         if (st.done) break;
@@ -1397,6 +1417,136 @@ int mlp_pcg(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, cons
         throw Error(buf, MLP_ERR_NOT_SPD);
     }
     if (st.status == 5) throw Error("maximum number of iterations reached", MLP_ERR_MAXIT);
+    MLP_API_END
+}
+
+// Many independent PCG solves at once (config 3: ensembles of systems).  Each
+// job has its own context, so its own stream, workspace and device state; the
+// solves start without host synchronisation (FIN_START computes the initial
+// norm and the eps floor on the device) and their iteration chunks are
+// enqueued round-robin, each job's state being polled through an event.  Per
+// job the kernels and their order are those of mlp_pcg, so every result is
+// bitwise the one a single mlp_pcg call on that job gives.
+int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
+    MLP_API_BEGIN
+    require(o && (jobs || !count), "null argument");
+    struct Run {
+        mlp_pcg_job* j;
+        hipEvent_t ev = nullptr;
+        mlp_ctx_s::GraphEntry* ge = nullptr;
+        uint32_t k = 0;
+        bool active = false;
+        std::chrono::steady_clock::time_point t0;
+    };
+    std::vector<Run> runs(count);
+    std::vector<mlp_ctx> seen;
+    const bool flex = o->flexible != 0;
+    const uint32_t chunk = o->check_every ? o->check_every : 16;
+    const uint32_t tre = o->true_res_every;
+    for (uint32_t i = 0; i < count; i++) {
+        mlp_pcg_job& J = jobs[i];
+        require(J.ctx && J.A && J.B && J.d_b && J.d_x, "null job argument");
+        require(J.B->n == J.A->n, "preconditioner size differs from A");
+        require(J.A->ctx == J.ctx && J.B->ctx == J.ctx, "a job's matrix and preconditioner must live on its context");
+        for (mlp_ctx c : seen) require(c != J.ctx, "two jobs share a context");
+        seen.push_back(J.ctx);
+        std::memset(&J.stats, 0, sizeof(J.stats));
+        J.status = MLP_OK;
+    }
+    for (uint32_t i = 0; i < count; i++) {
+        Run& R = runs[i];
+        R.j = &jobs[i];
+        mlp_ctx ctx = R.j->ctx;
+        mlp_mat A = R.j->A;
+        const uint32_t n = A->n, rg = red_grid(n);
+        R.t0 = std::chrono::steady_clock::now();
+        ensure_workspace(ctx, n, false);
+        fill_x0(ctx, n, o->x0_mode, R.j->d_x);
+        SolveState st0;
+        std::memset(&st0, 0, sizeof(st0));
+        st0.eps = o->eps;
+        st0.maxit = o->maxit;
+        std::memcpy(ctx->host_pinned, &st0, sizeof(st0));
+        MLP_HIP_CHECK(hipMemcpyAsync(ctx->st, ctx->host_pinned, sizeof(SolveState), hipMemcpyHostToDevice, ctx->stream));
+        launch_residual(ctx, A, R.j->d_b, R.j->d_x, ctx->r);
+        hipLaunchKernelGGL(k_dot, dim3(rg), dim3(kBS), 0, ctx->stream, n, (const double*)ctx->r, (const double*)ctx->r,
+                           ctx->partials);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, ctx->stream, int(FIN_START), rg,
+                           (const double*)ctx->partials, ctx->st, (double*)nullptr);
+        enqueue_pcg_start(ctx, R.j->B, n, flex);
+        R.ge = pcg_graphs(ctx, A, R.j->B, R.j->d_b, R.j->d_x, flex, tre, o->use_graph && R.j->B->impl->graph_safe());
+        MLP_HIP_CHECK(hipEventCreateWithFlags(&R.ev, hipEventDisableTiming));
+        R.active = true;
+    }
+    auto enqueue = [&](Run& R) {
+        mlp_ctx ctx = R.j->ctx;
+        enqueue_pcg_chunk(ctx, R.j->A, R.j->B, R.j->d_b, R.j->d_x, flex, tre, R.ge, R.k, chunk);
+        R.k += chunk;
+        MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, ctx->st, sizeof(SolveState), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        MLP_HIP_CHECK(hipEventRecord(R.ev, ctx->stream));
+    };
+    for (Run& R : runs) enqueue(R);
+    uint32_t left = count;
+    while (left) {
+        bool progressed = false;
+        for (Run& R : runs) {
+            if (!R.active) continue;
+            const hipError_t q = hipEventQuery(R.ev);
+            if (q == hipErrorNotReady) continue;
+            MLP_HIP_CHECK(q);
+            SolveState st;
+            std::memcpy(&st, R.j->ctx->host_pinned, sizeof(st));
+            progressed = true;
+            if (st.done) {
+                R.active = false;
+                left--;
+            } else {
+                enqueue(R);
+            }
+        }
+        if (!progressed) std::this_thread::yield();
+    }
+    int first = MLP_OK;
+    for (Run& R : runs) {
+        (void)hipEventDestroy(R.ev);
+        mlp_pcg_job& J = *R.j;
+        mlp_ctx ctx = J.ctx;
+        try {
+            SolveState st = read_state(ctx);
+            const double init_norm = st.init_norm, eps = st.eps;
+            if (st.status == 4) {
+                J.stats.init_norm = init_norm;
+                J.stats.status = MLP_ERR_NO_ITER;
+                throw Error("No need to iterate", MLP_ERR_NO_ITER);
+            }
+            if (flex) {
+                for (int restart = 0; restart < 8 && st.status == 0; restart++) {
+                    if (true_rel_res(ctx, J.A, J.d_b, J.d_x, ctx->r, init_norm) <= eps) break;
+                    bool again = false;
+                    st = pcg_phase(ctx, J.A, J.B, J.d_b, J.d_x, o, init_norm, eps, st.niter, &again);
+                    if (again) break;
+                }
+            }
+            MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            J.stats.niter = st.niter;
+            J.stats.init_norm = init_norm;
+            J.stats.final_norm = st.norm;
+            J.stats.eps_used = eps;
+            J.stats.t_solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - R.t0).count();
+            J.stats.status = int(st.status);
+            J.stats.true_rel_res = true_rel_res(ctx, J.A, J.d_b, J.d_x, ctx->r, init_norm);
+            if (st.status == 3) throw Error("<Ap, p> <= 0", MLP_ERR_NOT_SPD);
+            if (st.status == 5) throw Error("maximum number of iterations reached", MLP_ERR_MAXIT);
+        } catch (const Error& e) {
+            J.status = e.code();
+            if (first == MLP_OK) {
+                first = e.code();
+                set_last_error(e.what());
+            }
+        }
+    }
+    return first;
     MLP_API_END
 }
 

@@ -502,11 +502,11 @@ __global__ void __launch_bounds__(kBS) k_diag_apply(uint32_t n, const double* __
 // and updates the scalar recurrences of the device-resident SolveState.
 // ---------------------------------------------------------------------------
 enum { FIN_DOT = 0, FIN_APP = 1, FIN_XR = 2, FIN_XR_NOTEST = 3, FIN_RZ = 4, FIN_RZ_FLEX = 5, FIN_INIT = 6,
-       FIN_TRUE_RES = 7, FIN_STEP = 8 };
+       FIN_TRUE_RES = 7, FIN_STEP = 8, FIN_START = 9 };
 
 __global__ void __launch_bounds__(kBS) k_finalize(int op, uint32_t nb, const double* __restrict__ partials,
                                                   SolveState* st, double* out) {
-    if (op != FIN_DOT && op != FIN_INIT && st->done) return;
+    if (op != FIN_DOT && op != FIN_INIT && op != FIN_START && st->done) return;
     __shared__ double sh[kBS / 64];
     const int NV = (op == FIN_RZ_FLEX) ? 2 : 1;
     const double t0 = sum_partials<kBS>(partials, nb, NV, 0, sh);
@@ -548,6 +548,16 @@ __global__ void __launch_bounds__(kBS) k_finalize(int op, uint32_t nb, const dou
             st->norm = sqrt(t0);
             if (!(st->norm / st->init_norm > st->eps)) st->done = 1;
             break;
+        case FIN_START: {                                    // init norm, eps floor, first test   pcg.cc:25-31
+            const double nrm = sqrt(t0);
+            st->init_norm = nrm;
+            st->norm = nrm;
+            double e = st->eps;
+            if (nrm * e < 1e-14) e = 1e-14 / nrm;            // check_and_replace_eps, solvers.cc:49-58
+            st->eps = e;
+            if (!(nrm / nrm > e)) { st->status = 4; st->done = 1; }
+            break;
+        }
         case FIN_STEP:                                       // Chebyshev / simple iteration: norm, count, test
             st->rr = t0;
             st->norm = sqrt(t0);
@@ -655,7 +665,9 @@ __global__ void __launch_bounds__(kBS) k_pcg_p(uint32_t n, const double* __restr
 // p = z ; partials of (r, z)   (pcg.cc:32-33)
 template <bool FLEX>
 __global__ void __launch_bounds__(kBS) k_pcg_init(uint32_t n, const double* __restrict__ r, const double* __restrict__ z,
-                                                  double* __restrict__ p, double* __restrict__ zp, double* partials) {
+                                                  double* __restrict__ p, double* __restrict__ zp, const SolveState* st,
+                                                  double* partials) {
+    if (st->done) return;
     double s = 0.0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         p[i] = z[i];

@@ -61,6 +61,11 @@ class SolveStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class PcgJob(C.Structure):
+    _fields_ = [("ctx", _vp), ("A", _vp), ("B", _vp), ("d_b", _vp), ("d_x", _vp), ("stats", SolveStats),
+                ("status", C.c_int)]
+
+
 EXPORTS = {
     "mlp_last_error": (C.c_char_p, []),
     "mlp_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
@@ -95,6 +100,7 @@ EXPORTS = {
     "mlp_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mlp_pcg": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "mlp_gmres": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "mlp_pcg_batch": (C.c_int, [C.c_uint32, C.POINTER(PcgJob), C.POINTER(SolveOpts)]),
     "mlp_cheb_solver": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, C.POINTER(SolveOpts),
                                   C.POINTER(SolveStats)]),
     "mlp_simple_solver": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
@@ -346,6 +352,23 @@ def pcg(ctx, A, B, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO,
     if rc and raise_on_error:
         raise MlpError(rc, lib().mlp_last_error().decode())
     return d
+
+
+def pcg_batch(jobs, eps=1e-8, maxit=0, x0_mode=X0_ZERO, flexible=False, true_res_every=0, check_every=16,
+              use_graph=False):
+    """Concurrent PCG solves.  jobs: sequence of (ctx, A, B, b, x), one context per job.
+    Returns (rc, [stats dict with 'status' per job])."""
+    o = _opts(eps, maxit, x0_mode, flexible, true_res_every, check_every, use_graph)
+    arr = (PcgJob * len(jobs))()
+    for i, (ctx, A, B, b, x) in enumerate(jobs):
+        arr[i].ctx, arr[i].A, arr[i].B, arr[i].d_b, arr[i].d_x = ctx.h, A.h, B.h, b.ptr, x.ptr
+    rc = lib().mlp_pcg_batch(len(jobs), arr, C.byref(o))
+    out = []
+    for j in arr:
+        d = j.stats.as_dict()
+        d["rc"] = j.status
+        out.append(d)
+    return rc, out
 
 
 def gmres(ctx, A, B, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO, raise_on_error=True):

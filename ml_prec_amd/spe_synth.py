@@ -223,6 +223,22 @@ def config_system(cfg: int, seed: int | None = None, kind="tpfa", c=0.0, nz: int
     raise ValueError(cfg)
 
 
+ENSEMBLE_NZ = (5, 10, 20, 40, 85)
+
+
+def ensemble_spec(count=64, base_seed=BASE_SEED + 3):
+    """Config 3: `count` systems, system i with its own field seed base_seed + 1000 + i and a layer count
+    drawn uniformly from ENSEMBLE_NZ by a generator seeded with base_seed.  Returns [(nz, seed)]."""
+    g = np.random.default_rng(base_seed)
+    nzs = g.choice(np.array(ENSEMBLE_NZ), size=count)
+    return [(int(nz), base_seed + 1000 + i) for i, nz in enumerate(nzs)]
+
+
+def ensemble_system(nz, seed, kind="spe", c=1.0):
+    """One member of the config-3 ensemble: the SPE10-shaped 60x220xnz field of `seed`."""
+    return make_system(60, 220, nz, seed, "spe10", kind, c)
+
+
 def write_csr_binary(path, ia, ja, a, ncol=None):
     """CSR BINARY layout: uint nrow, uint ncol, ia[n+1], ja[nnz], double a[nnz]
     (``modules/matrix/csrmatrix.cc:314-320``)."""
