@@ -135,6 +135,70 @@ def ensemble(N, S, args):
         j[0].close()
 
 
+def adjoint(N, S, args):
+    """Config 4: L = sum over 200 observed cells of (p_i - d_i)^2 with d = p(true field) + N(0, 0.01|p_i|);
+    theta = ln kx of a second field; one gradient = forward solve + adjoint solve (1e-10) + gradient kernel.
+    Checked against central differences (h = 1e-6) along --fd-dirs seeded directions."""
+    from ml_prec_amd.adjoint import SpeAdjointModel, misfit
+    nz = args.nz
+    shape = (nz, 220, 60)
+    n = 60 * 220 * nz
+    g = np.random.default_rng(S.BASE_SEED + 4)
+    theta_true = np.log(S.spe10_field(60, 220, nz, S.BASE_SEED + 1).ravel())
+    theta = np.log(S.spe10_field(60, 220, nz, S.BASE_SEED + 40).ravel())
+    b = S.rhs(n, S.BASE_SEED + 1)
+    ctx = N.Context(0)
+    model = SpeAdjointModel(ctx, shape, b, eps=1e-10)
+    obs = g.choice(n, 200, replace=False)
+    p_true = model.forward(theta_true)
+    d = p_true[obs] + g.normal(0.0, 1.0, obs.size) * 0.01 * np.abs(p_true[obs])
+    times = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        p = model.forward(theta)
+        L, gp = misfit(p, obs, d)
+        grad = model.vjp(gp)
+        times.append(time.perf_counter() - t0)
+    fw, adj = model.stats[-2][1], model.stats[-1][1]
+    log(f"adjoint gradient: {1e3 * np.median(times):.1f} ms (assembly + construction on the host, forward "
+        f"{fw['niter']} and adjoint {adj['niter']} PCG iterations, solve times {fw['t_solve_ms']:.2f} + "
+        f"{adj['t_solve_ms']:.2f} ms)")
+    errs = []
+    h = 1e-6
+    for k in range(args.fd_dirs):
+        v = np.random.default_rng(S.BASE_SEED + 100 + k).standard_normal(n)
+        Lp, _ = misfit(model.forward(theta + h * v), obs, d)
+        Lm, _ = misfit(model.forward(theta - h * v), obs, d)
+        fd = (Lp - Lm) / (2 * h)
+        errs.append(abs(fd - grad @ v) / abs(fd))
+    log(f"central differences along {args.fd_dirs} directions: max rel. difference {max(errs):.2e}, "
+        f"median {np.median(errs):.2e}")
+    out = {
+        "metric": "adjoint misfit gradient time (config 4, forward + adjoint PCG to 1e-10)",
+        "value": round(1e3 * float(np.median(times)), 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": len(times),
+        "warmup": 0,
+        "ms_per_step": round(1e3 * float(np.median(times)), 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"SPE10-shaped 60x220x{nz} ({n} cells), theta = ln kx, 200 observations, "
+                               "uh_cheb PCG forward and adjoint to 1e-10"},
+        "device_solve_ms": {"forward": fw["t_solve_ms"], "adjoint": adj["t_solve_ms"]},
+        "iterations": {"forward": fw["niter"], "adjoint": adj["niter"]},
+        "misfit": L,
+        "fd_check": {"directions": args.fd_dirs, "h": h, "max_rel_diff": max(errs),
+                     "median_rel_diff": float(np.median(errs))},
+    }
+    print(json.dumps(out), flush=True)
+    model.close()
+    ctx.close()
+
+
 def spmv_traffic_probe(N, s, reps):
     """Cold SpMV launches plus a calibration kernel of known traffic: x = 0.5 x
     over 2^27 doubles reads and writes exactly 1 GiB each.  Used under
@@ -172,6 +236,9 @@ def main():
     ap.add_argument("--ensemble", type=int, default=0, metavar="COUNT",
                     help="config 3 instead: COUNT independent systems (nz drawn from 5/10/20/40/85) solved "
                          "concurrently; prints its own JSON line")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="config 4 instead: misfit gradient by the adjoint solve, checked by central differences")
+    ap.add_argument("--fd-dirs", type=int, default=20)
     ap.add_argument("--spmv-traffic", action="store_true",
                     help="only run cold SpMV launches and a 1 GiB scale kernel (for HBM counter runs), then exit")
     args = ap.parse_args()
@@ -185,6 +252,9 @@ def main():
 
     if args.ensemble:
         ensemble(N, S, args)
+        return
+    if args.adjoint:
+        adjoint(N, S, args)
         return
 
     t0 = time.perf_counter()

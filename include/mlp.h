@@ -200,6 +200,13 @@ typedef struct {
     int status;               /* out */
 } mlp_pcg_job;
 int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* opts);
+/* Adjoint gradient for the two-point-flux assembly of modules/mesh/spe.cc:69-131 (no counterpart in the
+ * reference: config 4 differentiates a misfit through the solve).  Parameters theta_i = ln kx_i per cell
+ * (ky = kx, kz = kz_ratio kx); fx, fy, fz are the per-direction face factors (1/h^2 in spe.cc's scaling);
+ * cells are numbered x fastest.  Writes grad_i = -lam^T (dA/dtheta_i) p, which is dL/dtheta_i when
+ * A p = b and A lam = dL/dp (A symmetric).  All vectors are device pointers of nx*ny*nz doubles. */
+int mlp_spe_adjoint_grad(mlp_ctx ctx, uint32_t nx, uint32_t ny, uint32_t nz, double fx, double fy, double fz,
+                         double kz_ratio, const double* d_kx, const double* d_p, const double* d_lam, double* d_grad);
 /* ChebSolver(A, lmin, lmax, b, B, x, eps, norm, silent)  modules/solvers/cheb.cc:9-100 (NORM_L2).
  * [lmin, lmax] bound the spectrum of B A, e.g. Prec::lmin()/lmax() (cheb_prec.h:122-123) = level 0 of
  * the hierarchy.  stats->niter follows the reference's count (cheb.cc:81).  opts->flexible and

@@ -1550,6 +1550,22 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
     MLP_API_END
 }
 
+// Gradient of lam^T A(theta) p with respect to theta = ln kx for the
+// two-point-flux assembly of modules/mesh/spe.cc:69-131 (see k_spe_grad):
+// with lam the adjoint solution of A lam = dL/dp this is -dL/dtheta.
+int mlp_spe_adjoint_grad(mlp_ctx ctx, uint32_t nx, uint32_t ny, uint32_t nz, double fx, double fy, double fz,
+                         double kz_ratio, const double* kx, const double* p, const double* lam, double* grad) {
+    MLP_API_BEGIN
+    require(ctx && kx && p && lam && grad, "null argument");
+    const uint64_t n = uint64_t(nx) * ny * nz;
+    require(n > 0 && n < (1ull << 32), "grid size out of range");
+    require(fx > 0 && fy > 0 && fz > 0 && kz_ratio > 0, "face factors must be positive");
+    hipLaunchKernelGGL(k_spe_grad, dim3(cdiv(uint32_t(n), kBS)), dim3(kBS), 0, ctx->stream, nx, ny, nz, fx, fy, fz,
+                       kz_ratio, kx, p, lam, grad);
+    MLP_HIP_CHECK(hipGetLastError());
+    MLP_API_END
+}
+
 // GMRESSolver (modules/solvers/gmres.cc:13-168): restart 30, classical
 // Gram-Schmidt on the device, Givens rotations and the triangular solve on the
 // host in FP64 exactly as the reference orders them (including its

@@ -726,6 +726,38 @@ __global__ void __launch_bounds__(kBS) k_add(uint32_t n, const double* __restric
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = x[i] + z[i];
 }
 
+// Adjoint gradient of the 7-point two-point-flux matrix (the reference's
+// assembly, modules/mesh/spe.cc:69-131: a face between cells a and b
+// contributes T = 2/(1/k_a + 1/k_b) * fac to both diagonals and -T to both
+// off-diagonals) with respect to theta = ln kx per cell, ky = kx and
+// kz = kzr * kx:  g_i = -lam^T (dA/dtheta_i) p
+//                    = -sum_{faces f of i} dT_f/dtheta_i (lam_a - lam_b)(p_a - p_b),
+// dT_f/dtheta_i = T_f^2 / (2 fac k_i) with k_i the cell's permeability along
+// the face.  Cells are numbered x fastest, then y, then z.
+__global__ void __launch_bounds__(kBS) k_spe_grad(uint32_t nx, uint32_t ny, uint32_t nz, double fx, double fy,
+                                                  double fz, double kzr, const double* __restrict__ kx,
+                                                  const double* __restrict__ p, const double* __restrict__ lam,
+                                                  double* __restrict__ grad) {
+    const uint32_t n = nx * ny * nz;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+    const double ki = kx[i], pi = p[i], li = lam[i];
+    double g = 0.0;
+    auto face = [&](uint32_t j, double fac, double kr) {
+        const double ka = kr * ki, kb = kr * kx[j];
+        const double T = 2.0 / (1.0 / ka + 1.0 / kb) * fac;
+        g -= T * T / (2.0 * fac * ka) * (li - lam[j]) * (pi - p[j]);
+    };
+    if (z > 0) face(i - nx * ny, fz, kzr);
+    if (y > 0) face(i - nx, fy, 1.0);
+    if (x > 0) face(i - 1, fx, 1.0);
+    if (x + 1 < nx) face(i + 1, fx, 1.0);
+    if (y + 1 < ny) face(i + nx, fy, 1.0);
+    if (z + 1 < nz) face(i + nx * ny, fz, kzr);
+    grad[i] = g;
+}
+
 // GMRES: the i+1 classical Gram-Schmidt dots (v_{i+1}, v_k), k = 0..i (gmres.cc:80-82)
 __global__ void __launch_bounds__(kBS) k_mdot(uint32_t n, uint32_t nv, const double* __restrict__ V, size_t ld,
                                               const double* __restrict__ w, double* partials) {

@@ -100,6 +100,8 @@ EXPORTS = {
     "mlp_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mlp_pcg": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "mlp_gmres": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "mlp_spe_adjoint_grad": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, _vp, _vp, _vp, _vp]),
     "mlp_pcg_batch": (C.c_int, [C.c_uint32, C.POINTER(PcgJob), C.POINTER(SolveOpts)]),
     "mlp_cheb_solver": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, C.POINTER(SolveOpts),
                                   C.POINTER(SolveStats)]),
@@ -407,6 +409,14 @@ def simple_solver(ctx, A, B, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mod
     if rc and raise_on_error:
         raise MlpError(rc, lib().mlp_last_error().decode())
     return d
+
+
+def spe_adjoint_grad(ctx, shape, face_factors, kz_ratio, kx: DevArray, p: DevArray, lam: DevArray,
+                     grad: DevArray):
+    """grad = -lam^T dA/d(ln kx) p for the two-point-flux assembly; shape = (nz, ny, nx)."""
+    nz, ny, nx = shape
+    fx, fy, fz = face_factors
+    check(lib().mlp_spe_adjoint_grad(ctx.h, nx, ny, nz, fx, fy, fz, kz_ratio, kx.ptr, p.ptr, lam.ptr, grad.ptr))
 
 
 def ddot(ctx, x: DevArray, y: DevArray):
