@@ -283,6 +283,33 @@ __device__ __forceinline__ void block_down(const ProgView<VT>& P, const uint32_t
     }
     for (uint32_t li = threadIdx.x; li < nl; li += blockDim.x) slv[li] = P.lvlL[B[2] + li];
     __syncthreads();
+    if (S <= blockDim.x) {
+        // Thread p owns node p: its value, entry range and first two entries are
+        // read once; each depth level then costs one gather of solved values,
+        // the update and the barrier.
+        const uint32_t p = threadIdx.x;
+        uint32_t my = nl, q0 = 0, q1 = 0, l0 = 0, l1 = 0;
+        VT acc = VT(0), v0 = VT(0), v1 = VT(0);
+        if (p < S) {
+            my = 0;
+            while (p >= slv[my]) my++;
+            acc = sw[p];
+            q0 = sptr[p];
+            q1 = sptr[p + 1];
+            if (q1 > q0) { l0 = sloc[q0]; v0 = sval[q0]; }
+            if (q1 > q0 + 1) { l1 = sloc[q0 + 1]; v1 = sval[q0 + 1]; }
+        }
+        for (uint32_t li = 0; li < nl; li++) {
+            if (li == my) {
+                if (q1 > q0) acc -= v0 * sw[l0];
+                if (q1 > q0 + 1) acc -= v1 * sw[l1];
+                for (uint32_t q = q0 + 2; q < q1; q++) acc -= sval[q] * sw[sloc[q]];
+                sw[p] = acc;
+            }
+            __syncthreads();
+        }
+        return;
+    }
     uint32_t s = 0;
     for (uint32_t li = 0; li < nl; li++) {
         const uint32_t e = slv[li];
@@ -329,6 +356,46 @@ __device__ __forceinline__ void block_up(const ProgView<VT>& P, const uint32_t* 
     }
     for (uint32_t li = threadIdx.x; li < nl; li += blockDim.x) slv[li] = P.lvlU[B[4] + li];
     __syncthreads();
+    if (nu <= blockDim.x) {
+        // thread u owns row u of the U list; static data read once (see block_down)
+        const uint32_t u = threadIdx.x;
+        uint32_t my = nl, q0 = 0, q1 = 0, c0 = 0, c1 = 0, loc = 0;
+        VT z = VT(0), d = VT(1), v0 = VT(0), v1 = VT(0), r0 = VT(0), r1 = VT(0);
+        if (u < nu) {
+            my = 0;
+            while (u >= slv[my]) my++;
+            z = sz[u];
+            d = sdiag[u];
+            loc = sloc[u];
+            q0 = sptr[u];
+            q1 = sptr[u + 1];
+            if (q1 > q0) {
+                c0 = scol[q0];
+                v0 = sval[q0];
+                if (c0 & kGlobalBit) r0 = sref[q0];
+            }
+            if (q1 > q0 + 1) {
+                c1 = scol[q0 + 1];
+                v1 = sval[q0 + 1];
+                if (c1 & kGlobalBit) r1 = sref[q0 + 1];
+            }
+        }
+        for (uint32_t li = 0; li < nl; li++) {
+            if (li == my) {
+                if (q1 > q0) z -= v0 * ((c0 & kGlobalBit) ? r0 : sx[c0]);
+                if (q1 > q0 + 1) z -= v1 * ((c1 & kGlobalBit) ? r1 : sx[c1]);
+                for (uint32_t q = q0 + 2; q < q1; q++) {
+                    const uint32_t c = scol[q];
+                    z -= sval[q] * ((c & kGlobalBit) ? sref[q] : sx[c]);
+                }
+                const VT v = z / d;
+                sx[loc] = v;
+                sz[u] = v;
+            }
+            __syncthreads();
+        }
+        return;
+    }
     uint32_t s = 0;
     for (uint32_t li = 0; li < nl; li++) {
         const uint32_t e = slv[li];
