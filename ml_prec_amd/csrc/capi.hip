@@ -276,6 +276,8 @@ template <class VT> struct DevHier : DevPrecBase {
                 G.Lptr = mem.upload(P.Lptr, s);
                 G.Lloc = mem.upload(P.Lloc, s);
                 G.Lval = mem.upload_as<VT>(P.Lval, s);
+                G.Ldep = mem.upload(P.Ldep, s);
+                G.Udep = mem.upload(P.Udep, s);
                 G.uloc = mem.upload(P.uloc, s);
                 G.uxidx = mem.upload(P.uxidx, s);
                 G.Uptr = mem.upload(P.Uptr, s);
@@ -284,9 +286,16 @@ template <class VT> struct DevHier : DevPrecBase {
                 G.Udiag = mem.upload_as<VT>(P.Udiag, s);
                 L.wt = mem.alloc<VT>(P.nodes);
             }
-            L.lds_down = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, false).total;
-            L.lds_up = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, false, true).total;
-            L.lds_solve = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true).total;
+            if (P.max_nodes <= lbs && P.max_Unodes <= lbs) {
+                // every block takes the owned-node path, which keeps only w (sw) and x (sx) in LDS
+                const uint32_t v = (P.max_nodes * uint32_t(sizeof(VT)) + 7u) & ~7u;
+                L.lds_down = v;
+                L.lds_up = L.lds_solve = 2 * v;
+            } else {
+                L.lds_down = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, false).total;
+                L.lds_up = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, false, true).total;
+                L.lds_solve = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true).total;
+            }
             std::vector<double> dv(D.dval, D.dval + D.Md);
             L.dval = mem.upload_as<VT>(dv, s);
             if (l > 0) {

@@ -164,6 +164,8 @@ template <class VT> struct ProgView {
     const uint32_t* Lptr = nullptr;
     const uint32_t* Lloc = nullptr;
     const VT* Lval = nullptr;
+    const uint32_t* Ldep = nullptr;    // depth level of each node within its block
+    const uint32_t* Udep = nullptr;    // U depth level of each U-list row within its block
     const uint32_t* uloc = nullptr;
     const uint32_t* uxidx = nullptr;
     const uint32_t* Uptr = nullptr;
@@ -283,33 +285,6 @@ __device__ __forceinline__ void block_down(const ProgView<VT>& P, const uint32_t
     }
     for (uint32_t li = threadIdx.x; li < nl; li += blockDim.x) slv[li] = P.lvlL[B[2] + li];
     __syncthreads();
-    if (S <= blockDim.x) {
-        // Thread p owns node p: its value, entry range and first two entries are
-        // read once; each depth level then costs one gather of solved values,
-        // the update and the barrier.
-        const uint32_t p = threadIdx.x;
-        uint32_t my = nl, q0 = 0, q1 = 0, l0 = 0, l1 = 0;
-        VT acc = VT(0), v0 = VT(0), v1 = VT(0);
-        if (p < S) {
-            my = 0;
-            while (p >= slv[my]) my++;
-            acc = sw[p];
-            q0 = sptr[p];
-            q1 = sptr[p + 1];
-            if (q1 > q0) { l0 = sloc[q0]; v0 = sval[q0]; }
-            if (q1 > q0 + 1) { l1 = sloc[q0 + 1]; v1 = sval[q0 + 1]; }
-        }
-        for (uint32_t li = 0; li < nl; li++) {
-            if (li == my) {
-                if (q1 > q0) acc -= v0 * sw[l0];
-                if (q1 > q0 + 1) acc -= v1 * sw[l1];
-                for (uint32_t q = q0 + 2; q < q1; q++) acc -= sval[q] * sw[sloc[q]];
-                sw[p] = acc;
-            }
-            __syncthreads();
-        }
-        return;
-    }
     uint32_t s = 0;
     for (uint32_t li = 0; li < nl; li++) {
         const uint32_t e = slv[li];
@@ -356,46 +331,6 @@ __device__ __forceinline__ void block_up(const ProgView<VT>& P, const uint32_t* 
     }
     for (uint32_t li = threadIdx.x; li < nl; li += blockDim.x) slv[li] = P.lvlU[B[4] + li];
     __syncthreads();
-    if (nu <= blockDim.x) {
-        // thread u owns row u of the U list; static data read once (see block_down)
-        const uint32_t u = threadIdx.x;
-        uint32_t my = nl, q0 = 0, q1 = 0, c0 = 0, c1 = 0, loc = 0;
-        VT z = VT(0), d = VT(1), v0 = VT(0), v1 = VT(0), r0 = VT(0), r1 = VT(0);
-        if (u < nu) {
-            my = 0;
-            while (u >= slv[my]) my++;
-            z = sz[u];
-            d = sdiag[u];
-            loc = sloc[u];
-            q0 = sptr[u];
-            q1 = sptr[u + 1];
-            if (q1 > q0) {
-                c0 = scol[q0];
-                v0 = sval[q0];
-                if (c0 & kGlobalBit) r0 = sref[q0];
-            }
-            if (q1 > q0 + 1) {
-                c1 = scol[q0 + 1];
-                v1 = sval[q0 + 1];
-                if (c1 & kGlobalBit) r1 = sref[q0 + 1];
-            }
-        }
-        for (uint32_t li = 0; li < nl; li++) {
-            if (li == my) {
-                if (q1 > q0) z -= v0 * ((c0 & kGlobalBit) ? r0 : sx[c0]);
-                if (q1 > q0 + 1) z -= v1 * ((c1 & kGlobalBit) ? r1 : sx[c1]);
-                for (uint32_t q = q0 + 2; q < q1; q++) {
-                    const uint32_t c = scol[q];
-                    z -= sval[q] * ((c & kGlobalBit) ? sref[q] : sx[c]);
-                }
-                const VT v = z / d;
-                sx[loc] = v;
-                sz[u] = v;
-            }
-            __syncthreads();
-        }
-        return;
-    }
     uint32_t s = 0;
     for (uint32_t li = 0; li < nl; li++) {
         const uint32_t e = slv[li];
@@ -413,6 +348,85 @@ __device__ __forceinline__ void block_up(const ProgView<VT>& P, const uint32_t* 
         s = e;
         __syncthreads();
     }
+}
+
+// Owned-node sweeps, used when a block has no more nodes (U-list rows) than
+// threads: thread p owns node p, reads its right-hand side, depth level, entry
+// range and first two entries straight from global memory once, and at its
+// depth level gathers the solved values it needs from LDS.  A node's value is
+// read only by nodes of later levels, after the barrier that ends its own
+// level, so nothing is staged before the sweep.  Returns the thread's solved
+// value (0 for threads without a node); sw holds every node's w afterwards.
+template <class VT, class FT, int IN>
+__device__ __forceinline__ VT owned_down(const ProgView<VT>& P, const uint32_t* B, const InSrc<VT, FT, IN>& f,
+                                         VT* sw) {
+    const uint32_t base = B[0], S = B[1], nl = B[3] - B[2];
+    const uint32_t p = threadIdx.x;
+    uint32_t my = nl, q0 = 0, q1 = 0, l0 = 0, l1 = 0;
+    VT acc = VT(0), v0 = VT(0), v1 = VT(0);
+    if (p < S) {
+        my = P.Ldep[base + p];
+        q0 = P.Lptr[base + p];
+        q1 = P.Lptr[base + p + 1];
+        acc = fetch_at(P, f, base + p);
+        if (q1 > q0) { l0 = P.Lloc[q0]; v0 = P.Lval[q0]; }
+        if (q1 > q0 + 1) { l1 = P.Lloc[q0 + 1]; v1 = P.Lval[q0 + 1]; }
+    }
+    for (uint32_t li = 0; li < nl; li++) {
+        if (li == my) {
+            if (q1 > q0) acc -= v0 * sw[l0];
+            if (q1 > q0 + 1) acc -= v1 * sw[l1];
+            for (uint32_t q = q0 + 2; q < q1; q++) acc -= P.Lval[q] * sw[P.Lloc[q]];
+            sw[p] = acc;
+        }
+        __syncthreads();
+    }
+    return acc;
+}
+
+// U sweep of the owned-node path.  w of the row comes from sw (after
+// owned_down in the same launch) or from wt; kept columns from the combine op.
+// Solved values land in sx (local L positions); returns the thread's value.
+template <class VT, int OP, bool W_IN_LDS>
+__device__ __forceinline__ VT owned_up(const ProgView<VT>& P, const uint32_t* B, const VT* __restrict__ wt,
+                                       const Combine<VT>& C, const VT* sw, VT* sx) {
+    const uint32_t base = B[0], ub = B[10], nu = B[11], nl = B[5] - B[4];
+    const uint32_t u = threadIdx.x;
+    uint32_t my = nl, q0 = 0, q1 = 0, c0 = 0, c1 = 0, loc = 0;
+    VT z = VT(0), d = VT(1), v0 = VT(0), v1 = VT(0), r0 = VT(0), r1 = VT(0);
+    if (u < nu) {
+        my = P.Udep[ub + u];
+        loc = P.uloc[ub + u];
+        d = P.Udiag[ub + u];
+        q0 = P.Uptr[ub + u];
+        q1 = P.Uptr[ub + u + 1];
+        z = W_IN_LDS ? sw[loc] : wt[base + loc];
+        if (q1 > q0) {
+            c0 = P.Ucol[q0];
+            v0 = P.Uval[q0];
+            if (OP != OP_NONE && (c0 & kGlobalBit)) r0 = combine<VT, OP>(C, c0 & ~kGlobalBit);
+        }
+        if (q1 > q0 + 1) {
+            c1 = P.Ucol[q0 + 1];
+            v1 = P.Uval[q0 + 1];
+            if (OP != OP_NONE && (c1 & kGlobalBit)) r1 = combine<VT, OP>(C, c1 & ~kGlobalBit);
+        }
+    }
+    for (uint32_t li = 0; li < nl; li++) {
+        if (li == my) {
+            if (q1 > q0) z -= v0 * ((c0 & kGlobalBit) ? r0 : sx[c0]);
+            if (q1 > q0 + 1) z -= v1 * ((c1 & kGlobalBit) ? r1 : sx[c1]);
+            for (uint32_t q = q0 + 2; q < q1; q++) {
+                const uint32_t c = P.Ucol[q];
+                const VT xv = (c & kGlobalBit) ? combine<VT, OP>(C, c & ~kGlobalBit) : sx[c];
+                z -= P.Uval[q] * xv;
+            }
+            z = z / d;
+            sx[loc] = z;
+        }
+        __syncthreads();
+    }
+    return z;
 }
 
 template <class XT, int MODE>
@@ -450,6 +464,14 @@ __global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, InSrc<VT, F
         const uint32_t base = h.w[0], S = h.w[1];
         // the store targets of this thread's first node, fetched ahead of the sweep
         const uint32_t o0 = threadIdx.x < S ? P.out[base + threadIdx.x] : 0u;
+        if (S <= blockDim.x) {
+            const VT v = owned_down(P, h.w, f, reinterpret_cast<VT*>(lds_raw + lp.sw));
+            if (threadIdx.x < S) {
+                if (o0 & kGlobalBit) F[o0 & ~kGlobalBit] = v;
+                else wt[base + threadIdx.x] = v;
+            }
+            return;
+        }
         block_down(P, h.w, f, lds_raw, lp);
         const VT* sw = reinterpret_cast<const VT*>(lds_raw + lp.sw);
         for (uint32_t p = threadIdx.x; p < S; p += blockDim.x) {
@@ -483,6 +505,11 @@ __global__ void __launch_bounds__(kLBS) k_level_up(ProgView<VT> P, const VT* __r
         const BlkHdr h = load_hdr(P.blk);
         const uint32_t ub = h.w[10], nu = h.w[11];
         const uint32_t x0 = threadIdx.x < nu ? P.uxidx[ub + threadIdx.x] : 0u;
+        if (nu <= blockDim.x) {
+            const VT v = owned_up<VT, OP, false>(P, h.w, wt, C, nullptr, reinterpret_cast<VT*>(lds_raw + lp.sx));
+            if (threadIdx.x < nu) put<XT, UMODE>(x, x0, XT(v));
+            return;
+        }
         block_up<VT, OP, false>(P, h.w, wt, C, lds_raw, lp);
         const VT* sz = reinterpret_cast<const VT*>(lds_raw + lp.sz);
         for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x)
@@ -506,6 +533,14 @@ __global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, InSrc<VT, 
         const BlkHdr h = load_hdr(P.blk);
         const uint32_t ub = h.w[10], nu = h.w[11];
         const uint32_t x0 = threadIdx.x < nu ? P.uxidx[ub + threadIdx.x] : 0u;
+        if (h.w[1] <= blockDim.x && nu <= blockDim.x) {
+            VT* sw = reinterpret_cast<VT*>(lds_raw + lp.sw);
+            owned_down(P, h.w, f, sw);
+            Combine<VT> none;
+            const VT v = owned_up<VT, OP_NONE, true>(P, h.w, nullptr, none, sw, reinterpret_cast<VT*>(lds_raw + lp.sx));
+            if (threadIdx.x < nu) put<XT, MODE>(x, x0, XT(v));
+            return;
+        }
         block_down(P, h.w, f, lds_raw, lp);
         Combine<VT> none;
         block_up<VT, OP_NONE, true>(P, h.w, nullptr, none, lds_raw, lp);

@@ -123,9 +123,11 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
     P.nodes = total;
     P.qrow.resize(total);
     P.out.resize(total);
+    P.Ldep.resize(total);
     P.Lptr.assign(total + 1, 0);
     P.uloc.resize(M);
     P.uxidx.resize(M);
+    P.Udep.resize(M);
     P.Udiag.resize(M);
     P.Uptr.assign(M + 1, 0);
     std::vector<uint32_t> local(K, kNoNode);
@@ -146,6 +148,7 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
         B[6] = uint32_t(P.Lloc.size());
         for (uint32_t k = 0; k < lo.size(); k++) {
             const uint32_t i = lo[k];
+            P.Ldep[base + k] = uint32_t(P.lvlL.size()) - B[2];
             if (k + 1 == lo.size() || dL[lo[k + 1]] != dL[i]) P.lvlL.push_back(k + 1);
             P.qrow[base + k] = lv.map[i];
             P.out[base + k] = i >= M ? (kGlobalRef | (i - M)) : k;
@@ -163,6 +166,7 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
         B[11] = uint32_t(uo.size());
         for (uint32_t k = 0; k < uo.size(); k++) {
             const uint32_t i = uo[k];
+            P.Udep[upos + k] = uint32_t(P.lvlU.size()) - B[4];
             if (k + 1 == uo.size() || dU[uo[k + 1]] != dU[i]) P.lvlU.push_back(k + 1);
             P.uloc[upos + k] = local[i];
             P.uxidx[upos + k] = lv.map[i];

@@ -59,9 +59,11 @@ struct LevelProgram {
     std::vector<uint32_t> Lptr;          // nodes+1 (global entry offsets)
     std::vector<uint32_t> Lloc;          // local position of the L column within the block
     std::vector<double> Lval;
+    std::vector<uint32_t> Ldep;          // depth level of the node within its block (0-based)
     // U order (eliminated rows only, size M)
     std::vector<uint32_t> uloc;          // local position (L order) of the row
     std::vector<uint32_t> uxidx;         // map[i]: where the row writes the level output
+    std::vector<uint32_t> Udep;          // U depth level of the row within its block (0-based)
     std::vector<uint32_t> Uptr;          // M+1
     std::vector<uint32_t> Ucol;          // local position, or kGlobalRef | (j - M) for kept columns
     std::vector<double> Uval, Udiag;
