@@ -47,6 +47,9 @@ extern "C" {
 #define MLP_PREC_MULTI_SPLIT 2  /* MultiSplitPrec            modules/prec/multi_split/multi_split_prec.h:15 */
 #define MLP_PREC_DIAG        3  /* DiagPrec                  modules/prec/diag/diag_prec.cc:7-23 */
 #define MLP_PREC_ID          4  /* IdPrec                    modules/prec/id/id_prec.h:6-19 */
+#define MLP_PREC_GS          5  /* GSPrec                    modules/prec/gs/gs_prec.cc:9-65 */
+#define MLP_PREC_COMPOSITE   6  /* CompositePrec             modules/prec/composite/composite_prec.cc:9-36 */
+#define MLP_PREC_SYM         7  /* SymPrec ("sym_split")     modules/prec/sym/sym_prec.cc:8-38 */
 
 /* ---- multi_split elimination orders (ConfigFile key elim_order) ---- */
 #define MLP_ORDER_NONE     0
@@ -153,6 +156,17 @@ int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_des
                     mlp_mat A0, int precision, mlp_prec* out);
 int mlp_prec_from_hier(mlp_ctx ctx, mlp_hier h, mlp_mat A0, int precision, mlp_prec* out);
 int mlp_prec_simple(mlp_ctx ctx, int kind, mlp_mat A0, mlp_prec* out);   /* DIAG or ID */
+/* GSPrec(const SkylineMatrix& A)  gs_prec.cc:26-65: one forward Gauss-Seidel sweep (lower triangle
+ * with the diagonal; a non-positive diagonal is an error as in the reference). */
+int mlp_prec_gs(mlp_ctx ctx, mlp_mat A, mlp_prec* out);
+/* CompositePrec(A, cfg)  composite_prec.cc:18-36: GSPrec(A) followed by MultiSplitPrec(A, cfg); ms is the
+ * multi_split hierarchy built on A (mlp_hier_build_multi_split with the same matrix). */
+int mlp_prec_composite(mlp_ctx ctx, mlp_mat A, mlp_hier ms, mlp_prec* out);
+/* SymPrec(A, cfg)  sym_prec.cc:17-38: uh_cheb (sigmas, niters, use_tails as for mlp_hier_build_uh_cheb)
+ * built on max(A, A^T) off the diagonal; each apply is a PCG solve of that matrix to 1e-14 from
+ * generate_x0 (sym_prec.cc:8-15).  Its apply synchronises with the host (no graph capture). */
+int mlp_prec_sym(mlp_ctx ctx, mlp_mat A, const double* sigmas, uint32_t nsigmas, const uint32_t* niters,
+                 uint32_t nniters, int use_tails, mlp_prec* out);
 int mlp_prec_destroy(mlp_prec p);
 /* x = B f: PrecBase::solve(f, x)   modules/prec/prec_base.h:8-12,
  * Prec::solve cheb_solve.cc:9-123, MultiSplitPrec::solve multi_split_solve.cc:59-157 */

@@ -89,6 +89,7 @@ void upload_sell(DevMem& mem, hipStream_t s, uint32_t n, const uint32_t* ia, con
 struct mlp_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool owns_stream = true;           // false for a sub-context on its parent's stream
     mlp::DevMem mem;
     double* partials = nullptr;        // kPartials doubles
     unsigned* counters = nullptr;      // 8
@@ -545,6 +546,89 @@ struct SimplePrec : DevPrecBase {
     uint32_t launches_per_apply() const override { return 1; }
 };
 
+// GSPrec (modules/prec/gs/gs_prec.cc:9-65): one forward Gauss-Seidel sweep over
+// the lower triangle of A.  The rows are grouped into wavefront levels (a row's
+// level is one more than the largest level of its lower columns) and each level
+// is one launch; rows keep the reference's entry order.
+struct DevGS : DevPrecBase {
+    mlp_ctx ctx;
+    uint32_t n = 0, nnzl = 0;
+    DevMem mem;
+    uint32_t *rows = nullptr, *ptr = nullptr, *col = nullptr;
+    double *val = nullptr, *diag = nullptr;
+    std::vector<uint32_t> lvl;             // level k: level-order rows [lvl[k], lvl[k+1])
+
+    void build(const std::vector<uint32_t>& ia, const std::vector<uint32_t>& ja, const std::vector<double>& a) {
+        n = uint32_t(ia.size()) - 1;
+        std::vector<uint32_t> lev(n, 0), lo_end(n);
+        uint32_t nlev = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            if (!(a[ia[i]] > 0))
+                throw Error("Something is very strange: (" + std::to_string(i) + ") has " + std::to_string(a[ia[i]]) +
+                                " on diagonal, aborting...",
+                            MLP_ERR_INVALID);
+            uint32_t e = ia[i] + 1;
+            while (e < ia[i + 1] && ja[e] < i) {       // the Skyline row is sorted after its diagonal
+                lev[i] = std::max(lev[i], lev[ja[e]] + 1);
+                e++;
+            }
+            lo_end[i] = e;
+            nlev = std::max(nlev, lev[i] + 1);
+        }
+        lvl.assign(nlev + 1, 0);
+        for (uint32_t i = 0; i < n; i++) lvl[lev[i] + 1]++;
+        for (uint32_t k = 0; k < nlev; k++) lvl[k + 1] += lvl[k];
+        std::vector<uint32_t> order(n), fill(lvl.begin(), lvl.end() - 1);
+        for (uint32_t i = 0; i < n; i++) order[fill[lev[i]]++] = i;
+        std::vector<uint32_t> hptr(n + 1, 0), hcol;
+        std::vector<double> hval, hdiag(n);
+        for (uint32_t k = 0; k < n; k++) {
+            const uint32_t i = order[k];
+            hdiag[k] = a[ia[i]];
+            for (uint32_t e = ia[i] + 1; e < lo_end[i]; e++) {
+                hcol.push_back(ja[e]);
+                hval.push_back(a[e]);
+            }
+            hptr[k + 1] = uint32_t(hcol.size());
+        }
+        nnzl = uint32_t(hcol.size());
+        if (hcol.empty()) { hcol.push_back(0); hval.push_back(0.0); }
+        hipStream_t s = ctx->stream;
+        rows = mem.upload(order, s);
+        ptr = mem.upload(hptr, s);
+        col = mem.upload(hcol, s);
+        val = mem.upload(hval, s);
+        diag = mem.upload(hdiag, s);
+        MLP_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    void apply(const double* f, double* x, const uint32_t* guard) override {
+        for (size_t k = 0; k + 1 < lvl.size(); k++) {
+            const uint32_t r0 = lvl[k], r1 = lvl[k + 1];
+            hipLaunchKernelGGL(k_gs_level, dim3(cdiv(r1 - r0, kBS)), dim3(kBS), 0, ctx->stream, r0, r1,
+                               (const uint32_t*)rows, (const uint32_t*)ptr, (const uint32_t*)col, (const double*)val,
+                               (const double*)diag, f, x, guard);
+        }
+    }
+    double bytes_per_apply() const override { return 12.0 * nnzl + 28.0 * n; }
+    uint32_t launches_per_apply() const override { return uint32_t(lvl.size()) - 1; }
+};
+
+// CompositePrec (modules/prec/composite/composite_prec.cc:9-36): x = B2(B1(f)).
+struct DevComposite : DevPrecBase {
+    std::unique_ptr<DevPrecBase> first, second;
+    DevMem mem;
+    double* tmp = nullptr;
+    void apply(const double* f, double* x, const uint32_t* guard) override {
+        first->apply(f, tmp, guard);
+        second->apply(tmp, x, guard);
+    }
+    double bytes_per_apply() const override { return first->bytes_per_apply() + second->bytes_per_apply(); }
+    uint32_t launches_per_apply() const override {
+        return first->launches_per_apply() + second->launches_per_apply();
+    }
+    bool graph_safe() const override { return first->graph_safe() && second->graph_safe(); }
+};
+
 }  // namespace
 }  // namespace mlp
 
@@ -555,6 +639,125 @@ struct mlp_prec_s {
     uint32_t n = 0;
     std::unique_ptr<mlp::DevPrecBase> impl;
 };
+
+namespace mlp {
+namespace {
+// scratch of a context: reduction partials, counters, scalars, solve state, pinned staging
+void context_setup(mlp_ctx c) {
+    c->partials = c->mem.alloc<double>(kPartials);
+    c->counters = c->mem.alloc<unsigned>(8);
+    c->scal = c->mem.alloc<double>(4);
+    c->st = c->mem.alloc<SolveState>(1);
+    MLP_HIP_CHECK(hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned), c->stream));
+    MLP_HIP_CHECK(hipHostMalloc((void**)&c->host_pinned, 64 * sizeof(double), hipHostMallocDefault));
+    MLP_HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+void context_release(mlp_ctx ctx) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->drop_graphs(0);
+    ctx->ws_mem.release();
+    ctx->mem.release();
+    if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+    ctx->host_pinned = nullptr;
+    if (ctx->owns_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    ctx->stream = nullptr;
+}
+
+// A context with its own workspace and device state that runs on `parent`'s
+// stream (used by preconditioners that run a solver inside their apply).
+std::unique_ptr<mlp_ctx_s, void (*)(mlp_ctx)> make_sub_context(mlp_ctx parent) {
+    std::unique_ptr<mlp_ctx_s, void (*)(mlp_ctx)> c(new mlp_ctx_s(), [](mlp_ctx p) {
+        context_release(p);
+        delete p;
+    });
+    c->device = parent->device;
+    c->stream = parent->stream;
+    c->owns_stream = false;
+    context_setup(c.get());
+    return c;
+}
+}  // namespace
+}  // namespace mlp
+
+namespace mlp {
+namespace {
+// SymPrec (modules/prec/sym/sym_prec.cc:8-38): Asym = A with each off-diagonal
+// a_ij raised to a_ji when that is larger (max(A, A^T) off the diagonal), the
+// uh_cheb hierarchy built on Asym, and an apply that is a PCG solve of Asym to
+// 1e-14 from generate_x0.  The inner solve runs on a sub-context that shares
+// the caller's stream; it synchronises with the host, so the apply is not
+// captured in graphs.
+struct DevSym : DevPrecBase {
+    mlp_ctx parent;
+    std::unique_ptr<mlp_ctx_s, void (*)(mlp_ctx)> sub;
+    mlp_mat Asym = nullptr;
+    mlp_prec B = nullptr;
+    DevMem mem;
+    double* x0 = nullptr;                  // generate_x0's vector, made once
+    uint32_t n = 0;
+    explicit DevSym(mlp_ctx c) : parent(c), sub(make_sub_context(c)) {}
+    ~DevSym() override {
+        if (B) mlp_prec_destroy(B);
+        if (Asym) mlp_mat_destroy(Asym);
+    }
+    void build(mlp_mat A, const double* sigmas, uint32_t nsig, const uint32_t* niters, uint32_t nnit, int use_tails) {
+        n = A->n;
+        std::vector<double> a = A->a;
+        const std::vector<uint32_t>&ia = A->ia, &ja = A->ja;
+        auto at = [&](uint32_t r, uint32_t c) -> double {      // Asym(r, c) with the updates made so far
+            if (ja[ia[r]] == c) return a[ia[r]];
+            auto b = ja.begin() + ia[r] + 1, e = ja.begin() + ia[r + 1];
+            auto it = std::lower_bound(b, e, c);
+            return (it != e && *it == c) ? a[size_t(it - ja.begin())] : 0.0;
+        };
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t j = ia[i] + 1; j < ia[i + 1]; j++) {
+                const double d = at(ja[j], i);
+                if (d > a[j]) a[j] = d;
+            }
+        int rc = mlp_csr_upload(sub.get(), n, ia.data(), ja.data(), a.data(), &Asym);
+        if (rc) throw Error(mlp_last_error(), rc);
+        mlp_hier h = nullptr;
+        rc = mlp_hier_build_uh_cheb(n, ia.data(), ja.data(), a.data(), sigmas, nsig, niters, nnit, use_tails, &h);
+        if (rc) throw Error(mlp_last_error(), rc);
+        std::unique_ptr<mlp_hier_s> hold(h);
+        rc = mlp_prec_from_hier(sub.get(), h, Asym, MLP_FP64, &B);
+        if (rc) throw Error(mlp_last_error(), rc);
+        std::vector<double> hx(n);
+        srandom(3);                                            // generate_x0, modules/solvers/solvers.cc:9-21
+        for (uint32_t i = 0; i < n; i++) hx[i] = 20. * (random() - 0.5 * RAND_MAX) / RAND_MAX + 100;
+        x0 = mem.upload(hx, parent->stream);
+        MLP_HIP_CHECK(hipStreamSynchronize(parent->stream));
+    }
+    void apply(const double* f, double* x, const uint32_t* guard) override {
+        if (guard) {                                           // the enclosing solve has already converged
+            MLP_HIP_CHECK(hipMemcpyAsync(sub->host_pinned, guard, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                         parent->stream));
+            MLP_HIP_CHECK(hipStreamSynchronize(parent->stream));
+            uint32_t g = 0;
+            std::memcpy(&g, sub->host_pinned, sizeof(g));
+            if (g) return;
+        }
+        MLP_HIP_CHECK(hipMemcpyAsync(x, x0, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, parent->stream));
+        mlp_solve_opts o;
+        std::memset(&o, 0, sizeof(o));
+        o.eps = 1e-14;                                         // sym_prec.cc:37
+        o.maxit = 100000;
+        o.x0_mode = MLP_X0_GIVEN;
+        o.check_every = 8;
+        o.use_graph = 1;
+        mlp_solve_stats st;
+        const int rc = mlp_pcg(sub.get(), Asym, B, f, x, &o, &st);
+        if (rc) throw Error(mlp_last_error(), rc);
+    }
+    double bytes_per_apply() const override { return 0.0; }
+    uint32_t launches_per_apply() const override { return 0; }
+    bool graph_safe() const override { return false; }
+};
+}  // namespace
+}  // namespace mlp
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -589,30 +792,21 @@ int mlp_ctx_create(int device, mlp_ctx* out) {
     MLP_HIP_CHECK(hipGetDeviceCount(&ndev));
     require(device >= 0 && device < ndev, "no such HIP device");
     MLP_HIP_CHECK(hipSetDevice(device));
-    auto* c = new mlp_ctx_s();
+    std::unique_ptr<mlp_ctx_s, void (*)(mlp_ctx)> c(new mlp_ctx_s(), [](mlp_ctx p) {
+        context_release(p);
+        delete p;
+    });
     c->device = device;
     MLP_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    c->partials = c->mem.alloc<double>(kPartials);
-    c->counters = c->mem.alloc<unsigned>(8);
-    c->scal = c->mem.alloc<double>(4);
-    c->st = c->mem.alloc<SolveState>(1);
-    MLP_HIP_CHECK(hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned), c->stream));
-    MLP_HIP_CHECK(hipHostMalloc((void**)&c->host_pinned, 64 * sizeof(double), hipHostMallocDefault));
-    MLP_HIP_CHECK(hipStreamSynchronize(c->stream));
-    *out = c;
+    context_setup(c.get());
+    *out = c.release();
     MLP_API_END
 }
 
 int mlp_ctx_destroy(mlp_ctx ctx) {
     MLP_API_BEGIN
     if (!ctx) return MLP_OK;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    ctx->drop_graphs(0);
-    ctx->ws_mem.release();
-    ctx->mem.release();
-    if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
-    (void)hipStreamDestroy(ctx->stream);
+    context_release(ctx);
     delete ctx;
     MLP_API_END
 }
@@ -983,6 +1177,66 @@ int mlp_prec_simple(mlp_ctx ctx, int kind, mlp_mat A0, mlp_prec* out) {
     p->kind = kind;
     p->n = A0->n;
     p->impl = std::move(sp);
+    *out = p.release();
+    MLP_API_END
+}
+
+int mlp_prec_gs(mlp_ctx ctx, mlp_mat A, mlp_prec* out) {
+    MLP_API_BEGIN
+    require(ctx && A && out, "null argument");
+    auto g = std::make_unique<DevGS>();
+    g->ctx = ctx;
+    g->build(A->ia, A->ja, A->a);
+    auto p = std::make_unique<mlp_prec_s>();
+    p->ctx = ctx;
+    p->id = ctx->next_id++;
+    p->kind = MLP_PREC_GS;
+    p->n = A->n;
+    p->impl = std::move(g);
+    *out = p.release();
+    MLP_API_END
+}
+
+int mlp_prec_composite(mlp_ctx ctx, mlp_mat A, mlp_hier ms, mlp_prec* out) {
+    MLP_API_BEGIN
+    require(ctx && A && ms && out, "null argument");
+    require(ms->H.kind == PrecKind::MULTI_SPLIT, "the second stage must be a multi_split hierarchy");
+    require(ms->A0.nrow == A->n && ms->A0.ia == A->ia && ms->A0.ja == A->ja && ms->A0.a == A->a,
+            "the multi_split hierarchy must be built on A");
+    auto g = std::make_unique<DevGS>();
+    g->ctx = ctx;
+    g->build(A->ia, A->ja, A->a);
+    mlp_prec second = nullptr;
+    int rc = mlp_prec_from_hier(ctx, ms, A, MLP_FP64, &second);
+    if (rc) return rc;
+    std::unique_ptr<mlp_prec_s> holder(second);
+    auto c = std::make_unique<DevComposite>();
+    c->first = std::move(g);
+    c->second = std::move(holder->impl);
+    c->tmp = c->mem.alloc<double>(std::max<uint32_t>(A->n, 1));
+    auto p = std::make_unique<mlp_prec_s>();
+    p->ctx = ctx;
+    p->id = ctx->next_id++;
+    p->kind = MLP_PREC_COMPOSITE;
+    p->n = A->n;
+    p->impl = std::move(c);
+    *out = p.release();
+    MLP_API_END
+}
+
+int mlp_prec_sym(mlp_ctx ctx, mlp_mat A, const double* sigmas, uint32_t nsigmas, const uint32_t* niters,
+                 uint32_t nniters, int use_tails, mlp_prec* out) {
+    MLP_API_BEGIN
+    require(ctx && A && out, "null argument");
+    require(sigmas && nsigmas && niters && nniters, "sigmas and niters are required");
+    auto y = std::make_unique<DevSym>(ctx);
+    y->build(A, sigmas, nsigmas, niters, nniters, use_tails);
+    auto p = std::make_unique<mlp_prec_s>();
+    p->ctx = ctx;
+    p->id = ctx->next_id++;
+    p->kind = MLP_PREC_SYM;
+    p->n = A->n;
+    p->impl = std::move(y);
     *out = p.release();
     MLP_API_END
 }

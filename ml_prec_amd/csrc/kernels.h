@@ -860,6 +860,23 @@ __global__ void __launch_bounds__(kBS) k_spe_grad(uint32_t nx, uint32_t ny, uint
     grad[i] = g;
 }
 
+// One level of the forward Gauss-Seidel sweep (GSPrec::solve, gs_prec.cc:9-23):
+// the rows of a level read only x of rows in earlier levels.  Row k of the
+// level order is row rows[k]; its lower entries in ascending column order.
+__global__ void __launch_bounds__(kBS) k_gs_level(uint32_t r0, uint32_t r1, const uint32_t* __restrict__ rows,
+                                                  const uint32_t* __restrict__ ptr, const uint32_t* __restrict__ col,
+                                                  const double* __restrict__ val, const double* __restrict__ diag,
+                                                  const double* __restrict__ f, double* __restrict__ x,
+                                                  const uint32_t* guard) {
+    if (guarded(guard)) return;
+    const uint32_t k = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= r1) return;
+    const uint32_t i = rows[k];
+    double d = f[i];
+    for (uint32_t q = ptr[k]; q < ptr[k + 1]; q++) d -= val[q] * x[col[q]];
+    x[i] = d / diag[k];
+}
+
 // GMRES: the i+1 classical Gram-Schmidt dots (v_{i+1}, v_k), k = 0..i (gmres.cc:80-82)
 __global__ void __launch_bounds__(kBS) k_mdot(uint32_t n, uint32_t nv, const double* __restrict__ V, size_t ld,
                                               const double* __restrict__ w, double* partials) {

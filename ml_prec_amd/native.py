@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmlp.so")
 
 MLP_FP64, MLP_FP32 = 0, 1
-UH_CHEB, MULTI_SPLIT, DIAG, ID = 1, 2, 3, 4
+UH_CHEB, MULTI_SPLIT, DIAG, ID, GS, COMPOSITE, SYM = 1, 2, 3, 4, 5, 6, 7
 X0_ZERO, X0_GIVEN, X0_REFERENCE = 0, 1, 2
 ORDERS = {"none": 0, "original": 1, "block": 2, "simple_1": 3}
 STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NOT_SPD", 4: "NO_ITER", 5: "MAXIT", 6: "CONSTRUCT", 7: "CONFIG"}
@@ -96,6 +96,9 @@ EXPORTS = {
     "mlp_prec_upload": (C.c_int, [_vp, C.c_int, C.c_uint32, _vp, _vp, C.c_int, C.POINTER(_vp)]),
     "mlp_prec_from_hier": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(_vp)]),
     "mlp_prec_simple": (C.c_int, [_vp, C.c_int, _vp, C.POINTER(_vp)]),
+    "mlp_prec_gs": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    "mlp_prec_composite": (C.c_int, [_vp, _vp, _vp, C.POINTER(_vp)]),
+    "mlp_prec_sym": (C.c_int, [_vp, _vp, _f64p, C.c_uint32, _u32p, C.c_uint32, C.c_int, C.POINTER(_vp)]),
     "mlp_prec_destroy": (C.c_int, [_vp]),
     "mlp_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mlp_pcg": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
@@ -308,9 +311,19 @@ class DevPrec:
         self.A = A
         self.h = _vp()
         self.hier = hier
-        if hier is not None:
+        if kind == COMPOSITE:                      # GSPrec(A) then the multi_split hierarchy `hier` of A
+            check(lib().mlp_prec_composite(ctx.h, A.h, hier.h, C.byref(self.h)))
+            self.kind = kind
+        elif hier is not None:
             check(lib().mlp_prec_from_hier(ctx.h, hier.h, A.h, precision, C.byref(self.h)))
             self.kind = hier.kind
+        elif kind == GS:
+            check(lib().mlp_prec_gs(ctx.h, A.h, C.byref(self.h)))
+            self.kind = kind
+        elif kind == SYM:                          # SymPrec with the default uh_cheb parameters
+            check(lib().mlp_prec_sym(ctx.h, A.h, _f64(np.array([3.0])), 1, _u32(np.array([2])), 1, 1,
+                                     C.byref(self.h)))
+            self.kind = kind
         else:
             check(lib().mlp_prec_simple(ctx.h, kind, A.h, C.byref(self.h)))
             self.kind = kind

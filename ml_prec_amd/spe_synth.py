@@ -194,6 +194,33 @@ def assemble(kx3, kind="tpfa", c=0.0):
     return ia, ja, a
 
 
+def make_nonsymmetric(ia, ja, a, tau, seed=1):
+    """The reference's nonsymmetric M-matrix (modules/mesh/spe.cc:138-182): row by row, every off-diagonal
+    a_ij is changed by d = random(-tau, tau) * a_ij (a_ij -= d, a_ii += d; include/tools.h:41-44 draws
+    -tau + 2 tau random()/RAND_MAX), so row sums are kept.  The draws come from glibc random() seeded with
+    `seed` (1 is glibc's state at program start, the reference's case).  Returns a new value array."""
+    import ctypes
+    import ctypes.util
+    if not tau:
+        raise ValueError("Nonsymmetricity parameter is 0")
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    libc.random.restype = ctypes.c_long
+    libc.srandom.argtypes = [ctypes.c_uint]
+    rand_max = 2147483647
+    libc.srandom(seed)
+    ia = np.asarray(ia, dtype=np.int64)
+    a = np.array(a, dtype=np.float64)
+    n = ia.shape[0] - 1
+    for i in range(n):
+        dind = ia[i]
+        for j in range(ia[i] + 1, ia[i + 1]):
+            r = -tau + (tau - (-tau)) * (float(libc.random()) / rand_max)
+            d = r * a[j]
+            a[j] -= d
+            a[dind] += d
+    return a
+
+
 def rhs(n, seed):
     rng = np.random.default_rng(seed)
     return rng.uniform(-1.0, 1.0, n)

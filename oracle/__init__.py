@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-UH_CHEB, MULTI_SPLIT, DIAG, ID = 1, 2, 3, 4
+UH_CHEB, MULTI_SPLIT, DIAG, ID, GS, COMPOSITE, SYM = 1, 2, 3, 4, 5, 6, 7
 ORDERS = {"none": 0, "original": 1, "block": 2, "simple_1": 3}
 
 _lib = None
@@ -47,6 +47,14 @@ def lib():
         L.or_diag_build.argtypes = [C.c_uint32, _u32p, _u32p, _f64p]
         L.or_id_build.restype = C.c_void_p
         L.or_id_build.argtypes = [C.c_uint32]
+        L.or_gs_build.restype = C.c_void_p
+        L.or_gs_build.argtypes = [C.c_uint32, _u32p, _u32p, _f64p, C.c_char_p, C.c_size_t]
+        L.or_composite_make.restype = C.c_void_p
+        L.or_composite_make.argtypes = [C.c_void_p, C.c_void_p]
+        L.or_sym_build.restype = C.c_void_p
+        L.or_sym_build.argtypes = [C.c_uint32, _u32p, _u32p, _f64p, _f64p, C.c_uint32, _u32p, C.c_uint32,
+                                   C.c_int, C.c_int, C.c_char_p, C.c_size_t]
+        L.or_sym_matrix.argtypes = [C.c_void_p, _f64p]
         L.or_prec_free.argtypes = [C.c_void_p]
         L.or_prec_apply.argtypes = [C.c_void_p, _f64p, _f64p]
         L.or_prec_nlevels.restype = C.c_uint32
@@ -105,7 +113,8 @@ class Prec:
 
     def apply(self, f):
         x = np.zeros(self.n)
-        lib().or_prec_apply(self.h, _f64(f), x)
+        if lib().or_prec_apply(self.h, _f64(f), x):
+            raise OracleError("preconditioner apply failed")
         return x
 
     @property
@@ -173,6 +182,43 @@ def msplit_prec(ia, ja, a, q=(0.8,), niters=(5,), epss=(1e-2,), inner="fixed", o
     if not h:
         raise OracleError(err.value.decode())
     return Prec(h, MULTI_SPLIT, n)
+
+
+def gs_prec(ia, ja, a):
+    """GSPrec(A): one forward Gauss-Seidel sweep."""
+    n = len(ia) - 1
+    err = C.create_string_buffer(512)
+    h = lib().or_gs_build(n, _u32(ia), _u32(ja), _f64(a), err, 512)
+    if not h:
+        raise OracleError(err.value.decode())
+    return Prec(h, GS, n)
+
+
+def composite_prec(first, second):
+    """CompositePrec: x = second(first(f)); the new object owns both stages."""
+    h = lib().or_composite_make(first.h, second.h)
+    P = Prec(h, COMPOSITE, first.n)
+    first.h = second.h = None              # freed with the composite
+    return P
+
+
+def sym_prec(ia, ja, a, sigmas=(3.0,), niters=(2,), use_tails=True, optimize_storage=True):
+    """SymPrec(A, cfg): uh_cheb on max(A, A^T), applied as PCG to 1e-14."""
+    n = len(ia) - 1
+    err = C.create_string_buffer(512)
+    h = lib().or_sym_build(n, _u32(ia), _u32(ja), _f64(a), _f64(sigmas), len(sigmas), _u32(niters), len(niters),
+                           int(use_tails), int(optimize_storage), err, 512)
+    if not h:
+        raise OracleError(err.value.decode())
+    P = Prec(h, SYM, n)
+    return P
+
+
+def sym_matrix(P, nnz):
+    out = np.zeros(nnz)
+    if lib().or_sym_matrix(P.h, out):
+        raise OracleError("not a SymPrec")
+    return out
 
 
 def diag_prec(ia, ja, a):

@@ -320,6 +320,12 @@ struct or_prec {
     /* diag */
     double* d;
     uint32_t n;
+    /* GS: the lower triangle with its diagonal first (gs_prec.cc:26-65) */
+    or_mat Lgs;
+    /* composite: the two stages; sym: sub[0] is the uh_cheb hierarchy of Asym */
+    or_prec* sub[2];
+    or_mat Asym;
+    double sym_eps;
 };
 
 static void lev_free(lev_t* L) {
@@ -333,6 +339,10 @@ void or_prec_free(or_prec* P) {
     for (uint32_t l = 0; l < P->nalloc; l++) lev_free(&P->lv[l]);
     free(P->lv);
     mat_free(&P->A0);
+    mat_free(&P->Lgs);
+    mat_free(&P->Asym);
+    or_prec_free(P->sub[0]);
+    or_prec_free(P->sub[1]);
     free(P->d);
     free(P);
 }
@@ -990,12 +1000,128 @@ or_prec* or_id_build(uint32_t n) {
     return P;
 }
 
+/* GSPrec(A): L keeps the diagonal, then the entries with column < i (gs_prec.cc:26-65) */
+or_prec* or_gs_build(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, char* err, size_t errlen) {
+    or_prec* P = xcalloc(1, sizeof(or_prec));
+    P->kind = OR_PREC_GS;
+    P->n = n;
+    or_mat* L = &P->Lgs;
+    L->nrow = L->ncol = n;
+    L->ia = xmalloc(((size_t)n + 1) * sizeof(uint32_t));
+    L->ja = xmalloc(((size_t)ia[n] + 1) * sizeof(uint32_t));
+    L->a = xmalloc(((size_t)ia[n] + 1) * sizeof(double));
+    uint32_t ind = 0;
+    L->ia[0] = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        if (!(a[ia[i]] > 0)) {
+            set_err(err, errlen, "Something is very strange: (%u) has %g on diagonal, aborting...", i, a[ia[i]]);
+            or_prec_free(P);
+            return NULL;
+        }
+        L->ja[ind] = i;
+        L->a[ind++] = a[ia[i]];
+        for (uint32_t j = ia[i] + 1; j < ia[i + 1] && ja[j] <= i; j++) {   /* upper_bound(i) over the sorted rest */
+            L->ja[ind] = ja[j];
+            L->a[ind++] = a[j];
+        }
+        L->ia[i + 1] = ind;
+    }
+    return P;
+}
+
+/* GSPrec::solve (gs_prec.cc:9-23) */
+static void gs_solve(const or_prec* P, const double* f, double* x) {
+    const or_mat* L = &P->Lgs;
+    for (uint32_t i = 0; i < P->n; i++) {
+        double d = f[i];
+        for (uint32_t j = L->ia[i] + 1; j < L->ia[i + 1]; j++) d -= L->a[j] * x[L->ja[j]];
+        x[i] = d / L->a[L->ia[i]];
+    }
+}
+
+or_prec* or_composite_make(or_prec* first, or_prec* second) {
+    or_prec* P = xcalloc(1, sizeof(or_prec));
+    P->kind = OR_PREC_COMPOSITE;
+    P->n = first->kind == OR_PREC_GS || first->kind == OR_PREC_DIAG || first->kind == OR_PREC_ID ? first->n
+                                                                                                  : first->A0.nrow;
+    P->sub[0] = first;
+    P->sub[1] = second;
+    return P;
+}
+
+/* element (r, c) of a Skyline matrix (diagonal first, then ascending columns); 0 when absent */
+static double sky_at(const or_mat* A, uint32_t r, uint32_t c) {
+    if (A->ja[A->ia[r]] == c) return A->a[A->ia[r]];
+    uint32_t lo = A->ia[r] + 1, hi = A->ia[r + 1];
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (A->ja[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < A->ia[r + 1] && A->ja[lo] == c) ? A->a[lo] : 0.0;
+}
+
+/* SymPrec(A, cfg) (sym_prec.cc:17-38): Asym = A, then row by row each off-diagonal a_ij is raised to
+ * Asym(j, i) when that is larger (rows before i are already symmetrised), so Asym = max(A, A^T) off
+ * the diagonal; the uh_cheb hierarchy is built on Asym. */
+or_prec* or_sym_build(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
+                      const double* sigmas, uint32_t nsig, const uint32_t* niters, uint32_t nnit,
+                      int use_tails, int optimize_storage, char* err, size_t errlen) {
+    or_prec* P = xcalloc(1, sizeof(or_prec));
+    P->kind = OR_PREC_SYM;
+    P->n = n;
+    or_mat* S = &P->Asym;
+    S->nrow = S->ncol = n;
+    const size_t nnz = ia[n];
+    S->ia = xmalloc(((size_t)n + 1) * sizeof(uint32_t));
+    S->ja = xmalloc((nnz + 1) * sizeof(uint32_t));
+    S->a = xmalloc((nnz + 1) * sizeof(double));
+    memcpy(S->ia, ia, ((size_t)n + 1) * sizeof(uint32_t));
+    memcpy(S->ja, ja, nnz * sizeof(uint32_t));
+    memcpy(S->a, a, nnz * sizeof(double));
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t j = S->ia[i] + 1; j < S->ia[i + 1]; j++) {
+            double d = sky_at(S, S->ja[j], i);
+            if (d > S->a[j]) S->a[j] = d;
+        }
+    P->sub[0] = or_cheb_build(n, S->ia, S->ja, S->a, sigmas, nsig, niters, nnit, use_tails, optimize_storage, err,
+                              errlen);
+    if (!P->sub[0]) {
+        or_prec_free(P);
+        return NULL;
+    }
+    P->sym_eps = 1e-14;
+    return P;
+}
+
+int or_sym_matrix(const or_prec* P, double* out) {
+    if (P->kind != OR_PREC_SYM) return 1;
+    memcpy(out, P->Asym.a, (size_t)P->Asym.ia[P->n] * sizeof(double));
+    return 0;
+}
+
 int or_prec_apply(const or_prec* P, const double* f, double* x) {
     switch (P->kind) {
         case OR_PREC_UH_CHEB: cheb_solve(P, 0, f, x); return 0;
         case OR_PREC_MULTI_SPLIT: ms_solve(P, 0, f, x); return 0;
         case OR_PREC_DIAG: for (uint32_t i = 0; i < P->n; i++) x[i] = f[i] * P->d[i]; return 0;
         case OR_PREC_ID: memcpy(x, f, (size_t)P->n * sizeof(double)); return 0;
+        case OR_PREC_GS: gs_solve(P, f, x); return 0;
+        case OR_PREC_COMPOSITE: {                      /* composite_prec.cc:9-16 */
+            double* x1 = xcalloc((size_t)P->n + 1, sizeof(double));
+            int rc = or_prec_apply(P->sub[0], f, x);
+            if (!rc) {
+                memcpy(x1, x, (size_t)P->n * sizeof(double));
+                rc = or_prec_apply(P->sub[1], x1, x);
+            }
+            free(x1);
+            return rc;
+        }
+        case OR_PREC_SYM: {                            /* sym_prec.cc:8-15: PCGSolver(Asym, f, *B, x, 1e-14) */
+            char e[256];
+            or_stats st;
+            return or_pcg(&P->Asym, f, P->sub[0], x, &st, P->sym_eps, 0, 0, e, sizeof(e));
+        }
     }
     return 1;
 }

@@ -57,7 +57,8 @@ double or_cheb(double x, uint32_t k);
 /* ---- preconditioners ---- */
 typedef struct or_prec or_prec;   /* opaque: uh_cheb or multi_split hierarchy */
 
-enum { OR_PREC_UH_CHEB = 1, OR_PREC_MULTI_SPLIT = 2, OR_PREC_DIAG = 3, OR_PREC_ID = 4 };
+enum { OR_PREC_UH_CHEB = 1, OR_PREC_MULTI_SPLIT = 2, OR_PREC_DIAG = 3, OR_PREC_ID = 4, OR_PREC_GS = 5,
+       OR_PREC_COMPOSITE = 6, OR_PREC_SYM = 7 };
 enum { OR_ORDER_NONE = 0, OR_ORDER_ORIGINAL = 1, OR_ORDER_BLOCK = 2, OR_ORDER_SIMPLE_1 = 3 };
 
 /* uh_cheb Prec(A, cfg) (modules/prec/cheb/cheb_constr.cc:163-248).  Returns NULL and fills
@@ -76,6 +77,18 @@ or_prec* or_msplit_build(uint32_t n, const uint32_t* ia, const uint32_t* ja, con
 
 or_prec* or_diag_build(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a);
 or_prec* or_id_build(uint32_t n);
+/* GSPrec(A) (modules/prec/gs/gs_prec.cc:26-65): forward Gauss-Seidel on the lower triangle. */
+or_prec* or_gs_build(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, char* err, size_t errlen);
+/* CompositePrec (modules/prec/composite/composite_prec.cc:18-36): x = B2(B1(f)); takes ownership of
+ * both (the reference builds B1 = GSPrec(A), B2 = MultiSplitPrec(A, cfg)). */
+or_prec* or_composite_make(or_prec* first, or_prec* second);
+/* SymPrec(A, cfg) (modules/prec/sym/sym_prec.cc:17-38): uh_cheb built on max(A, A^T) off the diagonal;
+ * solve runs PCG on that matrix to 1e-14 from generate_x0. */
+or_prec* or_sym_build(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
+                      const double* sigmas, uint32_t nsig, const uint32_t* niters, uint32_t nnit,
+                      int use_tails, int optimize_storage, char* err, size_t errlen);
+/* the symmetrised matrix of a SymPrec (same pattern as A): copies nnz values into out */
+int or_sym_matrix(const or_prec* P, double* out);
 
 void     or_prec_free(or_prec* P);
 int      or_prec_apply(const or_prec* P, const double* f, double* x);   /* PrecBase::solve */

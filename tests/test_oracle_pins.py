@@ -466,3 +466,61 @@ def test_simple_solver_no_need_to_iterate():
     assert st["rc"] != 0 and "No need to iterate" in st["err"]
     assert st["niter"] == 1
     assert np.array_equal(x, s.b * (1.0 / np.diag(A)))     # DiagPrec multiplies by 1/a_ii (diag_prec.cc)
+
+
+def _nonsym(name="tiny", tau=0.3):
+    s = system(name)
+    return s, S.make_nonsymmetric(s.ia, s.ja, s.a, tau)
+
+
+def test_nonsymmetric_generator_follows_glibc_random():
+    """spe.cc:138-182 with glibc random() seeded 1: its first outputs are the well-known
+    1804289383, 846930886, 1681692777; row sums are kept and every change is within tau |a_ij|."""
+    tau = 0.3
+    s, an = _nonsym(tau=tau)
+    rm = 2147483647
+    first = [1804289383, 846930886, 1681692777]
+    # row 0: its off-diagonals are the first draws, in storage order
+    for k, r in enumerate(first[: int(s.ia[1] - s.ia[0] - 1)]):
+        j = int(s.ia[0]) + 1 + k
+        d = (-tau + (tau - (-tau)) * (float(r) / rm)) * s.a[j]
+        assert an[j] == s.a[j] - d
+    A0, A1 = dense(s), dense(s.ia, s.ja, an)
+    assert np.allclose(A1.sum(1), A0.sum(1), rtol=0, atol=1e-12 * np.abs(A0).max())
+    off = ~np.eye(s.n, dtype=bool)
+    assert np.all(np.abs(A1 - A0)[off] <= tau * np.abs(A0)[off] * (1 + 1e-12))
+    assert np.abs(A1 - A1.T).max() > 0
+
+
+def test_gs_is_forward_triangular_solve():
+    from scipy.linalg import solve_triangular
+    s, an = _nonsym()
+    A = dense(s.ia, s.ja, an)
+    f = rng(51).standard_normal(s.n)
+    x = O.gs_prec(s.ia, s.ja, an).apply(f)
+    ref = solve_triangular(np.tril(A), f, lower=True)
+    assert np.allclose(x, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+
+
+def test_sym_prec_matrix_and_solve():
+    """SymPrec: Asym = max(A, A^T) off the diagonal (diagonal kept), and the apply is a PCG solve of
+    Asym to 1e-14 (checked against a dense solve)."""
+    s, an = _nonsym("tiny_tpfa_c1")
+    A = dense(s.ia, s.ja, an)
+    P = O.sym_prec(s.ia, s.ja, an)
+    As = dense(s.ia, s.ja, O.sym_matrix(P, len(an)))
+    expect = np.maximum(A, A.T)
+    np.fill_diagonal(expect, np.diag(A))
+    assert np.array_equal(As, expect)
+    f = rng(52).standard_normal(s.n)
+    y = P.apply(f)
+    ref = np.linalg.solve(As, f)
+    assert np.linalg.norm(y - ref) <= 1e-10 * np.linalg.norm(ref)
+
+
+def test_composite_is_gs_then_multi_split():
+    s, an = _nonsym()
+    f = rng(53).standard_normal(s.n)
+    Cp = O.composite_prec(O.gs_prec(s.ia, s.ja, an), O.msplit_prec(s.ia, s.ja, an))
+    G, M = O.gs_prec(s.ia, s.ja, an), O.msplit_prec(s.ia, s.ja, an)
+    assert np.array_equal(Cp.apply(f), M.apply(G.apply(f)))
