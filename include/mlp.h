@@ -167,6 +167,8 @@ int mlp_prec_composite(mlp_ctx ctx, mlp_mat A, mlp_hier ms, mlp_prec* out);
  * generate_x0 (sym_prec.cc:8-15).  Its apply synchronises with the host (no graph capture). */
 int mlp_prec_sym(mlp_ctx ctx, mlp_mat A, const double* sigmas, uint32_t nsigmas, const uint32_t* niters,
                  uint32_t nniters, int use_tails, mlp_prec* out);
+/* SymPrec::set_eps  sym_prec.h:26-28: tolerance of the inner PCG (default 1e-14) */
+int mlp_prec_sym_set_eps(mlp_prec p, double eps);
 int mlp_prec_destroy(mlp_prec p);
 /* x = B f: PrecBase::solve(f, x)   modules/prec/prec_base.h:8-12,
  * Prec::solve cheb_solve.cc:9-123, MultiSplitPrec::solve multi_split_solve.cc:59-157 */

@@ -45,6 +45,9 @@ int main(int argc, char** argv) {
         else if (prec == "multi_split") B = new MultiSplitPrec(A, cfg);
         else if (prec == "diag") B = new DiagPrec(A);
         else if (prec == "id") B = new IdPrec(A);
+        else if (prec == "gs") B = new GSPrec(A);
+        else if (prec == "comp") B = new CompositePrec(A, cfg);
+        else if (prec == "sym_split") B = new SymPrec(A, cfg);
         else THROW_EXCEPTION("Unknown prec type \"" << prec << "\"");
 
         Vector x(A.size());
@@ -55,7 +58,7 @@ int main(int argc, char** argv) {
         else if (solver == "cheb") {
             // ChebSolver needs the spectral bounds of the Chebyshev preconditioner (project/main.cc:142-149)
             Prec& Bcheb = dynamic_cast<Prec&>(*B);
-            ChebSolver(A, Bcheb.lmin(), Bcheb.lmax(), b, Bcheb, x, eps, NORM_L2, false);
+            ChebSolver(A, Bcheb.lmin(), Bcheb.lmax(), b, Bcheb, x, eps, NORM_L2, true);
         } else THROW_EXCEPTION("Unknown solver type \"" << solver << "\"");
 
         Vector z(A.size());

@@ -257,7 +257,8 @@ Prec::~Prec() {
     if (hier_) mlp_hier_destroy(hier_);
 }
 
-MultiSplitPrec::MultiSplitPrec(const SkylineMatrix& A, const Config& cfg) : level0_A(A) {
+// MultiSplitPrec's hierarchy from a Config (with its prec config file, if any)
+static mlp_hier build_msplit(const SkylineMatrix& A, const Config& cfg) {
     // without a config file the command-line values apply (multi_split_constr.cc:234-244)
     mlp_msplit_params p;
     std::memset(&p, 0, sizeof(p));
@@ -275,8 +276,14 @@ MultiSplitPrec::MultiSplitPrec(const SkylineMatrix& A, const Config& cfg) : leve
     p.max_levels = cfg.max_levels;
     p.coarse_n = cfg.coarse_n;
     const char* path = cfg.prec_conf_file.empty() ? nullptr : cfg.prec_conf_file.c_str();
+    mlp_hier h = nullptr;
     check(mlp_hier_build_multi_split(A.size(), A.get_ia().data(), A.get_ja().data(), A.get_a().data(), path, &p,
-                                     &hier_));
+                                     &h));
+    return h;
+}
+
+MultiSplitPrec::MultiSplitPrec(const SkylineMatrix& A, const Config& cfg) : level0_A(A) {
+    hier_ = build_msplit(A, cfg);
     check(mlp_hier_nlevels(hier_, &nlevels_));
     check(mlp_prec_from_hier(context(), hier_, A.device(), cfg.gpu_precision, &prec_));
     precision_ = cfg.gpu_precision;
@@ -291,6 +298,26 @@ DiagPrec::DiagPrec(const SkylineMatrix& A) { check(mlp_prec_simple(context(), ML
 DiagPrec::~DiagPrec() { if (prec_) mlp_prec_destroy(prec_); }
 IdPrec::IdPrec(const SkylineMatrix& A) { check(mlp_prec_simple(context(), MLP_PREC_ID, A.device(), &prec_)); }
 IdPrec::~IdPrec() { if (prec_) mlp_prec_destroy(prec_); }
+
+GSPrec::GSPrec(const SkylineMatrix& A) { check(mlp_prec_gs(context(), A.device(), &prec_)); }
+GSPrec::~GSPrec() { if (prec_) mlp_prec_destroy(prec_); }
+
+CompositePrec::CompositePrec(const SkylineMatrix& A, const Config& cfg) {
+    hier_ = build_msplit(A, cfg);
+    check(mlp_prec_composite(context(), A.device(), hier_, &prec_));
+}
+CompositePrec::~CompositePrec() {
+    if (prec_) mlp_prec_destroy(prec_);
+    if (hier_) mlp_hier_destroy(hier_);
+}
+
+SymPrec::SymPrec(const SkylineMatrix& A, const Config& cfg) {
+    // the inner preconditioner is Prec(Asym, cfg) with nonsym_matrix = false (sym_prec.cc:33-36)
+    check(mlp_prec_sym(context(), A.device(), cfg.sigmas.data(), uint32_t(cfg.sigmas.size()), cfg.niters.data(),
+                       uint32_t(cfg.niters.size()), cfg.use_tails, &prec_));
+}
+SymPrec::~SymPrec() { if (prec_) mlp_prec_destroy(prec_); }
+void SymPrec::set_eps(double eps) { check(mlp_prec_sym_set_eps(prec_, eps)); }
 
 // ----------------------------------------------------------------- solvers --
 std::ostream& operator<<(std::ostream& os, const SolverStats& stats) {

@@ -11,6 +11,7 @@
 //   Prec(A, cfg)  ("uh_cheb")                  modules/prec/cheb/cheb_prec.h:129
 //   MultiSplitPrec(A, cfg)                     modules/prec/multi_split/multi_split_prec.h:105
 //   DiagPrec / IdPrec                          modules/prec/diag/diag_prec.h, id/id_prec.h
+//   GSPrec / CompositePrec / SymPrec           modules/prec/gs, composite, sym
 //   Config                                     project/config.h:54-86
 //   SolverStats, PCG/GMRES/Cheb/SimpleSolver   modules/solvers/solvers.h:15-38
 //   ConfigFile                                 modules/config_file/config_file.h:54-116
@@ -228,6 +229,35 @@ class IdPrec : public PrecBase {
 public:
     explicit IdPrec(const SkylineMatrix& A);
     ~IdPrec() override;
+    mlp_prec device() const override { return prec_; }
+private:
+    mlp_prec prec_ = nullptr;
+};
+
+class GSPrec : public PrecBase {                     // modules/prec/gs/gs_prec.h
+public:
+    explicit GSPrec(const SkylineMatrix& A);
+    ~GSPrec() override;
+    mlp_prec device() const override { return prec_; }
+private:
+    mlp_prec prec_ = nullptr;
+};
+
+class CompositePrec : public PrecBase {              // GSPrec then MultiSplitPrec, composite_prec.h
+public:
+    CompositePrec(const SkylineMatrix& A, const Config& cfg);
+    ~CompositePrec() override;
+    mlp_prec device() const override { return prec_; }
+private:
+    mlp_hier hier_ = nullptr;
+    mlp_prec prec_ = nullptr;
+};
+
+class SymPrec : public PrecBase {                    // "sym_split", sym_prec.h
+public:
+    SymPrec(const SkylineMatrix& A, const Config& cfg);
+    ~SymPrec() override;
+    void set_eps(double eps);
     mlp_prec device() const override { return prec_; }
 private:
     mlp_prec prec_ = nullptr;

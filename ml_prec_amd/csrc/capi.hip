@@ -697,6 +697,7 @@ struct DevSym : DevPrecBase {
     DevMem mem;
     double* x0 = nullptr;                  // generate_x0's vector, made once
     uint32_t n = 0;
+    double eps = 1e-14;                    // sym_prec.cc:37, SymPrec::set_eps (sym_prec.h:26-28)
     explicit DevSym(mlp_ctx c) : parent(c), sub(make_sub_context(c)) {}
     ~DevSym() override {
         if (B) mlp_prec_destroy(B);
@@ -743,7 +744,7 @@ struct DevSym : DevPrecBase {
         MLP_HIP_CHECK(hipMemcpyAsync(x, x0, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, parent->stream));
         mlp_solve_opts o;
         std::memset(&o, 0, sizeof(o));
-        o.eps = 1e-14;                                         // sym_prec.cc:37
+        o.eps = eps;
         o.maxit = 100000;
         o.x0_mode = MLP_X0_GIVEN;
         o.check_every = 8;
@@ -1238,6 +1239,14 @@ int mlp_prec_sym(mlp_ctx ctx, mlp_mat A, const double* sigmas, uint32_t nsigmas,
     p->n = A->n;
     p->impl = std::move(y);
     *out = p.release();
+    MLP_API_END
+}
+
+int mlp_prec_sym_set_eps(mlp_prec p, double eps) {
+    MLP_API_BEGIN
+    require(p && p->kind == MLP_PREC_SYM, "not a SymPrec");
+    require(eps > 0, "eps must be positive");
+    static_cast<DevSym*>(p->impl.get())->eps = eps;
     MLP_API_END
 }
 

@@ -98,6 +98,7 @@ EXPORTS = {
     "mlp_prec_simple": (C.c_int, [_vp, C.c_int, _vp, C.POINTER(_vp)]),
     "mlp_prec_gs": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
     "mlp_prec_composite": (C.c_int, [_vp, _vp, _vp, C.POINTER(_vp)]),
+    "mlp_prec_sym_set_eps": (C.c_int, [_vp, C.c_double]),
     "mlp_prec_sym": (C.c_int, [_vp, _vp, _f64p, C.c_uint32, _u32p, C.c_uint32, C.c_int, C.POINTER(_vp)]),
     "mlp_prec_destroy": (C.c_int, [_vp]),
     "mlp_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp]),

@@ -79,3 +79,48 @@ def test_dropin_mixed_precision(tmp_path):
     P = O.cheb_prec(s.ia, s.ja, s.a)
     zo = P.apply(s.b)
     assert np.linalg.norm(z - zo) <= 1e-5 * np.linalg.norm(zo)
+
+
+MSPLIT_CFG = ("q = 0.8\ninner_iter = fixed\nlevel_niter = 5\nelim_order = simple_1\n"
+              "elim_stop = degree\ndegree_max = 2\ncoarse_n = 0\nmax_levels = 15\n")
+MSPLIT_KW = dict(q=(0.8,), niters=(5,), order="simple_1", degree_max=2)
+
+
+def test_dropin_cheb_and_simple_solvers(tmp_path):
+    s = system("spe10_z12")
+    _, _, x, _ = run_demo(tmp_path, s, "uh_cheb", "cheb", 1e-8)
+    P = O.cheb_prec(s.ia, s.ja, s.a)
+    li = P.level_info(0)
+    xo, so = O.cheb_solver(s.ia, s.ja, s.a, s.b, P, li["lmin"], li["lmax"], eps=1e-8, x0=None, maxit=20000)
+    assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
+    conf = tmp_path / "msplit.cfg"
+    conf.write_text(MSPLIT_CFG)
+    niter, _, x, _ = run_demo(tmp_path, s, "multi_split", "simple", 1e-8, str(conf))
+    P = O.msplit_prec(s.ia, s.ja, s.a, **MSPLIT_KW)
+    xo, so = O.simple_solver(s.ia, s.ja, s.a, s.b, P, eps=1e-8, x0=None, maxit=5000)
+    assert abs(niter - so["niter"]) <= 1
+    assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("prec", ["comp", "gs", "sym_split"])
+def test_dropin_gmres_nonsymmetric(tmp_path, prec):
+    s0 = system("cfg0_spe")
+    a = S.make_nonsymmetric(s0.ia, s0.ja, s0.a, 0.3)
+    s = S.System(**{**s0.__dict__, "a": a})
+    conf = tmp_path / "msplit.cfg"
+    conf.write_text(MSPLIT_CFG)
+    niter, _, x, z = run_demo(tmp_path, s, prec, "gmres", 1e-8, str(conf))
+    if prec == "comp":
+        P = O.composite_prec(O.gs_prec(s.ia, s.ja, a), O.msplit_prec(s.ia, s.ja, a, **MSPLIT_KW))
+    elif prec == "gs":
+        P = O.gs_prec(s.ia, s.ja, a)
+    else:
+        P = O.sym_prec(s.ia, s.ja, a)
+    xo, so = O.gmres(s.ia, s.ja, a, s.b, P, eps=1e-8, x0=None, maxit=3000)
+    assert abs(niter - so["niter"]) <= 1
+    assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
+    zo = P.apply(s.b)
+    if prec == "sym_split":
+        assert np.linalg.norm(z - zo) <= 1e-10 * np.linalg.norm(zo)
+    else:
+        assert np.array_equal(z, zo)
