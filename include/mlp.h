@@ -167,6 +167,11 @@ int mlp_prec_composite(mlp_ctx ctx, mlp_mat A, mlp_hier ms, mlp_prec* out);
  * generate_x0 (sym_prec.cc:8-15).  Its apply synchronises with the host (no graph capture). */
 int mlp_prec_sym(mlp_ctx ctx, mlp_mat A, const double* sigmas, uint32_t nsigmas, const uint32_t* niters,
                  uint32_t nniters, int use_tails, mlp_prec* out);
+/* A copy of a uh_cheb or multi_split preconditioner on context ctx (same device) that shares src's device data
+ * (level programs and matrices, which stay alive while any copy does) and has level vectors of its own, so that
+ * solves on different contexts can apply one hierarchy at the same time (no counterpart in the reference, whose
+ * solve() mutates per-level scratch and is not reentrant, cheb_prec.h:35-36). */
+int mlp_prec_clone(mlp_ctx ctx, mlp_prec src, mlp_prec* out);
 /* SymPrec::set_eps  sym_prec.h:26-28: tolerance of the inner PCG (default 1e-14) */
 int mlp_prec_sym_set_eps(mlp_prec p, double eps);
 int mlp_prec_destroy(mlp_prec p);

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <thread>
 #include <cmath>
@@ -34,6 +35,9 @@ inline bool host_is_one(double a) { return std::fabs(a - 1.0) <= 2.2204460492503
 // ---------------------------------------------------------------------------
 struct DevMem {
     std::vector<void*> ptrs;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
     ~DevMem() { release(); }
     void release() {
         for (void* p : ptrs) (void)hipFree(p);
@@ -59,6 +63,12 @@ struct DevMem {
         return upload(c, s);
     }
 };
+
+// handle ids key the graph cache; unique across contexts, never reused
+inline uint64_t new_handle_id() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1);
+}
 
 template <class VT> struct DevSell {
     SellView<VT> v;
@@ -101,7 +111,6 @@ struct mlp_ctx_s {
     mlp::DevMem ws_mem;
     double *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *zp = nullptr, *V = nullptr;
     double* mdot = nullptr;            // 32 results of the Gram-Schmidt dots
-    uint64_t next_id = 1;
     // instantiated PCG iteration graphs, reused while the same matrix,
     // preconditioner and vectors are solved again
     struct GraphEntry {
@@ -181,6 +190,8 @@ struct DevPrecBase {
     virtual double bytes_per_apply() const = 0;
     virtual uint32_t launches_per_apply() const = 0;
     virtual bool graph_safe() const { return true; }
+    // a copy on another context that shares the device data (nullptr: not supported)
+    virtual std::unique_ptr<DevPrecBase> clone_on(mlp_ctx) const { return nullptr; }
 };
 
 constexpr uint32_t kLdsBudget = 60 * 1024;
@@ -190,7 +201,8 @@ template <class VT> struct DevHier : DevPrecBase {
     int kind;
     uint32_t nlevels = 0;
     std::vector<DevLevel<VT>> lv;
-    DevMem mem;
+    std::shared_ptr<DevMem> data = std::make_shared<DevMem>();   // programs, matrices: shared by clones
+    DevMem scratch;                                              // per-instance level vectors
     mlp_mat A0 = nullptr;
     mutable uint32_t launches = 0;
     bool dynamic = false;
@@ -258,7 +270,7 @@ template <class VT> struct DevHier : DevPrecBase {
             L.n = L.N - L.M - L.Md;
             if (L.eps != 0.0) dynamic = true;
             std::vector<uint32_t> mapv(D.map, D.map + D.N);
-            L.map = mem.upload(mapv, s);
+            L.map = data->upload(mapv, s);
             L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
             L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
             ProgView<VT>& G = L.prog;
@@ -267,25 +279,25 @@ template <class VT> struct DevHier : DevPrecBase {
             G.nflat = uint32_t(P.flat.size());
             G.max_nodes = P.max_nodes; G.max_Lent = P.max_Lent; G.max_Uent = P.max_Uent; G.max_Unodes = P.max_Unodes;
             G.max_levels = P.max_levels;
-            G.qrow = mem.upload(P.qrow, s);
-            G.flatq = mem.upload(P.flatq, s);
+            G.qrow = data->upload(P.qrow, s);
+            G.flatq = data->upload(P.flatq, s);
             if (P.nblk) {
-                G.blk = mem.upload(P.blk, s);
-                G.lvlL = mem.upload(P.lvlL, s);
-                G.lvlU = mem.upload(P.lvlU, s);
-                G.out = mem.upload(P.out, s);
-                G.Lptr = mem.upload(P.Lptr, s);
-                G.Lloc = mem.upload(P.Lloc, s);
-                G.Lval = mem.upload_as<VT>(P.Lval, s);
-                G.Ldep = mem.upload(P.Ldep, s);
-                G.Udep = mem.upload(P.Udep, s);
-                G.uloc = mem.upload(P.uloc, s);
-                G.uxidx = mem.upload(P.uxidx, s);
-                G.Uptr = mem.upload(P.Uptr, s);
-                G.Ucol = mem.upload(P.Ucol, s);
-                G.Uval = mem.upload_as<VT>(P.Uval, s);
-                G.Udiag = mem.upload_as<VT>(P.Udiag, s);
-                L.wt = mem.alloc<VT>(P.nodes);
+                G.blk = data->upload(P.blk, s);
+                G.lvlL = data->upload(P.lvlL, s);
+                G.lvlU = data->upload(P.lvlU, s);
+                G.out = data->upload(P.out, s);
+                G.Lptr = data->upload(P.Lptr, s);
+                G.Lloc = data->upload(P.Lloc, s);
+                G.Lval = data->upload_as<VT>(P.Lval, s);
+                G.Ldep = data->upload(P.Ldep, s);
+                G.Udep = data->upload(P.Udep, s);
+                G.uloc = data->upload(P.uloc, s);
+                G.uxidx = data->upload(P.uxidx, s);
+                G.Uptr = data->upload(P.Uptr, s);
+                G.Ucol = data->upload(P.Ucol, s);
+                G.Uval = data->upload_as<VT>(P.Uval, s);
+                G.Udiag = data->upload_as<VT>(P.Udiag, s);
+                L.wt = scratch.alloc<VT>(P.nodes);
             }
             if (P.max_nodes <= lbs && P.max_Unodes <= lbs) {
                 // every block takes the owned-node path, which keeps only w (sw) and x (sx) in LDS
@@ -298,21 +310,21 @@ template <class VT> struct DevHier : DevPrecBase {
                 L.lds_solve = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true).total;
             }
             std::vector<double> dv(D.dval, D.dval + D.Md);
-            L.dval = mem.upload_as<VT>(dv, s);
+            L.dval = data->upload_as<VT>(dv, s);
             if (l > 0) {
                 // uh_cheb reads A_l only through the fused Chebyshev residual, row by row in consumption order
                 if (kind == MLP_PREC_UH_CHEB)
-                    upload_sell<VT>(mem, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A, P.qrow.data());
+                    upload_sell<VT>(*data, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A, P.qrow.data());
                 else
-                    upload_sell<VT>(mem, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A);
+                    upload_sell<VT>(*data, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A);
             }
             if (L.n) {
-                L.F = mem.alloc<VT>(L.n);
-                L.u1 = mem.alloc<VT>(L.n);
-                L.u0 = mem.alloc<VT>(L.n);
-                L.x2 = mem.alloc<VT>(L.n);
+                L.F = scratch.alloc<VT>(L.n);
+                L.u1 = scratch.alloc<VT>(L.n);
+                L.u0 = scratch.alloc<VT>(L.n);
+                L.x2 = scratch.alloc<VT>(L.n);
             }
-            if (kind == MLP_PREC_MULTI_SPLIT && l > 0) L.r = mem.alloc<VT>(L.N);
+            if (kind == MLP_PREC_MULTI_SPLIT && l > 0) L.r = scratch.alloc<VT>(L.N);
         }
         if (kind == MLP_PREC_UH_CHEB) {
             for (uint32_t l = 0; l + 1 < nlevels; l++) {
@@ -338,6 +350,35 @@ template <class VT> struct DevHier : DevPrecBase {
             throw Error("multi_split level 0 runs one iteration", 1);
         MLP_HIP_CHECK(hipStreamSynchronize(s));
     }
+
+    // A preconditioner on context c that shares this one's device data and has
+    // level vectors of its own, so both can be applied at the same time.
+    std::unique_ptr<DevHier<VT>> clone(mlp_ctx c) const {
+        auto h = std::make_unique<DevHier<VT>>();
+        h->ctx = c;
+        h->kind = kind;
+        h->nlevels = nlevels;
+        h->lv = lv;
+        h->data = data;
+        h->A0 = A0;
+        h->dynamic = dynamic;
+        h->lbs = lbs;
+        h->cap0 = cap0;
+        DevMem& sc = h->scratch;
+        for (auto& L : h->lv) {
+            if (L.wt) L.wt = sc.template alloc<VT>(std::max<uint32_t>(L.prog.nodes, 1));
+            if (L.F) {
+                L.F = sc.template alloc<VT>(L.n);
+                L.u1 = sc.template alloc<VT>(L.n);
+                L.u0 = sc.template alloc<VT>(L.n);
+                L.x2 = sc.template alloc<VT>(L.n);
+            }
+            if (L.r) L.r = sc.template alloc<VT>(L.N);
+        }
+        return h;
+    }
+
+    std::unique_ptr<DevPrecBase> clone_on(mlp_ctx c) const override { return clone(c); }
 
     // --- launch helpers ---
     template <class K, class... Args>
@@ -879,7 +920,7 @@ int mlp_csr_upload(mlp_ctx ctx, uint32_t n, const uint32_t* ia, const uint32_t* 
     auto* m = new mlp_mat_s();
     try {
         m->ctx = ctx;
-        m->id = ctx->next_id++;
+        m->id = mlp::new_handle_id();
         m->n = n;
         m->nnz = ia[n];
         m->ia.assign(ia, ia + n + 1);
@@ -1122,7 +1163,7 @@ int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_des
     require(levels[0].N == A0->n, "level 0 size differs from A0");
     auto p = std::make_unique<mlp_prec_s>();
     p->ctx = ctx;
-    p->id = ctx->next_id++;
+    p->id = mlp::new_handle_id();
     p->kind = kind;
     p->precision = precision;
     p->n = A0->n;
@@ -1174,7 +1215,7 @@ int mlp_prec_simple(mlp_ctx ctx, int kind, mlp_mat A0, mlp_prec* out) {
     }
     auto p = std::make_unique<mlp_prec_s>();
     p->ctx = ctx;
-    p->id = ctx->next_id++;
+    p->id = mlp::new_handle_id();
     p->kind = kind;
     p->n = A0->n;
     p->impl = std::move(sp);
@@ -1190,7 +1231,7 @@ int mlp_prec_gs(mlp_ctx ctx, mlp_mat A, mlp_prec* out) {
     g->build(A->ia, A->ja, A->a);
     auto p = std::make_unique<mlp_prec_s>();
     p->ctx = ctx;
-    p->id = ctx->next_id++;
+    p->id = mlp::new_handle_id();
     p->kind = MLP_PREC_GS;
     p->n = A->n;
     p->impl = std::move(g);
@@ -1217,7 +1258,7 @@ int mlp_prec_composite(mlp_ctx ctx, mlp_mat A, mlp_hier ms, mlp_prec* out) {
     c->tmp = c->mem.alloc<double>(std::max<uint32_t>(A->n, 1));
     auto p = std::make_unique<mlp_prec_s>();
     p->ctx = ctx;
-    p->id = ctx->next_id++;
+    p->id = mlp::new_handle_id();
     p->kind = MLP_PREC_COMPOSITE;
     p->n = A->n;
     p->impl = std::move(c);
@@ -1234,10 +1275,27 @@ int mlp_prec_sym(mlp_ctx ctx, mlp_mat A, const double* sigmas, uint32_t nsigmas,
     y->build(A, sigmas, nsigmas, niters, nniters, use_tails);
     auto p = std::make_unique<mlp_prec_s>();
     p->ctx = ctx;
-    p->id = ctx->next_id++;
+    p->id = mlp::new_handle_id();
     p->kind = MLP_PREC_SYM;
     p->n = A->n;
     p->impl = std::move(y);
+    *out = p.release();
+    MLP_API_END
+}
+
+int mlp_prec_clone(mlp_ctx ctx, mlp_prec src, mlp_prec* out) {
+    MLP_API_BEGIN
+    require(ctx && src && out, "null argument");
+    require(ctx->device == src->ctx->device, "the clone must be on the device of the source");
+    std::unique_ptr<DevPrecBase> impl = src->impl->clone_on(ctx);
+    require(impl != nullptr, "only uh_cheb and multi_split preconditioners can be cloned");
+    auto p = std::make_unique<mlp_prec_s>();
+    p->ctx = ctx;
+    p->id = mlp::new_handle_id();
+    p->kind = src->kind;
+    p->precision = src->precision;
+    p->n = src->n;
+    p->impl = std::move(impl);
     *out = p.release();
     MLP_API_END
 }
@@ -1719,7 +1777,8 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
         mlp_pcg_job& J = jobs[i];
         require(J.ctx && J.A && J.B && J.d_b && J.d_x, "null job argument");
         require(J.B->n == J.A->n, "preconditioner size differs from A");
-        require(J.A->ctx == J.ctx && J.B->ctx == J.ctx, "a job's matrix and preconditioner must live on its context");
+        require(J.B->ctx == J.ctx, "a job's preconditioner must live on its context (mlp_prec_clone)");
+        require(J.A->ctx->device == J.ctx->device, "a job's matrix must be on the device of its context");
         for (mlp_ctx c : seen) require(c != J.ctx, "two jobs share a context");
         seen.push_back(J.ctx);
         std::memset(&J.stats, 0, sizeof(J.stats));

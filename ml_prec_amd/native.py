@@ -99,6 +99,7 @@ EXPORTS = {
     "mlp_prec_gs": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
     "mlp_prec_composite": (C.c_int, [_vp, _vp, _vp, C.POINTER(_vp)]),
     "mlp_prec_sym_set_eps": (C.c_int, [_vp, C.c_double]),
+    "mlp_prec_clone": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
     "mlp_prec_sym": (C.c_int, [_vp, _vp, _f64p, C.c_uint32, _u32p, C.c_uint32, C.c_int, C.POINTER(_vp)]),
     "mlp_prec_destroy": (C.c_int, [_vp]),
     "mlp_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp]),
@@ -334,6 +335,13 @@ class DevPrec:
         if getattr(self, "h", None) and _lib is not None and self.ctx.h:
             _lib.mlp_prec_destroy(self.h)
         self.h = None
+
+    def clone(self, ctx):
+        """A copy on `ctx` sharing this hierarchy's device data, with its own level vectors."""
+        c = DevPrec.__new__(DevPrec)
+        c.ctx, c.A, c.hier, c.kind, c.precision, c.h = ctx, self.A, self.hier, self.kind, self.precision, _vp()
+        check(lib().mlp_prec_clone(ctx.h, self.h, C.byref(c.h)))
+        return c
 
     def __del__(self):
         self.close()

@@ -100,3 +100,40 @@ def test_batch_rejects_shared_context():
     B.close()
     A.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_shared_hierarchy_many_rhs(mixed):
+    """One matrix and one hierarchy, several right-hand sides solved concurrently: the jobs share
+    the matrix and use clones of one preconditioner (mlp_prec_clone); each result is bitwise the
+    single solve with the source preconditioner.  The source is closed before the batch runs, so
+    the clones keep the shared device data alive."""
+    s = S.ensemble_system(12, S.BASE_SEED + 110)
+    prec = N.MLP_FP32 if mixed else N.MLP_FP64
+    kw = dict(eps=1e-8, maxit=5000, x0_mode=N.X0_ZERO, check_every=6, use_graph=True)
+    if mixed:
+        kw.update(flexible=True, true_res_every=20)
+    ctx0 = N.Context(0)
+    A = N.DevMatrix(ctx0, s.ia, s.ja, s.a)
+    B0 = N.DevPrec(ctx0, A, N.build_uh_cheb(s.ia, s.ja, s.a), precision=prec)
+    rhs = [np.random.default_rng(120 + k).standard_normal(s.n) for k in range(4)]
+    single = []
+    for f in rhs:
+        x = ctx0.array(s.n)
+        st = N.pcg(ctx0, A, B0, ctx0.to_device(f), x, **kw)
+        single.append((st["niter"], x.get()))
+    ctxs = [N.Context(0) for _ in rhs]
+    clones = [B0.clone(c) for c in ctxs]
+    B0.close()
+    jobs = [(c, A, Bc, c.to_device(f), c.array(s.n)) for c, Bc, f in zip(ctxs, clones, rhs)]
+    rc, stats = N.pcg_batch(jobs, **kw)
+    assert rc == 0
+    for (niter, x1), st, job in zip(single, stats, jobs):
+        assert st["niter"] == niter
+        assert np.array_equal(job[4].get(), x1)
+    for Bc in clones:
+        Bc.close()
+    A.close()
+    for c in ctxs:
+        c.close()
+    ctx0.close()
