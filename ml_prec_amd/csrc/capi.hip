@@ -83,7 +83,8 @@ void upload_sell(DevMem& mem, hipStream_t s, uint32_t n, const uint32_t* ia, con
     plan_sell(n, ia, ja, a, L, rows);
     out.v.n = n;
     out.v.off64 = mem.upload(L.off64, s);
-    out.v.col = mem.upload(L.col, s);
+    if (!L.dcol.empty()) out.v.dcol = mem.upload(L.dcol, s);
+    else out.v.col = mem.upload(L.col, s);
     out.v.val = mem.upload_as<VT>(L.val, s);
     out.nent = L.nent;
     out.nnz = n ? ia[n] : 0;
@@ -927,7 +928,7 @@ int mlp_csr_upload(mlp_ctx ctx, uint32_t n, const uint32_t* ia, const uint32_t* 
         m->ja.assign(ja, ja + m->nnz);
         m->a.assign(a, a + m->nnz);
         upload_sell<double>(m->mem, ctx->stream, n, ia, ja, a, m->A);
-        m->bytes = m->A.nent * 12 + uint64_t(m->A.v.n / 64 + 2) * 4;
+        m->bytes = m->A.nent * (m->A.v.dcol ? 10 : 12) + uint64_t(m->A.v.n / 64 + 2) * 4;
     } catch (...) {
         delete m;
         throw;
@@ -1336,22 +1337,24 @@ int mlp_time_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y, uint32_t r
     // launch (cache-warm).  Launches run back to back between two events.
     std::vector<SellView<double>> views{A->A.v};
     std::vector<void*> extra;
-    const size_t mbytes = size_t(A->A.nent) * (sizeof(uint32_t) + sizeof(double));
+    const size_t mbytes = size_t(A->A.nent) * ((A->A.v.dcol ? sizeof(int16_t) : sizeof(uint32_t)) + sizeof(double));
     if (flush_bytes && mbytes) {
         const size_t nc = std::max<size_t>(2, (flush_bytes + mbytes - 1) / mbytes);
         for (size_t c = 1; c < nc; c++) {
             SellView<double> v = A->A.v;
-            uint32_t* col = nullptr;
+            const size_t ib = A->A.v.dcol ? sizeof(int16_t) : sizeof(uint32_t);
+            void* idx = nullptr;
             double* val = nullptr;
-            MLP_HIP_CHECK(hipMalloc(&col, A->A.nent * sizeof(uint32_t)));
-            extra.push_back(col);
+            MLP_HIP_CHECK(hipMalloc(&idx, A->A.nent * ib));
+            extra.push_back(idx);
             MLP_HIP_CHECK(hipMalloc(&val, A->A.nent * sizeof(double)));
             extra.push_back(val);
-            MLP_HIP_CHECK(hipMemcpyAsync(col, A->A.v.col, A->A.nent * sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                                         ctx->stream));
+            MLP_HIP_CHECK(hipMemcpyAsync(idx, A->A.v.dcol ? (const void*)A->A.v.dcol : (const void*)A->A.v.col,
+                                         A->A.nent * ib, hipMemcpyDeviceToDevice, ctx->stream));
             MLP_HIP_CHECK(hipMemcpyAsync(val, A->A.v.val, A->A.nent * sizeof(double), hipMemcpyDeviceToDevice,
                                          ctx->stream));
-            v.col = col;
+            if (A->A.v.dcol) v.dcol = static_cast<const int16_t*>(idx);
+            else v.col = static_cast<const uint32_t*>(idx);
             v.val = val;
             views.push_back(v);
         }

@@ -19,10 +19,16 @@ constexpr int kRedBlocks = 16384;        // max blocks of a one- or two-value re
 constexpr int kMdotBlocks = 1024;        // max blocks of the Gram-Schmidt multi-dot (32 partials each)
 constexpr size_t kPartials = 32768;      // doubles of partial-sum scratch (2 * kRedBlocks = 32 * kMdotBlocks)
 
+// Sliced ELL (64-row slices, column-major inside a slice).  Column indices
+// are either uint32 (col) or, when every |column - row| fits, int16 deltas
+// from the row (dcol, kPad16 marks padding), which cuts the index stream in
+// half; exactly one of col / dcol is set.
+constexpr int16_t kPad16 = -32768;
 template <class VT> struct SellView {
     uint32_t n = 0;
     const uint32_t* off64 = nullptr;
     const uint32_t* col = nullptr;
+    const int16_t* dcol = nullptr;
     const VT* val = nullptr;
 };
 
@@ -104,17 +110,30 @@ __device__ __forceinline__ VT sell_row(const SellView<VT>& A, const XT* __restri
     VT acc = VT(0);
     for (uint32_t k = b; k < e; k += G) {
         uint32_t c[G];
+        bool ok[G];
         VT v[G];
         XT xv[G];
+        if (A.dcol) {
 #pragma unroll
-        for (uint32_t j = 0; j < G; j++) c[j] = k + j < e ? A.col[size_t(k + j) * 64 + lane] : kPad;
+            for (uint32_t j = 0; j < G; j++) {
+                const int16_t d = k + j < e ? A.dcol[size_t(k + j) * 64 + lane] : kPad16;
+                ok[j] = d != kPad16;
+                c[j] = uint32_t(int32_t(r) + int32_t(d));
+            }
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < G; j++) {
+                c[j] = k + j < e ? A.col[size_t(k + j) * 64 + lane] : kPad;
+                ok[j] = c[j] != kPad;
+            }
+        }
 #pragma unroll
         for (uint32_t j = 0; j < G; j++) v[j] = k + j < e ? A.val[size_t(k + j) * 64 + lane] : VT(0);   // not gated on c
 #pragma unroll
-        for (uint32_t j = 0; j < G; j++) xv[j] = c[j] != kPad ? x[c[j]] : XT(0);
+        for (uint32_t j = 0; j < G; j++) xv[j] = ok[j] ? x[c[j]] : XT(0);
 #pragma unroll
         for (uint32_t j = 0; j < G; j++) {
-            if (c[j] != kPad) {
+            if (ok[j]) {
                 VT t = VT(xv[j]);
                 if (SCALE) t = t * s;
                 acc += v[j] * t;

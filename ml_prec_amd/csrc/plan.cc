@@ -226,6 +226,25 @@ void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double*
             S.val[e] = a[ia[r] + k];
         }
     }
+    // 16-bit column deltas when the rows are in their own order and every |j - i| fits
+    if (!rows) {
+        bool fits = true;
+        for (uint32_t r = 0; r < n && fits; r++)
+            for (uint32_t p = ia[r]; p < ia[r + 1]; p++) {
+                const int64_t d = int64_t(ja[p]) - int64_t(r);
+                if (d < -32767 || d > 32767) { fits = false; break; }
+            }
+        if (fits) {
+            S.dcol.assign(S.col.size(), int16_t(-32768));
+            for (uint32_t q = 0; q < n; q++) {
+                const uint32_t s = q / 64, t = q % 64;
+                for (uint32_t k = 0; k < ia[q + 1] - ia[q]; k++) {
+                    const uint64_t e = (uint64_t(S.off64[s]) + k) * 64 + t;
+                    S.dcol[e] = int16_t(int64_t(ja[ia[q] + k]) - int64_t(q));
+                }
+            }
+        }
+    }
 }
 
 }  // namespace mlp

@@ -81,6 +81,7 @@ struct SellLayout {
     uint64_t nent = 0;
     std::vector<uint32_t> off64;          // nslices+1
     std::vector<uint32_t> col;
+    std::vector<int16_t> dcol;            // column - row, when every entry fits (rows unpermuted); -32768 = padding
     std::vector<double> val;
 };
 
