@@ -1575,6 +1575,21 @@ void enqueue_pcg_chunk(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, doub
     }
 }
 
+// Iterations to enqueue before the next look at the device state.  An
+// iteration enqueued after convergence still launches its kernels (they return
+// at once), and each look costs a host round trip, so after the first chunk
+// the count is predicted from the mean reduction per iteration so far:
+// ceil(log(eps / rel) / log(rate)), between 1 and max(check_every, 64).
+uint32_t next_chunk(const SolveState& st, uint32_t check_every) {
+    const double rel = st.init_norm > 0 ? st.norm / st.init_norm : 0.0;
+    if (!(rel > 0 && rel < 1) || st.niter == 0 || !(st.eps > 0)) return check_every;
+    const double rate = std::pow(rel, 1.0 / st.niter);
+    if (!(rate > 0 && rate < 1)) return check_every;
+    const double need = std::ceil(std::log(st.eps / rel) / std::log(rate));
+    if (!(need >= 1)) return 1;
+    return uint32_t(std::min<double>(need, std::max<uint32_t>(check_every, 64)));
+}
+
 // One PCG phase from the current x: r = b - A x, z = B r, p = z, then
 // iterations until the device sets `done` (pcg.cc:24-76).
 SolveState pcg_phase(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, const mlp_solve_opts* o,
@@ -1598,13 +1613,14 @@ SolveState pcg_phase(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double
     const uint32_t tre = o->true_res_every;
     mlp_ctx_s::GraphEntry* ge = pcg_graphs(ctx, A, B, b, x, flex, tre, o->use_graph && B->impl->graph_safe());
     SolveState st;
-    uint32_t k = 0;
+    uint32_t k = 0, c = chunk;
     while (true) {
-        enqueue_pcg_chunk(ctx, A, B, b, x, flex, tre, ge, k, chunk);
-        k += chunk;
+        enqueue_pcg_chunk(ctx, A, B, b, x, flex, tre, ge, k, c);
+        k += c;
         MLP_HIP_CHECK(hipGetLastError());
         st = read_state(ctx);
         if (st.done) break;
+        c = next_chunk(st, chunk);
     }
     return st;
 }
@@ -1666,10 +1682,10 @@ SolveState run_outer(mlp_ctx ctx, int solver, mlp_mat A, mlp_prec B, const void*
         }
     }
     SolveState st;
-    uint32_t k = 0;
+    uint32_t k = 0, n = chunk;
     while (true) {
-        before_chunk(k + chunk);
-        for (uint32_t c = 0; c < chunk; c++) {
+        before_chunk(k + n);
+        for (uint32_t c = 0; c < n; c++) {
             k++;
             const int v = pick(k);
             if (use_graph) MLP_HIP_CHECK(hipGraphLaunch(ge->e[v], s));
@@ -1678,6 +1694,7 @@ SolveState run_outer(mlp_ctx ctx, int solver, mlp_mat A, mlp_prec B, const void*
         MLP_HIP_CHECK(hipGetLastError());
         st = read_state(ctx);
         if (st.done) break;
+        n = next_chunk(st, chunk);
     }
     return st;
 }
@@ -1767,7 +1784,7 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
         mlp_pcg_job* j;
         hipEvent_t ev = nullptr;
         mlp_ctx_s::GraphEntry* ge = nullptr;
-        uint32_t k = 0;
+        uint32_t k = 0, c = 0;
         bool active = false;
         std::chrono::steady_clock::time_point t0;
     };
@@ -1814,8 +1831,9 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
     }
     auto enqueue = [&](Run& R) {
         mlp_ctx ctx = R.j->ctx;
-        enqueue_pcg_chunk(ctx, R.j->A, R.j->B, R.j->d_b, R.j->d_x, flex, tre, R.ge, R.k, chunk);
-        R.k += chunk;
+        const uint32_t c = R.c ? R.c : chunk;
+        enqueue_pcg_chunk(ctx, R.j->A, R.j->B, R.j->d_b, R.j->d_x, flex, tre, R.ge, R.k, c);
+        R.k += c;
         MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, ctx->st, sizeof(SolveState), hipMemcpyDeviceToHost,
                                      ctx->stream));
         MLP_HIP_CHECK(hipEventRecord(R.ev, ctx->stream));
@@ -1836,6 +1854,7 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
                 R.active = false;
                 left--;
             } else {
+                R.c = next_chunk(st, chunk);
                 enqueue(R);
             }
         }
