@@ -1717,56 +1717,24 @@ void init_state(mlp_ctx ctx, double norm, double init_norm, double eps, uint32_t
 
 extern "C" {
 
-// PCGSolver (modules/solvers/pcg.cc:8-102) with device-resident scalars.
+// PCGSolver (modules/solvers/pcg.cc:8-102) with device-resident scalars: a
+// batch of one (mlp_pcg_batch), so the solve starts without host round trips.
 // flexible: Polak-Ribiere beta and, after convergence of the recurrence
 // residual, restarts from the FP64 true residual until it meets eps as well.
 int mlp_pcg(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, const mlp_solve_opts* o,
             mlp_solve_stats* stats) {
     MLP_API_BEGIN
     require(ctx && A && B && b && x && o, "null argument");
-    require(B->n == A->n, "preconditioner size differs from A");
-    const auto t0 = std::chrono::steady_clock::now();
-    const uint32_t n = A->n;
-    ensure_workspace(ctx, n, false);
-    fill_x0(ctx, n, o->x0_mode, x);
-    launch_residual(ctx, A, b, x, ctx->r);
-    const double init_norm = std::sqrt(dev_dot(ctx, n, ctx->r, ctx->r));
-    double eps = o->eps;
-    if (init_norm * eps < 1e-14) eps = 1e-14 / init_norm;      // check_and_replace_eps, solvers.cc:49-58
-    if (stats) std::memset(stats, 0, sizeof(*stats));
-    bool no_iter = false;
-    SolveState st = pcg_phase(ctx, A, B, b, x, o, init_norm, eps, 0, &no_iter);
-    if (no_iter) {
-        if (stats) { stats->init_norm = init_norm; stats->status = MLP_ERR_NO_ITER; }
-        throw Error("No need to iterate", MLP_ERR_NO_ITER);
-    }
-    double rel = 0.0;
-    if (o->flexible) {
-        for (int restart = 0; restart < 8 && st.status == 0; restart++) {
-            rel = true_rel_res(ctx, A, b, x, ctx->r, init_norm);
-            if (rel <= eps) break;
-            bool again = false;
-            st = pcg_phase(ctx, A, B, b, x, o, init_norm, eps, st.niter, &again);
-            if (again) break;
-        }
-    }
-    MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    const auto t1 = std::chrono::steady_clock::now();
-    if (stats) {
-        stats->niter = st.niter;
-        stats->init_norm = init_norm;
-        stats->final_norm = st.norm;
-        stats->eps_used = eps;
-        stats->t_solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-        stats->status = int(st.status);
-        stats->true_rel_res = true_rel_res(ctx, A, b, x, ctx->r, init_norm);
-    }
-    if (st.status == 3) {
-        char buf[160];
-        std::snprintf(buf, sizeof(buf), "<Ap, p> = %g", st.app);
-        throw Error(buf, MLP_ERR_NOT_SPD);
-    }
-    if (st.status == 5) throw Error("maximum number of iterations reached", MLP_ERR_MAXIT);
+    mlp_pcg_job J;
+    std::memset(&J, 0, sizeof(J));
+    J.ctx = ctx;
+    J.A = A;
+    J.B = B;
+    J.d_b = b;
+    J.d_x = x;
+    const int rc = mlp_pcg_batch(1, &J, o);
+    if (stats) *stats = J.stats;
+    if (rc) return rc;
     MLP_API_END
 }
 
@@ -1889,7 +1857,11 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
             J.stats.t_solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - R.t0).count();
             J.stats.status = int(st.status);
             J.stats.true_rel_res = true_rel_res(ctx, J.A, J.d_b, J.d_x, ctx->r, init_norm);
-            if (st.status == 3) throw Error("<Ap, p> <= 0", MLP_ERR_NOT_SPD);
+            if (st.status == 3) {
+                char buf[160];
+                std::snprintf(buf, sizeof(buf), "<Ap, p> = %g", st.app);
+                throw Error(buf, MLP_ERR_NOT_SPD);
+            }
             if (st.status == 5) throw Error("maximum number of iterations reached", MLP_ERR_MAXIT);
         } catch (const Error& e) {
             J.status = e.code();
