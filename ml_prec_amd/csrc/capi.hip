@@ -979,6 +979,7 @@ int mlp_residual(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, doubl
 // (8 per CU), which keeps the one-block finalize short.
 constexpr uint32_t kRedGrid = 2048;
 static_assert(kRedGrid <= kRedBlocks, "partials buffer too small");
+static_assert(2 * kRedGrid <= kPartials / 2, "each half of the partials buffer must hold two values per block");
 static uint32_t red_grid(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n, kBS), kRedGrid)); }
 static uint32_t mdot_grid(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n, kBS), kMdotBlocks)); }
 
@@ -1493,36 +1494,35 @@ struct GraphPair {
 void enqueue_pcg_iteration(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, bool flex, bool replace) {
     const uint32_t n = A->n, rg = red_grid(n);
     hipStream_t s = ctx->stream;
-    auto fin = [&](int op) {
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, op, rg, (const double*)ctx->partials, ctx->st,
-                           (double*)nullptr);
+    // two partial-sum buffers: k_pcg_xr reads the (Ap, p) partials while it writes the (r, r) ones
+    double* P0 = ctx->partials;
+    double* P1 = ctx->partials + kPartials / 2;
+    auto fin = [&](int op, const double* part) {
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, op, rg, part, ctx->st, (double*)nullptr);
     };
     hipLaunchKernelGGL(k_pcg_spmv_dot, dim3(rg), dim3(kBS), 0, s, A->A.v, (const double*)ctx->p, ctx->Ap,
-                       (const SolveState*)ctx->st, ctx->partials);
-    fin(FIN_APP);
+                       (const SolveState*)ctx->st, P0);
     hipLaunchKernelGGL(k_pcg_xr, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->p, (const double*)ctx->Ap, x,
-                       ctx->r, (const SolveState*)ctx->st, ctx->partials);
+                       ctx->r, ctx->st, (const double*)P0, rg, P1);
     if (replace) {
-        fin(FIN_XR_NOTEST);
+        fin(FIN_XR_NOTEST, P1);
         hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)x, ctx->r,
-                           (const SolveState*)ctx->st, ctx->partials);
-        fin(FIN_TRUE_RES);
+                           (const SolveState*)ctx->st, P0);
+        fin(FIN_TRUE_RES, P0);
     } else {
-        fin(FIN_XR);
+        fin(FIN_XR, P1);
     }
     B->impl->apply(ctx->r, ctx->z, &ctx->st->done);
     if (flex) {
         hipLaunchKernelGGL(k_pcg_rz<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                           (const double*)ctx->zp, (const SolveState*)ctx->st, ctx->partials);
-        fin(FIN_RZ_FLEX);
+                           (const double*)ctx->zp, (const SolveState*)ctx->st, P0);
         hipLaunchKernelGGL(k_pcg_p<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp,
-                           (const SolveState*)ctx->st);
+                           ctx->st, (const double*)P0, rg);
     } else {
         hipLaunchKernelGGL(k_pcg_rz<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                           (const double*)ctx->zp, (const SolveState*)ctx->st, ctx->partials);
-        fin(FIN_RZ);
+                           (const double*)ctx->zp, (const SolveState*)ctx->st, P0);
         hipLaunchKernelGGL(k_pcg_p<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp,
-                           (const SolveState*)ctx->st);
+                           ctx->st, (const double*)P0, rg);
     }
 }
 

@@ -36,8 +36,8 @@ template <class VT> struct SellView {
 // Device-resident state of a PCG/GMRES solve.  Host code never needs these
 // values between iterations; it polls `done` every few iterations.
 struct SolveState {
-    double rz, app, alpha, beta, rr, norm, init_norm, eps;
-    double rz_prev;
+    double rzs[2];                       // (r, z) of the iteration with niter of that parity
+    double app, alpha, beta, rr, norm, init_norm, eps;
     uint32_t niter, done, status, maxit;
     uint32_t alpha_one, malpha_one, beta_one;
     uint32_t pad;                        // Chebyshev/simple: niter offset of the maxit test
@@ -84,6 +84,29 @@ __device__ __forceinline__ double sum_partials(const double* partials, uint32_t 
     double s = 0.0;
     for (uint32_t b = threadIdx.x; b < nb; b += BS) s += partials[NV * b + k];
     return block_sum<BS>(s, sh);
+}
+
+// st->done read once per block, so that every thread of a block takes the
+// same branch even while another block of the same launch sets it
+__device__ __forceinline__ bool block_done(const SolveState* st) {
+    __shared__ uint32_t d;
+    if (threadIdx.x == 0) d = *((volatile const uint32_t*)&st->done);
+    __syncthreads();
+    const bool r = d != 0;
+    __syncthreads();
+    return r;
+}
+
+// sum_partials, returned to every thread of the block
+template <int BS>
+__device__ __forceinline__ double total_of_partials(const double* partials, uint32_t nb, int NV, int k, double* sh) {
+    const double t = sum_partials<BS>(partials, nb, NV, k, sh);
+    __shared__ double out;
+    if (threadIdx.x == 0) out = t;
+    __syncthreads();
+    const double r = out;
+    __syncthreads();
+    return r;
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
@@ -622,32 +645,19 @@ __global__ void __launch_bounds__(kBS) k_diag_apply(uint32_t n, const double* __
 // The streaming kernels store per-block partial sums; k_finalize adds them
 // and updates the scalar recurrences of the device-resident SolveState.
 // ---------------------------------------------------------------------------
-enum { FIN_DOT = 0, FIN_APP = 1, FIN_XR = 2, FIN_XR_NOTEST = 3, FIN_RZ = 4, FIN_RZ_FLEX = 5, FIN_INIT = 6,
-       FIN_TRUE_RES = 7, FIN_STEP = 8, FIN_START = 9 };
+enum { FIN_DOT = 0, FIN_XR = 2, FIN_XR_NOTEST = 3, FIN_INIT = 6, FIN_TRUE_RES = 7, FIN_STEP = 8, FIN_START = 9 };
 
 __global__ void __launch_bounds__(kBS) k_finalize(int op, uint32_t nb, const double* __restrict__ partials,
                                                   SolveState* st, double* out) {
     if (op != FIN_DOT && op != FIN_INIT && op != FIN_START && st->done) return;
     __shared__ double sh[kBS / 64];
-    const int NV = (op == FIN_RZ_FLEX) ? 2 : 1;
+    const int NV = 1;
     const double t0 = sum_partials<kBS>(partials, nb, NV, 0, sh);
     const double t1 = NV == 2 ? sum_partials<kBS>(partials, nb, NV, 1, sh) : 0.0;
     if (threadIdx.x != 0) return;
     switch (op) {
         case FIN_DOT: *out = t0; break;
-        case FIN_INIT: st->rz = t0; break;
-        case FIN_APP:                                        // alpha = rz0 / (Ap, p)   pcg.cc:51-57
-            st->app = t0;
-            if (t0 <= 0) {
-                st->status = 3;
-                st->done = 1;
-            } else {
-                const double a = st->rz / t0;
-                st->alpha = a;
-                st->alpha_one = dev_is_one(a);
-                st->malpha_one = dev_is_one(-a);
-            }
-            break;
+        case FIN_INIT: st->rzs[st->niter & 1] = t0; break;
         case FIN_XR:                                         // norm, niter, stop test   pcg.cc:42, 61, 75
         case FIN_XR_NOTEST:
             st->rr = t0;
@@ -656,14 +666,6 @@ __global__ void __launch_bounds__(kBS) k_finalize(int op, uint32_t nb, const dou
             if (op == FIN_XR && !(st->norm / st->init_norm > st->eps)) st->done = 1;
             else if (st->maxit && st->niter >= st->maxit) { st->status = 5; st->done = 1; }
             break;
-        case FIN_RZ:                                         // beta = rz1 / rz0   pcg.cc:68-69
-        case FIN_RZ_FLEX: {
-            const double b = (op == FIN_RZ_FLEX ? t1 : t0) / st->rz;
-            st->beta = b;
-            st->beta_one = dev_is_one(b);
-            st->rz = t0;
-            break;
-        }
         case FIN_TRUE_RES:
             st->rr = t0;
             st->norm = sqrt(t0);
@@ -732,13 +734,34 @@ __global__ void __launch_bounds__(kBS) k_pcg_spmv_dot(SellView<double> A, const 
     block_partials<kBS, 1>(part, partials);
 }
 
-// x += alpha p ; r -= alpha Ap ; partials of (r, r)   (pcg.cc:58-61)
+// alpha = rz / (Ap, p) (pcg.cc:51-57), summed by every block from the
+// partials of k_pcg_spmv_dot in the same fixed order; then x += alpha p,
+// r -= alpha Ap and partials of (r, r) (pcg.cc:58-61).  Block 0 records alpha
+// (and "<Ap, p> <= 0", which stops the solve) in the state.
 __global__ void __launch_bounds__(kBS) k_pcg_xr(uint32_t n, const double* __restrict__ p, const double* __restrict__ Ap,
-                                                double* __restrict__ x, double* __restrict__ r, const SolveState* st,
+                                                double* __restrict__ x, double* __restrict__ r, SolveState* st,
+                                                const double* __restrict__ app_partials, uint32_t nb,
                                                 double* partials) {
-    if (st->done) return;
-    const double a = st->alpha, ma = -a;
-    const int one = st->alpha_one, mone = st->malpha_one;
+    if (block_done(st)) return;
+    __shared__ double sh[kBS / 64];
+    const double app = total_of_partials<kBS>(app_partials, nb, 1, 0, sh);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!(app > 0)) {
+        if (lead) {
+            st->app = app;
+            st->status = 3;
+            st->done = 1;
+        }
+        return;
+    }
+    const double a = st->rzs[st->niter & 1] / app, ma = -a;
+    const int one = dev_is_one(a), mone = dev_is_one(-a);
+    if (lead) {
+        st->app = app;
+        st->alpha = a;
+        st->alpha_one = one;
+        st->malpha_one = mone;
+    }
     double s = 0.0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] = one ? x[i] + p[i] : x[i] + a * p[i];
@@ -769,13 +792,27 @@ __global__ void __launch_bounds__(kBS) k_pcg_rz(uint32_t n, const double* __rest
     }
 }
 
-// p = beta p + z  (dscal + daxpy(1, z, p), pcg.cc:71-72); flexible keeps z for the next beta
+// beta = rz1 / rz0 (pcg.cc:68-69; flexible: (r, z - z_prev) / rz0), summed by
+// every block from the partials of k_pcg_rz; then p = beta p + z (dscal +
+// daxpy(1, z, p), pcg.cc:71-72); flexible keeps z for the next beta.  The new
+// rz goes to the state slot of this iteration's parity, so no block reads a
+// slot another block writes.
 template <bool FLEX>
 __global__ void __launch_bounds__(kBS) k_pcg_p(uint32_t n, const double* __restrict__ z, double* __restrict__ p,
-                                               double* __restrict__ zp, const SolveState* st) {
-    if (st->done) return;
-    const double b = st->beta;
-    const int one = st->beta_one;
+                                               double* __restrict__ zp, SolveState* st,
+                                               const double* __restrict__ rz_partials, uint32_t nb) {
+    if (block_done(st)) return;
+    __shared__ double sh[kBS / 64];
+    const double t0 = total_of_partials<kBS>(rz_partials, nb, FLEX ? 2 : 1, 0, sh);
+    const double t1 = FLEX ? total_of_partials<kBS>(rz_partials, nb, 2, 1, sh) : 0.0;
+    const uint32_t k = st->niter;
+    const double b = (FLEX ? t1 : t0) / st->rzs[(k + 1) & 1];
+    const int one = dev_is_one(b);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rzs[k & 1] = t0;
+        st->beta = b;
+        st->beta_one = one;
+    }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double pi = one ? p[i] : p[i] * b;
         p[i] = pi + z[i];
