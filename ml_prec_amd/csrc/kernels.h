@@ -498,7 +498,10 @@ template <class VT, class FT, int IN, class XT, int DMODE>
 __global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, InSrc<VT, FT, IN> f, VT* __restrict__ wt,
                                                      VT* __restrict__ F, uint32_t Md, const VT* __restrict__ dval,
                                                      XT* __restrict__ x, const uint32_t* guard) {
-    if (guarded(guard)) return;
+    // the solve's done flag: loaded now, tested before the first store, so that
+    // its latency overlaps the block's own loads (a converged iteration only
+    // writes the apply's scratch, never an output)
+    const uint32_t gflag = guard ? *guard : 0u;
     if (blockIdx.x < P.nblk) {
         extern __shared__ __align__(16) unsigned char lds_raw[];
         const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, false);
@@ -508,6 +511,7 @@ __global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, InSrc<VT, F
         const uint32_t o0 = threadIdx.x < S ? P.out[base + threadIdx.x] : 0u;
         if (S <= blockDim.x) {
             const VT v = owned_down(P, h.w, f, reinterpret_cast<VT*>(lds_raw + lp.sw));
+            if (gflag) return;
             if (threadIdx.x < S) {
                 if (o0 & kGlobalBit) F[o0 & ~kGlobalBit] = v;
                 else wt[base + threadIdx.x] = v;
@@ -515,6 +519,7 @@ __global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, InSrc<VT, F
             return;
         }
         block_down(P, h.w, f, lds_raw, lp);
+        if (gflag) return;
         const VT* sw = reinterpret_cast<const VT*>(lds_raw + lp.sw);
         for (uint32_t p = threadIdx.x; p < S; p += blockDim.x) {
             const uint32_t o = p == threadIdx.x ? o0 : P.out[base + p];
@@ -525,10 +530,12 @@ __global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, InSrc<VT, F
         const uint32_t i = (blockIdx.x - P.nblk) * blockDim.x + threadIdx.x;
         const uint32_t q = P.nodes + i;
         if (i < P.nflat) {
-            F[P.flatq[i]] = fetch_at(P, f, q);
+            const VT v = fetch_at(P, f, q);
+            if (!gflag) F[P.flatq[i]] = v;
         } else if (i < P.nflat + Md) {
             const uint32_t j = P.qrow[q];
-            put<XT, DMODE>(x, j, XT(dval[i - P.nflat] * fetch(f, q, j)));
+            const XT v = XT(dval[i - P.nflat] * fetch(f, q, j));
+            if (!gflag) put<XT, DMODE>(x, j, v);
         }
     }
 }
@@ -540,7 +547,10 @@ template <class VT, class XT, int OP, int UMODE>
 __global__ void __launch_bounds__(kLBS) k_level_up(ProgView<VT> P, const VT* __restrict__ wt, Combine<VT> C,
                                                    const uint32_t* __restrict__ mapM, uint32_t n, XT* __restrict__ x,
                                                    const uint32_t* guard) {
-    if (guarded(guard)) return;
+    // the solve's done flag: loaded now, tested before the first store, so that
+    // its latency overlaps the block's own loads (a converged iteration only
+    // writes the apply's scratch, never an output)
+    const uint32_t gflag = guard ? *guard : 0u;
     if (blockIdx.x < P.nblk) {
         extern __shared__ __align__(16) unsigned char lds_raw[];
         const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, false, true);
@@ -549,16 +559,21 @@ __global__ void __launch_bounds__(kLBS) k_level_up(ProgView<VT> P, const VT* __r
         const uint32_t x0 = threadIdx.x < nu ? P.uxidx[ub + threadIdx.x] : 0u;
         if (nu <= blockDim.x) {
             const VT v = owned_up<VT, OP, false>(P, h.w, wt, C, nullptr, reinterpret_cast<VT*>(lds_raw + lp.sx));
+            if (gflag) return;
             if (threadIdx.x < nu) put<XT, UMODE>(x, x0, XT(v));
             return;
         }
         block_up<VT, OP, false>(P, h.w, wt, C, lds_raw, lp);
+        if (gflag) return;
         const VT* sz = reinterpret_cast<const VT*>(lds_raw + lp.sz);
         for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x)
             put<XT, UMODE>(x, u == threadIdx.x ? x0 : P.uxidx[ub + u], XT(sz[u]));
     } else {
         const uint32_t r = (blockIdx.x - P.nblk) * blockDim.x + threadIdx.x;
-        if (r < n) put<XT, UMODE>(x, mapM[r], XT(combine<VT, OP>(C, r)));
+        if (r < n) {
+            const XT v = XT(combine<VT, OP>(C, r));
+            if (!gflag) put<XT, UMODE>(x, mapM[r], v);
+        }
     }
 }
 
@@ -568,7 +583,10 @@ template <class VT, class FT, int IN, class XT, int MODE>
 __global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, InSrc<VT, FT, IN> f, uint32_t Md,
                                                       const VT* __restrict__ dval, XT* __restrict__ x,
                                                       const uint32_t* guard) {
-    if (guarded(guard)) return;
+    // the solve's done flag: loaded now, tested before the first store, so that
+    // its latency overlaps the block's own loads (a converged iteration only
+    // writes the apply's scratch, never an output)
+    const uint32_t gflag = guard ? *guard : 0u;
     if (blockIdx.x < P.nblk) {
         extern __shared__ __align__(16) unsigned char lds_raw[];
         const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true);
@@ -580,12 +598,14 @@ __global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, InSrc<VT, 
             owned_down(P, h.w, f, sw);
             Combine<VT> none;
             const VT v = owned_up<VT, OP_NONE, true>(P, h.w, nullptr, none, sw, reinterpret_cast<VT*>(lds_raw + lp.sx));
+            if (gflag) return;
             if (threadIdx.x < nu) put<XT, MODE>(x, x0, XT(v));
             return;
         }
         block_down(P, h.w, f, lds_raw, lp);
         Combine<VT> none;
         block_up<VT, OP_NONE, true>(P, h.w, nullptr, none, lds_raw, lp);
+        if (gflag) return;
         const VT* sz = reinterpret_cast<const VT*>(lds_raw + lp.sz);
         for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x)
             put<XT, MODE>(x, u == threadIdx.x ? x0 : P.uxidx[ub + u], XT(sz[u]));
@@ -594,7 +614,8 @@ __global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, InSrc<VT, 
         if (k < Md) {
             const uint32_t q = P.nodes + P.nflat + k;
             const uint32_t j = P.qrow[q];
-            put<XT, MODE>(x, j, XT(dval[k] * fetch(f, q, j)));
+            const XT v = XT(dval[k] * fetch(f, q, j));
+            if (!gflag) put<XT, MODE>(x, j, v);
         }
     }
 }
