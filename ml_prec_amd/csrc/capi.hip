@@ -212,8 +212,10 @@ template <class VT> struct DevHier : DevPrecBase {
 
     // Launch shape of the component blocks: `lbs` threads per block and at
     // most `cap` nodes per block (a larger component gets a block of its own).
-    // MLP_LEVEL_BLOCK / MLP_LEVEL_CAP override the defaults for tuning.
-    uint32_t lbs = kLBS, cap0 = 256;
+    // MLP_LEVEL_BLOCK / MLP_LEVEL_CAP override the defaults for tuning, and
+    // MLP_LEVEL_LDS lowers the LDS budget of a block (smaller budgets send
+    // more blocks to the global-memory sweeps).
+    uint32_t lbs = kLBS, cap0 = 256, lds_budget = kLdsBudget;
 
     void read_launch_shape() {
         if (const char* e = getenv("MLP_LEVEL_BLOCK")) {
@@ -226,17 +228,42 @@ template <class VT> struct DevHier : DevPrecBase {
             if (v < 8 || v > 4096) throw Error("MLP_LEVEL_CAP must lie in [8, 4096]", 1);
             cap0 = v;
         }
+        if (const char* e = getenv("MLP_LEVEL_LDS")) {
+            const uint32_t v = uint32_t(atoi(e));
+            if (v < 1024 || v > kLdsBudget) throw Error("MLP_LEVEL_LDS must lie in [1024, 61440]", 1);
+            lds_budget = v;
+        }
     }
 
-    // plan a level with the largest block size whose LDS needs fit the budget
+    // LDS bytes of the kernels' worst case over the LDS-resident blocks of P
+    uint32_t lds_need(uint32_t nodes, uint32_t lent, uint32_t uent, uint32_t unodes, uint32_t levels) const {
+        if (nodes <= lbs) return 2 * ((nodes * uint32_t(sizeof(VT)) + 7u) & ~7u);   // owned-node path: w and x
+        return lds_plan<VT>(nodes, lent, uent, unodes, levels, true, true).total;
+    }
+
+    // Plan a level with the largest packing whose LDS needs fit the budget.
+    // A component that does not fit alone runs from global memory; if the
+    // LDS-resident blocks fit one at a time but not together (the kernels size
+    // LDS for the maxima), every block beyond the owned-node path does.
     void plan_fitting(const LevelArrays& la, LevelProgram& P) const {
+        const BlockFits alone = [this](uint32_t nodes, uint32_t lent, uint32_t uent, uint32_t unodes, uint32_t lev) {
+            return lds_need(nodes, lent, uent, unodes, lev) <= lds_budget;
+        };
+        auto together = [&]() {
+            return lds_need(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels) <= lds_budget;
+        };
         for (uint32_t cap = cap0;; cap /= 2) {
-            plan_level(la, cap, P);
-            const LdsPlan s = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true);
-            if (s.total <= kLdsBudget || cap <= 8) break;
+            plan_level(la, cap, P, alone);
+            if (together() || cap <= 8) break;
         }
-        const LdsPlan s = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true);
-        if (s.total > 64 * 1024) throw Error("an elimination component is too large for LDS", 1);
+        if (!together()) {
+            const uint32_t b = lbs;
+            plan_level(la, std::min(cap0, b), P,
+                       [b, &alone](uint32_t nodes, uint32_t lent, uint32_t uent, uint32_t unodes, uint32_t lev) {
+                           return nodes <= b && alone(nodes, lent, uent, unodes, lev);
+                       });
+        }
+        if (!together()) throw Error("level blocks do not fit LDS", 1);
     }
 
     void build(const std::vector<mlp_level_desc>& d, int kind_, mlp_mat A0_) {
@@ -299,6 +326,7 @@ template <class VT> struct DevHier : DevPrecBase {
                 G.Uval = data->upload_as<VT>(P.Uval, s);
                 G.Udiag = data->upload_as<VT>(P.Udiag, s);
                 L.wt = scratch.alloc<VT>(P.nodes);
+                if (P.has_global) G.gx = scratch.alloc<VT>(P.nodes);
             }
             if (P.max_nodes <= lbs && P.max_Unodes <= lbs) {
                 // every block takes the owned-node path, which keeps only w (sw) and x (sx) in LDS
@@ -365,9 +393,11 @@ template <class VT> struct DevHier : DevPrecBase {
         h->dynamic = dynamic;
         h->lbs = lbs;
         h->cap0 = cap0;
+        h->lds_budget = lds_budget;
         DevMem& sc = h->scratch;
         for (auto& L : h->lv) {
             if (L.wt) L.wt = sc.template alloc<VT>(std::max<uint32_t>(L.prog.nodes, 1));
+            if (L.prog.gx) L.prog.gx = sc.template alloc<VT>(L.prog.nodes);
             if (L.F) {
                 L.F = sc.template alloc<VT>(L.n);
                 L.u1 = sc.template alloc<VT>(L.n);

@@ -215,7 +215,12 @@ template <class VT> struct ProgView {
     const VT* Uval = nullptr;
     const VT* Udiag = nullptr;
     const uint32_t* flatq = nullptr;   // F slot written by flat row i
+    VT* gx = nullptr;                  // x of the global-memory blocks (nodes long; scratch of the instance)
 };
+
+// header word 1 flag of a block whose values stay in global memory (a
+// component too large for LDS)
+constexpr uint32_t kGlobalBlk = 0x80000000u;
 
 // Value of kept row r of the level output, produced by the iteration on the
 // next level (cheb_solve.cc:61-104, multi_split_solve.cc:106-108, 131-133).
@@ -471,6 +476,55 @@ __device__ __forceinline__ VT owned_up(const ProgView<VT>& P, const uint32_t* B,
     return z;
 }
 
+// Sweeps of a block too large for LDS: the same depth levels and per-row
+// arithmetic as block_down / block_up, with every value in global memory
+// (writes of one depth level are read by the next after the workgroup
+// barrier).  global_down leaves w of every node in gw[base + p].
+template <class VT, class FT, int IN>
+__device__ __forceinline__ void global_down(const ProgView<VT>& P, const uint32_t* B, const InSrc<VT, FT, IN>& f,
+                                            VT* gw) {
+    const uint32_t base = B[0], S = B[1] & ~kGlobalBlk, nl = B[3] - B[2];
+    for (uint32_t p = threadIdx.x; p < S; p += blockDim.x) gw[base + p] = fetch_at(P, f, base + p);
+    __syncthreads();
+    uint32_t s = 0;
+    for (uint32_t li = 0; li < nl; li++) {
+        const uint32_t e = P.lvlL[B[2] + li];
+        for (uint32_t p = s + threadIdx.x; p < e; p += blockDim.x) {
+            VT acc = gw[base + p];
+            for (uint32_t q = P.Lptr[base + p]; q < P.Lptr[base + p + 1]; q++) acc -= P.Lval[q] * gw[base + P.Lloc[q]];
+            gw[base + p] = acc;
+        }
+        s = e;
+        __syncthreads();
+    }
+}
+
+// U sweep of a global-memory block: w of a row from ws[base + loc], solved
+// values into P.gx[base + loc] (ws may be P.gx itself: a row reads its own w
+// before it writes its x there, and other rows read only solved columns).
+template <class VT, int OP>
+__device__ __forceinline__ void global_up(const ProgView<VT>& P, const uint32_t* B, const VT* ws,
+                                          const Combine<VT>& C) {
+    VT* gx = P.gx;
+    const uint32_t base = B[0], ub = B[10], nl = B[5] - B[4];
+    uint32_t s = 0;
+    for (uint32_t li = 0; li < nl; li++) {
+        const uint32_t e = P.lvlU[B[4] + li];
+        for (uint32_t u = s + threadIdx.x; u < e; u += blockDim.x) {
+            const uint32_t loc = P.uloc[ub + u];
+            VT z = ws[base + loc];
+            for (uint32_t q = P.Uptr[ub + u]; q < P.Uptr[ub + u + 1]; q++) {
+                const uint32_t c = P.Ucol[q];
+                const VT xv = (c & kGlobalBit) ? combine<VT, OP>(C, c & ~kGlobalBit) : gx[base + c];
+                z -= P.Uval[q] * xv;
+            }
+            gx[base + loc] = z / P.Udiag[ub + u];
+        }
+        s = e;
+        __syncthreads();
+    }
+}
+
 template <class XT, int MODE>
 __device__ __forceinline__ void put(XT* x, uint32_t j, XT v) {
     if (MODE == 0) x[j] = v;
@@ -506,7 +560,16 @@ __global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, InSrc<VT, F
         extern __shared__ __align__(16) unsigned char lds_raw[];
         const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, false);
         const BlkHdr h = load_hdr(P.blk);
-        const uint32_t base = h.w[0], S = h.w[1];
+        const uint32_t base = h.w[0], S = h.w[1] & ~kGlobalBlk;
+        if (h.w[1] & kGlobalBlk) {
+            global_down(P, h.w, f, wt);
+            if (gflag) return;
+            for (uint32_t p = threadIdx.x; p < S; p += blockDim.x) {
+                const uint32_t o = P.out[base + p];
+                if (o & kGlobalBit) F[o & ~kGlobalBit] = wt[base + p];
+            }
+            return;
+        }
         // the store targets of this thread's first node, fetched ahead of the sweep
         const uint32_t o0 = threadIdx.x < S ? P.out[base + threadIdx.x] : 0u;
         if (S <= blockDim.x) {
@@ -556,6 +619,13 @@ __global__ void __launch_bounds__(kLBS) k_level_up(ProgView<VT> P, const VT* __r
         const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, false, true);
         const BlkHdr h = load_hdr(P.blk);
         const uint32_t ub = h.w[10], nu = h.w[11];
+        if (h.w[1] & kGlobalBlk) {
+            global_up<VT, OP>(P, h.w, wt, C);
+            if (gflag) return;
+            for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x)
+                put<XT, UMODE>(x, P.uxidx[ub + u], XT(P.gx[h.w[0] + P.uloc[ub + u]]));
+            return;
+        }
         const uint32_t x0 = threadIdx.x < nu ? P.uxidx[ub + threadIdx.x] : 0u;
         if (nu <= blockDim.x) {
             const VT v = owned_up<VT, OP, false>(P, h.w, wt, C, nullptr, reinterpret_cast<VT*>(lds_raw + lp.sx));
@@ -592,6 +662,15 @@ __global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, InSrc<VT, 
         const LdsPlan lp = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true);
         const BlkHdr h = load_hdr(P.blk);
         const uint32_t ub = h.w[10], nu = h.w[11];
+        if (h.w[1] & kGlobalBlk) {
+            Combine<VT> none;
+            global_down(P, h.w, f, P.gx);
+            global_up<VT, OP_NONE>(P, h.w, P.gx, none);
+            if (gflag) return;
+            for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x)
+                put<XT, MODE>(x, P.uxidx[ub + u], XT(P.gx[h.w[0] + P.uloc[ub + u]]));
+            return;
+        }
         const uint32_t x0 = threadIdx.x < nu ? P.uxidx[ub + threadIdx.x] : 0u;
         if (h.w[1] <= blockDim.x && nu <= blockDim.x) {
             VT* sw = reinterpret_cast<VT*>(lds_raw + lp.sw);

@@ -41,7 +41,7 @@ void finish_order(const LevelArrays& lv, LevelProgram& P) {
 
 }  // namespace
 
-void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
+void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P, const BlockFits& fits) {
     const uint32_t N = lv.N, M = lv.M, Md = lv.Md, K = N - Md, n = N - M - Md;
     P = LevelProgram();
     P.M = M; P.n = n; P.Md = Md; P.N = N;
@@ -105,7 +105,8 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
     std::vector<uint32_t> csize(ncomp, 0);
     for (uint32_t i = 0; i < K; i++) if (is_node[i]) csize[comp_of[i]]++;
     P.max_comp_size = *std::max_element(csize.begin(), csize.end());
-    cap = std::max(cap, P.max_comp_size);
+    // components in order, a new block whenever the next one would overflow
+    // cap (so a component larger than cap sits alone)
     std::vector<uint32_t> comp_blk(ncomp);
     uint32_t nblk = 0, fill = 0;
     for (uint32_t c = 0; c < ncomp; c++) {
@@ -180,11 +181,17 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P) {
         }
         B[5] = uint32_t(P.lvlU.size());
         B[9] = uint32_t(P.Ucol.size());
-        P.max_nodes = std::max<uint32_t>(P.max_nodes, uint32_t(lo.size()));
-        P.max_Unodes = std::max<uint32_t>(P.max_Unodes, uint32_t(uo.size()));
-        P.max_Lent = std::max<uint32_t>(P.max_Lent, B[7] - B[6]);
-        P.max_Uent = std::max<uint32_t>(P.max_Uent, B[9] - B[8]);
-        P.max_levels = std::max<uint32_t>(P.max_levels, std::max(B[3] - B[2], B[5] - B[4]));
+        const uint32_t nlev = std::max(B[3] - B[2], B[5] - B[4]);
+        if (fits && !fits(uint32_t(lo.size()), B[7] - B[6], B[9] - B[8], uint32_t(uo.size()), nlev)) {
+            B[1] |= kGlobalBlock;
+            P.has_global = true;
+        } else {
+            P.max_nodes = std::max<uint32_t>(P.max_nodes, uint32_t(lo.size()));
+            P.max_Unodes = std::max<uint32_t>(P.max_Unodes, uint32_t(uo.size()));
+            P.max_Lent = std::max<uint32_t>(P.max_Lent, B[7] - B[6]);
+            P.max_Uent = std::max<uint32_t>(P.max_Uent, B[9] - B[8]);
+            P.max_levels = std::max<uint32_t>(P.max_levels, nlev);
+        }
         pos += uint32_t(lo.size());
         upos += uint32_t(uo.size());
     }

@@ -12,6 +12,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 namespace mlp {
@@ -26,6 +27,7 @@ struct LevelArrays {
 };
 
 constexpr uint32_t kGlobalRef = 0x80000000u;   // refers to a kept row (index into the next level)
+constexpr uint32_t kGlobalBlock = 0x80000000u; // header word 1 flag: the block's values live in global memory
 constexpr uint32_t kBlkFields = 12;
 
 // Component-blocked program of one level.
@@ -33,7 +35,7 @@ constexpr uint32_t kBlkFields = 12;
 // Nodes are the eliminated rows (i < M) and the kept rows with L entries
 // ("roots", M <= i < N-Md).  Rows that depend on each other form small
 // connected components; whole components are packed into blocks of at most
-// `cap` nodes.  Inside a block the nodes are ordered by L depth (a node's L
+// `cap` nodes, and a component larger than `cap` gets a block of its own.  Inside a block the nodes are ordered by L depth (a node's L
 // columns lie in earlier depth levels), and the eliminated rows are listed a
 // second time ordered by U depth.  A block runs its depth levels with a
 // barrier between levels.
@@ -41,6 +43,11 @@ constexpr uint32_t kBlkFields = 12;
 // The kernels read the level input in "consumption order": position q runs
 // over the block nodes, then the kept rows without L entries (flat), then
 // the isolated rows; qrow[q] is the level row read at q.
+//
+// A block that the `fits` predicate of plan_level rejects (a component too
+// large for the workgroup's LDS) keeps its values in global memory instead:
+// kGlobalBlock is set in its header word 1, and the LDS maxima (max_nodes,
+// ...) cover the other blocks only.
 //
 // Block b header blk[12b ..]: [0] first node, [1] nodes, [2..3] its L depth
 // levels in lvlL (local end offsets), [4..5] its U levels in lvlU, [6..7] its
@@ -51,6 +58,7 @@ struct LevelProgram {
     uint32_t nblk = 0, nodes = 0;
     uint32_t max_nodes = 0, max_Lent = 0, max_Uent = 0, max_Unodes = 0, max_levels = 0;
     uint32_t max_depth = 0, max_comp_size = 0;
+    bool has_global = false;             // some block runs from global memory (the maxima exclude it)
     std::vector<uint32_t> blk;           // kBlkFields per block
     std::vector<uint32_t> lvlL, lvlU;
     std::vector<uint32_t> qrow;          // N: level row read at consumption position q
@@ -85,7 +93,10 @@ struct SellLayout {
     std::vector<double> val;
 };
 
-void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& out);
+// Whether a block with these sizes (nodes, L entries, U entries, U-list rows,
+// depth levels) runs from LDS.
+using BlockFits = std::function<bool(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t)>;
+void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& out, const BlockFits& fits = nullptr);
 void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, SellLayout& out,
                const uint32_t* rows = nullptr);
 // Makes level l write its kept-row right-hand side F in the consumption order
