@@ -176,6 +176,7 @@ template <class VT> struct DevLevel {
     uint32_t* map = nullptr;
     ProgView<VT> prog;
     uint32_t lds_down = 0, lds_up = 0, lds_solve = 0;
+    uint32_t lbs = kLBS;                       // threads per block of the level kernels (64 or kLBS)
     VT* dval = nullptr;
     DevSell<VT> A;                             // this level's matrix (levels >= 1); uh_cheb stores its rows
                                                // in the order the level kernels read them
@@ -210,23 +211,26 @@ template <class VT> struct DevHier : DevPrecBase {
 
     hipStream_t stream() const { return ctx->stream; }
 
-    // Launch shape of the component blocks: `lbs` threads per block and at
-    // most `cap` nodes per block (a larger component gets a block of its own).
-    // MLP_LEVEL_BLOCK / MLP_LEVEL_CAP override the defaults for tuning, and
-    // MLP_LEVEL_LDS lowers the LDS budget of a block (smaller budgets send
-    // more blocks to the global-memory sweeps).
-    uint32_t lbs = kLBS, cap0 = 256, lds_budget = kLdsBudget;
+    // Launch shape of the component blocks of a level: `lbs` threads per block
+    // and at most `cap` nodes per block (a larger component gets a block of its
+    // own).  By default a level whose elimination components all fit one
+    // wavefront runs 64-thread blocks of at most 64 nodes (no workgroup
+    // barriers in its sweeps), any other level 256-thread blocks of at most
+    // 256 nodes.  MLP_LEVEL_BLOCK / MLP_LEVEL_CAP fix the shape of every level
+    // for tuning (the cap defaults to 256 then), and MLP_LEVEL_LDS lowers the LDS budget of a block (smaller
+    // budgets send more blocks to the global-memory sweeps).
+    uint32_t env_lbs = 0, env_cap = 0, lds_budget = kLdsBudget;
 
     void read_launch_shape() {
         if (const char* e = getenv("MLP_LEVEL_BLOCK")) {
             const uint32_t v = uint32_t(atoi(e));
             if (v != 64 && v != 128 && v != 256) throw Error("MLP_LEVEL_BLOCK must be 64, 128 or 256", 1);
-            lbs = v;
+            env_lbs = v;
         }
         if (const char* e = getenv("MLP_LEVEL_CAP")) {
             const uint32_t v = uint32_t(atoi(e));
             if (v < 8 || v > 4096) throw Error("MLP_LEVEL_CAP must lie in [8, 4096]", 1);
-            cap0 = v;
+            env_cap = v;
         }
         if (const char* e = getenv("MLP_LEVEL_LDS")) {
             const uint32_t v = uint32_t(atoi(e));
@@ -236,34 +240,50 @@ template <class VT> struct DevHier : DevPrecBase {
     }
 
     // LDS bytes of the kernels' worst case over the LDS-resident blocks of P
-    uint32_t lds_need(uint32_t nodes, uint32_t lent, uint32_t uent, uint32_t unodes, uint32_t levels) const {
+    static uint32_t lds_need(uint32_t lbs, uint32_t nodes, uint32_t lent, uint32_t uent, uint32_t unodes,
+                             uint32_t levels) {
         if (nodes <= lbs) return 2 * ((nodes * uint32_t(sizeof(VT)) + 7u) & ~7u);   // owned-node path: w and x
         return lds_plan<VT>(nodes, lent, uent, unodes, levels, true, true).total;
     }
 
-    // Plan a level with the largest packing whose LDS needs fit the budget.
-    // A component that does not fit alone runs from global memory; if the
-    // LDS-resident blocks fit one at a time but not together (the kernels size
-    // LDS for the maxima), every block beyond the owned-node path does.
-    void plan_fitting(const LevelArrays& la, LevelProgram& P) const {
-        const BlockFits alone = [this](uint32_t nodes, uint32_t lent, uint32_t uent, uint32_t unodes, uint32_t lev) {
-            return lds_need(nodes, lent, uent, unodes, lev) <= lds_budget;
+    // Plan a level for blocks of `lbs` threads with the largest packing (at
+    // most cap0 nodes) whose LDS needs fit the budget.  A component that does
+    // not fit alone runs from global memory; if the LDS-resident blocks fit one
+    // at a time but not together (the kernels size LDS for the maxima), every
+    // block beyond the owned-node path does.
+    void plan_fitting(const LevelArrays& la, LevelProgram& P, uint32_t lbs, uint32_t cap0) const {
+        const uint32_t budget = lds_budget;
+        const BlockFits alone = [lbs, budget](uint32_t nodes, uint32_t lent, uint32_t uent, uint32_t unodes,
+                                              uint32_t lev) {
+            return lds_need(lbs, nodes, lent, uent, unodes, lev) <= budget;
         };
         auto together = [&]() {
-            return lds_need(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels) <= lds_budget;
+            return lds_need(lbs, P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels) <= budget;
         };
         for (uint32_t cap = cap0;; cap /= 2) {
             plan_level(la, cap, P, alone);
             if (together() || cap <= 8) break;
         }
         if (!together()) {
-            const uint32_t b = lbs;
-            plan_level(la, std::min(cap0, b), P,
-                       [b, &alone](uint32_t nodes, uint32_t lent, uint32_t uent, uint32_t unodes, uint32_t lev) {
-                           return nodes <= b && alone(nodes, lent, uent, unodes, lev);
+            plan_level(la, std::min(cap0, lbs), P,
+                       [lbs, &alone](uint32_t nodes, uint32_t lent, uint32_t uent, uint32_t unodes, uint32_t lev) {
+                           return nodes <= lbs && alone(nodes, lent, uent, unodes, lev);
                        });
         }
         if (!together()) throw Error("level blocks do not fit LDS", 1);
+    }
+
+    // the launch shape of one level (see env_lbs) and its program
+    uint32_t plan_shaped(const LevelArrays& la, LevelProgram& P) const {
+        if (env_lbs || env_cap) {
+            const uint32_t lbs = env_lbs ? env_lbs : kLBS;
+            plan_fitting(la, P, lbs, env_cap ? env_cap : kLBS);
+            return lbs;
+        }
+        plan_fitting(la, P, 64, 64);
+        if (P.max_nodes <= 64 && !P.has_global) return 64;
+        plan_fitting(la, P, kLBS, kLBS);
+        return kLBS;
     }
 
     void build(const std::vector<mlp_level_desc>& d, int kind_, mlp_mat A0_) {
@@ -286,7 +306,7 @@ template <class VT> struct DevHier : DevPrecBase {
             la.N = D.N; la.M = D.M; la.Md = D.Md; la.map = D.map;
             la.L_ia = D.L_ia; la.L_ja = D.L_ja; la.L_a = D.L_a;
             la.U_ia = D.U_ia; la.U_ja = D.U_ja; la.U_a = D.U_a; la.dval = D.dval;
-            plan_fitting(la, progs[l]);
+            lv[l].lbs = plan_shaped(la, progs[l]);
         }
         if (kind == MLP_PREC_UH_CHEB)
             for (uint32_t l = 0; l + 1 < nlevels; l++) link_levels(progs[l], progs[l + 1]);
@@ -328,7 +348,7 @@ template <class VT> struct DevHier : DevPrecBase {
                 L.wt = scratch.alloc<VT>(P.nodes);
                 if (P.has_global) G.gx = scratch.alloc<VT>(P.nodes);
             }
-            if (P.max_nodes <= lbs && P.max_Unodes <= lbs) {
+            if (P.max_nodes <= L.lbs && P.max_Unodes <= L.lbs) {
                 // every block takes the owned-node path, which keeps only w (sw) and x (sx) in LDS
                 const uint32_t v = (P.max_nodes * uint32_t(sizeof(VT)) + 7u) & ~7u;
                 L.lds_down = v;
@@ -391,8 +411,8 @@ template <class VT> struct DevHier : DevPrecBase {
         h->data = data;
         h->A0 = A0;
         h->dynamic = dynamic;
-        h->lbs = lbs;
-        h->cap0 = cap0;
+        h->env_lbs = env_lbs;
+        h->env_cap = env_cap;
         h->lds_budget = lds_budget;
         DevMem& sc = h->scratch;
         for (auto& L : h->lv) {
@@ -421,20 +441,33 @@ template <class VT> struct DevHier : DevPrecBase {
 
     template <class FT, int IN, class XT, int DMODE>
     void down(const DevLevel<VT>& L, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
-        const uint32_t grid = L.prog.nblk + cdiv(uint64_t(L.prog.nflat) + L.Md, lbs);
-        launch(k_level_down<VT, FT, IN, XT, DMODE>, grid, lbs, L.lds_down, L.prog, f, L.wt, L.F, L.Md,
-               (const VT*)L.dval, x, g);
+        const uint32_t grid = L.prog.nblk + cdiv(uint64_t(L.prog.nflat) + L.Md, L.lbs);
+        if (L.lbs == 64)
+            launch(k_level_down<VT, FT, IN, XT, DMODE, 64>, grid, L.lbs, L.lds_down, L.prog, f, L.wt, L.F, L.Md,
+                   (const VT*)L.dval, x, g);
+        else
+            launch(k_level_down<VT, FT, IN, XT, DMODE, kLBS>, grid, L.lbs, L.lds_down, L.prog, f, L.wt, L.F, L.Md,
+                   (const VT*)L.dval, x, g);
     }
     template <class XT, int OP, int UMODE>
     void up(const DevLevel<VT>& L, const Combine<VT>& C, XT* x, const uint32_t* g) const {
-        const uint32_t grid = L.prog.nblk + cdiv(L.n, lbs);
-        launch(k_level_up<VT, XT, OP, UMODE>, grid, lbs, L.lds_up, L.prog, (const VT*)L.wt, C,
-               (const uint32_t*)(L.map + L.M), L.n, x, g);
+        const uint32_t grid = L.prog.nblk + cdiv(L.n, L.lbs);
+        if (L.lbs == 64)
+            launch(k_level_up<VT, XT, OP, UMODE, 64>, grid, L.lbs, L.lds_up, L.prog, (const VT*)L.wt, C,
+                   (const uint32_t*)(L.map + L.M), L.n, x, g);
+        else
+            launch(k_level_up<VT, XT, OP, UMODE, kLBS>, grid, L.lbs, L.lds_up, L.prog, (const VT*)L.wt, C,
+                   (const uint32_t*)(L.map + L.M), L.n, x, g);
     }
     template <class FT, int IN, class XT, int MODE>
     void solve_last(const DevLevel<VT>& L, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
-        const uint32_t grid = L.prog.nblk + cdiv(L.Md, lbs);
-        launch(k_level_solve<VT, FT, IN, XT, MODE>, grid, lbs, L.lds_solve, L.prog, f, L.Md, (const VT*)L.dval, x, g);
+        const uint32_t grid = L.prog.nblk + cdiv(L.Md, L.lbs);
+        if (L.lbs == 64)
+            launch(k_level_solve<VT, FT, IN, XT, MODE, 64>, grid, L.lbs, L.lds_solve, L.prog, f, L.Md,
+                   (const VT*)L.dval, x, g);
+        else
+            launch(k_level_solve<VT, FT, IN, XT, MODE, kLBS>, grid, L.lbs, L.lds_solve, L.prog, f, L.Md,
+                   (const VT*)L.dval, x, g);
     }
 
     static InSrc<VT, VT, IN_PLAIN> plain(const VT* f) {

@@ -193,7 +193,10 @@ __global__ void __launch_bounds__(kBS) k_residual(SellView<VT> A, const XT* __re
 // level programs: triangular sweeps over the eliminated rows
 // (cheb_solve.cc:33-38, 108-122; multi_split_solve.cc:12-53)
 // ---------------------------------------------------------------------------
-constexpr int kLBS = 256;              // threads per block of the level kernels
+constexpr int kLBS = 256;              // largest block of the level kernels
+// The level kernels are instantiated for 64-thread blocks (one wavefront:
+// the compiler drops the workgroup barriers of the depth-level loops, so a
+// level costs an LDS round trip, not a hardware barrier) and for kLBS.
 
 template <class VT> struct ProgView {
     uint32_t nblk = 0, nflat = 0, nodes = 0;
@@ -397,6 +400,16 @@ __device__ __forceinline__ void block_up(const ProgView<VT>& P, const uint32_t* 
     }
 }
 
+// End of one depth level of an owned-node sweep.  A one-wavefront block only
+// needs the compiler not to move LDS accesses across it: the LDS executes a
+// wavefront's accesses in program order, so a value written at one level is
+// seen by every lane that reads it at a later level.
+template <int BS>
+__device__ __forceinline__ void level_sync() {
+    if constexpr (BS <= 64) __builtin_amdgcn_wave_barrier();
+    else __syncthreads();
+}
+
 // Owned-node sweeps, used when a block has no more nodes (U-list rows) than
 // threads: thread p owns node p, reads its right-hand side, depth level, entry
 // range and first two entries straight from global memory once, and at its
@@ -404,12 +417,12 @@ __device__ __forceinline__ void block_up(const ProgView<VT>& P, const uint32_t* 
 // read only by nodes of later levels, after the barrier that ends its own
 // level, so nothing is staged before the sweep.  Returns the thread's solved
 // value (0 for threads without a node); sw holds every node's w afterwards.
-template <class VT, class FT, int IN>
+template <int BS, class VT, class FT, int IN>
 __device__ __forceinline__ VT owned_down(const ProgView<VT>& P, const uint32_t* B, const InSrc<VT, FT, IN>& f,
                                          VT* sw) {
     const uint32_t base = B[0], S = B[1], nl = B[3] - B[2];
     const uint32_t p = threadIdx.x;
-    uint32_t my = nl, q0 = 0, q1 = 0, l0 = 0, l1 = 0;
+    uint32_t my = nl, q0 = 0, q1 = 0, l0 = p, l1 = p;
     VT acc = VT(0), v0 = VT(0), v1 = VT(0);
     if (p < S) {
         my = P.Ldep[base + p];
@@ -419,14 +432,19 @@ __device__ __forceinline__ VT owned_down(const ProgView<VT>& P, const uint32_t* 
         if (q1 > q0) { l0 = P.Lloc[q0]; v0 = P.Lval[q0]; }
         if (q1 > q0 + 1) { l1 = P.Lloc[q0 + 1]; v1 = P.Lval[q0 + 1]; }
     }
+    // absent entries read the thread's own slot and are selected away, so the
+    // level body has no branches on the entry count
+    const bool h0 = q1 > q0, h1 = q1 > q0 + 1, more = q1 > q0 + 2;
     for (uint32_t li = 0; li < nl; li++) {
         if (li == my) {
-            if (q1 > q0) acc -= v0 * sw[l0];
-            if (q1 > q0 + 1) acc -= v1 * sw[l1];
-            for (uint32_t q = q0 + 2; q < q1; q++) acc -= P.Lval[q] * sw[P.Lloc[q]];
+            const VT x0 = sw[l0], x1 = sw[l1];
+            acc = h0 ? acc - v0 * x0 : acc;
+            acc = h1 ? acc - v1 * x1 : acc;
+            if (more)
+                for (uint32_t q = q0 + 2; q < q1; q++) acc -= P.Lval[q] * sw[P.Lloc[q]];
             sw[p] = acc;
         }
-        __syncthreads();
+        level_sync<BS>();
     }
     return acc;
 }
@@ -434,12 +452,12 @@ __device__ __forceinline__ VT owned_down(const ProgView<VT>& P, const uint32_t* 
 // U sweep of the owned-node path.  w of the row comes from sw (after
 // owned_down in the same launch) or from wt; kept columns from the combine op.
 // Solved values land in sx (local L positions); returns the thread's value.
-template <class VT, int OP, bool W_IN_LDS>
+template <int BS, class VT, int OP, bool W_IN_LDS>
 __device__ __forceinline__ VT owned_up(const ProgView<VT>& P, const uint32_t* B, const VT* __restrict__ wt,
                                        const Combine<VT>& C, const VT* sw, VT* sx) {
     const uint32_t base = B[0], ub = B[10], nu = B[11], nl = B[5] - B[4];
     const uint32_t u = threadIdx.x;
-    uint32_t my = nl, q0 = 0, q1 = 0, c0 = 0, c1 = 0, loc = 0;
+    uint32_t my = nl, q0 = 0, q1 = 0, c0 = kGlobalBit, c1 = kGlobalBit, loc = 0;
     VT z = VT(0), d = VT(1), v0 = VT(0), v1 = VT(0), r0 = VT(0), r1 = VT(0);
     if (u < nu) {
         my = P.Udep[ub + u];
@@ -459,19 +477,27 @@ __device__ __forceinline__ VT owned_up(const ProgView<VT>& P, const uint32_t* B,
             if (OP != OP_NONE && (c1 & kGlobalBit)) r1 = combine<VT, OP>(C, c1 & ~kGlobalBit);
         }
     }
+    // kept columns and absent entries read the thread's own slot and are
+    // selected away, so the level body has no branches on the entry kinds
+    const bool h0 = q1 > q0, h1 = q1 > q0 + 1, more = q1 > q0 + 2;
+    const bool g0 = (c0 & kGlobalBit) != 0, g1 = (c1 & kGlobalBit) != 0;
+    const uint32_t a0 = g0 ? loc : c0, a1 = g1 ? loc : c1;
     for (uint32_t li = 0; li < nl; li++) {
         if (li == my) {
-            if (q1 > q0) z -= v0 * ((c0 & kGlobalBit) ? r0 : sx[c0]);
-            if (q1 > q0 + 1) z -= v1 * ((c1 & kGlobalBit) ? r1 : sx[c1]);
-            for (uint32_t q = q0 + 2; q < q1; q++) {
-                const uint32_t c = P.Ucol[q];
-                const VT xv = (c & kGlobalBit) ? combine<VT, OP>(C, c & ~kGlobalBit) : sx[c];
-                z -= P.Uval[q] * xv;
-            }
+            const VT s0 = sx[a0], s1 = sx[a1];
+            const VT x0 = g0 ? r0 : s0, x1 = g1 ? r1 : s1;
+            z = h0 ? z - v0 * x0 : z;
+            z = h1 ? z - v1 * x1 : z;
+            if (more)
+                for (uint32_t q = q0 + 2; q < q1; q++) {
+                    const uint32_t c = P.Ucol[q];
+                    const VT xv = (c & kGlobalBit) ? combine<VT, OP>(C, c & ~kGlobalBit) : sx[c];
+                    z -= P.Uval[q] * xv;
+                }
             z = z / d;
             sx[loc] = z;
         }
-        __syncthreads();
+        level_sync<BS>();
     }
     return z;
 }
@@ -548,8 +574,8 @@ __device__ __forceinline__ BlkHdr load_hdr(const uint32_t* blk) {
 // feed; the remaining blocks move the right-hand side of the other kept rows
 // into F and apply the isolated-node diagonal x[j] = dval * f[j] (DMODE 1
 // adds it to x).
-template <class VT, class FT, int IN, class XT, int DMODE>
-__global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, InSrc<VT, FT, IN> f, VT* __restrict__ wt,
+template <class VT, class FT, int IN, class XT, int DMODE, int BS>
+__global__ void __launch_bounds__(BS) k_level_down(ProgView<VT> P, InSrc<VT, FT, IN> f, VT* __restrict__ wt,
                                                      VT* __restrict__ F, uint32_t Md, const VT* __restrict__ dval,
                                                      XT* __restrict__ x, const uint32_t* guard) {
     // the solve's done flag: loaded now, tested before the first store, so that
@@ -573,7 +599,7 @@ __global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, InSrc<VT, F
         // the store targets of this thread's first node, fetched ahead of the sweep
         const uint32_t o0 = threadIdx.x < S ? P.out[base + threadIdx.x] : 0u;
         if (S <= blockDim.x) {
-            const VT v = owned_down(P, h.w, f, reinterpret_cast<VT*>(lds_raw + lp.sw));
+            const VT v = owned_down<BS>(P, h.w, f, reinterpret_cast<VT*>(lds_raw + lp.sw));
             if (gflag) return;
             if (threadIdx.x < S) {
                 if (o0 & kGlobalBit) F[o0 & ~kGlobalBit] = v;
@@ -606,8 +632,8 @@ __global__ void __launch_bounds__(kLBS) k_level_down(ProgView<VT> P, InSrc<VT, F
 // Backward part of a level: component blocks run the U sweep (kept columns
 // from the combine op) and write their rows of x; the remaining blocks write
 // the kept rows x[map[M+r]] = combine(r) (UMODE 1 adds into x).
-template <class VT, class XT, int OP, int UMODE>
-__global__ void __launch_bounds__(kLBS) k_level_up(ProgView<VT> P, const VT* __restrict__ wt, Combine<VT> C,
+template <class VT, class XT, int OP, int UMODE, int BS>
+__global__ void __launch_bounds__(BS) k_level_up(ProgView<VT> P, const VT* __restrict__ wt, Combine<VT> C,
                                                    const uint32_t* __restrict__ mapM, uint32_t n, XT* __restrict__ x,
                                                    const uint32_t* guard) {
     // the solve's done flag: loaded now, tested before the first store, so that
@@ -628,7 +654,7 @@ __global__ void __launch_bounds__(kLBS) k_level_up(ProgView<VT> P, const VT* __r
         }
         const uint32_t x0 = threadIdx.x < nu ? P.uxidx[ub + threadIdx.x] : 0u;
         if (nu <= blockDim.x) {
-            const VT v = owned_up<VT, OP, false>(P, h.w, wt, C, nullptr, reinterpret_cast<VT*>(lds_raw + lp.sx));
+            const VT v = owned_up<BS, VT, OP, false>(P, h.w, wt, C, nullptr, reinterpret_cast<VT*>(lds_raw + lp.sx));
             if (gflag) return;
             if (threadIdx.x < nu) put<XT, UMODE>(x, x0, XT(v));
             return;
@@ -649,8 +675,8 @@ __global__ void __launch_bounds__(kLBS) k_level_up(ProgView<VT> P, const VT* __r
 
 // A whole level without kept rows (the coarsest level): L sweep, U sweep and
 // diagonal in one launch.  MODE 1 adds the result into x.
-template <class VT, class FT, int IN, class XT, int MODE>
-__global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, InSrc<VT, FT, IN> f, uint32_t Md,
+template <class VT, class FT, int IN, class XT, int MODE, int BS>
+__global__ void __launch_bounds__(BS) k_level_solve(ProgView<VT> P, InSrc<VT, FT, IN> f, uint32_t Md,
                                                       const VT* __restrict__ dval, XT* __restrict__ x,
                                                       const uint32_t* guard) {
     // the solve's done flag: loaded now, tested before the first store, so that
@@ -674,9 +700,10 @@ __global__ void __launch_bounds__(kLBS) k_level_solve(ProgView<VT> P, InSrc<VT, 
         const uint32_t x0 = threadIdx.x < nu ? P.uxidx[ub + threadIdx.x] : 0u;
         if (h.w[1] <= blockDim.x && nu <= blockDim.x) {
             VT* sw = reinterpret_cast<VT*>(lds_raw + lp.sw);
-            owned_down(P, h.w, f, sw);
+            owned_down<BS>(P, h.w, f, sw);
             Combine<VT> none;
-            const VT v = owned_up<VT, OP_NONE, true>(P, h.w, nullptr, none, sw, reinterpret_cast<VT*>(lds_raw + lp.sx));
+            const VT v = owned_up<BS, VT, OP_NONE, true>(P, h.w, nullptr, none, sw,
+                                                         reinterpret_cast<VT*>(lds_raw + lp.sx));
             if (gflag) return;
             if (threadIdx.x < nu) put<XT, MODE>(x, x0, XT(v));
             return;

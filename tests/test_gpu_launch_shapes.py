@@ -4,7 +4,8 @@ A level's component blocks run one of three sweeps, chosen per block when the
 level is planned: owned-node (a block no larger than the workgroup, values in
 LDS), staged (a larger block whose rows and values are staged in LDS) and
 global (a block too large for the LDS budget, values in global memory).  The
-default launch shape on the test matrices takes the owned-node path almost
+default launch shape (64-thread blocks of at most 64 nodes on levels whose
+components all fit, else 256-thread blocks) takes the owned-node path almost
 everywhere; the MLP_LEVEL_BLOCK / MLP_LEVEL_CAP / MLP_LEVEL_LDS knobs move
 blocks onto the other two (on spe10_z12: "staged" puts every block but one on
 the staged path, "global" nearly every block on the global path, "mixed"
@@ -25,7 +26,7 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = {
     "staged": {"MLP_LEVEL_BLOCK": "64"},
-    "global": {"MLP_LEVEL_LDS": "2048"},
+    "global": {"MLP_LEVEL_BLOCK": "256", "MLP_LEVEL_LDS": "2048"},
     "mixed": {"MLP_LEVEL_BLOCK": "64", "MLP_LEVEL_CAP": "512", "MLP_LEVEL_LDS": "24576"},
 }
 
@@ -75,6 +76,7 @@ def test_giant_component_apply(ctx):
 
 def test_global_blocks_pcg_graph(ctx, monkeypatch):
     """PCG through a captured graph with global-memory blocks on every level."""
+    monkeypatch.setenv("MLP_LEVEL_BLOCK", "256")
     monkeypatch.setenv("MLP_LEVEL_LDS", "2048")
     s = system("spe10_z12")
     b = rng(11).standard_normal(s.n)
