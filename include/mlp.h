@@ -240,6 +240,21 @@ int mlp_cheb_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, double lmin, double lmax
 int mlp_simple_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, double* d_x,
                       const mlp_solve_opts* opts, mlp_solve_stats* stats);
 
+/* ---- matrix assembly ---- */
+/* Seven-point matrix of an nx*ny*nz grid (cell i + nx*(j + ny*k)) in Skyline CSR,
+ * SPEMesh::construct_matrix (modules/mesh/spe.cc:69-131, XYZ order): row = diagonal
+ * c + the face values of the present neighbours, then -v for the neighbours z-, y-,
+ * x-, x+, y+, z+, with v = 2/(1/k_a + 1/k_b) / h^2 on the face's axis (kind
+ * MLP_ASM_SPE).  kx = ky per cell, kz = kz_ratio * kx (the reference reads kx, ky,
+ * kz from spe_perm.dat).  MLP_ASM_TPFA scales the harmonic mean by area/spacing and
+ * adds Dirichlet half-cells on x = 0 and x = max to the diagonal.  ia holds n+1
+ * entries, ja and a mlp_spe_nnz(nx, ny, nz); host arrays. */
+#define MLP_ASM_SPE 0
+#define MLP_ASM_TPFA 1
+uint64_t mlp_spe_nnz(uint32_t nx, uint32_t ny, uint32_t nz);
+int mlp_spe_assemble(int kind, uint32_t nx, uint32_t ny, uint32_t nz, const double* kx, double kz_ratio,
+                     double c, double hx, double hy, double hz, uint32_t* ia, uint32_t* ja, double* a);
+
 /* ---- diagnostics (no counterpart in the reference; used for roofline reporting) ---- */
 /* algorithmic HBM bytes of one apply (sum over level visits) */
 int mlp_prec_bytes(mlp_prec p, double* bytes_per_apply, uint32_t* launches_per_apply);

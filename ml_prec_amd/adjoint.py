@@ -48,7 +48,8 @@ class SpeAdjointModel:
 
     def _assemble(self, theta):
         kx = np.exp(np.asarray(theta, dtype=np.float64))
-        ia, ja, a = S.assemble(kx.reshape(self.shape), kind="spe", c=self.c)
+        ia, ja, a = N.spe_assemble(kx.reshape(self.shape), kind="spe", c=self.c, kz_ratio=KZ_RATIO,
+                                   h=(S.HX, S.HY, S.HZ))
         return kx, ia, ja, a
 
     def close(self):

@@ -28,6 +28,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [
     (os.path.join(CSRC, "hierarchy.cc"), False),
     (os.path.join(CSRC, "plan.cc"), False),
+    (os.path.join(CSRC, "assembly.cc"), False),
     (os.path.join(CPP, "ml_prec.cc"), False),
     (os.path.join(CSRC, "capi.hip"), True),
 ]

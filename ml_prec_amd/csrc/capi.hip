@@ -2,6 +2,7 @@
 // preconditioner hierarchies, PCG and GMRES.  See include/mlp.h.
 #include "../../include/mlp.h"
 
+#include "assembly.h"
 #include "hierarchy.h"
 #include "kernels.h"
 #include "mlp_error.h"
@@ -1040,7 +1041,7 @@ int mlp_residual(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, doubl
 // ---- BLAS-1 ----
 // Streaming reductions run a grid-stride loop over at most kRedGrid blocks
 // (8 per CU), which keeps the one-block finalize short.
-constexpr uint32_t kRedGrid = 2048;
+constexpr uint32_t kRedGrid = 1024;
 static_assert(kRedGrid <= kRedBlocks, "partials buffer too small");
 static_assert(2 * kRedGrid <= kPartials / 2, "each half of the partials buffer must hold two values per block");
 static uint32_t red_grid(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n, kBS), kRedGrid)); }
@@ -1460,6 +1461,17 @@ int mlp_time_prec_apply(mlp_ctx ctx, mlp_prec p, const double* f, double* x, uin
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *avg_ms = double(ms) / reps;
+    MLP_API_END
+}
+
+uint64_t mlp_spe_nnz(uint32_t nx, uint32_t ny, uint32_t nz) { return spe_nnz(nx, ny, nz); }
+
+int mlp_spe_assemble(int kind, uint32_t nx, uint32_t ny, uint32_t nz, const double* kx, double kz_ratio,
+                     double c, double hx, double hy, double hz, uint32_t* ia, uint32_t* ja, double* a) {
+    MLP_API_BEGIN
+    require(kx && ia && ja && a, "null argument");
+    require(kind == MLP_ASM_SPE || kind == MLP_ASM_TPFA, "kind must be MLP_ASM_SPE or MLP_ASM_TPFA");
+    spe_assemble(SpeKind(kind), nx, ny, nz, kx, kz_ratio, c, hx, hy, hz, ia, ja, a);
     MLP_API_END
 }
 

@@ -11,9 +11,11 @@
 #include "hierarchy.h"
 
 #include "mlp_error.h"
+#include "par.h"
 #include "../cpp/config_file.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -37,8 +39,12 @@ inline uint32_t skyline_find(const HostCSR& A, uint32_t i, uint32_t j) {
 // (the stable insertion sort psort, cheb_constr.cc:26-36).
 void order_by_magnitude(const double* a, uint32_t n, std::vector<uint32_t>& idx) {
     idx.resize(n);
-    std::iota(idx.begin(), idx.end(), 0u);
-    std::stable_sort(idx.begin(), idx.end(), [a](uint32_t x, uint32_t y) { return std::fabs(a[x]) < std::fabs(a[y]); });
+    for (uint32_t t = 0; t < n; t++) {
+        const double v = std::fabs(a[t]);
+        uint32_t j = t;
+        for (; j > 0 && std::fabs(a[idx[j - 1]]) > v; j--) idx[j] = idx[j - 1];
+        idx[j] = t;
+    }
 }
 
 // Link state for uh_cheb: a removal counter per upper-triangular entry that
@@ -47,10 +53,20 @@ void order_by_magnitude(const double* a, uint32_t n, std::vector<uint32_t>& idx)
 struct TwoSidedLinks {
     const HostCSR* A;
     std::vector<uint32_t> cnt;      // upper entries: counter; lower entries: position of the twin
-    explicit TwoSidedLinks(const HostCSR& M) : A(&M), cnt(M.nnz(), 2u) {
-        for (uint32_t i = 0; i < M.nrow; i++)
-            for (uint32_t p = M.ia[i] + 1; p < M.ia[i + 1]; p++)
-                if (M.ja[p] < i) cnt[p] = skyline_find(M, M.ja[p], i);
+    explicit TwoSidedLinks(const HostCSR& M) : A(&M), cnt(M.nnz()) {
+        std::atomic<bool> asym{false};
+        parallel_for(M.nrow, 4096, [&](uint64_t b, uint64_t e) {
+            for (uint32_t i = uint32_t(b); i < e; i++) {
+                cnt[M.ia[i]] = 2u;
+                for (uint32_t p = M.ia[i] + 1; p < M.ia[i + 1]; p++) {
+                    const uint32_t j = M.ja[p];
+                    const uint32_t t = skyline_find(M, j, i);           // the twin entry (j, i)
+                    if (t == kNone) asym = true;
+                    cnt[p] = j < i ? t : 2u;
+                }
+            }
+        });
+        if (asym) throw Error("uh_cheb needs a structurally symmetric matrix", 1);
     }
     uint32_t slot(uint32_t row, uint32_t p) const { return A->ja[p] < row ? cnt[p] : p; }
     bool present(uint32_t row, uint32_t p) const { return cnt[slot(row, p)] != 0; }
@@ -83,7 +99,10 @@ struct SortedSet {
 };
 
 // Sparse elimination of the first M permuted rows, producing L, U and the
-// Schur complement on the next M..N-Md-1 rows (sparse_lu.cc:6-133).
+// Schur complement on the next M..N-Md-1 rows (sparse_lu.cc:6-133).  Row i
+// only reads U rows k < min(i, M): the eliminated rows run in order, then the
+// kept rows, which read the finished U only, run in parallel chunks whose L
+// and Schur-complement rows are stitched together in row order.
 template <class Links>
 void eliminate(const HostCSR& A, const std::vector<uint32_t>& map, const std::vector<uint32_t>& rmap,
                uint32_t Md, uint32_t M, const Links& links, double beta, const std::vector<double>& aux,
@@ -92,73 +111,119 @@ void eliminate(const HostCSR& A, const std::vector<uint32_t>& map, const std::ve
     L = HostCSR();  L.nrow = L.ncol = nkeep;
     U = HostCSR();  U.nrow = M; U.ncol = nkeep;
     next = HostCSR(); next.nrow = next.ncol = n;
-    L.ja.reserve(2 * size_t(nkeep)); L.a.reserve(2 * size_t(nkeep));
     U.ja.reserve(3 * size_t(M)); U.a.reserve(3 * size_t(M));
-    next.ja.reserve(5 * size_t(n)); next.a.reserve(5 * size_t(n));
 
-    std::vector<int32_t> slot(N, -1);     // column -> position in vals
-    SortedSet cols;
-    std::vector<double> vals;
-    for (uint32_t i = 0; i < nkeep; i++) {
-        for (uint32_t c : cols.v) slot[c] = -1;
-        cols.clear();
-        vals.clear();
+    // one row: its L entries (lja, la), then its U row (i < M) or its Schur-complement row
+    struct Work {
+        std::vector<int32_t> slot;
+        SortedSet cols;
+        std::vector<double> vals;
+    };
+    auto row = [&](uint32_t i, Work& w, std::vector<uint32_t>& lja, std::vector<double>& la,
+                   std::vector<uint32_t>& oja, std::vector<double>& oa) {
+        for (uint32_t c : w.cols.v) w.slot[c] = -1;
+        w.cols.clear();
+        w.vals.clear();
         const uint32_t r = map[i];
-        slot[i] = 0;
-        cols.insert(i);
-        vals.push_back(aux[r]);                          // diagonal: c_i - sum(kept a_ij / beta)
+        w.slot[i] = 0;
+        w.cols.insert(i);
+        w.vals.push_back(aux[r]);                          // diagonal: c_i - sum(kept a_ij / beta)
         for (uint32_t p = A.ia[r] + 1; p < A.ia[r + 1]; p++) {
             if (!links.present(r, p)) continue;
             uint32_t c = rmap[A.ja[p]];
-            slot[c] = int32_t(vals.size());
-            cols.insert(c);
+            w.slot[c] = int32_t(w.vals.size());
+            w.cols.insert(c);
             double z = A.a[p] / beta;
-            vals.push_back(z);
-            vals[0] -= z;
+            w.vals.push_back(z);
+            w.vals[0] -= z;
         }
         const uint32_t lim = std::min(i, M);
-        uint32_t k = cols.v.front();
+        uint32_t k = w.cols.v.front();
         while (k < lim) {
             const double piv = U.a[U.ia[k]];
-            const double lik = vals[slot[k]] / piv;
-            L.ja.push_back(k);
-            L.a.push_back(lik);
+            const double lik = w.vals[w.slot[k]] / piv;
+            lja.push_back(k);
+            la.push_back(lik);
             for (uint32_t p = U.ia[k] + 1; p < U.ia[k + 1]; p++) {
                 uint32_t c = U.ja[p];
-                if (slot[c] != -1) {
-                    vals[slot[c]] -= lik * U.a[p];
+                if (w.slot[c] != -1) {
+                    w.vals[w.slot[c]] -= lik * U.a[p];
                 } else {
-                    slot[c] = int32_t(vals.size());
-                    cols.insert(c);
-                    vals.push_back(-lik * U.a[p]);
+                    w.slot[c] = int32_t(w.vals.size());
+                    w.cols.insert(c);
+                    w.vals.push_back(-lik * U.a[p]);
                 }
             }
-            k = cols.v[cols.upper(k)];
+            k = w.cols.v[w.cols.upper(k)];
         }
-        L.ia.push_back(uint32_t(L.ja.size()));
         if (i < M) {
-            for (size_t t = cols.lower(i); t < cols.v.size(); t++) {
-                U.ja.push_back(cols.v[t]);
-                U.a.push_back(vals[slot[cols.v[t]]]);
+            for (size_t t = w.cols.lower(i); t < w.cols.v.size(); t++) {
+                oja.push_back(w.cols.v[t]);
+                oa.push_back(w.vals[w.slot[w.cols.v[t]]]);
             }
-            U.ia.push_back(uint32_t(U.ja.size()));
         } else {
-            next.ja.push_back(i - M);
-            next.a.push_back(vals[0]);
-            for (size_t t = cols.lower(M); t < cols.v.size(); t++) {
-                uint32_t c = cols.v[t];
+            oja.push_back(i - M);
+            oa.push_back(w.vals[0]);
+            for (size_t t = w.cols.lower(M); t < w.cols.v.size(); t++) {
+                uint32_t c = w.cols.v[t];
                 if (c == i) continue;
-                next.ja.push_back(c - M);
-                next.a.push_back(vals[slot[c]]);
+                oja.push_back(c - M);
+                oa.push_back(w.vals[w.slot[c]]);
             }
-            next.ia.push_back(uint32_t(next.ja.size()));
         }
+    };
+
+    // eliminated rows, in order (each reads the U rows before it)
+    {
+        Work w;
+        w.slot.assign(N, -1);
+        for (uint32_t i = 0; i < M; i++) {
+            row(i, w, L.ja, L.a, U.ja, U.a);
+            L.ia.push_back(uint32_t(L.ja.size()));
+            U.ia.push_back(uint32_t(U.ja.size()));
+        }
+    }
+    // kept rows in parallel chunks
+    struct Chunk {
+        std::vector<uint32_t> lja, nja, lcnt, ncnt;
+        std::vector<double> la, na;
+    };
+    const uint32_t nch = std::max<uint32_t>(1, std::min<uint32_t>(64, (n + 8191) / 8192));
+    std::vector<Chunk> ch(nch);
+    const uint64_t step = (uint64_t(n) + nch - 1) / nch;
+    parallel_for(nch, 1, [&](uint64_t c0, uint64_t c1) {
+        Work w;
+        w.slot.assign(N, -1);
+        for (uint64_t c = c0; c < c1; c++) {
+            Chunk& C = ch[c];
+            const uint32_t b = M + uint32_t(std::min<uint64_t>(n, c * step)), e = M + uint32_t(std::min<uint64_t>(n, (c + 1) * step));
+            for (uint32_t i = b; i < e; i++) {
+                const size_t l0 = C.lja.size(), n0 = C.nja.size();
+                row(i, w, C.lja, C.la, C.nja, C.na);
+                C.lcnt.push_back(uint32_t(C.lja.size() - l0));
+                C.ncnt.push_back(uint32_t(C.nja.size() - n0));
+            }
+        }
+    });
+    size_t ltot = L.ja.size(), ntot = 0;
+    for (const Chunk& C : ch) { ltot += C.lja.size(); ntot += C.nja.size(); }
+    L.ja.reserve(ltot); L.a.reserve(ltot); L.ia.reserve(size_t(nkeep) + 1);
+    next.ja.reserve(ntot); next.a.reserve(ntot); next.ia.reserve(size_t(n) + 1);
+    for (const Chunk& C : ch) {
+        L.ja.insert(L.ja.end(), C.lja.begin(), C.lja.end());
+        L.a.insert(L.a.end(), C.la.begin(), C.la.end());
+        next.ja.insert(next.ja.end(), C.nja.begin(), C.nja.end());
+        next.a.insert(next.a.end(), C.na.begin(), C.na.end());
+        for (uint32_t x : C.lcnt) L.ia.push_back(L.ia.back() + x);
+        for (uint32_t x : C.ncnt) next.ia.push_back(next.ia.back() + x);
     }
 }
 
 std::vector<uint32_t> inverse(const std::vector<uint32_t>& map) {
     std::vector<uint32_t> r(map.size());
-    for (uint32_t i = 0; i < map.size(); i++) r[map[i]] = i;
+    parallel_for(map.size(), 1 << 16, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; i++) r[map[i]] = uint32_t(i);
+    });
     return r;
 }
 
@@ -215,31 +280,47 @@ struct ChebBuilder {
         li.N = A.nrow;
         li.nnz = A.nnz();
         const uint32_t N = li.N;
-        std::vector<int> deg(N, 0);
+        std::vector<int> deg(N);
         TwoSidedLinks links(A);
-        li.aux.assign(N, 0.0);
-        std::vector<uint32_t> order;
-        // drop weak links against the row budget (cheb_constr.cc:62-103, to_remove cheb_prec.h:71-99)
-        for (uint32_t i = 0; i < N; i++) {
-            const uint32_t b = A.ia[i], e = A.ia[i + 1], nrz = e - b - 1;
-            deg[i] += int(nrz);
-            double c = 0.0;
-            for (uint32_t p = b; p < e; p++) c += A.a[p];
-            li.aux[i] = c;
-            order_by_magnitude(A.a.data() + b + 1, nrz, order);
-            double s = 0.0;
-            for (uint32_t t = 0; t < nrz; t++) {
-                const uint32_t p = b + 1 + order[t];
-                const double v = A.a[p];
-                const double share = -2 * v / (c * (li.beta - 1));
-                if (s + share > 1) break;
-                s += share;
-                if (links.mark(i, p)) {
-                    deg[i]--;
-                    deg[A.ja[p]]--;
+        li.aux.resize(N);
+        // drop weak links against the row budget (cheb_constr.cc:62-103, to_remove cheb_prec.h:71-99).
+        // A row's choice depends only on its own values, so the rows choose in
+        // parallel (marked[p]: row i asks to drop the link at its entry p); a
+        // link is removed once both of its rows asked, which lowers both degrees.
+        std::vector<unsigned char> marked(A.nnz());
+        parallel_for(N, 4096, [&](uint64_t r0, uint64_t r1) {
+            std::vector<uint32_t> order;
+            for (uint32_t i = uint32_t(r0); i < r1; i++) {
+                const uint32_t b = A.ia[i], e = A.ia[i + 1], nrz = e - b - 1;
+                double c = 0.0;
+                for (uint32_t p = b; p < e; p++) c += A.a[p];
+                li.aux[i] = c;
+                for (uint32_t p = b; p < e; p++) marked[p] = 0;
+                order_by_magnitude(A.a.data() + b + 1, nrz, order);
+                double s = 0.0;
+                for (uint32_t t = 0; t < nrz; t++) {
+                    const uint32_t p = b + 1 + order[t];
+                    const double v = A.a[p];
+                    const double share = -2 * v / (c * (li.beta - 1));
+                    if (s + share > 1) break;
+                    s += share;
+                    marked[p] = 1;
                 }
             }
-        }
+        });
+        parallel_for(N, 4096, [&](uint64_t r0, uint64_t r1) {
+            for (uint32_t i = uint32_t(r0); i < r1; i++) {
+                int d = int(A.ia[i + 1] - A.ia[i] - 1);
+                for (uint32_t p = A.ia[i] + 1; p < A.ia[i + 1]; p++) {
+                    const uint32_t u = links.slot(i, p);          // the upper entry of the link
+                    const uint32_t twin = u == p ? skyline_find(A, A.ja[p], i) : u;
+                    const uint32_t m = uint32_t(marked[p]) + uint32_t(marked[twin]);
+                    if (m == 2) d--;
+                    if (u == p) links.cnt[p] = 2u - m;
+                }
+                deg[i] = d;
+            }
+        });
         permutation(A, links, deg, li);
         li.rmap = inverse(li.map);
         HostLevel& ln = H.levels[l + 1];

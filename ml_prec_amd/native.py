@@ -114,6 +114,9 @@ EXPORTS = {
     "mlp_prec_bytes": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
     "mlp_time_spmv": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.c_size_t, C.POINTER(C.c_double)]),
     "mlp_time_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(C.c_double)]),
+    "mlp_spe_nnz": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    "mlp_spe_assemble": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, _vp, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_double, _vp, _vp, _vp]),
 }
 
 _lib = None
@@ -459,3 +462,21 @@ def daxpy(ctx, alpha, x: DevArray, y: DevArray):
 
 def dscal(ctx, alpha, x: DevArray):
     check(lib().mlp_dscal(ctx.h, x.n, alpha, x.ptr))
+
+
+def spe_assemble(kx3, kind="spe", c=0.0, kz_ratio=0.1, h=(20.0, 10.0, 2.0)):
+    """Seven-point matrix of a (nz, ny, nx) kx field on the host (mlp_spe_assemble): returns ia, ja, a."""
+    kx3 = np.ascontiguousarray(kx3, dtype=np.float64)
+    if kx3.ndim != 3:
+        raise ValueError("kx must be a (nz, ny, nx) array")
+    nz, ny, nx = kx3.shape
+    k = {"spe": 0, "tpfa": 1}.get(kind)
+    if k is None:
+        raise ValueError(f"unknown kind {kind!r}")
+    nnz = int(lib().mlp_spe_nnz(nx, ny, nz))
+    ia = np.empty(nx * ny * nz + 1, dtype=np.uint32)
+    ja = np.empty(nnz, dtype=np.uint32)
+    a = np.empty(nnz, dtype=np.float64)
+    check(lib().mlp_spe_assemble(k, nx, ny, nz, kx3.ctypes.data, kz_ratio, c, h[0], h[1], h[2],
+                                 ia.ctypes.data, ja.ctypes.data, a.ctypes.data))
+    return ia, ja, a

@@ -124,8 +124,15 @@ def assemble(kx3, kind="tpfa", c=0.0):
     kz3 = 0.1 * kx3
     if kind == "tpfa":
         fx, fy, fz = HY * HZ / HX, HX * HZ / HY, HX * HY / HZ
+
+        def scale(t, f, h):
+            return t * f
     elif kind == "spe":
-        fx, fy, fz = 1.0 / (HX * HX), 1.0 / (HY * HY), 1.0 / (HZ * HZ)
+        # spe.cc:80-86 (the ADD macro): the harmonic mean divided by h*h
+        fx, fy, fz = None, None, None
+
+        def scale(t, f, h):
+            return t / (h * h)
     else:
         raise ValueError(f"unknown kind {kind!r}")
 
@@ -133,9 +140,9 @@ def assemble(kx3, kind="tpfa", c=0.0):
     tx = np.zeros((nz, ny, nx))
     ty = np.zeros((nz, ny, nx))
     tz = np.zeros((nz, ny, nx))
-    tx[:, :, :-1] = _hm(kx3[:, :, :-1], kx3[:, :, 1:]) * fx
-    ty[:, :-1, :] = _hm(kx3[:, :-1, :], kx3[:, 1:, :]) * fy
-    tz[:-1, :, :] = _hm(kz3[:-1], kz3[1:]) * fz
+    tx[:, :, :-1] = scale(_hm(kx3[:, :, :-1], kx3[:, :, 1:]), fx, HX)
+    ty[:, :-1, :] = scale(_hm(kx3[:, :-1, :], kx3[:, 1:, :]), fy, HY)
+    tz[:-1, :, :] = scale(_hm(kz3[:-1], kz3[1:]), fz, HZ)
 
     idx = np.arange(n, dtype=np.int64).reshape(nz, ny, nx)
     # neighbour (column, value, present) in the reference's ascending order
