@@ -6,12 +6,14 @@
 #include "hierarchy.h"
 #include "kernels.h"
 #include "mlp_error.h"
+#include "par.h"
 #include "plan.h"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <atomic>
+#include <exception>
 #include <chrono>
 #include <thread>
 #include <cmath>
@@ -51,7 +53,7 @@ struct DevMem {
         return static_cast<T*>(p);
     }
     // synchronous: the host vectors are usually temporaries
-    template <class T> T* upload(const std::vector<T>& v, hipStream_t s) {
+    template <class T, class Al> T* upload(const std::vector<T, Al>& v, hipStream_t s) {
         T* d = alloc<T>(v.size());
         if (!v.empty()) {
             MLP_HIP_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
@@ -59,9 +61,16 @@ struct DevMem {
         }
         return d;
     }
-    template <class T, class S> T* upload_as(const std::vector<S>& v, hipStream_t s) {
-        std::vector<T> c(v.begin(), v.end());
-        return upload(c, s);
+    template <class T, class S, class Al> T* upload_as(const std::vector<S, Al>& v, hipStream_t s) {
+        if constexpr (std::is_same_v<T, S>) {
+            return upload(v, s);
+        } else {
+            HostVec<T> c(v.size());
+            parallel_for(v.size(), 1 << 16, [&](uint64_t b, uint64_t e) {
+                for (uint64_t i = b; i < e; i++) c[i] = T(v[i]);
+            });
+            return upload(c, s);
+        }
     }
 };
 
@@ -303,12 +312,25 @@ template <class VT> struct DevHier : DevPrecBase {
             const uint32_t n = D.N - D.M - D.Md;
             if (l + 1 == nlevels && n != 0) throw Error("the coarsest level must have no kept rows", 1);
             if (l + 1 < nlevels && d[l + 1].N != n) throw Error("level sizes do not chain", 1);
-            LevelArrays la;
-            la.N = D.N; la.M = D.M; la.Md = D.Md; la.map = D.map;
-            la.L_ia = D.L_ia; la.L_ja = D.L_ja; la.L_a = D.L_a;
-            la.U_ia = D.U_ia; la.U_ja = D.U_ja; la.U_a = D.U_a; la.dval = D.dval;
-            lv[l].lbs = plan_shaped(la, progs[l]);
         }
+        // the levels are planned independently: one thread each
+        std::vector<std::exception_ptr> err(nlevels);
+        parallel_for(nlevels, 1, [&](uint64_t l0, uint64_t l1) {
+            for (uint64_t l = l0; l < l1; l++) {
+                try {
+                    const mlp_level_desc& D = d[l];
+                    LevelArrays la;
+                    la.N = D.N; la.M = D.M; la.Md = D.Md; la.map = D.map;
+                    la.L_ia = D.L_ia; la.L_ja = D.L_ja; la.L_a = D.L_a;
+                    la.U_ia = D.U_ia; la.U_ja = D.U_ja; la.U_a = D.U_a; la.dval = D.dval;
+                    lv[l].lbs = plan_shaped(la, progs[l]);
+                } catch (...) {
+                    err[l] = std::current_exception();
+                }
+            }
+        });
+        for (auto& e : err)
+            if (e) std::rethrow_exception(e);
         if (kind == MLP_PREC_UH_CHEB)
             for (uint32_t l = 0; l + 1 < nlevels; l++) link_levels(progs[l], progs[l + 1]);
         for (uint32_t l = 0; l < nlevels; l++) {

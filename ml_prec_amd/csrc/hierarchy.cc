@@ -236,9 +236,15 @@ struct ChebBuilder {
 
     // Tails first (chains of nodes left with one link), isolated nodes last
     // (cheb_mark.cc:8-92).
-    void permutation(const HostCSR& A, const TwoSidedLinks& links_in, std::vector<int>& deg, HostLevel& li) {
+    void permutation(const HostCSR& A, const TwoSidedLinks& links, std::vector<int>& deg, HostLevel& li) {
         const uint32_t N = A.nrow;
-        TwoSidedLinks links = links_in;
+        // links the tail walks remove, flagged by upper-entry slot (a byte per
+        // entry instead of a copy of the link counters)
+        std::vector<unsigned char> gone(A.nnz(), 0);
+        auto present = [&](uint32_t r, uint32_t p) {
+            const uint32_t u = links.slot(r, p);
+            return links.cnt[u] != 0 && !gone[u];
+        };
         std::vector<uint32_t>& map = li.map;
         map.assign(N, 0);
         std::vector<char> done(N, 0);
@@ -252,8 +258,8 @@ struct ChebBuilder {
                     map[pos++] = cur;
                     done[cur] = 1;
                     for (uint32_t p = A.ia[cur] + 1; p < A.ia[cur + 1]; p++)
-                        if (links.present(cur, p)) { nb = A.ja[p]; break; }
-                    links.remove(cur, nb);
+                        if (present(cur, p)) { nb = A.ja[p]; break; }
+                    gone[skyline_find(A, std::min(cur, nb), std::max(cur, nb))] = 1;
                     deg[cur] = -1;
                     deg[nb]--;
                     cur = nb;

@@ -1,8 +1,11 @@
 #include "plan.h"
 
+#include "par.h"
+
 #include "mlp_error.h"
 
 #include <algorithm>
+#include <atomic>
 #include <numeric>
 #include <string>
 
@@ -217,41 +220,64 @@ void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double*
     S.nslices = (n + 63) / 64;
     S.off64.assign(S.nslices + 1, 0);
     auto row = [&](uint32_t q) { return rows ? rows[q] : q; };
-    for (uint32_t s = 0; s < S.nslices; s++) {
-        uint32_t w = 0;
-        for (uint32_t q = 64 * s; q < std::min(n, 64 * s + 64); q++) w = std::max(w, ia[row(q) + 1] - ia[row(q)]);
-        S.off64[s + 1] = S.off64[s] + w;
-    }
-    S.nent = uint64_t(S.off64[S.nslices]) * 64;
-    S.col.assign(S.nent ? S.nent : 1, 0xffffffffu);
-    S.val.assign(S.nent ? S.nent : 1, 0.0);
-    for (uint32_t q = 0; q < n; q++) {
-        const uint32_t r = row(q), s = q / 64, t = q % 64;
-        for (uint32_t k = 0; k < ia[r + 1] - ia[r]; k++) {
-            uint64_t e = (uint64_t(S.off64[s]) + k) * 64 + t;
-            S.col[e] = ja[ia[r] + k];
-            S.val[e] = a[ia[r] + k];
+    std::vector<uint32_t> width(S.nslices);
+    parallel_for(S.nslices, 1024, [&](uint64_t s0, uint64_t s1) {
+        for (uint64_t s = s0; s < s1; s++) {
+            uint32_t w = 0;
+            for (uint32_t q = uint32_t(64 * s); q < std::min<uint64_t>(n, 64 * s + 64); q++)
+                w = std::max(w, ia[row(q) + 1] - ia[row(q)]);
+            width[s] = w;
         }
-    }
+    });
+    for (uint32_t s = 0; s < S.nslices; s++) S.off64[s + 1] = S.off64[s] + width[s];
+    S.nent = uint64_t(S.off64[S.nslices]) * 64;
     // 16-bit column deltas when the rows are in their own order and every |j - i| fits
+    bool use16 = false;
     if (!rows) {
-        bool fits = true;
-        for (uint32_t r = 0; r < n && fits; r++)
-            for (uint32_t p = ia[r]; p < ia[r + 1]; p++) {
-                const int64_t d = int64_t(ja[p]) - int64_t(r);
-                if (d < -32767 || d > 32767) { fits = false; break; }
-            }
-        if (fits) {
-            S.dcol.assign(S.col.size(), int16_t(-32768));
-            for (uint32_t q = 0; q < n; q++) {
-                const uint32_t s = q / 64, t = q % 64;
-                for (uint32_t k = 0; k < ia[q + 1] - ia[q]; k++) {
+        std::atomic<bool> wide{false};
+        parallel_for(n, 1 << 15, [&](uint64_t r0, uint64_t r1) {
+            for (uint64_t r = r0; r < r1 && !wide.load(std::memory_order_relaxed); r++)
+                for (uint32_t p = ia[r]; p < ia[r + 1]; p++) {
+                    const int64_t d = int64_t(ja[p]) - int64_t(r);
+                    if (d < -32767 || d > 32767) { wide = true; break; }
+                }
+        });
+        use16 = !wide;
+    }
+    const uint64_t cap = S.nent ? S.nent : 1;
+    if (use16) S.dcol.resize(cap);
+    else S.col.resize(cap);
+    S.val.resize(cap);
+    if (!S.nent) {
+        if (use16) S.dcol[0] = int16_t(-32768);
+        else S.col[0] = 0xffffffffu;
+        S.val[0] = 0.0;
+        return;
+    }
+    // every slot of a slice written once: the row's entries, then padding
+    parallel_for(S.nslices, 256, [&](uint64_t s0, uint64_t s1) {
+        for (uint64_t s = s0; s < s1; s++) {
+            const uint32_t w = width[s];
+            for (uint32_t t = 0; t < 64; t++) {
+                const uint64_t q = 64 * s + t;
+                const uint32_t r = q < n ? row(uint32_t(q)) : 0u;
+                const uint32_t len = q < n ? ia[r + 1] - ia[r] : 0u;
+                for (uint32_t k = 0; k < w; k++) {
                     const uint64_t e = (uint64_t(S.off64[s]) + k) * 64 + t;
-                    S.dcol[e] = int16_t(int64_t(ja[ia[q] + k]) - int64_t(q));
+                    if (k < len) {
+                        const uint32_t j = ja[ia[r] + k];
+                        if (use16) S.dcol[e] = int16_t(int64_t(j) - int64_t(q));
+                        else S.col[e] = j;
+                        S.val[e] = a[ia[r] + k];
+                    } else {
+                        if (use16) S.dcol[e] = int16_t(-32768);
+                        else S.col[e] = 0xffffffffu;
+                        S.val[e] = 0.0;
+                    }
                 }
             }
         }
-    }
+    });
 }
 
 }  // namespace mlp

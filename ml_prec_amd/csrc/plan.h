@@ -13,6 +13,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <vector>
 
 namespace mlp {
@@ -80,6 +81,21 @@ struct LevelProgram {
     std::vector<uint32_t> flatq;         // F slot the flat row writes (r, or its position in the next level)
 };
 
+// Allocator whose value-less construct leaves elements uninitialised, so that
+// large host buffers are first written (and their pages first touched) by the
+// threads that fill them.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U> struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <class U> NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    template <class U> void construct(U* p) noexcept { ::new (static_cast<void*>(p)) U; }
+    template <class U, class... Args> void construct(U* p, Args&&... args) {
+        ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+    }
+};
+template <class T> using HostVec = std::vector<T, NoInitAlloc<T>>;
+
 // 64-row slices, column-major inside a slice: entry k of row 64s+t sits at
 // (off64[s] + k) * 64 + t.  Padding has col = 0xffffffff and val = 0.
 // With a row order `rows`, slot q holds matrix row rows[q] (entries and their
@@ -88,9 +104,9 @@ struct SellLayout {
     uint32_t n = 0, nslices = 0;
     uint64_t nent = 0;
     std::vector<uint32_t> off64;          // nslices+1
-    std::vector<uint32_t> col;
-    std::vector<int16_t> dcol;            // column - row, when every entry fits (rows unpermuted); -32768 = padding
-    std::vector<double> val;
+    HostVec<uint32_t> col;                // absent when dcol is used
+    HostVec<int16_t> dcol;                // column - row, when every entry fits (rows unpermuted); -32768 = padding
+    HostVec<double> val;
 };
 
 // Whether a block with these sizes (nodes, L entries, U entries, U-list rows,
