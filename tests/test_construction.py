@@ -162,3 +162,17 @@ def test_config_missing_file_keeps_parameters(tmp_path):
                             order="simple_1", degree_max=2)
     P = O.msplit_prec(s.ia, s.ja, s.a, q=(0.7,), niters=(3,), order="simple_1", degree_max=2)
     assert same_hierarchy(P, H)
+
+
+@pytest.mark.parametrize("threads", ["1", "3", "16"])
+def test_host_threads_do_not_change_the_hierarchy(monkeypatch, threads):
+    """The threaded construction steps (link marking, kept-row elimination) give the
+    same hierarchy for any thread count."""
+    s = system("spe10_z40")
+    ref = N.build_uh_cheb(s.ia, s.ja, s.a)
+    refm = N.build_multi_split(s.ia, s.ja, s.a, q=(0.8,), niters=(5,), order="simple_1", degree_max=2)
+    monkeypatch.setenv("MLP_HOST_THREADS", threads)
+    H = N.build_uh_cheb(s.ia, s.ja, s.a)
+    Hm = N.build_multi_split(s.ia, s.ja, s.a, q=(0.8,), niters=(5,), order="simple_1", degree_max=2)
+    assert same_hierarchy(ref, H)
+    assert same_hierarchy(refm, Hm)
