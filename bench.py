@@ -432,9 +432,10 @@ def main():
                                        "fp32": round(by32 / apply32_ms / 1e6, 1)},
         "t_const_host_s": round(t_const, 3),
         "roofline": {
-            "kernel": "k_spmv (FP64 sliced-ELL SpMV with 16-bit column deltas on the 1.12M-cell matrix; back-to-back "
-                      "launches rotating over 1 GiB of matrix copies, so the matrix streams from HBM; achieved counts "
-                      "the algorithmic 12 B per nonzero, traffic the bytes actually moved)",
+            "kernel": "k_spmv (FP64 SpMV of the 1.12M-cell matrix in the symmetric seven-point layout: diagonal and "
+                      "three lower-neighbour values per row, upper values read from the neighbouring rows; "
+                      "back-to-back launches rotating over 1 GiB of matrix copies, so the matrix streams from HBM; "
+                      "achieved counts the algorithmic CSR bytes (12 B per nonzero), traffic the bytes actually moved)",
             "bound": "hbm",
             "achieved": round(ach, 1),
             "peak": HBM_PEAK_GBS,

@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -84,11 +85,39 @@ template <class VT> struct DevSell {
     SellView<VT> v;
     uint64_t nent = 0;
     uint32_t nnz = 0;
+    bool stencil = false;                // v uses the seven-point layout
+    // bytes of the stored matrix (the layout's arrays)
+    uint64_t stored_bytes() const {
+        if (stencil) return uint64_t(v.n) * (4 * sizeof(VT) + 1);
+        return nent * ((v.dcol ? sizeof(int16_t) : sizeof(uint32_t)) + sizeof(VT)) + uint64_t(v.n / 64 + 2) * 4;
+    }
 };
 
+// A matrix in its own row order is stored in the seven-point layout when it is
+// a symmetric seven-point grid operator (MLP_NO_STENCIL=1 keeps the slices),
+// else as sliced ELL.
 template <class VT>
 void upload_sell(DevMem& mem, hipStream_t s, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
                  DevSell<VT>& out, const uint32_t* rows = nullptr) {
+    const char* off = getenv("MLP_NO_STENCIL");
+    if (!rows && !(off && off[0] == '1')) {
+        StencilLayout T;
+        if (plan_stencil(n, ia, ja, a, T)) {
+            out.v.n = n;
+            out.v.spres = mem.upload(T.pres, s);
+            out.v.sd = mem.upload_as<VT>(T.d, s);
+            out.v.slx = mem.upload_as<VT>(T.lx, s);
+            out.v.sly = mem.upload_as<VT>(T.ly, s);
+            out.v.slz = mem.upload_as<VT>(T.lz, s);
+            out.v.ssy = T.sy;
+            out.v.ssz = T.sz;
+            out.stencil = true;
+            out.nent = 0;
+            out.nnz = ia[n];
+            MLP_HIP_CHECK(hipStreamSynchronize(s));
+            return;
+        }
+    }
     SellLayout L;
     plan_sell(n, ia, ja, a, L, rows);
     out.v.n = n;
@@ -1014,7 +1043,7 @@ int mlp_csr_upload(mlp_ctx ctx, uint32_t n, const uint32_t* ia, const uint32_t* 
         m->ja.assign(ja, ja + m->nnz);
         m->a.assign(a, a + m->nnz);
         upload_sell<double>(m->mem, ctx->stream, n, ia, ja, a, m->A);
-        m->bytes = m->A.nent * (m->A.v.dcol ? 10 : 12) + uint64_t(m->A.v.n / 64 + 2) * 4;
+        m->bytes = m->A.stored_bytes();
     } catch (...) {
         delete m;
         throw;
@@ -1424,8 +1453,28 @@ int mlp_time_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y, uint32_t r
     // launch (cache-warm).  Launches run back to back between two events.
     std::vector<SellView<double>> views{A->A.v};
     std::vector<void*> extra;
-    const size_t mbytes = size_t(A->A.nent) * ((A->A.v.dcol ? sizeof(int16_t) : sizeof(uint32_t)) + sizeof(double));
-    if (flush_bytes && mbytes) {
+    const size_t mbytes = A->A.stored_bytes();
+    // a device copy of n elements of T (freed with the other copies)
+    auto copy_of = [&](const auto* src, size_t n) {
+        using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+        void* p = nullptr;
+        MLP_HIP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        extra.push_back(p);
+        MLP_HIP_CHECK(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+        return static_cast<const T*>(p);
+    };
+    if (flush_bytes && A->A.stencil) {
+        const size_t nc = std::max<size_t>(2, (flush_bytes + mbytes - 1) / mbytes);
+        for (size_t c = 1; c < nc; c++) {
+            SellView<double> v = A->A.v;
+            v.spres = copy_of(A->A.v.spres, A->n);
+            v.sd = copy_of(A->A.v.sd, A->n);
+            v.slx = copy_of(A->A.v.slx, A->n);
+            v.sly = copy_of(A->A.v.sly, A->n);
+            v.slz = copy_of(A->A.v.slz, A->n);
+            views.push_back(v);
+        }
+    } else if (flush_bytes && mbytes) {
         const size_t nc = std::max<size_t>(2, (flush_bytes + mbytes - 1) / mbytes);
         for (size_t c = 1; c < nc; c++) {
             SellView<double> v = A->A.v;

@@ -30,6 +30,10 @@ template <class VT> struct SellView {
     const uint32_t* col = nullptr;
     const int16_t* dcol = nullptr;
     const VT* val = nullptr;
+    // seven-point layout (plan.h:StencilLayout), used instead of the slices when sd is set
+    const uint8_t* spres = nullptr;
+    const VT *sd = nullptr, *slx = nullptr, *sly = nullptr, *slz = nullptr;
+    uint32_t ssy = 0, ssz = 0;
 };
 
 
@@ -125,6 +129,40 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
 // (padding slots hold a valid value 0), then the x gathers, so that a row's
 // loads are in flight before the first product; the sum itself still runs
 // entry by entry and skips padding.
+// The same row sum on the seven-point layout: the diagonal, then z-, y-, x-,
+// x+, y+, z+ (the row's storage order), the upper values read from the
+// neighbouring rows' lower values.  Every load in bounds is issued at once,
+// without waiting for the presence bits, which only select the terms.
+template <class VT, class XT, bool SCALE>
+__device__ __forceinline__ VT stencil_row(const SellView<VT>& A, const XT* __restrict__ x, uint32_t r, VT s) {
+    const uint32_t m = A.spres[r];
+    const uint32_t c[6] = {r - A.ssz, r - A.ssy, r - 1, r + 1, r + A.ssy, r + A.ssz};
+    const bool inb[6] = {r >= A.ssz, r >= A.ssy, r >= 1, r + 1 < A.n, r + A.ssy < A.n, r + A.ssz < A.n};
+    VT v[6];
+    XT xv[6];
+    v[0] = A.slz[r];
+    v[1] = A.sly[r];
+    v[2] = A.slx[r];
+    v[3] = inb[3] ? A.slx[r + 1] : VT(0);
+    v[4] = inb[4] ? A.sly[r + A.ssy] : VT(0);
+    v[5] = inb[5] ? A.slz[r + A.ssz] : VT(0);
+#pragma unroll
+    for (int t = 0; t < 6; t++) xv[t] = inb[t] ? x[c[t]] : XT(0);
+    VT t0 = VT(x[r]);
+    if (SCALE) t0 = t0 * s;
+    VT acc = VT(0);
+    acc += A.sd[r] * t0;
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        if ((m >> t) & 1u) {
+            VT u = VT(xv[t]);
+            if (SCALE) u = u * s;
+            acc += v[t] * u;
+        }
+    }
+    return acc;
+}
+
 template <class VT, class XT, bool SCALE>
 __device__ __forceinline__ VT sell_row(const SellView<VT>& A, const XT* __restrict__ x, uint32_t r, VT s) {
     constexpr uint32_t G = 8;
@@ -166,6 +204,13 @@ __device__ __forceinline__ VT sell_row(const SellView<VT>& A, const XT* __restri
     return acc;
 }
 
+// Row r of a matrix in either layout (the kernels that read a top-level
+// matrix; the level kernels' fused residuals only ever see slices).
+template <class VT, class XT>
+__device__ __forceinline__ VT mat_row(const SellView<VT>& A, const XT* __restrict__ x, uint32_t r) {
+    return A.sd ? stencil_row<VT, XT, false>(A, x, r, VT(1)) : sell_row<VT, XT, false>(A, x, r, VT(1));
+}
+
 // ---------------------------------------------------------------------------
 // SpMV family
 // ---------------------------------------------------------------------------
@@ -175,7 +220,7 @@ __global__ void __launch_bounds__(kBS) k_spmv(SellView<VT> A, const XT* __restri
     if (guarded(guard)) return;
     const uint32_t r = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (r >= A.n) return;
-    y[r] = YT(sell_row<VT, XT, false>(A, x, r, VT(1)));
+    y[r] = YT(mat_row<VT, XT>(A, x, r));
 }
 
 // r = b - A x   (residual, csrmatrix.cc:293-300)
@@ -185,7 +230,7 @@ __global__ void __launch_bounds__(kBS) k_residual(SellView<VT> A, const XT* __re
     if (guarded(guard)) return;
     const uint32_t r = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (r >= A.n) return;
-    VT acc = sell_row<VT, XT, false>(A, x, r, VT(1));
+    VT acc = mat_row<VT, XT>(A, x, r);
     rr[r] = XT(VT(b[r]) - acc);
 }
 
@@ -852,7 +897,7 @@ __global__ void __launch_bounds__(kBS) k_pcg_spmv_dot(SellView<double> A, const 
     for (uint32_t t = xcd_block(blockIdx.x, gridDim.x); t < tiles; t += gridDim.x) {
         const uint32_t r = t * blockDim.x + threadIdx.x;
         if (r < A.n) {
-            const double v = sell_row<double, double, false>(A, p, r, 1.0);
+            const double v = mat_row<double, double>(A, p, r);
             Ap[r] = v;
             s += v * p[r];
         }
@@ -973,7 +1018,7 @@ __global__ void __launch_bounds__(kBS) k_pcg_true_res(SellView<double> A, const 
     for (uint32_t t = xcd_block(blockIdx.x, gridDim.x); t < tiles; t += gridDim.x) {
         const uint32_t i = t * blockDim.x + threadIdx.x;
         if (i < A.n) {
-            const double v = b[i] - sell_row<double, double, false>(A, x, i, 1.0);
+            const double v = b[i] - mat_row<double, double>(A, x, i);
             r[i] = v;
             s += v * v;
         }

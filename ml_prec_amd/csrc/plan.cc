@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstring>
 #include <numeric>
 #include <string>
 
@@ -211,6 +212,59 @@ void link_levels(LevelProgram& P, const LevelProgram& next) {
     for (uint32_t& o : P.out)
         if (o & kGlobalRef) o = kGlobalRef | qpos[o & ~kGlobalRef];
     for (size_t i = 0; i < P.flat.size(); i++) P.flatq[i] = qpos[P.flat[i]];
+}
+
+bool plan_stencil(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, StencilLayout& S) {
+    S = StencilLayout();
+    if (n < 8 || ia[1] - ia[0] != 4 || ja[ia[0]] != 0) return false;
+    // row 0 is a grid corner: its neighbours are +1, +nx, +nx*ny
+    const uint32_t c1 = ja[1], sy = ja[2], sz = ja[3];
+    if (c1 != 1 || sy < 2 || sz <= sy || sz % sy || n % sz) return false;
+    const uint32_t nx = sy, ny = sz / sy, nz = n / sz;
+    if (ny < 2 || nz < 2) return false;
+    S.n = n; S.sy = sy; S.sz = sz;
+    S.pres.resize(n); S.d.resize(n); S.lx.resize(n); S.ly.resize(n); S.lz.resize(n);
+    std::atomic<bool> bad{false};
+    parallel_for(n, 1 << 14, [&](uint64_t r0, uint64_t r1) {
+        for (uint64_t rr = r0; rr < r1 && !bad.load(std::memory_order_relaxed); rr++) {
+            const uint32_t r = uint32_t(rr), i = r % nx, j = (r / nx) % ny, k = r / sz;
+            const uint32_t b = ia[r], e = ia[r + 1];
+            uint32_t want[7], m = 0, cnt = 0;
+            want[cnt++] = r;
+            if (k > 0) { want[cnt++] = r - sz; m |= 1; }
+            if (j > 0) { want[cnt++] = r - sy; m |= 2; }
+            if (i > 0) { want[cnt++] = r - 1; m |= 4; }
+            if (i + 1 < nx) { want[cnt++] = r + 1; m |= 8; }
+            if (j + 1 < ny) { want[cnt++] = r + sy; m |= 16; }
+            if (k + 1 < nz) { want[cnt++] = r + sz; m |= 32; }
+            if (e - b != cnt) { bad = true; break; }
+            for (uint32_t t = 0; t < cnt; t++)
+                if (ja[b + t] != want[t]) { bad = true; break; }
+            S.pres[r] = uint8_t(m);
+            S.d[r] = a[b];
+            uint32_t t = b + 1;
+            S.lz[r] = (m & 1) ? a[t++] : 0.0;
+            S.ly[r] = (m & 2) ? a[t++] : 0.0;
+            S.lx[r] = (m & 4) ? a[t++] : 0.0;
+        }
+    });
+    if (bad) return false;
+    // symmetry: each upper value equals the neighbour's lower value, bit for bit
+    auto same = [](double x, double y) { return std::memcmp(&x, &y, sizeof(double)) == 0; };
+    parallel_for(n, 1 << 14, [&](uint64_t r0, uint64_t r1) {
+        for (uint64_t rr = r0; rr < r1 && !bad.load(std::memory_order_relaxed); rr++) {
+            const uint32_t r = uint32_t(rr), m = S.pres[r];
+            uint32_t t = ia[r] + 1 + ((m & 1) != 0) + ((m & 2) != 0) + ((m & 4) != 0);
+            if ((m & 8) && !same(a[t++], S.lx[r + 1])) bad = true;
+            if ((m & 16) && !same(a[t++], S.ly[r + sy])) bad = true;
+            if ((m & 32) && !same(a[t++], S.lz[r + sz])) bad = true;
+        }
+    });
+    if (bad) {
+        S = StencilLayout();
+        return false;
+    }
+    return true;
 }
 
 void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, SellLayout& S,

@@ -113,6 +113,21 @@ struct SellLayout {
 // depth levels) runs from LDS.
 using BlockFits = std::function<bool(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t)>;
 void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& out, const BlockFits& fits = nullptr);
+
+// Seven-point layout of a symmetric matrix on a full nx*ny*nz grid (every
+// dimension >= 2; row r = i + nx*(j + ny*k), neighbours at r -+ 1, r -+ sy,
+// r -+ sz with sy = nx, sz = nx*ny): per row the diagonal and the values of
+// its three lower neighbours; a row's upper values are its upper neighbours'
+// lower values (the matrix is bitwise symmetric).  pres bits: 1 z-, 2 y-,
+// 4 x-, 8 x+, 16 y+, 32 z+.
+struct StencilLayout {
+    uint32_t n = 0, sy = 0, sz = 0;
+    HostVec<uint8_t> pres;
+    HostVec<double> d, lx, ly, lz;       // lx[r] = a(r, r-1), ly[r] = a(r, r-sy), lz[r] = a(r, r-sz); 0 if absent
+};
+// True (and S filled) when the Skyline CSR matrix has exactly that structure,
+// diagonal first then columns ascending, and a(r, c) == a(c, r) bitwise.
+bool plan_stencil(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, StencilLayout& S);
 void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, SellLayout& out,
                const uint32_t* rows = nullptr);
 // Makes level l write its kept-row right-hand side F in the consumption order
