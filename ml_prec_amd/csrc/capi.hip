@@ -146,6 +146,12 @@ struct mlp_ctx_s {
     double* scal = nullptr;            // 4 result slots for standalone BLAS-1
     mlp::SolveState* st = nullptr;
     double* host_pinned = nullptr;     // 64 doubles
+    // batched right-hand sides (mlp_pcg_multi): R vectors of each kind, R states, R partial buffers
+    mlp::DevMem multi_mem;
+    uint32_t multi_R = 0, multi_n = 0;
+    double *mr = nullptr, *mz = nullptr, *mp = nullptr, *mAp = nullptr, *mzp = nullptr, *mpart = nullptr;
+    mlp::SolveState* mst = nullptr;
+    mlp::SolveState* mst_host = nullptr;   // pinned, R states
     // solver workspace, grown on demand and kept for the next solve
     uint32_t ws_n = 0;
     mlp::DevMem ws_mem;
@@ -154,7 +160,8 @@ struct mlp_ctx_s {
     // instantiated PCG iteration graphs, reused while the same matrix,
     // preconditioner and vectors are solved again
     struct GraphEntry {
-        int solver = 0;                // 0 PCG, 1 Chebyshev, 2 simple
+        int solver = 0;                // 0 PCG, 1 Chebyshev, 2 simple, 3 batched PCG
+        uint32_t R = 1;                // right-hand sides per launch
         uint64_t A, B;
         const void *b, *x;
         int flex;
@@ -233,6 +240,9 @@ struct DevPrecBase {
     virtual bool graph_safe() const { return true; }
     // a copy on another context that shares the device data (nullptr: not supported)
     virtual std::unique_ptr<DevPrecBase> clone_on(mlp_ctx) const { return nullptr; }
+    // a copy that applies to rhs right-hand sides per call, vectors stored one
+    // after another (nullptr: not supported)
+    virtual std::unique_ptr<DevPrecBase> widen(mlp_ctx, uint32_t) const { return nullptr; }
 };
 
 constexpr uint32_t kLdsBudget = 60 * 1024;
@@ -241,6 +251,7 @@ template <class VT> struct DevHier : DevPrecBase {
     mlp_ctx ctx;
     int kind;
     uint32_t nlevels = 0;
+    uint32_t R = 1;                          // right-hand sides per apply (level vectors R times over)
     std::vector<DevLevel<VT>> lv;
     std::shared_ptr<DevMem> data = std::make_shared<DevMem>();   // programs, matrices: shared by clones
     DevMem scratch;                                              // per-instance level vectors
@@ -454,11 +465,12 @@ template <class VT> struct DevHier : DevPrecBase {
 
     // A preconditioner on context c that shares this one's device data and has
     // level vectors of its own, so both can be applied at the same time.
-    std::unique_ptr<DevHier<VT>> clone(mlp_ctx c) const {
+    std::unique_ptr<DevHier<VT>> clone(mlp_ctx c, uint32_t rhs = 1) const {
         auto h = std::make_unique<DevHier<VT>>();
         h->ctx = c;
         h->kind = kind;
         h->nlevels = nlevels;
+        h->R = rhs;
         h->lv = lv;
         h->data = data;
         h->A0 = A0;
@@ -467,28 +479,46 @@ template <class VT> struct DevHier : DevPrecBase {
         h->env_cap = env_cap;
         h->lds_budget = lds_budget;
         DevMem& sc = h->scratch;
+        const size_t k = rhs;
         for (auto& L : h->lv) {
-            if (L.wt) L.wt = sc.template alloc<VT>(std::max<uint32_t>(L.prog.nodes, 1));
-            if (L.prog.gx) L.prog.gx = sc.template alloc<VT>(L.prog.nodes);
+            if (L.wt) L.wt = sc.template alloc<VT>(k * std::max<uint32_t>(L.prog.nodes, 1));
+            if (L.prog.gx) L.prog.gx = sc.template alloc<VT>(k * L.prog.nodes);
             if (L.F) {
-                L.F = sc.template alloc<VT>(L.n);
-                L.u1 = sc.template alloc<VT>(L.n);
-                L.u0 = sc.template alloc<VT>(L.n);
-                L.x2 = sc.template alloc<VT>(L.n);
+                L.F = sc.template alloc<VT>(k * L.n);
+                L.u1 = sc.template alloc<VT>(k * L.n);
+                L.u0 = sc.template alloc<VT>(k * L.n);
+                L.x2 = sc.template alloc<VT>(k * L.n);
             }
-            if (L.r) L.r = sc.template alloc<VT>(L.N);
+            if (L.r) L.r = sc.template alloc<VT>(k * L.N);
         }
         return h;
+    }
+
+    // a copy on context c that applies to `rhs` right-hand sides at once
+    // (uh_cheb; multi_split's dynamic inner loops are per right-hand side)
+    std::unique_ptr<DevPrecBase> widen(mlp_ctx c, uint32_t rhs) const override {
+        if (kind != MLP_PREC_UH_CHEB) return nullptr;
+        return clone(c, rhs);
     }
 
     std::unique_ptr<DevPrecBase> clone_on(mlp_ctx c) const override { return clone(c); }
 
     // --- launch helpers ---
+    // every launch covers the R right-hand sides of a batched apply (grid.y)
     template <class K, class... Args>
     void launch(K kern, uint32_t grid, uint32_t block, uint32_t smem, Args... args) const {
         if (grid == 0) return;
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), smem, ctx->stream, args...);
+        hipLaunchKernelGGL(kern, dim3(grid, R), dim3(block), smem, ctx->stream, args...);
         launches++;
+    }
+    // per-right-hand-side strides of level L's vectors (R independent problems)
+    static LevelBatch level_batch(const DevLevel<VT>& L) {
+        LevelBatch b;
+        b.N = L.N;
+        b.nodes = L.prog.nodes;
+        b.n = L.n;
+        b.st = uint32_t(sizeof(SolveState) / sizeof(uint32_t));
+        return b;
     }
 
     template <class FT, int IN, class XT, int DMODE>
@@ -496,30 +526,30 @@ template <class VT> struct DevHier : DevPrecBase {
         const uint32_t grid = L.prog.nblk + cdiv(uint64_t(L.prog.nflat) + L.Md, L.lbs);
         if (L.lbs == 64)
             launch(k_level_down<VT, FT, IN, XT, DMODE, 64>, grid, L.lbs, L.lds_down, L.prog, f, L.wt, L.F, L.Md,
-                   (const VT*)L.dval, x, g);
+                   (const VT*)L.dval, x, g, level_batch(L));
         else
             launch(k_level_down<VT, FT, IN, XT, DMODE, kLBS>, grid, L.lbs, L.lds_down, L.prog, f, L.wt, L.F, L.Md,
-                   (const VT*)L.dval, x, g);
+                   (const VT*)L.dval, x, g, level_batch(L));
     }
     template <class XT, int OP, int UMODE>
     void up(const DevLevel<VT>& L, const Combine<VT>& C, XT* x, const uint32_t* g) const {
         const uint32_t grid = L.prog.nblk + cdiv(L.n, L.lbs);
         if (L.lbs == 64)
             launch(k_level_up<VT, XT, OP, UMODE, 64>, grid, L.lbs, L.lds_up, L.prog, (const VT*)L.wt, C,
-                   (const uint32_t*)(L.map + L.M), L.n, x, g);
+                   (const uint32_t*)(L.map + L.M), L.n, x, g, level_batch(L));
         else
             launch(k_level_up<VT, XT, OP, UMODE, kLBS>, grid, L.lbs, L.lds_up, L.prog, (const VT*)L.wt, C,
-                   (const uint32_t*)(L.map + L.M), L.n, x, g);
+                   (const uint32_t*)(L.map + L.M), L.n, x, g, level_batch(L));
     }
     template <class FT, int IN, class XT, int MODE>
     void solve_last(const DevLevel<VT>& L, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
         const uint32_t grid = L.prog.nblk + cdiv(L.Md, L.lbs);
         if (L.lbs == 64)
             launch(k_level_solve<VT, FT, IN, XT, MODE, 64>, grid, L.lbs, L.lds_solve, L.prog, f, L.Md,
-                   (const VT*)L.dval, x, g);
+                   (const VT*)L.dval, x, g, level_batch(L));
         else
             launch(k_level_solve<VT, FT, IN, XT, MODE, kLBS>, grid, L.lbs, L.lds_solve, L.prog, f, L.Md,
-                   (const VT*)L.dval, x, g);
+                   (const VT*)L.dval, x, g, level_batch(L));
     }
 
     static InSrc<VT, VT, IN_PLAIN> plain(const VT* f) {
@@ -578,7 +608,7 @@ template <class VT> struct DevHier : DevPrecBase {
         VT* spare = nullptr;
         for (VT* b : {L.u1, L.u0}) if (b != prev) spare = b;
         launch(k_cheb2<VT, XT, true>, gb, kBS, 0, n, (const VT*)L.x2, (const VT*)prev, VT(L.ma2), L.s2_skip, VT(L.opb2),
-               L.opb2_one, (const uint32_t*)nullptr, (XT*)nullptr, spare, g);
+               L.opb2_one, (const uint32_t*)nullptr, (XT*)nullptr, spare, g, level_batch(L));
         VT* cur = spare;
         spare = L.x2;
         for (uint32_t i = 3; i <= L.nit; i++) {
@@ -589,7 +619,7 @@ template <class VT> struct DevHier : DevPrecBase {
                 up<XT, OP_CHEBK, 0>(L, C, x, g);
             } else {
                 launch(k_cheb_step<VT, XT, true>, gb, kBS, 0, n, (const VT*)cur, (const VT*)prev, (const VT*)spare, a, b,
-                       (const uint32_t*)nullptr, (XT*)nullptr, spare, g);
+                       (const uint32_t*)nullptr, (XT*)nullptr, spare, g, level_batch(L));
                 VT* t = prev;
                 prev = cur;
                 cur = spare;
@@ -795,6 +825,8 @@ struct mlp_prec_s {
     int kind = 0, precision = 0;
     uint32_t n = 0;
     std::unique_ptr<mlp::DevPrecBase> impl;
+    std::unique_ptr<mlp::DevPrecBase> wide;   // impl widened to wide_R right-hand sides (mlp_pcg_multi)
+    uint32_t wide_R = 0;
 };
 
 namespace mlp {
@@ -815,9 +847,12 @@ void context_release(mlp_ctx ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     ctx->drop_graphs(0);
     ctx->ws_mem.release();
+    ctx->multi_mem.release();
     ctx->mem.release();
     if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
     ctx->host_pinned = nullptr;
+    if (ctx->mst_host) (void)hipHostFree(ctx->mst_host);
+    ctx->mst_host = nullptr;
     if (ctx->owns_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     ctx->stream = nullptr;
 }
@@ -1093,6 +1128,7 @@ int mlp_residual(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, doubl
 // Streaming reductions run a grid-stride loop over at most kRedGrid blocks
 // (8 per CU), which keeps the one-block finalize short.
 constexpr uint32_t kRedGrid = 1024;
+constexpr uint32_t kMaxRhs = 64;          // right-hand sides of one mlp_pcg_multi call
 static_assert(kRedGrid <= kRedBlocks, "partials buffer too small");
 static_assert(2 * kRedGrid <= kPartials / 2, "each half of the partials buffer must hold two values per block");
 static uint32_t red_grid(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n, kBS), kRedGrid)); }
@@ -1102,7 +1138,7 @@ static double dev_dot(mlp_ctx ctx, uint32_t n, const double* x, const double* y)
     const uint32_t g = red_grid(n);
     hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBS), 0, ctx->stream, n, x, y, ctx->partials);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, ctx->stream, int(FIN_DOT), g, (const double*)ctx->partials,
-                       ctx->st, ctx->scal);
+                       ctx->st, ctx->scal, Batch{});
     MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, ctx->scal, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return ctx->host_pinned[0];
@@ -1633,58 +1669,87 @@ struct GraphPair {
     }
 };
 
-// One PCG iteration (pcg.cc:46-76) on the context's stream; every kernel is a
-// no-op once the device state says `done`.  replace = true: the recurrence
-// residual is replaced by b - A x before the preconditioner is applied, and
-// the stop test uses the replaced residual.
-void enqueue_pcg_iteration(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, bool flex, bool replace) {
+// The vectors, states and partial sums of R PCG solves run together (R = 1:
+// the context's own); vectors and buffers of solve y sit y strides on.
+struct PcgVecs {
+    uint32_t R = 1;
+    double *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *zp = nullptr;
+    SolveState* st = nullptr;
+    double* partials = nullptr;          // kPartials doubles per solve
+    Batch bt;
+};
+
+PcgVecs ctx_vecs(mlp_ctx ctx, uint32_t n) {
+    PcgVecs V;
+    V.r = ctx->r; V.z = ctx->z; V.p = ctx->p; V.Ap = ctx->Ap; V.zp = ctx->zp;
+    V.st = ctx->st;
+    V.partials = ctx->partials;
+    V.bt.n = n;
+    V.bt.part = uint32_t(kPartials);
+    return V;
+}
+
+// One PCG iteration (pcg.cc:46-76) on the context's stream for the solves of
+// V; every kernel is a no-op for a solve whose device state says `done`.
+// replace = true: the recurrence residual is replaced by b - A x before the
+// preconditioner is applied, and the stop test uses the replaced residual.
+void enqueue_pcg_iteration(mlp_ctx ctx, mlp_mat A, DevPrecBase* B, const double* b, double* x, bool flex,
+                           bool replace, const PcgVecs& V) {
     const uint32_t n = A->n, rg = red_grid(n);
     hipStream_t s = ctx->stream;
+    const dim3 grid(rg, V.R), one(1, V.R);
     // two partial-sum buffers: k_pcg_xr reads the (Ap, p) partials while it writes the (r, r) ones
-    double* P0 = ctx->partials;
-    double* P1 = ctx->partials + kPartials / 2;
+    double* P0 = V.partials;
+    double* P1 = V.partials + kPartials / 2;
     auto fin = [&](int op, const double* part) {
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, op, rg, part, ctx->st, (double*)nullptr);
+        hipLaunchKernelGGL(k_finalize, one, dim3(kBS), 0, s, op, rg, part, V.st, (double*)nullptr, V.bt);
     };
-    hipLaunchKernelGGL(k_pcg_spmv_dot, dim3(rg), dim3(kBS), 0, s, A->A.v, (const double*)ctx->p, ctx->Ap,
-                       (const SolveState*)ctx->st, P0);
-    hipLaunchKernelGGL(k_pcg_xr, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->p, (const double*)ctx->Ap, x,
-                       ctx->r, ctx->st, (const double*)P0, rg, P1);
+    hipLaunchKernelGGL(k_pcg_spmv_dot, grid, dim3(kBS), 0, s, A->A.v, (const double*)V.p, V.Ap,
+                       (const SolveState*)V.st, P0, V.bt);
+    hipLaunchKernelGGL(k_pcg_xr, grid, dim3(kBS), 0, s, n, (const double*)V.p, (const double*)V.Ap, x, V.r, V.st,
+                       (const double*)P0, rg, P1, V.bt);
     if (replace) {
         fin(FIN_XR_NOTEST, P1);
-        hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)x, ctx->r,
-                           (const SolveState*)ctx->st, P0);
+        hipLaunchKernelGGL(k_pcg_true_res, grid, dim3(kBS), 0, s, A->A.v, b, (const double*)x, V.r,
+                           (const SolveState*)V.st, P0, V.bt);
         fin(FIN_TRUE_RES, P0);
     } else {
         fin(FIN_XR, P1);
     }
-    B->impl->apply(ctx->r, ctx->z, &ctx->st->done);
+    B->apply(V.r, V.z, &V.st->done);
     if (flex) {
-        hipLaunchKernelGGL(k_pcg_rz<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                           (const double*)ctx->zp, (const SolveState*)ctx->st, P0);
-        hipLaunchKernelGGL(k_pcg_p<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp,
-                           ctx->st, (const double*)P0, rg);
+        hipLaunchKernelGGL(k_pcg_rz<true>, grid, dim3(kBS), 0, s, n, (const double*)V.r, (const double*)V.z,
+                           (const double*)V.zp, (const SolveState*)V.st, P0, V.bt);
+        hipLaunchKernelGGL(k_pcg_p<true>, grid, dim3(kBS), 0, s, n, (const double*)V.z, V.p, V.zp, V.st,
+                           (const double*)P0, rg, V.bt);
     } else {
-        hipLaunchKernelGGL(k_pcg_rz<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                           (const double*)ctx->zp, (const SolveState*)ctx->st, P0);
-        hipLaunchKernelGGL(k_pcg_p<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, ctx->p, ctx->zp,
-                           ctx->st, (const double*)P0, rg);
+        hipLaunchKernelGGL(k_pcg_rz<false>, grid, dim3(kBS), 0, s, n, (const double*)V.r, (const double*)V.z,
+                           (const double*)V.zp, (const SolveState*)V.st, P0, V.bt);
+        hipLaunchKernelGGL(k_pcg_p<false>, grid, dim3(kBS), 0, s, n, (const double*)V.z, V.p, V.zp, V.st,
+                           (const double*)P0, rg, V.bt);
     }
+}
+void enqueue_pcg_iteration(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, bool flex, bool replace) {
+    enqueue_pcg_iteration(ctx, A, B->impl.get(), b, x, flex, replace, ctx_vecs(ctx, A->n));
 }
 
 // p = z, rz = (r, z) after z = B r (pcg.cc:32-33); no-ops when `done`
-void enqueue_pcg_start(mlp_ctx ctx, mlp_prec B, uint32_t n, bool flex) {
+void enqueue_pcg_start(mlp_ctx ctx, DevPrecBase* B, uint32_t n, bool flex, const PcgVecs& V) {
     const uint32_t rg = red_grid(n);
     hipStream_t s = ctx->stream;
-    B->impl->apply(ctx->r, ctx->z, &ctx->st->done);
+    const dim3 grid(rg, V.R), one(1, V.R);
+    B->apply(V.r, V.z, &V.st->done);
     if (flex)
-        hipLaunchKernelGGL(k_pcg_init<true>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                           ctx->p, ctx->zp, (const SolveState*)ctx->st, ctx->partials);
+        hipLaunchKernelGGL(k_pcg_init<true>, grid, dim3(kBS), 0, s, n, (const double*)V.r, (const double*)V.z, V.p,
+                           V.zp, (const SolveState*)V.st, V.partials, V.bt);
     else
-        hipLaunchKernelGGL(k_pcg_init<false>, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->r, (const double*)ctx->z,
-                           ctx->p, ctx->zp, (const SolveState*)ctx->st, ctx->partials);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, int(FIN_INIT), rg, (const double*)ctx->partials, ctx->st,
-                       (double*)nullptr);
+        hipLaunchKernelGGL(k_pcg_init<false>, grid, dim3(kBS), 0, s, n, (const double*)V.r, (const double*)V.z, V.p,
+                           V.zp, (const SolveState*)V.st, V.partials, V.bt);
+    hipLaunchKernelGGL(k_finalize, one, dim3(kBS), 0, s, int(FIN_INIT), rg, (const double*)V.partials, V.st,
+                       (double*)nullptr, V.bt);
+}
+void enqueue_pcg_start(mlp_ctx ctx, mlp_prec B, uint32_t n, bool flex) {
+    enqueue_pcg_start(ctx, B->impl.get(), n, flex, ctx_vecs(ctx, n));
 }
 
 // The cached graphs of one PCG iteration (plain and, with true_res_every,
@@ -1706,6 +1771,67 @@ mlp_ctx_s::GraphEntry* pcg_graphs(mlp_ctx ctx, mlp_mat A, mlp_prec B, const doub
         MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[v]));
         MLP_HIP_CHECK(hipGraphInstantiate(&e.e[v], e.g[v], nullptr, nullptr, 0));
     }
+    if (ctx->graphs.size() >= 16) ctx->drop_graphs(ctx->graphs.front().A);
+    ctx->graphs.push_back(e);
+    return &ctx->graphs.back();
+}
+
+// Workspace of R batched solves of size n on the context (kept for the next
+// call; growing it drops the batched graphs, which hold its addresses).
+PcgVecs ensure_multi(mlp_ctx ctx, uint32_t n, uint32_t R) {
+    if (ctx->multi_R < R || ctx->multi_n < n) {
+        MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < ctx->graphs.size();) {
+            auto& g = ctx->graphs[i];
+            if (g.solver != 3) { i++; continue; }
+            for (int k = 0; k < 3; k++) {
+                if (g.e[k]) (void)hipGraphExecDestroy(g.e[k]);
+                if (g.g[k]) (void)hipGraphDestroy(g.g[k]);
+            }
+            ctx->graphs.erase(ctx->graphs.begin() + i);
+        }
+        ctx->multi_mem.release();
+        if (ctx->mst_host) (void)hipHostFree(ctx->mst_host);
+        ctx->mst_host = nullptr;
+        const uint32_t r = std::max(R, ctx->multi_R), m = std::max(n, ctx->multi_n);
+        const size_t v = size_t(r) * m;
+        ctx->mr = ctx->multi_mem.alloc<double>(v);
+        ctx->mz = ctx->multi_mem.alloc<double>(v);
+        ctx->mp = ctx->multi_mem.alloc<double>(v);
+        ctx->mAp = ctx->multi_mem.alloc<double>(v);
+        ctx->mzp = ctx->multi_mem.alloc<double>(v);
+        ctx->mpart = ctx->multi_mem.alloc<double>(size_t(r) * kPartials);
+        ctx->mst = ctx->multi_mem.alloc<SolveState>(r);
+        MLP_HIP_CHECK(hipHostMalloc((void**)&ctx->mst_host, size_t(r) * sizeof(SolveState), hipHostMallocDefault));
+        ctx->multi_R = r;
+        ctx->multi_n = m;
+    }
+    PcgVecs V;
+    V.R = R;
+    V.r = ctx->mr; V.z = ctx->mz; V.p = ctx->mp; V.Ap = ctx->mAp; V.zp = ctx->mzp;
+    V.st = ctx->mst;
+    V.partials = ctx->mpart;
+    V.bt.n = n;
+    V.bt.part = uint32_t(kPartials);
+    return V;
+}
+
+// The cached graph of one batched PCG iteration for these operands.
+mlp_ctx_s::GraphEntry* multi_graph(mlp_ctx ctx, mlp_mat A, mlp_prec B, DevPrecBase* W, const double* b, double* x,
+                                   const PcgVecs& V) {
+    for (auto& e : ctx->graphs)
+        if (e.solver == 3 && e.A == A->id && e.B == B->id && e.b == b && e.x == x && e.R == V.R &&
+            e.ws_n == ctx->multi_n)
+            return &e;
+    hipStream_t s = ctx->stream;
+    MLP_HIP_CHECK(hipStreamSynchronize(s));
+    mlp_ctx_s::GraphEntry e;
+    e.solver = 3;
+    e.A = A->id; e.B = B->id; e.b = b; e.x = x; e.flex = 0; e.tre = 0; e.ws_n = ctx->multi_n; e.R = V.R;
+    MLP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    enqueue_pcg_iteration(ctx, A, W, b, x, false, false, V);
+    MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[0]));
+    MLP_HIP_CHECK(hipGraphInstantiate(&e.e[0], e.g[0], nullptr, nullptr, 0));
     if (ctx->graphs.size() >= 16) ctx->drop_graphs(ctx->graphs.front().A);
     ctx->graphs.push_back(e);
     return &ctx->graphs.back();
@@ -1884,6 +2010,109 @@ int mlp_pcg(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, cons
     MLP_API_END
 }
 
+// R PCG solves with one matrix and preconditioner (many right-hand sides):
+// b and x hold R vectors of length n one after another.  Every launch of an
+// iteration covers all R (grid.y), so the latency of the preconditioner's
+// dependent launches is paid once per iteration for the batch.  Each solve
+// runs mlp_pcg's kernels on its own vectors with its own state and stops on
+// its own test; the batch iterates until all have stopped.  Standard PCG only
+// (flexible 0, true_res_every 0); the preconditioner must be uh_cheb.
+int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const double* b, double* x,
+                  const mlp_solve_opts* o, mlp_solve_stats* stats) {
+    MLP_API_BEGIN
+    require(ctx && A && B && b && x && o && stats, "null argument");
+    require(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs must lie in [1, 64]");
+    require(B->n == A->n, "preconditioner size differs from A");
+    require(B->ctx == ctx && A->ctx->device == ctx->device, "the preconditioner must live on the context");
+    require(!o->flexible && !o->true_res_every, "mlp_pcg_multi runs standard PCG (flexible 0, true_res_every 0)");
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = A->n, rg = red_grid(n), R = nrhs;
+    if (!B->wide || B->wide_R != R) {
+        MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->drop_graphs(B->id);
+        B->wide = B->impl->widen(ctx, R);
+        B->wide_R = B->wide ? R : 0;
+        if (!B->wide) throw Error("this preconditioner has no batched apply (uh_cheb has)", MLP_ERR_INVALID);
+    }
+    DevPrecBase* W = B->wide.get();
+    ensure_workspace(ctx, n, false);
+    const PcgVecs V = ensure_multi(ctx, n, R);
+    hipStream_t s = ctx->stream;
+    for (uint32_t k = 0; k < R; k++) fill_x0(ctx, n, o->x0_mode, x + size_t(k) * n);
+    for (uint32_t k = 0; k < R; k++) {
+        SolveState& st0 = ctx->mst_host[k];
+        std::memset(&st0, 0, sizeof(st0));
+        st0.eps = o->eps;
+        st0.maxit = o->maxit;
+    }
+    MLP_HIP_CHECK(hipMemcpyAsync(V.st, ctx->mst_host, R * sizeof(SolveState), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_residual_batch, dim3(cdiv(n, kBS), R), dim3(kBS), 0, s, A->A.v, b, (const double*)x, V.r,
+                       V.bt);
+    hipLaunchKernelGGL(k_dot_batch, dim3(rg, R), dim3(kBS), 0, s, n, (const double*)V.r, V.partials, V.bt);
+    hipLaunchKernelGGL(k_finalize, dim3(1, R), dim3(kBS), 0, s, int(FIN_START), rg, (const double*)V.partials, V.st,
+                       (double*)nullptr, V.bt);
+    enqueue_pcg_start(ctx, W, n, false, V);
+    mlp_ctx_s::GraphEntry* ge = o->use_graph && W->graph_safe() ? multi_graph(ctx, A, B, W, b, x, V) : nullptr;
+    const uint32_t chunk = o->check_every ? o->check_every : 16;
+    uint32_t c = chunk;
+    while (true) {
+        for (uint32_t i = 0; i < c; i++) {
+            if (ge) MLP_HIP_CHECK(hipGraphLaunch(ge->e[0], s));
+            else enqueue_pcg_iteration(ctx, A, W, b, x, false, false, V);
+        }
+        MLP_HIP_CHECK(hipGetLastError());
+        MLP_HIP_CHECK(hipMemcpyAsync(ctx->mst_host, V.st, R * sizeof(SolveState), hipMemcpyDeviceToHost, s));
+        MLP_HIP_CHECK(hipStreamSynchronize(s));
+        bool all = true;
+        uint32_t next = 1;
+        for (uint32_t k = 0; k < R; k++) {
+            const SolveState& st = ctx->mst_host[k];
+            if (st.done) continue;
+            all = false;
+            next = std::max(next, next_chunk(st, chunk));
+        }
+        if (all) break;
+        c = next;
+    }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::vector<SolveState> fin(ctx->mst_host, ctx->mst_host + R);
+    int first = MLP_OK;
+    std::string msg;
+    for (uint32_t k = 0; k < R; k++) {
+        const SolveState& st = fin[k];
+        mlp_solve_stats& S = stats[k];
+        std::memset(&S, 0, sizeof(S));
+        S.niter = st.niter;
+        S.init_norm = st.init_norm;
+        S.final_norm = st.norm;
+        S.eps_used = st.eps;
+        S.t_solve_ms = ms;
+        S.true_rel_res = true_rel_res(ctx, A, b + size_t(k) * n, x + size_t(k) * n, ctx->r, st.init_norm);
+        int code = MLP_OK;
+        const char* what = "";
+        if (st.status == 4) {
+            code = MLP_ERR_NO_ITER;
+            what = "No need to iterate";
+        } else if (st.status == 3) {
+            code = MLP_ERR_NOT_SPD;
+            what = "<Ap, p> <= 0";
+        } else if (st.status == 5) {
+            code = MLP_ERR_MAXIT;
+            what = "maximum number of iterations reached";
+        }
+        S.status = code;
+        if (code && first == MLP_OK) {
+            first = code;
+            msg = "right-hand side " + std::to_string(k) + ": " + what;
+        }
+    }
+    if (first != MLP_OK) {
+        set_last_error(msg.c_str());
+        return first;
+    }
+    MLP_API_END
+}
+
 // Many independent PCG solves at once (config 3: ensembles of systems).  Each
 // job has its own context, so its own stream, workspace and device state; the
 // solves start without host synchronisation (FIN_START computes the initial
@@ -1937,7 +2166,7 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
         hipLaunchKernelGGL(k_dot, dim3(rg), dim3(kBS), 0, ctx->stream, n, (const double*)ctx->r, (const double*)ctx->r,
                            ctx->partials);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, ctx->stream, int(FIN_START), rg,
-                           (const double*)ctx->partials, ctx->st, (double*)nullptr);
+                           (const double*)ctx->partials, ctx->st, (double*)nullptr, Batch{});
         enqueue_pcg_start(ctx, R.j->B, n, flex);
         R.ge = pcg_graphs(ctx, A, R.j->B, R.j->d_b, R.j->d_x, flex, tre, o->use_graph && R.j->B->impl->graph_safe());
         MLP_HIP_CHECK(hipEventCreateWithFlags(&R.ev, hipEventDisableTiming));
@@ -2200,9 +2429,9 @@ int mlp_cheb_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, double lmin, double lmax
             hipLaunchKernelGGL(k_cheb_outer, dim3(rg), dim3(kBS), 0, s, n, X, (const double*)U1, (const double*)U0,
                                (const double*)ctx->coef, (const SolveState*)ctx->st);
             hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)X, ctx->r,
-                               (const SolveState*)ctx->st, ctx->partials);
+                               (const SolveState*)ctx->st, ctx->partials, Batch{});
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, int(FIN_STEP), rg, (const double*)ctx->partials,
-                               ctx->st, (double*)nullptr);
+                               ctx->st, (double*)nullptr, Batch{});
         };
         ensure_coef(ctx, lmin, lmax, 2);
         st = run_outer(ctx, 1, A, B, b, x, 3, o, iteration, [](uint32_t k) { return int((k - 1) % 3); },
@@ -2256,9 +2485,9 @@ int mlp_simple_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, doubl
             B->impl->apply(ctx->r, ctx->z, &ctx->st->done);
             hipLaunchKernelGGL(k_add, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, x, (const SolveState*)ctx->st);
             hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)x, ctx->r,
-                               (const SolveState*)ctx->st, ctx->partials);
+                               (const SolveState*)ctx->st, ctx->partials, Batch{});
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, int(FIN_STEP), rg, (const double*)ctx->partials,
-                               ctx->st, (double*)nullptr);
+                               ctx->st, (double*)nullptr, Batch{});
         };
         st = run_outer(ctx, 2, A, B, b, x, 1, o, iteration, [](uint32_t) { return 0; }, [](uint32_t) {});
     }

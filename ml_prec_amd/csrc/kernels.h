@@ -48,6 +48,25 @@ struct SolveState {
     unsigned counter[8];                 // last-block counters of the fused reductions
 };
 
+// Batched right-hand sides: a launch with gridDim.y = R runs R independent
+// problems on one matrix and preconditioner.  Problem y = blockIdx.y finds
+// each of its vectors y * stride elements after problem 0's (the strides by
+// vector kind below), its solve state at st + y and its partial sums
+// y * part doubles on.  With R = 1 every offset is 0.
+struct Batch {
+    uint32_t n = 0;        // the length-n solve vectors
+    uint32_t part = 0;     // partial-sum buffers
+};
+struct LevelBatch {
+    uint32_t N = 0;        // level vectors: the input f, the output x, the parent's u, F and us
+    uint32_t nodes = 0;    // wt and the global-block scratch gx
+    uint32_t n = 0;        // kept-row vectors: F, x2, u1, u0
+    uint32_t st = 0;       // solve states, in 32-bit words (the done-flag guard)
+};
+// p + off, leaving null pointers null
+template <class T>
+__device__ __forceinline__ T* shift(T* p, size_t off) { return p ? p + off : p; }
+
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
@@ -622,7 +641,13 @@ __device__ __forceinline__ BlkHdr load_hdr(const uint32_t* blk) {
 template <class VT, class FT, int IN, class XT, int DMODE, int BS>
 __global__ void __launch_bounds__(BS) k_level_down(ProgView<VT> P, InSrc<VT, FT, IN> f, VT* __restrict__ wt,
                                                      VT* __restrict__ F, uint32_t Md, const VT* __restrict__ dval,
-                                                     XT* __restrict__ x, const uint32_t* guard) {
+                                                     XT* __restrict__ x, const uint32_t* guard, LevelBatch lb) {
+    {
+        const size_t y = blockIdx.y;
+        f.f = shift(f.f, y * lb.N); f.u = shift(f.u, y * lb.N); f.F = shift(f.F, y * lb.N); f.us = shift(f.us, y * lb.N);
+        wt = shift(wt, y * lb.nodes); P.gx = shift(P.gx, y * lb.nodes); F = shift(F, y * lb.n);
+        x = shift(x, y * lb.N); guard = shift(guard, y * lb.st);
+    }
     // the solve's done flag: loaded now, tested before the first store, so that
     // its latency overlaps the block's own loads (a converged iteration only
     // writes the apply's scratch, never an output)
@@ -680,7 +705,13 @@ __global__ void __launch_bounds__(BS) k_level_down(ProgView<VT> P, InSrc<VT, FT,
 template <class VT, class XT, int OP, int UMODE, int BS>
 __global__ void __launch_bounds__(BS) k_level_up(ProgView<VT> P, const VT* __restrict__ wt, Combine<VT> C,
                                                    const uint32_t* __restrict__ mapM, uint32_t n, XT* __restrict__ x,
-                                                   const uint32_t* guard) {
+                                                   const uint32_t* guard, LevelBatch lb) {
+    {
+        const size_t y = blockIdx.y;
+        wt = shift(wt, y * lb.nodes); P.gx = shift(P.gx, y * lb.nodes);
+        C.x2 = shift(C.x2, y * lb.n); C.u1 = shift(C.u1, y * lb.n); C.u0 = shift(C.u0, y * lb.n);
+        x = shift(x, y * lb.N); guard = shift(guard, y * lb.st);
+    }
     // the solve's done flag: loaded now, tested before the first store, so that
     // its latency overlaps the block's own loads (a converged iteration only
     // writes the apply's scratch, never an output)
@@ -723,7 +754,12 @@ __global__ void __launch_bounds__(BS) k_level_up(ProgView<VT> P, const VT* __res
 template <class VT, class FT, int IN, class XT, int MODE, int BS>
 __global__ void __launch_bounds__(BS) k_level_solve(ProgView<VT> P, InSrc<VT, FT, IN> f, uint32_t Md,
                                                       const VT* __restrict__ dval, XT* __restrict__ x,
-                                                      const uint32_t* guard) {
+                                                      const uint32_t* guard, LevelBatch lb) {
+    {
+        const size_t y = blockIdx.y;
+        f.f = shift(f.f, y * lb.N); f.u = shift(f.u, y * lb.N); f.F = shift(f.F, y * lb.N); f.us = shift(f.us, y * lb.N);
+        P.gx = shift(P.gx, y * lb.nodes); x = shift(x, y * lb.N); guard = shift(guard, y * lb.st);
+    }
     // the solve's done flag: loaded now, tested before the first store, so that
     // its latency overlaps the block's own loads (a converged iteration only
     // writes the apply's scratch, never an output)
@@ -776,7 +812,12 @@ template <class VT, class XT, bool STORE>
 __global__ void __launch_bounds__(kBS) k_cheb2(uint32_t n, const VT* __restrict__ x2, const VT* __restrict__ u1s,
                                                VT ma, int skip, VT opb, int opb_one,
                                                const uint32_t* __restrict__ mapM, XT* __restrict__ x,
-                                               VT* __restrict__ x2out, const uint32_t* guard) {
+                                               VT* __restrict__ x2out, const uint32_t* guard, LevelBatch lb) {
+    {
+        const size_t y = blockIdx.y;
+        x2 = shift(x2, y * lb.n); u1s = shift(u1s, y * lb.n); x2out = shift(x2out, y * lb.n);
+        x = shift(x, y * lb.N); guard = shift(guard, y * lb.st);
+    }
     if (guarded(guard)) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -793,7 +834,12 @@ template <class VT, class XT, bool STORE>
 __global__ void __launch_bounds__(kBS) k_cheb_step(uint32_t n, const VT* __restrict__ u1, const VT* __restrict__ u0,
                                                    const VT* __restrict__ x2, VT a, VT b,
                                                    const uint32_t* __restrict__ mapM, XT* __restrict__ x,
-                                                   VT* __restrict__ x2out, const uint32_t* guard) {
+                                                   VT* __restrict__ x2out, const uint32_t* guard, LevelBatch lb) {
+    {
+        const size_t y = blockIdx.y;
+        u1 = shift(u1, y * lb.n); u0 = shift(u0, y * lb.n); x2 = shift(x2, y * lb.n); x2out = shift(x2out, y * lb.n);
+        x = shift(x, y * lb.N); guard = shift(guard, y * lb.st);
+    }
     if (guarded(guard)) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -820,7 +866,9 @@ __global__ void __launch_bounds__(kBS) k_diag_apply(uint32_t n, const double* __
 enum { FIN_DOT = 0, FIN_XR = 2, FIN_XR_NOTEST = 3, FIN_INIT = 6, FIN_TRUE_RES = 7, FIN_STEP = 8, FIN_START = 9 };
 
 __global__ void __launch_bounds__(kBS) k_finalize(int op, uint32_t nb, const double* __restrict__ partials,
-                                                  SolveState* st, double* out) {
+                                                  SolveState* st, double* out, Batch bt) {
+    partials += size_t(blockIdx.y) * bt.part;
+    if (st) st += blockIdx.y;
     if (op != FIN_DOT && op != FIN_INIT && op != FIN_START && st->done) return;
     __shared__ double sh[kBS / 64];
     const int NV = 1;
@@ -871,6 +919,28 @@ __global__ void __launch_bounds__(kBS) k_dot(uint32_t n, const double* __restric
     block_partials<kBS, 1>(part, partials);
 }
 
+// The start of a batch of solves (grid.y = right-hand side): r = b - A x and
+// the partials of (r, r), with the per-row and per-block operations of
+// k_residual and k_dot, so each solve starts from the same values as alone.
+__global__ void __launch_bounds__(kBS) k_residual_batch(SellView<double> A, const double* __restrict__ b,
+                                                        const double* __restrict__ x, double* __restrict__ rr,
+                                                        Batch bt) {
+    const size_t y = blockIdx.y;
+    b += y * bt.n; x += y * bt.n; rr += y * bt.n;
+    const uint32_t r = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (r >= A.n) return;
+    rr[r] = b[r] - mat_row<double, double>(A, x, r);
+}
+__global__ void __launch_bounds__(kBS) k_dot_batch(uint32_t n, const double* __restrict__ x, double* partials,
+                                                   Batch bt) {
+    const size_t y = blockIdx.y;
+    x += y * bt.n; partials += y * bt.part;
+    double s = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += x[i] * x[i];
+    double part[1] = {s};
+    block_partials<kBS, 1>(part, partials);
+}
+
 __global__ void __launch_bounds__(kBS) k_axpy(uint32_t n, double a, int one, const double* __restrict__ x,
                                               double* __restrict__ y) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -889,7 +959,10 @@ __global__ void __launch_bounds__(kBS) k_copy(uint32_t n, const double* __restri
 
 // Ap = A p ; partials of (Ap, p)
 __global__ void __launch_bounds__(kBS) k_pcg_spmv_dot(SellView<double> A, const double* __restrict__ p,
-                                                      double* __restrict__ Ap, const SolveState* st, double* partials) {
+                                                      double* __restrict__ Ap, const SolveState* st, double* partials,
+                                                      Batch bt) {
+    const uint32_t y = blockIdx.y;
+    p += size_t(y) * bt.n; Ap += size_t(y) * bt.n; st += y; partials += size_t(y) * bt.part;
     if (st->done) return;
     double s = 0.0;
     // grid-stride over 256-row tiles, XCD-grouped within each sweep of the grid
@@ -913,7 +986,10 @@ __global__ void __launch_bounds__(kBS) k_pcg_spmv_dot(SellView<double> A, const 
 __global__ void __launch_bounds__(kBS) k_pcg_xr(uint32_t n, const double* __restrict__ p, const double* __restrict__ Ap,
                                                 double* __restrict__ x, double* __restrict__ r, SolveState* st,
                                                 const double* __restrict__ app_partials, uint32_t nb,
-                                                double* partials) {
+                                                double* partials, Batch bt) {
+    const uint32_t y = blockIdx.y;
+    p += size_t(y) * bt.n; Ap += size_t(y) * bt.n; x += size_t(y) * bt.n; r += size_t(y) * bt.n; st += y;
+    app_partials += size_t(y) * bt.part; partials += size_t(y) * bt.part;
     if (block_done(st)) return;
     __shared__ double sh[kBS / 64];
     const double app = total_of_partials<kBS>(app_partials, nb, 1, 0, sh);
@@ -948,7 +1024,10 @@ __global__ void __launch_bounds__(kBS) k_pcg_xr(uint32_t n, const double* __rest
 // partials of (r, z) and, flexible, (r, z - z_prev)   (pcg.cc:68)
 template <bool FLEX>
 __global__ void __launch_bounds__(kBS) k_pcg_rz(uint32_t n, const double* __restrict__ r, const double* __restrict__ z,
-                                                const double* __restrict__ zp, const SolveState* st, double* partials) {
+                                                const double* __restrict__ zp, const SolveState* st, double* partials,
+                                                Batch bt) {
+    const uint32_t y = blockIdx.y;
+    r += size_t(y) * bt.n; z += size_t(y) * bt.n; zp += size_t(y) * bt.n; st += y; partials += size_t(y) * bt.part;
     if (st->done) return;
     double s0 = 0.0, s1 = 0.0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -972,7 +1051,10 @@ __global__ void __launch_bounds__(kBS) k_pcg_rz(uint32_t n, const double* __rest
 template <bool FLEX>
 __global__ void __launch_bounds__(kBS) k_pcg_p(uint32_t n, const double* __restrict__ z, double* __restrict__ p,
                                                double* __restrict__ zp, SolveState* st,
-                                               const double* __restrict__ rz_partials, uint32_t nb) {
+                                               const double* __restrict__ rz_partials, uint32_t nb, Batch bt) {
+    const uint32_t y = blockIdx.y;
+    z += size_t(y) * bt.n; p += size_t(y) * bt.n; zp += size_t(y) * bt.n; st += y;
+    rz_partials += size_t(y) * bt.part;
     if (block_done(st)) return;
     __shared__ double sh[kBS / 64];
     const double t0 = total_of_partials<kBS>(rz_partials, nb, FLEX ? 2 : 1, 0, sh);
@@ -996,7 +1078,10 @@ __global__ void __launch_bounds__(kBS) k_pcg_p(uint32_t n, const double* __restr
 template <bool FLEX>
 __global__ void __launch_bounds__(kBS) k_pcg_init(uint32_t n, const double* __restrict__ r, const double* __restrict__ z,
                                                   double* __restrict__ p, double* __restrict__ zp, const SolveState* st,
-                                                  double* partials) {
+                                                  double* partials, Batch bt) {
+    const uint32_t y = blockIdx.y;
+    r += size_t(y) * bt.n; z += size_t(y) * bt.n; p += size_t(y) * bt.n; zp += size_t(y) * bt.n; st += y;
+    partials += size_t(y) * bt.part;
     if (st->done) return;
     double s = 0.0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1011,7 +1096,9 @@ __global__ void __launch_bounds__(kBS) k_pcg_init(uint32_t n, const double* __re
 // true residual replacement: r = b - A x ; partials of (r, r)
 __global__ void __launch_bounds__(kBS) k_pcg_true_res(SellView<double> A, const double* __restrict__ b,
                                                       const double* __restrict__ x, double* __restrict__ r,
-                                                      const SolveState* st, double* partials) {
+                                                      const SolveState* st, double* partials, Batch bt) {
+    const uint32_t y = blockIdx.y;
+    b += size_t(y) * bt.n; x += size_t(y) * bt.n; r += size_t(y) * bt.n; st += y; partials += size_t(y) * bt.part;
     if (st->done) return;
     double s = 0.0;
     const uint32_t tiles = (A.n + blockDim.x - 1) / blockDim.x;

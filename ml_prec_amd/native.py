@@ -108,6 +108,7 @@ EXPORTS = {
     "mlp_spe_adjoint_grad": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                        C.c_double, _vp, _vp, _vp, _vp]),
     "mlp_pcg_batch": (C.c_int, [C.c_uint32, C.POINTER(PcgJob), C.POINTER(SolveOpts)]),
+    "mlp_pcg_multi": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "mlp_cheb_solver": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, C.POINTER(SolveOpts),
                                   C.POINTER(SolveStats)]),
     "mlp_simple_solver": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
@@ -396,6 +397,16 @@ def pcg_batch(jobs, eps=1e-8, maxit=0, x0_mode=X0_ZERO, flexible=False, true_res
         d["rc"] = j.status
         out.append(d)
     return rc, out
+
+
+def pcg_multi(ctx, A, B, nrhs, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO, check_every=16,
+              use_graph=True):
+    """nrhs PCG solves of one system at once (mlp_pcg_multi); b and x hold nrhs vectors of length n one after
+    another.  Returns (rc, [stats dict per right-hand side])."""
+    o = _opts(eps, maxit, x0_mode, False, 0, check_every, use_graph)
+    arr = (SolveStats * nrhs)()
+    rc = lib().mlp_pcg_multi(ctx.h, A.h, B.h, nrhs, b.ptr, x.ptr, C.byref(o), arr)
+    return rc, [st.as_dict() for st in arr]
 
 
 def gmres(ctx, A, B, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO, raise_on_error=True):
