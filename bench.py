@@ -169,6 +169,18 @@ def ensemble_rhs(N, S, args):
         return st
 
     ms_multi, st_multi = timed(multi)
+    # the same batch with the FP32 hierarchy in FP64 flexible PCG (true residual every 20 iterations)
+    B32 = N.DevPrec(ctx0, A, H, precision=N.MLP_FP32)
+    xm32 = ctx0.array(R * s.n)
+
+    def multi32():
+        rc, st = N.pcg_multi(ctx0, A, B32, R, bm, xm32, flexible=True, true_res_every=20, **kw)
+        if rc:
+            raise RuntimeError(f"batched mixed solve failed: {rc}")
+        return st
+
+    ms_multi32, st_multi32 = timed(multi32)
+    B32.close()
     # concurrent jobs on clones
     jobs = []
     for k in range(R):
@@ -192,6 +204,9 @@ def ensemble_rhs(N, S, args):
     ms_seq = 1e3 * (time.perf_counter() - t0)
     same = all(np.array_equal(xm.get()[k * s.n:(k + 1) * s.n], jobs[k][4].get()) for k in range(R))
     iters = [d["niter"] for d in st_multi]
+    log(f"{R} right-hand sides, mixed (FP32 hierarchy): batched {ms_multi32:.2f} ms "
+        f"({R / ms_multi32 * 1e3:.0f} solves/s), {ms_multi / ms_multi32:.2f}x the FP64 batch, worst true rel res "
+        f"{max(d['true_rel_res'] for d in st_multi32):.2e}")
     log(f"{R} right-hand sides: batched {ms_multi:.2f} ms ({R / ms_multi * 1e3:.0f} solves/s), concurrent jobs "
         f"{ms_batch:.2f} ms ({R / ms_batch * 1e3:.0f} solves/s), one after another {ms_seq:.2f} ms, "
         f"iterations {min(iters)}..{max(iters)}, batched == single bitwise: {same}")
@@ -203,6 +218,10 @@ def ensemble_rhs(N, S, args):
         "config": {"workload": f"SPE10-shaped 60x220x{args.nz}, {R} seeded right-hand sides, one uh_cheb hierarchy, "
                                "PCG x0=0 to 1e-8, all right-hand sides in every launch (mlp_pcg_multi)"},
         "ms_concurrent_jobs": round(ms_batch, 3), "ms_one_after_another": round(ms_seq, 3),
+        "mixed_fp32_hierarchy": {"ms_per_step": round(ms_multi32, 3),
+                                 "solves_per_s": round(R / ms_multi32 * 1e3, 2),
+                                 "speedup_vs_fp64_batch": round(ms_multi / ms_multi32, 3),
+                                 "worst_true_rel_res": max(d["true_rel_res"] for d in st_multi32)},
         "speedup_vs_one_after_another": round(ms_seq / ms_multi, 3), "bitwise_equal_to_single": same,
         "iterations": iters}), flush=True)
     for c, Aj, Bj, b, x in jobs:

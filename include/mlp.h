@@ -228,9 +228,10 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* opts)
  * d_b and d_x holding nrhs device vectors of length n one after another.
  * Every kernel launch of an iteration covers all right-hand sides; each solve
  * runs PCGSolver's steps (pcg.cc:8-102) on its own vectors and stops on its
- * own test, giving bitwise the result of mlp_pcg on that right-hand side.
- * Standard PCG only (opts->flexible 0, true_res_every 0).  stats has nrhs
- * entries; the return code is the first failing right-hand side's. */
+ * own test, giving bitwise the result of mlp_pcg on that right-hand side
+ * (flexible solves take mlp_pcg's true-residual restarts one by one after the
+ * batch).  stats has nrhs entries; the return code is the first failing
+ * right-hand side's. */
 int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const double* d_b, double* d_x,
                   const mlp_solve_opts* opts, mlp_solve_stats* stats);
 /* Adjoint gradient for the two-point-flux assembly of modules/mesh/spe.cc:69-131 (no counterpart in the

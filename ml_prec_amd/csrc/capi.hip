@@ -1816,22 +1816,25 @@ PcgVecs ensure_multi(mlp_ctx ctx, uint32_t n, uint32_t R) {
     return V;
 }
 
-// The cached graph of one batched PCG iteration for these operands.
+// The cached graphs of one batched PCG iteration (plain and, with
+// true_res_every, residual-replacing) for these operands.
 mlp_ctx_s::GraphEntry* multi_graph(mlp_ctx ctx, mlp_mat A, mlp_prec B, DevPrecBase* W, const double* b, double* x,
-                                   const PcgVecs& V) {
+                                   bool flex, uint32_t tre, const PcgVecs& V) {
     for (auto& e : ctx->graphs)
         if (e.solver == 3 && e.A == A->id && e.B == B->id && e.b == b && e.x == x && e.R == V.R &&
-            e.ws_n == ctx->multi_n)
+            e.flex == int(flex) && e.tre == tre && e.ws_n == ctx->multi_n)
             return &e;
     hipStream_t s = ctx->stream;
     MLP_HIP_CHECK(hipStreamSynchronize(s));
     mlp_ctx_s::GraphEntry e;
     e.solver = 3;
-    e.A = A->id; e.B = B->id; e.b = b; e.x = x; e.flex = 0; e.tre = 0; e.ws_n = ctx->multi_n; e.R = V.R;
-    MLP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    enqueue_pcg_iteration(ctx, A, W, b, x, false, false, V);
-    MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[0]));
-    MLP_HIP_CHECK(hipGraphInstantiate(&e.e[0], e.g[0], nullptr, nullptr, 0));
+    e.A = A->id; e.B = B->id; e.b = b; e.x = x; e.flex = int(flex); e.tre = tre; e.ws_n = ctx->multi_n; e.R = V.R;
+    for (int v = 0; v < (tre ? 2 : 1); v++) {
+        MLP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        enqueue_pcg_iteration(ctx, A, W, b, x, flex, v == 1, V);
+        MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[v]));
+        MLP_HIP_CHECK(hipGraphInstantiate(&e.e[v], e.g[v], nullptr, nullptr, 0));
+    }
     if (ctx->graphs.size() >= 16) ctx->drop_graphs(ctx->graphs.front().A);
     ctx->graphs.push_back(e);
     return &ctx->graphs.back();
@@ -2015,8 +2018,9 @@ int mlp_pcg(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, cons
 // iteration covers all R (grid.y), so the latency of the preconditioner's
 // dependent launches is paid once per iteration for the batch.  Each solve
 // runs mlp_pcg's kernels on its own vectors with its own state and stops on
-// its own test; the batch iterates until all have stopped.  Standard PCG only
-// (flexible 0, true_res_every 0); the preconditioner must be uh_cheb.
+// its own test; the batch iterates until all have stopped.  Flexible solves
+// then take mlp_pcg's true-residual restarts one by one.  The preconditioner
+// must be uh_cheb.
 int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const double* b, double* x,
                   const mlp_solve_opts* o, mlp_solve_stats* stats) {
     MLP_API_BEGIN
@@ -2024,7 +2028,8 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
     require(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs must lie in [1, 64]");
     require(B->n == A->n, "preconditioner size differs from A");
     require(B->ctx == ctx && A->ctx->device == ctx->device, "the preconditioner must live on the context");
-    require(!o->flexible && !o->true_res_every, "mlp_pcg_multi runs standard PCG (flexible 0, true_res_every 0)");
+    const bool flex = o->flexible != 0;
+    const uint32_t tre = o->true_res_every;
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t n = A->n, rg = red_grid(n), R = nrhs;
     if (!B->wide || B->wide_R != R) {
@@ -2051,14 +2056,17 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
     hipLaunchKernelGGL(k_dot_batch, dim3(rg, R), dim3(kBS), 0, s, n, (const double*)V.r, V.partials, V.bt);
     hipLaunchKernelGGL(k_finalize, dim3(1, R), dim3(kBS), 0, s, int(FIN_START), rg, (const double*)V.partials, V.st,
                        (double*)nullptr, V.bt);
-    enqueue_pcg_start(ctx, W, n, false, V);
-    mlp_ctx_s::GraphEntry* ge = o->use_graph && W->graph_safe() ? multi_graph(ctx, A, B, W, b, x, V) : nullptr;
+    enqueue_pcg_start(ctx, W, n, flex, V);
+    mlp_ctx_s::GraphEntry* ge =
+        o->use_graph && W->graph_safe() ? multi_graph(ctx, A, B, W, b, x, flex, tre, V) : nullptr;
     const uint32_t chunk = o->check_every ? o->check_every : 16;
-    uint32_t c = chunk;
+    uint32_t c = chunk, k = 0;
     while (true) {
         for (uint32_t i = 0; i < c; i++) {
-            if (ge) MLP_HIP_CHECK(hipGraphLaunch(ge->e[0], s));
-            else enqueue_pcg_iteration(ctx, A, W, b, x, false, false, V);
+            k++;
+            const bool replace = tre && (k % tre == 0);
+            if (ge) MLP_HIP_CHECK(hipGraphLaunch(ge->e[replace ? 1 : 0], s));
+            else enqueue_pcg_iteration(ctx, A, W, b, x, flex, replace, V);
         }
         MLP_HIP_CHECK(hipGetLastError());
         MLP_HIP_CHECK(hipMemcpyAsync(ctx->mst_host, V.st, R * sizeof(SolveState), hipMemcpyDeviceToHost, s));
@@ -2074,8 +2082,26 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
         if (all) break;
         c = next;
     }
-    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     std::vector<SolveState> fin(ctx->mst_host, ctx->mst_host + R);
+    if (flex) {
+        // mlp_pcg's restarts from the FP64 true residual, per right-hand side
+        for (uint32_t q = 0; q < R; q++) {
+            SolveState& st = fin[q];
+            const double* bq = b + size_t(q) * n;
+            double* xq = x + size_t(q) * n;
+            for (int restart = 0; restart < 8 && st.status == 0; restart++) {
+                if (true_rel_res(ctx, A, bq, xq, ctx->r, st.init_norm) <= st.eps) break;
+                bool again = false;
+                const double init_norm = st.init_norm, eps = st.eps;
+                st = pcg_phase(ctx, A, B, bq, xq, o, init_norm, eps, st.niter, &again);
+                st.init_norm = init_norm;
+                st.eps = eps;
+                if (again) break;
+            }
+        }
+        MLP_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     int first = MLP_OK;
     std::string msg;
     for (uint32_t k = 0; k < R; k++) {

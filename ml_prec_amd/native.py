@@ -399,11 +399,11 @@ def pcg_batch(jobs, eps=1e-8, maxit=0, x0_mode=X0_ZERO, flexible=False, true_res
     return rc, out
 
 
-def pcg_multi(ctx, A, B, nrhs, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO, check_every=16,
-              use_graph=True):
+def pcg_multi(ctx, A, B, nrhs, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO, flexible=False,
+              true_res_every=0, check_every=16, use_graph=True):
     """nrhs PCG solves of one system at once (mlp_pcg_multi); b and x hold nrhs vectors of length n one after
     another.  Returns (rc, [stats dict per right-hand side])."""
-    o = _opts(eps, maxit, x0_mode, False, 0, check_every, use_graph)
+    o = _opts(eps, maxit, x0_mode, flexible, true_res_every, check_every, use_graph)
     arr = (SolveStats * nrhs)()
     rc = lib().mlp_pcg_multi(ctx.h, A.h, B.h, nrhs, b.ptr, x.ptr, C.byref(o), arr)
     return rc, [st.as_dict() for st in arr]

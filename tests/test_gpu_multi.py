@@ -4,7 +4,8 @@ Every launch covers all right-hand sides (grid.y), each with its own vectors,
 state and partial sums, so each solve must equal the single-right-hand-side
 mlp_pcg on the same vector bit for bit: same iteration count, same x.  Also
 the per-right-hand-side outcomes (one right-hand side that needs no iteration,
-an iteration cap) and the argument checks.
+an iteration cap), the mixed-precision mode (FP32 hierarchy, flexible PCG with
+true-residual replacement and restarts) and the argument checks.
 """
 import numpy as np
 import pytest
@@ -45,6 +46,25 @@ def test_multi_equals_single(ctx, name, R, graph):
         assert np.array_equal(xs[k], x1), k
         assert stats[k]["final_norm"] == st["final_norm"]
         assert stats[k]["true_rel_res"] <= 1e-7
+
+
+def test_multi_mixed_precision_equals_single(ctx):
+    """FP32 hierarchy, flexible PCG, true residual every 7 iterations, then the restarts."""
+    s = system("spe10_z12")
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, N.build_uh_cheb(s.ia, s.ja, s.a), precision=N.MLP_FP32)
+    R = 4
+    bs = _rhs(s, R, 24)
+    X = ctx.array(R * s.n)
+    kw = dict(eps=1e-10, maxit=2000, flexible=True, true_res_every=7)
+    rc, stats = N.pcg_multi(ctx, A, B, R, ctx.to_device(bs.ravel()), X, **kw)
+    assert rc == 0, stats
+    xs = X.get().reshape(R, s.n)
+    for k in range(R):
+        st, x1 = _single(ctx, A, B, bs[k], x0_mode=N.X0_ZERO, use_graph=True, **kw)
+        assert stats[k]["niter"] == st["niter"], (k, stats[k]["niter"], st["niter"])
+        assert np.array_equal(xs[k], x1), k
+        assert stats[k]["true_rel_res"] <= 1e-9
 
 
 def test_multi_reference_start_and_repeat(ctx):
