@@ -521,35 +521,34 @@ template <class VT> struct DevHier : DevPrecBase {
         return b;
     }
 
+    // Each level kernel comes in four instances: one-wavefront (64) or kLBS
+    // blocks, and with or without the batched right-hand-side offsets.
     template <class FT, int IN, class XT, int DMODE>
     void down(const DevLevel<VT>& L, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
         const uint32_t grid = L.prog.nblk + cdiv(uint64_t(L.prog.nflat) + L.Md, L.lbs);
-        if (L.lbs == 64)
-            launch(k_level_down<VT, FT, IN, XT, DMODE, 64>, grid, L.lbs, L.lds_down, L.prog, f, L.wt, L.F, L.Md,
-                   (const VT*)L.dval, x, g, level_batch(L));
-        else
-            launch(k_level_down<VT, FT, IN, XT, DMODE, kLBS>, grid, L.lbs, L.lds_down, L.prog, f, L.wt, L.F, L.Md,
-                   (const VT*)L.dval, x, g, level_batch(L));
+        auto k = L.lbs == 64 ? (R > 1 ? k_level_down<VT, FT, IN, XT, DMODE, 64, true>
+                                      : k_level_down<VT, FT, IN, XT, DMODE, 64, false>)
+                             : (R > 1 ? k_level_down<VT, FT, IN, XT, DMODE, kLBS, true>
+                                      : k_level_down<VT, FT, IN, XT, DMODE, kLBS, false>);
+        launch(k, grid, L.lbs, L.lds_down, L.prog, f, L.wt, L.F, L.Md, (const VT*)L.dval, x, g, level_batch(L));
     }
     template <class XT, int OP, int UMODE>
     void up(const DevLevel<VT>& L, const Combine<VT>& C, XT* x, const uint32_t* g) const {
         const uint32_t grid = L.prog.nblk + cdiv(L.n, L.lbs);
-        if (L.lbs == 64)
-            launch(k_level_up<VT, XT, OP, UMODE, 64>, grid, L.lbs, L.lds_up, L.prog, (const VT*)L.wt, C,
-                   (const uint32_t*)(L.map + L.M), L.n, x, g, level_batch(L));
-        else
-            launch(k_level_up<VT, XT, OP, UMODE, kLBS>, grid, L.lbs, L.lds_up, L.prog, (const VT*)L.wt, C,
-                   (const uint32_t*)(L.map + L.M), L.n, x, g, level_batch(L));
+        auto k = L.lbs == 64 ? (R > 1 ? k_level_up<VT, XT, OP, UMODE, 64, true> : k_level_up<VT, XT, OP, UMODE, 64, false>)
+                             : (R > 1 ? k_level_up<VT, XT, OP, UMODE, kLBS, true>
+                                      : k_level_up<VT, XT, OP, UMODE, kLBS, false>);
+        launch(k, grid, L.lbs, L.lds_up, L.prog, (const VT*)L.wt, C, (const uint32_t*)(L.map + L.M), L.n, x, g,
+               level_batch(L));
     }
     template <class FT, int IN, class XT, int MODE>
     void solve_last(const DevLevel<VT>& L, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
         const uint32_t grid = L.prog.nblk + cdiv(L.Md, L.lbs);
-        if (L.lbs == 64)
-            launch(k_level_solve<VT, FT, IN, XT, MODE, 64>, grid, L.lbs, L.lds_solve, L.prog, f, L.Md,
-                   (const VT*)L.dval, x, g, level_batch(L));
-        else
-            launch(k_level_solve<VT, FT, IN, XT, MODE, kLBS>, grid, L.lbs, L.lds_solve, L.prog, f, L.Md,
-                   (const VT*)L.dval, x, g, level_batch(L));
+        auto k = L.lbs == 64 ? (R > 1 ? k_level_solve<VT, FT, IN, XT, MODE, 64, true>
+                                      : k_level_solve<VT, FT, IN, XT, MODE, 64, false>)
+                             : (R > 1 ? k_level_solve<VT, FT, IN, XT, MODE, kLBS, true>
+                                      : k_level_solve<VT, FT, IN, XT, MODE, kLBS, false>);
+        launch(k, grid, L.lbs, L.lds_solve, L.prog, f, L.Md, (const VT*)L.dval, x, g, level_batch(L));
     }
 
     static InSrc<VT, VT, IN_PLAIN> plain(const VT* f) {
@@ -607,7 +606,8 @@ template <class VT> struct DevHier : DevPrecBase {
         VT* prev = u1s;
         VT* spare = nullptr;
         for (VT* b : {L.u1, L.u0}) if (b != prev) spare = b;
-        launch(k_cheb2<VT, XT, true>, gb, kBS, 0, n, (const VT*)L.x2, (const VT*)prev, VT(L.ma2), L.s2_skip, VT(L.opb2),
+        auto cheb2 = R > 1 ? k_cheb2<VT, XT, true, true> : k_cheb2<VT, XT, true, false>;
+        launch(cheb2, gb, kBS, 0, n, (const VT*)L.x2, (const VT*)prev, VT(L.ma2), L.s2_skip, VT(L.opb2),
                L.opb2_one, (const uint32_t*)nullptr, (XT*)nullptr, spare, g, level_batch(L));
         VT* cur = spare;
         spare = L.x2;
@@ -618,7 +618,8 @@ template <class VT> struct DevHier : DevPrecBase {
                 C.x2 = spare; C.u1 = cur; C.u0 = prev; C.a = a; C.b = b;
                 up<XT, OP_CHEBK, 0>(L, C, x, g);
             } else {
-                launch(k_cheb_step<VT, XT, true>, gb, kBS, 0, n, (const VT*)cur, (const VT*)prev, (const VT*)spare, a, b,
+                auto step = R > 1 ? k_cheb_step<VT, XT, true, true> : k_cheb_step<VT, XT, true, false>;
+                launch(step, gb, kBS, 0, n, (const VT*)cur, (const VT*)prev, (const VT*)spare, a, b,
                        (const uint32_t*)nullptr, (XT*)nullptr, spare, g, level_batch(L));
                 VT* t = prev;
                 prev = cur;

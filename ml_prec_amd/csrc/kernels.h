@@ -638,11 +638,11 @@ __device__ __forceinline__ BlkHdr load_hdr(const uint32_t* blk) {
 // feed; the remaining blocks move the right-hand side of the other kept rows
 // into F and apply the isolated-node diagonal x[j] = dval * f[j] (DMODE 1
 // adds it to x).
-template <class VT, class FT, int IN, class XT, int DMODE, int BS>
+template <class VT, class FT, int IN, class XT, int DMODE, int BS, bool BAT>
 __global__ void __launch_bounds__(BS) k_level_down(ProgView<VT> P, InSrc<VT, FT, IN> f, VT* __restrict__ wt,
                                                      VT* __restrict__ F, uint32_t Md, const VT* __restrict__ dval,
                                                      XT* __restrict__ x, const uint32_t* guard, LevelBatch lb) {
-    {
+    if constexpr (BAT) {                 // batched right-hand sides (compiled out for single solves)
         const size_t y = blockIdx.y;
         f.f = shift(f.f, y * lb.N); f.u = shift(f.u, y * lb.N); f.F = shift(f.F, y * lb.N); f.us = shift(f.us, y * lb.N);
         wt = shift(wt, y * lb.nodes); P.gx = shift(P.gx, y * lb.nodes); F = shift(F, y * lb.n);
@@ -702,11 +702,11 @@ __global__ void __launch_bounds__(BS) k_level_down(ProgView<VT> P, InSrc<VT, FT,
 // Backward part of a level: component blocks run the U sweep (kept columns
 // from the combine op) and write their rows of x; the remaining blocks write
 // the kept rows x[map[M+r]] = combine(r) (UMODE 1 adds into x).
-template <class VT, class XT, int OP, int UMODE, int BS>
+template <class VT, class XT, int OP, int UMODE, int BS, bool BAT>
 __global__ void __launch_bounds__(BS) k_level_up(ProgView<VT> P, const VT* __restrict__ wt, Combine<VT> C,
                                                    const uint32_t* __restrict__ mapM, uint32_t n, XT* __restrict__ x,
                                                    const uint32_t* guard, LevelBatch lb) {
-    {
+    if constexpr (BAT) {                 // batched right-hand sides (compiled out for single solves)
         const size_t y = blockIdx.y;
         wt = shift(wt, y * lb.nodes); P.gx = shift(P.gx, y * lb.nodes);
         C.x2 = shift(C.x2, y * lb.n); C.u1 = shift(C.u1, y * lb.n); C.u0 = shift(C.u0, y * lb.n);
@@ -751,11 +751,11 @@ __global__ void __launch_bounds__(BS) k_level_up(ProgView<VT> P, const VT* __res
 
 // A whole level without kept rows (the coarsest level): L sweep, U sweep and
 // diagonal in one launch.  MODE 1 adds the result into x.
-template <class VT, class FT, int IN, class XT, int MODE, int BS>
+template <class VT, class FT, int IN, class XT, int MODE, int BS, bool BAT>
 __global__ void __launch_bounds__(BS) k_level_solve(ProgView<VT> P, InSrc<VT, FT, IN> f, uint32_t Md,
                                                       const VT* __restrict__ dval, XT* __restrict__ x,
                                                       const uint32_t* guard, LevelBatch lb) {
-    {
+    if constexpr (BAT) {                 // batched right-hand sides (compiled out for single solves)
         const size_t y = blockIdx.y;
         f.f = shift(f.f, y * lb.N); f.u = shift(f.u, y * lb.N); f.F = shift(f.F, y * lb.N); f.us = shift(f.us, y * lb.N);
         P.gx = shift(P.gx, y * lb.nodes); x = shift(x, y * lb.N); guard = shift(guard, y * lb.st);
@@ -808,12 +808,12 @@ __global__ void __launch_bounds__(BS) k_level_solve(ProgView<VT> P, InSrc<VT, FT
 }
 
 // step 2: x2 = (-alpha) x2 ; x2 += (1+beta) u1   (cheb_solve.cc:79-81), then store or scatter
-template <class VT, class XT, bool STORE>
+template <class VT, class XT, bool STORE, bool BAT>
 __global__ void __launch_bounds__(kBS) k_cheb2(uint32_t n, const VT* __restrict__ x2, const VT* __restrict__ u1s,
                                                VT ma, int skip, VT opb, int opb_one,
                                                const uint32_t* __restrict__ mapM, XT* __restrict__ x,
                                                VT* __restrict__ x2out, const uint32_t* guard, LevelBatch lb) {
-    {
+    if constexpr (BAT) {                 // batched right-hand sides (compiled out for single solves)
         const size_t y = blockIdx.y;
         x2 = shift(x2, y * lb.n); u1s = shift(u1s, y * lb.n); x2out = shift(x2out, y * lb.n);
         x = shift(x, y * lb.N); guard = shift(guard, y * lb.st);
@@ -830,12 +830,12 @@ __global__ void __launch_bounds__(kBS) k_cheb2(uint32_t n, const VT* __restrict_
 }
 
 // steps >= 3: x2 = u1 - alpha x2 + beta (u1 - u0)   (cheb_solve.cc:99-100)
-template <class VT, class XT, bool STORE>
+template <class VT, class XT, bool STORE, bool BAT>
 __global__ void __launch_bounds__(kBS) k_cheb_step(uint32_t n, const VT* __restrict__ u1, const VT* __restrict__ u0,
                                                    const VT* __restrict__ x2, VT a, VT b,
                                                    const uint32_t* __restrict__ mapM, XT* __restrict__ x,
                                                    VT* __restrict__ x2out, const uint32_t* guard, LevelBatch lb) {
-    {
+    if constexpr (BAT) {                 // batched right-hand sides (compiled out for single solves)
         const size_t y = blockIdx.y;
         u1 = shift(u1, y * lb.n); u0 = shift(u0, y * lb.n); x2 = shift(x2, y * lb.n); x2out = shift(x2out, y * lb.n);
         x = shift(x, y * lb.N); guard = shift(guard, y * lb.st);
