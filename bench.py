@@ -297,6 +297,161 @@ def adjoint(N, S, args):
     ctx.close()
 
 
+MSPLIT_CFG = dict(q=(0.8,), niters=(5,), order="simple_1", degree_max=2)   # project/msplit.cfg:7-77
+GMRES_CAP = 145       # five restart cycles of 29 steps
+SOLVER_CASES = (
+    # name, matrix, outer solver, preconditioner, iteration cap (None: solve to eps)
+    ("pcg_uh_cheb", "sym", "pcg", "uh_cheb", None),
+    ("pcg_multi_split", "sym", "pcg", "multi_split", None),
+    ("gmres_uh_cheb", "sym", "gmres", "uh_cheb", None),
+    ("cheb_solver_uh_cheb", "sym", "cheb", "uh_cheb", None),
+    ("simple_solver_multi_split", "sym", "simple", "multi_split", None),
+    # GMRES(30) with GS alone or GS + multi_split stalls on the 1e6-contrast nonsymmetric system (relative
+    # residuals around 1e-5 and 5e-4 after five cycles, in the oracle too): timed over five cycles
+    ("gmres_gs", "nonsym", "gmres", "gs", GMRES_CAP),
+    ("gmres_composite", "nonsym", "gmres", "composite", GMRES_CAP),
+    ("gmres_sym", "nonsym", "gmres", "sym", None),
+)
+
+
+def cpu_case_baseline(O, s, a, case, gpu_niter, budget_s):
+    """The C oracle (one thread) on the same case: a full solve to 1e-8 when the estimate (GPU iteration count x
+    one timed preconditioner apply, x2.5 for GMRES's orthogonalisation, x1.3 otherwise) fits `budget_s`;
+    otherwise only the preconditioner applies are priced (GPU iteration count x one apply), a lower bound of the
+    CPU solve, marked as such."""
+    _, _, solver, prec, cap = case
+    t0 = time.perf_counter()
+    if prec == "uh_cheb":
+        P = O.cheb_prec(s.ia, s.ja, a)
+    elif prec == "multi_split":
+        P = O.msplit_prec(s.ia, s.ja, a, **MSPLIT_CFG)
+    elif prec == "gs":
+        P = O.gs_prec(s.ia, s.ja, a)
+    elif prec == "composite":
+        P = O.composite_prec(O.gs_prec(s.ia, s.ja, a), O.msplit_prec(s.ia, s.ja, a, **MSPLIT_CFG))
+    else:
+        P = O.sym_prec(s.ia, s.ja, a)
+    t_const = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    P.apply(s.b)
+    t_apply = time.perf_counter() - t0
+    est = t_apply * gpu_niter * (2.5 if solver == "gmres" else 1.3)
+    rec = {"apply_ms": round(1e3 * t_apply, 3), "construction_s": round(t_const, 3), "cores": 1, "kind": "port"}
+    if est > budget_s:
+        rec.update(ms_per_solve=round(1e3 * t_apply * gpu_niter, 2), niter=None,
+                   sample=f"one preconditioner apply x {gpu_niter} iterations (lower bound; a full solve is "
+                          f"estimated at {est:.0f} s)")
+        return rec, None
+    kw = dict(eps=1e-8, x0=np.zeros(s.n), maxit=cap or 20000, check=cap is None)
+    t0 = time.perf_counter()
+    if solver == "pcg":
+        x, st = O.pcg(s.ia, s.ja, a, s.b, P, **kw)
+    elif solver == "gmres":
+        x, st = O.gmres(s.ia, s.ja, a, s.b, P, **kw)
+    elif solver == "cheb":
+        li = P.level_info(0)
+        x, st = O.cheb_solver(s.ia, s.ja, a, s.b, P, li["lmin"], li["lmax"], **kw)
+    else:
+        x, st = O.simple_solver(s.ia, s.ja, a, s.b, P, **kw)
+    rec.update(ms_per_solve=round(1e3 * (time.perf_counter() - t0), 2), niter=int(st["niter"]),
+               sample="one full solve" if cap is None else f"one solve capped at {cap} iterations")
+    return rec, x
+
+
+def solvers(N, S, args):
+    """Every outer solver with the preconditioners it is used with, on the 60x220x85 system (and, for GMRES
+    with GS / Composite / Sym, on the reference's nonsymmetric variant of it, spe.cc:138-182 with tau = 0.3):
+    device solve time to 1e-8 from x0 = 0, iterations, true residual, preconditioner apply time, and the
+    single-core C oracle on the same case.  Prints one JSON line with a record per case."""
+    s = S.config_system(1, kind="spe", c=1.0, nz=args.nz)
+    a_ns = S.make_nonsymmetric(s.ia, s.ja, s.a, 0.3)
+    cases = [c for c in SOLVER_CASES if not args.case or c[0] in args.case.split(",")]
+    if not cases:
+        raise SystemExit(f"--case: none of {args.case!r} in {[c[0] for c in SOLVER_CASES]}")
+    if not args.no_cpu:
+        import oracle as O
+    ctx = N.Context(0)
+    out = []
+    for case in cases:
+        name, mat, solver, prec, cap = case
+        a = s.a if mat == "sym" else a_ns
+        t0 = time.perf_counter()
+        A = N.DevMatrix(ctx, s.ia, s.ja, a)
+        H = None
+        if prec == "uh_cheb":
+            H = N.build_uh_cheb(s.ia, s.ja, a)
+            B = N.DevPrec(ctx, A, H)
+        elif prec == "multi_split":
+            H = N.build_multi_split(s.ia, s.ja, a, **MSPLIT_CFG)
+            B = N.DevPrec(ctx, A, H)
+        elif prec == "gs":
+            B = N.DevPrec(ctx, A, kind=N.GS)
+        elif prec == "composite":
+            H = N.build_multi_split(s.ia, s.ja, a, **MSPLIT_CFG)
+            B = N.DevPrec(ctx, A, H, kind=N.COMPOSITE)
+        else:
+            B = N.DevPrec(ctx, A, kind=N.SYM)
+        ctx.sync()
+        t_setup = time.perf_counter() - t0
+        b = ctx.to_device(s.b)
+        x = ctx.array(s.n)
+        r = ctx.array(s.n)
+        kw = dict(eps=args.eps, maxit=cap or 20000, x0_mode=N.X0_ZERO, raise_on_error=cap is None)
+        if solver == "pcg":
+            run = lambda: N.pcg(ctx, A, B, b, x, check_every=8, use_graph=bool(args.graph), **kw)
+        elif solver == "gmres":
+            run = lambda: N.gmres(ctx, A, B, b, x, **kw)
+        elif solver == "cheb":
+            li = H.level_info(0)
+            run = lambda: N.cheb_solver(ctx, A, B, li["lmin"], li["lmax"], b, x, check_every=8,
+                                        use_graph=bool(args.graph), **kw)
+        else:
+            run = lambda: N.simple_solver(ctx, A, B, b, x, check_every=8, use_graph=bool(args.graph), **kw)
+        for _ in range(max(1, args.warmup)):
+            run()
+        ctx.sync()
+        times = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            st = run()
+            ctx.sync()
+            times.append(1e3 * (time.perf_counter() - t0))
+            if st["rc"] not in (0, 5):                   # 5: the iteration cap of a capped case
+                raise RuntimeError(f"{name}: solve failed with status {st['rc']}")
+        A.residual(b, x, r)
+        true_res = N.dnrm2(ctx, r) / N.dnrm2(ctx, b)
+        f = ctx.to_device(np.random.default_rng(5).standard_normal(s.n))
+        apply_ms = B.time_apply(f, r, reps=10)
+        rec = {"case": name, "matrix": "SPE10-shaped 60x220x%d%s" % (args.nz, "" if mat == "sym" else
+                                                                      ", nonsymmetric (tau 0.3)"),
+               "solver": solver, "preconditioner": prec, "ms_per_solve": round(float(np.median(times)), 3),
+               "niter": int(st["niter"]), "iteration_cap": cap, "true_rel_res": true_res,
+               "apply_us": round(1e3 * apply_ms, 1), "setup_s": round(t_setup, 3)}
+        log(f"{name}: {rec['ms_per_solve']:.2f} ms/solve, {rec['niter']} iterations, true rel res {true_res:.2e}, "
+            f"apply {rec['apply_us']:.1f} us, setup {t_setup:.2f} s")
+        if not args.no_cpu:
+            cpu, xo = cpu_case_baseline(O, s, a, case, rec["niter"], args.cpu_budget)
+            rec["cpu_baseline"] = cpu
+            rec["speedup_vs_cpu"] = round(cpu["ms_per_solve"] / rec["ms_per_solve"], 1)
+            if xo is not None:
+                rec["rel_diff_vs_cpu"] = float(np.linalg.norm(x.get() - xo) / np.linalg.norm(xo))
+            log(f"  oracle: {cpu['ms_per_solve']:.1f} ms/solve ({cpu['sample']}), iterations {cpu['niter']}"
+                f"{', solution rel diff %.1e' % rec['rel_diff_vs_cpu'] if xo is not None else ''}")
+        out.append(rec)
+        B.close()
+        A.close()
+        del H, b, x, r, f
+    print(json.dumps({
+        "metric": "solve time to 1e-8 rel-res @1.12M cells per outer solver and preconditioner",
+        "value": round(sum(r["ms_per_solve"] for r in out), 3), "unit": "ms (sum over cases)", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"SPE10-shaped 60x220x{args.nz}, reference spe.cc assembly c=1, x0=0, eps "
+                               f"{args.eps:g}; multi_split with project/msplit.cfg settings"},
+        "cases": out}), flush=True)
+    ctx.close()
+
+
 def spmv_traffic_probe(N, s, reps):
     """Cold SpMV launches plus a calibration kernel of known traffic: x = 0.5 x
     over 2^27 doubles reads and writes exactly 1 GiB each.  Used under
@@ -339,6 +494,11 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="config 4 instead: misfit gradient by the adjoint solve, checked by central differences")
     ap.add_argument("--fd-dirs", type=int, default=20)
+    ap.add_argument("--solvers", action="store_true",
+                    help="every outer solver / preconditioner pair at 1.12M cells instead; prints its own JSON line")
+    ap.add_argument("--case", default="", help="with --solvers: comma-separated case names to run")
+    ap.add_argument("--cpu-budget", type=float, default=25.0,
+                    help="with --solvers: estimated seconds of single-core oracle solve allowed per case")
     ap.add_argument("--spmv-traffic", action="store_true",
                     help="only run cold SpMV launches and a 1 GiB scale kernel (for HBM counter runs), then exit")
     args = ap.parse_args()
@@ -358,6 +518,9 @@ def main():
         return
     if args.ensemble_rhs:
         ensemble_rhs(N, S, args)
+        return
+    if args.solvers:
+        solvers(N, S, args)
         return
 
     t0 = time.perf_counter()

@@ -1753,11 +1753,21 @@ void enqueue_pcg_start(mlp_ctx ctx, mlp_prec B, uint32_t n, bool flex) {
     enqueue_pcg_start(ctx, B->impl.get(), n, flex, ctx_vecs(ctx, n));
 }
 
+// MLP_GRAPHS=0 turns hipGraph capture off in every solver, whatever the
+// options ask (for tracing tools that follow plain launches only).
+bool graphs_allowed() {
+    static const bool on = [] {
+        const char* e = getenv("MLP_GRAPHS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 // The cached graphs of one PCG iteration (plain and, with true_res_every,
 // residual-replacing) for these operands; nullptr when graphs are off.
 mlp_ctx_s::GraphEntry* pcg_graphs(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, bool flex,
                                   uint32_t tre, bool use_graph) {
-    if (!use_graph) return nullptr;
+    if (!use_graph || !graphs_allowed()) return nullptr;
     for (auto& e : ctx->graphs)
         if (e.solver == 0 && e.A == A->id && e.B == B->id && e.b == b && e.x == x && e.flex == int(flex) &&
             e.tre == tre && e.ws_n == ctx->ws_n)
@@ -1936,7 +1946,7 @@ SolveState run_outer(mlp_ctx ctx, int solver, mlp_mat A, mlp_prec B, const void*
                      const mlp_solve_opts* o, Iter iteration, Pick pick, Before before_chunk) {
     hipStream_t s = ctx->stream;
     const uint32_t chunk = o->check_every ? o->check_every : 16;
-    const bool use_graph = o->use_graph && B->impl->graph_safe();
+    const bool use_graph = o->use_graph && B->impl->graph_safe() && graphs_allowed();
     mlp_ctx_s::GraphEntry* ge = nullptr;
     if (use_graph) {
         for (auto& e : ctx->graphs)
@@ -2059,7 +2069,8 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
                        (double*)nullptr, V.bt);
     enqueue_pcg_start(ctx, W, n, flex, V);
     mlp_ctx_s::GraphEntry* ge =
-        o->use_graph && W->graph_safe() ? multi_graph(ctx, A, B, W, b, x, flex, tre, V) : nullptr;
+        o->use_graph && W->graph_safe() && graphs_allowed() ? multi_graph(ctx, A, B, W, b, x, flex, tre, V)
+                                                            : nullptr;
     const uint32_t chunk = o->check_every ? o->check_every : 16;
     uint32_t c = chunk, k = 0;
     while (true) {
