@@ -743,7 +743,9 @@ struct DevGS : DevPrecBase {
     uint32_t n = 0, nnzl = 0;
     DevMem mem;
     uint32_t *rows = nullptr, *ptr = nullptr, *col = nullptr;
+    uint8_t* cnt = nullptr;
     double *val = nullptr, *diag = nullptr;
+    uint32_t W = 0;                        // fixed lower-entry slots per row (0: row pointers)
     std::vector<uint32_t> lvl;             // level k: level-order rows [lvl[k], lvl[k+1])
 
     void build(const std::vector<uint32_t>& ia, const std::vector<uint32_t>& ja, const std::vector<double>& a) {
@@ -780,21 +782,49 @@ struct DevGS : DevPrecBase {
             hptr[k + 1] = uint32_t(hcol.size());
         }
         nnzl = uint32_t(hcol.size());
-        if (hcol.empty()) { hcol.push_back(0); hval.push_back(0.0); }
+        uint32_t wmax = 0;
+        for (uint32_t k = 0; k < n; k++) wmax = std::max(wmax, hptr[k + 1] - hptr[k]);
         hipStream_t s = ctx->stream;
+        if (wmax <= 8) {
+            // fixed slots: the common case (a seven-point matrix has at most three lower entries)
+            W = wmax == 0 ? 1 : wmax <= 4 ? wmax : 8;
+            std::vector<uint8_t> hcnt(n);
+            std::vector<uint32_t> ecol(size_t(W) * n, 0);
+            std::vector<double> eval(size_t(W) * n, 0.0);
+            for (uint32_t k = 0; k < n; k++) {
+                hcnt[k] = uint8_t(hptr[k + 1] - hptr[k]);
+                for (uint32_t q = hptr[k]; q < hptr[k + 1]; q++) {
+                    ecol[size_t(q - hptr[k]) * n + k] = hcol[q];
+                    eval[size_t(q - hptr[k]) * n + k] = hval[q];
+                }
+            }
+            cnt = mem.upload(hcnt, s);
+            col = mem.upload(ecol, s);
+            val = mem.upload(eval, s);
+        } else {
+            W = 0;
+            if (hcol.empty()) { hcol.push_back(0); hval.push_back(0.0); }
+            ptr = mem.upload(hptr, s);
+            col = mem.upload(hcol, s);
+            val = mem.upload(hval, s);
+        }
         rows = mem.upload(order, s);
-        ptr = mem.upload(hptr, s);
-        col = mem.upload(hcol, s);
-        val = mem.upload(hval, s);
         diag = mem.upload(hdiag, s);
         MLP_HIP_CHECK(hipStreamSynchronize(s));
     }
     void apply(const double* f, double* x, const uint32_t* guard) override {
+        auto ell = W == 1 ? k_gs_level_ell<1> : W == 2 ? k_gs_level_ell<2> : W == 3 ? k_gs_level_ell<3>
+                 : W == 4 ? k_gs_level_ell<4> : k_gs_level_ell<8>;
         for (size_t k = 0; k + 1 < lvl.size(); k++) {
             const uint32_t r0 = lvl[k], r1 = lvl[k + 1];
-            hipLaunchKernelGGL(k_gs_level, dim3(cdiv(r1 - r0, kBS)), dim3(kBS), 0, ctx->stream, r0, r1,
-                               (const uint32_t*)rows, (const uint32_t*)ptr, (const uint32_t*)col, (const double*)val,
-                               (const double*)diag, f, x, guard);
+            if (W)
+                hipLaunchKernelGGL(ell, dim3(cdiv(r1 - r0, kBS)), dim3(kBS), 0, ctx->stream, r0, r1, n,
+                                   (const uint32_t*)rows, (const uint8_t*)cnt, (const uint32_t*)col,
+                                   (const double*)val, (const double*)diag, f, x, guard);
+            else
+                hipLaunchKernelGGL(k_gs_level, dim3(cdiv(r1 - r0, kBS)), dim3(kBS), 0, ctx->stream, r0, r1,
+                                   (const uint32_t*)rows, (const uint32_t*)ptr, (const uint32_t*)col,
+                                   (const double*)val, (const double*)diag, f, x, guard);
         }
     }
     double bytes_per_apply() const override { return 12.0 * nnzl + 28.0 * n; }
@@ -2362,21 +2392,20 @@ int mlp_gmres(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, co
             B->impl->apply(v(i), z, nullptr);
             hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(n, kBS)), dim3(kBS), 0, s, A->A.v,
                                (const double*)z, v(i + 1), (const uint32_t*)nullptr);
+            // Gram-Schmidt column and the new vector's norm stay on the device until
+            // one copy brings h_0..h_i and (v_{i+1}, v_{i+1}) to the host
             hipLaunchKernelGGL(k_mdot, dim3(mdot_grid(n)), dim3(kBS), 0, s, n, i + 1, (const double*)ctx->V, ld,
                                (const double*)v(i + 1), ctx->partials);
-            hipLaunchKernelGGL(k_mdot_finalize, dim3(1), dim3(kBS), 0, s, mdot_grid(n), i + 1,
+            hipLaunchKernelGGL(k_mdot_finalize, dim3(i + 1), dim3(kBS), 0, s, mdot_grid(n), i + 1,
                                (const double*)ctx->partials, ctx->mdot);
-            MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, ctx->mdot, (i + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+            hipLaunchKernelGGL(k_cgs_update, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->V, ld,
+                               (const double*)ctx->mdot, i + 1, v(i + 1), ctx->partials);
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, int(FIN_DOT), rg, (const double*)ctx->partials,
+                               ctx->st, ctx->mdot + i + 1, Batch{});
+            MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, ctx->mdot, (i + 2) * sizeof(double), hipMemcpyDeviceToHost, s));
             MLP_HIP_CHECK(hipStreamSynchronize(s));
-            MaxpyCoef cf;
-            cf.nk = i + 1;
-            for (uint32_t k = 0; k <= i; k++) {
-                hh(k, i) = ctx->host_pinned[k];
-                cf.c[k] = -hh(k, i);
-                cf.one[k] = host_is_one(-hh(k, i));
-            }
-            hipLaunchKernelGGL(k_maxpy, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->V, ld, cf, v(i + 1), 0);
-            hh(i + 1, i) = std::sqrt(dev_dot(ctx, n, v(i + 1), v(i + 1)));
+            for (uint32_t k = 0; k <= i; k++) hh(k, i) = ctx->host_pinned[k];
+            hh(i + 1, i) = std::sqrt(ctx->host_pinned[i + 1]);
             scal(1. / hh(i + 1, i), v(i + 1));
             for (uint32_t k = 1; k <= i; k++) {
                 const uint32_t k1 = k - 1;

@@ -1192,29 +1192,93 @@ __global__ void __launch_bounds__(kBS) k_gs_level(uint32_t r0, uint32_t r1, cons
     x[i] = d / diag[k];
 }
 
-// GMRES: the i+1 classical Gram-Schmidt dots (v_{i+1}, v_k), k = 0..i (gmres.cc:80-82)
+// The same level with the lower entries in W fixed slots per level-order
+// position (slot t of position k at t * nrow + k, cnt[k] of them used), so the
+// row, its entries and its diagonal load together and only x[col] and f[row]
+// wait for them: two dependent load rounds instead of four.
+template <int W>
+__global__ void __launch_bounds__(kBS) k_gs_level_ell(uint32_t r0, uint32_t r1, uint32_t nrow,
+                                                      const uint32_t* __restrict__ rows,
+                                                      const uint8_t* __restrict__ cnt,
+                                                      const uint32_t* __restrict__ col,
+                                                      const double* __restrict__ val,
+                                                      const double* __restrict__ diag,
+                                                      const double* __restrict__ f, double* __restrict__ x,
+                                                      const uint32_t* guard) {
+    if (guarded(guard)) return;
+    const uint32_t k = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= r1) return;
+    const uint32_t i = rows[k], c = cnt[k];
+    const double dg = diag[k];
+    uint32_t cl[W];
+    double vl[W];
+#pragma unroll
+    for (int t = 0; t < W; t++) {
+        if (uint32_t(t) < c) {
+            cl[t] = col[size_t(t) * nrow + k];
+            vl[t] = val[size_t(t) * nrow + k];
+        }
+    }
+    double d = f[i];
+#pragma unroll
+    for (int t = 0; t < W; t++)
+        if (uint32_t(t) < c) d -= vl[t] * x[cl[t]];
+    x[i] = d / dg;
+}
+
+// GMRES: the i+1 classical Gram-Schmidt dots (v_{i+1}, v_k), k = 0..i (gmres.cc:80-82).
+// The k loops are unrolled to the 32-slot maximum under a k < nv test, so the
+// sums stay in registers and the nv loads of a row are in flight together.
 __global__ void __launch_bounds__(kBS) k_mdot(uint32_t n, uint32_t nv, const double* __restrict__ V, size_t ld,
                                               const double* __restrict__ w, double* partials) {
     double s[32];
+#pragma unroll
     for (uint32_t k = 0; k < 32; k++) s[k] = 0.0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double wi = w[i];
-        for (uint32_t k = 0; k < nv; k++) s[k] += wi * V[k * ld + i];
+#pragma unroll
+        for (uint32_t k = 0; k < 32; k++)
+            if (k < nv) s[k] += wi * V[k * ld + i];
     }
     __shared__ double sh[kBS / 64];
-    for (uint32_t k = 0; k < nv; k++) {
-        const double b = block_sum<kBS>(s[k], sh);
-        if (threadIdx.x == 0) partials[32 * blockIdx.x + k] = b;
+#pragma unroll
+    for (uint32_t k = 0; k < 32; k++) {
+        if (k < nv) {
+            const double b = block_sum<kBS>(s[k], sh);
+            if (threadIdx.x == 0) partials[32 * blockIdx.x + k] = b;
+        }
     }
 }
 
+// out[k] = the sum of the nb partials of dot k; block k adds dot k
 __global__ void __launch_bounds__(kBS) k_mdot_finalize(uint32_t nb, uint32_t nv, const double* __restrict__ partials,
                                                        double* out) {
     __shared__ double sh[kBS / 64];
-    for (uint32_t k = 0; k < nv; k++) {
-        const double a = sum_partials<kBS>(partials, nb, 32, int(k), sh);
-        if (threadIdx.x == 0) out[k] = a;
+    const uint32_t k = blockIdx.x;
+    if (k >= nv) return;
+    const double a = sum_partials<kBS>(partials, nb, 32, int(k), sh);
+    if (threadIdx.x == 0) out[k] = a;
+}
+
+// The Gram-Schmidt update y -= sum_k h_k v_k with h from device memory, one
+// daxpy(-h_k) at a time in k order (gmres.cc:83-84), and the partials of
+// (y, y) with k_dot's grid-stride order and block sums, so the norm that
+// follows is the one dnrm2(y) gives (gmres.cc:85).
+__global__ void __launch_bounds__(kBS) k_cgs_update(uint32_t n, const double* __restrict__ V, size_t ld,
+                                                   const double* __restrict__ h, uint32_t nk, double* __restrict__ y,
+                                                   double* partials) {
+    double s = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double acc = y[i];
+        for (uint32_t k = 0; k < nk; k++) {
+            const double c = -h[k], v = V[k * ld + i];
+            acc = dev_is_one(c) ? acc + v : acc + c * v;            // daxpy's alpha == 1 path, blas.cc:31-38
+        }
+        y[i] = acc;
+        s += acc * acc;
     }
+    double part[1] = {s};
+    block_partials<kBS, 1>(part, partials);
 }
 
 struct MaxpyCoef {

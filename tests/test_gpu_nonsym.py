@@ -85,3 +85,35 @@ def test_gmres_nonsym_vs_oracle(ctx, prec):
     sg = N.gmres(ctx, A, B, dev(ctx, s.b), x, eps=1e-8, maxit=3000, x0_mode=N.X0_REFERENCE)
     assert abs(int(sg["niter"]) - so["niter"]) <= 1, (sg["niter"], so["niter"])
     assert np.linalg.norm(x.get() - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def _random_lower(n, k, seed):
+    """Skyline CSR of an n x n M-matrix whose row i has min(i, k) lower entries (random columns) and their
+    mirrors, with random nonsymmetric values and a dominant positive diagonal."""
+    g = rng(seed)
+    rows = [set() for _ in range(n)]
+    for i in range(1, n):
+        for j in g.choice(i, size=min(i, k), replace=False):
+            rows[i].add(int(j))
+            rows[int(j)].add(i)
+    ia, ja, a = [0], [], []
+    for i in range(n):
+        cols = sorted(rows[i])
+        off = -g.uniform(0.1, 1.0, len(cols))
+        ja += [i] + cols
+        a += [float(-off.sum() + 1.0)] + off.tolist()
+        ia.append(len(ja))
+    return np.array(ia, np.uint32), np.array(ja, np.uint32), np.array(a)
+
+
+@pytest.mark.parametrize("k", [1, 3, 6, 12])
+def test_gs_apply_row_widths(ctx, k):
+    """Up to 4 lower entries a row use that many fixed slots, 5..8 use 8 slots, more use row pointers:
+    every width gives the oracle's sweep bit for bit."""
+    ia, ja, a = _random_lower(3000, k, 70 + k)
+    A = N.DevMatrix(ctx, ia, ja, a)
+    B = N.DevPrec(ctx, A, kind=N.GS)
+    f = rng(64).standard_normal(len(ia) - 1)
+    x = ctx.array(len(ia) - 1)
+    B.apply(dev(ctx, f), x)
+    assert np.array_equal(x.get(), O.gs_prec(ia, ja, a).apply(f))
