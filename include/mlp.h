@@ -102,6 +102,14 @@ int mlp_hier_build_uh_cheb(uint32_t n, const uint32_t* ia, const uint32_t* ja, c
                            const double* sigmas, uint32_t nsigmas,
                            const uint32_t* niters, uint32_t nniters,
                            int use_tails, mlp_hier* out);
+/* The same construction with each level's link phase on ctx's device: row sums, the magnitude order of
+ * the links (psort, cheb_constr.cc:26-36), the budget marks (cheb_constr.cc:62-103, to_remove
+ * cheb_prec.h:71-99) and the two-sided counters and degrees (cheb_misc.h:45-96).  The tail walk and the
+ * sparse LU stay on the host.  The hierarchy is identical to mlp_hier_build_uh_cheb's. */
+int mlp_hier_build_uh_cheb_dev(mlp_ctx ctx, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
+                               const double* sigmas, uint32_t nsigmas,
+                               const uint32_t* niters, uint32_t nniters,
+                               int use_tails, mlp_hier* out);
 
 typedef struct {
     const double* q;        uint32_t nq;          /* key q            */

@@ -68,6 +68,8 @@ struct TwoSidedLinks {
         });
         if (asym) throw Error("uh_cheb needs a structurally symmetric matrix", 1);
     }
+    // counters and twin positions computed elsewhere (ChebLinkPhase)
+    TwoSidedLinks(const HostCSR& M, std::vector<uint32_t>&& c) : A(&M), cnt(std::move(c)) {}
     uint32_t slot(uint32_t row, uint32_t p) const { return A->ja[p] < row ? cnt[p] : p; }
     bool present(uint32_t row, uint32_t p) const { return cnt[slot(row, p)] != 0; }
     bool mark(uint32_t row, uint32_t p) { return --cnt[slot(row, p)] == 0; }   // true: removed from both ends
@@ -233,6 +235,7 @@ struct ChebBuilder {
     Hierarchy& H;
     bool use_tails;
     uint32_t max_levels;
+    ChebLinkPhase* dev = nullptr;
 
     // Tails first (chains of nodes left with one link), isolated nodes last
     // (cheb_mark.cc:8-92).
@@ -287,8 +290,15 @@ struct ChebBuilder {
         li.nnz = A.nnz();
         const uint32_t N = li.N;
         std::vector<int> deg(N);
-        TwoSidedLinks links(A);
         li.aux.resize(N);
+        if (dev) {
+            std::vector<uint32_t> cnt;
+            dev->run(A, li.beta, cnt, deg, li.aux);
+            TwoSidedLinks links(A, std::move(cnt));
+            finish(l, A, links, deg);
+            return;
+        }
+        TwoSidedLinks links(A);
         // drop weak links against the row budget (cheb_constr.cc:62-103, to_remove cheb_prec.h:71-99).
         // A row's choice depends only on its own values, so the rows choose in
         // parallel (marked[p]: row i asks to drop the link at its entry p); a
@@ -327,6 +337,13 @@ struct ChebBuilder {
                 deg[i] = d;
             }
         });
+        finish(l, A, links, deg);
+    }
+
+    // tails, permutation, elimination and the next level (cheb_constr.cc:104-161)
+    void finish(uint32_t l, const HostCSR& A, TwoSidedLinks& links, std::vector<int>& deg) {
+        HostLevel& li = H.levels[l];
+        const uint32_t N = li.N;
         permutation(A, links, deg, li);
         li.rmap = inverse(li.map);
         HostLevel& ln = H.levels[l + 1];
@@ -535,7 +552,7 @@ double cheb_poly(double x, uint32_t k) {
     }
 }
 
-void build_uh_cheb(const HostCSR& A, const ChebParams& p, Hierarchy& H) {
+void build_uh_cheb(const HostCSR& A, const ChebParams& p, Hierarchy& H, ChebLinkPhase* dev) {
     if (p.sigmas.empty() || p.niters.empty()) throw Error("uh_cheb needs at least one sigma and one niter");
     for (double s : p.sigmas)
         if (!(s > 1)) throw Error("All sigmas must be > 1");
@@ -549,7 +566,7 @@ void build_uh_cheb(const HostCSR& A, const ChebParams& p, Hierarchy& H) {
         H.levels[l].alpha = 1.0;
         H.levels[l].beta = l < p.sigmas.size() ? p.sigmas[l] : p.sigmas.back();
     }
-    ChebBuilder b{H, p.use_tails, kMaxLevels};
+    ChebBuilder b{H, p.use_tails, kMaxLevels, dev};
     b.level(0, A);
     H.levels.resize(H.nlevels + 1);    // keep one slot: the last level's (empty) successor
     H.levels.back() = HostLevel();

@@ -64,8 +64,22 @@ struct Hierarchy {
     const HostCSR* A0 = nullptr;     // level-0 matrix, kept by reference like level0_A
 };
 
+// The link phase of one uh_cheb level (cheb_constr.cc:62-103 with the two-sided
+// counters of cheb_misc.h:45-96), computed somewhere other than the host
+// threads (a device).  For the level matrix A and the level's beta it fills
+// aux[i] = the row sum of row i (storage order), deg[i] = the links row i keeps,
+// and cnt[p] per stored entry: 2 on the diagonal; 2 minus the number of end
+// points that asked to drop the link on an upper entry; the position of the
+// twin entry (j, i) on a lower entry.  Everything must equal the host's values;
+// a structurally asymmetric A throws the host's error.
+struct ChebLinkPhase {
+    virtual ~ChebLinkPhase() = default;
+    virtual void run(const HostCSR& A, double beta, std::vector<uint32_t>& cnt, std::vector<int>& deg,
+                     std::vector<double>& aux) = 0;
+};
+
 // Throws mlp::Error (derived from std::runtime_error) with the reference's messages.
-void build_uh_cheb(const HostCSR& A, const ChebParams& p, Hierarchy& H);
+void build_uh_cheb(const HostCSR& A, const ChebParams& p, Hierarchy& H, ChebLinkPhase* dev = nullptr);
 void build_multi_split(const HostCSR& A, const MultiSplitParams& p, Hierarchy& H);
 
 // cheb(x,k): modules/prec/misc/misc.cc:232-242 (host libm for k >= 4)

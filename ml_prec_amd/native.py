@@ -87,6 +87,8 @@ EXPORTS = {
     "mlp_dscal": (C.c_int, [_vp, C.c_uint32, C.c_double, _vp]),
     "mlp_hier_build_uh_cheb": (C.c_int, [C.c_uint32, _u32p, _u32p, _f64p, _f64p, C.c_uint32, _u32p, C.c_uint32,
                                          C.c_int, C.POINTER(_vp)]),
+    "mlp_hier_build_uh_cheb_dev": (C.c_int, [_vp, C.c_uint32, _u32p, _u32p, _f64p, _f64p, C.c_uint32, _u32p,
+                                            C.c_uint32, C.c_int, C.POINTER(_vp)]),
     "mlp_hier_build_multi_split": (C.c_int, [C.c_uint32, _u32p, _u32p, _f64p, C.c_char_p, C.POINTER(MsplitParams),
                                              C.POINTER(_vp)]),
     "mlp_hier_destroy": (C.c_int, [_vp]),
@@ -197,12 +199,17 @@ class Hierarchy:
         return info, out
 
 
-def build_uh_cheb(ia, ja, a, sigmas=(3.0,), niters=(2,), use_tails=True):
-    """Prec(A, cfg) construction (defaults: sigma 3, ncheb 2, tails on)."""
+def build_uh_cheb(ia, ja, a, sigmas=(3.0,), niters=(2,), use_tails=True, ctx=None):
+    """Prec(A, cfg) construction (defaults: sigma 3, ncheb 2, tails on).  With a Context, each level's link
+    phase runs on its device (mlp_hier_build_uh_cheb_dev); the hierarchy is the same."""
     h = _vp()
     n = len(ia) - 1
-    check(lib().mlp_hier_build_uh_cheb(n, _u32(ia), _u32(ja), _f64(a), _f64(sigmas), len(sigmas), _u32(niters),
-                                       len(niters), int(use_tails), C.byref(h)))
+    args = (n, _u32(ia), _u32(ja), _f64(a), _f64(sigmas), len(sigmas), _u32(niters), len(niters), int(use_tails),
+            C.byref(h))
+    if ctx is None:
+        check(lib().mlp_hier_build_uh_cheb(*args))
+    else:
+        check(lib().mlp_hier_build_uh_cheb_dev(ctx.h, *args))
     return Hierarchy(h, UH_CHEB, n)
 
 
