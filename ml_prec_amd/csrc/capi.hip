@@ -747,6 +747,66 @@ struct DevGS : DevPrecBase {
     double *val = nullptr, *diag = nullptr;
     uint32_t W = 0;                        // fixed lower-entry slots per row (0: row pointers)
     std::vector<uint32_t> lvl;             // level k: level-order rows [lvl[k], lvl[k+1])
+    // brick sweep of a seven-point lower triangle (k_gs_brick): grid sizes, natural-order values, presence
+    // bits, brick codes grouped by brick wavefront [blev[w], blev[w+1])
+    uint32_t gx = 0, gy = 0, gz = 0;
+    double *bvz = nullptr, *bvy = nullptr, *bvx = nullptr;
+    uint8_t* bpres = nullptr;
+    uint32_t* bricks = nullptr;
+    std::vector<uint32_t> blev;
+
+    // The lower entries of every row are at offsets {nx ny, nx, 1} of a grid
+    // numbered x fastest (each only where that neighbour exists): set up the
+    // brick sweep and return true.
+    bool plan_bricks(const std::vector<uint32_t>& ia, const std::vector<uint32_t>& ja, const std::vector<double>& a,
+                     const std::vector<uint32_t>& lo_end) {
+        uint32_t off[3] = {0, 0, 0};
+        int no = 0;
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t e = ia[i] + 1; e < lo_end[i]; e++) {
+                const uint32_t d = i - ja[e];
+                if (d == off[0] || d == off[1] || d == off[2]) continue;
+                if (no == 3) return false;
+                off[no++] = d;
+            }
+        if (no != 3) return false;
+        std::sort(off, off + 3);
+        if (off[0] != 1 || off[1] < 2 || off[2] % off[1] || n % off[2]) return false;
+        const uint32_t nx = off[1], ny = off[2] / off[1], nz = n / off[2];
+        const uint32_t BX = (nx + kBrick - 1) / kBrick, BY = (ny + kBrick - 1) / kBrick, BZ = (nz + kBrick - 1) / kBrick;
+        if (ny < 2 || BX > 1024 || BY > 1024 || BZ > 4096) return false;
+        const uint32_t nxy = nx * ny;
+        std::vector<double> hz(n, 0.0), hy(n, 0.0), hx(n, 0.0), hd(n);
+        std::vector<uint8_t> hp(n, 0);
+        for (uint32_t i = 0; i < n; i++) {
+            hd[i] = a[ia[i]];
+            for (uint32_t e = ia[i] + 1; e < lo_end[i]; e++) {
+                const uint32_t d = i - ja[e];
+                if (d == nxy && i >= nxy) { hz[i] = a[e]; hp[i] |= 1; }
+                else if (d == nx && (i / nx) % ny != 0) { hy[i] = a[e]; hp[i] |= 2; }
+                else if (d == 1 && i % nx != 0) { hx[i] = a[e]; hp[i] |= 4; }
+                else return false;                       // an offset that wraps around the grid
+            }
+        }
+        blev.assign(BX + BY + BZ - 1, 0);
+        for (uint32_t z = 0; z < BZ; z++)
+            for (uint32_t y = 0; y < BY; y++)
+                for (uint32_t x = 0; x < BX; x++) blev[x + y + z + 1]++;
+        for (size_t w = 0; w + 1 < blev.size(); w++) blev[w + 1] += blev[w];
+        std::vector<uint32_t> codes(size_t(BX) * BY * BZ), fill(blev.begin(), blev.end() - 1);
+        for (uint32_t z = 0; z < BZ; z++)
+            for (uint32_t y = 0; y < BY; y++)
+                for (uint32_t x = 0; x < BX; x++) codes[fill[x + y + z]++] = x | (y << 10) | (z << 20);
+        gx = nx; gy = ny; gz = nz;
+        hipStream_t s = ctx->stream;
+        diag = mem.upload(hd, s);
+        bvz = mem.upload(hz, s);
+        bvy = mem.upload(hy, s);
+        bvx = mem.upload(hx, s);
+        bpres = mem.upload(hp, s);
+        bricks = mem.upload(codes, s);
+        return true;
+    }
 
     void build(const std::vector<uint32_t>& ia, const std::vector<uint32_t>& ja, const std::vector<double>& a) {
         n = uint32_t(ia.size()) - 1;
@@ -765,6 +825,9 @@ struct DevGS : DevPrecBase {
             lo_end[i] = e;
             nlev = std::max(nlev, lev[i] + 1);
         }
+        for (uint32_t i = 0; i < n; i++) nnzl += lo_end[i] - ia[i] - 1;
+        if (plan_bricks(ia, ja, a, lo_end)) return;
+        nnzl = 0;
         lvl.assign(nlev + 1, 0);
         for (uint32_t i = 0; i < n; i++) lvl[lev[i] + 1]++;
         for (uint32_t k = 0; k < nlev; k++) lvl[k + 1] += lvl[k];
@@ -813,6 +876,14 @@ struct DevGS : DevPrecBase {
         MLP_HIP_CHECK(hipStreamSynchronize(s));
     }
     void apply(const double* f, double* x, const uint32_t* guard) override {
+        if (bricks) {
+            for (size_t w = 0; w + 1 < blev.size(); w++)
+                hipLaunchKernelGGL(k_gs_brick, dim3(blev[w + 1] - blev[w]), dim3(kBrick * kBrick * kBrick), 0,
+                                   ctx->stream, gx, gy, gz, (const uint32_t*)bricks, blev[w], (const double*)diag,
+                                   (const double*)bvz, (const double*)bvy, (const double*)bvx,
+                                   (const uint8_t*)bpres, f, x, guard);
+            return;
+        }
         auto ell = W == 1 ? k_gs_level_ell<1> : W == 2 ? k_gs_level_ell<2> : W == 3 ? k_gs_level_ell<3>
                  : W == 4 ? k_gs_level_ell<4> : k_gs_level_ell<8>;
         for (size_t k = 0; k + 1 < lvl.size(); k++) {
@@ -828,7 +899,9 @@ struct DevGS : DevPrecBase {
         }
     }
     double bytes_per_apply() const override { return 12.0 * nnzl + 28.0 * n; }
-    uint32_t launches_per_apply() const override { return uint32_t(lvl.size()) - 1; }
+    uint32_t launches_per_apply() const override {
+        return bricks ? uint32_t(blev.size()) - 1 : uint32_t(lvl.size()) - 1;
+    }
 };
 
 // CompositePrec (modules/prec/composite/composite_prec.cc:9-36): x = B2(B1(f)).

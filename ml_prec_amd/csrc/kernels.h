@@ -1226,6 +1226,54 @@ __global__ void __launch_bounds__(kBS) k_gs_level_ell(uint32_t r0, uint32_t r1, 
     x[i] = d / dg;
 }
 
+// The forward Gauss-Seidel sweep of a matrix whose lower triangle is a
+// seven-point grid (row i = x + nx (y + ny z) has lower entries at most at
+// i - nx ny, i - nx, i - 1), by 8x8x8 bricks: a launch runs every brick of one
+// brick wavefront (bx + by + bz = const), whose -x, -y, -z neighbour bricks
+// ran in earlier launches; inside a brick the cells go by diagonals
+// (lx + ly + lz = s) through LDS.  Each row is the row-pointer sweep's
+// operation sequence: d = f_i, minus a_ij x_j over its lower entries in
+// ascending column order (z, y, x), then d / a_ii.  pres bits: 1 z, 2 y, 4 x.
+constexpr int kBrick = 8;
+__global__ void __launch_bounds__(kBrick * kBrick * kBrick)
+k_gs_brick(uint32_t nx, uint32_t ny, uint32_t nz, const uint32_t* __restrict__ bricks, uint32_t b0,
+           const double* __restrict__ diag, const double* __restrict__ vz, const double* __restrict__ vy,
+           const double* __restrict__ vx, const uint8_t* __restrict__ pres, const double* __restrict__ f,
+           double* __restrict__ x, const uint32_t* guard) {
+    if (guarded(guard)) return;
+    __shared__ double sx[kBrick][kBrick][kBrick];
+    const uint32_t code = bricks[b0 + blockIdx.x];
+    const uint32_t lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
+    const uint32_t cx = (code & 0x3ffu) * kBrick + lx, cy = ((code >> 10) & 0x3ffu) * kBrick + ly,
+                   cz = (code >> 20) * kBrick + lz;
+    const bool live = cx < nx && cy < ny && cz < nz;
+    const size_t nxy = size_t(nx) * ny;
+    const size_t i = live ? cx + size_t(nx) * cy + nxy * cz : 0;
+    uint32_t pr = 0;
+    double fi = 0.0, dg = 1.0, az = 0.0, ay = 0.0, ax = 0.0, ez = 0.0, ey = 0.0, ex = 0.0;
+    if (live) {
+        pr = pres[i];
+        fi = f[i];
+        dg = diag[i];
+        if (pr & 1u) { az = vz[i]; if (lz == 0) ez = x[i - nxy]; }   // neighbours outside the brick are final
+        if (pr & 2u) { ay = vy[i]; if (ly == 0) ey = x[i - nx]; }
+        if (pr & 4u) { ax = vx[i]; if (lx == 0) ex = x[i - 1]; }
+    }
+    const uint32_t mine = lx + ly + lz;
+    for (uint32_t st = 0; st <= 3 * (kBrick - 1); st++) {
+        if (live && mine == st) {
+            double d = fi;
+            if (pr & 1u) d -= az * (lz ? sx[lz - 1][ly][lx] : ez);
+            if (pr & 2u) d -= ay * (ly ? sx[lz][ly - 1][lx] : ey);
+            if (pr & 4u) d -= ax * (lx ? sx[lz][ly][lx - 1] : ex);
+            const double xi = d / dg;
+            sx[lz][ly][lx] = xi;
+            x[i] = xi;
+        }
+        __syncthreads();
+    }
+}
+
 // GMRES: the i+1 classical Gram-Schmidt dots (v_{i+1}, v_k), k = 0..i (gmres.cc:80-82).
 // The k loops are unrolled to the 32-slot maximum under a k < nv test, so the
 // sums stay in registers and the nv loads of a row are in flight together.

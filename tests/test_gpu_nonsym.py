@@ -117,3 +117,38 @@ def test_gs_apply_row_widths(ctx, k):
     x = ctx.array(len(ia) - 1)
     B.apply(dev(ctx, f), x)
     assert np.array_equal(x.get(), O.gs_prec(ia, ja, a).apply(f))
+
+
+def _grid7(nx, ny, nz, seed, drop=0.2):
+    """Nonsymmetric seven-point M-matrix on an nx x ny x nz grid (x fastest) with a share `drop` of the
+    couplings left out, so rows miss some of their lower neighbours."""
+    g = rng(seed)
+    n = nx * ny * nz
+    ia, ja, a = [0], [], []
+    for i in range(n):
+        x, y, z = i % nx, (i // nx) % ny, i // (nx * ny)
+        nb = []
+        for d, ok in ((nx * ny, z > 0), (nx, y > 0), (1, x > 0)):
+            if ok and g.uniform() >= drop:
+                nb.append(i - d)
+        for d, ok in ((1, x + 1 < nx), (nx, y + 1 < ny), (nx * ny, z + 1 < nz)):
+            if ok and g.uniform() >= drop:
+                nb.append(i + d)
+        off = -g.uniform(0.1, 1.0, len(nb))
+        ja += [i] + nb
+        a += [float(-off.sum() + 0.5)] + off.tolist()
+        ia.append(len(ja))
+    return np.array(ia, np.uint32), np.array(ja, np.uint32), np.array(a)
+
+
+@pytest.mark.parametrize("dims", [(13, 9, 7), (8, 8, 8), (5, 30, 17), (60, 4, 3)])
+def test_gs_apply_grid_bricks(ctx, dims):
+    """Seven-point lower triangles take the brick sweep (8x8x8 bricks, partial at the grid's far faces, rows
+    with missing lower neighbours): bit for bit the oracle's sweep."""
+    ia, ja, a = _grid7(*dims, seed=sum(dims))
+    A = N.DevMatrix(ctx, ia, ja, a)
+    B = N.DevPrec(ctx, A, kind=N.GS)
+    f = rng(65).standard_normal(len(ia) - 1)
+    x = ctx.array(len(ia) - 1)
+    B.apply(dev(ctx, f), x)
+    assert np.array_equal(x.get(), O.gs_prec(ia, ja, a).apply(f))
