@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <functional>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -75,6 +76,12 @@ struct DevMem {
     }
 };
 
+// dst = src[0, n), copied by the host threads (the first touch of dst's pages is spread over them too)
+template <class T> void par_copy(HostVec<T>& dst, const T* src, size_t n) {
+    dst.resize(n);
+    parallel_for(n, 1 << 18, [&](uint64_t b, uint64_t e) { std::memcpy(dst.data() + b, src + b, (e - b) * sizeof(T)); });
+}
+
 // handle ids key the graph cache; unique across contexts, never reused
 inline uint64_t new_handle_id() {
     static std::atomic<uint64_t> next{1};
@@ -93,42 +100,6 @@ template <class VT> struct DevSell {
     }
 };
 
-// A matrix in its own row order is stored in the seven-point layout when it is
-// a symmetric seven-point grid operator (MLP_NO_STENCIL=1 keeps the slices),
-// else as sliced ELL.
-template <class VT>
-void upload_sell(DevMem& mem, hipStream_t s, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
-                 DevSell<VT>& out, const uint32_t* rows = nullptr) {
-    const char* off = getenv("MLP_NO_STENCIL");
-    if (!rows && !(off && off[0] == '1')) {
-        StencilLayout T;
-        if (plan_stencil(n, ia, ja, a, T)) {
-            out.v.n = n;
-            out.v.spres = mem.upload(T.pres, s);
-            out.v.sd = mem.upload_as<VT>(T.d, s);
-            out.v.slx = mem.upload_as<VT>(T.lx, s);
-            out.v.sly = mem.upload_as<VT>(T.ly, s);
-            out.v.slz = mem.upload_as<VT>(T.lz, s);
-            out.v.ssy = T.sy;
-            out.v.ssz = T.sz;
-            out.stencil = true;
-            out.nent = 0;
-            out.nnz = ia[n];
-            MLP_HIP_CHECK(hipStreamSynchronize(s));
-            return;
-        }
-    }
-    SellLayout L;
-    plan_sell(n, ia, ja, a, L, rows);
-    out.v.n = n;
-    out.v.off64 = mem.upload(L.off64, s);
-    if (!L.dcol.empty()) out.v.dcol = mem.upload(L.dcol, s);
-    else out.v.col = mem.upload(L.col, s);
-    out.v.val = mem.upload_as<VT>(L.val, s);
-    out.nent = L.nent;
-    out.nnz = n ? ia[n] : 0;
-    MLP_HIP_CHECK(hipStreamSynchronize(s));
-}
 
 }  // namespace
 }  // namespace mlp
@@ -146,6 +117,8 @@ struct mlp_ctx_s {
     double* scal = nullptr;            // 4 result slots for standalone BLAS-1
     mlp::SolveState* st = nullptr;
     double* host_pinned = nullptr;     // 64 doubles
+    char* stage = nullptr;             // pinned staging for uploads (Stager), grown on demand
+    size_t stage_bytes = 0;
     // batched right-hand sides (mlp_pcg_multi): R vectors of each kind, R states, R partial buffers
     mlp::DevMem multi_mem;
     uint32_t multi_R = 0, multi_n = 0;
@@ -196,8 +169,8 @@ struct mlp_mat_s {
     uint32_t n = 0, nnz = 0;
     mlp::DevMem mem;
     mlp::DevSell<double> A;
-    std::vector<uint32_t> ia, ja;      // host copy for diagnostics and DiagPrec
-    std::vector<double> a;
+    mlp::HostVec<uint32_t> ia, ja;     // host copy for DiagPrec, GS, Composite and SymPrec
+    mlp::HostVec<double> a;
     uint64_t bytes = 0;
 };
 
@@ -246,6 +219,95 @@ struct DevPrecBase {
 };
 
 constexpr uint32_t kLdsBudget = 60 * 1024;
+
+// Uploads a set of host arrays as one device block: the host threads pack
+// them (converting element types where asked) into the context's pinned
+// staging buffer, one copy moves the block, and flush() then points every
+// registered field at its array.  The host arrays must live until flush().
+struct Stager {
+    mlp_ctx ctx;
+    DevMem& mem;
+    struct Item {
+        size_t off;
+        std::function<void(char*)> write, set;
+    };
+    std::vector<Item> items;
+    size_t total = 0;
+    Stager(mlp_ctx c, DevMem& m) : ctx(c), mem(m) {}
+    template <class P, class S> void add(const S* src, size_t n, P& field) {
+        using T = std::remove_const_t<std::remove_pointer_t<P>>;
+        const size_t off = total;
+        total += (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~size_t(255);
+        items.push_back({off,
+                         [src, n](char* dst) {
+                             T* d = reinterpret_cast<T*>(dst);
+                             parallel_for(n, 1 << 16, [&](uint64_t b, uint64_t e) {
+                                 if constexpr (std::is_same_v<T, S>) std::memcpy(d + b, src + b, (e - b) * sizeof(T));
+                                 else for (uint64_t i = b; i < e; i++) d[i] = T(src[i]);
+                             });
+                         },
+                         [&field](char* base) { field = reinterpret_cast<P>(base); }});
+    }
+    template <class P, class S, class Al> void add(const std::vector<S, Al>& v, P& field) {
+        add(v.data(), v.size(), field);
+    }
+    void flush() {
+        if (items.empty()) return;
+        if (ctx->stage_bytes < total) {
+            if (ctx->stage) MLP_HIP_CHECK(hipHostFree(ctx->stage));
+            ctx->stage = nullptr;
+            ctx->stage_bytes = 0;
+            MLP_HIP_CHECK(hipHostMalloc((void**)&ctx->stage, total, hipHostMallocDefault));
+            ctx->stage_bytes = total;
+        }
+        for (Item& it : items) it.write(ctx->stage + it.off);
+        char* dev = mem.alloc<char>(total);
+        MLP_HIP_CHECK(hipMemcpyAsync(dev, ctx->stage, total, hipMemcpyHostToDevice, ctx->stream));
+        MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (Item& it : items) it.set(dev + it.off);
+        items.clear();
+        total = 0;
+    }
+};
+
+// A matrix in its own row order is stored in the seven-point layout when it is
+// a symmetric seven-point grid operator (MLP_NO_STENCIL=1 keeps the slices),
+// else as sliced ELL.
+template <class VT>
+void upload_sell(mlp_ctx ctx, DevMem& mem, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
+                 DevSell<VT>& out, const uint32_t* rows = nullptr) {
+    const char* off = getenv("MLP_NO_STENCIL");
+    if (!rows && !(off && off[0] == '1')) {
+        StencilLayout T;
+        if (plan_stencil(n, ia, ja, a, T)) {
+            out.v.n = n;
+            Stager up(ctx, mem);
+            up.add(T.pres, out.v.spres);
+            up.add(T.d, out.v.sd);
+            up.add(T.lx, out.v.slx);
+            up.add(T.ly, out.v.sly);
+            up.add(T.lz, out.v.slz);
+            up.flush();
+            out.v.ssy = T.sy;
+            out.v.ssz = T.sz;
+            out.stencil = true;
+            out.nent = 0;
+            out.nnz = ia[n];
+            return;
+        }
+    }
+    SellLayout L;
+    plan_sell(n, ia, ja, a, L, rows);
+    out.v.n = n;
+    Stager up(ctx, mem);
+    up.add(L.off64, out.v.off64);
+    if (!L.dcol.empty()) up.add(L.dcol, out.v.dcol);
+    else up.add(L.col, out.v.col);
+    up.add(L.val, out.v.val);
+    up.flush();
+    out.nent = L.nent;
+    out.nnz = n ? ia[n] : 0;
+}
 
 template <class VT> struct DevHier : DevPrecBase {
     mlp_ctx ctx;
@@ -380,8 +442,8 @@ template <class VT> struct DevHier : DevPrecBase {
             L.N = D.N; L.M = D.M; L.Md = D.Md; L.nit = D.nit; L.eps = D.eps;
             L.n = L.N - L.M - L.Md;
             if (L.eps != 0.0) dynamic = true;
-            std::vector<uint32_t> mapv(D.map, D.map + D.N);
-            L.map = data->upload(mapv, s);
+            Stager up(ctx, *data);
+            up.add(D.map, D.N, L.map);
             L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
             L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
             ProgView<VT>& G = L.prog;
@@ -390,24 +452,24 @@ template <class VT> struct DevHier : DevPrecBase {
             G.nflat = uint32_t(P.flat.size());
             G.max_nodes = P.max_nodes; G.max_Lent = P.max_Lent; G.max_Uent = P.max_Uent; G.max_Unodes = P.max_Unodes;
             G.max_levels = P.max_levels;
-            G.qrow = data->upload(P.qrow, s);
-            G.flatq = data->upload(P.flatq, s);
+            up.add(P.qrow, G.qrow);
+            up.add(P.flatq, G.flatq);
             if (P.nblk) {
-                G.blk = data->upload(P.blk, s);
-                G.lvlL = data->upload(P.lvlL, s);
-                G.lvlU = data->upload(P.lvlU, s);
-                G.out = data->upload(P.out, s);
-                G.Lptr = data->upload(P.Lptr, s);
-                G.Lloc = data->upload(P.Lloc, s);
-                G.Lval = data->upload_as<VT>(P.Lval, s);
-                G.Ldep = data->upload(P.Ldep, s);
-                G.Udep = data->upload(P.Udep, s);
-                G.uloc = data->upload(P.uloc, s);
-                G.uxidx = data->upload(P.uxidx, s);
-                G.Uptr = data->upload(P.Uptr, s);
-                G.Ucol = data->upload(P.Ucol, s);
-                G.Uval = data->upload_as<VT>(P.Uval, s);
-                G.Udiag = data->upload_as<VT>(P.Udiag, s);
+                up.add(P.blk, G.blk);
+                up.add(P.lvlL, G.lvlL);
+                up.add(P.lvlU, G.lvlU);
+                up.add(P.out, G.out);
+                up.add(P.Lptr, G.Lptr);
+                up.add(P.Lloc, G.Lloc);
+                up.add(P.Lval, G.Lval);
+                up.add(P.Ldep, G.Ldep);
+                up.add(P.Udep, G.Udep);
+                up.add(P.uloc, G.uloc);
+                up.add(P.uxidx, G.uxidx);
+                up.add(P.Uptr, G.Uptr);
+                up.add(P.Ucol, G.Ucol);
+                up.add(P.Uval, G.Uval);
+                up.add(P.Udiag, G.Udiag);
                 L.wt = scratch.alloc<VT>(P.nodes);
                 if (P.has_global) G.gx = scratch.alloc<VT>(P.nodes);
             }
@@ -421,14 +483,14 @@ template <class VT> struct DevHier : DevPrecBase {
                 L.lds_up = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, false, true).total;
                 L.lds_solve = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true).total;
             }
-            std::vector<double> dv(D.dval, D.dval + D.Md);
-            L.dval = data->upload_as<VT>(dv, s);
+            up.add(D.dval, D.Md, L.dval);
+            up.flush();
             if (l > 0) {
                 // uh_cheb reads A_l only through the fused Chebyshev residual, row by row in consumption order
                 if (kind == MLP_PREC_UH_CHEB)
-                    upload_sell<VT>(*data, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A, P.qrow.data());
+                    upload_sell<VT>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A, P.qrow.data());
                 else
-                    upload_sell<VT>(*data, s, D.N, D.A_ia, D.A_ja, D.A_a, L.A);
+                    upload_sell<VT>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A);
             }
             if (L.n) {
                 L.F = scratch.alloc<VT>(L.n);
@@ -758,8 +820,8 @@ struct DevGS : DevPrecBase {
     // The lower entries of every row are at offsets {nx ny, nx, 1} of a grid
     // numbered x fastest (each only where that neighbour exists): set up the
     // brick sweep and return true.
-    bool plan_bricks(const std::vector<uint32_t>& ia, const std::vector<uint32_t>& ja, const std::vector<double>& a,
-                     const std::vector<uint32_t>& lo_end) {
+    template <class VI, class VD>
+    bool plan_bricks(const VI& ia, const VI& ja, const VD& a, const std::vector<uint32_t>& lo_end) {
         uint32_t off[3] = {0, 0, 0};
         int no = 0;
         for (uint32_t i = 0; i < n; i++)
@@ -808,7 +870,7 @@ struct DevGS : DevPrecBase {
         return true;
     }
 
-    void build(const std::vector<uint32_t>& ia, const std::vector<uint32_t>& ja, const std::vector<double>& a) {
+    template <class VI, class VD> void build(const VI& ia, const VI& ja, const VD& a) {
         n = uint32_t(ia.size()) - 1;
         std::vector<uint32_t> lev(n, 0), lo_end(n);
         uint32_t nlev = 0;
@@ -957,6 +1019,9 @@ void context_release(mlp_ctx ctx) {
     ctx->host_pinned = nullptr;
     if (ctx->mst_host) (void)hipHostFree(ctx->mst_host);
     ctx->mst_host = nullptr;
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
+    ctx->stage = nullptr;
+    ctx->stage_bytes = 0;
     if (ctx->owns_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     ctx->stream = nullptr;
 }
@@ -1001,8 +1066,8 @@ struct DevSym : DevPrecBase {
     }
     void build(mlp_mat A, const double* sigmas, uint32_t nsig, const uint32_t* niters, uint32_t nnit, int use_tails) {
         n = A->n;
-        std::vector<double> a = A->a;
-        const std::vector<uint32_t>&ia = A->ia, &ja = A->ja;
+        std::vector<double> a(A->a.begin(), A->a.end());
+        const HostVec<uint32_t>&ia = A->ia, &ja = A->ja;
         auto at = [&](uint32_t r, uint32_t c) -> double {      // Asym(r, c) with the updates made so far
             if (ja[ia[r]] == c) return a[ia[r]];
             auto b = ja.begin() + ia[r] + 1, e = ja.begin() + ia[r + 1];
@@ -1178,10 +1243,10 @@ int mlp_csr_upload(mlp_ctx ctx, uint32_t n, const uint32_t* ia, const uint32_t* 
         m->id = mlp::new_handle_id();
         m->n = n;
         m->nnz = ia[n];
-        m->ia.assign(ia, ia + n + 1);
-        m->ja.assign(ja, ja + m->nnz);
-        m->a.assign(a, a + m->nnz);
-        upload_sell<double>(m->mem, ctx->stream, n, ia, ja, a, m->A);
+        par_copy(m->ia, ia, size_t(n) + 1);
+        par_copy(m->ja, ja, m->nnz);
+        par_copy(m->a, a, m->nnz);
+        upload_sell<double>(ctx, m->mem, n, ia, ja, a, m->A);
         m->bytes = m->A.stored_bytes();
     } catch (...) {
         delete m;
@@ -1563,7 +1628,9 @@ int mlp_prec_composite(mlp_ctx ctx, mlp_mat A, mlp_hier ms, mlp_prec* out) {
     MLP_API_BEGIN
     require(ctx && A && ms && out, "null argument");
     require(ms->H.kind == PrecKind::MULTI_SPLIT, "the second stage must be a multi_split hierarchy");
-    require(ms->A0.nrow == A->n && ms->A0.ia == A->ia && ms->A0.ja == A->ja && ms->A0.a == A->a,
+    require(ms->A0.nrow == A->n && std::equal(A->ia.begin(), A->ia.end(), ms->A0.ia.begin(), ms->A0.ia.end()) &&
+                std::equal(A->ja.begin(), A->ja.end(), ms->A0.ja.begin(), ms->A0.ja.end()) &&
+                std::equal(A->a.begin(), A->a.end(), ms->A0.a.begin(), ms->A0.a.end()),
             "the multi_split hierarchy must be built on A");
     auto g = std::make_unique<DevGS>();
     g->ctx = ctx;
