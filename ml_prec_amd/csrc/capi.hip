@@ -222,8 +222,10 @@ constexpr uint32_t kLdsBudget = 60 * 1024;
 
 // Uploads a set of host arrays as one device block: the host threads pack
 // them (converting element types where asked) into the context's pinned
-// staging buffer, one copy moves the block, and flush() then points every
-// registered field at its array.  The host arrays must live until flush().
+// staging buffer (kept for the next upload; a block over kMaxPinned goes
+// through a pageable buffer instead), one copy moves the block, and flush()
+// then points every registered field at its array.  The host arrays must live
+// until flush().
 struct Stager {
     mlp_ctx ctx;
     DevMem& mem;
@@ -251,18 +253,26 @@ struct Stager {
     template <class P, class S, class Al> void add(const std::vector<S, Al>& v, P& field) {
         add(v.data(), v.size(), field);
     }
+    static constexpr size_t kMaxPinned = size_t(256) << 20;   // larger blocks go through pageable memory
     void flush() {
         if (items.empty()) return;
-        if (ctx->stage_bytes < total) {
-            if (ctx->stage) MLP_HIP_CHECK(hipHostFree(ctx->stage));
-            ctx->stage = nullptr;
-            ctx->stage_bytes = 0;
-            MLP_HIP_CHECK(hipHostMalloc((void**)&ctx->stage, total, hipHostMallocDefault));
-            ctx->stage_bytes = total;
-        }
-        for (Item& it : items) it.write(ctx->stage + it.off);
         char* dev = mem.alloc<char>(total);
-        MLP_HIP_CHECK(hipMemcpyAsync(dev, ctx->stage, total, hipMemcpyHostToDevice, ctx->stream));
+        if (total <= kMaxPinned) {
+            if (ctx->stage_bytes < total) {
+                if (ctx->stage) MLP_HIP_CHECK(hipHostFree(ctx->stage));
+                ctx->stage = nullptr;
+                ctx->stage_bytes = 0;
+                MLP_HIP_CHECK(hipHostMalloc((void**)&ctx->stage, total, hipHostMallocDefault));
+                ctx->stage_bytes = total;
+            }
+            for (Item& it : items) it.write(ctx->stage + it.off);
+            MLP_HIP_CHECK(hipMemcpyAsync(dev, ctx->stage, total, hipMemcpyHostToDevice, ctx->stream));
+        } else {
+            HostVec<char> buf(total);
+            for (Item& it : items) it.write(buf.data() + it.off);
+            MLP_HIP_CHECK(hipMemcpyAsync(dev, buf.data(), total, hipMemcpyHostToDevice, ctx->stream));
+            MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        }
         MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         for (Item& it : items) it.set(dev + it.off);
         items.clear();
