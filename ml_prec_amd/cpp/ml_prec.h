@@ -196,7 +196,7 @@ public:
     uint nlevels() const { return nlevels_; }
     mlp_prec device() const override { return prec_; }
 private:
-    const SkylineMatrix& level0_A;
+    [[maybe_unused]] const SkylineMatrix& level0_A;   // kept by reference, as cheb_prec.h:45
     mlp_hier hier_ = nullptr;
     mlp_prec prec_ = nullptr;
     double lmin_ = 0, lmax_ = 0;
@@ -210,7 +210,7 @@ public:
     uint nlevels() const { return nlevels_; }
     mlp_prec device() const override { return prec_; }
 private:
-    const SkylineMatrix& level0_A;
+    [[maybe_unused]] const SkylineMatrix& level0_A;   // kept by reference, as multi_split_prec.h:70
     mlp_hier hier_ = nullptr;
     mlp_prec prec_ = nullptr;
     uint nlevels_ = 0;
