@@ -453,6 +453,7 @@ template <class VT> struct DevHier : DevPrecBase {
             L.n = L.N - L.M - L.Md;
             if (L.eps != 0.0) dynamic = true;
             Stager up(ctx, *data);
+            std::vector<double> urcp;
             up.add(D.map, D.N, L.map);
             L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
             L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
@@ -479,7 +480,13 @@ template <class VT> struct DevHier : DevPrecBase {
                 up.add(P.Uptr, G.Uptr);
                 up.add(P.Ucol, G.Ucol);
                 up.add(P.Uval, G.Uval);
-                up.add(P.Udiag, G.Udiag);
+                if constexpr (std::is_same_v<VT, float>) {         // udiv: FP32 multiplies by 1 / u_ii
+                    urcp.resize(P.Udiag.size());
+                    for (size_t i = 0; i < urcp.size(); i++) urcp[i] = 1.0 / P.Udiag[i];
+                    up.add(urcp, G.Udiag);
+                } else {
+                    up.add(P.Udiag, G.Udiag);
+                }
                 L.wt = scratch.alloc<VT>(P.nodes);
                 if (P.has_global) G.gx = scratch.alloc<VT>(P.nodes);
             }

@@ -76,6 +76,21 @@ __device__ __forceinline__ bool dev_is_one(double a) {   // include/tools.h:28-3
     return fabs(a - 1.0) <= 2.220446049250313e-16;
 }
 
+// acc - v x in a row update.  FP64 rounds the product and the difference
+// separately (the reference's arithmetic, no contraction); FP32 fuses them,
+// one instruction per entry on the latency-bound chains.
+template <class VT> __device__ __forceinline__ VT msub(VT acc, VT v, VT x);
+template <> __device__ __forceinline__ double msub<double>(double acc, double v, double x) { return acc - v * x; }
+template <> __device__ __forceinline__ float msub<float>(float acc, float v, float x) { return fmaf(-v, x, acc); }
+
+// x / u_ii at the end of a U row.  FP64 divides (IEEE, the reference's
+// arithmetic); an FP32 hierarchy stores 1 / u_ii (rounded from the FP64
+// reciprocal) and multiplies, one instruction instead of a division sequence
+// on every step of the latency-bound U chains.
+template <class VT> __device__ __forceinline__ VT udiv(VT z, VT d);
+template <> __device__ __forceinline__ double udiv<double>(double z, double d) { return z / d; }
+template <> __device__ __forceinline__ float udiv<float>(float z, float rcp) { return z * rcp; }
+
 template <int BS>
 __device__ __forceinline__ double block_sum(double v, double* sh) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -404,7 +419,7 @@ __device__ __forceinline__ void block_down(const ProgView<VT>& P, const uint32_t
         const uint32_t e = slv[li];
         for (uint32_t p = s + threadIdx.x; p < e; p += blockDim.x) {
             VT acc = sw[p];
-            for (uint32_t q = sptr[p]; q < sptr[p + 1]; q++) acc -= sval[q] * sw[sloc[q]];
+            for (uint32_t q = sptr[p]; q < sptr[p + 1]; q++) acc = msub(acc, sval[q], sw[sloc[q]]);
             sw[p] = acc;
         }
         s = e;
@@ -453,9 +468,9 @@ __device__ __forceinline__ void block_up(const ProgView<VT>& P, const uint32_t* 
             for (uint32_t q = sptr[u]; q < sptr[u + 1]; q++) {
                 const uint32_t c = scol[q];
                 const VT xv = (c & kGlobalBit) ? sref[q] : sx[c];
-                z -= sval[q] * xv;
+                z = msub(z, sval[q], xv);
             }
-            const VT v = z / sdiag[u];
+            const VT v = udiv(z, sdiag[u]);
             sx[sloc[u]] = v;
             sz[u] = v;
         }
@@ -502,10 +517,10 @@ __device__ __forceinline__ VT owned_down(const ProgView<VT>& P, const uint32_t* 
     for (uint32_t li = 0; li < nl; li++) {
         if (li == my) {
             const VT x0 = sw[l0], x1 = sw[l1];
-            acc = h0 ? acc - v0 * x0 : acc;
-            acc = h1 ? acc - v1 * x1 : acc;
+            acc = h0 ? msub(acc, v0, x0) : acc;
+            acc = h1 ? msub(acc, v1, x1) : acc;
             if (more)
-                for (uint32_t q = q0 + 2; q < q1; q++) acc -= P.Lval[q] * sw[P.Lloc[q]];
+                for (uint32_t q = q0 + 2; q < q1; q++) acc = msub(acc, P.Lval[q], sw[P.Lloc[q]]);
             sw[p] = acc;
         }
         level_sync<BS>();
@@ -550,15 +565,15 @@ __device__ __forceinline__ VT owned_up(const ProgView<VT>& P, const uint32_t* B,
         if (li == my) {
             const VT s0 = sx[a0], s1 = sx[a1];
             const VT x0 = g0 ? r0 : s0, x1 = g1 ? r1 : s1;
-            z = h0 ? z - v0 * x0 : z;
-            z = h1 ? z - v1 * x1 : z;
+            z = h0 ? msub(z, v0, x0) : z;
+            z = h1 ? msub(z, v1, x1) : z;
             if (more)
                 for (uint32_t q = q0 + 2; q < q1; q++) {
                     const uint32_t c = P.Ucol[q];
                     const VT xv = (c & kGlobalBit) ? combine<VT, OP>(C, c & ~kGlobalBit) : sx[c];
-                    z -= P.Uval[q] * xv;
+                    z = msub(z, P.Uval[q], xv);
                 }
-            z = z / d;
+            z = udiv(z, d);
             sx[loc] = z;
         }
         level_sync<BS>();
@@ -581,7 +596,7 @@ __device__ __forceinline__ void global_down(const ProgView<VT>& P, const uint32_
         const uint32_t e = P.lvlL[B[2] + li];
         for (uint32_t p = s + threadIdx.x; p < e; p += blockDim.x) {
             VT acc = gw[base + p];
-            for (uint32_t q = P.Lptr[base + p]; q < P.Lptr[base + p + 1]; q++) acc -= P.Lval[q] * gw[base + P.Lloc[q]];
+            for (uint32_t q = P.Lptr[base + p]; q < P.Lptr[base + p + 1]; q++) acc = msub(acc, P.Lval[q], gw[base + P.Lloc[q]]);
             gw[base + p] = acc;
         }
         s = e;
@@ -606,9 +621,9 @@ __device__ __forceinline__ void global_up(const ProgView<VT>& P, const uint32_t*
             for (uint32_t q = P.Uptr[ub + u]; q < P.Uptr[ub + u + 1]; q++) {
                 const uint32_t c = P.Ucol[q];
                 const VT xv = (c & kGlobalBit) ? combine<VT, OP>(C, c & ~kGlobalBit) : gx[base + c];
-                z -= P.Uval[q] * xv;
+                z = msub(z, P.Uval[q], xv);
             }
-            gx[base + loc] = z / P.Udiag[ub + u];
+            gx[base + loc] = udiv(z, P.Udiag[ub + u]);
         }
         s = e;
         __syncthreads();
