@@ -63,7 +63,7 @@ def cpu_baseline(s, eps, budget_s=20.0):
 
 
 def ensemble(N, S, args):
-    """Config 3: COUNT SPE10-shaped systems of 60x220xnz cells, nz drawn from 5/10/20/40/85, each with its own
+    """Ensemble: COUNT SPE10-shaped systems of 60x220xnz cells, nz drawn from 5/10/20/40/85, each with its own
     field, uh_cheb hierarchy and context (stream).  A step solves all of them to 1e-8 from x0 = 0, concurrently
     through mlp_pcg_batch; the same systems solved one after another are timed beside it."""
     from concurrent.futures import ThreadPoolExecutor
@@ -105,7 +105,7 @@ def ensemble(N, S, args):
     ms_seq = 1e3 * (time.perf_counter() - t0)
     log(f"one after another: {ms_seq:.2f} ms ({len(jobs) / ms_seq * 1e3:.0f} solves/s)")
     out = {
-        "metric": "ensemble PCG solves per second (config 3, 1e-8 rel-res, FP64)",
+        "metric": "ensemble PCG solves per second (SPE10-shaped 60x220xnz systems, 1e-8 rel-res, FP64)",
         "value": round(len(jobs) / ms_batch * 1e3, 2),
         "unit": "solves/s",
         "n_gpus": 1,
@@ -136,10 +136,10 @@ def ensemble(N, S, args):
 
 
 def ensemble_rhs(N, S, args):
-    """Config 3 variant: one 60x220x85 system with COUNT seeded right-hand sides.  Batched: one mlp_pcg_multi call
+    """Many right-hand sides: one 60x220x85 system with COUNT seeded right-hand sides.  Batched: one mlp_pcg_multi call
     whose every launch covers all right-hand sides (the reported value).  For comparison: the same solves as
     concurrent jobs on clones of the preconditioner (mlp_pcg_batch) and one after another (mlp_pcg)."""
-    s = S.config_system(1, kind="spe", c=1.0, nz=args.nz)
+    s = S.spe10_system(kind="spe", c=1.0, nz=args.nz)
     H = N.build_uh_cheb(s.ia, s.ja, s.a)
     R = args.ensemble_rhs
     ctx0 = N.Context(0)
@@ -211,7 +211,7 @@ def ensemble_rhs(N, S, args):
         f"{ms_batch:.2f} ms ({R / ms_batch * 1e3:.0f} solves/s), one after another {ms_seq:.2f} ms, "
         f"iterations {min(iters)}..{max(iters)}, batched == single bitwise: {same}")
     print(json.dumps({
-        "metric": "PCG solves per second, one 1.12M-cell system with many right-hand sides (config 3 variant)",
+        "metric": "PCG solves per second, one 1.12M-cell system with many right-hand sides",
         "value": round(R / ms_multi * 1e3, 2), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_multi, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -234,7 +234,7 @@ def ensemble_rhs(N, S, args):
 
 
 def adjoint(N, S, args):
-    """Config 4: L = sum over 200 observed cells of (p_i - d_i)^2 with d = p(true field) + N(0, 0.01|p_i|);
+    """Adjoint gradient: L = sum over 200 observed cells of (p_i - d_i)^2 with d = p(true field) + N(0, 0.01|p_i|);
     theta = ln kx of a second field; one gradient = forward solve + adjoint solve (1e-10) + gradient kernel.
     Checked against central differences (h = 1e-6) along --fd-dirs seeded directions."""
     from ml_prec_amd.adjoint import SpeAdjointModel, misfit
@@ -272,7 +272,7 @@ def adjoint(N, S, args):
     log(f"central differences along {args.fd_dirs} directions: max rel. difference {max(errs):.2e}, "
         f"median {np.median(errs):.2e}")
     out = {
-        "metric": "adjoint misfit gradient time (config 4, forward + adjoint PCG to 1e-10)",
+        "metric": "adjoint misfit gradient time at 1.12M cells (forward + adjoint PCG to 1e-10)",
         "value": round(1e3 * float(np.median(times)), 3),
         "unit": "ms",
         "n_gpus": 1,
@@ -363,7 +363,7 @@ def solvers(N, S, args):
     with GS / Composite / Sym, on the reference's nonsymmetric variant of it, spe.cc:138-182 with tau = 0.3):
     device solve time to 1e-8 from x0 = 0, iterations, true residual, preconditioner apply time, and the
     single-core C oracle on the same case.  Prints one JSON line with a record per case."""
-    s = S.config_system(1, kind="spe", c=1.0, nz=args.nz)
+    s = S.spe10_system(kind="spe", c=1.0, nz=args.nz)
     a_ns = S.make_nonsymmetric(s.ia, s.ja, s.a, 0.3)
     cases = [c for c in SOLVER_CASES if not args.case or c[0] in args.case.split(",")]
     if not cases:
@@ -487,12 +487,12 @@ def main():
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--nz", type=int, default=85)
     ap.add_argument("--ensemble", type=int, default=0, metavar="COUNT",
-                    help="config 3 instead: COUNT independent systems (nz drawn from 5/10/20/40/85) solved "
+                    help="an ensemble instead: COUNT independent systems (nz drawn from 5/10/20/40/85) solved "
                          "concurrently; prints its own JSON line")
     ap.add_argument("--ensemble-rhs", type=int, default=0, metavar="COUNT",
-                    help="config 3 variant: the 60x220x85 system with COUNT right-hand sides, one shared hierarchy")
+                    help="the 60x220x85 system with COUNT right-hand sides instead, one shared hierarchy")
     ap.add_argument("--adjoint", action="store_true",
-                    help="config 4 instead: misfit gradient by the adjoint solve, checked by central differences")
+                    help="the misfit gradient by the adjoint solve instead, checked by central differences")
     ap.add_argument("--fd-dirs", type=int, default=20)
     ap.add_argument("--solvers", action="store_true",
                     help="every outer solver / preconditioner pair at 1.12M cells instead; prints its own JSON line")
@@ -504,7 +504,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (each system is solved on one "
-              "device; the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+              "device; this build has no multi-GPU path)", file=sys.stderr)
         sys.exit(2)
 
     from ml_prec_amd import native as N
@@ -524,7 +524,7 @@ def main():
         return
 
     t0 = time.perf_counter()
-    s = S.config_system(1, kind="spe", c=1.0, nz=args.nz)
+    s = S.spe10_system(kind="spe", c=1.0, nz=args.nz)
     log(f"system {s.nx}x{s.ny}x{s.nz}: n={s.n} nnz={s.nnz} generated in {time.perf_counter() - t0:.2f}s")
     if args.spmv_traffic:
         spmv_traffic_probe(N, s, args.spmv_reps)

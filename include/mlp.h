@@ -231,7 +231,7 @@ typedef struct {
 int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* opts);
 
 /* Many right-hand sides of one system (no counterpart in the reference, which
- * solves one at a time; config 3's "1 matrix x 32 right-hand sides"): nrhs
+ * solves one at a time; for e.g. one matrix with 32 right-hand sides): nrhs
  * (1..64) PCG solves with matrix A and preconditioner B (uh_cheb, on ctx),
  * d_b and d_x holding nrhs device vectors of length n one after another.
  * Every kernel launch of an iteration covers all right-hand sides; each solve
@@ -243,7 +243,7 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* opts)
 int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const double* d_b, double* d_x,
                   const mlp_solve_opts* opts, mlp_solve_stats* stats);
 /* Adjoint gradient for the two-point-flux assembly of modules/mesh/spe.cc:69-131 (no counterpart in the
- * reference: config 4 differentiates a misfit through the solve).  Parameters theta_i = ln kx_i per cell
+ * reference; for differentiating a misfit through the solve).  Parameters theta_i = ln kx_i per cell
  * (ky = kx, kz = kz_ratio kx); fx, fy, fz are the per-direction face factors (1/h^2 in spe.cc's scaling);
  * cells are numbered x fastest.  Writes grad_i = -lam^T (dA/dtheta_i) p, which is dL/dtheta_i when
  * A p = b and A lam = dL/dp (A symmetric).  All vectors are device pointers of nx*ny*nz doubles. */

@@ -1,4 +1,4 @@
-"""Adjoint gradients through the device solve (config 4).
+"""Adjoint gradients through the device solve.
 
 The model: A(theta) p = b with A the reference's two-point-flux matrix
 (modules/mesh/spe.cc:69-131, reaction coefficient c, ky = kx, kz = 0.1 kx) and

@@ -2401,7 +2401,7 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
     MLP_API_END
 }
 
-// Many independent PCG solves at once (config 3: ensembles of systems).  Each
+// Many independent PCG solves at once (ensembles of systems).  Each
 // job has its own context, so its own stream, workspace and device state; the
 // solves start without host synchronisation (FIN_START computes the initial
 // norm and the eps floor on the device) and their iteration chunks are

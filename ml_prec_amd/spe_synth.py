@@ -246,22 +246,21 @@ def make_system(nx, ny, nz, seed=BASE_SEED, field="lognormal", kind="tpfa", c=0.
     return System(nx, ny, nz, ia, ja, a, b, k.ravel(), kind, c)
 
 
-def config_system(cfg: int, seed: int | None = None, kind="tpfa", c=0.0, nz: int | None = None):
-    """Workload configs: 0 = 60x220x5 log-normal (sigma 0.5); 1 = 60x220x85 SPE10-shaped."""
-    if cfg == 0:
-        s = BASE_SEED if seed is None else seed
-        return make_system(60, 220, 5 if nz is None else nz, s, "lognormal", kind, c, 0.5)
-    if cfg in (1, 2, 4):
-        s = BASE_SEED + 1 if seed is None else seed
-        return make_system(60, 220, 85 if nz is None else nz, s, "spe10", kind, c)
-    raise ValueError(cfg)
+def lognormal_system(seed: int | None = None, kind="tpfa", c=0.0, nz: int = 5):
+    """The small log-normal workload: 60x220x5 cells (nz overridable), ln k with sigma 0.5, seed BASE_SEED."""
+    return make_system(60, 220, nz, BASE_SEED if seed is None else seed, "lognormal", kind, c, 0.5)
+
+
+def spe10_system(seed: int | None = None, kind="tpfa", c=0.0, nz: int | None = None):
+    """The SPE10-shaped workload: 60x220x85 cells (nz overridable), seed BASE_SEED + 1."""
+    return make_system(60, 220, 85 if nz is None else nz, BASE_SEED + 1 if seed is None else seed, "spe10", kind, c)
 
 
 ENSEMBLE_NZ = (5, 10, 20, 40, 85)
 
 
 def ensemble_spec(count=64, base_seed=BASE_SEED + 3):
-    """Config 3: `count` systems, system i with its own field seed base_seed + 1000 + i and a layer count
+    """The ensemble: `count` systems, system i with its own field seed base_seed + 1000 + i and a layer count
     drawn uniformly from ENSEMBLE_NZ by a generator seeded with base_seed.  Returns [(nz, seed)]."""
     g = np.random.default_rng(base_seed)
     nzs = g.choice(np.array(ENSEMBLE_NZ), size=count)

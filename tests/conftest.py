@@ -28,18 +28,18 @@ def system(name):
     """Small seeded systems shared by the tests (cached per session)."""
     from ml_prec_amd import spe_synth as S
     if name not in _SYSTEMS:
-        if name == "cfg0_spe":
-            s = S.config_system(0, kind="spe", c=1.0)
-        elif name == "cfg0_tpfa_c1":
-            s = S.config_system(0, kind="tpfa", c=1.0)
-        elif name == "cfg0_tpfa":
-            s = S.config_system(0, kind="tpfa", c=0.0)
+        if name == "ln5_spe":
+            s = S.lognormal_system(kind="spe", c=1.0)
+        elif name == "ln5_tpfa_c1":
+            s = S.lognormal_system(kind="tpfa", c=1.0)
+        elif name == "ln5_tpfa":
+            s = S.lognormal_system(kind="tpfa", c=0.0)
         elif name == "spe10_z12":
             s = S.make_system(60, 220, 12, S.BASE_SEED + 11, "spe10", "spe", 1.0)
         elif name == "spe10_z40":
             s = S.make_system(60, 220, 40, S.BASE_SEED + 12, "spe10", "spe", 1.0)
-        elif name == "cfg1_spe":
-            s = S.config_system(1, kind="spe", c=1.0)
+        elif name == "spe10_z85":
+            s = S.spe10_system(kind="spe", c=1.0)
         elif name == "tiny":
             s = S.make_system(6, 10, 3, 7, "spe10", "spe", 1.0)
         elif name == "tiny_tpfa_c1":

@@ -1,4 +1,4 @@
-"""Adjoint gradient of the two-point-flux solve (config 4).
+"""Adjoint gradient of the two-point-flux solve.
 
 CPU: the gradient formula g_i = -lam^T dA/dtheta_i p (theta = ln kx) is
 checked against central differences of the assembled matrix itself

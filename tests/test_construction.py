@@ -36,7 +36,7 @@ def same_hierarchy(P, H):
     return True
 
 
-@pytest.mark.parametrize("name", ["tiny", "small_spe10", "tiny_tpfa_c1", "cfg0_tpfa_c1", "spe10_z12", "spe10_z40"])
+@pytest.mark.parametrize("name", ["tiny", "small_spe10", "tiny_tpfa_c1", "ln5_tpfa_c1", "spe10_z12", "spe10_z40"])
 @pytest.mark.parametrize("sigmas,niters,tails", [((3.0,), (2,), True), ((2.0, 3.0), (1, 2), True),
                                                  ((3.0,), (3,), False)])
 def test_uh_cheb_bitexact(name, sigmas, niters, tails):
@@ -46,7 +46,7 @@ def test_uh_cheb_bitexact(name, sigmas, niters, tails):
     assert same_hierarchy(P, H)
 
 
-@pytest.mark.parametrize("name", ["tiny_tpfa_c1", "cfg0_tpfa_c1", "spe10_z12", "spe10_z40"])
+@pytest.mark.parametrize("name", ["tiny_tpfa_c1", "ln5_tpfa_c1", "spe10_z12", "spe10_z40"])
 @pytest.mark.parametrize("order,dm", [("none", 3), ("original", 3), ("simple_1", 2), ("simple_1", 1)])
 @pytest.mark.parametrize("q", [(0.8,), (0.6, 0.8)])
 def test_multi_split_bitexact(name, order, dm, q):
@@ -57,14 +57,14 @@ def test_multi_split_bitexact(name, order, dm, q):
 
 
 def test_full_size_uh_cheb_bitexact():
-    s = system("cfg1_spe")
+    s = system("spe10_z85")
     assert same_hierarchy(O.cheb_prec(s.ia, s.ja, s.a), N.build_uh_cheb(s.ia, s.ja, s.a))
 
 
 def test_too_many_levels_on_pure_dirichlet_matrix():
     """Interior row sums of a pure Dirichlet pressure matrix are 0 up to rounding, so no
     link can be dropped and the hierarchy never shrinks (cheb_constr.cc:153-154)."""
-    s = system("cfg0_tpfa")
+    s = system("ln5_tpfa")
     with pytest.raises(O.OracleError, match="Too many levels: 30"):
         O.cheb_prec(s.ia, s.ja, s.a)
     with pytest.raises(N.MlpError, match="Too many levels: 30"):

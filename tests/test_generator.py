@@ -6,12 +6,12 @@ from ml_prec_amd import spe_synth as S
 
 
 def test_config_sizes_and_symmetry():
-    s0 = S.config_system(0)
+    s0 = S.lognormal_system()
     assert (s0.n, s0.nnz) == (66_000, 432_800)
-    s1 = S.config_system(1)
+    s1 = S.spe10_system()
     assert (s1.n, s1.nnz) == (1_122_000, 7_780_000)
     assert s1.kx.max() / s1.kx.min() >= 1e6                       # channel contrast
-    for s in (s0, S.config_system(0, kind="spe", c=1.0)):
+    for s in (s0, S.lognormal_system(kind="spe", c=1.0)):
         A = sp.csr_matrix((s.a, s.ja, s.ia), shape=(s.n, s.n))
         assert (A != A.T).nnz == 0                               # bitwise symmetric
         assert np.all(s.ja[s.ia[:-1]] == np.arange(s.n))         # Skyline: diagonal first
@@ -21,14 +21,14 @@ def test_config_sizes_and_symmetry():
 
 
 def test_reference_formula_row_sums_equal_c():
-    s = S.config_system(0, kind="spe", c=1.0)
+    s = S.lognormal_system(kind="spe", c=1.0)
     rs = np.add.reduceat(s.a, s.ia[:-1].astype(np.int64))
     assert np.allclose(rs, 1.0, atol=1e-12)
 
 
 def test_determinism():
-    assert S.config_system(0).digest() == S.config_system(0).digest()
-    assert S.config_system(0).digest() != S.config_system(0, seed=5).digest()
+    assert S.lognormal_system().digest() == S.lognormal_system().digest()
+    assert S.lognormal_system().digest() != S.lognormal_system(seed=5).digest()
 
 
 def test_csr_binary_roundtrip(tmp_path):

@@ -27,14 +27,14 @@ def assert_same_hierarchy(Hh, Hd):
             assert np.array_equal(ah[k], ad[k]), (l, k)
 
 
-@pytest.mark.parametrize("name", ["tiny", "small_spe10", "cfg0_spe", "cfg0_tpfa_c1", "spe10_z12", "cfg1_spe"])
+@pytest.mark.parametrize("name", ["tiny", "small_spe10", "ln5_spe", "ln5_tpfa_c1", "spe10_z12", "spe10_z85"])
 def test_device_links_same_hierarchy(ctx, name):
     s = system(name)
     assert_same_hierarchy(N.build_uh_cheb(s.ia, s.ja, s.a), N.build_uh_cheb(s.ia, s.ja, s.a, ctx=ctx))
 
 
 def test_device_links_parameters(ctx):
-    s = system("cfg0_spe")
+    s = system("ln5_spe")
     kw = dict(sigmas=(2.0, 4.0), niters=(3, 2), use_tails=False)
     assert_same_hierarchy(N.build_uh_cheb(s.ia, s.ja, s.a, **kw), N.build_uh_cheb(s.ia, s.ja, s.a, ctx=ctx, **kw))
 

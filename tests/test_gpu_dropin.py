@@ -104,7 +104,7 @@ def test_dropin_cheb_and_simple_solvers(tmp_path):
 
 @pytest.mark.parametrize("prec", ["comp", "gs", "sym_split"])
 def test_dropin_gmres_nonsymmetric(tmp_path, prec):
-    s0 = system("cfg0_spe")
+    s0 = system("ln5_spe")
     a = S.make_nonsymmetric(s0.ia, s0.ja, s0.a, 0.3)
     s = S.System(**{**s0.__dict__, "a": a})
     conf = tmp_path / "msplit.cfg"

@@ -68,7 +68,7 @@ def test_multi_split_block_paths(ctx, monkeypatch, shape):
 
 def test_giant_component_apply(ctx):
     """One ~25k-row elimination component: the coarsest-level solve from global memory."""
-    s = system("cfg0_tpfa_c1")
+    s = system("ln5_tpfa_c1")
     P = O.msplit_prec(s.ia, s.ja, s.a, q=(0.8,), niters=(3,), order="simple_1", degree_max=3)
     H = N.build_multi_split(s.ia, s.ja, s.a, q=(0.8,), niters=(3,), order="simple_1", degree_max=3)
     _apply_check(ctx, s, P, H, seeds=(9,))

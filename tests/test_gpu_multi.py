@@ -30,7 +30,7 @@ def _single(ctx, A, B, b, **kw):
     return st, x.get()
 
 
-@pytest.mark.parametrize("name,R,graph", [("spe10_z12", 5, True), ("spe10_z12", 3, False), ("cfg0_tpfa_c1", 8, True)])
+@pytest.mark.parametrize("name,R,graph", [("spe10_z12", 5, True), ("spe10_z12", 3, False), ("ln5_tpfa_c1", 8, True)])
 def test_multi_equals_single(ctx, name, R, graph):
     s = system(name)
     A = N.DevMatrix(ctx, s.ia, s.ja, s.a)

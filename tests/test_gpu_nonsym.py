@@ -32,7 +32,7 @@ def dev(ctx, x):
 MS = dict(q=(0.8,), niters=(5,), order="original", degree_max=3)
 
 
-@pytest.mark.parametrize("name", ["cfg0_spe", "spe10_z12"])
+@pytest.mark.parametrize("name", ["ln5_spe", "spe10_z12"])
 def test_gs_apply_bitexact(ctx, name):
     s, a = nonsym(name)
     A = N.DevMatrix(ctx, s.ia, s.ja, a)
@@ -44,7 +44,7 @@ def test_gs_apply_bitexact(ctx, name):
 
 
 def test_composite_apply_bitexact(ctx):
-    s, a = nonsym("cfg0_spe")
+    s, a = nonsym("ln5_spe")
     A = N.DevMatrix(ctx, s.ia, s.ja, a)
     H = N.build_multi_split(s.ia, s.ja, a, **MS)
     B = N.DevPrec(ctx, A, H, kind=N.COMPOSITE)
@@ -56,7 +56,7 @@ def test_composite_apply_bitexact(ctx):
 
 
 def test_sym_prec_apply(ctx):
-    s, a = nonsym("cfg0_spe")
+    s, a = nonsym("ln5_spe")
     A = N.DevMatrix(ctx, s.ia, s.ja, a)
     B = N.DevPrec(ctx, A, kind=N.SYM)
     P = O.sym_prec(s.ia, s.ja, a)
@@ -69,7 +69,7 @@ def test_sym_prec_apply(ctx):
 
 @pytest.mark.parametrize("prec", ["composite", "gs", "sym"])
 def test_gmres_nonsym_vs_oracle(ctx, prec):
-    s, a = nonsym("cfg0_spe")
+    s, a = nonsym("ln5_spe")
     A = N.DevMatrix(ctx, s.ia, s.ja, a)
     if prec == "composite":
         B = N.DevPrec(ctx, A, N.build_multi_split(s.ia, s.ja, a, **MS), kind=N.COMPOSITE)

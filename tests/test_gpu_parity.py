@@ -25,7 +25,7 @@ def dev(ctx, x):
     return ctx.to_device(np.ascontiguousarray(x, dtype=np.float64))
 
 
-@pytest.mark.parametrize("name", ["cfg0_spe", "cfg0_tpfa", "spe10_z12"])
+@pytest.mark.parametrize("name", ["ln5_spe", "ln5_tpfa", "spe10_z12"])
 def test_spmv_residual_bitexact(ctx, name):
     s = system(name)
     A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
@@ -59,7 +59,7 @@ def test_blas1(ctx):
         assert np.array_equal(dx2.get(), O.dscal(alpha, x))
 
 
-CHEB_CASES = [("cfg0_tpfa_c1", (3.0,), (2,)), ("spe10_z12", (3.0,), (2,)), ("spe10_z12", (3.0,), (3,)),
+CHEB_CASES = [("ln5_tpfa_c1", (3.0,), (2,)), ("spe10_z12", (3.0,), (2,)), ("spe10_z12", (3.0,), (3,)),
               ("spe10_z12", (2.0, 3.0), (1, 2)), ("spe10_z40", (3.0,), (4,))]
 
 
@@ -80,7 +80,7 @@ def test_uh_cheb_apply_bitexact(ctx, name, sigmas, niters):
 
 
 MS_CASES = [("spe10_z12", "simple_1", 2, (5,)), ("spe10_z12", "original", 3, (5,)),
-            ("spe10_z12", "none", 3, (2, 3)), ("cfg0_tpfa_c1", "original", 3, (3,)),
+            ("spe10_z12", "none", 3, (2, 3)), ("ln5_tpfa_c1", "original", 3, (3,)),
             ("small_spe10", "simple_1", 2, (4,))]
 
 
@@ -129,7 +129,7 @@ def _pcg_pair(ctx, s, prec="cheb", x0_ref=True, eps=1e-8, use_graph=False):
     return xo, so, x.get(), sg
 
 
-@pytest.mark.parametrize("name", ["cfg0_tpfa_c1", "spe10_z12"])
+@pytest.mark.parametrize("name", ["ln5_tpfa_c1", "spe10_z12"])
 @pytest.mark.parametrize("x0_ref", [True, False])
 def test_pcg_uh_cheb_vs_oracle(ctx, name, x0_ref):
     s = system(name)
@@ -169,7 +169,7 @@ def test_gmres_vs_oracle(ctx):
 
 
 def test_pcg_errors(ctx):
-    s = system("cfg0_spe")
+    s = system("ln5_spe")
     A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
     B = N.DevPrec(ctx, A, kind=N.DIAG)
     # b = A x0 with x0 = 0 -> b = 0 -> init norm 0 -> "No need to iterate"
@@ -186,7 +186,7 @@ def test_pcg_errors(ctx):
 
 
 def test_diag_prec_pcg_vs_oracle(ctx):
-    s = system("cfg0_tpfa")                      # the pure Dirichlet matrix, Jacobi-preconditioned
+    s = system("ln5_tpfa")                      # the pure Dirichlet matrix, Jacobi-preconditioned
     P = O.diag_prec(s.ia, s.ja, s.a)
     xo, so = O.pcg(s.ia, s.ja, s.a, s.b, P, eps=1e-8, x0=np.zeros(s.n), maxit=20000)
     A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
@@ -213,7 +213,7 @@ def test_mixed_precision_pcg(ctx):
 
 
 def test_full_size_pcg_fp64(ctx):
-    s = system("cfg1_spe")
+    s = system("spe10_z85")
     xo, so, xg, sg = _pcg_pair(ctx, s, "cheb", False, use_graph=True)
     assert abs(int(sg["niter"]) - so["niter"]) <= 1
     assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo) or abs(int(sg["niter"]) - so["niter"]) == 1
@@ -234,7 +234,7 @@ def _cheb_solver_pair(ctx, s, eps=1e-8, use_graph=True, check_every=0):
     return xo, so, x.get(), sg
 
 
-@pytest.mark.parametrize("name", ["cfg0_tpfa_c1", "spe10_z12"])
+@pytest.mark.parametrize("name", ["ln5_tpfa_c1", "spe10_z12"])
 def test_cheb_solver_vs_oracle(ctx, name):
     """ChebSolver: the coefficients depend only on the iteration count, so the
     iterates are bit-exact; only the stop test (a reduction) may differ by one."""
@@ -296,7 +296,7 @@ def test_simple_solver_vs_oracle(ctx, use_graph):
 
 
 def test_simple_solver_errors(ctx):
-    s = system("cfg0_spe")
+    s = system("ln5_spe")
     A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
     B = N.DevPrec(ctx, A, kind=N.DIAG)
     x = ctx.array(s.n)

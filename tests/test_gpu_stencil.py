@@ -29,7 +29,7 @@ def _pair(ctx, monkeypatch, s):
     return A1, A0
 
 
-@pytest.mark.parametrize("name", ["cfg0_spe", "cfg0_tpfa", "spe10_z12", "small_spe10"])
+@pytest.mark.parametrize("name", ["ln5_spe", "ln5_tpfa", "spe10_z12", "small_spe10"])
 def test_stencil_spmv_and_residual(ctx, monkeypatch, name):
     s = system(name)
     A1, A0 = _pair(ctx, monkeypatch, s)
