@@ -222,7 +222,7 @@ constexpr uint32_t kLdsBudget = 60 * 1024;
 
 // Uploads a set of host arrays as one device block: the host threads pack
 // them (converting element types where asked) into the context's pinned
-// staging buffer (kept for the next upload; a block over kMaxPinned goes
+// staging buffer (kept for the next upload; a block over max_pinned() goes
 // through a pageable buffer instead), one copy moves the block, and flush()
 // then points every registered field at its array.  The host arrays must live
 // until flush().
@@ -253,11 +253,15 @@ struct Stager {
     template <class P, class S, class Al> void add(const std::vector<S, Al>& v, P& field) {
         add(v.data(), v.size(), field);
     }
-    static constexpr size_t kMaxPinned = size_t(256) << 20;   // larger blocks go through pageable memory
+    // blocks up to this size use the pinned buffer: 256 MB, or MLP_STAGE_MAX bytes
+    static size_t max_pinned() {
+        const char* e = getenv("MLP_STAGE_MAX");
+        return e ? size_t(std::strtoull(e, nullptr, 10)) : size_t(256) << 20;
+    }
     void flush() {
         if (items.empty()) return;
         char* dev = mem.alloc<char>(total);
-        if (total <= kMaxPinned) {
+        if (total <= max_pinned()) {
             if (ctx->stage_bytes < total) {
                 if (ctx->stage) MLP_HIP_CHECK(hipHostFree(ctx->stage));
                 ctx->stage = nullptr;

@@ -84,3 +84,28 @@ def test_non_stencil_matrices_keep_slices(ctx):
     y = ctx.array(600)
     B.spmv(dev(ctx, x), y)
     assert np.array_equal(y.get(), O.spmv(ia, ja, av, x))
+
+
+@pytest.mark.parametrize("precision", [N.MLP_FP64, N.MLP_FP32])
+def test_pageable_staging_same_results(ctx, monkeypatch, precision):
+    """Uploads staged through pageable memory (MLP_STAGE_MAX=0, the path of blocks too large for the pinned
+    buffer) give the same device data: SpMV and preconditioner apply bitwise equal to the pinned path."""
+    s = system("spe10_z12")
+    H = N.build_uh_cheb(s.ia, s.ja, s.a)
+    f = ctx.to_device(rng(31).standard_normal(s.n))
+    out = []
+    for cap in (None, "0"):
+        if cap is None:
+            monkeypatch.delenv("MLP_STAGE_MAX", raising=False)
+        else:
+            monkeypatch.setenv("MLP_STAGE_MAX", cap)
+        A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+        B = N.DevPrec(ctx, A, H, precision=precision)
+        y, z = ctx.array(s.n), ctx.array(s.n)
+        A.spmv(f, y)
+        B.apply(f, z)
+        out.append((y.get(), z.get()))
+        B.close()
+        A.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
