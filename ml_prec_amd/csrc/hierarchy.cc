@@ -120,9 +120,58 @@ void eliminate(const HostCSR& A, const std::vector<uint32_t>& map, const std::ve
         std::vector<int32_t> slot;
         SortedSet cols;
         std::vector<double> vals;
+        std::vector<std::pair<uint32_t, double>> links;     // the direct path's (column, a/beta), sorted
+    };
+    // The direct path for the common row: every eliminated column k < min(i, M)
+    // it links to has the U row {k, i}, as a tail's does, so elimination only
+    // lowers the diagonal.  It performs the generic path's operations in the
+    // same order (the diagonal minus the links' a/beta in storage order, then
+    // minus l_ik u_ki in ascending k) and returns false, untouched, otherwise.
+    auto row_direct = [&](uint32_t i, Work& w, std::vector<uint32_t>& lja, std::vector<double>& la,
+                          std::vector<uint32_t>& oja, std::vector<double>& oa) -> bool {
+        const uint32_t r = map[i], lim = std::min(i, M);
+        auto& lk = w.links;
+        lk.clear();
+        double d = aux[r];
+        for (uint32_t p = A.ia[r] + 1; p < A.ia[r + 1]; p++) {
+            if (!links.present(r, p)) continue;
+            const uint32_t c = rmap[A.ja[p]];
+            if (c < lim && !(U.ia[c + 1] - U.ia[c] == 2 && U.ja[U.ia[c] + 1] == i)) return false;
+            const double z = A.a[p] / beta;
+            d -= z;
+            lk.emplace_back(c, z);
+        }
+        for (size_t t = 1; t < lk.size(); t++)                   // columns ascending (a handful of links)
+            for (size_t u = t; u > 0 && lk[u - 1].first > lk[u].first; u--) std::swap(lk[u - 1], lk[u]);
+        size_t t = 0;
+        for (; t < lk.size() && lk[t].first < lim; t++) {
+            const uint32_t k = lk[t].first;
+            const double lik = lk[t].second / U.a[U.ia[k]];
+            lja.push_back(k);
+            la.push_back(lik);
+            d -= lik * U.a[U.ia[k] + 1];
+        }
+        if (i < M) {
+            oja.push_back(i);
+            oa.push_back(d);
+            for (; t < lk.size(); t++) {
+                oja.push_back(lk[t].first);
+                oa.push_back(lk[t].second);
+            }
+        } else {
+            oja.push_back(i - M);
+            oa.push_back(d);
+            for (; t < lk.size(); t++)
+                if (lk[t].first >= M) {
+                    oja.push_back(lk[t].first - M);
+                    oa.push_back(lk[t].second);
+                }
+        }
+        return true;
     };
     auto row = [&](uint32_t i, Work& w, std::vector<uint32_t>& lja, std::vector<double>& la,
                    std::vector<uint32_t>& oja, std::vector<double>& oa) {
+        if (row_direct(i, w, lja, la, oja, oa)) return;
         for (uint32_t c : w.cols.v) w.slot[c] = -1;
         w.cols.clear();
         w.vals.clear();
