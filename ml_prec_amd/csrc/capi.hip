@@ -1373,7 +1373,7 @@ namespace {
 struct DevChebLinks : ChebLinkPhase {
     mlp_ctx ctx;
     explicit DevChebLinks(mlp_ctx c) : ctx(c) {}
-    void run(const HostCSR& A, double beta, std::vector<uint32_t>& cnt, std::vector<int>& deg,
+    void run(const HostCSR& A, double beta, HostVec<uint32_t>& cnt, std::vector<int>& deg,
              std::vector<double>& aux) override {
         const uint32_t n = A.nrow;
         const size_t nnz = A.nnz();

@@ -52,24 +52,45 @@ void order_by_magnitude(const double* a, uint32_t n, std::vector<uint32_t>& idx)
 // Lower entries point at their upper twin.
 struct TwoSidedLinks {
     const HostCSR* A;
-    std::vector<uint32_t> cnt;      // upper entries: counter; lower entries: position of the twin
-    explicit TwoSidedLinks(const HostCSR& M) : A(&M), cnt(M.nnz()) {
+    HostVec<uint32_t> cnt;          // upper entries: counter; lower entries: position of the twin
+    HostVec<uint32_t> utwin;        // upper entries: position of the twin (host construction only)
+    explicit TwoSidedLinks(const HostCSR& M) : A(&M), cnt(M.nnz()), utwin(M.nnz()) {
         std::atomic<bool> asym{false};
         parallel_for(M.nrow, 4096, [&](uint64_t b, uint64_t e) {
             for (uint32_t i = uint32_t(b); i < e; i++) {
                 cnt[M.ia[i]] = 2u;
                 for (uint32_t p = M.ia[i] + 1; p < M.ia[i + 1]; p++) {
                     const uint32_t j = M.ja[p];
+                    if (j > i) {                                       // its twin's row writes utwin[p]
+                        cnt[p] = 2u;
+                        continue;
+                    }
                     const uint32_t t = skyline_find(M, j, i);           // the twin entry (j, i)
-                    if (t == kNone) asym = true;
-                    cnt[p] = j < i ? t : 2u;
+                    if (t == kNone) {
+                        asym = true;
+                        continue;
+                    }
+                    cnt[p] = t;
+                    utwin[t] = p;
                 }
             }
         });
         if (asym) throw Error("uh_cheb needs a structurally symmetric matrix", 1);
+        // every upper entry must have been found as a twin: a structurally
+        // symmetric matrix has as many lower as upper entries, each lower entry
+        // found exactly one upper twin, so equal counts settle it
+        std::atomic<uint64_t> lower{0}, upper{0};
+        parallel_for(M.nrow, 4096, [&](uint64_t b, uint64_t e) {
+            uint64_t lo = 0, up = 0;
+            for (uint32_t i = uint32_t(b); i < e; i++)
+                for (uint32_t p = M.ia[i] + 1; p < M.ia[i + 1]; p++) (M.ja[p] < i ? lo : up)++;
+            lower += lo;
+            upper += up;
+        });
+        if (lower != upper) throw Error("uh_cheb needs a structurally symmetric matrix", 1);
     }
     // counters and twin positions computed elsewhere (ChebLinkPhase)
-    TwoSidedLinks(const HostCSR& M, std::vector<uint32_t>&& c) : A(&M), cnt(std::move(c)) {}
+    TwoSidedLinks(const HostCSR& M, HostVec<uint32_t>&& c) : A(&M), cnt(std::move(c)) {}
     uint32_t slot(uint32_t row, uint32_t p) const { return A->ja[p] < row ? cnt[p] : p; }
     bool present(uint32_t row, uint32_t p) const { return cnt[slot(row, p)] != 0; }
     bool mark(uint32_t row, uint32_t p) { return --cnt[slot(row, p)] == 0; }   // true: removed from both ends
@@ -172,6 +193,7 @@ void eliminate(const HostCSR& A, const std::vector<uint32_t>& map, const std::ve
     auto row = [&](uint32_t i, Work& w, std::vector<uint32_t>& lja, std::vector<double>& la,
                    std::vector<uint32_t>& oja, std::vector<double>& oa) {
         if (row_direct(i, w, lja, la, oja, oa)) return;
+        if (w.slot.empty()) w.slot.assign(N, -1);         // only rows that need the generic path pay for it
         for (uint32_t c : w.cols.v) w.slot[c] = -1;
         w.cols.clear();
         w.vals.clear();
@@ -227,7 +249,6 @@ void eliminate(const HostCSR& A, const std::vector<uint32_t>& map, const std::ve
     // eliminated rows, in order (each reads the U rows before it)
     {
         Work w;
-        w.slot.assign(N, -1);
         for (uint32_t i = 0; i < M; i++) {
             row(i, w, L.ja, L.a, U.ja, U.a);
             L.ia.push_back(uint32_t(L.ja.size()));
@@ -244,7 +265,6 @@ void eliminate(const HostCSR& A, const std::vector<uint32_t>& map, const std::ve
     const uint64_t step = (uint64_t(n) + nch - 1) / nch;
     parallel_for(nch, 1, [&](uint64_t c0, uint64_t c1) {
         Work w;
-        w.slot.assign(N, -1);
         for (uint64_t c = c0; c < c1; c++) {
             Chunk& C = ch[c];
             const uint32_t b = M + uint32_t(std::min<uint64_t>(n, c * step)), e = M + uint32_t(std::min<uint64_t>(n, (c + 1) * step));
@@ -341,7 +361,7 @@ struct ChebBuilder {
         std::vector<int> deg(N);
         li.aux.resize(N);
         if (dev) {
-            std::vector<uint32_t> cnt;
+            HostVec<uint32_t> cnt;
             dev->run(A, li.beta, cnt, deg, li.aux);
             TwoSidedLinks links(A, std::move(cnt));
             finish(l, A, links, deg);
@@ -352,7 +372,7 @@ struct ChebBuilder {
         // A row's choice depends only on its own values, so the rows choose in
         // parallel (marked[p]: row i asks to drop the link at its entry p); a
         // link is removed once both of its rows asked, which lowers both degrees.
-        std::vector<unsigned char> marked(A.nnz());
+        HostVec<unsigned char> marked(A.nnz());                    // every row clears its own entries
         parallel_for(N, 4096, [&](uint64_t r0, uint64_t r1) {
             std::vector<uint32_t> order;
             for (uint32_t i = uint32_t(r0); i < r1; i++) {
@@ -378,7 +398,7 @@ struct ChebBuilder {
                 int d = int(A.ia[i + 1] - A.ia[i] - 1);
                 for (uint32_t p = A.ia[i] + 1; p < A.ia[i + 1]; p++) {
                     const uint32_t u = links.slot(i, p);          // the upper entry of the link
-                    const uint32_t twin = u == p ? skyline_find(A, A.ja[p], i) : u;
+                    const uint32_t twin = u == p ? links.utwin[p] : u;
                     const uint32_t m = uint32_t(marked[p]) + uint32_t(marked[twin]);
                     if (m == 2) d--;
                     if (u == p) links.cnt[p] = 2u - m;

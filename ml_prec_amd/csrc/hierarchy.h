@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include "par.h"
+
 namespace mlp {
 
 // Skyline CSR (diagonal first, then ascending columns): modules/matrix/matrix.h:269-291
@@ -74,7 +76,7 @@ struct Hierarchy {
 // a structurally asymmetric A throws the host's error.
 struct ChebLinkPhase {
     virtual ~ChebLinkPhase() = default;
-    virtual void run(const HostCSR& A, double beta, std::vector<uint32_t>& cnt, std::vector<int>& deg,
+    virtual void run(const HostCSR& A, double beta, HostVec<uint32_t>& cnt, std::vector<int>& deg,
                      std::vector<double>& aux) = 0;
 };
 

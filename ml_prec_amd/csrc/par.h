@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -31,6 +32,21 @@ inline unsigned host_threads() {
     const unsigned h = std::thread::hardware_concurrency();
     return std::max(1u, std::min(h ? h : 1u, 16u));
 }
+
+// Allocator whose value-less construct leaves elements uninitialised, so that
+// large host buffers are first written (and their pages first touched) by the
+// threads that fill them.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U> struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <class U> NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    template <class U> void construct(U* p) noexcept { ::new (static_cast<void*>(p)) U; }
+    template <class U, class... Args> void construct(U* p, Args&&... args) {
+        ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+    }
+};
+template <class T> using HostVec = std::vector<T, NoInitAlloc<T>>;
 
 namespace detail {
 

@@ -16,6 +16,8 @@
 #include <memory>
 #include <vector>
 
+#include "par.h"
+
 namespace mlp {
 
 // Raw arrays of one level (pointers into host memory).
@@ -81,20 +83,6 @@ struct LevelProgram {
     std::vector<uint32_t> flatq;         // F slot the flat row writes (r, or its position in the next level)
 };
 
-// Allocator whose value-less construct leaves elements uninitialised, so that
-// large host buffers are first written (and their pages first touched) by the
-// threads that fill them.
-template <class T>
-struct NoInitAlloc : std::allocator<T> {
-    template <class U> struct rebind { using other = NoInitAlloc<U>; };
-    NoInitAlloc() = default;
-    template <class U> NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
-    template <class U> void construct(U* p) noexcept { ::new (static_cast<void*>(p)) U; }
-    template <class U, class... Args> void construct(U* p, Args&&... args) {
-        ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
-    }
-};
-template <class T> using HostVec = std::vector<T, NoInitAlloc<T>>;
 
 // 64-row slices, column-major inside a slice: entry k of row 64s+t sits at
 // (off64[s] + k) * 64 + t.  Padding has col = 0xffffffff and val = 0.

@@ -176,3 +176,20 @@ def test_host_threads_do_not_change_the_hierarchy(monkeypatch, threads):
     Hm = N.build_multi_split(s.ia, s.ja, s.a, q=(0.8,), niters=(5,), order="simple_1", degree_max=2)
     assert same_hierarchy(ref, H)
     assert same_hierarchy(refm, Hm)
+
+
+@pytest.mark.parametrize("rows", [
+    [[0, 1], [1], [2, 1]],          # row 2 links to 1, row 1 not back (a lower entry without twin)
+    [[0, 1, 2], [1], [2]],          # row 0 links to 1 and 2, neither links back (upper entries without twin)
+    [[0, 2], [1, 2], [2, 0]],       # row 1 links to 2 alone; counts of lower and upper entries differ
+])
+def test_uh_cheb_rejects_asymmetric_structure(rows):
+    """uh_cheb's two-sided link counters need a structurally symmetric matrix: every stored (i, j) needs (j, i)."""
+    ia, ja, a = [0], [], []
+    for i, cols in enumerate(rows):
+        ja += cols
+        a += [4.0] + [-1.0] * (len(cols) - 1)
+        ia.append(len(ja))
+    with pytest.raises(N.MlpError) as e:
+        N.build_uh_cheb(np.array(ia, np.uint32), np.array(ja, np.uint32), np.array(a))
+    assert "structurally symmetric" in str(e.value)
