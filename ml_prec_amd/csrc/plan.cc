@@ -136,15 +136,29 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P, const Bloc
     P.Udiag.resize(M);
     P.Uptr.assign(M + 1, 0);
     std::vector<uint32_t> local(K, kNoNode);
+    // the rows of `in` (and, for the U list, only those < M) ordered by key,
+    // equal keys in their order in `in` (a stable sort, done by counting)
+    std::vector<uint32_t> count;
+    auto by_depth = [&count](const std::vector<uint32_t>& in, const std::vector<uint32_t>& key, uint32_t below,
+                             std::vector<uint32_t>& out) {
+        uint32_t kmax = 0, m = 0;
+        for (uint32_t i : in)
+            if (i < below) { kmax = std::max(kmax, key[i]); m++; }
+        count.assign(size_t(kmax) + 2, 0);
+        for (uint32_t i : in)
+            if (i < below) count[key[i] + 1]++;
+        for (uint32_t k = 0; k <= kmax; k++) count[k + 1] += count[k];
+        out.resize(m);
+        for (uint32_t i : in)
+            if (i < below) out[count[key[i]]++] = i;
+    };
+    std::vector<uint32_t> lo, uo;
     uint32_t pos = 0, upos = 0;
     for (uint32_t b = 0; b < nblk; b++) {
         auto& R = rows[b];
         const uint32_t base = pos;
-        std::vector<uint32_t> lo = R;
-        std::stable_sort(lo.begin(), lo.end(), [&](uint32_t x, uint32_t y) { return dL[x] < dL[y]; });
-        std::vector<uint32_t> uo;
-        for (uint32_t i : R) if (i < M) uo.push_back(i);
-        std::stable_sort(uo.begin(), uo.end(), [&](uint32_t x, uint32_t y) { return dU[x] < dU[y]; });
+        by_depth(R, dL, K, lo);
+        by_depth(R, dU, M, uo);
         for (uint32_t k = 0; k < lo.size(); k++) local[lo[k]] = k;
         uint32_t* B = &P.blk[kBlkFields * b];
         B[0] = base;
