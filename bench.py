@@ -452,6 +452,34 @@ def solvers(N, S, args):
     ctx.close()
 
 
+def apply_traffic_probe(N, s, reps):
+    """Preconditioner applies (FP64 and FP32 hierarchies, `reps` each, launched from the host) and the same
+    1 GiB calibration kernel as the SpMV probe, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs: the level
+    kernels' counters summed per apply give the DRAM bytes one apply moves."""
+    H = N.build_uh_cheb(s.ia, s.ja, s.a)
+    ctx = N.Context(0)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    f = ctx.to_device(np.random.default_rng(2).standard_normal(s.n))
+    z = ctx.array(s.n)
+    out = {"apply_traffic_probe": True, "reps": reps}
+    for prec, name in ((N.MLP_FP64, "fp64"), (N.MLP_FP32, "fp32")):
+        B = N.DevPrec(ctx, A, H, precision=prec)
+        ms = B.time_apply(f, z, reps=reps)
+        out[name] = {"algorithmic_bytes_per_apply": B.bytes_per_apply(), "us_per_apply": round(ms * 1e3, 2)}
+        log(f"{name}: {reps} applies, {ms * 1e3:.1f} us each")
+        ctx.sync()
+        B.close()
+    big = ctx.array(1 << 27)
+    for _ in range(3):
+        N.dscal(ctx, 0.5, big)
+    ctx.sync()
+    log("calibration: 3 launches of x = 0.5 x over 2^27 doubles (1 GiB read + 1 GiB written each)")
+    print(json.dumps(out), flush=True)
+    A.close()
+    del f, z, big
+    ctx.close()
+
+
 def spmv_traffic_probe(N, s, reps):
     """Cold SpMV launches plus a calibration kernel of known traffic: x = 0.5 x
     over 2^27 doubles reads and writes exactly 1 GiB each.  Used under
@@ -499,6 +527,8 @@ def main():
     ap.add_argument("--case", default="", help="with --solvers: comma-separated case names to run")
     ap.add_argument("--cpu-budget", type=float, default=25.0,
                     help="with --solvers: estimated seconds of single-core oracle solve allowed per case")
+    ap.add_argument("--apply-traffic", action="store_true",
+                    help="only run preconditioner applies and a 1 GiB scale kernel (for HBM counter runs), then exit")
     ap.add_argument("--spmv-traffic", action="store_true",
                     help="only run cold SpMV launches and a 1 GiB scale kernel (for HBM counter runs), then exit")
     args = ap.parse_args()
@@ -528,6 +558,9 @@ def main():
     log(f"system {s.nx}x{s.ny}x{s.nz}: n={s.n} nnz={s.nnz} generated in {time.perf_counter() - t0:.2f}s")
     if args.spmv_traffic:
         spmv_traffic_probe(N, s, args.spmv_reps)
+        return
+    if args.apply_traffic:
+        apply_traffic_probe(N, s, 20)
         return
 
     t0 = time.perf_counter()
