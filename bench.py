@@ -62,12 +62,44 @@ def cpu_baseline(s, eps, budget_s=20.0):
             "t_const_s": t_const, "niter": iters}, x
 
 
+def _ensemble_cpu_job(item):
+    """One ensemble member on one host core: the oracle's construction, then one timed PCG solve."""
+    nz, seed, eps = item
+    import oracle as O
+    from ml_prec_amd import spe_synth as S
+    s = S.ensemble_system(nz, seed)
+    P = O.cheb_prec(s.ia, s.ja, s.a)
+    t0 = time.perf_counter()
+    _, st = O.pcg(s.ia, s.ja, s.a, s.b, P, eps=eps, x0=np.zeros(s.n), maxit=10000)
+    return time.perf_counter() - t0, int(st["niter"])
+
+
+def ensemble_cpu_baseline(spec, eps, procs):
+    """The C oracle on every ensemble member, one process per host core (spawned before this process uses the
+    GPU): the aggregate rate is the member count over the summed single-core solve time divided by the processes."""
+    import multiprocessing as mp
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(procs) as pool:
+        res = pool.map(_ensemble_cpu_job, [(nz, seed, eps) for nz, seed in spec], chunksize=1)
+    wall = time.perf_counter() - t0
+    tsum = sum(t for t, _ in res)
+    return {"value": round(len(spec) / (tsum / procs), 2), "unit": "solves/s", "cores": procs, "kind": "port",
+            "sample": f"all {len(spec)} members, one oracle PCG solve each (construction excluded), in {procs} "
+                      f"processes; summed single-core solve time {tsum:.2f} s, wall time with construction "
+                      f"{wall:.1f} s", "iterations": [it for _, it in res]}
+
+
 def ensemble(N, S, args):
     """Ensemble: COUNT SPE10-shaped systems of 60x220xnz cells, nz drawn from 5/10/20/40/85, each with its own
     field, uh_cheb hierarchy and context (stream).  A step solves all of them to 1e-8 from x0 = 0, concurrently
     through mlp_pcg_batch; the same systems solved one after another are timed beside it."""
     from concurrent.futures import ThreadPoolExecutor
     spec = S.ensemble_spec(args.ensemble)
+    cpu = None
+    if not args.no_cpu:
+        procs = max(1, min(16, os.cpu_count() or 1))
+        cpu = ensemble_cpu_baseline(spec, args.eps, procs)
+        log(f"CPU oracle: {cpu['value']:.1f} solves/s aggregate on {procs} processes ({cpu['sample']})")
     t0 = time.perf_counter()
     with ThreadPoolExecutor(8) as ex:
         systems = list(ex.map(lambda p: S.ensemble_system(*p), spec))
@@ -125,6 +157,7 @@ def ensemble(N, S, args):
         "speedup_concurrent": round(ms_seq / ms_batch, 3),
         "iterations": iters,
         "worst_true_rel_res": worst,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     for ctx, A, B, b, x in jobs:
