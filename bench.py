@@ -168,13 +168,50 @@ def ensemble(N, S, args):
         j[0].close()
 
 
+def _rhs_cpu_job(item):
+    """Right-hand sides of the 60x220xnz system on one host core: oracle construction, then one timed PCG solve
+    per right-hand side."""
+    nz, seeds, eps = item
+    import oracle as O
+    from ml_prec_amd import spe_synth as S
+    s = S.spe10_system(kind="spe", c=1.0, nz=nz)
+    P = O.cheb_prec(s.ia, s.ja, s.a)
+    out = []
+    for k in seeds:
+        f = S.rhs(s.n, S.BASE_SEED + 500 + k)
+        t0 = time.perf_counter()
+        O.pcg(s.ia, s.ja, s.a, f, P, eps=eps, x0=np.zeros(s.n), maxit=10000)
+        out.append(time.perf_counter() - t0)
+    return out
+
+
+def rhs_cpu_baseline(nz, R, eps, procs):
+    """The C oracle on the R right-hand sides, spread over `procs` processes (each builds the hierarchy once;
+    spawned before this process uses the GPU); aggregate = R over the summed solve time divided by procs."""
+    import multiprocessing as mp
+    procs = min(procs, R)
+    parts = [list(range(k, R, procs)) for k in range(procs)]
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(procs) as pool:
+        res = pool.map(_rhs_cpu_job, [(nz, part, eps) for part in parts], chunksize=1)
+    wall = time.perf_counter() - t0
+    tsum = sum(sum(r) for r in res)
+    return {"value": round(R / (tsum / procs), 2), "unit": "solves/s", "cores": procs, "kind": "port",
+            "sample": f"all {R} right-hand sides, one oracle PCG solve each in {procs} processes (construction "
+                      f"excluded); summed single-core solve time {tsum:.2f} s, wall time with construction {wall:.1f} s"}
+
+
 def ensemble_rhs(N, S, args):
     """Many right-hand sides: one 60x220x85 system with COUNT seeded right-hand sides.  Batched: one mlp_pcg_multi call
     whose every launch covers all right-hand sides (the reported value).  For comparison: the same solves as
     concurrent jobs on clones of the preconditioner (mlp_pcg_batch) and one after another (mlp_pcg)."""
+    R = args.ensemble_rhs
+    cpu = None
+    if not args.no_cpu:
+        cpu = rhs_cpu_baseline(args.nz, R, args.eps, max(1, min(16, os.cpu_count() or 1)))
+        log(f"CPU oracle: {cpu['value']:.1f} solves/s aggregate ({cpu['sample']})")
     s = S.spe10_system(kind="spe", c=1.0, nz=args.nz)
     H = N.build_uh_cheb(s.ia, s.ja, s.a)
-    R = args.ensemble_rhs
     ctx0 = N.Context(0)
     A = N.DevMatrix(ctx0, s.ia, s.ja, s.a)
     B0 = N.DevPrec(ctx0, A, H)
@@ -256,7 +293,7 @@ def ensemble_rhs(N, S, args):
                                  "speedup_vs_fp64_batch": round(ms_multi / ms_multi32, 3),
                                  "worst_true_rel_res": max(d["true_rel_res"] for d in st_multi32)},
         "speedup_vs_one_after_another": round(ms_seq / ms_multi, 3), "bitwise_equal_to_single": same,
-        "iterations": iters}), flush=True)
+        "iterations": iters, "cpu_baseline": cpu}), flush=True)
     for c, Aj, Bj, b, x in jobs:
         Bj.close()
     B0.close()
@@ -264,6 +301,35 @@ def ensemble_rhs(N, S, args):
     for j in jobs:
         j[0].close()
     ctx0.close()
+
+
+def adjoint_cpu_baseline(S, shape, theta, b, obs, d, eps=1e-10):
+    """One misfit gradient on one host core with the C oracle's construction and PCG solves (forward and
+    adjoint to eps), the numpy assembly and the face-by-face gradient formula in numpy."""
+    import oracle as O
+    from ml_prec_amd.adjoint import KZ_RATIO, misfit
+    t0 = time.perf_counter()
+    kx = np.exp(theta)
+    ia, ja, a = S.assemble(kx.reshape(shape), kind="spe", c=1.0)
+    P = O.cheb_prec(ia, ja, a)
+    n = kx.size
+    p, st_f = O.pcg(ia, ja, a, b, P, eps=eps, x0=np.zeros(n), maxit=20000)
+    _, g = misfit(p, obs, d)
+    lam, st_a = O.pcg(ia, ja, a, g, P, eps=eps, x0=np.zeros(n), maxit=20000)
+    k3, p3, l3 = kx.reshape(shape), p.reshape(shape), lam.reshape(shape)
+    grad = np.zeros(shape)
+    for axis, fac, kr in ((2, 1 / S.HX**2, 1.0), (1, 1 / S.HY**2, 1.0), (0, 1 / S.HZ**2, KZ_RATIO)):
+        lo, hi = [slice(None)] * 3, [slice(None)] * 3
+        lo[axis], hi[axis] = slice(None, -1), slice(1, None)
+        ka, kb = kr * k3[tuple(lo)], kr * k3[tuple(hi)]
+        T = 2.0 / (1.0 / ka + 1.0 / kb) * fac
+        w = (l3[tuple(lo)] - l3[tuple(hi)]) * (p3[tuple(lo)] - p3[tuple(hi)])
+        grad[tuple(lo)] -= T * T / (2 * fac * ka) * w
+        grad[tuple(hi)] -= T * T / (2 * fac * kb) * w
+    t = time.perf_counter() - t0
+    return {"value": round(1e3 * t, 1), "unit": "ms per gradient", "cores": 1, "kind": "port",
+            "sample": f"one gradient: numpy assembly, oracle construction, forward and adjoint oracle PCG to {eps:g} "
+                      f"({st_f['niter']} and {st_a['niter']} iterations), numpy gradient formula"}, grad.ravel()
 
 
 def adjoint(N, S, args):
@@ -304,6 +370,12 @@ def adjoint(N, S, args):
         errs.append(abs(fd - grad @ v) / abs(fd))
     log(f"central differences along {args.fd_dirs} directions: max rel. difference {max(errs):.2e}, "
         f"median {np.median(errs):.2e}")
+    cpu = None
+    if not args.no_cpu:
+        cpu, gcpu = adjoint_cpu_baseline(S, shape, theta, b, obs, d)
+        cpu["rel_diff_vs_gpu"] = float(np.linalg.norm(gcpu - grad) / np.linalg.norm(grad))
+        log(f"CPU oracle: {cpu['value']:.0f} ms per gradient ({cpu['sample']}); gradients differ by "
+            f"{cpu['rel_diff_vs_gpu']:.1e} relative")
     out = {
         "metric": "adjoint misfit gradient time at 1.12M cells (forward + adjoint PCG to 1e-10)",
         "value": round(1e3 * float(np.median(times)), 3),
@@ -324,6 +396,7 @@ def adjoint(N, S, args):
         "misfit": L,
         "fd_check": {"directions": args.fd_dirs, "h": h, "max_rel_diff": max(errs),
                      "median_rel_diff": float(np.median(errs))},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     model.close()
