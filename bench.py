@@ -32,6 +32,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host's CPU model and logical CPU count (recorded beside every CPU baseline)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs visible"
+
+
 def load_traffic():
     """HBM bytes per SpMV launch from the committed rocprofv3 PMC summary, if present."""
     p = os.path.join(ROOT, "profiles", "k_spmv_spe10_60x220x85_traffic.json")
@@ -56,7 +70,7 @@ def cpu_baseline(s, eps, budget_s=20.0):
         iters = st["niter"]
         if sum(times) + t_const > budget_s / 2 or len(times) >= 5:
             break
-    return {"value": 1e3 * float(np.median(times)), "unit": "ms per solve", "cores": 1, "kind": "port",
+    return {"value": 1e3 * float(np.median(times)), "unit": "ms per solve", "cores": 1, "kind": "port", "host": host_cpu(),
             "sample": f"{len(times)} full PCG solves of the same 60x220x85 system (x0=0, eps={eps:g}, "
                       f"{iters} iterations each) by the C oracle on one host core; construction {t_const:.2f} s",
             "t_const_s": t_const, "niter": iters}, x
@@ -84,6 +98,7 @@ def ensemble_cpu_baseline(spec, eps, procs):
     wall = time.perf_counter() - t0
     tsum = sum(t for t, _ in res)
     return {"value": round(len(spec) / (tsum / procs), 2), "unit": "solves/s", "cores": procs, "kind": "port",
+            "host": host_cpu(),
             "sample": f"all {len(spec)} members, one oracle PCG solve each (construction excluded), in {procs} "
                       f"processes; summed single-core solve time {tsum:.2f} s, wall time with construction "
                       f"{wall:.1f} s", "iterations": [it for _, it in res]}
@@ -197,6 +212,7 @@ def rhs_cpu_baseline(nz, R, eps, procs):
     wall = time.perf_counter() - t0
     tsum = sum(sum(r) for r in res)
     return {"value": round(R / (tsum / procs), 2), "unit": "solves/s", "cores": procs, "kind": "port",
+            "host": host_cpu(),
             "sample": f"all {R} right-hand sides, one oracle PCG solve each in {procs} processes (construction "
                       f"excluded); summed single-core solve time {tsum:.2f} s, wall time with construction {wall:.1f} s"}
 
@@ -327,7 +343,7 @@ def adjoint_cpu_baseline(S, shape, theta, b, obs, d, eps=1e-10):
         grad[tuple(lo)] -= T * T / (2 * fac * ka) * w
         grad[tuple(hi)] -= T * T / (2 * fac * kb) * w
     t = time.perf_counter() - t0
-    return {"value": round(1e3 * t, 1), "unit": "ms per gradient", "cores": 1, "kind": "port",
+    return {"value": round(1e3 * t, 1), "unit": "ms per gradient", "cores": 1, "kind": "port", "host": host_cpu(),
             "sample": f"one gradient: numpy assembly, oracle construction, forward and adjoint oracle PCG to {eps:g} "
                       f"({st_f['niter']} and {st_a['niter']} iterations), numpy gradient formula"}, grad.ravel()
 
@@ -442,7 +458,8 @@ def cpu_case_baseline(O, s, a, case, gpu_niter, budget_s):
     P.apply(s.b)
     t_apply = time.perf_counter() - t0
     est = t_apply * gpu_niter * (2.5 if solver == "gmres" else 1.3)
-    rec = {"apply_ms": round(1e3 * t_apply, 3), "construction_s": round(t_const, 3), "cores": 1, "kind": "port"}
+    rec = {"apply_ms": round(1e3 * t_apply, 3), "construction_s": round(t_const, 3), "cores": 1, "kind": "port",
+           "host": host_cpu()}
     if est > budget_s:
         rec.update(ms_per_solve=round(1e3 * t_apply * gpu_niter, 2), niter=None,
                    sample=f"one preconditioner apply x {gpu_niter} iterations (lower bound; a full solve is "
