@@ -17,6 +17,7 @@ import pytest
 import oracle as O
 from conftest import rng, system
 from ml_prec_amd import native as N
+from ml_prec_amd import spe_synth as S
 
 pytestmark = pytest.mark.gpu
 
@@ -310,3 +311,27 @@ def test_simple_solver_errors(ctx):
     with pytest.raises(N.MlpError) as e:
         N.simple_solver(ctx, A, B, dev(ctx, s.b), x, eps=1e-12, maxit=4, x0_mode=N.X0_ZERO)
     assert e.value.code == 5
+
+
+def test_full_size_applies_bitexact(ctx):
+    """At 60x220x85: the uh_cheb and multi_split (project/msplit.cfg settings) FP64 applies equal the oracle's bit
+    for bit, and so does GS on the nonsymmetric variant of the matrix (the brick sweep)."""
+    s = system("spe10_z85")
+    f = rng(91).standard_normal(s.n)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    cases = [(N.DevPrec(ctx, A, N.build_uh_cheb(s.ia, s.ja, s.a)), O.cheb_prec(s.ia, s.ja, s.a)),
+             (N.DevPrec(ctx, A, N.build_multi_split(s.ia, s.ja, s.a, q=(0.8,), niters=(5,), order="simple_1",
+                                                    degree_max=2)),
+              O.msplit_prec(s.ia, s.ja, s.a, q=(0.8,), niters=(5,), order="simple_1", degree_max=2))]
+    for B, P in cases:
+        x = ctx.array(s.n)
+        B.apply(dev(ctx, f), x)
+        assert np.array_equal(x.get(), P.apply(f)), B.kind
+        B.close()
+    A.close()
+    a_ns = S.make_nonsymmetric(s.ia, s.ja, s.a, 0.3)
+    A = N.DevMatrix(ctx, s.ia, s.ja, a_ns)
+    B = N.DevPrec(ctx, A, kind=N.GS)
+    x = ctx.array(s.n)
+    B.apply(dev(ctx, f), x)
+    assert np.array_equal(x.get(), O.gs_prec(s.ia, s.ja, a_ns).apply(f))
