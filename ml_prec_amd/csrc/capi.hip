@@ -1160,6 +1160,13 @@ using namespace mlp;
         return MLP_ERR_INVALID;                       \
     }
 
+// Calls on a context run on its device, whichever device the calling thread
+// had current (allocations and copies follow the current device).
+static void use_device(mlp_ctx c) {
+    int d = -1;
+    if (c && (hipGetDevice(&d) != hipSuccess || d != c->device)) MLP_HIP_CHECK(hipSetDevice(c->device));
+}
+
 static void require(bool c, const char* msg) {
     if (!c) throw Error(msg, MLP_ERR_INVALID);
 }
@@ -1197,6 +1204,7 @@ int mlp_ctx_destroy(mlp_ctx ctx) {
 int mlp_ctx_sync(mlp_ctx ctx) {
     MLP_API_BEGIN
     require(ctx, "null context");
+    use_device(ctx);
     MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     MLP_API_END
 }
@@ -1204,6 +1212,7 @@ int mlp_ctx_sync(mlp_ctx ctx) {
 int mlp_ctx_stream(mlp_ctx ctx, void** s) {
     MLP_API_BEGIN
     require(ctx && s, "null argument");
+    use_device(ctx);
     *s = (void*)ctx->stream;
     MLP_API_END
 }
@@ -1211,6 +1220,7 @@ int mlp_ctx_stream(mlp_ctx ctx, void** s) {
 int mlp_dev_alloc(mlp_ctx ctx, size_t bytes, void** d_out) {
     MLP_API_BEGIN
     require(ctx && d_out, "null argument");
+    use_device(ctx);
     MLP_HIP_CHECK(hipMalloc(d_out, std::max<size_t>(bytes, 1)));
     MLP_API_END
 }
@@ -1218,6 +1228,7 @@ int mlp_dev_alloc(mlp_ctx ctx, size_t bytes, void** d_out) {
 int mlp_dev_free(mlp_ctx ctx, void* p) {
     MLP_API_BEGIN
     require(ctx, "null context");
+    use_device(ctx);
     if (p) MLP_HIP_CHECK(hipFree(p));
     MLP_API_END
 }
@@ -1225,6 +1236,7 @@ int mlp_dev_free(mlp_ctx ctx, void* p) {
 int mlp_copy_h2d(mlp_ctx ctx, void* dst, const void* src, size_t bytes) {
     MLP_API_BEGIN
     require(ctx, "null context");
+    use_device(ctx);
     if (bytes) {
         MLP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
         MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1235,6 +1247,7 @@ int mlp_copy_h2d(mlp_ctx ctx, void* dst, const void* src, size_t bytes) {
 int mlp_copy_d2h(mlp_ctx ctx, void* dst, const void* src, size_t bytes) {
     MLP_API_BEGIN
     require(ctx, "null context");
+    use_device(ctx);
     if (bytes) {
         MLP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
         MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1256,6 +1269,7 @@ static void check_skyline(uint32_t n, const uint32_t* ia, const uint32_t* ja) {
 int mlp_csr_upload(mlp_ctx ctx, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, mlp_mat* out) {
     MLP_API_BEGIN
     require(ctx && out, "null argument");
+    use_device(ctx);
     check_skyline(n, ia, ja);
     require(a || n == 0, "null values");
     auto* m = new mlp_mat_s();
@@ -1279,6 +1293,7 @@ int mlp_csr_upload(mlp_ctx ctx, uint32_t n, const uint32_t* ia, const uint32_t* 
 
 int mlp_mat_destroy(mlp_mat m) {
     MLP_API_BEGIN
+    use_device(m ? m->ctx : nullptr);
     if (m) {
         (void)hipStreamSynchronize(m->ctx->stream);
         m->ctx->drop_graphs(m->id);
@@ -1299,6 +1314,7 @@ int mlp_mat_info(mlp_mat m, uint32_t* n, uint32_t* nnz, uint64_t* bytes) {
 int mlp_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y) {
     MLP_API_BEGIN
     require(ctx && A, "null argument");
+    use_device(ctx);
     hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream, A->A.v, x, y,
                        (const uint32_t*)nullptr);
     MLP_HIP_CHECK(hipGetLastError());
@@ -1308,6 +1324,7 @@ int mlp_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y) {
 int mlp_residual(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, double* r) {
     MLP_API_BEGIN
     require(ctx && A, "null argument");
+    use_device(ctx);
     hipLaunchKernelGGL((k_residual<double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream, A->A.v, b, x, r,
                        (const uint32_t*)nullptr);
     MLP_HIP_CHECK(hipGetLastError());
@@ -1337,6 +1354,7 @@ static double dev_dot(mlp_ctx ctx, uint32_t n, const double* x, const double* y)
 int mlp_ddot(mlp_ctx ctx, uint32_t n, const double* x, const double* y, double* out) {
     MLP_API_BEGIN
     require(ctx && out, "null argument");
+    use_device(ctx);
     *out = dev_dot(ctx, n, x, y);
     MLP_API_END
 }
@@ -1344,6 +1362,7 @@ int mlp_ddot(mlp_ctx ctx, uint32_t n, const double* x, const double* y, double* 
 int mlp_dnrm2(mlp_ctx ctx, uint32_t n, const double* x, double* out) {
     MLP_API_BEGIN
     require(ctx && out, "null argument");
+    use_device(ctx);
     *out = std::sqrt(dev_dot(ctx, n, x, x));
     MLP_API_END
 }
@@ -1351,6 +1370,7 @@ int mlp_dnrm2(mlp_ctx ctx, uint32_t n, const double* x, double* out) {
 int mlp_daxpy(mlp_ctx ctx, uint32_t n, double alpha, const double* x, double* y) {
     MLP_API_BEGIN
     require(ctx, "null context");
+    use_device(ctx);
     if (n)
         hipLaunchKernelGGL(k_axpy, dim3(red_grid(n)), dim3(kBS), 0, ctx->stream, n, alpha, (int)host_is_one(alpha), x, y);
     MLP_HIP_CHECK(hipGetLastError());
@@ -1360,6 +1380,7 @@ int mlp_daxpy(mlp_ctx ctx, uint32_t n, double alpha, const double* x, double* y)
 int mlp_dscal(mlp_ctx ctx, uint32_t n, double alpha, double* x) {
     MLP_API_BEGIN
     require(ctx, "null context");
+    use_device(ctx);
     if (n && !host_is_one(alpha))
         hipLaunchKernelGGL(k_scal, dim3(red_grid(n)), dim3(kBS), 0, ctx->stream, n, alpha, x);
     MLP_HIP_CHECK(hipGetLastError());
@@ -1426,6 +1447,7 @@ static int build_uh_cheb_entry(mlp_ctx ctx, uint32_t n, const uint32_t* ia, cons
     p.niters.assign(niters, niters + nniters);
     p.use_tails = use_tails != 0;
     if (ctx) {
+        use_device(ctx);
         DevChebLinks dev(ctx);
         build_uh_cheb(h->A0, p, h->H, &dev);
     } else {
@@ -1567,6 +1589,7 @@ int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_des
     require(kind == MLP_PREC_UH_CHEB || kind == MLP_PREC_MULTI_SPLIT, "unknown hierarchy kind");
     require(precision == MLP_FP64 || precision == MLP_FP32, "unknown precision");
     require(levels[0].N == A0->n, "level 0 size differs from A0");
+    use_device(ctx);
     auto p = std::make_unique<mlp_prec_s>();
     p->ctx = ctx;
     p->id = mlp::new_handle_id();
@@ -1582,6 +1605,7 @@ int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_des
 int mlp_prec_from_hier(mlp_ctx ctx, mlp_hier h, mlp_mat A0, int precision, mlp_prec* out) {
     MLP_API_BEGIN
     require(ctx && h && A0 && out, "null argument");
+    use_device(ctx);
     const Hierarchy& H = h->H;
     std::vector<mlp_level_desc> d(H.nlevels);
     for (uint32_t l = 0; l < H.nlevels; l++) {
@@ -1607,6 +1631,7 @@ int mlp_prec_simple(mlp_ctx ctx, int kind, mlp_mat A0, mlp_prec* out) {
     MLP_API_BEGIN
     require(ctx && A0 && out, "null argument");
     require(kind == MLP_PREC_DIAG || kind == MLP_PREC_ID, "kind must be DIAG or ID");
+    use_device(ctx);
     auto sp = std::make_unique<SimplePrec>();
     sp->ctx = ctx;
     sp->n = A0->n;
@@ -1632,6 +1657,7 @@ int mlp_prec_simple(mlp_ctx ctx, int kind, mlp_mat A0, mlp_prec* out) {
 int mlp_prec_gs(mlp_ctx ctx, mlp_mat A, mlp_prec* out) {
     MLP_API_BEGIN
     require(ctx && A && out, "null argument");
+    use_device(ctx);
     auto g = std::make_unique<DevGS>();
     g->ctx = ctx;
     g->build(A->ia, A->ja, A->a);
@@ -1653,6 +1679,7 @@ int mlp_prec_composite(mlp_ctx ctx, mlp_mat A, mlp_hier ms, mlp_prec* out) {
                 std::equal(A->ja.begin(), A->ja.end(), ms->A0.ja.begin(), ms->A0.ja.end()) &&
                 std::equal(A->a.begin(), A->a.end(), ms->A0.a.begin(), ms->A0.a.end()),
             "the multi_split hierarchy must be built on A");
+    use_device(ctx);
     auto g = std::make_unique<DevGS>();
     g->ctx = ctx;
     g->build(A->ia, A->ja, A->a);
@@ -1679,6 +1706,7 @@ int mlp_prec_sym(mlp_ctx ctx, mlp_mat A, const double* sigmas, uint32_t nsigmas,
     MLP_API_BEGIN
     require(ctx && A && out, "null argument");
     require(sigmas && nsigmas && niters && nniters, "sigmas and niters are required");
+    use_device(ctx);
     auto y = std::make_unique<DevSym>(ctx);
     y->build(A, sigmas, nsigmas, niters, nniters, use_tails);
     auto p = std::make_unique<mlp_prec_s>();
@@ -1695,6 +1723,7 @@ int mlp_prec_clone(mlp_ctx ctx, mlp_prec src, mlp_prec* out) {
     MLP_API_BEGIN
     require(ctx && src && out, "null argument");
     require(ctx->device == src->ctx->device, "the clone must be on the device of the source");
+    use_device(ctx);
     std::unique_ptr<DevPrecBase> impl = src->impl->clone_on(ctx);
     require(impl != nullptr, "only uh_cheb and multi_split preconditioners can be cloned");
     auto p = std::make_unique<mlp_prec_s>();
@@ -1718,6 +1747,7 @@ int mlp_prec_sym_set_eps(mlp_prec p, double eps) {
 
 int mlp_prec_destroy(mlp_prec p) {
     MLP_API_BEGIN
+    use_device(p ? p->ctx : nullptr);
     if (p) {
         (void)hipStreamSynchronize(p->ctx->stream);
         p->ctx->drop_graphs(p->id);
@@ -1729,6 +1759,7 @@ int mlp_prec_destroy(mlp_prec p) {
 int mlp_prec_apply(mlp_ctx ctx, mlp_prec p, const double* f, double* x) {
     MLP_API_BEGIN
     require(ctx && p, "null argument");
+    use_device(ctx);
     p->impl->apply(f, x, nullptr);
     MLP_HIP_CHECK(hipGetLastError());
     MLP_API_END
@@ -1738,6 +1769,7 @@ int mlp_time_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y, uint32_t r
                   double* avg_ms) {
     MLP_API_BEGIN
     require(ctx && A && avg_ms && reps, "null argument");
+    use_device(ctx);
     // flush_bytes > 0: launches rotate over copies of the matrix totalling at
     // least flush_bytes, so each launch finds its matrix evicted from L2 and
     // the Infinity Cache by the launches before it; 0: the same matrix every
@@ -1811,6 +1843,7 @@ int mlp_time_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y, uint32_t r
 int mlp_time_prec_apply(mlp_ctx ctx, mlp_prec p, const double* f, double* x, uint32_t reps, double* avg_ms) {
     MLP_API_BEGIN
     require(ctx && p && avg_ms && reps, "null argument");
+    use_device(ctx);
     hipEvent_t e0, e1;
     MLP_HIP_CHECK(hipEventCreate(&e0));
     MLP_HIP_CHECK(hipEventCreate(&e1));
@@ -2265,6 +2298,7 @@ int mlp_pcg(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, cons
             mlp_solve_stats* stats) {
     MLP_API_BEGIN
     require(ctx && A && B && b && x && o, "null argument");
+    use_device(ctx);
     mlp_pcg_job J;
     std::memset(&J, 0, sizeof(J));
     J.ctx = ctx;
@@ -2293,6 +2327,7 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
     require(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs must lie in [1, 64]");
     require(B->n == A->n, "preconditioner size differs from A");
     require(B->ctx == ctx && A->ctx->device == ctx->device, "the preconditioner must live on the context");
+    use_device(ctx);
     const bool flex = o->flexible != 0;
     const uint32_t tre = o->true_res_every;
     const auto t0 = std::chrono::steady_clock::now();
@@ -2443,6 +2478,7 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
         Run& R = runs[i];
         R.j = &jobs[i];
         mlp_ctx ctx = R.j->ctx;
+        use_device(ctx);
         mlp_mat A = R.j->A;
         const uint32_t n = A->n, rg = red_grid(n);
         R.t0 = std::chrono::steady_clock::now();
@@ -2549,6 +2585,7 @@ int mlp_spe_adjoint_grad(mlp_ctx ctx, uint32_t nx, uint32_t ny, uint32_t nz, dou
                          double kz_ratio, const double* kx, const double* p, const double* lam, double* grad) {
     MLP_API_BEGIN
     require(ctx && kx && p && lam && grad, "null argument");
+    use_device(ctx);
     const uint64_t n = uint64_t(nx) * ny * nz;
     require(n > 0 && n < (1ull << 32), "grid size out of range");
     require(fx > 0 && fy > 0 && fz > 0 && kz_ratio > 0, "face factors must be positive");
@@ -2567,6 +2604,7 @@ int mlp_gmres(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, co
     MLP_API_BEGIN
     require(ctx && A && B && b && x && o, "null argument");
     require(B->n == A->n, "preconditioner size differs from A");
+    use_device(ctx);
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t n = A->n, m = kGmresRestart;
     ensure_workspace(ctx, n, true);
@@ -2681,6 +2719,7 @@ int mlp_cheb_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, double lmin, double lmax
     require(ctx && A && B && b && x && o, "null argument");
     require(B->n == A->n, "preconditioner size differs from A");
     require(lmax > lmin && lmin > 0, "Chebyshev bounds need 0 < lmin < lmax");
+    use_device(ctx);
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t n = A->n;
     hipStream_t s = ctx->stream;
@@ -2756,6 +2795,7 @@ int mlp_simple_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, doubl
     MLP_API_BEGIN
     require(ctx && A && B && b && x && o, "null argument");
     require(B->n == A->n, "preconditioner size differs from A");
+    use_device(ctx);
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t n = A->n;
     hipStream_t s = ctx->stream;
