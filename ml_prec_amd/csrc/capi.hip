@@ -117,8 +117,6 @@ struct mlp_ctx_s {
     double* scal = nullptr;            // 4 result slots for standalone BLAS-1
     mlp::SolveState* st = nullptr;
     double* host_pinned = nullptr;     // 64 doubles
-    char* stage = nullptr;             // pinned staging for uploads (Stager), grown on demand
-    size_t stage_bytes = 0;
     // batched right-hand sides (mlp_pcg_multi): R vectors of each kind, R states, R partial buffers
     mlp::DevMem multi_mem;
     uint32_t multi_R = 0, multi_n = 0;
@@ -220,12 +218,26 @@ struct DevPrecBase {
 
 constexpr uint32_t kLdsBudget = 60 * 1024;
 
+// The pinned staging buffer of one device, shared by its contexts (grown on
+// demand, kept until the process ends; a flush holds the lock from packing to
+// the end of its copy).
+struct PinnedStage {
+    std::mutex m;
+    char* p = nullptr;
+    size_t n = 0;
+};
+constexpr int kMaxStageDevices = 64;
+inline PinnedStage& pinned_stage(int device) {
+    static PinnedStage st[kMaxStageDevices];
+    if (device < 0 || device >= kMaxStageDevices) throw Error("device index out of range", MLP_ERR_INVALID);
+    return st[device];
+}
+
 // Uploads a set of host arrays as one device block: the host threads pack
-// them (converting element types where asked) into the context's pinned
-// staging buffer (kept for the next upload; a block over max_pinned() goes
-// through a pageable buffer instead), one copy moves the block, and flush()
-// then points every registered field at its array.  The host arrays must live
-// until flush().
+// them (converting element types where asked) into the device's pinned
+// staging buffer (a block over max_pinned() goes through a pageable buffer
+// instead), one copy moves the block, and flush() then points every
+// registered field at its array.  The host arrays must live until flush().
 struct Stager {
     mlp_ctx ctx;
     DevMem& mem;
@@ -262,22 +274,24 @@ struct Stager {
         if (items.empty()) return;
         char* dev = mem.alloc<char>(total);
         if (total <= max_pinned()) {
-            if (ctx->stage_bytes < total) {
-                if (ctx->stage) MLP_HIP_CHECK(hipHostFree(ctx->stage));
-                ctx->stage = nullptr;
-                ctx->stage_bytes = 0;
-                MLP_HIP_CHECK(hipHostMalloc((void**)&ctx->stage, total, hipHostMallocDefault));
-                ctx->stage_bytes = total;
+            PinnedStage& st = pinned_stage(ctx->device);
+            std::lock_guard<std::mutex> hold(st.m);
+            if (st.n < total) {
+                if (st.p) MLP_HIP_CHECK(hipHostFree(st.p));
+                st.p = nullptr;
+                st.n = 0;
+                MLP_HIP_CHECK(hipHostMalloc((void**)&st.p, total, hipHostMallocDefault));
+                st.n = total;
             }
-            for (Item& it : items) it.write(ctx->stage + it.off);
-            MLP_HIP_CHECK(hipMemcpyAsync(dev, ctx->stage, total, hipMemcpyHostToDevice, ctx->stream));
+            for (Item& it : items) it.write(st.p + it.off);
+            MLP_HIP_CHECK(hipMemcpyAsync(dev, st.p, total, hipMemcpyHostToDevice, ctx->stream));
+            MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         } else {
             HostVec<char> buf(total);
             for (Item& it : items) it.write(buf.data() + it.off);
             MLP_HIP_CHECK(hipMemcpyAsync(dev, buf.data(), total, hipMemcpyHostToDevice, ctx->stream));
             MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         }
-        MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         for (Item& it : items) it.set(dev + it.off);
         items.clear();
         total = 0;
@@ -1040,9 +1054,6 @@ void context_release(mlp_ctx ctx) {
     ctx->host_pinned = nullptr;
     if (ctx->mst_host) (void)hipHostFree(ctx->mst_host);
     ctx->mst_host = nullptr;
-    if (ctx->stage) (void)hipHostFree(ctx->stage);
-    ctx->stage = nullptr;
-    ctx->stage_bytes = 0;
     if (ctx->owns_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     ctx->stream = nullptr;
 }
