@@ -148,8 +148,7 @@ void eliminate(const HostCSR& A, const std::vector<uint32_t>& map, const std::ve
     // lowers the diagonal.  It performs the generic path's operations in the
     // same order (the diagonal minus the links' a/beta in storage order, then
     // minus l_ik u_ki in ascending k) and returns false, untouched, otherwise.
-    auto row_direct = [&](uint32_t i, Work& w, std::vector<uint32_t>& lja, std::vector<double>& la,
-                          std::vector<uint32_t>& oja, std::vector<double>& oa) -> bool {
+    auto row_direct = [&](uint32_t i, Work& w, auto& lja, auto& la, auto& oja, auto& oa) -> bool {
         const uint32_t r = map[i], lim = std::min(i, M);
         auto& lk = w.links;
         lk.clear();
@@ -190,8 +189,7 @@ void eliminate(const HostCSR& A, const std::vector<uint32_t>& map, const std::ve
         }
         return true;
     };
-    auto row = [&](uint32_t i, Work& w, std::vector<uint32_t>& lja, std::vector<double>& la,
-                   std::vector<uint32_t>& oja, std::vector<double>& oa) {
+    auto row = [&](uint32_t i, Work& w, auto& lja, auto& la, auto& oja, auto& oa) {
         if (row_direct(i, w, lja, la, oja, oa)) return;
         if (w.slot.empty()) w.slot.assign(N, -1);         // only rows that need the generic path pay for it
         for (uint32_t c : w.cols.v) w.slot[c] = -1;
@@ -276,18 +274,33 @@ void eliminate(const HostCSR& A, const std::vector<uint32_t>& map, const std::ve
             }
         }
     });
-    size_t ltot = L.ja.size(), ntot = 0;
-    for (const Chunk& C : ch) { ltot += C.lja.size(); ntot += C.nja.size(); }
-    L.ja.reserve(ltot); L.a.reserve(ltot); L.ia.reserve(size_t(nkeep) + 1);
-    next.ja.reserve(ntot); next.a.reserve(ntot); next.ia.reserve(size_t(n) + 1);
-    for (const Chunk& C : ch) {
-        L.ja.insert(L.ja.end(), C.lja.begin(), C.lja.end());
-        L.a.insert(L.a.end(), C.la.begin(), C.la.end());
-        next.ja.insert(next.ja.end(), C.nja.begin(), C.nja.end());
-        next.a.insert(next.a.end(), C.na.begin(), C.na.end());
-        for (uint32_t x : C.lcnt) L.ia.push_back(L.ia.back() + x);
-        for (uint32_t x : C.ncnt) next.ia.push_back(next.ia.back() + x);
+    // stitch the chunks in row order: offsets first, then every chunk copies its
+    // rows into place on its own (the destination arrays are not zero-filled)
+    std::vector<size_t> roff(nch + 1, 0), loff(nch + 1, 0), noff(nch + 1, 0);
+    for (uint32_t c = 0; c < nch; c++) {
+        roff[c + 1] = roff[c] + ch[c].lcnt.size();
+        loff[c + 1] = loff[c] + ch[c].lja.size();
+        noff[c + 1] = noff[c] + ch[c].nja.size();
     }
+    const size_t l0 = L.ja.size(), r0 = L.ia.size();
+    L.ja.resize(l0 + loff[nch]);
+    L.a.resize(l0 + loff[nch]);
+    L.ia.resize(r0 + roff[nch]);
+    next.ja.resize(noff[nch]);
+    next.a.resize(noff[nch]);
+    next.ia.resize(1 + roff[nch]);
+    parallel_for(nch, 1, [&](uint64_t c0, uint64_t c1) {
+        for (uint64_t c = c0; c < c1; c++) {
+            const Chunk& C = ch[c];
+            std::copy(C.lja.begin(), C.lja.end(), L.ja.begin() + (l0 + loff[c]));
+            std::copy(C.la.begin(), C.la.end(), L.a.begin() + (l0 + loff[c]));
+            std::copy(C.nja.begin(), C.nja.end(), next.ja.begin() + noff[c]);
+            std::copy(C.na.begin(), C.na.end(), next.a.begin() + noff[c]);
+            uint32_t lp = uint32_t(l0 + loff[c]), np = uint32_t(noff[c]);
+            for (size_t t = 0; t < C.lcnt.size(); t++) L.ia[r0 + roff[c] + t] = lp += C.lcnt[t];
+            for (size_t t = 0; t < C.ncnt.size(); t++) next.ia[1 + roff[c] + t] = np += C.ncnt[t];
+        }
+    });
 }
 
 std::vector<uint32_t> inverse(const std::vector<uint32_t>& map) {

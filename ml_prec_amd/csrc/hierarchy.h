@@ -15,8 +15,8 @@ namespace mlp {
 // Skyline CSR (diagonal first, then ascending columns): modules/matrix/matrix.h:269-291
 struct HostCSR {
     uint32_t nrow = 0, ncol = 0;
-    std::vector<uint32_t> ia{0}, ja;
-    std::vector<double> a;
+    HostVec<uint32_t> ia{0}, ja;
+    HostVec<double> a;
     uint32_t nnz() const { return ia.empty() ? 0 : ia.back(); }
 };
 
