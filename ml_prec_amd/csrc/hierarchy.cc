@@ -342,9 +342,10 @@ struct ChebBuilder {
                 do {
                     map[pos++] = cur;
                     done[cur] = 1;
+                    uint32_t u = kNone;                     // the link's upper-entry slot
                     for (uint32_t p = A.ia[cur] + 1; p < A.ia[cur + 1]; p++)
-                        if (present(cur, p)) { nb = A.ja[p]; break; }
-                    gone[skyline_find(A, std::min(cur, nb), std::max(cur, nb))] = 1;
+                        if (present(cur, p)) { nb = A.ja[p]; u = links.slot(cur, p); break; }
+                    gone[u != kNone ? u : skyline_find(A, std::min(cur, nb), std::max(cur, nb))] = 1;
                     deg[cur] = -1;
                     deg[nb]--;
                     cur = nb;
