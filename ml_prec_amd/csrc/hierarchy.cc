@@ -584,29 +584,32 @@ struct MultiSplitBuilder {
         std::vector<int> dout(N), din(N, 0);
         OneSidedLinks links(A);
         li.aux.assign(N, 0.0);
-        std::vector<uint32_t> order;
-        // one-sided link removal against q/(1-q)*c (multi_split_constr.cc:70-113)
-        for (uint32_t i = 0; i < N; i++) {
-            const uint32_t b = A.ia[i], e = A.ia[i + 1], nrz = e - b - 1;
-            dout[i] = int(nrz);
-            double c = A.a[b];
-            for (uint32_t p = b + 1; p < e; p++) {
-                din[A.ja[p]]++;
-                c += A.a[p];
+        // one-sided link removal against q/(1-q)*c (multi_split_constr.cc:70-113).
+        // A row's removals depend only on its own values, so the rows decide in
+        // parallel; the in-degrees count the links left in each column (integer
+        // atomics: the same counts in any order).
+        parallel_for(N, 4096, [&](uint64_t r0, uint64_t r1) {
+            std::vector<uint32_t> order;
+            for (uint32_t i = uint32_t(r0); i < r1; i++) {
+                const uint32_t b = A.ia[i], e = A.ia[i + 1], nrz = e - b - 1;
+                dout[i] = int(nrz);
+                double c = A.a[b];
+                for (uint32_t p = b + 1; p < e; p++) c += A.a[p];
+                order_by_magnitude(A.a.data() + b + 1, nrz, order);
+                double budget = li.q / (1 - li.q) * c;
+                for (uint32_t t = 0; t < nrz; t++) {
+                    const uint32_t p = b + 1 + order[t];
+                    const double v = -A.a[p];
+                    if (!(v <= budget)) break;
+                    links.on[p] = 0;
+                    dout[i]--;
+                    budget -= v;
+                }
+                li.aux[i] = c * (1. / (1 - li.q));
+                for (uint32_t p = b + 1; p < e; p++)
+                    if (links.on[p]) __atomic_fetch_add(&din[A.ja[p]], 1, __ATOMIC_RELAXED);
             }
-            order_by_magnitude(A.a.data() + b + 1, nrz, order);
-            double budget = li.q / (1 - li.q) * c;
-            for (uint32_t t = 0; t < nrz; t++) {
-                const uint32_t p = b + 1 + order[t];
-                const double v = -A.a[p];
-                if (!(v <= budget)) break;
-                links.on[p] = 0;
-                dout[i]--;
-                din[A.ja[p]]--;
-                budget -= v;
-            }
-            li.aux[i] = c * (1. / (1 - li.q));
-        }
+        });
         switch (P.order) {
             case ElimOrder::NONE: order_none(N, din, dout, li); break;
             case ElimOrder::ORIGINAL: order_original(A, links, din, dout, li); break;
