@@ -425,6 +425,11 @@ SOLVER_CASES = (
     # name, matrix, outer solver, preconditioner, iteration cap (None: solve to eps)
     ("pcg_uh_cheb", "sym", "pcg", "uh_cheb", None),
     ("pcg_multi_split", "sym", "pcg", "multi_split", None),
+    # FP32 hierarchy inside FP64 flexible PCG, true residual every 20 iterations
+    ("pcg_mixed_uh_cheb", "sym", "pcg_mixed", "uh_cheb", None),
+    ("pcg_mixed_multi_split", "sym", "pcg_mixed", "multi_split", None),
+    # FP64 hierarchy in the same flexible PCG (multi_split is not a symmetric preconditioner)
+    ("pcg_flex_multi_split", "sym", "pcg_flex", "multi_split", None),
     ("gmres_uh_cheb", "sym", "gmres", "uh_cheb", None),
     ("cheb_solver_uh_cheb", "sym", "cheb", "uh_cheb", None),
     ("simple_solver_multi_split", "sym", "simple", "multi_split", None),
@@ -467,7 +472,7 @@ def cpu_case_baseline(O, s, a, case, gpu_niter, budget_s):
         return rec, None
     kw = dict(eps=1e-8, x0=np.zeros(s.n), maxit=cap or 20000, check=cap is None)
     t0 = time.perf_counter()
-    if solver == "pcg":
+    if solver in ("pcg", "pcg_mixed", "pcg_flex"):        # the oracle's PCG (no FP32 hierarchy, no flexible beta)
         x, st = O.pcg(s.ia, s.ja, a, s.b, P, **kw)
     elif solver == "gmres":
         x, st = O.gmres(s.ia, s.ja, a, s.b, P, **kw)
@@ -501,12 +506,13 @@ def solvers(N, S, args):
         t0 = time.perf_counter()
         A = N.DevMatrix(ctx, s.ia, s.ja, a)
         H = None
+        precision = N.MLP_FP32 if solver == "pcg_mixed" else N.MLP_FP64
         if prec == "uh_cheb":
             H = N.build_uh_cheb(s.ia, s.ja, a)
-            B = N.DevPrec(ctx, A, H)
+            B = N.DevPrec(ctx, A, H, precision=precision)
         elif prec == "multi_split":
             H = N.build_multi_split(s.ia, s.ja, a, **MSPLIT_CFG)
-            B = N.DevPrec(ctx, A, H)
+            B = N.DevPrec(ctx, A, H, precision=precision)
         elif prec == "gs":
             B = N.DevPrec(ctx, A, kind=N.GS)
         elif prec == "composite":
@@ -522,6 +528,9 @@ def solvers(N, S, args):
         kw = dict(eps=args.eps, maxit=cap or 20000, x0_mode=N.X0_ZERO, raise_on_error=cap is None)
         if solver == "pcg":
             run = lambda: N.pcg(ctx, A, B, b, x, check_every=8, use_graph=bool(args.graph), **kw)
+        elif solver in ("pcg_mixed", "pcg_flex"):
+            run = lambda: N.pcg(ctx, A, B, b, x, check_every=8, use_graph=bool(args.graph), flexible=True,
+                                true_res_every=20, **kw)
         elif solver == "gmres":
             run = lambda: N.gmres(ctx, A, B, b, x, **kw)
         elif solver == "cheb":
