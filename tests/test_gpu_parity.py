@@ -335,3 +335,20 @@ def test_full_size_applies_bitexact(ctx):
     x = ctx.array(s.n)
     B.apply(dev(ctx, f), x)
     assert np.array_equal(x.get(), O.gs_prec(s.ia, s.ja, a_ns).apply(f))
+
+
+@pytest.mark.parametrize("precision", [N.MLP_FP64, N.MLP_FP32])
+def test_flexible_pcg_multi_split(ctx, precision):
+    """Flexible PCG (beta from (r, z - z_prev)) with the multi_split hierarchy, FP64 or FP32: beside the parity
+    bar (the reference's PCG has no flexible beta), it must reach the tolerance on the true residual, agree with
+    the oracle's solution to the accuracy both solves have, and need no more iterations than the oracle's PCG."""
+    s = system("spe10_z12")
+    P, H = _msplit(s)
+    xo, so = O.pcg(s.ia, s.ja, s.a, s.b, P, eps=1e-8, x0=np.zeros(s.n), maxit=5000)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, H, precision=precision)
+    x = ctx.array(s.n)
+    sg = N.pcg(ctx, A, B, dev(ctx, s.b), x, eps=1e-8, maxit=5000, flexible=True, true_res_every=20)
+    assert sg["true_rel_res"] <= 1e-8
+    assert np.linalg.norm(x.get() - xo) <= 1e-5 * np.linalg.norm(xo)
+    assert sg["niter"] <= so["niter"]
