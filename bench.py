@@ -666,7 +666,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (each system is solved on one "
-              "device; this build has no multi-GPU path)", file=sys.stderr)
+              "device; there is no multi-GPU path)", file=sys.stderr)
         sys.exit(2)
 
     from ml_prec_amd import native as N

@@ -1,8 +1,8 @@
 """ctypes wrapper around the C oracle (``liboracle.so``) -- TEST INFRASTRUCTURE ONLY.
 
 Only the test suite, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
-``bench.py`` may import this module.  It is the checker for the HIP product
-path, never part of it.
+``bench.py`` may import this module.  The HIP product path is tested against
+it and never calls it.
 """
 from __future__ import annotations
 

@@ -1,8 +1,8 @@
 /*
  * CPU oracle for the ml_prec iterative-solve path -- TEST INFRASTRUCTURE ONLY.
  *
- * A plain-C restatement of the reference's serial algorithms.  It is the
- * checker for the HIP product path: only tests, the smoke check and the
+ * A plain-C restatement of the reference's serial algorithms, against which
+ * the HIP product path is tested: only tests, the smoke check and the
  * benchmark's CPU-baseline leg load it.  Citations are reference file:line.
  *
  * Compiled with -ffp-contract=off and no -march, so every a*b+c is a rounded

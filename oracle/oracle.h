@@ -2,7 +2,7 @@
  * CPU oracle for the ml_prec iterative-solve path.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference's
- * serial algorithms, used as the checker for the HIP product path.  Only the
+ * serial algorithms, against which the HIP product path is tested.  Only the
  * test suite, the smoke check and the CPU-baseline leg of the benchmark load
  * it; the product library never links or calls it.
  *
