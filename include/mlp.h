@@ -11,7 +11,7 @@
  *     prefix are host pointers;
  *   - all work of a context is ordered on the context's HIP stream; a context
  *     is not thread-safe, like the reference's solvers (whose scratch vectors
- *     are `mutable` members, modules/prec/cheb/cheb_prec.h:49-50);
+ *     are `mutable` members, modules/prec/cheb/cheb_prec.h:35-36);
  *   - matrices use the reference's Skyline CSR layout: uint32 ia[n+1],
  *     uint32 ja[nnz], double a[nnz], diagonal first in every row, then
  *     ascending columns (modules/matrix/matrix.h:187-291).
@@ -37,13 +37,15 @@ extern "C" {
 #define MLP_ERR_MAXIT       5   /* iteration cap reached (no counterpart: the reference loops on) */
 #define MLP_ERR_CONSTRUCT   6   /* hierarchy construction ("Too many levels", ...) */
 #define MLP_ERR_CONFIG      7   /* prec config file errors   multi_split_constr.cc:204-228 */
+#define MLP_ERR_NO_CONV     8   /* flexible PCG: the FP64 true residual missed eps after the restarts
+                                   (no counterpart: the reference has no flexible PCG) */
 
 /* ---- precision of a device preconditioner hierarchy ---- */
 #define MLP_FP64 0
 #define MLP_FP32 1
 
 /* ---- preconditioner kinds ---- */
-#define MLP_PREC_UH_CHEB     1  /* Prec, "uh_cheb"           modules/prec/cheb/cheb_prec.h:27 */
+#define MLP_PREC_UH_CHEB     1  /* Prec, "uh_cheb"           modules/prec/cheb/cheb_prec.h:13 */
 #define MLP_PREC_MULTI_SPLIT 2  /* MultiSplitPrec            modules/prec/multi_split/multi_split_prec.h:15 */
 #define MLP_PREC_DIAG        3  /* DiagPrec                  modules/prec/diag/diag_prec.cc:7-23 */
 #define MLP_PREC_ID          4  /* IdPrec                    modules/prec/id/id_prec.h:6-19 */
@@ -147,7 +149,7 @@ int mlp_hier_level_array(mlp_hier h, uint32_t level, int which, void* out);
 
 /* ---- device preconditioners ----
  * A hierarchy as plain per-level arrays: what Prec::Level / MultiSplitPrec::Level
- * hold (cheb_prec.h:31-54, multi_split_prec.h:19-44).  A_next is levels[l+1].A. */
+ * hold (cheb_prec.h:17-40, multi_split_prec.h:19-44).  A_next is levels[l+1].A. */
 typedef struct {
     uint32_t N, M, Md;
     uint32_t nit;                            /* ncheb or niter */
@@ -196,7 +198,11 @@ typedef struct {
     double eps;               /* relative tolerance on the recurrence residual */
     uint32_t maxit;           /* 0 = no cap */
     int x0_mode;
-    int flexible;             /* PCG: Polak-Ribiere beta (for an inexact preconditioner) */
+    int flexible;             /* PCG: Polak-Ribiere beta (for an inexact preconditioner); once the
+                                 recurrence residual meets eps the FP64 true residual b - A x is
+                                 computed, and while it is above eps PCG restarts from it, at most 8
+                                 times; still above eps after that: MLP_ERR_NO_CONV (x is the last
+                                 iterate, stats filled) */
     uint32_t true_res_every;  /* PCG: recompute r = b - A x every k iterations (0 = never) */
     uint32_t check_every;     /* iterations enqueued between host convergence checks (0 = 16) */
     int use_graph;            /* capture one iteration in a hipGraph and replay it */

@@ -21,6 +21,8 @@
 #include <cstring>
 #include <memory>
 #include <functional>
+#include <list>
+#include <mutex>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -140,24 +142,41 @@ struct mlp_ctx_s {
         hipGraph_t g[3] = {nullptr, nullptr, nullptr};
         hipGraphExec_t e[3] = {nullptr, nullptr, nullptr};
     };
-    std::vector<GraphEntry> graphs;
+    // a list, so that an entry stays where it is while others are added or dropped
+    std::list<GraphEntry> graphs;
+    static constexpr size_t kMaxGraphs = 16;
     // Chebyshev outer-iteration coefficients alpha_k, beta_k (k = iteration
     // count) for the bounds (coef_lmin, coef_lmax), filled up to coef_filled
     double* coef = nullptr;
     uint32_t coef_filled = 0;
     double coef_lmin = 0.0, coef_lmax = 0.0;
+    static void destroy_graph(GraphEntry& g) {
+        for (int k = 0; k < 3; k++) {
+            if (g.e[k]) (void)hipGraphExecDestroy(g.e[k]);
+            if (g.g[k]) (void)hipGraphDestroy(g.g[k]);
+            g.e[k] = nullptr;
+            g.g[k] = nullptr;
+        }
+    }
+    // drops the graphs of matrix or preconditioner `id` (0: all); the stream must be idle
     void drop_graphs(uint64_t id) {
-        for (size_t i = 0; i < graphs.size();) {
-            if (id == 0 || graphs[i].A == id || graphs[i].B == id) {
-                for (int k = 0; k < 3; k++) {
-                    if (graphs[i].e[k]) (void)hipGraphExecDestroy(graphs[i].e[k]);
-                    if (graphs[i].g[k]) (void)hipGraphDestroy(graphs[i].g[k]);
-                }
-                graphs.erase(graphs.begin() + i);
+        for (auto it = graphs.begin(); it != graphs.end();) {
+            if (id == 0 || it->A == id || it->B == id) {
+                destroy_graph(*it);
+                it = graphs.erase(it);
             } else {
-                i++;
+                ++it;
             }
         }
+    }
+    // caches an instantiated entry, dropping the oldest beyond kMaxGraphs; the stream must be idle
+    GraphEntry* add_graph(const GraphEntry& e) {
+        while (graphs.size() >= kMaxGraphs) {
+            destroy_graph(graphs.front());
+            graphs.pop_front();
+        }
+        graphs.push_back(e);
+        return &graphs.back();
     }
 };
 
@@ -1081,7 +1100,10 @@ namespace {
 // uh_cheb hierarchy built on Asym, and an apply that is a PCG solve of Asym to
 // 1e-14 from generate_x0.  The inner solve runs on a sub-context that shares
 // the caller's stream; it synchronises with the host, so the apply is not
-// captured in graphs.
+// captured in graphs.  The inner PCG always solves from the instance's own
+// right-hand side and solution buffers (f is copied in, the solution out), so
+// its iteration graph is captured once and replayed by every apply, whatever
+// vectors the caller passes (GMRES passes a different Krylov vector each step).
 struct DevSym : DevPrecBase {
     mlp_ctx parent;
     std::unique_ptr<mlp_ctx_s, void (*)(mlp_ctx)> sub;
@@ -1089,6 +1111,8 @@ struct DevSym : DevPrecBase {
     mlp_prec B = nullptr;
     DevMem mem;
     double* x0 = nullptr;                  // generate_x0's vector, made once
+    double* fin = nullptr;                 // the inner solve's right-hand side and solution
+    double* xin = nullptr;
     uint32_t n = 0;
     double eps = 1e-14;                    // sym_prec.cc:37, SymPrec::set_eps (sym_prec.h:26-28)
     explicit DevSym(mlp_ctx c) : parent(c), sub(make_sub_context(c)) {}
@@ -1123,6 +1147,8 @@ struct DevSym : DevPrecBase {
         srandom(3);                                            // generate_x0, modules/solvers/solvers.cc:9-21
         for (uint32_t i = 0; i < n; i++) hx[i] = 20. * (random() - 0.5 * RAND_MAX) / RAND_MAX + 100;
         x0 = mem.upload(hx, parent->stream);
+        fin = mem.alloc<double>(n);
+        xin = mem.alloc<double>(n);
         MLP_HIP_CHECK(hipStreamSynchronize(parent->stream));
     }
     void apply(const double* f, double* x, const uint32_t* guard) override {
@@ -1134,7 +1160,9 @@ struct DevSym : DevPrecBase {
             std::memcpy(&g, sub->host_pinned, sizeof(g));
             if (g) return;
         }
-        MLP_HIP_CHECK(hipMemcpyAsync(x, x0, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, parent->stream));
+        const size_t bytes = size_t(n) * sizeof(double);
+        MLP_HIP_CHECK(hipMemcpyAsync(fin, f, bytes, hipMemcpyDeviceToDevice, parent->stream));
+        MLP_HIP_CHECK(hipMemcpyAsync(xin, x0, bytes, hipMemcpyDeviceToDevice, parent->stream));
         mlp_solve_opts o;
         std::memset(&o, 0, sizeof(o));
         o.eps = eps;
@@ -1143,8 +1171,9 @@ struct DevSym : DevPrecBase {
         o.check_every = 8;
         o.use_graph = 1;
         mlp_solve_stats st;
-        const int rc = mlp_pcg(sub.get(), Asym, B, f, x, &o, &st);
+        const int rc = mlp_pcg(sub.get(), Asym, B, fin, xin, &o, &st);
         if (rc) throw Error(mlp_last_error(), rc);
+        MLP_HIP_CHECK(hipMemcpyAsync(x, xin, bytes, hipMemcpyDeviceToDevice, parent->stream));
     }
     double bytes_per_apply() const override { return 0.0; }
     uint32_t launches_per_apply() const override { return 0; }
@@ -1898,6 +1927,9 @@ namespace mlp {
 namespace {
 
 constexpr uint32_t kGmresRestart = 30;     // gmres.cc:21
+// Flexible PCG: PCG phases restarted from the FP64 true residual b - A x after the
+// recurrence residual met eps, at most this many times (include/mlp.h).
+constexpr uint32_t kFlexRestarts = 8;
 
 void ensure_workspace(mlp_ctx ctx, uint32_t n, bool gmres) {
     if (ctx->ws_n >= n && (!gmres || ctx->V)) return;
@@ -2080,9 +2112,7 @@ mlp_ctx_s::GraphEntry* pcg_graphs(mlp_ctx ctx, mlp_mat A, mlp_prec B, const doub
         MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[v]));
         MLP_HIP_CHECK(hipGraphInstantiate(&e.e[v], e.g[v], nullptr, nullptr, 0));
     }
-    if (ctx->graphs.size() >= 16) ctx->drop_graphs(ctx->graphs.front().A);
-    ctx->graphs.push_back(e);
-    return &ctx->graphs.back();
+    return ctx->add_graph(e);
 }
 
 // Workspace of R batched solves of size n on the context (kept for the next
@@ -2090,14 +2120,10 @@ mlp_ctx_s::GraphEntry* pcg_graphs(mlp_ctx ctx, mlp_mat A, mlp_prec B, const doub
 PcgVecs ensure_multi(mlp_ctx ctx, uint32_t n, uint32_t R) {
     if (ctx->multi_R < R || ctx->multi_n < n) {
         MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        for (size_t i = 0; i < ctx->graphs.size();) {
-            auto& g = ctx->graphs[i];
-            if (g.solver != 3) { i++; continue; }
-            for (int k = 0; k < 3; k++) {
-                if (g.e[k]) (void)hipGraphExecDestroy(g.e[k]);
-                if (g.g[k]) (void)hipGraphDestroy(g.g[k]);
-            }
-            ctx->graphs.erase(ctx->graphs.begin() + i);
+        for (auto it = ctx->graphs.begin(); it != ctx->graphs.end();) {
+            if (it->solver != 3) { ++it; continue; }
+            mlp_ctx_s::destroy_graph(*it);
+            it = ctx->graphs.erase(it);
         }
         ctx->multi_mem.release();
         if (ctx->mst_host) (void)hipHostFree(ctx->mst_host);
@@ -2144,9 +2170,7 @@ mlp_ctx_s::GraphEntry* multi_graph(mlp_ctx ctx, mlp_mat A, mlp_prec B, DevPrecBa
         MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[v]));
         MLP_HIP_CHECK(hipGraphInstantiate(&e.e[v], e.g[v], nullptr, nullptr, 0));
     }
-    if (ctx->graphs.size() >= 16) ctx->drop_graphs(ctx->graphs.front().A);
-    ctx->graphs.push_back(e);
-    return &ctx->graphs.back();
+    return ctx->add_graph(e);
 }
 
 // iterations k0+1 .. k0+chunk (graph replays or direct launches)
@@ -2260,9 +2284,7 @@ SolveState run_outer(mlp_ctx ctx, int solver, mlp_mat A, mlp_prec B, const void*
                 MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[v]));
                 MLP_HIP_CHECK(hipGraphInstantiate(&e.e[v], e.g[v], nullptr, nullptr, 0));
             }
-            if (ctx->graphs.size() >= 16) ctx->drop_graphs(ctx->graphs.front().A);
-            ctx->graphs.push_back(e);
-            ge = &ctx->graphs.back();
+            ge = ctx->add_graph(e);
         }
     }
     SolveState st;
@@ -2401,7 +2423,7 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
             SolveState& st = fin[q];
             const double* bq = b + size_t(q) * n;
             double* xq = x + size_t(q) * n;
-            for (int restart = 0; restart < 8 && st.status == 0; restart++) {
+            for (uint32_t restart = 0; restart < kFlexRestarts && st.status == 0; restart++) {
                 if (true_rel_res(ctx, A, bq, xq, ctx->r, st.init_norm) <= st.eps) break;
                 bool again = false;
                 const double init_norm = st.init_norm, eps = st.eps;
@@ -2437,6 +2459,9 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
         } else if (st.status == 5) {
             code = MLP_ERR_MAXIT;
             what = "maximum number of iterations reached";
+        } else if (flex && S.true_rel_res > st.eps) {
+            code = MLP_ERR_NO_CONV;
+            what = "flexible PCG: the FP64 true residual is above eps after the restarts";
         }
         S.status = code;
         if (code && first == MLP_OK) {
@@ -2556,7 +2581,7 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
                 throw Error("No need to iterate", MLP_ERR_NO_ITER);
             }
             if (flex) {
-                for (int restart = 0; restart < 8 && st.status == 0; restart++) {
+                for (uint32_t restart = 0; restart < kFlexRestarts && st.status == 0; restart++) {
                     if (true_rel_res(ctx, J.A, J.d_b, J.d_x, ctx->r, init_norm) <= eps) break;
                     bool again = false;
                     st = pcg_phase(ctx, J.A, J.B, J.d_b, J.d_x, o, init_norm, eps, st.niter, &again);
@@ -2577,6 +2602,14 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
                 throw Error(buf, MLP_ERR_NOT_SPD);
             }
             if (st.status == 5) throw Error("maximum number of iterations reached", MLP_ERR_MAXIT);
+            if (flex && J.stats.true_rel_res > eps) {
+                J.stats.status = MLP_ERR_NO_CONV;
+                char buf[200];
+                std::snprintf(buf, sizeof(buf),
+                              "flexible PCG: FP64 true relative residual %g is above eps %g after %u restarts",
+                              J.stats.true_rel_res, eps, kFlexRestarts);
+                throw Error(buf, MLP_ERR_NO_CONV);
+            }
         } catch (const Error& e) {
             J.status = e.code();
             if (first == MLP_OK) {

@@ -345,7 +345,12 @@ struct ChebBuilder {
                     uint32_t u = kNone;                     // the link's upper-entry slot
                     for (uint32_t p = A.ia[cur] + 1; p < A.ia[cur + 1]; p++)
                         if (present(cur, p)) { nb = A.ja[p]; u = links.slot(cur, p); break; }
-                    gone[u != kNone ? u : skyline_find(A, std::min(cur, nb), std::max(cur, nb))] = 1;
+                    if (u == kNone) {                       // degree and link counters disagree (cheb_mark.cc:45)
+                        std::ostringstream os;
+                        os << "Row " << cur << ": could not find remaining link";
+                        throw Error(os.str());
+                    }
+                    gone[u] = 1;
                     deg[cur] = -1;
                     deg[nb]--;
                     cur = nb;

@@ -20,7 +20,7 @@ struct HostCSR {
     uint32_t nnz() const { return ia.empty() ? 0 : ia.back(); }
 };
 
-// One level of the hierarchy; fields follow Prec::Level (modules/prec/cheb/cheb_prec.h:31-54)
+// One level of the hierarchy; fields follow Prec::Level (modules/prec/cheb/cheb_prec.h:17-40)
 // and MultiSplitPrec::Level (modules/prec/multi_split/multi_split_prec.h:19-44).
 struct HostLevel {
     uint32_t N = 0, nnz = 0, M = 0, Md = 0;

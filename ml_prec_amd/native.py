@@ -17,7 +17,8 @@ MLP_FP64, MLP_FP32 = 0, 1
 UH_CHEB, MULTI_SPLIT, DIAG, ID, GS, COMPOSITE, SYM = 1, 2, 3, 4, 5, 6, 7
 X0_ZERO, X0_GIVEN, X0_REFERENCE = 0, 1, 2
 ORDERS = {"none": 0, "original": 1, "block": 2, "simple_1": 3}
-STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NOT_SPD", 4: "NO_ITER", 5: "MAXIT", 6: "CONSTRUCT", 7: "CONFIG"}
+STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NOT_SPD", 4: "NO_ITER", 5: "MAXIT", 6: "CONSTRUCT", 7: "CONFIG",
+          8: "NO_CONV"}
 
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -59,6 +60,15 @@ class SolveStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LevelDesc(C.Structure):
+    """mlp_level_desc: one level of a hierarchy as plain arrays (Prec::Level / MultiSplitPrec::Level)."""
+    _fields_ = [("N", C.c_uint32), ("M", C.c_uint32), ("Md", C.c_uint32), ("nit", C.c_uint32),
+                ("lmin", C.c_double), ("lmax", C.c_double), ("eps", C.c_double),
+                ("map", _vp), ("L_ia", _vp), ("L_ja", _vp), ("L_a", _vp),
+                ("U_ia", _vp), ("U_ja", _vp), ("U_a", _vp), ("dval", _vp),
+                ("A_ia", _vp), ("A_ja", _vp), ("A_a", _vp)]
 
 
 class PcgJob(C.Structure):
@@ -346,6 +356,32 @@ class DevPrec:
         if getattr(self, "h", None) and _lib is not None and self.ctx.h:
             _lib.mlp_prec_destroy(self.h)
         self.h = None
+
+    @classmethod
+    def upload(cls, ctx, A: DevMatrix, kind, levels, precision=MLP_FP64):
+        """mlp_prec_upload: a uh_cheb or multi_split hierarchy given as per-level dicts with the keys of
+        Hierarchy.level_arrays (map, L_ia, L_ja, L_a, U_ia, U_ja, U_a, dval; A_ia, A_ja, A_a = this level's
+        matrix, unused for level 0) and the scalars N, M, Md, nit, lmin, lmax, eps."""
+        keep = []
+        descs = (LevelDesc * len(levels))()
+        for d, L in zip(descs, levels):
+            d.N, d.M, d.Md, d.nit = int(L["N"]), int(L["M"]), int(L["Md"]), int(L["nit"])
+            d.lmin, d.lmax, d.eps = float(L.get("lmin", 0.0)), float(L.get("lmax", 0.0)), float(L.get("eps", 0.0))
+            for name, dt in (("map", np.uint32), ("L_ia", np.uint32), ("L_ja", np.uint32), ("L_a", np.float64),
+                             ("U_ia", np.uint32), ("U_ja", np.uint32), ("U_a", np.float64), ("dval", np.float64),
+                             ("A_ia", np.uint32), ("A_ja", np.uint32), ("A_a", np.float64)):
+                v = L.get(name)
+                if v is None:
+                    continue
+                arr = np.ascontiguousarray(v, dtype=dt)
+                if arr.size == 0:
+                    arr = np.zeros(1, dtype=dt)
+                keep.append(arr)
+                setattr(d, name, arr.ctypes.data)
+        p = cls.__new__(cls)
+        p.ctx, p.A, p.hier, p.kind, p.precision, p.h = ctx, A, None, kind, precision, _vp()
+        check(lib().mlp_prec_upload(ctx.h, kind, len(levels), descs, A.h, precision, C.byref(p.h)))
+        return p
 
     def clone(self, ctx):
         """A copy on `ctx` sharing this hierarchy's device data, with its own level vectors."""

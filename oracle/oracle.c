@@ -381,16 +381,16 @@ static void construct_sparse_lu(const or_mat* A, const uint32_t* map, const uint
 
     vu32_push(&Lia, 0); vu32_push(&Uia, 0); vu32_push(&Aia, 0);
     for (uint32_t i = 0; i < N - Md; i++) {
-        for (uint32_t k = 0; k < jw.n; k++) jr[jw.p[k]] = -1;       /* :164-168 */
+        for (uint32_t k = 0; k < jw.n; k++) jr[jw.p[k]] = -1;       /* :41-45 */
         jw.n = 0;
         w.n = 0;
         uint32_t arow = map[i];
-        jr[i] = 0;                                                  /* :174-176 */
+        jr[i] = 0;                                                  /* :50-53 */
         sset_insert(&jw, i);
         vf64_push(&w, aux[map[i]]);
         uint32_t max_num = 1;
         lt->row = arow;
-        for (uint32_t j_ = A->ia[arow] + 1; j_ < A->ia[arow + 1]; j_++) {   /* :181-197 */
+        for (uint32_t j_ = A->ia[arow] + 1; j_ < A->ia[arow + 1]; j_++) {   /* :56-74 */
             uint32_t j = A->ja[j_];
             if (lt_stat(lt, j_) == PRESENT) {
                 uint32_t new_j = rmap[j];
@@ -401,7 +401,7 @@ static void construct_sparse_lu(const or_mat* A, const uint32_t* map, const uint
                 w.p[0] -= z;
             }
         }
-        uint32_t m = i < M ? i : M;                                 /* :200-227 */
+        uint32_t m = i < M ? i : M;                                 /* :76-105 */
         uint32_t k = jw.p[0];
         while (k < m) {
             double lik = w.p[jr[k]] / Ua.p[Uia.p[k]];
@@ -420,13 +420,13 @@ static void construct_sparse_lu(const or_mat* A, const uint32_t* map, const uint
             k = jw.p[upper_bound_u32(jw.p, 0, jw.n, k)];
         }
         vu32_push(&Lia, (uint32_t)Lja.n);
-        if (i < M) {                                                /* :231-238 */
+        if (i < M) {                                                /* :108-115 */
             for (uint32_t it = lower_bound_u32(jw.p, 0, jw.n, i); it < jw.n; it++) {
                 vu32_push(&Uja, jw.p[it]);
                 vf64_push(&Ua, w.p[jr[jw.p[it]]]);
             }
             vu32_push(&Uia, (uint32_t)Uja.n);
-        } else {                                                    /* :239-254 */
+        } else {                                                    /* :116-131 */
             vu32_push(&Aja, i - M);
             vf64_push(&Aa, w.p[0]);
             for (uint32_t it = lower_bound_u32(jw.p, 0, jw.n, M); it < jw.n; it++)

@@ -73,7 +73,8 @@ def test_batch_vs_oracle():
         xo, so = O.pcg(s.ia, s.ja, s.a, s.b, P, eps=1e-8, x0=np.zeros(s.n), maxit=5000)
         assert abs(int(st["niter"]) - so["niter"]) <= 1
         xg = x.get()
-        assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo) or abs(int(st["niter"]) - so["niter"]) == 1
+        tol = 1e-10 if st["niter"] == so["niter"] else 1e-6
+        assert np.linalg.norm(xg - xo) <= tol * np.linalg.norm(xo)
     _close(jobs)
 
 

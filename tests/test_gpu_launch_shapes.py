@@ -87,5 +87,6 @@ def test_global_blocks_pcg_graph(ctx, monkeypatch):
     st = N.pcg(ctx, A, B, dev(ctx, b), x, eps=1e-8, maxit=5000, x0_mode=N.X0_ZERO, use_graph=True)
     assert abs(int(st["niter"]) - so["niter"]) <= 1, (st["niter"], so["niter"])
     got = x.get()
-    assert np.linalg.norm(got - xo) <= 1e-10 * np.linalg.norm(xo) or abs(int(st["niter"]) - so["niter"]) == 1
-    assert st["true_rel_res"] <= 1e-7
+    tol = 1e-10 if st["niter"] == so["niter"] else 1e-6
+    assert np.linalg.norm(got - xo) <= tol * np.linalg.norm(xo)
+    assert st["true_rel_res"] <= 1e-8 * (1 + 1e-3)

@@ -45,7 +45,7 @@ def test_multi_equals_single(ctx, name, R, graph):
         assert stats[k]["niter"] == st["niter"], (k, stats[k]["niter"], st["niter"])
         assert np.array_equal(xs[k], x1), k
         assert stats[k]["final_norm"] == st["final_norm"]
-        assert stats[k]["true_rel_res"] <= 1e-7
+        assert stats[k]["true_rel_res"] <= 1e-8 * (1 + 1e-3)
 
 
 def test_multi_mixed_precision_equals_single(ctx):
@@ -95,7 +95,7 @@ def test_multi_per_rhs_outcomes(ctx):
     rc, stats = N.pcg_multi(ctx, A, B, 3, ctx.to_device(bs.ravel()), X, eps=1e-8, maxit=2000)
     assert rc == ERR_NO_ITER
     assert [st["status"] for st in stats] == [0, ERR_NO_ITER, 0]
-    assert stats[0]["true_rel_res"] <= 1e-7 and stats[2]["true_rel_res"] <= 1e-7
+    assert stats[0]["true_rel_res"] <= 1e-8 * (1 + 1e-3) and stats[2]["true_rel_res"] <= 1e-8 * (1 + 1e-3)
     rc, stats = N.pcg_multi(ctx, A, B, 2, ctx.to_device(bs[[0, 2]].ravel()), X, eps=1e-12, maxit=3)
     assert rc == ERR_MAXIT
     assert all(st["status"] == ERR_MAXIT and st["niter"] == 3 for st in stats)

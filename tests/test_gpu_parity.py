@@ -6,8 +6,10 @@ Bars:
   are compiled without FMA contraction);
 * dot / nrm2: relative 1e-13 (a parallel reduction rounds differently from the
   sequential sum);
-* PCG / GMRES in FP64: iteration counts within +-1 of the oracle and solutions
-  within 1e-10 relative (the reductions differ only by rounding);
+* PCG / GMRES in FP64: iteration counts within +-1 of the oracle; solutions
+  within 1e-10 relative when the counts agree and within 1e-6 when they differ
+  by one (the reductions differ only by rounding); the FP64 true residual of
+  the device solution within eps (1 + 1e-3) of the initial residual;
 * mixed precision (FP32 hierarchy, FP64 flexible PCG): FP64 true relative
   residual <= 1e-8 and solution error vs the FP64 oracle <= 1e-6 relative.
 """
@@ -114,6 +116,23 @@ def test_msplit_config_file_apply(ctx, tmp_path):
     assert np.array_equal(x.get(), P.apply(f))
 
 
+def assert_solve_parity(s, xg, sg, xo, so, eps, implicit=False):
+    """Iterations within one; the solution within 1e-10 (same count) or 1e-6 (one apart) of the oracle's; the
+    FP64 true residual, recomputed here, within eps (1 + 1e-3) of the initial residual.  GMRES stops on its
+    implicit residual (gmres.cc:104-110), whose true residual may lie above eps in the oracle too
+    (implicit=True): then the true residual must match the oracle's within 1e-3 relative."""
+    d = abs(int(sg["niter"]) - int(so["niter"]))
+    assert d <= 1, (sg["niter"], so["niter"])
+    tol = 1e-10 if d == 0 else 1e-6
+    assert np.linalg.norm(xg - xo) <= tol * np.linalg.norm(xo), np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    rg = np.linalg.norm(s.b - O.spmv(s.ia, s.ja, s.a, xg))
+    if implicit:
+        ro = np.linalg.norm(s.b - O.spmv(s.ia, s.ja, s.a, xo))
+        assert abs(rg - ro) <= 1e-3 * ro, (rg, ro)
+    else:
+        assert rg <= eps * (1 + 1e-3) * sg["init_norm"], rg / sg["init_norm"]
+
+
 def _pcg_pair(ctx, s, prec="cheb", x0_ref=True, eps=1e-8, use_graph=False):
     if prec == "cheb":
         P = O.cheb_prec(s.ia, s.ja, s.a)
@@ -135,9 +154,7 @@ def _pcg_pair(ctx, s, prec="cheb", x0_ref=True, eps=1e-8, use_graph=False):
 def test_pcg_uh_cheb_vs_oracle(ctx, name, x0_ref):
     s = system(name)
     xo, so, xg, sg = _pcg_pair(ctx, s, "cheb", x0_ref)
-    assert abs(int(sg["niter"]) - so["niter"]) <= 1, (sg["niter"], so["niter"])
-    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo) or abs(int(sg["niter"]) - so["niter"]) == 1
-    assert sg["true_rel_res"] <= 1e-7
+    assert_solve_parity(s, xg, sg, xo, so, 1e-8)
 
 
 def test_pcg_graph_equals_stream(ctx):
@@ -151,8 +168,7 @@ def test_pcg_graph_equals_stream(ctx):
 def test_pcg_multi_split_vs_oracle(ctx):
     s = system("spe10_z12")
     xo, so, xg, sg = _pcg_pair(ctx, s, "msplit", True)
-    assert abs(int(sg["niter"]) - so["niter"]) <= 1
-    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo) or abs(int(sg["niter"]) - so["niter"]) == 1
+    assert_solve_parity(s, xg, sg, xo, so, 1e-8)
 
 
 def test_gmres_vs_oracle(ctx):
@@ -164,9 +180,7 @@ def test_gmres_vs_oracle(ctx):
     B = N.DevPrec(ctx, A, H)
     x = ctx.array(s.n)
     sg = N.gmres(ctx, A, B, dev(ctx, s.b), x, eps=1e-8, maxit=3000, x0_mode=N.X0_REFERENCE)
-    assert abs(int(sg["niter"]) - so["niter"]) <= 1
-    xg = x.get()
-    assert np.linalg.norm(xg - xo) <= 1e-9 * np.linalg.norm(xo)
+    assert_solve_parity(s, x.get(), sg, xo, so, 1e-8, implicit=True)
 
 
 def test_pcg_errors(ctx):
@@ -194,8 +208,7 @@ def test_diag_prec_pcg_vs_oracle(ctx):
     B = N.DevPrec(ctx, A, kind=N.DIAG)
     x = ctx.array(s.n)
     sg = N.pcg(ctx, A, B, dev(ctx, s.b), x, eps=1e-8, maxit=20000)
-    assert abs(int(sg["niter"]) - so["niter"]) <= 1
-    assert np.linalg.norm(x.get() - xo) <= 1e-9 * np.linalg.norm(xo)
+    assert_solve_parity(s, x.get(), sg, xo, so, 1e-8)
 
 
 def test_mixed_precision_pcg(ctx):
@@ -216,9 +229,7 @@ def test_mixed_precision_pcg(ctx):
 def test_full_size_pcg_fp64(ctx):
     s = system("spe10_z85")
     xo, so, xg, sg = _pcg_pair(ctx, s, "cheb", False, use_graph=True)
-    assert abs(int(sg["niter"]) - so["niter"]) <= 1
-    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo) or abs(int(sg["niter"]) - so["niter"]) == 1
-    assert sg["true_rel_res"] <= 1e-8 * 1.5
+    assert_solve_parity(s, xg, sg, xo, so, 1e-8)
 
 
 def _cheb_solver_pair(ctx, s, eps=1e-8, use_graph=True, check_every=0):
@@ -244,7 +255,9 @@ def test_cheb_solver_vs_oracle(ctx, name):
     assert abs(int(sg["niter"]) - so["niter"]) <= 1, (sg["niter"], so["niter"])
     if sg["niter"] == so["niter"]:
         assert np.array_equal(xg, xo)
-    assert sg["true_rel_res"] <= 1e-8 * 1.0001
+    else:
+        assert np.linalg.norm(xg - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert sg["true_rel_res"] <= 1e-8 * (1 + 1e-3)
 
 
 def test_cheb_solver_graph_equals_stream(ctx):
@@ -293,7 +306,9 @@ def test_simple_solver_vs_oracle(ctx, use_graph):
     assert abs(int(sg["niter"]) - so["niter"]) <= 1, (sg["niter"], so["niter"])
     if sg["niter"] == so["niter"]:
         assert np.array_equal(x.get(), xo)
-    assert sg["true_rel_res"] <= 1e-8 * 1.0001
+    else:
+        assert np.linalg.norm(x.get() - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert sg["true_rel_res"] <= 1e-8 * (1 + 1e-3)
 
 
 def test_simple_solver_errors(ctx):
@@ -352,3 +367,21 @@ def test_flexible_pcg_multi_split(ctx, precision):
     assert sg["true_rel_res"] <= 1e-8
     assert np.linalg.norm(x.get() - xo) <= 1e-5 * np.linalg.norm(xo)
     assert sg["niter"] <= so["niter"]
+
+
+def test_flexible_pcg_reports_missed_tolerance(ctx):
+    """Flexible PCG restarts from the FP64 true residual at most 8 times; a tolerance below what FP64 can
+    reach on this matrix (1e-15 relative) leaves the true residual above eps, which must be reported as
+    MLP_ERR_NO_CONV (8) with the stats filled, not as success."""
+    s = system("spe10_z12")
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B = N.DevPrec(ctx, A, N.build_uh_cheb(s.ia, s.ja, s.a), precision=N.MLP_FP32)
+    x = ctx.array(s.n)
+    st = N.pcg(ctx, A, B, dev(ctx, s.b), x, eps=1e-15, maxit=20000, flexible=True, true_res_every=20,
+               raise_on_error=False)
+    assert st["rc"] == 8 and st["status"] == 8, st
+    assert st["true_rel_res"] > st["eps_used"]
+    assert st["true_rel_res"] < 1e-9                     # the solve itself went as far as FP64 allows
+    # the same solve at a reachable tolerance succeeds
+    st = N.pcg(ctx, A, B, dev(ctx, s.b), x, eps=1e-10, maxit=20000, flexible=True, true_res_every=20)
+    assert st["rc"] == 0 and st["true_rel_res"] <= 1e-10
