@@ -220,6 +220,17 @@ int mlp_pcg(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, double* d_x,
             const mlp_solve_opts* opts, mlp_solve_stats* stats);
 int mlp_gmres(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, double* d_x,
               const mlp_solve_opts* opts, mlp_solve_stats* stats);
+/* mlp_pcg for callers that cannot allocate device memory (hot serving loops, code that must not call
+ * hipMalloc): no counterpart in the reference.  It runs mlp_pcg's kernels in mlp_pcg's order, so x, the
+ * iteration count and the stats equal mlp_pcg's, but it allocates nothing: the solve state, the reduction
+ * partial sums and the five PCG vectors live in one static device array per GPU (47.4 MB, held from the
+ * library's load until the process ends) and the iterations are plain launches (no graph is instantiated;
+ * opts->use_graph is ignored).  Limits: n <= 1,179,648 rows (MLP_ERR_INVALID beyond); x0_mode ZERO or
+ * GIVEN; preconditioners whose apply synchronises with the host (SymPrec, dynamic multi_split) are refused.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (two host threads, or two
+ * streams) race and must be serialised by the caller. */
+int mlp_pcg_noalloc(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, double* d_x,
+                    const mlp_solve_opts* opts, mlp_solve_stats* stats);
 /* Ensembles (no counterpart in the reference, which solves one system per run): `count` independent
  * PCGSolver calls (pcg.cc:8-102) run concurrently.  Every job needs its own context (stream, workspace,
  * device state) holding its matrix and preconditioner.  Each job's x and stats are bitwise those of a

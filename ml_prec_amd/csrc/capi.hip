@@ -1964,19 +1964,22 @@ void fill_x0(mlp_ctx ctx, uint32_t n, int mode, double* x) {
     MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
-SolveState read_state(mlp_ctx ctx) {
+// the device state `st` through the context's pinned buffer
+SolveState read_state(mlp_ctx ctx, const SolveState* st) {
     SolveState s;
-    MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, ctx->st, sizeof(SolveState), hipMemcpyDeviceToHost, ctx->stream));
+    MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, st, sizeof(SolveState), hipMemcpyDeviceToHost, ctx->stream));
     MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     std::memcpy(&s, ctx->host_pinned, sizeof(SolveState));
     return s;
 }
+SolveState read_state(mlp_ctx ctx) { return read_state(ctx, ctx->st); }
 
-void write_state(mlp_ctx ctx, const SolveState& s) {
+void write_state(mlp_ctx ctx, SolveState* st, const SolveState& s) {
     std::memcpy(ctx->host_pinned, &s, sizeof(SolveState));
-    MLP_HIP_CHECK(hipMemcpyAsync(ctx->st, ctx->host_pinned, sizeof(SolveState), hipMemcpyHostToDevice, ctx->stream));
+    MLP_HIP_CHECK(hipMemcpyAsync(st, ctx->host_pinned, sizeof(SolveState), hipMemcpyHostToDevice, ctx->stream));
     MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
+void write_state(mlp_ctx ctx, const SolveState& s) { write_state(ctx, ctx->st, s); }
 
 void launch_residual(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, double* r) {
     hipLaunchKernelGGL((k_residual<double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream, A->A.v, b, x, r,
@@ -2007,6 +2010,7 @@ struct PcgVecs {
     double *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *zp = nullptr;
     SolveState* st = nullptr;
     double* partials = nullptr;          // kPartials doubles per solve
+    double* slot = nullptr;              // result of a standalone dot (R = 1)
     Batch bt;
 };
 
@@ -2015,9 +2019,28 @@ PcgVecs ctx_vecs(mlp_ctx ctx, uint32_t n) {
     V.r = ctx->r; V.z = ctx->z; V.p = ctx->p; V.Ap = ctx->Ap; V.zp = ctx->zp;
     V.st = ctx->st;
     V.partials = ctx->partials;
+    V.slot = ctx->scal;
     V.bt.n = n;
     V.bt.part = uint32_t(kPartials);
     return V;
+}
+
+// (x, y) with the partial sums and result slot of workspace V (dev_dot's kernels and grid)
+double dev_dot(mlp_ctx ctx, uint32_t n, const double* x, const double* y, const PcgVecs& V) {
+    const uint32_t g = red_grid(n);
+    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBS), 0, ctx->stream, n, x, y, V.partials);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, ctx->stream, int(FIN_DOT), g, (const double*)V.partials,
+                       V.st, V.slot, Batch{});
+    MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, V.slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return ctx->host_pinned[0];
+}
+
+// ||b - A x|| / init_norm, the residual left in V.r
+double true_rel_res(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, const PcgVecs& V, double init_norm) {
+    launch_residual(ctx, A, b, x, V.r);
+    const double nr = std::sqrt(dev_dot(ctx, A->n, V.r, V.r, V));
+    return init_norm > 0 ? nr / init_norm : nr;
 }
 
 // One PCG iteration (pcg.cc:46-76) on the context's stream for the solves of
@@ -2175,11 +2198,11 @@ mlp_ctx_s::GraphEntry* multi_graph(mlp_ctx ctx, mlp_mat A, mlp_prec B, DevPrecBa
 
 // iterations k0+1 .. k0+chunk (graph replays or direct launches)
 void enqueue_pcg_chunk(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, bool flex, uint32_t tre,
-                       mlp_ctx_s::GraphEntry* ge, uint32_t k0, uint32_t chunk) {
+                       mlp_ctx_s::GraphEntry* ge, uint32_t k0, uint32_t chunk, const PcgVecs& V) {
     for (uint32_t k = k0 + 1; k <= k0 + chunk; k++) {
         const bool replace = tre && (k % tre == 0);
         if (ge) MLP_HIP_CHECK(hipGraphLaunch(ge->e[replace ? 1 : 0], ctx->stream));
-        else enqueue_pcg_iteration(ctx, A, B, b, x, flex, replace);
+        else enqueue_pcg_iteration(ctx, A, B->impl.get(), b, x, flex, replace, V);
     }
 }
 
@@ -2199,13 +2222,15 @@ uint32_t next_chunk(const SolveState& st, uint32_t check_every) {
 }
 
 // One PCG phase from the current x: r = b - A x, z = B r, p = z, then
-// iterations until the device sets `done` (pcg.cc:24-76).
+// iterations until the device sets `done` (pcg.cc:24-76).  W: a workspace
+// other than the context's (the iterations then run as plain launches).
 SolveState pcg_phase(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, const mlp_solve_opts* o,
-                     double init_norm, double eps, uint32_t niter0, bool* no_iter) {
+                     double init_norm, double eps, uint32_t niter0, bool* no_iter, const PcgVecs* W = nullptr) {
     const uint32_t n = A->n;
     const bool flex = o->flexible != 0;
-    launch_residual(ctx, A, b, x, ctx->r);
-    const double norm = std::sqrt(dev_dot(ctx, n, ctx->r, ctx->r));
+    const PcgVecs V = W ? *W : ctx_vecs(ctx, n);
+    launch_residual(ctx, A, b, x, V.r);
+    const double norm = std::sqrt(dev_dot(ctx, n, V.r, V.r, V));
     SolveState st0;
     std::memset(&st0, 0, sizeof(st0));
     st0.norm = norm;
@@ -2213,20 +2238,21 @@ SolveState pcg_phase(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double
     st0.eps = eps;
     st0.maxit = o->maxit;
     st0.niter = niter0;
-    write_state(ctx, st0);
+    write_state(ctx, V.st, st0);
     *no_iter = !(norm / init_norm > eps);
     if (*no_iter) return st0;
-    enqueue_pcg_start(ctx, B, n, flex);
+    enqueue_pcg_start(ctx, B->impl.get(), n, flex, V);
     const uint32_t chunk = o->check_every ? o->check_every : 16;
     const uint32_t tre = o->true_res_every;
-    mlp_ctx_s::GraphEntry* ge = pcg_graphs(ctx, A, B, b, x, flex, tre, o->use_graph && B->impl->graph_safe());
+    mlp_ctx_s::GraphEntry* ge =
+        W ? nullptr : pcg_graphs(ctx, A, B, b, x, flex, tre, o->use_graph && B->impl->graph_safe());
     SolveState st;
     uint32_t k = 0, c = chunk;
     while (true) {
-        enqueue_pcg_chunk(ctx, A, B, b, x, flex, tre, ge, k, c);
+        enqueue_pcg_chunk(ctx, A, B, b, x, flex, tre, ge, k, c, V);
         k += c;
         MLP_HIP_CHECK(hipGetLastError());
-        st = read_state(ctx);
+        st = read_state(ctx, V.st);
         if (st.done) break;
         c = next_chunk(st, chunk);
     }
@@ -2539,7 +2565,7 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
     auto enqueue = [&](Run& R) {
         mlp_ctx ctx = R.j->ctx;
         const uint32_t c = R.c ? R.c : chunk;
-        enqueue_pcg_chunk(ctx, R.j->A, R.j->B, R.j->d_b, R.j->d_x, flex, tre, R.ge, R.k, c);
+        enqueue_pcg_chunk(ctx, R.j->A, R.j->B, R.j->d_b, R.j->d_x, flex, tre, R.ge, R.k, c, ctx_vecs(ctx, R.j->A->n));
         R.k += c;
         MLP_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, ctx->st, sizeof(SolveState), hipMemcpyDeviceToHost,
                                      ctx->stream));
@@ -2882,6 +2908,161 @@ int mlp_simple_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, doubl
         if (stats) stats->status = MLP_ERR_NO_ITER;
         throw Error("No need to iterate", MLP_ERR_NO_ITER);
     }
+    MLP_API_END
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// allocation-free PCG (no counterpart in the reference): mlp_pcg with its
+// workspace in a static device array instead of context allocations
+// ---------------------------------------------------------------------------
+namespace mlp {
+namespace {
+constexpr uint32_t kNoAllocRows = (1u << 20) + (1u << 17);     // 1,179,648 rows
+constexpr size_t kNoAllocHead = 32;                            // doubles (256 bytes) per small slot
+constexpr size_t kNoAllocDoubles = kNoAllocHead               // SolveState
+                                   + kPartials                // reduction partial sums
+                                   + kNoAllocHead             // result of a standalone dot
+                                   + 5 * size_t(kNoAllocRows); // r, z, p, Ap, zp
+static_assert(sizeof(SolveState) <= kNoAllocHead * sizeof(double), "solve state slot too small");
+static_assert((kNoAllocRows * sizeof(double)) % 256 == 0, "vectors keep 256-byte alignment");
+}  // namespace
+}  // namespace mlp
+
+// The workspace of mlp_pcg_noalloc: 47.4 MB per GPU, held from the library's
+// load until the process ends.  Host code never uses this name as an address:
+// it takes the device address from hipGetSymbolAddress once per device.
+static __device__ __align__(256) double scratch[mlp::kNoAllocDoubles];
+
+namespace mlp {
+namespace {
+struct NoAllocDevice {
+    std::once_flag once;
+    hipError_t rc = hipSuccess;
+    double* ptr = nullptr;
+};
+NoAllocDevice g_noalloc[kMaxStageDevices];
+
+// The device address of `scratch` on `device` (which the caller has made
+// current), looked up on the first call for that device and kept.
+double* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxStageDevices) throw Error("device index out of range", MLP_ERR_INVALID);
+    NoAllocDevice& d = g_noalloc[device];
+    std::call_once(d.once, [&d] { d.rc = hipGetSymbolAddress((void**)&d.ptr, HIP_SYMBOL(scratch)); });
+    if (d.rc != hipSuccess || !d.ptr)
+        throw Error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(d.rc), MLP_ERR_HIP);
+    return d.ptr;
+}
+
+// The PCG workspace carved out of the device's scratch for an n-row solve.
+PcgVecs noalloc_vecs(double* s, uint32_t n) {
+    PcgVecs V;
+    V.st = reinterpret_cast<SolveState*>(s);
+    V.partials = s + kNoAllocHead;
+    V.slot = V.partials + kPartials;
+    double* v = V.slot + kNoAllocHead;
+    V.r = v;
+    V.z = v + kNoAllocRows;
+    V.p = v + 2 * size_t(kNoAllocRows);
+    V.Ap = v + 3 * size_t(kNoAllocRows);
+    V.zp = v + 4 * size_t(kNoAllocRows);
+    V.bt.n = n;
+    V.bt.part = uint32_t(kPartials);
+    return V;
+}
+}  // namespace
+}  // namespace mlp
+
+extern "C" {
+
+// mlp_pcg without allocations: the same kernels in the same order as mlp_pcg
+// (so the same x, iteration count and stats), as plain launches (no graph is
+// instantiated), with the solve state, partial sums and vectors in `scratch`.
+int mlp_pcg_noalloc(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, const mlp_solve_opts* o,
+                    mlp_solve_stats* stats) {
+    MLP_API_BEGIN
+    require(ctx && A && B && b && x && o, "null argument");
+    require(B->n == A->n, "preconditioner size differs from A");
+    require(B->ctx == ctx && A->ctx->device == ctx->device, "the matrix and preconditioner must live on ctx's device");
+    if (A->n > kNoAllocRows)
+        throw Error("mlp_pcg_noalloc: " + std::to_string(A->n) + " rows exceed its static workspace (at most " +
+                        std::to_string(kNoAllocRows) + ")",
+                    MLP_ERR_INVALID);
+    require(o->x0_mode == MLP_X0_ZERO || o->x0_mode == MLP_X0_GIVEN,
+            "mlp_pcg_noalloc: x0_mode must be MLP_X0_ZERO or MLP_X0_GIVEN");
+    require(B->impl->graph_safe(),
+            "mlp_pcg_noalloc: this preconditioner's apply synchronises with the host (SymPrec, dynamic multi_split)");
+    use_device(ctx);
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = A->n, rg = red_grid(n);
+    const PcgVecs V = noalloc_vecs(noalloc_scratch(ctx->device), n);
+    hipStream_t s = ctx->stream;
+    const bool flex = o->flexible != 0;
+    const uint32_t chunk = o->check_every ? o->check_every : 16;
+    const uint32_t tre = o->true_res_every;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    fill_x0(ctx, n, o->x0_mode, x);
+    SolveState st0;                                   // every field of the shared state starts from zero
+    std::memset(&st0, 0, sizeof(st0));
+    st0.eps = o->eps;
+    st0.maxit = o->maxit;
+    write_state(ctx, V.st, st0);
+    launch_residual(ctx, A, b, x, V.r);
+    hipLaunchKernelGGL(k_dot, dim3(rg), dim3(kBS), 0, s, n, (const double*)V.r, (const double*)V.r, V.partials);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, int(FIN_START), rg, (const double*)V.partials, V.st,
+                       (double*)nullptr, Batch{});
+    enqueue_pcg_start(ctx, B->impl.get(), n, flex, V);
+    SolveState st;
+    uint32_t k = 0, c = chunk;
+    while (true) {
+        enqueue_pcg_chunk(ctx, A, B, b, x, flex, tre, nullptr, k, c, V);
+        k += c;
+        MLP_HIP_CHECK(hipGetLastError());
+        st = read_state(ctx, V.st);
+        if (st.done) break;
+        c = next_chunk(st, chunk);
+    }
+    const double init_norm = st.init_norm, eps = st.eps;
+    int code = MLP_OK;
+    std::string msg;
+    if (st.status == 4) {
+        code = MLP_ERR_NO_ITER;
+        msg = "No need to iterate";
+    } else {
+        if (flex) {
+            for (uint32_t restart = 0; restart < kFlexRestarts && st.status == 0; restart++) {
+                if (true_rel_res(ctx, A, b, x, V, init_norm) <= eps) break;
+                bool again = false;
+                st = pcg_phase(ctx, A, B, b, x, o, init_norm, eps, st.niter, &again, &V);
+                if (again) break;
+            }
+        }
+        MLP_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    const double trr = code == MLP_OK ? true_rel_res(ctx, A, b, x, V, init_norm) : 0.0;
+    if (code == MLP_OK) {
+        if (st.status == 3) {
+            code = MLP_ERR_NOT_SPD;
+            msg = "<Ap, p> = " + std::to_string(st.app);
+        } else if (st.status == 5) {
+            code = MLP_ERR_MAXIT;
+            msg = "maximum number of iterations reached";
+        } else if (flex && trr > eps) {
+            code = MLP_ERR_NO_CONV;
+            msg = "flexible PCG: the FP64 true residual is above eps after the restarts";
+        }
+    }
+    if (stats) {
+        stats->niter = st.niter;
+        stats->init_norm = init_norm;
+        stats->final_norm = st.norm;
+        stats->eps_used = eps;
+        stats->true_rel_res = trr;
+        stats->t_solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->status = code;
+    }
+    if (code != MLP_OK) throw Error(msg, code);
     MLP_API_END
 }
 

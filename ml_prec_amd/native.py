@@ -117,6 +117,7 @@ EXPORTS = {
     "mlp_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mlp_pcg": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "mlp_gmres": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "mlp_pcg_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "mlp_spe_adjoint_grad": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                        C.c_double, _vp, _vp, _vp, _vp]),
     "mlp_pcg_batch": (C.c_int, [C.c_uint32, C.POINTER(PcgJob), C.POINTER(SolveOpts)]),
@@ -418,6 +419,19 @@ def pcg(ctx, A, B, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO,
     o = _opts(eps, maxit, x0_mode, flexible, true_res_every, check_every, use_graph)
     st = SolveStats()
     rc = lib().mlp_pcg(ctx.h, A.h, B.h, b.ptr, x.ptr, C.byref(o), C.byref(st))
+    d = st.as_dict()
+    d["rc"] = rc
+    if rc and raise_on_error:
+        raise MlpError(rc, lib().mlp_last_error().decode())
+    return d
+
+
+def pcg_noalloc(ctx, A, B, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO, flexible=False,
+                true_res_every=0, check_every=16, raise_on_error=True):
+    """mlp_pcg_noalloc: PCGSolver with its workspace in the library's static device array."""
+    o = _opts(eps, maxit, x0_mode, flexible, true_res_every, check_every, False)
+    st = SolveStats()
+    rc = lib().mlp_pcg_noalloc(ctx.h, A.h, B.h, b.ptr, x.ptr, C.byref(o), C.byref(st))
     d = st.as_dict()
     d["rc"] = rc
     if rc and raise_on_error:
