@@ -220,7 +220,11 @@ template <class VT> struct DevLevel {
     VT* r = nullptr;                           // multi_split inner residual
     // bookkeeping for the byte model
     uint32_t nnzL = 0, nnzU = 0;
+    // the level run fused with the coarsest level (two launches per visit)
+    bool fused = false;
+    PairView<VT> pair;
 };
+static_assert(kPairHdr == kPairFields, "pair block header size");
 
 struct DevPrecBase {
     virtual ~DevPrecBase() {}
@@ -367,6 +371,7 @@ template <class VT> struct DevHier : DevPrecBase {
     mlp_mat A0 = nullptr;
     mutable uint32_t launches = 0;
     bool dynamic = false;
+    int pair_l = -1;                         // level fused with the coarsest level (-1: none)
 
     hipStream_t stream() const { return ctx->stream; }
 
@@ -445,6 +450,56 @@ template <class VT> struct DevHier : DevPrecBase {
         return kLBS;
     }
 
+    // the device arrays of the fused pair program on level L (see plan.h: PairProgram)
+    void upload_pair(uint32_t l, const mlp_level_desc& D, const PairProgram& PP, uint32_t pbs, DevLevel<VT>& L) {
+        L.fused = true;
+        L.lbs = pbs;
+        L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
+        L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
+        PairView<VT>& V = L.pair;
+        V.nblk = PP.nblk; V.nodes = PP.nodes; V.nodes2 = PP.nodes2; V.nflat2 = PP.nflat2; V.Md = D.Md;
+        Stager up(ctx, *data);
+        std::vector<double> r1, r2;
+        up.add(D.map, D.N, L.map);
+        up.add(PP.blk, V.blk);
+        up.add(PP.qrow, V.qrow);
+        up.add(PP.Ldep, V.Ldep); up.add(PP.Lptr, V.Lptr); up.add(PP.Lloc, V.Lloc); up.add(PP.kept2, V.kept2);
+        up.add(PP.Lval, V.Lval);
+        up.add(PP.Udep, V.Udep); up.add(PP.uloc, V.uloc); up.add(PP.uxidx, V.uxidx); up.add(PP.Uptr, V.Uptr);
+        up.add(PP.Ucol, V.Ucol); up.add(PP.Uval, V.Uval);
+        up.add(PP.src2, V.src2); up.add(PP.row2, V.row2); up.add(PP.xl2, V.xl2); up.add(PP.Ldep2, V.Ldep2);
+        up.add(PP.Lptr2, V.Lptr2); up.add(PP.Lloc2, V.Lloc2); up.add(PP.Lval2, V.Lval2); up.add(PP.d2, V.d2);
+        up.add(PP.Udep2, V.Udep2); up.add(PP.uloc2, V.uloc2); up.add(PP.Uptr2, V.Uptr2); up.add(PP.Ucol2, V.Ucol2);
+        up.add(PP.Uval2, V.Uval2);
+        up.add(PP.frow2, V.frow2); up.add(PP.fxl, V.fxl); up.add(PP.fd2, V.fd2);
+        if constexpr (std::is_same_v<VT, float>) {             // udiv: FP32 multiplies by 1 / u_ii
+            r1.resize(PP.Udiag.size());
+            r2.resize(PP.Udiag2.size());
+            for (size_t i = 0; i < r1.size(); i++) r1[i] = 1.0 / PP.Udiag[i];
+            for (size_t i = 0; i < r2.size(); i++) r2[i] = 1.0 / PP.Udiag2[i];
+            up.add(r1, V.Udiag);
+            up.add(r2, V.Udiag2);
+        } else {
+            up.add(PP.Udiag, V.Udiag);
+            up.add(PP.Udiag2, V.Udiag2);
+        }
+        up.add(D.dval, D.Md, L.dval);
+        up.flush();
+        V.dval = L.dval;
+        // clones size the per-instance w of the block nodes from prog.nodes
+        L.prog = ProgView<VT>();
+        L.prog.nodes = PP.nodes;
+        L.wt = scratch.alloc<VT>(std::max<uint32_t>(PP.nodes, 1));
+        // level l's own matrix (l >= 1), read by its parent's fused Chebyshev residual in its consumption order
+        if (l > 0) upload_sell<VT>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A, PP.qrow.data());
+        if (L.n) {
+            L.F = scratch.alloc<VT>(L.n);
+            L.u1 = scratch.alloc<VT>(L.n);
+            L.u0 = scratch.alloc<VT>(L.n);
+            L.x2 = scratch.alloc<VT>(L.n);
+        }
+    }
+
     void build(const std::vector<mlp_level_desc>& d, int kind_, mlp_mat A0_) {
         kind = kind_;
         A0 = A0_;
@@ -480,8 +535,31 @@ template <class VT> struct DevHier : DevPrecBase {
         });
         for (auto& e : err)
             if (e) std::rethrow_exception(e);
+        // the last two uh_cheb levels run fused when every merged component fits a block
+        // (MLP_NO_PAIR=1 keeps them separate)
+        PairProgram PP;
+        uint32_t pbs = 0;
+        const char* nopair = getenv("MLP_NO_PAIR");
+        if (kind == MLP_PREC_UH_CHEB && nlevels >= 2 && d[nlevels - 2].nit == 2 && !(nopair && nopair[0] == '1')) {
+            auto arrays = [&](uint32_t l) {
+                const mlp_level_desc& D = d[l];
+                LevelArrays la;
+                la.N = D.N; la.M = D.M; la.Md = D.Md; la.map = D.map;
+                la.L_ia = D.L_ia; la.L_ja = D.L_ja; la.L_a = D.L_a;
+                la.U_ia = D.U_ia; la.U_ja = D.U_ja; la.U_a = D.U_a; la.dval = D.dval;
+                return la;
+            };
+            const LevelArrays la = arrays(nlevels - 2), lb = arrays(nlevels - 1);
+            if (plan_pair(la, lb, 64, PP)) pbs = 64;
+            else if (plan_pair(la, lb, kLBS, PP)) pbs = kLBS;
+            if (pbs) pair_l = int(nlevels - 2);
+        }
         if (kind == MLP_PREC_UH_CHEB)
-            for (uint32_t l = 0; l + 1 < nlevels; l++) link_levels(progs[l], progs[l + 1]);
+            for (uint32_t l = 0; l + 1 < nlevels; l++) {
+                if (int(l) == pair_l) continue;
+                if (int(l + 1) == pair_l) link_levels(progs[l], PP.qrow);
+                else link_levels(progs[l], progs[l + 1]);
+            }
         for (uint32_t l = 0; l < nlevels; l++) {
             const mlp_level_desc& D = d[l];
             const LevelProgram& P = progs[l];
@@ -489,6 +567,18 @@ template <class VT> struct DevHier : DevPrecBase {
             L.N = D.N; L.M = D.M; L.Md = D.Md; L.nit = D.nit; L.eps = D.eps;
             L.n = L.N - L.M - L.Md;
             if (L.eps != 0.0) dynamic = true;
+            if (int(l) == pair_l) {
+                upload_pair(l, D, PP, pbs, L);
+                continue;
+            }
+            if (pair_l >= 0 && int(l) == pair_l + 1) {
+                // the coarsest level runs inside the pair kernels; only its matrix is read here,
+                // by step 2's residual, rows in the pair's level-(l+1) consumption order
+                upload_sell<VT>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A, PP.qrow2.data());
+                L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
+                L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
+                continue;
+            }
             Stager up(ctx, *data);
             std::vector<double> urcp;
             up.add(D.map, D.N, L.map);
@@ -591,6 +681,7 @@ template <class VT> struct DevHier : DevPrecBase {
         h->data = data;
         h->A0 = A0;
         h->dynamic = dynamic;
+        h->pair_l = pair_l;
         h->env_lbs = env_lbs;
         h->env_cap = env_cap;
         h->lds_budget = lds_budget;
@@ -692,9 +783,39 @@ template <class VT> struct DevHier : DevPrecBase {
     // Prec::solve(level, f, x)  (modules/prec/cheb/cheb_solve.cc:13-123).  The
     // Chebyshev residual tmp = A_{l+1} u - F of steps >= 2 is not stored: the
     // visit of level l+1 computes each row where it reads it.
+    // a visit of the level fused with the coarsest level: K1 and K2 (kernels.h: k_pair_down, k_pair_up)
+    template <class FT, int IN, class XT>
+    void pair_visit(const DevLevel<VT>& L, const DevLevel<VT>& Lc, const InSrc<VT, FT, IN>& f, XT* x,
+                    const uint32_t* g) const {
+        const PairView<VT>& P = L.pair;
+        const LevelBatch lb = level_batch(L);
+        const uint32_t g1 = P.nblk + cdiv(uint64_t(P.nflat2) + P.Md, L.lbs);
+        auto k1 = L.lbs == 64 ? (R > 1 ? k_pair_down<VT, FT, IN, XT, 64, true> : k_pair_down<VT, FT, IN, XT, 64, false>)
+                              : (R > 1 ? k_pair_down<VT, FT, IN, XT, kLBS, true>
+                                       : k_pair_down<VT, FT, IN, XT, kLBS, false>);
+        launch(k1, g1, L.lbs, 0, P, f, L.wt, L.F, L.u1, x, g, lb);
+        const uint32_t g2 = P.nblk + cdiv(P.nflat2, L.lbs);
+        const VT* wt = L.wt;
+        if (L.s1_skip) {
+            auto k2 = L.lbs == 64 ? (R > 1 ? k_pair_up<VT, XT, false, 64, true> : k_pair_up<VT, XT, false, 64, false>)
+                                  : (R > 1 ? k_pair_up<VT, XT, false, kLBS, true> : k_pair_up<VT, XT, false, kLBS, false>);
+            launch(k2, g2, L.lbs, 0, P, Lc.A.v, (const VT*)L.u1, VT(1), (const VT*)L.F, wt, VT(L.ma2), L.s2_skip,
+                   VT(L.opb2), L.opb2_one, x, g, lb);
+        } else {
+            auto k2 = L.lbs == 64 ? (R > 1 ? k_pair_up<VT, XT, true, 64, true> : k_pair_up<VT, XT, true, 64, false>)
+                                  : (R > 1 ? k_pair_up<VT, XT, true, kLBS, true> : k_pair_up<VT, XT, true, kLBS, false>);
+            launch(k2, g2, L.lbs, 0, P, Lc.A.v, (const VT*)L.u1, VT(L.a1), (const VT*)L.F, wt, VT(L.ma2),
+                   L.s2_skip, VT(L.opb2), L.opb2_one, x, g, lb);
+        }
+    }
+
     template <class FT, int IN, class XT>
     void cheb(uint32_t l, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
         const DevLevel<VT>& L = lv[l];
+        if (int(l) == pair_l) {
+            pair_visit(L, lv[l + 1], f, x, g);
+            return;
+        }
         if (l + 1 == nlevels) {
             solve_last<FT, IN, XT, 0>(L, f, x, g);
             return;

@@ -822,6 +822,252 @@ __global__ void __launch_bounds__(BS) k_level_solve(ProgView<VT> P, InSrc<VT, FT
     }
 }
 
+// ---------------------------------------------------------------------------
+// fused last pair: a uh_cheb level l and the coarsest level l+1 (plan.h:
+// PairProgram).  Two launches per visit of level l with two Chebyshev steps
+// (cheb_solve.cc:33-104 on level l, :108-122 on level l+1); every row keeps
+// the arithmetic of k_level_down / k_level_solve / k_level_up.
+// ---------------------------------------------------------------------------
+template <class VT> struct PairView {
+    uint32_t nblk = 0, nodes = 0, nodes2 = 0, nflat2 = 0, Md = 0;
+    const uint32_t* blk = nullptr;
+    const uint32_t* qrow = nullptr;
+    const uint32_t *Ldep = nullptr, *Lptr = nullptr, *Lloc = nullptr, *kept2 = nullptr;
+    const VT* Lval = nullptr;
+    const uint32_t *Udep = nullptr, *uloc = nullptr, *uxidx = nullptr, *Uptr = nullptr, *Ucol = nullptr;
+    const VT *Uval = nullptr, *Udiag = nullptr;
+    const uint32_t *src2 = nullptr, *row2 = nullptr, *xl2 = nullptr, *Ldep2 = nullptr, *Lptr2 = nullptr,
+                   *Lloc2 = nullptr;
+    const VT *Lval2 = nullptr, *d2 = nullptr;
+    const uint32_t *Udep2 = nullptr, *uloc2 = nullptr, *Uptr2 = nullptr, *Ucol2 = nullptr;
+    const VT *Uval2 = nullptr, *Udiag2 = nullptr;
+    const uint32_t *frow2 = nullptr, *fxl = nullptr;
+    const VT* fd2 = nullptr;
+    const VT* dval = nullptr;          // level l's isolated rows
+};
+constexpr uint32_t kPairHdr = 12;          // header words per block of a pair program
+constexpr uint32_t kPairIso2 = 0x80000000u;
+constexpr uint32_t kPairNoRef = 0xffffffffu;
+
+// Owned-node L sweep (as owned_down) of n nodes at base with input acc:
+// thread p owns node p; sw holds every node's w afterwards.
+template <int BS, class VT>
+__device__ __forceinline__ VT pair_L(uint32_t n, uint32_t nl, uint32_t base, const uint32_t* __restrict__ Ldep,
+                                     const uint32_t* __restrict__ Lptr, const uint32_t* __restrict__ Lloc,
+                                     const VT* __restrict__ Lval, VT acc, VT* sw) {
+    const uint32_t p = threadIdx.x;
+    uint32_t my = nl, q0 = 0, q1 = 0, l0 = p, l1 = p;
+    VT v0 = VT(0), v1 = VT(0);
+    if (p < n) {
+        my = Ldep[base + p];
+        q0 = Lptr[base + p];
+        q1 = Lptr[base + p + 1];
+        if (q1 > q0) { l0 = Lloc[q0]; v0 = Lval[q0]; }
+        if (q1 > q0 + 1) { l1 = Lloc[q0 + 1]; v1 = Lval[q0 + 1]; }
+    }
+    const bool h0 = q1 > q0, h1 = q1 > q0 + 1, more = q1 > q0 + 2;
+    for (uint32_t li = 0; li < nl; li++) {
+        if (li == my) {
+            const VT x0 = sw[l0], x1 = sw[l1];
+            acc = h0 ? msub(acc, v0, x0) : acc;
+            acc = h1 ? msub(acc, v1, x1) : acc;
+            if (more)
+                for (uint32_t q = q0 + 2; q < q1; q++) acc = msub(acc, Lval[q], sw[Lloc[q]]);
+            sw[p] = acc;
+        }
+        level_sync<BS>();
+    }
+    return acc;
+}
+
+// Owned-node U sweep (as owned_up) of the U-list entries [ub, ub + nu):
+// w of a row from zsrc[zbase + loc], columns local (into sx) or, flagged
+// kGlobalBit, kept values kept[c]; solved values land in sx[loc].
+// KEPT = false: every column is local (kept is not read).
+template <int BS, class VT, bool KEPT>
+__device__ __forceinline__ VT pair_U(uint32_t nu, uint32_t nl, uint32_t ub, const uint32_t* __restrict__ Udep,
+                                     const uint32_t* __restrict__ uloc, const VT* __restrict__ Udiag,
+                                     const uint32_t* __restrict__ Uptr, const uint32_t* __restrict__ Ucol,
+                                     const VT* __restrict__ Uval, const VT* zsrc, uint32_t zbase, const VT* kept,
+                                     VT* sx) {
+    const uint32_t u = threadIdx.x;
+    uint32_t my = nl, q0 = 0, q1 = 0, c0 = kGlobalBit, c1 = kGlobalBit, loc = 0;
+    VT z = VT(0), d = VT(1), v0 = VT(0), v1 = VT(0), r0 = VT(0), r1 = VT(0);
+    if (u < nu) {
+        my = Udep[ub + u];
+        loc = uloc[ub + u];
+        d = Udiag[ub + u];
+        q0 = Uptr[ub + u];
+        q1 = Uptr[ub + u + 1];
+        z = zsrc[zbase + loc];
+        if (q1 > q0) {
+            c0 = Ucol[q0];
+            v0 = Uval[q0];
+            if (KEPT && (c0 & kGlobalBit)) r0 = kept[c0 & ~kGlobalBit];
+        }
+        if (q1 > q0 + 1) {
+            c1 = Ucol[q0 + 1];
+            v1 = Uval[q0 + 1];
+            if (KEPT && (c1 & kGlobalBit)) r1 = kept[c1 & ~kGlobalBit];
+        }
+    }
+    const bool h0 = q1 > q0, h1 = q1 > q0 + 1, more = q1 > q0 + 2;
+    const bool g0 = KEPT && (c0 & kGlobalBit) != 0, g1 = KEPT && (c1 & kGlobalBit) != 0;
+    const uint32_t a0 = (g0 || !h0) ? loc : c0, a1 = (g1 || !h1) ? loc : c1;
+    for (uint32_t li = 0; li < nl; li++) {
+        if (li == my) {
+            const VT s0 = sx[a0], s1 = sx[a1];
+            const VT x0 = g0 ? r0 : s0, x1 = g1 ? r1 : s1;
+            z = h0 ? msub(z, v0, x0) : z;
+            z = h1 ? msub(z, v1, x1) : z;
+            if (more)
+                for (uint32_t q = q0 + 2; q < q1; q++) {
+                    const uint32_t c = Ucol[q];
+                    VT xv;
+                    if (KEPT && (c & kGlobalBit)) xv = kept[c & ~kGlobalBit];
+                    else xv = sx[c];
+                    z = msub(z, Uval[q], xv);
+                }
+            z = udiv(z, d);
+            sx[loc] = z;
+        }
+        level_sync<BS>();
+    }
+    return z;
+}
+
+// K1: level l's L sweep, then step 1 on level l+1 for the kept rows of the
+// block: writes w of level l's eliminated rows (wt), F of every kept row (at
+// its level-(l+1) consumption position), u1 = B_{l+1} F (level-(l+1) rows)
+// and x = dval f of level l's isolated rows.
+template <class VT, class FT, int IN, class XT, int BS, bool BAT>
+__global__ void __launch_bounds__(BS) k_pair_down(PairView<VT> P, InSrc<VT, FT, IN> f, VT* __restrict__ wt,
+                                                    VT* __restrict__ F, VT* __restrict__ u1, XT* __restrict__ x,
+                                                    const uint32_t* guard, LevelBatch lb) {
+    if constexpr (BAT) {
+        const size_t y = blockIdx.y;
+        f.f = shift(f.f, y * lb.N); f.u = shift(f.u, y * lb.N); f.F = shift(f.F, y * lb.N); f.us = shift(f.us, y * lb.N);
+        wt = shift(wt, y * lb.nodes); F = shift(F, y * lb.n); u1 = shift(u1, y * lb.n);
+        x = shift(x, y * lb.N); guard = shift(guard, y * lb.st);
+    }
+    const uint32_t gflag = guard ? *guard : 0u;
+    if (blockIdx.x < P.nblk) {
+        __shared__ VT sw[BS], s2w[BS], s2x[BS];
+        const uint32_t* B = P.blk + kPairHdr * blockIdx.x;
+        uint32_t h[kPairHdr];
+#pragma unroll
+        for (int k = 0; k < int(kPairHdr); k++) h[k] = B[k];
+        const uint32_t base = h[0], S = h[1], base2 = h[6], nk = h[7];
+        const uint32_t t = threadIdx.x;
+        // level-(l+1) node data, independent of the level-l sweep
+        uint32_t src = 0, row = 0;
+        VT dd = VT(0);
+        if (t < nk) {
+            src = P.src2[base2 + t];
+            row = P.row2[base2 + t];
+            dd = P.d2[base2 + t];
+        }
+        const uint32_t k2 = t < S ? P.kept2[base + t] : kPairNoRef;
+        VT acc = VT(0);
+        if (t < S) acc = fetch(f, base + t, P.qrow[base + t]);
+        const VT w = pair_L<BS>(S, h[2], base, P.Ldep, P.Lptr, P.Lloc, P.Lval, acc, sw);
+        if (!gflag && t < S && k2 == kPairNoRef) wt[base + t] = w;
+        // step 1 on level l+1: its right-hand side is the w of the block's kept rows
+        const VT f2 = t < nk ? sw[src] : VT(0);
+        if (!gflag && t < nk) F[base2 + t] = f2;
+        const VT w2 = pair_L<BS>(nk, h[8], base2, P.Ldep2, P.Lptr2, P.Lloc2, P.Lval2, f2, s2w);
+        (void)w2;
+        const VT v = pair_U<BS, VT, false>(h[10], h[11], h[9], P.Udep2, P.uloc2, P.Udiag2, P.Uptr2, P.Ucol2,
+                                           P.Uval2, (const VT*)s2w, 0u, (const VT*)nullptr, s2x);
+        const uint32_t vrow = t < h[10] ? P.row2[base2 + P.uloc2[h[9] + t]] & ~kPairIso2 : 0u;
+        if (gflag) return;
+        if (t < h[10]) u1[vrow] = v;
+        if (t < nk && (row & kPairIso2)) u1[row & ~kPairIso2] = dd * f2;
+    } else {
+        const uint32_t i = (blockIdx.x - P.nblk) * BS + threadIdx.x;
+        if (i < P.nflat2) {
+            const uint32_t q = P.nodes + i;
+            const VT v = fetch(f, q, P.qrow[q]);
+            if (!gflag) {
+                F[P.nodes2 + i] = v;
+                u1[P.frow2[i]] = P.fd2[i] * v;
+            }
+        } else if (i < P.nflat2 + P.Md) {
+            const uint32_t k = i - P.nflat2, q = P.nodes + i, j = P.qrow[q];
+            const XT v = XT(P.dval[k] * fetch(f, q, j));
+            if (!gflag) x[j] = v;
+        }
+    }
+}
+
+// step 2's combine of a kept row: x2 = (-alpha) x2 + (1+beta) u1s (cheb_solve.cc:79-81), as combine<OP_CHEB2>
+template <class VT>
+__device__ __forceinline__ VT pair_cheb2(VT x2, VT u, VT ma, int skip, VT opb, int opb_one) {
+    const VT v = skip ? x2 : x2 * ma;
+    return opb_one ? v + u : v + opb * u;
+}
+
+// K2: step 2 on level l+1 from the residual A_{l+1} (s u1) - F (A2 stored in
+// level (l+1)'s consumption order), the Chebyshev combine of step 2
+// (cheb_solve.cc:79-81) for every kept row, and level l's U sweep.
+template <class VT, class XT, bool SCALE, int BS, bool BAT>
+__global__ void __launch_bounds__(BS) k_pair_up(PairView<VT> P, SellView<VT> A2, const VT* __restrict__ u1, VT s,
+                                                  const VT* __restrict__ F, const VT* __restrict__ wt, VT ma,
+                                                  int skip, VT opb, int opb_one, XT* __restrict__ x,
+                                                  const uint32_t* guard, LevelBatch lb) {
+    if constexpr (BAT) {
+        const size_t y = blockIdx.y;
+        u1 = shift(u1, y * lb.n); F = shift(F, y * lb.n); wt = shift(wt, y * lb.nodes);
+        x = shift(x, y * lb.N); guard = shift(guard, y * lb.st);
+    }
+    const uint32_t gflag = guard ? *guard : 0u;
+    if (blockIdx.x < P.nblk) {
+        __shared__ VT s2w[BS], s2x[BS], s2c[BS], sx[BS];
+        const uint32_t* B = P.blk + kPairHdr * blockIdx.x;
+        uint32_t h[kPairHdr];
+#pragma unroll
+        for (int k = 0; k < int(kPairHdr); k++) h[k] = B[k];
+        const uint32_t base = h[0], base2 = h[6], nk = h[7];
+        const uint32_t t = threadIdx.x;
+        uint32_t row = 0, xl = 0;
+        VT dd = VT(0), f2 = VT(0), us = VT(0);
+        if (t < nk) {
+            row = P.row2[base2 + t];
+            xl = P.xl2[base2 + t];
+            dd = P.d2[base2 + t];
+            const uint32_t r = row & ~kPairIso2;
+            f2 = sell_row<VT, VT, SCALE>(A2, u1, base2 + t, s) - F[base2 + t];
+            us = SCALE ? u1[r] * s : u1[r];
+        }
+        pair_L<BS>(nk, h[8], base2, P.Ldep2, P.Lptr2, P.Lloc2, P.Lval2, f2, s2w);
+        if (t < nk && (row & kPairIso2)) s2x[t] = dd * f2;
+        pair_U<BS, VT, false>(h[10], h[11], h[9], P.Udep2, P.uloc2, P.Udiag2, P.Uptr2, P.Ucol2, P.Uval2,
+                              (const VT*)s2w, 0u, (const VT*)nullptr, s2x);
+        // the value of every kept row: the combine of step 2
+        VT c = VT(0);
+        if (t < nk) {
+            c = pair_cheb2(s2x[t], us, ma, skip, opb, opb_one);
+            s2c[t] = c;
+        }
+        __syncthreads();
+        const VT v = pair_U<BS, VT, true>(h[4], h[5], h[3], P.Udep, P.uloc, P.Udiag, P.Uptr, P.Ucol, P.Uval, wt,
+                                          base, (const VT*)s2c, sx);
+        if (gflag) return;
+        if (t < nk) x[xl] = XT(c);
+        if (t < h[4]) x[P.uxidx[h[3] + t]] = XT(v);
+    } else {
+        const uint32_t i = (blockIdx.x - P.nblk) * BS + threadIdx.x;
+        if (i < P.nflat2) {
+            const uint32_t q = P.nodes2 + i, r = P.frow2[i];
+            const VT f2 = sell_row<VT, VT, SCALE>(A2, u1, q, s) - F[q];
+            const VT x2 = P.fd2[i] * f2;
+            const VT us = SCALE ? u1[r] * s : u1[r];
+            const XT v = XT(pair_cheb2(x2, us, ma, skip, opb, opb_one));
+            if (!gflag) x[P.fxl[i]] = v;
+        }
+    }
+}
+
 // step 2: x2 = (-alpha) x2 ; x2 += (1+beta) u1   (cheb_solve.cc:79-81), then store or scatter
 template <class VT, class XT, bool STORE, bool BAT>
 __global__ void __launch_bounds__(kBS) k_cheb2(uint32_t n, const VT* __restrict__ x2, const VT* __restrict__ u1s,

@@ -219,13 +219,238 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P, const Bloc
     if (P.lvlU.empty()) P.lvlU.push_back(0);
 }
 
-void link_levels(LevelProgram& P, const LevelProgram& next) {
-    if (next.N != P.n) throw Error("level sizes do not chain", 1);
+void link_levels(LevelProgram& P, const std::vector<uint32_t>& next_qrow) {
+    if (next_qrow.size() != P.n) throw Error("level sizes do not chain", 1);
     std::vector<uint32_t> qpos(P.n);
-    for (uint32_t q = 0; q < next.N; q++) qpos[next.qrow[q]] = q;
+    for (uint32_t q = 0; q < P.n; q++) qpos[next_qrow[q]] = q;
     for (uint32_t& o : P.out)
         if (o & kGlobalRef) o = kGlobalRef | qpos[o & ~kGlobalRef];
     for (size_t i = 0; i < P.flat.size(); i++) P.flatq[i] = qpos[P.flat[i]];
+}
+
+void link_levels(LevelProgram& P, const LevelProgram& next) {
+    if (next.N != P.n) throw Error("level sizes do not chain", 1);
+    link_levels(P, next.qrow);
+}
+
+bool plan_pair(const LevelArrays& la, const LevelArrays& lb, uint32_t cap, PairProgram& out) {
+    const uint32_t N = la.N, M = la.M, Md = la.Md, K = N - Md, n = N - M - Md;
+    const uint32_t M2 = lb.M, Md2 = lb.Md;
+    if (lb.N != n || M2 + Md2 != lb.N) return false;          // level l+1 must keep no rows
+    // the structure the kernels rely on (as plan_level checks it)
+    for (uint32_t i = 0; i < K; i++)
+        for (uint32_t p = la.L_ia[i]; p < la.L_ia[i + 1]; p++)
+            if (la.L_ja[p] >= i || la.L_ja[p] >= M) throw Error("L row has a column outside the eliminated block", 1);
+    for (uint32_t i = 0; i < M; i++) {
+        if (la.U_ia[i + 1] <= la.U_ia[i] || la.U_ja[la.U_ia[i]] != i) throw Error("U row does not start with its pivot", 1);
+        for (uint32_t p = la.U_ia[i] + 1; p < la.U_ia[i + 1]; p++)
+            if (la.U_ja[p] <= i || la.U_ja[p] >= K) throw Error("U row has a column outside (i, N-Md)", 1);
+    }
+    for (uint32_t i = 0; i < M2; i++) {
+        for (uint32_t p = lb.L_ia[i]; p < lb.L_ia[i + 1]; p++)
+            if (lb.L_ja[p] >= i) throw Error("L row has a column outside the eliminated block", 1);
+        if (lb.U_ia[i + 1] <= lb.U_ia[i] || lb.U_ja[lb.U_ia[i]] != i) throw Error("U row does not start with its pivot", 1);
+        for (uint32_t p = lb.U_ia[i] + 1; p < lb.U_ia[i + 1]; p++)
+            if (lb.U_ja[p] <= i || lb.U_ja[p] >= M2) throw Error("U row has a column outside (i, N-Md)", 1);
+    }
+    std::vector<uint32_t> pos2(n);                          // level-(l+1) permuted index of row r
+    for (uint32_t i = 0; i < n; i++) pos2[lb.map[i]] = i;
+    // nodes: eliminated rows, kept rows with L entries or U references, kept rows eliminated on level l+1
+    std::vector<char> is_node(K, 0);
+    for (uint32_t i = 0; i < M; i++) is_node[i] = 1;
+    for (uint32_t i = M; i < K; i++)
+        if (la.L_ia[i + 1] > la.L_ia[i] || pos2[i - M] < M2) is_node[i] = 1;
+    for (uint32_t i = 0; i < M; i++)
+        for (uint32_t p = la.U_ia[i] + 1; p < la.U_ia[i + 1]; p++) is_node[la.U_ja[p]] = 1;
+    DSU dsu(std::max<uint32_t>(K, 1));
+    std::vector<uint32_t> dL(K, 0), dU(M, 0), dL2(M2, 0), dU2(M2, 0);
+    for (uint32_t i = 0; i < K; i++)
+        for (uint32_t p = la.L_ia[i]; p < la.L_ia[i + 1]; p++) {
+            dsu.unite(i, la.L_ja[p]);
+            dL[i] = std::max(dL[i], dL[la.L_ja[p]] + 1);
+        }
+    for (uint32_t i = M; i-- > 0;)
+        for (uint32_t p = la.U_ia[i] + 1; p < la.U_ia[i + 1]; p++) {
+            const uint32_t j = la.U_ja[p];
+            dsu.unite(i, j);
+            if (j < M) dU[i] = std::max(dU[i], dU[j] + 1);
+        }
+    for (uint32_t i = 0; i < M2; i++) {
+        const uint32_t a = M + lb.map[i];
+        for (uint32_t p = lb.L_ia[i]; p < lb.L_ia[i + 1]; p++) {
+            dsu.unite(a, M + lb.map[lb.L_ja[p]]);
+            dL2[i] = std::max(dL2[i], dL2[lb.L_ja[p]] + 1);
+        }
+    }
+    for (uint32_t i = M2; i-- > 0;)
+        for (uint32_t p = lb.U_ia[i] + 1; p < lb.U_ia[i + 1]; p++) {
+            const uint32_t j = lb.U_ja[p];
+            dsu.unite(M + lb.map[i], M + lb.map[j]);
+            dU2[i] = std::max(dU2[i], dU2[j] + 1);
+        }
+    // components in order of their first row; per component its four counts
+    std::vector<uint32_t> comp_of(K, kNoNode), root_comp(std::max<uint32_t>(K, 1), kNoNode);
+    struct Count { uint32_t S = 0, nk = 0, nu = 0, nu2 = 0; };
+    std::vector<Count> cnt;
+    for (uint32_t i = 0; i < K; i++) {
+        if (!is_node[i]) continue;
+        const uint32_t r = dsu.find(i);
+        if (root_comp[r] == kNoNode) { root_comp[r] = uint32_t(cnt.size()); cnt.emplace_back(); }
+        const uint32_t c = comp_of[i] = root_comp[r];
+        cnt[c].S++;
+        if (i < M) cnt[c].nu++;
+        else {
+            cnt[c].nk++;
+            if (pos2[i - M] < M2) cnt[c].nu2++;
+        }
+    }
+    for (const Count& c : cnt)
+        if (c.S > cap || c.nk > cap || c.nu > cap || c.nu2 > cap) return false;
+    std::vector<uint32_t> comp_blk(cnt.size());
+    uint32_t nblk = 0;
+    Count fill;
+    for (size_t c = 0; c < cnt.size(); c++) {
+        const Count& k = cnt[c];
+        if (c == 0 || fill.S + k.S > cap || fill.nk + k.nk > cap || fill.nu + k.nu > cap || fill.nu2 + k.nu2 > cap) {
+            nblk++;
+            fill = Count();
+        }
+        comp_blk[c] = nblk - 1;
+        fill.S += k.S; fill.nk += k.nk; fill.nu += k.nu; fill.nu2 += k.nu2;
+    }
+    std::vector<std::vector<uint32_t>> rows(nblk);
+    for (uint32_t i = 0; i < K; i++)
+        if (is_node[i]) rows[comp_blk[comp_of[i]]].push_back(i);
+
+    PairProgram P;
+    P.N = N; P.M = M; P.Md = Md; P.n = n; P.nblk = nblk;
+    P.blk.assign(size_t(kPairFields) * nblk, 0);
+    P.Lptr.push_back(0);
+    P.Uptr.push_back(0);
+    P.Lptr2.push_back(0);
+    P.Uptr2.push_back(0);
+    std::vector<uint32_t> local(K, kNoNode), local2(K, kNoNode);
+    // rows of `in` passing `keep`, ordered by key (stable)
+    auto by_key = [](const std::vector<uint32_t>& in, auto key, auto keep) {
+        std::vector<uint32_t> o;
+        for (uint32_t i : in) if (keep(i)) o.push_back(i);
+        std::stable_sort(o.begin(), o.end(), [&](uint32_t a, uint32_t b) { return key(a) < key(b); });
+        return o;
+    };
+    auto levels_of = [](const std::vector<uint32_t>& o, auto key) {
+        uint32_t nl = 0;
+        for (size_t k = 0; k < o.size(); k++) nl = std::max(nl, key(o[k]) + 1);
+        return nl;
+    };
+    for (uint32_t b = 0; b < nblk; b++) {
+        const auto& R = rows[b];
+        auto lo = by_key(R, [&](uint32_t i) { return dL[i]; }, [](uint32_t) { return true; });
+        auto uo = by_key(R, [&](uint32_t i) { return dU[i]; }, [&](uint32_t i) { return i < M; });
+        // kept nodes, by level-(l+1) L depth (rows isolated there at depth 0)
+        auto key2 = [&](uint32_t i) { const uint32_t q = pos2[i - M]; return q < M2 ? dL2[q] : 0u; };
+        auto ko = by_key(R, key2, [&](uint32_t i) { return i >= M; });
+        auto keyU2 = [&](uint32_t i) { return dU2[pos2[i - M]]; };
+        auto u2o = by_key(R, keyU2, [&](uint32_t i) { return i >= M && pos2[i - M] < M2; });
+        for (uint32_t k = 0; k < lo.size(); k++) local[lo[k]] = k;
+        for (uint32_t k = 0; k < ko.size(); k++) local2[ko[k]] = k;
+        uint32_t* B = &P.blk[size_t(kPairFields) * b];
+        B[0] = uint32_t(P.Ldep.size());
+        B[1] = uint32_t(lo.size());
+        B[2] = levels_of(lo, [&](uint32_t i) { return dL[i]; });
+        B[3] = uint32_t(P.Udep.size());
+        B[4] = uint32_t(uo.size());
+        B[5] = levels_of(uo, [&](uint32_t i) { return dU[i]; });
+        B[6] = uint32_t(P.src2.size());
+        B[7] = uint32_t(ko.size());
+        B[8] = levels_of(ko, key2);
+        B[9] = uint32_t(P.Udep2.size());
+        B[10] = uint32_t(u2o.size());
+        B[11] = levels_of(u2o, keyU2);
+        for (uint32_t i : lo) {
+            P.qrow.push_back(la.map[i]);
+            P.Ldep.push_back(dL[i]);
+            P.kept2.push_back(i >= M ? local2[i] : kNoRef);
+            for (uint32_t p = la.L_ia[i]; p < la.L_ia[i + 1]; p++) {
+                P.Lloc.push_back(local[la.L_ja[p]]);
+                P.Lval.push_back(la.L_a[p]);
+            }
+            P.Lptr.push_back(uint32_t(P.Lloc.size()));
+        }
+        for (uint32_t i : uo) {
+            P.Udep.push_back(dU[i]);
+            P.uloc.push_back(local[i]);
+            P.uxidx.push_back(la.map[i]);
+            P.Udiag.push_back(la.U_a[la.U_ia[i]]);
+            for (uint32_t p = la.U_ia[i] + 1; p < la.U_ia[i + 1]; p++) {
+                const uint32_t j = la.U_ja[p];
+                P.Ucol.push_back(j < M ? local[j] : (kGlobalRef | local2[j]));
+                P.Uval.push_back(la.U_a[p]);
+            }
+            P.Uptr.push_back(uint32_t(P.Ucol.size()));
+        }
+        for (uint32_t i : ko) {
+            const uint32_t r = i - M, q = pos2[r];
+            P.qrow2.push_back(r);
+            P.src2.push_back(local[i]);
+            P.row2.push_back(q < M2 ? r : (r | kIso2));
+            P.xl2.push_back(la.map[i]);
+            P.Ldep2.push_back(q < M2 ? dL2[q] : 0u);
+            P.d2.push_back(q < M2 ? 0.0 : lb.dval[q - M2]);
+            if (q < M2)
+                for (uint32_t p = lb.L_ia[q]; p < lb.L_ia[q + 1]; p++) {
+                    P.Lloc2.push_back(local2[M + lb.map[lb.L_ja[p]]]);
+                    P.Lval2.push_back(lb.L_a[p]);
+                }
+            P.Lptr2.push_back(uint32_t(P.Lloc2.size()));
+        }
+        for (uint32_t i : u2o) {
+            const uint32_t q = pos2[i - M];
+            P.Udep2.push_back(dU2[q]);
+            P.uloc2.push_back(local2[i]);
+            P.Udiag2.push_back(lb.U_a[lb.U_ia[q]]);
+            for (uint32_t p = lb.U_ia[q] + 1; p < lb.U_ia[q + 1]; p++) {
+                P.Ucol2.push_back(local2[M + lb.map[lb.U_ja[p]]]);
+                P.Uval2.push_back(lb.U_a[p]);
+            }
+            P.Uptr2.push_back(uint32_t(P.Ucol2.size()));
+        }
+        P.max_nodes = std::max<uint32_t>(P.max_nodes, B[1]);
+        P.max_U = std::max<uint32_t>(P.max_U, B[4]);
+        P.max_nodes2 = std::max<uint32_t>(P.max_nodes2, B[7]);
+        P.max_U2 = std::max<uint32_t>(P.max_U2, B[10]);
+    }
+    P.nodes = uint32_t(P.Ldep.size());
+    P.nU = uint32_t(P.Udep.size());
+    P.nodes2 = uint32_t(P.src2.size());
+    P.nU2 = uint32_t(P.Udep2.size());
+    // flat kept rows, then isolated rows
+    for (uint32_t i = M; i < K; i++) {
+        if (is_node[i]) continue;
+        const uint32_t r = i - M;
+        P.qrow.push_back(la.map[i]);
+        P.qrow2.push_back(r);
+        P.frow2.push_back(r);
+        P.fxl.push_back(la.map[i]);
+        P.fd2.push_back(lb.dval[pos2[r] - M2]);
+    }
+    P.nflat2 = uint32_t(P.frow2.size());
+    for (uint32_t k = 0; k < Md; k++) P.qrow.push_back(la.map[K + k]);
+    if (P.qrow.size() != N || P.qrow2.size() != n) throw Error("pair program does not cover the levels' rows", 1);
+    {
+        std::vector<char> seen(N, 0);
+        for (uint32_t j : P.qrow) {
+            if (j >= N || seen[j]) throw Error("pair program does not cover the level's rows once", 1);
+            seen[j] = 1;
+        }
+    }
+    // no empty device arrays
+    for (auto* v : {&P.Lloc, &P.Ucol, &P.Lloc2, &P.Ucol2, &P.kept2, &P.Ldep, &P.Udep, &P.uloc, &P.uxidx, &P.src2,
+                    &P.row2, &P.xl2, &P.Ldep2, &P.Udep2, &P.uloc2, &P.frow2, &P.fxl, &P.blk})
+        if (v->empty()) v->push_back(0);
+    for (auto* v : {&P.Lval, &P.Uval, &P.Udiag, &P.Lval2, &P.d2, &P.Uval2, &P.Udiag2, &P.fd2})
+        if (v->empty()) v->push_back(0.0);
+    out = std::move(P);
+    return true;
 }
 
 bool plan_stencil(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, StencilLayout& S) {

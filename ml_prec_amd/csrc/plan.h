@@ -122,5 +122,60 @@ void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double*
 // of level l+1 (F slot of kept row r = position of row r in next.qrow), so
 // that level l+1 reads it with plain coalesced loads.
 void link_levels(LevelProgram& P, const LevelProgram& next);
+// The same for a next level whose consumption order is next_qrow.
+void link_levels(LevelProgram& P, const std::vector<uint32_t>& next_qrow);
+
+// Fused program of a uh_cheb level l and the coarsest level l+1 (which keeps
+// no rows).  A visit of level l with two Chebyshev steps then takes two
+// launches instead of four:
+//   K1: level l's L sweep, then, for the kept rows the block produced, step 1
+//       on level l+1 (its L sweep, U sweep and diagonal): u1 = B_{l+1} F;
+//   K2: step 2 on level l+1 from the residual A_{l+1} (a1 u1) - F, the
+//       Chebyshev combine of the kept rows, and level l's U sweep.
+// Every level-(l+1) component must lie in one block together with the
+// level-l components whose roots (or flat kept rows) it consists of: blocks
+// are formed from the connected components of level l's L and U couplings
+// joined with level (l+1)'s.  Kept rows of level l that are isolated on
+// level l+1 and feed no level-l component ("flat") run outside the blocks.
+//
+// Block b header pblk[12b ..]: [0] first level-l node, [1] level-l nodes,
+// [2] L depth levels, [3] first level-l U-list entry, [4] U-list length,
+// [5] U depth levels, [6] first level-(l+1) node, [7] level-(l+1) nodes (the
+// block's kept rows), [8] their L depth levels, [9] first level-(l+1) U-list
+// entry, [10] its length, [11] its U depth levels.
+//
+// Consumption orders: level l reads its input at positions [block nodes]
+// [flat kept rows][isolated rows] (qrow); level l+1 at [block level-(l+1)
+// nodes][flat kept rows] (qrow2), which is also where F is stored.
+constexpr uint32_t kPairFields = 12;
+constexpr uint32_t kIso2 = 0x80000000u;   // row2 flag: the row is isolated on level l+1
+struct PairProgram {
+    uint32_t N = 0, M = 0, Md = 0, n = 0;
+    uint32_t nblk = 0, nodes = 0, nodes2 = 0, nflat2 = 0, nU = 0, nU2 = 0;
+    uint32_t max_nodes = 0, max_nodes2 = 0, max_U = 0, max_U2 = 0;
+    std::vector<uint32_t> blk;
+    std::vector<uint32_t> qrow, qrow2;
+    // level-l nodes: eliminated rows and kept rows of the block, by L depth
+    std::vector<uint32_t> Ldep, Lptr, Lloc, kept2;   // kept2: level-(l+1) local index of a kept node, else kNoRef
+    std::vector<double> Lval;
+    // level-l U list: eliminated rows by U depth; kept columns as kGlobalRef | level-(l+1) local index
+    std::vector<uint32_t> Udep, uloc, uxidx, Uptr, Ucol;
+    std::vector<double> Uval, Udiag;
+    // level-(l+1) nodes: src (level-l local index), row2 (level-(l+1) row, kIso2 when isolated there),
+    // xl (level-l output row of the kept row), L' by depth
+    std::vector<uint32_t> src2, row2, xl2, Ldep2, Lptr2, Lloc2;
+    std::vector<double> Lval2, d2;                   // d2: 1 / diagonal of a row isolated on level l+1
+    // level-(l+1) U list (rows eliminated there), by U depth, columns as local indices
+    std::vector<uint32_t> Udep2, uloc2, Uptr2, Ucol2;
+    std::vector<double> Uval2, Udiag2;
+    // flat kept rows (outside the blocks): level-(l+1) row, level-l output row, d2
+    std::vector<uint32_t> frow2, fxl;
+    std::vector<double> fd2;
+};
+constexpr uint32_t kNoRef = 0xffffffffu;
+// Plans the pair (la = level l, lb = level l+1, which must keep no rows).
+// Returns false (P untouched) when a merged component exceeds max_nodes
+// level-l nodes, max_nodes level-(l+1) nodes or max_nodes U-list entries.
+bool plan_pair(const LevelArrays& la, const LevelArrays& lb, uint32_t max_nodes, PairProgram& P);
 
 }  // namespace mlp
