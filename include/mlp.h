@@ -294,7 +294,8 @@ int mlp_spe_assemble(int kind, uint32_t nx, uint32_t ny, uint32_t nz, const doub
                      double c, double hx, double hy, double hz, uint32_t* ia, uint32_t* ja, double* a);
 
 /* ---- diagnostics (no counterpart in the reference; used for roofline reporting) ---- */
-/* algorithmic HBM bytes of one apply (sum over level visits) */
+/* algorithmic HBM bytes of one apply (sum over level visits) and the kernels the last apply launched
+ * (0 before the first apply) */
 int mlp_prec_bytes(mlp_prec p, double* bytes_per_apply, uint32_t* launches_per_apply);
 /* Average device time of `reps` back-to-back SpMV launches y = A x (two HIP
  * events on the context stream around the whole run).  flush_bytes > 0 makes

@@ -370,6 +370,7 @@ template <class VT> struct DevHier : DevPrecBase {
     DevMem scratch;                                              // per-instance level vectors
     mlp_mat A0 = nullptr;
     mutable uint32_t launches = 0;
+    uint32_t last_launches = 0;
     bool dynamic = false;
     int pair_l = -1;                         // level fused with the coarsest level (-1: none)
 
@@ -916,6 +917,7 @@ template <class VT> struct DevHier : DevPrecBase {
     }
 
     void apply(const double* f, double* x, const uint32_t* guard) override {
+        const uint32_t l0 = launches;
         if (kind == MLP_PREC_UH_CHEB) {
             InSrc<VT, double, IN_PLAIN> fs;
             fs.f = f;
@@ -923,6 +925,7 @@ template <class VT> struct DevHier : DevPrecBase {
         } else {
             msplit<double, double>(0, f, x, guard);
         }
+        last_launches = launches - l0;
     }
     bool graph_safe() const override { return !dynamic; }
 
@@ -953,10 +956,8 @@ template <class VT> struct DevHier : DevPrecBase {
         }
         return tot;
     }
-    uint32_t launches_per_apply() const override {
-        uint32_t before = launches;
-        return before;
-    }
+    // kernels the last apply launched (0 before the first apply)
+    uint32_t launches_per_apply() const override { return last_launches; }
 };
 
 struct SimplePrec : DevPrecBase {
@@ -2035,7 +2036,7 @@ int mlp_prec_bytes(mlp_prec p, double* bytes, uint32_t* launches) {
     MLP_API_BEGIN
     require(p, "null argument");
     if (bytes) *bytes = p->impl->bytes_per_apply();
-    if (launches) *launches = 0;
+    if (launches) *launches = p->impl->launches_per_apply();
     MLP_API_END
 }
 

@@ -849,87 +849,101 @@ constexpr uint32_t kPairHdr = 12;          // header words per block of a pair p
 constexpr uint32_t kPairIso2 = 0x80000000u;
 constexpr uint32_t kPairNoRef = 0xffffffffu;
 
-// Owned-node L sweep (as owned_down) of n nodes at base with input acc:
-// thread p owns node p; sw holds every node's w afterwards.
-template <int BS, class VT>
-__device__ __forceinline__ VT pair_L(uint32_t n, uint32_t nl, uint32_t base, const uint32_t* __restrict__ Ldep,
-                                     const uint32_t* __restrict__ Lptr, const uint32_t* __restrict__ Lloc,
-                                     const VT* __restrict__ Lval, VT acc, VT* sw) {
+// Per-thread data of an owned-node L sweep (as owned_down): thread p owns
+// node p of the n nodes at base.  Loaded at kernel start, so that the loads
+// overlap whatever runs before the sweep.
+template <class VT> struct OwnL {
+    uint32_t my, q0, q1, l0, l1;
+    VT v0, v1;
+};
+template <class VT>
+__device__ __forceinline__ OwnL<VT> own_L(uint32_t n, uint32_t nl, uint32_t base, const uint32_t* __restrict__ Ldep,
+                                          const uint32_t* __restrict__ Lptr, const uint32_t* __restrict__ Lloc,
+                                          const VT* __restrict__ Lval) {
     const uint32_t p = threadIdx.x;
-    uint32_t my = nl, q0 = 0, q1 = 0, l0 = p, l1 = p;
-    VT v0 = VT(0), v1 = VT(0);
+    OwnL<VT> o{nl, 0u, 0u, p, p, VT(0), VT(0)};
     if (p < n) {
-        my = Ldep[base + p];
-        q0 = Lptr[base + p];
-        q1 = Lptr[base + p + 1];
-        if (q1 > q0) { l0 = Lloc[q0]; v0 = Lval[q0]; }
-        if (q1 > q0 + 1) { l1 = Lloc[q0 + 1]; v1 = Lval[q0 + 1]; }
+        o.my = Ldep[base + p];
+        o.q0 = Lptr[base + p];
+        o.q1 = Lptr[base + p + 1];
+        if (o.q1 > o.q0) { o.l0 = Lloc[o.q0]; o.v0 = Lval[o.q0]; }
+        if (o.q1 > o.q0 + 1) { o.l1 = Lloc[o.q0 + 1]; o.v1 = Lval[o.q0 + 1]; }
     }
-    const bool h0 = q1 > q0, h1 = q1 > q0 + 1, more = q1 > q0 + 2;
+    return o;
+}
+// The sweep: at its depth level a node subtracts its L entries from acc;
+// sw holds every node's w afterwards.
+template <int BS, class VT>
+__device__ __forceinline__ VT sweep_L(const OwnL<VT>& o, uint32_t nl, const uint32_t* __restrict__ Lloc,
+                                      const VT* __restrict__ Lval, VT acc, VT* sw) {
+    const bool h0 = o.q1 > o.q0, h1 = o.q1 > o.q0 + 1, more = o.q1 > o.q0 + 2;
     for (uint32_t li = 0; li < nl; li++) {
-        if (li == my) {
-            const VT x0 = sw[l0], x1 = sw[l1];
-            acc = h0 ? msub(acc, v0, x0) : acc;
-            acc = h1 ? msub(acc, v1, x1) : acc;
+        if (li == o.my) {
+            const VT x0 = sw[o.l0], x1 = sw[o.l1];
+            acc = h0 ? msub(acc, o.v0, x0) : acc;
+            acc = h1 ? msub(acc, o.v1, x1) : acc;
             if (more)
-                for (uint32_t q = q0 + 2; q < q1; q++) acc = msub(acc, Lval[q], sw[Lloc[q]]);
-            sw[p] = acc;
+                for (uint32_t q = o.q0 + 2; q < o.q1; q++) acc = msub(acc, Lval[q], sw[Lloc[q]]);
+            sw[threadIdx.x] = acc;
         }
         level_sync<BS>();
     }
     return acc;
 }
 
-// Owned-node U sweep (as owned_up) of the U-list entries [ub, ub + nu):
-// w of a row from zsrc[zbase + loc], columns local (into sx) or, flagged
-// kGlobalBit, kept values kept[c]; solved values land in sx[loc].
-// KEPT = false: every column is local (kept is not read).
-template <int BS, class VT, bool KEPT>
-__device__ __forceinline__ VT pair_U(uint32_t nu, uint32_t nl, uint32_t ub, const uint32_t* __restrict__ Udep,
-                                     const uint32_t* __restrict__ uloc, const VT* __restrict__ Udiag,
-                                     const uint32_t* __restrict__ Uptr, const uint32_t* __restrict__ Ucol,
-                                     const VT* __restrict__ Uval, const VT* zsrc, uint32_t zbase, const VT* kept,
-                                     VT* sx) {
+// Per-thread data of an owned-node U sweep (as owned_up) over the U-list
+// entries [ub, ub + nu): columns are local, or (KEPT, flagged kGlobalBit)
+// kept rows whose values come from an LDS array at the sweep.
+template <class VT> struct OwnU {
+    uint32_t my, q0, q1, loc, c0, c1;
+    VT d, v0, v1;
+};
+template <class VT>
+__device__ __forceinline__ OwnU<VT> own_U(uint32_t nu, uint32_t nl, uint32_t ub, const uint32_t* __restrict__ Udep,
+                                          const uint32_t* __restrict__ uloc, const VT* __restrict__ Udiag,
+                                          const uint32_t* __restrict__ Uptr, const uint32_t* __restrict__ Ucol,
+                                          const VT* __restrict__ Uval) {
     const uint32_t u = threadIdx.x;
-    uint32_t my = nl, q0 = 0, q1 = 0, c0 = kGlobalBit, c1 = kGlobalBit, loc = 0;
-    VT z = VT(0), d = VT(1), v0 = VT(0), v1 = VT(0), r0 = VT(0), r1 = VT(0);
+    OwnU<VT> o{nl, 0u, 0u, 0u, 0u, 0u, VT(1), VT(0), VT(0)};
     if (u < nu) {
-        my = Udep[ub + u];
-        loc = uloc[ub + u];
-        d = Udiag[ub + u];
-        q0 = Uptr[ub + u];
-        q1 = Uptr[ub + u + 1];
-        z = zsrc[zbase + loc];
-        if (q1 > q0) {
-            c0 = Ucol[q0];
-            v0 = Uval[q0];
-            if (KEPT && (c0 & kGlobalBit)) r0 = kept[c0 & ~kGlobalBit];
-        }
-        if (q1 > q0 + 1) {
-            c1 = Ucol[q0 + 1];
-            v1 = Uval[q0 + 1];
-            if (KEPT && (c1 & kGlobalBit)) r1 = kept[c1 & ~kGlobalBit];
-        }
+        o.my = Udep[ub + u];
+        o.loc = uloc[ub + u];
+        o.d = Udiag[ub + u];
+        o.q0 = Uptr[ub + u];
+        o.q1 = Uptr[ub + u + 1];
+        if (o.q1 > o.q0) { o.c0 = Ucol[o.q0]; o.v0 = Uval[o.q0]; }
+        if (o.q1 > o.q0 + 1) { o.c1 = Ucol[o.q0 + 1]; o.v1 = Uval[o.q0 + 1]; }
     }
-    const bool h0 = q1 > q0, h1 = q1 > q0 + 1, more = q1 > q0 + 2;
-    const bool g0 = KEPT && (c0 & kGlobalBit) != 0, g1 = KEPT && (c1 & kGlobalBit) != 0;
-    const uint32_t a0 = (g0 || !h0) ? loc : c0, a1 = (g1 || !h1) ? loc : c1;
+    return o;
+}
+// The sweep from the row's w z; solved values land in sx[loc].
+template <int BS, class VT, bool KEPT>
+__device__ __forceinline__ VT sweep_U(const OwnU<VT>& o, uint32_t nl, const uint32_t* __restrict__ Ucol,
+                                      const VT* __restrict__ Uval, VT z, const VT* kept, VT* sx) {
+    const bool h0 = o.q1 > o.q0, h1 = o.q1 > o.q0 + 1, more = o.q1 > o.q0 + 2;
+    const bool g0 = KEPT && h0 && (o.c0 & kGlobalBit) != 0, g1 = KEPT && h1 && (o.c1 & kGlobalBit) != 0;
+    VT r0 = VT(0), r1 = VT(0);
+    if (KEPT) {
+        if (g0) r0 = kept[o.c0 & ~kGlobalBit];
+        if (g1) r1 = kept[o.c1 & ~kGlobalBit];
+    }
+    const uint32_t a0 = (g0 || !h0) ? o.loc : o.c0, a1 = (g1 || !h1) ? o.loc : o.c1;
     for (uint32_t li = 0; li < nl; li++) {
-        if (li == my) {
+        if (li == o.my) {
             const VT s0 = sx[a0], s1 = sx[a1];
             const VT x0 = g0 ? r0 : s0, x1 = g1 ? r1 : s1;
-            z = h0 ? msub(z, v0, x0) : z;
-            z = h1 ? msub(z, v1, x1) : z;
+            z = h0 ? msub(z, o.v0, x0) : z;
+            z = h1 ? msub(z, o.v1, x1) : z;
             if (more)
-                for (uint32_t q = q0 + 2; q < q1; q++) {
+                for (uint32_t q = o.q0 + 2; q < o.q1; q++) {
                     const uint32_t c = Ucol[q];
                     VT xv;
                     if (KEPT && (c & kGlobalBit)) xv = kept[c & ~kGlobalBit];
                     else xv = sx[c];
                     z = msub(z, Uval[q], xv);
                 }
-            z = udiv(z, d);
-            sx[loc] = z;
+            z = udiv(z, o.d);
+            sx[o.loc] = z;
         }
         level_sync<BS>();
     }
@@ -959,7 +973,10 @@ __global__ void __launch_bounds__(BS) k_pair_down(PairView<VT> P, InSrc<VT, FT, 
         for (int k = 0; k < int(kPairHdr); k++) h[k] = B[k];
         const uint32_t base = h[0], S = h[1], base2 = h[6], nk = h[7];
         const uint32_t t = threadIdx.x;
-        // level-(l+1) node data, independent of the level-l sweep
+        // every per-node load of both levels first, so that they overlap
+        const OwnL<VT> oa = own_L(S, h[2], base, P.Ldep, P.Lptr, P.Lloc, P.Lval);
+        const OwnL<VT> ob = own_L(nk, h[8], base2, P.Ldep2, P.Lptr2, P.Lloc2, P.Lval2);
+        const OwnU<VT> ou = own_U(h[10], h[11], h[9], P.Udep2, P.uloc2, P.Udiag2, P.Uptr2, P.Ucol2, P.Uval2);
         uint32_t src = 0, row = 0;
         VT dd = VT(0);
         if (t < nk) {
@@ -967,19 +984,18 @@ __global__ void __launch_bounds__(BS) k_pair_down(PairView<VT> P, InSrc<VT, FT, 
             row = P.row2[base2 + t];
             dd = P.d2[base2 + t];
         }
+        const uint32_t vrow = t < h[10] ? P.row2[base2 + ou.loc] & ~kPairIso2 : 0u;
         const uint32_t k2 = t < S ? P.kept2[base + t] : kPairNoRef;
         VT acc = VT(0);
         if (t < S) acc = fetch(f, base + t, P.qrow[base + t]);
-        const VT w = pair_L<BS>(S, h[2], base, P.Ldep, P.Lptr, P.Lloc, P.Lval, acc, sw);
+        const VT w = sweep_L<BS>(oa, h[2], P.Lloc, P.Lval, acc, sw);
         if (!gflag && t < S && k2 == kPairNoRef) wt[base + t] = w;
         // step 1 on level l+1: its right-hand side is the w of the block's kept rows
         const VT f2 = t < nk ? sw[src] : VT(0);
         if (!gflag && t < nk) F[base2 + t] = f2;
-        const VT w2 = pair_L<BS>(nk, h[8], base2, P.Ldep2, P.Lptr2, P.Lloc2, P.Lval2, f2, s2w);
-        (void)w2;
-        const VT v = pair_U<BS, VT, false>(h[10], h[11], h[9], P.Udep2, P.uloc2, P.Udiag2, P.Uptr2, P.Ucol2,
-                                           P.Uval2, (const VT*)s2w, 0u, (const VT*)nullptr, s2x);
-        const uint32_t vrow = t < h[10] ? P.row2[base2 + P.uloc2[h[9] + t]] & ~kPairIso2 : 0u;
+        sweep_L<BS>(ob, h[8], P.Lloc2, P.Lval2, f2, s2w);
+        const VT z = t < h[10] ? s2w[ou.loc] : VT(0);
+        const VT v = sweep_U<BS, VT, false>(ou, h[11], P.Ucol2, P.Uval2, z, (const VT*)nullptr, s2x);
         if (gflag) return;
         if (t < h[10]) u1[vrow] = v;
         if (t < nk && (row & kPairIso2)) u1[row & ~kPairIso2] = dd * f2;
@@ -1029,6 +1045,12 @@ __global__ void __launch_bounds__(BS) k_pair_up(PairView<VT> P, SellView<VT> A2,
         for (int k = 0; k < int(kPairHdr); k++) h[k] = B[k];
         const uint32_t base = h[0], base2 = h[6], nk = h[7];
         const uint32_t t = threadIdx.x;
+        // every per-node load first: level l+1's sweeps and level l's U sweep with its w
+        const OwnL<VT> ob = own_L(nk, h[8], base2, P.Ldep2, P.Lptr2, P.Lloc2, P.Lval2);
+        const OwnU<VT> o2 = own_U(h[10], h[11], h[9], P.Udep2, P.uloc2, P.Udiag2, P.Uptr2, P.Ucol2, P.Uval2);
+        const OwnU<VT> oa = own_U(h[4], h[5], h[3], P.Udep, P.uloc, P.Udiag, P.Uptr, P.Ucol, P.Uval);
+        const VT za = t < h[4] ? wt[base + oa.loc] : VT(0);
+        const uint32_t xa = t < h[4] ? P.uxidx[h[3] + t] : 0u;
         uint32_t row = 0, xl = 0;
         VT dd = VT(0), f2 = VT(0), us = VT(0);
         if (t < nk) {
@@ -1039,10 +1061,10 @@ __global__ void __launch_bounds__(BS) k_pair_up(PairView<VT> P, SellView<VT> A2,
             f2 = sell_row<VT, VT, SCALE>(A2, u1, base2 + t, s) - F[base2 + t];
             us = SCALE ? u1[r] * s : u1[r];
         }
-        pair_L<BS>(nk, h[8], base2, P.Ldep2, P.Lptr2, P.Lloc2, P.Lval2, f2, s2w);
+        sweep_L<BS>(ob, h[8], P.Lloc2, P.Lval2, f2, s2w);
         if (t < nk && (row & kPairIso2)) s2x[t] = dd * f2;
-        pair_U<BS, VT, false>(h[10], h[11], h[9], P.Udep2, P.uloc2, P.Udiag2, P.Uptr2, P.Ucol2, P.Uval2,
-                              (const VT*)s2w, 0u, (const VT*)nullptr, s2x);
+        const VT z2 = t < h[10] ? s2w[o2.loc] : VT(0);
+        sweep_U<BS, VT, false>(o2, h[11], P.Ucol2, P.Uval2, z2, (const VT*)nullptr, s2x);
         // the value of every kept row: the combine of step 2
         VT c = VT(0);
         if (t < nk) {
@@ -1050,11 +1072,10 @@ __global__ void __launch_bounds__(BS) k_pair_up(PairView<VT> P, SellView<VT> A2,
             s2c[t] = c;
         }
         __syncthreads();
-        const VT v = pair_U<BS, VT, true>(h[4], h[5], h[3], P.Udep, P.uloc, P.Udiag, P.Uptr, P.Ucol, P.Uval, wt,
-                                          base, (const VT*)s2c, sx);
+        const VT v = sweep_U<BS, VT, true>(oa, h[5], P.Ucol, P.Uval, za, (const VT*)s2c, sx);
         if (gflag) return;
         if (t < nk) x[xl] = XT(c);
-        if (t < h[4]) x[P.uxidx[h[3] + t]] = XT(v);
+        if (t < h[4]) x[xa] = XT(v);
     } else {
         const uint32_t i = (blockIdx.x - P.nblk) * BS + threadIdx.x;
         if (i < P.nflat2) {

@@ -408,6 +408,13 @@ class DevPrec:
         check(lib().mlp_prec_bytes(self.h, C.byref(b), C.byref(n)))
         return b.value
 
+    def launches_per_apply(self):
+        """Kernels the last apply launched (0 before the first apply)."""
+        b = C.c_double()
+        n = C.c_uint32()
+        check(lib().mlp_prec_bytes(self.h, C.byref(b), C.byref(n)))
+        return n.value
+
 
 def _opts(eps, maxit, x0_mode, flexible, true_res_every, check_every, use_graph):
     return SolveOpts(eps, maxit, x0_mode, int(flexible), true_res_every, check_every, int(use_graph))
