@@ -595,6 +595,91 @@ def solvers(N, S, args):
     ctx.close()
 
 
+def level_descs(H, first, s):
+    """Levels first..last of host hierarchy H as mlp_level_desc dicts (a standalone hierarchy whose top level is
+    `first`) and the top level's matrix (ia, ja, a)."""
+    out = []
+    prev = None
+    for k in range(H.nlevels):
+        info, arr = H.level_arrays(k)
+        if k >= first:
+            d = dict(N=info["N"], M=info["M"], Md=info["Md"], nit=info["nit"], lmin=info["lmin"], lmax=info["lmax"],
+                     eps=info["eps"], map=arr["map"], L_ia=arr["L_ia"], L_ja=arr["L_ja"], L_a=arr["L_a"],
+                     U_ia=arr["U_ia"], U_ja=arr["U_ja"], U_a=arr["U_a"], dval=arr["dval"])
+            if k > first:
+                d.update(A_ia=prev["A_ia"], A_ja=prev["A_ja"], A_a=prev["A_a"])
+            out.append(d)
+        if k + 1 == first:
+            top = (arr["A_ia"], arr["A_ja"], arr["A_a"])
+        prev = arr
+    if first == 0:
+        top = (s.ia, s.ja, s.a)
+    return out, top
+
+
+def error_budget(N, S, args):
+    """Error of the FP32 hierarchy level by level: for each level l the hierarchy from l down to the coarsest
+    (whose solve is FP64 in both) is uploaded in FP64 and in FP32 and applied to the same seeded vectors; the
+    relative difference of the two results is the error the FP32 storage and arithmetic of levels l.. add.  Also
+    the apply times and, for the whole hierarchy, the effect on PCG."""
+    s = S.spe10_system(kind="spe", c=1.0, nz=args.nz)
+    H = N.build_uh_cheb(s.ia, s.ja, s.a)
+    ctx = N.Context(0)
+    rows = []
+    g = np.random.default_rng(11)
+    for l in range(H.nlevels):
+        levels, (ia, ja, a) = level_descs(H, l, s)
+        A = N.DevMatrix(ctx, ia, ja, a)
+        B64 = N.DevPrec.upload(ctx, A, N.UH_CHEB, levels, N.MLP_FP64)
+        B32 = N.DevPrec.upload(ctx, A, N.UH_CHEB, levels, N.MLP_FP32)
+        n = len(ia) - 1
+        errs = []
+        y64, y32 = ctx.array(n), ctx.array(n)
+        for k in range(4):
+            f = ctx.to_device(g.standard_normal(n))
+            B64.apply(f, y64)
+            B32.apply(f, y32)
+            a64, a32 = y64.get(), y32.get()
+            errs.append(float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64)))
+        f = ctx.to_device(g.standard_normal(n))
+        t64 = min(B64.time_apply(f, y64, reps=20) for _ in range(3))
+        t32 = min(B32.time_apply(f, y32, reps=20) for _ in range(3))
+        row = {"level": l, "N": n, "fp32_rel_err_max": max(errs), "fp32_rel_err_mean": float(np.mean(errs)),
+               "apply_us_fp64": round(t64 * 1e3, 2), "apply_us_fp32": round(t32 * 1e3, 2),
+               "launches": B64.launches_per_apply()}
+        log(f"levels {l}..{H.nlevels - 1}: N={n} FP32 operator rel err {max(errs):.3e}; apply FP64 "
+            f"{t64 * 1e3:.1f} us, FP32 {t32 * 1e3:.1f} us")
+        rows.append(row)
+        B32.close()
+        B64.close()
+        A.close()
+    # the whole hierarchy inside PCG, to 1e-8 and 1e-10: iterations and error against the FP64 solution
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    B64 = N.DevPrec(ctx, A, H, precision=N.MLP_FP64)
+    B32 = N.DevPrec(ctx, A, H, precision=N.MLP_FP32)
+    b = ctx.to_device(s.b)
+    x = ctx.array(s.n)
+    pcg = {}
+    for eps in (1e-8, 1e-10):
+        st64 = N.pcg(ctx, A, B64, b, x, eps=eps, maxit=10000, use_graph=True, check_every=8)
+        x64 = x.get()
+        st32 = N.pcg(ctx, A, B32, b, x, eps=eps, maxit=10000, flexible=True, true_res_every=20, use_graph=True,
+                     check_every=8)
+        x32 = x.get()
+        pcg[f"{eps:g}"] = {"niter_fp64": int(st64["niter"]), "niter_mixed": int(st32["niter"]),
+                           "true_rel_res_mixed": st32["true_rel_res"],
+                           "mixed_rel_err_vs_fp64": float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))}
+        log(f"PCG to {eps:g}: FP64 {st64['niter']} it, mixed {st32['niter']} it, mixed vs FP64 "
+            f"{pcg[f'{eps:g}']['mixed_rel_err_vs_fp64']:.3e}")
+    print(json.dumps({"error_budget": rows, "pcg": pcg,
+                      "workload": f"SPE10-shaped {s.nx}x{s.ny}x{s.nz}, uh_cheb sigma 3 ncheb 2, "
+                                  f"{H.nlevels} levels"}), flush=True)
+    for o in (B32, B64, A):
+        o.close()
+    del b, x
+    ctx.close()
+
+
 def apply_traffic_probe(N, s, reps):
     """Preconditioner applies (FP64 and FP32 hierarchies, `reps` each, launched from the host) and the same
     1 GiB calibration kernel as the SpMV probe, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs: the level
@@ -670,6 +755,8 @@ def main():
     ap.add_argument("--case", default="", help="with --solvers: comma-separated case names to run")
     ap.add_argument("--cpu-budget", type=float, default=25.0,
                     help="with --solvers: estimated seconds of single-core oracle solve allowed per case")
+    ap.add_argument("--error-budget", action="store_true",
+                    help="the FP32 hierarchy's error level by level against FP64 instead; prints its own JSON line")
     ap.add_argument("--apply-traffic", action="store_true",
                     help="only run preconditioner applies and a 1 GiB scale kernel (for HBM counter runs), then exit")
     ap.add_argument("--spmv-traffic", action="store_true",
@@ -694,6 +781,9 @@ def main():
         return
     if args.solvers:
         solvers(N, S, args)
+        return
+    if args.error_budget:
+        error_budget(N, S, args)
         return
 
     t0 = time.perf_counter()

@@ -220,9 +220,11 @@ template <class VT> struct DevLevel {
     VT* r = nullptr;                           // multi_split inner residual
     // bookkeeping for the byte model
     uint32_t nnzL = 0, nnzU = 0;
-    // the level run fused with the coarsest level (two launches per visit)
+    // the level run fused with the coarsest level (two launches per visit); the coarsest level's values
+    // and its matrix (read by step 2's residual) are FP64 in every hierarchy
     bool fused = false;
-    PairView<VT> pair;
+    PairView<VT, double> pair;
+    DevSell<double> A64;
 };
 static_assert(kPairHdr == kPairFields, "pair block header size");
 
@@ -457,7 +459,7 @@ template <class VT> struct DevHier : DevPrecBase {
         L.lbs = pbs;
         L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
         L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
-        PairView<VT>& V = L.pair;
+        PairView<VT, double>& V = L.pair;
         V.nblk = PP.nblk; V.nodes = PP.nodes; V.nodes2 = PP.nodes2; V.nflat2 = PP.nflat2; V.Md = D.Md;
         Stager up(ctx, *data);
         std::vector<double> r1, r2;
@@ -475,15 +477,12 @@ template <class VT> struct DevHier : DevPrecBase {
         up.add(PP.frow2, V.frow2); up.add(PP.fxl, V.fxl); up.add(PP.fd2, V.fd2);
         if constexpr (std::is_same_v<VT, float>) {             // udiv: FP32 multiplies by 1 / u_ii
             r1.resize(PP.Udiag.size());
-            r2.resize(PP.Udiag2.size());
             for (size_t i = 0; i < r1.size(); i++) r1[i] = 1.0 / PP.Udiag[i];
-            for (size_t i = 0; i < r2.size(); i++) r2[i] = 1.0 / PP.Udiag2[i];
             up.add(r1, V.Udiag);
-            up.add(r2, V.Udiag2);
         } else {
             up.add(PP.Udiag, V.Udiag);
-            up.add(PP.Udiag2, V.Udiag2);
         }
+        up.add(PP.Udiag2, V.Udiag2);                            // FP64: the coarsest level divides
         up.add(D.dval, D.Md, L.dval);
         up.flush();
         V.dval = L.dval;
@@ -573,9 +572,10 @@ template <class VT> struct DevHier : DevPrecBase {
                 continue;
             }
             if (pair_l >= 0 && int(l) == pair_l + 1) {
-                // the coarsest level runs inside the pair kernels; only its matrix is read here,
+                // the coarsest level runs inside the pair kernels; only its matrix is read here (FP64),
                 // by step 2's residual, rows in the pair's level-(l+1) consumption order
-                upload_sell<VT>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A, PP.qrow2.data());
+                upload_sell<double>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A64, PP.qrow2.data());
+                L.A.nnz = L.A64.nnz;                            // for the byte model
                 L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
                 L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
                 continue;
@@ -771,7 +771,7 @@ template <class VT> struct DevHier : DevPrecBase {
         return S;
     }
     // f = A_{l+1} (s u) - F_l, read row by row by the visit of level l+1
-    template <int IN> InSrc<VT, VT, IN> residual_src(uint32_t l, const VT* u, VT s, VT* us) const {
+    template <int IN> InSrc<VT, VT, IN> residual_src(uint32_t l, const VT* u, double s, VT* us) const {
         InSrc<VT, VT, IN> S;
         S.A = lv[l + 1].A.v;
         S.u = u;
@@ -788,25 +788,30 @@ template <class VT> struct DevHier : DevPrecBase {
     template <class FT, int IN, class XT>
     void pair_visit(const DevLevel<VT>& L, const DevLevel<VT>& Lc, const InSrc<VT, FT, IN>& f, XT* x,
                     const uint32_t* g) const {
-        const PairView<VT>& P = L.pair;
+        const PairView<VT, double>& P = L.pair;
         const LevelBatch lb = level_batch(L);
         const uint32_t g1 = P.nblk + cdiv(uint64_t(P.nflat2) + P.Md, L.lbs);
-        auto k1 = L.lbs == 64 ? (R > 1 ? k_pair_down<VT, FT, IN, XT, 64, true> : k_pair_down<VT, FT, IN, XT, 64, false>)
-                              : (R > 1 ? k_pair_down<VT, FT, IN, XT, kLBS, true>
-                                       : k_pair_down<VT, FT, IN, XT, kLBS, false>);
+        auto k1 = L.lbs == 64 ? (R > 1 ? k_pair_down<VT, double, FT, IN, XT, 64, true>
+                                       : k_pair_down<VT, double, FT, IN, XT, 64, false>)
+                              : (R > 1 ? k_pair_down<VT, double, FT, IN, XT, kLBS, true>
+                                       : k_pair_down<VT, double, FT, IN, XT, kLBS, false>);
         launch(k1, g1, L.lbs, 0, P, f, L.wt, L.F, L.u1, x, g, lb);
         const uint32_t g2 = P.nblk + cdiv(P.nflat2, L.lbs);
         const VT* wt = L.wt;
         if (L.s1_skip) {
-            auto k2 = L.lbs == 64 ? (R > 1 ? k_pair_up<VT, XT, false, 64, true> : k_pair_up<VT, XT, false, 64, false>)
-                                  : (R > 1 ? k_pair_up<VT, XT, false, kLBS, true> : k_pair_up<VT, XT, false, kLBS, false>);
-            launch(k2, g2, L.lbs, 0, P, Lc.A.v, (const VT*)L.u1, VT(1), (const VT*)L.F, wt, VT(L.ma2), L.s2_skip,
-                   VT(L.opb2), L.opb2_one, x, g, lb);
+            auto k2 = L.lbs == 64 ? (R > 1 ? k_pair_up<VT, double, XT, false, 64, true>
+                                           : k_pair_up<VT, double, XT, false, 64, false>)
+                                  : (R > 1 ? k_pair_up<VT, double, XT, false, kLBS, true>
+                                           : k_pair_up<VT, double, XT, false, kLBS, false>);
+            launch(k2, g2, L.lbs, 0, P, Lc.A64.v, (const VT*)L.u1, 1.0, (const VT*)L.F, wt, L.ma2, L.s2_skip,
+                   L.opb2, L.opb2_one, x, g, lb);
         } else {
-            auto k2 = L.lbs == 64 ? (R > 1 ? k_pair_up<VT, XT, true, 64, true> : k_pair_up<VT, XT, true, 64, false>)
-                                  : (R > 1 ? k_pair_up<VT, XT, true, kLBS, true> : k_pair_up<VT, XT, true, kLBS, false>);
-            launch(k2, g2, L.lbs, 0, P, Lc.A.v, (const VT*)L.u1, VT(L.a1), (const VT*)L.F, wt, VT(L.ma2),
-                   L.s2_skip, VT(L.opb2), L.opb2_one, x, g, lb);
+            auto k2 = L.lbs == 64 ? (R > 1 ? k_pair_up<VT, double, XT, true, 64, true>
+                                           : k_pair_up<VT, double, XT, true, 64, false>)
+                                  : (R > 1 ? k_pair_up<VT, double, XT, true, kLBS, true>
+                                           : k_pair_up<VT, double, XT, true, kLBS, false>);
+            launch(k2, g2, L.lbs, 0, P, Lc.A64.v, (const VT*)L.u1, L.a1, (const VT*)L.F, wt, L.ma2,
+                   L.s2_skip, L.opb2, L.opb2_one, x, g, lb);
         }
     }
 
@@ -826,17 +831,17 @@ template <class VT> struct DevHier : DevPrecBase {
         Combine<VT> C;
         if (L.nit == 1) {
             cheb(l + 1, perm(L.F), L.x2, g);
-            C.x2 = L.x2; C.a = VT(L.a1); C.skip = L.s1_skip;
+            C.x2 = L.x2; C.a = L.a1; C.skip = L.s1_skip;
             up<XT, OP_SCALE, 0>(L, C, x, g);
             return;
         }
         cheb(l + 1, perm(L.F), L.u1, g);
         // step 2 on tmp = A (a1 u1) - F, with u1s = a1 u1 kept in u0
-        if (L.s1_skip) cheb(l + 1, residual_src<IN_RES>(l, L.u1, VT(1), nullptr), L.x2, g);
-        else cheb(l + 1, residual_src<IN_RES_SCALE>(l, L.u1, VT(L.a1), L.u0), L.x2, g);
+        if (L.s1_skip) cheb(l + 1, residual_src<IN_RES>(l, L.u1, 1.0, nullptr), L.x2, g);
+        else cheb(l + 1, residual_src<IN_RES_SCALE>(l, L.u1, L.a1, L.u0), L.x2, g);
         VT* u1s = L.s1_skip ? L.u1 : L.u0;
         if (L.nit == 2) {
-            C.x2 = L.x2; C.u1 = u1s; C.a = VT(L.ma2); C.skip = L.s2_skip; C.b = VT(L.opb2); C.one = L.opb2_one;
+            C.x2 = L.x2; C.u1 = u1s; C.a = L.ma2; C.skip = L.s2_skip; C.b = L.opb2; C.one = L.opb2_one;
             up<XT, OP_CHEB2, 0>(L, C, x, g);
             return;
         }
@@ -845,13 +850,13 @@ template <class VT> struct DevHier : DevPrecBase {
         VT* spare = nullptr;
         for (VT* b : {L.u1, L.u0}) if (b != prev) spare = b;
         auto cheb2 = R > 1 ? k_cheb2<VT, XT, true, true> : k_cheb2<VT, XT, true, false>;
-        launch(cheb2, gb, kBS, 0, n, (const VT*)L.x2, (const VT*)prev, VT(L.ma2), L.s2_skip, VT(L.opb2),
+        launch(cheb2, gb, kBS, 0, n, (const VT*)L.x2, (const VT*)prev, L.ma2, L.s2_skip, L.opb2,
                L.opb2_one, (const uint32_t*)nullptr, (XT*)nullptr, spare, g, level_batch(L));
         VT* cur = spare;
         spare = L.x2;
         for (uint32_t i = 3; i <= L.nit; i++) {
-            cheb(l + 1, residual_src<IN_RES>(l, cur, VT(1), nullptr), spare, g);
-            const VT a = VT(L.ak[i - 3]), b = VT(L.bk[i - 3]);
+            cheb(l + 1, residual_src<IN_RES>(l, cur, 1.0, nullptr), spare, g);
+            const double a = L.ak[i - 3], b = L.bk[i - 3];
             if (i == L.nit) {
                 C.x2 = spare; C.u1 = cur; C.u0 = prev; C.a = a; C.b = b;
                 up<XT, OP_CHEBK, 0>(L, C, x, g);

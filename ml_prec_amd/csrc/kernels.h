@@ -197,8 +197,11 @@ __device__ __forceinline__ VT stencil_row(const SellView<VT>& A, const XT* __res
     return acc;
 }
 
+// SCALE: the row of A (s x).  An FP64 row scales every entry first (the
+// reference's dscal then multiply, bit for bit); an FP32 row scales its sum.
 template <class VT, class XT, bool SCALE>
-__device__ __forceinline__ VT sell_row(const SellView<VT>& A, const XT* __restrict__ x, uint32_t r, VT s) {
+__device__ __forceinline__ VT sell_row(const SellView<VT>& A, const XT* __restrict__ x, uint32_t r, double s) {
+    constexpr bool kScaleEntries = sizeof(VT) == sizeof(double);
     constexpr uint32_t G = 8;
     const uint32_t sl = r >> 6, lane = r & 63;
     const uint32_t b = A.off64[sl], e = A.off64[sl + 1];
@@ -230,11 +233,13 @@ __device__ __forceinline__ VT sell_row(const SellView<VT>& A, const XT* __restri
         for (uint32_t j = 0; j < G; j++) {
             if (ok[j]) {
                 VT t = VT(xv[j]);
-                if (SCALE) t = t * s;
+                if (SCALE && kScaleEntries) t = VT(double(t) * s);   // the reference's dscal, then the row
                 acc += v[j] * t;
             }
         }
     }
+    // below FP64 the Chebyshev coefficient is applied once to the row, in FP64: A (s u) = s (A u)
+    if (SCALE && !kScaleEntries) acc = VT(double(acc) * s);
     return acc;
 }
 
@@ -242,7 +247,7 @@ __device__ __forceinline__ VT sell_row(const SellView<VT>& A, const XT* __restri
 // matrix; the level kernels' fused residuals only ever see slices).
 template <class VT, class XT>
 __device__ __forceinline__ VT mat_row(const SellView<VT>& A, const XT* __restrict__ x, uint32_t r) {
-    return A.sd ? stencil_row<VT, XT, false>(A, x, r, VT(1)) : sell_row<VT, XT, false>(A, x, r, VT(1));
+    return A.sd ? stencil_row<VT, XT, false>(A, x, r, VT(1)) : sell_row<VT, XT, false>(A, x, r, 1.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -311,20 +316,23 @@ template <class VT> struct Combine {
     const VT* x2 = nullptr;
     const VT* u1 = nullptr;
     const VT* u0 = nullptr;
-    VT a = VT(0), b = VT(0);
+    double a = 0.0, b = 0.0;             // Chebyshev coefficients, applied in FP64 (one rounding to VT)
     int skip = 0, one = 0;
 };
 
 template <class VT, int OP>
 __device__ __forceinline__ VT combine(const Combine<VT>& C, uint32_t r) {
     if (OP == OP_COPY) return C.x2[r];
-    if (OP == OP_SCALE) return C.skip ? C.x2[r] : C.x2[r] * C.a;                  // dscal(alpha, x2)
+    if (OP == OP_SCALE) return C.skip ? C.x2[r] : VT(double(C.x2[r]) * C.a);      // dscal(alpha, x2)
     if (OP == OP_CHEB2) {                                                          // dscal(-alpha,x2); daxpy(1+beta,u1,x2)
-        VT v = C.skip ? C.x2[r] : C.x2[r] * C.a;
-        const VT u = C.u1[r];
-        return C.one ? v + u : v + C.b * u;
+        const double v = C.skip ? double(C.x2[r]) : double(C.x2[r]) * C.a;
+        const double u = double(C.u1[r]);
+        return VT(C.one ? v + u : v + C.b * u);
     }
-    if (OP == OP_CHEBK) return C.u1[r] - C.a * C.x2[r] + C.b * (C.u1[r] - C.u0[r]);
+    if (OP == OP_CHEBK) {
+        const double u1 = double(C.u1[r]), x2 = double(C.x2[r]), u0 = double(C.u0[r]);
+        return VT(u1 - C.a * x2 + C.b * (u1 - u0));
+    }
     return VT(0);
 }
 
@@ -347,7 +355,7 @@ template <class VT, class FT, int IN> struct InSrc {
     const VT* u = nullptr;
     const VT* F = nullptr;
     VT* us = nullptr;
-    VT s = VT(1);
+    double s = 1.0;                      // Chebyshev coefficient (FP64)
 };
 
 template <class VT, class FT, int IN>
@@ -355,7 +363,7 @@ __device__ __forceinline__ VT fetch(const InSrc<VT, FT, IN>& S, uint32_t q, uint
     if (IN == IN_PLAIN) return VT(S.f[j]);
     if (IN == IN_PERM) return VT(S.f[q]);
     const VT acc = sell_row<VT, VT, IN == IN_RES_SCALE>(S.A, S.u, q, S.s);
-    if (IN == IN_RES_SCALE) S.us[j] = S.u[j] * S.s;
+    if (IN == IN_RES_SCALE) S.us[j] = VT(double(S.u[j]) * S.s);
     return acc - S.F[q];
 }
 
@@ -828,7 +836,9 @@ __global__ void __launch_bounds__(BS) k_level_solve(ProgView<VT> P, InSrc<VT, FT
 // (cheb_solve.cc:33-104 on level l, :108-122 on level l+1); every row keeps
 // the arithmetic of k_level_down / k_level_solve / k_level_up.
 // ---------------------------------------------------------------------------
-template <class VT> struct PairView {
+// VT: level l's values; CT: the coarsest level's (FP64 in every hierarchy, so
+// that an FP32 hierarchy still solves its coarsest level in FP64).
+template <class VT, class CT> struct PairView {
     uint32_t nblk = 0, nodes = 0, nodes2 = 0, nflat2 = 0, Md = 0;
     const uint32_t* blk = nullptr;
     const uint32_t* qrow = nullptr;
@@ -838,11 +848,11 @@ template <class VT> struct PairView {
     const VT *Uval = nullptr, *Udiag = nullptr;
     const uint32_t *src2 = nullptr, *row2 = nullptr, *xl2 = nullptr, *Ldep2 = nullptr, *Lptr2 = nullptr,
                    *Lloc2 = nullptr;
-    const VT *Lval2 = nullptr, *d2 = nullptr;
+    const CT *Lval2 = nullptr, *d2 = nullptr;
     const uint32_t *Udep2 = nullptr, *uloc2 = nullptr, *Uptr2 = nullptr, *Ucol2 = nullptr;
-    const VT *Uval2 = nullptr, *Udiag2 = nullptr;
+    const CT *Uval2 = nullptr, *Udiag2 = nullptr;
     const uint32_t *frow2 = nullptr, *fxl = nullptr;
-    const VT* fd2 = nullptr;
+    const CT* fd2 = nullptr;
     const VT* dval = nullptr;          // level l's isolated rows
 };
 constexpr uint32_t kPairHdr = 12;          // header words per block of a pair program
@@ -954,8 +964,8 @@ __device__ __forceinline__ VT sweep_U(const OwnU<VT>& o, uint32_t nl, const uint
 // block: writes w of level l's eliminated rows (wt), F of every kept row (at
 // its level-(l+1) consumption position), u1 = B_{l+1} F (level-(l+1) rows)
 // and x = dval f of level l's isolated rows.
-template <class VT, class FT, int IN, class XT, int BS, bool BAT>
-__global__ void __launch_bounds__(BS) k_pair_down(PairView<VT> P, InSrc<VT, FT, IN> f, VT* __restrict__ wt,
+template <class VT, class CT, class FT, int IN, class XT, int BS, bool BAT>
+__global__ void __launch_bounds__(BS) k_pair_down(PairView<VT, CT> P, InSrc<VT, FT, IN> f, VT* __restrict__ wt,
                                                     VT* __restrict__ F, VT* __restrict__ u1, XT* __restrict__ x,
                                                     const uint32_t* guard, LevelBatch lb) {
     if constexpr (BAT) {
@@ -966,7 +976,8 @@ __global__ void __launch_bounds__(BS) k_pair_down(PairView<VT> P, InSrc<VT, FT, 
     }
     const uint32_t gflag = guard ? *guard : 0u;
     if (blockIdx.x < P.nblk) {
-        __shared__ VT sw[BS], s2w[BS], s2x[BS];
+        __shared__ VT sw[BS];
+        __shared__ CT s2w[BS], s2x[BS];
         const uint32_t* B = P.blk + kPairHdr * blockIdx.x;
         uint32_t h[kPairHdr];
 #pragma unroll
@@ -975,10 +986,10 @@ __global__ void __launch_bounds__(BS) k_pair_down(PairView<VT> P, InSrc<VT, FT, 
         const uint32_t t = threadIdx.x;
         // every per-node load of both levels first, so that they overlap
         const OwnL<VT> oa = own_L(S, h[2], base, P.Ldep, P.Lptr, P.Lloc, P.Lval);
-        const OwnL<VT> ob = own_L(nk, h[8], base2, P.Ldep2, P.Lptr2, P.Lloc2, P.Lval2);
-        const OwnU<VT> ou = own_U(h[10], h[11], h[9], P.Udep2, P.uloc2, P.Udiag2, P.Uptr2, P.Ucol2, P.Uval2);
+        const OwnL<CT> ob = own_L(nk, h[8], base2, P.Ldep2, P.Lptr2, P.Lloc2, P.Lval2);
+        const OwnU<CT> ou = own_U(h[10], h[11], h[9], P.Udep2, P.uloc2, P.Udiag2, P.Uptr2, P.Ucol2, P.Uval2);
         uint32_t src = 0, row = 0;
-        VT dd = VT(0);
+        CT dd = CT(0);
         if (t < nk) {
             src = P.src2[base2 + t];
             row = P.row2[base2 + t];
@@ -991,14 +1002,15 @@ __global__ void __launch_bounds__(BS) k_pair_down(PairView<VT> P, InSrc<VT, FT, 
         const VT w = sweep_L<BS>(oa, h[2], P.Lloc, P.Lval, acc, sw);
         if (!gflag && t < S && k2 == kPairNoRef) wt[base + t] = w;
         // step 1 on level l+1: its right-hand side is the w of the block's kept rows
-        const VT f2 = t < nk ? sw[src] : VT(0);
-        if (!gflag && t < nk) F[base2 + t] = f2;
+        const VT fv = t < nk ? sw[src] : VT(0);
+        if (!gflag && t < nk) F[base2 + t] = fv;
+        const CT f2 = CT(fv);
         sweep_L<BS>(ob, h[8], P.Lloc2, P.Lval2, f2, s2w);
-        const VT z = t < h[10] ? s2w[ou.loc] : VT(0);
-        const VT v = sweep_U<BS, VT, false>(ou, h[11], P.Ucol2, P.Uval2, z, (const VT*)nullptr, s2x);
+        const CT z = t < h[10] ? s2w[ou.loc] : CT(0);
+        const CT v = sweep_U<BS, CT, false>(ou, h[11], P.Ucol2, P.Uval2, z, (const CT*)nullptr, s2x);
         if (gflag) return;
-        if (t < h[10]) u1[vrow] = v;
-        if (t < nk && (row & kPairIso2)) u1[row & ~kPairIso2] = dd * f2;
+        if (t < h[10]) u1[vrow] = VT(v);
+        if (t < nk && (row & kPairIso2)) u1[row & ~kPairIso2] = VT(dd * f2);
     } else {
         const uint32_t i = (blockIdx.x - P.nblk) * BS + threadIdx.x;
         if (i < P.nflat2) {
@@ -1006,7 +1018,7 @@ __global__ void __launch_bounds__(BS) k_pair_down(PairView<VT> P, InSrc<VT, FT, 
             const VT v = fetch(f, q, P.qrow[q]);
             if (!gflag) {
                 F[P.nodes2 + i] = v;
-                u1[P.frow2[i]] = P.fd2[i] * v;
+                u1[P.frow2[i]] = VT(P.fd2[i] * CT(v));
             }
         } else if (i < P.nflat2 + P.Md) {
             const uint32_t k = i - P.nflat2, q = P.nodes + i, j = P.qrow[q];
@@ -1018,18 +1030,19 @@ __global__ void __launch_bounds__(BS) k_pair_down(PairView<VT> P, InSrc<VT, FT, 
 
 // step 2's combine of a kept row: x2 = (-alpha) x2 + (1+beta) u1s (cheb_solve.cc:79-81), as combine<OP_CHEB2>
 template <class VT>
-__device__ __forceinline__ VT pair_cheb2(VT x2, VT u, VT ma, int skip, VT opb, int opb_one) {
-    const VT v = skip ? x2 : x2 * ma;
-    return opb_one ? v + u : v + opb * u;
+__device__ __forceinline__ VT pair_cheb2(VT x2, VT u, double ma, int skip, double opb, int opb_one) {
+    const double v = skip ? double(x2) : double(x2) * ma;
+    const double w = double(u);
+    return VT(opb_one ? v + w : v + opb * w);
 }
 
 // K2: step 2 on level l+1 from the residual A_{l+1} (s u1) - F (A2 stored in
 // level (l+1)'s consumption order), the Chebyshev combine of step 2
 // (cheb_solve.cc:79-81) for every kept row, and level l's U sweep.
-template <class VT, class XT, bool SCALE, int BS, bool BAT>
-__global__ void __launch_bounds__(BS) k_pair_up(PairView<VT> P, SellView<VT> A2, const VT* __restrict__ u1, VT s,
-                                                  const VT* __restrict__ F, const VT* __restrict__ wt, VT ma,
-                                                  int skip, VT opb, int opb_one, XT* __restrict__ x,
+template <class VT, class CT, class XT, bool SCALE, int BS, bool BAT>
+__global__ void __launch_bounds__(BS) k_pair_up(PairView<VT, CT> P, SellView<CT> A2, const VT* __restrict__ u1, double s,
+                                                  const VT* __restrict__ F, const VT* __restrict__ wt, double ma,
+                                                  int skip, double opb, int opb_one, XT* __restrict__ x,
                                                   const uint32_t* guard, LevelBatch lb) {
     if constexpr (BAT) {
         const size_t y = blockIdx.y;
@@ -1038,7 +1051,8 @@ __global__ void __launch_bounds__(BS) k_pair_up(PairView<VT> P, SellView<VT> A2,
     }
     const uint32_t gflag = guard ? *guard : 0u;
     if (blockIdx.x < P.nblk) {
-        __shared__ VT s2w[BS], s2x[BS], s2c[BS], sx[BS];
+        __shared__ CT s2w[BS], s2x[BS];
+        __shared__ VT s2c[BS], sx[BS];
         const uint32_t* B = P.blk + kPairHdr * blockIdx.x;
         uint32_t h[kPairHdr];
 #pragma unroll
@@ -1046,29 +1060,29 @@ __global__ void __launch_bounds__(BS) k_pair_up(PairView<VT> P, SellView<VT> A2,
         const uint32_t base = h[0], base2 = h[6], nk = h[7];
         const uint32_t t = threadIdx.x;
         // every per-node load first: level l+1's sweeps and level l's U sweep with its w
-        const OwnL<VT> ob = own_L(nk, h[8], base2, P.Ldep2, P.Lptr2, P.Lloc2, P.Lval2);
-        const OwnU<VT> o2 = own_U(h[10], h[11], h[9], P.Udep2, P.uloc2, P.Udiag2, P.Uptr2, P.Ucol2, P.Uval2);
+        const OwnL<CT> ob = own_L(nk, h[8], base2, P.Ldep2, P.Lptr2, P.Lloc2, P.Lval2);
+        const OwnU<CT> o2 = own_U(h[10], h[11], h[9], P.Udep2, P.uloc2, P.Udiag2, P.Uptr2, P.Ucol2, P.Uval2);
         const OwnU<VT> oa = own_U(h[4], h[5], h[3], P.Udep, P.uloc, P.Udiag, P.Uptr, P.Ucol, P.Uval);
         const VT za = t < h[4] ? wt[base + oa.loc] : VT(0);
         const uint32_t xa = t < h[4] ? P.uxidx[h[3] + t] : 0u;
         uint32_t row = 0, xl = 0;
-        VT dd = VT(0), f2 = VT(0), us = VT(0);
+        CT dd = CT(0), f2 = CT(0), us = CT(0);
         if (t < nk) {
             row = P.row2[base2 + t];
             xl = P.xl2[base2 + t];
             dd = P.d2[base2 + t];
             const uint32_t r = row & ~kPairIso2;
-            f2 = sell_row<VT, VT, SCALE>(A2, u1, base2 + t, s) - F[base2 + t];
-            us = SCALE ? u1[r] * s : u1[r];
+            f2 = sell_row<CT, VT, SCALE>(A2, u1, base2 + t, s) - CT(F[base2 + t]);
+            us = SCALE ? CT(double(u1[r]) * s) : CT(u1[r]);
         }
         sweep_L<BS>(ob, h[8], P.Lloc2, P.Lval2, f2, s2w);
         if (t < nk && (row & kPairIso2)) s2x[t] = dd * f2;
-        const VT z2 = t < h[10] ? s2w[o2.loc] : VT(0);
-        sweep_U<BS, VT, false>(o2, h[11], P.Ucol2, P.Uval2, z2, (const VT*)nullptr, s2x);
+        const CT z2 = t < h[10] ? s2w[o2.loc] : CT(0);
+        sweep_U<BS, CT, false>(o2, h[11], P.Ucol2, P.Uval2, z2, (const CT*)nullptr, s2x);
         // the value of every kept row: the combine of step 2
         VT c = VT(0);
         if (t < nk) {
-            c = pair_cheb2(s2x[t], us, ma, skip, opb, opb_one);
+            c = VT(pair_cheb2(s2x[t], us, ma, skip, opb, opb_one));
             s2c[t] = c;
         }
         __syncthreads();
@@ -1080,10 +1094,10 @@ __global__ void __launch_bounds__(BS) k_pair_up(PairView<VT> P, SellView<VT> A2,
         const uint32_t i = (blockIdx.x - P.nblk) * BS + threadIdx.x;
         if (i < P.nflat2) {
             const uint32_t q = P.nodes2 + i, r = P.frow2[i];
-            const VT f2 = sell_row<VT, VT, SCALE>(A2, u1, q, s) - F[q];
-            const VT x2 = P.fd2[i] * f2;
-            const VT us = SCALE ? u1[r] * s : u1[r];
-            const XT v = XT(pair_cheb2(x2, us, ma, skip, opb, opb_one));
+            const CT f2 = sell_row<CT, VT, SCALE>(A2, u1, q, s) - CT(F[q]);
+            const CT x2 = P.fd2[i] * f2;
+            const CT us = SCALE ? CT(double(u1[r]) * s) : CT(u1[r]);
+            const XT v = XT(VT(pair_cheb2(x2, us, ma, skip, opb, opb_one)));
             if (!gflag) x[P.fxl[i]] = v;
         }
     }
@@ -1092,7 +1106,7 @@ __global__ void __launch_bounds__(BS) k_pair_up(PairView<VT> P, SellView<VT> A2,
 // step 2: x2 = (-alpha) x2 ; x2 += (1+beta) u1   (cheb_solve.cc:79-81), then store or scatter
 template <class VT, class XT, bool STORE, bool BAT>
 __global__ void __launch_bounds__(kBS) k_cheb2(uint32_t n, const VT* __restrict__ x2, const VT* __restrict__ u1s,
-                                               VT ma, int skip, VT opb, int opb_one,
+                                               double ma, int skip, double opb, int opb_one,
                                                const uint32_t* __restrict__ mapM, XT* __restrict__ x,
                                                VT* __restrict__ x2out, const uint32_t* guard, LevelBatch lb) {
     if constexpr (BAT) {                 // batched right-hand sides (compiled out for single solves)
@@ -1103,18 +1117,18 @@ __global__ void __launch_bounds__(kBS) k_cheb2(uint32_t n, const VT* __restrict_
     if (guarded(guard)) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    VT v = x2[i];
+    double v = double(x2[i]);
     if (!skip) v = v * ma;
-    const VT u = u1s[i];
-    v = opb_one ? v + u : v + opb * u;
-    if (STORE) x2out[i] = v;
-    else x[mapM[i]] = XT(v);
+    const double u = double(u1s[i]);
+    const VT w = VT(opb_one ? v + u : v + opb * u);
+    if (STORE) x2out[i] = w;
+    else x[mapM[i]] = XT(w);
 }
 
 // steps >= 3: x2 = u1 - alpha x2 + beta (u1 - u0)   (cheb_solve.cc:99-100)
 template <class VT, class XT, bool STORE, bool BAT>
 __global__ void __launch_bounds__(kBS) k_cheb_step(uint32_t n, const VT* __restrict__ u1, const VT* __restrict__ u0,
-                                                   const VT* __restrict__ x2, VT a, VT b,
+                                                   const VT* __restrict__ x2, double a, double b,
                                                    const uint32_t* __restrict__ mapM, XT* __restrict__ x,
                                                    VT* __restrict__ x2out, const uint32_t* guard, LevelBatch lb) {
     if constexpr (BAT) {                 // batched right-hand sides (compiled out for single solves)
@@ -1125,7 +1139,8 @@ __global__ void __launch_bounds__(kBS) k_cheb_step(uint32_t n, const VT* __restr
     if (guarded(guard)) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const VT v = u1[i] - a * x2[i] + b * (u1[i] - u0[i]);
+    const double p1 = double(u1[i]), p0 = double(u0[i]);
+    const VT v = VT(p1 - a * double(x2[i]) + b * (p1 - p0));
     if (STORE) x2out[i] = v;
     else x[mapM[i]] = XT(v);
 }
