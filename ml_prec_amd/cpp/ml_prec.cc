@@ -180,8 +180,17 @@ void dump(const std::string& filename, const CSRMatrix& A, DumpType type) {
         os << "%%MatrixMarket matrix coordinate real general" << std::endl << m << " " << n << " " << nz << std::endl;
         for (uint i = 0; i < m; i++)
             for (uint j = ia[i]; j < ia[i + 1]; j++) os << i + 1 << " " << ja[j] + 1 << " " << a[j] << std::endl;
+    } else if (type == HYPRE) {
+        // IJ format of a Skyline matrix: "ilower iupper jlower jupper", then "i j a" per entry in storage order
+        if (!dynamic_cast<const SkylineMatrix*>(&A))
+            THROW_EXCEPTION("Dumping matrix in HYPRE format is not supported for CSRMatrix");
+        std::ofstream os(filename.c_str());
+        if (!os.good()) THROW_EXCEPTION("Could not open \"" << filename << "\" for write");
+        os << "0 " << m - 1 << " 0 " << n - 1 << std::endl << std::scientific << std::setprecision(15);
+        for (uint i = 0; i < m; i++)
+            for (uint j = ia[i]; j < ia[i + 1]; j++) os << i << " " << ja[j] << " " << a[j] << std::endl;
     } else {
-        THROW_EXCEPTION("Dumping matrix in HYPRE format is not supported");
+        THROW_EXCEPTION("Unknown dump type");
     }
 }
 

@@ -21,3 +21,59 @@ def test_tail_walk_rejects_inconsistent_link_counters(tail_walk_exe):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Row 0: could not find remaining link" in r.stdout
     assert r.stdout.startswith("OK 6 ")                    # MLP_ERR_CONSTRUCT
+
+
+def _fmt_e(v):
+    return "%.15e" % v
+
+
+@pytest.fixture(scope="module")
+def io_exe(tmp_path_factory):
+    import ml_prec_amd.native as N
+    out = str(tmp_path_factory.mktemp("io") / "io_check")
+    src = os.path.join(HERE, "cpp", "io_check.cc")
+    libdir = os.path.dirname(N.LIB_PATH)
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-o", out, src, "-L" + libdir, "-lmlp", "-Wl,-rpath," + libdir],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return out
+
+
+def test_matrix_file_formats(io_exe, tmp_path):
+    """BINARY, ASCII, MatrixMarket and HYPRE as csrmatrix.cc:172-355 writes and reads them: the dumps' text,
+    exactly, and the round trips through the readers (the MatrixMarket reader skips a comment line after the
+    header, which the reference's own dump does not write, so the test inserts one)."""
+    import numpy as np
+    from conftest import system
+    from ml_prec_amd import spe_synth as S
+    s = system("tiny")
+    a = s.a.copy()
+    a[::7] *= 1.0 + 2.0 ** -40                       # values that need all 16 printed digits
+    binp = str(tmp_path / "A.bin")
+    S.write_csr_binary(binp, s.ia, s.ja, a)
+    prefix = str(tmp_path / "A")
+    # the MatrixMarket file with the comment line, entries in reverse order and one duplicate (later wins)
+    n = s.n
+    ent = [(i + 1, int(s.ja[p]) + 1, float(a[p])) for i in range(n) for p in range(s.ia[i], s.ia[i + 1])]
+    first = ent[0]
+    lines = ["%%MatrixMarket matrix coordinate real general", "% written by the test", f"{n} {n} {len(ent) + 1}"]
+    lines += [f"{i} {j} {v!r}" for i, j, v in [(first[0], first[1], 99.0)] + ent[::-1]]
+    open(prefix + ".mmc", "w").write("\n".join(lines) + "\n")
+    r = subprocess.run([io_exe, binp, prefix], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    res = dict(line.split() for line in r.stdout.splitlines())
+    assert res == {"ascii_roundtrip": "1", "binary_roundtrip": "1", "hypre_csr_refused": "1",
+                   "hypre_load_refused": "1", "mm_roundtrip": "1"}, res
+    nnz = int(s.ia[-1])
+    # ASCII: three commented sections, values in scientific notation with 15 decimals (csrmatrix.cc:322-334)
+    want = ["# rows cols nonzeros", f"{n} {n} {nnz}", "# row_ptr"] + [str(v) for v in s.ia] + ["# col_ind"]
+    want += [str(v) for v in s.ja] + ["# value"] + [_fmt_e(v) for v in a]
+    assert open(prefix + ".ascii").read().splitlines() == want
+    # MatrixMarket: 1-based, fixed notation with 15 decimals (csrmatrix.cc:346-353)
+    want = ["%%MatrixMarket matrix coordinate real general", f"{n} {n} {nnz}"]
+    want += ["%d %d %.15f" % (i, j, v) for i, j, v in ent]
+    assert open(prefix + ".mm").read().splitlines() == want
+    # HYPRE: index ranges, then 0-based "i j a" in storage order, scientific (csrmatrix.cc:335-345)
+    want = [f"0 {n - 1} 0 {n - 1}"] + ["%d %d %s" % (i - 1, j - 1, _fmt_e(v)) for i, j, v in ent]
+    assert open(prefix + ".hypre").read().splitlines() == want
+    assert np.isfinite(a).all()
