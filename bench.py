@@ -620,8 +620,9 @@ def level_descs(H, first, s):
 def error_budget(N, S, args):
     """Error of the FP32 hierarchy level by level: for each level l the hierarchy from l down to the coarsest
     (whose solve is FP64 in both) is uploaded in FP64 and in FP32 and applied to the same seeded vectors; the
-    relative difference of the two results is the error the FP32 storage and arithmetic of levels l.. add.  Also
-    the apply times and, for the whole hierarchy, the effect on PCG."""
+    relative difference of the two results is the error the FP32 storage and arithmetic of levels l.. add.  The
+    same for the BF16 variant (level l's Chebyshev vectors in BF16 on top of FP32).  Also the apply times and, for
+    the whole hierarchy, the effect on PCG to 1e-8 and 1e-10 (iterations, solution error against FP64)."""
     s = S.spe10_system(kind="spe", c=1.0, nz=args.nz)
     H = N.build_uh_cheb(s.ia, s.ja, s.a)
     ctx = N.Context(0)
@@ -647,8 +648,25 @@ def error_budget(N, S, args):
         row = {"level": l, "N": n, "fp32_rel_err_max": max(errs), "fp32_rel_err_mean": float(np.mean(errs)),
                "apply_us_fp64": round(t64 * 1e3, 2), "apply_us_fp32": round(t32 * 1e3, 2),
                "launches": B64.launches_per_apply()}
+        msg = ""
+        if l + 1 < H.nlevels:
+            # BF16 on this level's Chebyshev vectors (the variant applied to the hierarchy from level l)
+            Bbf = N.DevPrec.upload(ctx, A, N.UH_CHEB, levels, N.MLP_FP32_BF16)
+            ebf = []
+            ybf = ctx.array(n)
+            for k in range(4):
+                f = ctx.to_device(g.standard_normal(n))
+                B64.apply(f, y64)
+                Bbf.apply(f, ybf)
+                a64 = y64.get()
+                ebf.append(float(np.linalg.norm(ybf.get() - a64) / np.linalg.norm(a64)))
+            tbf = min(Bbf.time_apply(f, ybf, reps=20) for _ in range(3))
+            row.update(bf16_rel_err_max=max(ebf), bf16_rel_err_mean=float(np.mean(ebf)),
+                       apply_us_fp32_bf16=round(tbf * 1e3, 2))
+            msg = f"; BF16 Chebyshev vectors rel err {max(ebf):.3e}, apply {tbf * 1e3:.1f} us"
+            Bbf.close()
         log(f"levels {l}..{H.nlevels - 1}: N={n} FP32 operator rel err {max(errs):.3e}; apply FP64 "
-            f"{t64 * 1e3:.1f} us, FP32 {t32 * 1e3:.1f} us")
+            f"{t64 * 1e3:.1f} us, FP32 {t32 * 1e3:.1f} us" + msg)
         rows.append(row)
         B32.close()
         B64.close()
@@ -657,24 +675,30 @@ def error_budget(N, S, args):
     A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
     B64 = N.DevPrec(ctx, A, H, precision=N.MLP_FP64)
     B32 = N.DevPrec(ctx, A, H, precision=N.MLP_FP32)
+    Bbf = N.DevPrec(ctx, A, H, precision=N.MLP_FP32_BF16)
     b = ctx.to_device(s.b)
     x = ctx.array(s.n)
     pcg = {}
     for eps in (1e-8, 1e-10):
+        e = {}
         st64 = N.pcg(ctx, A, B64, b, x, eps=eps, maxit=10000, use_graph=True, check_every=8)
         x64 = x.get()
-        st32 = N.pcg(ctx, A, B32, b, x, eps=eps, maxit=10000, flexible=True, true_res_every=20, use_graph=True,
-                     check_every=8)
-        x32 = x.get()
-        pcg[f"{eps:g}"] = {"niter_fp64": int(st64["niter"]), "niter_mixed": int(st32["niter"]),
-                           "true_rel_res_mixed": st32["true_rel_res"],
-                           "mixed_rel_err_vs_fp64": float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))}
-        log(f"PCG to {eps:g}: FP64 {st64['niter']} it, mixed {st32['niter']} it, mixed vs FP64 "
-            f"{pcg[f'{eps:g}']['mixed_rel_err_vs_fp64']:.3e}")
+        e["niter_fp64"] = int(st64["niter"])
+        for name, B in (("mixed", B32), ("mixed_bf16", Bbf)):
+            st = N.pcg(ctx, A, B, b, x, eps=eps, maxit=10000, flexible=True, true_res_every=20, use_graph=True,
+                       check_every=8)
+            xm = x.get()
+            e[f"niter_{name}"] = int(st["niter"])
+            e[f"true_rel_res_{name}"] = st["true_rel_res"]
+            e[f"{name}_rel_err_vs_fp64"] = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+        pcg[f"{eps:g}"] = e
+        log(f"PCG to {eps:g}: FP64 {e['niter_fp64']} it, mixed {e['niter_mixed']} it (vs FP64 "
+            f"{e['mixed_rel_err_vs_fp64']:.3e}), mixed+BF16 {e['niter_mixed_bf16']} it (vs FP64 "
+            f"{e['mixed_bf16_rel_err_vs_fp64']:.3e})")
     print(json.dumps({"error_budget": rows, "pcg": pcg,
                       "workload": f"SPE10-shaped {s.nx}x{s.ny}x{s.nz}, uh_cheb sigma 3 ncheb 2, "
                                   f"{H.nlevels} levels"}), flush=True)
-    for o in (B32, B64, A):
+    for o in (Bbf, B32, B64, A):
         o.close()
     del b, x
     ctx.close()

@@ -43,6 +43,10 @@ extern "C" {
 /* ---- precision of a device preconditioner hierarchy ---- */
 #define MLP_FP64 0
 #define MLP_FP32 1
+/* FP32 hierarchy whose level 0 keeps the vectors of its Chebyshev iteration on level 1 (F, u1, a1 u1,
+ * x2; cheb_solve.cc:45-101) in BF16; the kernels compute in FP32 and round to BF16 only where they store
+ * those vectors.  uh_cheb with two or more levels and ncheb = 2 on level 0. */
+#define MLP_FP32_BF16 2
 
 /* ---- preconditioner kinds ---- */
 #define MLP_PREC_UH_CHEB     1  /* Prec, "uh_cheb"           modules/prec/cheb/cheb_prec.h:13 */

@@ -218,6 +218,8 @@ template <class VT> struct DevLevel {
                                                // in the order the level kernels read them
     VT *wt = nullptr, *F = nullptr, *u1 = nullptr, *u0 = nullptr, *x2 = nullptr;
     VT* r = nullptr;                           // multi_split inner residual
+    // level 0 of the BF16 variant: F, u1, a1 u1 and x2 of the Chebyshev iteration on level 1 in BF16
+    hip_bfloat16 *bF = nullptr, *bu1 = nullptr, *bu0 = nullptr, *bx2 = nullptr;
     // bookkeeping for the byte model
     uint32_t nnzL = 0, nnzU = 0;
     // the level run fused with the coarsest level (two launches per visit); the coarsest level's values
@@ -375,6 +377,7 @@ template <class VT> struct DevHier : DevPrecBase {
     uint32_t last_launches = 0;
     bool dynamic = false;
     int pair_l = -1;                         // level fused with the coarsest level (-1: none)
+    bool bf16 = false;                       // level 0 keeps its Chebyshev vectors in BF16 (FP32 uh_cheb)
 
     hipStream_t stream() const { return ctx->stream; }
 
@@ -517,6 +520,8 @@ template <class VT> struct DevHier : DevPrecBase {
             if (l + 1 == nlevels && n != 0) throw Error("the coarsest level must have no kept rows", 1);
             if (l + 1 < nlevels && d[l + 1].N != n) throw Error("level sizes do not chain", 1);
         }
+        if (bf16 && (kind != MLP_PREC_UH_CHEB || nlevels < 2 || d[0].nit != 2))
+            throw Error("the BF16 variant needs a uh_cheb hierarchy of two or more levels with ncheb = 2 on level 0", 1);
         // the levels are planned independently: one thread each
         std::vector<std::exception_ptr> err(nlevels);
         parallel_for(nlevels, 1, [&](uint64_t l0, uint64_t l1) {
@@ -540,7 +545,9 @@ template <class VT> struct DevHier : DevPrecBase {
         PairProgram PP;
         uint32_t pbs = 0;
         const char* nopair = getenv("MLP_NO_PAIR");
-        if (kind == MLP_PREC_UH_CHEB && nlevels >= 2 && d[nlevels - 2].nit == 2 && !(nopair && nopair[0] == '1')) {
+        // (the BF16 variant runs level 0 on its own kernels)
+        if (kind == MLP_PREC_UH_CHEB && nlevels >= 2 && d[nlevels - 2].nit == 2 && !(nopair && nopair[0] == '1') &&
+            !(bf16 && nlevels == 2)) {
             auto arrays = [&](uint32_t l) {
                 const mlp_level_desc& D = d[l];
                 LevelArrays la;
@@ -637,7 +644,12 @@ template <class VT> struct DevHier : DevPrecBase {
                 else
                     upload_sell<VT>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A);
             }
-            if (L.n) {
+            if (L.n && bf16 && l == 0) {
+                L.bF = scratch.alloc<hip_bfloat16>(L.n);
+                L.bu1 = scratch.alloc<hip_bfloat16>(L.n);
+                L.bu0 = scratch.alloc<hip_bfloat16>(L.n);
+                L.bx2 = scratch.alloc<hip_bfloat16>(L.n);
+            } else if (L.n) {
                 L.F = scratch.alloc<VT>(L.n);
                 L.u1 = scratch.alloc<VT>(L.n);
                 L.u0 = scratch.alloc<VT>(L.n);
@@ -683,6 +695,7 @@ template <class VT> struct DevHier : DevPrecBase {
         h->A0 = A0;
         h->dynamic = dynamic;
         h->pair_l = pair_l;
+        h->bf16 = bf16;
         h->env_lbs = env_lbs;
         h->env_cap = env_cap;
         h->lds_budget = lds_budget;
@@ -696,6 +709,12 @@ template <class VT> struct DevHier : DevPrecBase {
                 L.u1 = sc.template alloc<VT>(k * L.n);
                 L.u0 = sc.template alloc<VT>(k * L.n);
                 L.x2 = sc.template alloc<VT>(k * L.n);
+            }
+            if (L.bF) {
+                L.bF = sc.template alloc<hip_bfloat16>(k * L.n);
+                L.bu1 = sc.template alloc<hip_bfloat16>(k * L.n);
+                L.bu0 = sc.template alloc<hip_bfloat16>(k * L.n);
+                L.bx2 = sc.template alloc<hip_bfloat16>(k * L.n);
             }
             if (L.r) L.r = sc.template alloc<VT>(k * L.N);
         }
@@ -731,21 +750,27 @@ template <class VT> struct DevHier : DevPrecBase {
 
     // Each level kernel comes in four instances: one-wavefront (64) or kLBS
     // blocks, and with or without the batched right-hand-side offsets.
+    // (F: where the kept rows' right-hand side goes, in type FV)
+    template <class FT, int IN, class XT, int DMODE, class FV>
+    void down(const DevLevel<VT>& L, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g, FV* F) const {
+        const uint32_t grid = L.prog.nblk + cdiv(uint64_t(L.prog.nflat) + L.Md, L.lbs);
+        auto k = L.lbs == 64 ? (R > 1 ? k_level_down<VT, FT, IN, XT, DMODE, 64, true, FV>
+                                      : k_level_down<VT, FT, IN, XT, DMODE, 64, false, FV>)
+                             : (R > 1 ? k_level_down<VT, FT, IN, XT, DMODE, kLBS, true, FV>
+                                      : k_level_down<VT, FT, IN, XT, DMODE, kLBS, false, FV>);
+        launch(k, grid, L.lbs, L.lds_down, L.prog, f, L.wt, F, L.Md, (const VT*)L.dval, x, g, level_batch(L));
+    }
     template <class FT, int IN, class XT, int DMODE>
     void down(const DevLevel<VT>& L, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
-        const uint32_t grid = L.prog.nblk + cdiv(uint64_t(L.prog.nflat) + L.Md, L.lbs);
-        auto k = L.lbs == 64 ? (R > 1 ? k_level_down<VT, FT, IN, XT, DMODE, 64, true>
-                                      : k_level_down<VT, FT, IN, XT, DMODE, 64, false>)
-                             : (R > 1 ? k_level_down<VT, FT, IN, XT, DMODE, kLBS, true>
-                                      : k_level_down<VT, FT, IN, XT, DMODE, kLBS, false>);
-        launch(k, grid, L.lbs, L.lds_down, L.prog, f, L.wt, L.F, L.Md, (const VT*)L.dval, x, g, level_batch(L));
+        down<FT, IN, XT, DMODE, VT>(L, f, x, g, L.F);
     }
-    template <class XT, int OP, int UMODE>
-    void up(const DevLevel<VT>& L, const Combine<VT>& C, XT* x, const uint32_t* g) const {
+    template <class XT, int OP, int UMODE, class CV = VT>
+    void up(const DevLevel<VT>& L, const Combine<VT, CV>& C, XT* x, const uint32_t* g) const {
         const uint32_t grid = L.prog.nblk + cdiv(L.n, L.lbs);
-        auto k = L.lbs == 64 ? (R > 1 ? k_level_up<VT, XT, OP, UMODE, 64, true> : k_level_up<VT, XT, OP, UMODE, 64, false>)
-                             : (R > 1 ? k_level_up<VT, XT, OP, UMODE, kLBS, true>
-                                      : k_level_up<VT, XT, OP, UMODE, kLBS, false>);
+        auto k = L.lbs == 64 ? (R > 1 ? k_level_up<VT, XT, OP, UMODE, 64, true, CV>
+                                      : k_level_up<VT, XT, OP, UMODE, 64, false, CV>)
+                             : (R > 1 ? k_level_up<VT, XT, OP, UMODE, kLBS, true, CV>
+                                      : k_level_up<VT, XT, OP, UMODE, kLBS, false, CV>);
         launch(k, grid, L.lbs, L.lds_up, L.prog, (const VT*)L.wt, C, (const uint32_t*)(L.map + L.M), L.n, x, g,
                level_batch(L));
     }
@@ -813,6 +838,33 @@ template <class VT> struct DevHier : DevPrecBase {
             launch(k2, g2, L.lbs, 0, P, Lc.A64.v, (const VT*)L.u1, L.a1, (const VT*)L.F, wt, L.ma2,
                    L.s2_skip, L.opb2, L.opb2_one, x, g, lb);
         }
+    }
+
+    // Level 0 of the BF16 variant (ncheb = 2): as cheb(0, ...), with the
+    // vectors of the Chebyshev iteration on level 1 (F, u1, a1 u1, x2) stored
+    // in BF16.  Every kernel still computes in the level's own type; only the
+    // stores round to BF16 and the loads widen from it.
+    void cheb_bf16(const InSrc<VT, double, IN_PLAIN>& f, double* x, const uint32_t* g) const {
+        using BF = hip_bfloat16;
+        const DevLevel<VT>& L = lv[0];
+        down<double, IN_PLAIN, double, 0, BF>(L, f, x, g, L.bF);
+        InSrc<VT, BF, IN_PERM> pf;
+        pf.f = L.bF;
+        cheb(1, pf, L.bu1, g);
+        auto res = [&](auto S, BF* us, double s) {
+            S.A = lv[1].A.v;
+            S.u = L.bu1;
+            S.F = L.bF;
+            S.us = us;
+            S.s = s;
+            cheb(1, S, L.bx2, g);
+        };
+        if (L.s1_skip) res(InSrc<VT, BF, IN_RES>(), nullptr, 1.0);
+        else res(InSrc<VT, BF, IN_RES_SCALE>(), L.bu0, L.a1);
+        Combine<VT, BF> C;
+        C.x2 = L.bx2; C.u1 = L.s1_skip ? L.bu1 : L.bu0;
+        C.a = L.ma2; C.skip = L.s2_skip; C.b = L.opb2; C.one = L.opb2_one;
+        up<double, OP_CHEB2, 0>(L, C, x, g);
     }
 
     template <class FT, int IN, class XT>
@@ -926,7 +978,13 @@ template <class VT> struct DevHier : DevPrecBase {
         if (kind == MLP_PREC_UH_CHEB) {
             InSrc<VT, double, IN_PLAIN> fs;
             fs.f = f;
-            cheb(0, fs, x, guard);
+            bool done = false;
+            if constexpr (std::is_same_v<VT, float>)     // the BF16 variant exists for FP32 hierarchies only
+                if (bf16) {
+                    cheb_bf16(fs, x, guard);
+                    done = true;
+                }
+            if (!done) cheb(0, fs, x, guard);
         } else {
             msplit<double, double>(0, f, x, guard);
         }
@@ -945,12 +1003,16 @@ template <class VT> struct DevHier : DevPrecBase {
             const DevLevel<VT>& L = lv[l];
             const bool cheb_kind = kind == MLP_PREC_UH_CHEB;
             const double steps = cheb_kind ? 1.0 : double(std::max<uint32_t>(1, L.nit));
-            const double sweep = (w + 4) * (double(L.nnzL) + double(L.nnzU)) + 3 * w * L.N + 4.0 * L.N;
+            // BF16 variant: level 1 reads its input and writes its output in BF16 (2 bytes), and so do
+            // level 0's Chebyshev vectors
+            const double wio = bf16 && l == 1 ? 2.0 : w;
+            const double sweep = (w + 4) * (double(L.nnzL) + double(L.nnzU)) + (w + 2 * wio) * L.N + 4.0 * L.N;
             double b = steps * sweep;
             if (cheb_kind && l + 1 < nlevels) {
                 const DevLevel<VT>& Ln = lv[l + 1];
-                const double bsp = (w + 4) * Ln.A.nnz + 4.0 * (Ln.N + 1) + 2 * w * Ln.N;
-                b += (double(L.nit) - 1) * (bsp + 6 * w * Ln.N);
+                const double wv = bf16 && l == 0 ? 2.0 : w;
+                const double bsp = (w + 4) * Ln.A.nnz + 4.0 * (Ln.N + 1) + 2 * wv * Ln.N;
+                b += (double(L.nit) - 1) * (bsp + 6 * wv * Ln.N);
             }
             if (!cheb_kind && steps > 1) {
                 const double bsp = (w + 4) * L.A.nnz + 4.0 * (L.N + 1) + 2 * w * L.N;
@@ -1735,9 +1797,11 @@ int mlp_hier_level_array(mlp_hier h, uint32_t l, int which, void* out) {
 // ---- device preconditioners ----
 }  // extern "C"
 template <class VT>
-static DevPrecBase* make_hier(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_desc* levels, mlp_mat A0) {
+static DevPrecBase* make_hier(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_desc* levels, mlp_mat A0,
+                              bool bf16 = false) {
     auto* h = new DevHier<VT>();
     h->ctx = ctx;
+    h->bf16 = bf16;
     try {
         std::vector<mlp_level_desc> d(levels, levels + nlevels);
         h->build(d, kind, A0);
@@ -1754,7 +1818,7 @@ int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_des
     MLP_API_BEGIN
     require(ctx && out && levels && nlevels && A0, "null argument");
     require(kind == MLP_PREC_UH_CHEB || kind == MLP_PREC_MULTI_SPLIT, "unknown hierarchy kind");
-    require(precision == MLP_FP64 || precision == MLP_FP32, "unknown precision");
+    require(precision == MLP_FP64 || precision == MLP_FP32 || precision == MLP_FP32_BF16, "unknown precision");
     require(levels[0].N == A0->n, "level 0 size differs from A0");
     use_device(ctx);
     auto p = std::make_unique<mlp_prec_s>();
@@ -1764,7 +1828,7 @@ int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_des
     p->precision = precision;
     p->n = A0->n;
     if (precision == MLP_FP64) p->impl.reset(make_hier<double>(ctx, kind, nlevels, levels, A0));
-    else p->impl.reset(make_hier<float>(ctx, kind, nlevels, levels, A0));
+    else p->impl.reset(make_hier<float>(ctx, kind, nlevels, levels, A0, precision == MLP_FP32_BF16));
     *out = p.release();
     MLP_API_END
 }

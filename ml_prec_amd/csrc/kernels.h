@@ -8,6 +8,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_bfloat16.h>
 #include <cstdint>
 
 namespace mlp {
@@ -312,18 +313,20 @@ constexpr uint32_t kGlobalBlk = 0x80000000u;
 // Value of kept row r of the level output, produced by the iteration on the
 // next level (cheb_solve.cc:61-104, multi_split_solve.cc:106-108, 131-133).
 enum { OP_NONE = 0, OP_COPY = 1, OP_SCALE = 2, OP_CHEB2 = 3, OP_CHEBK = 4 };
-template <class VT> struct Combine {
-    const VT* x2 = nullptr;
-    const VT* u1 = nullptr;
-    const VT* u0 = nullptr;
+// CV: the type the Chebyshev vectors x2, u1, u0 are stored in (the level's
+// own type, or BF16 for the finest level's vectors of a BF16 variant).
+template <class VT, class CV = VT> struct Combine {
+    const CV* x2 = nullptr;
+    const CV* u1 = nullptr;
+    const CV* u0 = nullptr;
     double a = 0.0, b = 0.0;             // Chebyshev coefficients, applied in FP64 (one rounding to VT)
     int skip = 0, one = 0;
 };
 
-template <class VT, int OP>
-__device__ __forceinline__ VT combine(const Combine<VT>& C, uint32_t r) {
+template <class VT, int OP, class CV>
+__device__ __forceinline__ VT combine(const Combine<VT, CV>& C, uint32_t r) {
     if (OP == OP_COPY) return C.x2[r];
-    if (OP == OP_SCALE) return C.skip ? C.x2[r] : VT(double(C.x2[r]) * C.a);      // dscal(alpha, x2)
+    if (OP == OP_SCALE) return C.skip ? VT(C.x2[r]) : VT(double(C.x2[r]) * C.a); // dscal(alpha, x2)
     if (OP == OP_CHEB2) {                                                          // dscal(-alpha,x2); daxpy(1+beta,u1,x2)
         const double v = C.skip ? double(C.x2[r]) : double(C.x2[r]) * C.a;
         const double u = double(C.u1[r]);
@@ -348,13 +351,14 @@ __device__ __forceinline__ VT combine(const Combine<VT>& C, uint32_t r) {
 // The kernels read every position exactly once, so each residual row is
 // computed once, with the same operations as a separate SpMV.
 enum { IN_PLAIN = 0, IN_PERM = 1, IN_RES = 2, IN_RES_SCALE = 3 };
+// FT: the type the parent's vectors (f, u, F, us) are stored in.
 template <class VT, class FT, int IN> struct InSrc {
     static constexpr bool kNeedRow = IN == IN_PLAIN || IN == IN_RES_SCALE;
     const FT* f = nullptr;
     SellView<VT> A{};
-    const VT* u = nullptr;
-    const VT* F = nullptr;
-    VT* us = nullptr;
+    const FT* u = nullptr;
+    const FT* F = nullptr;
+    FT* us = nullptr;
     double s = 1.0;                      // Chebyshev coefficient (FP64)
 };
 
@@ -362,9 +366,9 @@ template <class VT, class FT, int IN>
 __device__ __forceinline__ VT fetch(const InSrc<VT, FT, IN>& S, uint32_t q, uint32_t j) {
     if (IN == IN_PLAIN) return VT(S.f[j]);
     if (IN == IN_PERM) return VT(S.f[q]);
-    const VT acc = sell_row<VT, VT, IN == IN_RES_SCALE>(S.A, S.u, q, S.s);
-    if (IN == IN_RES_SCALE) S.us[j] = VT(double(S.u[j]) * S.s);
-    return acc - S.F[q];
+    const VT acc = sell_row<VT, FT, IN == IN_RES_SCALE>(S.A, S.u, q, S.s);
+    if (IN == IN_RES_SCALE) S.us[j] = FT(double(S.u[j]) * S.s);
+    return acc - VT(S.F[q]);
 }
 
 template <class VT, class FT, int IN>
@@ -438,9 +442,9 @@ __device__ __forceinline__ void block_down(const ProgView<VT>& P, const uint32_t
 // Back sweep of one block over its eliminated rows in U order.  Each row's w
 // comes from LDS (after block_down) or from wt; kept columns take their value
 // from the combine op.  Results land in sz.
-template <class VT, int OP, bool W_IN_LDS>
+template <class VT, int OP, bool W_IN_LDS, class CV>
 __device__ __forceinline__ void block_up(const ProgView<VT>& P, const uint32_t* B, const VT* __restrict__ wt,
-                                         const Combine<VT>& C, unsigned char* lds, const LdsPlan& lp) {
+                                         const Combine<VT, CV>& C, unsigned char* lds, const LdsPlan& lp) {
     VT* sw = reinterpret_cast<VT*>(lds + lp.sw);
     VT* sx = reinterpret_cast<VT*>(lds + lp.sx);
     VT* sval = reinterpret_cast<VT*>(lds + lp.uval);
@@ -539,9 +543,9 @@ __device__ __forceinline__ VT owned_down(const ProgView<VT>& P, const uint32_t* 
 // U sweep of the owned-node path.  w of the row comes from sw (after
 // owned_down in the same launch) or from wt; kept columns from the combine op.
 // Solved values land in sx (local L positions); returns the thread's value.
-template <int BS, class VT, int OP, bool W_IN_LDS>
+template <int BS, class VT, int OP, bool W_IN_LDS, class CV>
 __device__ __forceinline__ VT owned_up(const ProgView<VT>& P, const uint32_t* B, const VT* __restrict__ wt,
-                                       const Combine<VT>& C, const VT* sw, VT* sx) {
+                                       const Combine<VT, CV>& C, const VT* sw, VT* sx) {
     const uint32_t base = B[0], ub = B[10], nu = B[11], nl = B[5] - B[4];
     const uint32_t u = threadIdx.x;
     uint32_t my = nl, q0 = 0, q1 = 0, c0 = kGlobalBit, c1 = kGlobalBit, loc = 0;
@@ -615,9 +619,9 @@ __device__ __forceinline__ void global_down(const ProgView<VT>& P, const uint32_
 // U sweep of a global-memory block: w of a row from ws[base + loc], solved
 // values into P.gx[base + loc] (ws may be P.gx itself: a row reads its own w
 // before it writes its x there, and other rows read only solved columns).
-template <class VT, int OP>
+template <class VT, int OP, class CV>
 __device__ __forceinline__ void global_up(const ProgView<VT>& P, const uint32_t* B, const VT* ws,
-                                          const Combine<VT>& C) {
+                                          const Combine<VT, CV>& C) {
     VT* gx = P.gx;
     const uint32_t base = B[0], ub = B[10], nl = B[5] - B[4];
     uint32_t s = 0;
@@ -661,9 +665,9 @@ __device__ __forceinline__ BlkHdr load_hdr(const uint32_t* blk) {
 // feed; the remaining blocks move the right-hand side of the other kept rows
 // into F and apply the isolated-node diagonal x[j] = dval * f[j] (DMODE 1
 // adds it to x).
-template <class VT, class FT, int IN, class XT, int DMODE, int BS, bool BAT>
+template <class VT, class FT, int IN, class XT, int DMODE, int BS, bool BAT, class FV = VT>
 __global__ void __launch_bounds__(BS) k_level_down(ProgView<VT> P, InSrc<VT, FT, IN> f, VT* __restrict__ wt,
-                                                     VT* __restrict__ F, uint32_t Md, const VT* __restrict__ dval,
+                                                     FV* __restrict__ F, uint32_t Md, const VT* __restrict__ dval,
                                                      XT* __restrict__ x, const uint32_t* guard, LevelBatch lb) {
     if constexpr (BAT) {                 // batched right-hand sides (compiled out for single solves)
         const size_t y = blockIdx.y;
@@ -685,7 +689,7 @@ __global__ void __launch_bounds__(BS) k_level_down(ProgView<VT> P, InSrc<VT, FT,
             if (gflag) return;
             for (uint32_t p = threadIdx.x; p < S; p += blockDim.x) {
                 const uint32_t o = P.out[base + p];
-                if (o & kGlobalBit) F[o & ~kGlobalBit] = wt[base + p];
+                if (o & kGlobalBit) F[o & ~kGlobalBit] = FV(wt[base + p]);
             }
             return;
         }
@@ -695,7 +699,7 @@ __global__ void __launch_bounds__(BS) k_level_down(ProgView<VT> P, InSrc<VT, FT,
             const VT v = owned_down<BS>(P, h.w, f, reinterpret_cast<VT*>(lds_raw + lp.sw));
             if (gflag) return;
             if (threadIdx.x < S) {
-                if (o0 & kGlobalBit) F[o0 & ~kGlobalBit] = v;
+                if (o0 & kGlobalBit) F[o0 & ~kGlobalBit] = FV(v);
                 else wt[base + threadIdx.x] = v;
             }
             return;
@@ -705,7 +709,7 @@ __global__ void __launch_bounds__(BS) k_level_down(ProgView<VT> P, InSrc<VT, FT,
         const VT* sw = reinterpret_cast<const VT*>(lds_raw + lp.sw);
         for (uint32_t p = threadIdx.x; p < S; p += blockDim.x) {
             const uint32_t o = p == threadIdx.x ? o0 : P.out[base + p];
-            if (o & kGlobalBit) F[o & ~kGlobalBit] = sw[p];
+            if (o & kGlobalBit) F[o & ~kGlobalBit] = FV(sw[p]);
             else wt[base + p] = sw[p];
         }
     } else {
@@ -713,7 +717,7 @@ __global__ void __launch_bounds__(BS) k_level_down(ProgView<VT> P, InSrc<VT, FT,
         const uint32_t q = P.nodes + i;
         if (i < P.nflat) {
             const VT v = fetch_at(P, f, q);
-            if (!gflag) F[P.flatq[i]] = v;
+            if (!gflag) F[P.flatq[i]] = FV(v);
         } else if (i < P.nflat + Md) {
             const uint32_t j = P.qrow[q];
             const XT v = XT(dval[i - P.nflat] * fetch(f, q, j));
@@ -725,8 +729,8 @@ __global__ void __launch_bounds__(BS) k_level_down(ProgView<VT> P, InSrc<VT, FT,
 // Backward part of a level: component blocks run the U sweep (kept columns
 // from the combine op) and write their rows of x; the remaining blocks write
 // the kept rows x[map[M+r]] = combine(r) (UMODE 1 adds into x).
-template <class VT, class XT, int OP, int UMODE, int BS, bool BAT>
-__global__ void __launch_bounds__(BS) k_level_up(ProgView<VT> P, const VT* __restrict__ wt, Combine<VT> C,
+template <class VT, class XT, int OP, int UMODE, int BS, bool BAT, class CV = VT>
+__global__ void __launch_bounds__(BS) k_level_up(ProgView<VT> P, const VT* __restrict__ wt, Combine<VT, CV> C,
                                                    const uint32_t* __restrict__ mapM, uint32_t n, XT* __restrict__ x,
                                                    const uint32_t* guard, LevelBatch lb) {
     if constexpr (BAT) {                 // batched right-hand sides (compiled out for single solves)

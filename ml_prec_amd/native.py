@@ -13,7 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmlp.so")
 
-MLP_FP64, MLP_FP32 = 0, 1
+MLP_FP64, MLP_FP32, MLP_FP32_BF16 = 0, 1, 2
 UH_CHEB, MULTI_SPLIT, DIAG, ID, GS, COMPOSITE, SYM = 1, 2, 3, 4, 5, 6, 7
 X0_ZERO, X0_GIVEN, X0_REFERENCE = 0, 1, 2
 ORDERS = {"none": 0, "original": 1, "block": 2, "simple_1": 3}
