@@ -229,6 +229,7 @@ template <class VT> struct DevLevel {
     DevSell<double> A64;
 };
 static_assert(kPairHdr == kPairFields, "pair block header size");
+static_assert(kChainHdrW == kChainHdr, "chain block header size");
 
 struct DevPrecBase {
     virtual ~DevPrecBase() {}
@@ -378,6 +379,11 @@ template <class VT> struct DevHier : DevPrecBase {
     bool dynamic = false;
     int pair_l = -1;                         // level fused with the coarsest level (-1: none)
     bool bf16 = false;                       // level 0 keeps its Chebyshev vectors in BF16 (FP32 uh_cheb)
+    // chain of the levels chain_a..nlevels-1 (plan.h: ChainProgram); -1: none
+    int chain_a = -1;
+    ChainView<VT, double> cv;                // the chain program and the chain levels' vectors (device pointers)
+    std::vector<uint32_t> chain_blk;         // per chain level: blocks with rows on it
+    std::vector<uint32_t> chain_npt;         // per chain level and the coarsest: passthrough rows
 
     hipStream_t stream() const { return ctx->stream; }
 
@@ -503,6 +509,80 @@ template <class VT> struct DevHier : DevPrecBase {
         }
     }
 
+    static LevelArrays arrays_of(const mlp_level_desc& D) {
+        LevelArrays la;
+        la.N = D.N; la.M = D.M; la.Md = D.Md; la.map = D.map;
+        la.L_ia = D.L_ia; la.L_ja = D.L_ja; la.L_a = D.L_a;
+        la.U_ia = D.U_ia; la.U_ja = D.U_ja; la.U_a = D.U_a; la.dval = D.dval;
+        return la;
+    }
+
+    // the device arrays of chain level l (plan.h: ChainLevelProgram) and its vectors
+    template <class T>
+    void upload_chain_level(uint32_t l, const mlp_level_desc& D, const ChainLevelProgram& C, ChainLv<T>& V,
+                            DevLevel<VT>& L) {
+        const uint32_t c = nlevels - 1;
+        L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
+        L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
+        Stager up(ctx, *data);
+        std::vector<double> urcp;
+        up.add(D.map, D.N, L.map);
+        up.add(C.hdr, V.hdr); up.add(C.qrow, V.qrow); up.add(C.src, V.src); up.add(C.xl, V.xl);
+        up.add(C.Ldep, V.Ldep); up.add(C.Lptr, V.Lptr); up.add(C.Lloc, V.Lloc); up.add(C.Lval, V.Lval);
+        up.add(C.Udep, V.Udep); up.add(C.uloc, V.uloc); up.add(C.uxidx, V.uxidx); up.add(C.Uptr, V.Uptr);
+        up.add(C.Ucol, V.Ucol); up.add(C.Uval, V.Uval); up.add(C.dval, V.dval);
+        if constexpr (std::is_same_v<T, float>) {               // udiv: FP32 multiplies by 1 / u_ii
+            urcp.resize(C.Udiag.size());
+            for (size_t i = 0; i < urcp.size(); i++) urcp[i] = 1.0 / C.Udiag[i];
+            up.add(urcp, V.Udiag);
+        } else {
+            up.add(C.Udiag, V.Udiag);
+        }
+        up.flush();
+        V.nslots = C.nslots;
+        V.npt = C.npt;
+        chain_npt[l - chain_a] = C.npt;
+        // this level's matrix in its chain order: read by the Chebyshev residual of its second visits (the
+        // coarsest level's in FP64, by step 2)
+        if (l == c) {
+            upload_sell<double>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A64, C.qrow.data());
+            L.A.nnz = L.A64.nnz;
+        } else if (l > 0) {
+            upload_sell<VT>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A, C.qrow.data());
+        }
+        if (l < c) {
+            L.prog = ProgView<VT>();
+            L.prog.nodes = L.N;                                  // wt: one value per chain position
+            L.wt = scratch.alloc<VT>(std::max<uint32_t>(L.N, 1));
+            if (L.n) {
+                L.F = scratch.alloc<VT>(L.n);
+                L.u1 = scratch.alloc<VT>(L.n);
+                L.u0 = scratch.alloc<VT>(L.n);
+                L.x2 = scratch.alloc<VT>(L.n);
+            }
+        }
+    }
+
+    // the chain levels' vectors and combine coefficients in cv (after the coefficients are set, and again in
+    // a clone, whose vectors are its own)
+    void chain_vectors() {
+        if (chain_a < 0) return;
+        const uint32_t c = nlevels - 1;
+        cv.nv = c - uint32_t(chain_a);
+        cv.st = uint32_t(sizeof(SolveState) / sizeof(uint32_t));
+        for (uint32_t l = uint32_t(chain_a); l < c; l++) {
+            const DevLevel<VT>& L = lv[l];
+            ChainVec<VT>& V = cv.V[l - chain_a];
+            V.F = L.F; V.u1 = L.u1; V.u0 = L.u0; V.x2 = L.x2; V.wt = L.wt;
+            V.u1s = L.s1_skip ? L.u1 : L.u0;
+            V.ma2 = L.ma2; V.s2_skip = L.s2_skip; V.opb2 = L.opb2; V.opb2_one = L.opb2_one;
+            V.n = L.n;
+            V.N = L.N;
+        }
+        cv.a1c = lv[c - 1].a1;
+        cv.A2 = lv[c].A64.v;
+    }
+
     void build(const std::vector<mlp_level_desc>& d, int kind_, mlp_mat A0_) {
         kind = kind_;
         A0 = A0_;
@@ -522,9 +602,32 @@ template <class VT> struct DevHier : DevPrecBase {
         }
         if (bf16 && (kind != MLP_PREC_UH_CHEB || nlevels < 2 || d[0].nit != 2))
             throw Error("the BF16 variant needs a uh_cheb hierarchy of two or more levels with ncheb = 2 on level 0", 1);
+        // uh_cheb runs its last levels as a chain (descent and ascent launches) from the highest level a such that
+        // levels a..c-1 have ncheb = 2, at most kChainMax of them, and every merged component fits a wavefront
+        // (MLP_NO_CHAIN=1: level kernels and the fused last pair instead).  The BF16 variant keeps level 0 apart.
+        ChainProgram CP;
+        const char* nochain = getenv("MLP_NO_CHAIN");
+        if (kind == MLP_PREC_UH_CHEB && nlevels >= 2 && !(nochain && nochain[0] == '1')) {
+            const uint32_t c = nlevels - 1;
+            uint32_t a0 = c > uint32_t(kChainMax) ? c - uint32_t(kChainMax) : 0u;
+            if (bf16) a0 = std::max(a0, 1u);
+            for (uint32_t l = a0; l < c; l++)
+                if (d[l].nit != 2) a0 = l + 1;
+            for (uint32_t a = a0; a < c; a++) {
+                std::vector<LevelArrays> la;
+                for (uint32_t l = a; l <= c; l++) la.push_back(arrays_of(d[l]));
+                if (plan_chain(la, a, kChainBS, kChainSlots, CP)) {
+                    chain_a = int(a);
+                    chain_blk = CP.blk_from;
+                    chain_npt.assign(CP.nlev, 0);
+                    break;
+                }
+            }
+        }
+        const uint32_t nplan = chain_a >= 0 ? uint32_t(chain_a) : nlevels;
         // the levels are planned independently: one thread each
         std::vector<std::exception_ptr> err(nlevels);
-        parallel_for(nlevels, 1, [&](uint64_t l0, uint64_t l1) {
+        parallel_for(nplan, 1, [&](uint64_t l0, uint64_t l1) {
             for (uint64_t l = l0; l < l1; l++) {
                 try {
                     const mlp_level_desc& D = d[l];
@@ -546,8 +649,8 @@ template <class VT> struct DevHier : DevPrecBase {
         uint32_t pbs = 0;
         const char* nopair = getenv("MLP_NO_PAIR");
         // (the BF16 variant runs level 0 on its own kernels)
-        if (kind == MLP_PREC_UH_CHEB && nlevels >= 2 && d[nlevels - 2].nit == 2 && !(nopair && nopair[0] == '1') &&
-            !(bf16 && nlevels == 2)) {
+        if (kind == MLP_PREC_UH_CHEB && chain_a < 0 && nlevels >= 2 && d[nlevels - 2].nit == 2 &&
+            !(nopair && nopair[0] == '1') && !(bf16 && nlevels == 2)) {
             auto arrays = [&](uint32_t l) {
                 const mlp_level_desc& D = d[l];
                 LevelArrays la;
@@ -562,9 +665,10 @@ template <class VT> struct DevHier : DevPrecBase {
             if (pbs) pair_l = int(nlevels - 2);
         }
         if (kind == MLP_PREC_UH_CHEB)
-            for (uint32_t l = 0; l + 1 < nlevels; l++) {
+            for (uint32_t l = 0; l < nplan && l + 1 < nlevels; l++) {
                 if (int(l) == pair_l) continue;
                 if (int(l + 1) == pair_l) link_levels(progs[l], PP.qrow);
+                else if (int(l + 1) == chain_a) link_levels(progs[l], CP.lv[0].qrow);
                 else link_levels(progs[l], progs[l + 1]);
             }
         for (uint32_t l = 0; l < nlevels; l++) {
@@ -574,6 +678,11 @@ template <class VT> struct DevHier : DevPrecBase {
             L.N = D.N; L.M = D.M; L.Md = D.Md; L.nit = D.nit; L.eps = D.eps;
             L.n = L.N - L.M - L.Md;
             if (L.eps != 0.0) dynamic = true;
+            if (chain_a >= 0 && l >= uint32_t(chain_a)) {
+                if (l + 1 == nlevels) upload_chain_level(l, D, CP.lv[l - chain_a], cv.C, L);
+                else upload_chain_level(l, D, CP.lv[l - chain_a], cv.L[l - chain_a], L);
+                continue;
+            }
             if (int(l) == pair_l) {
                 upload_pair(l, D, PP, pbs, L);
                 continue;
@@ -679,6 +788,7 @@ template <class VT> struct DevHier : DevPrecBase {
         }
         if (kind == MLP_PREC_MULTI_SPLIT && nlevels && lv[0].nit > 1)
             throw Error("multi_split level 0 runs one iteration", 1);
+        chain_vectors();
         MLP_HIP_CHECK(hipStreamSynchronize(s));
     }
 
@@ -696,6 +806,10 @@ template <class VT> struct DevHier : DevPrecBase {
         h->dynamic = dynamic;
         h->pair_l = pair_l;
         h->bf16 = bf16;
+        h->chain_a = chain_a;
+        h->cv = cv;
+        h->chain_blk = chain_blk;
+        h->chain_npt = chain_npt;
         h->env_lbs = env_lbs;
         h->env_cap = env_cap;
         h->lds_budget = lds_budget;
@@ -718,6 +832,7 @@ template <class VT> struct DevHier : DevPrecBase {
             }
             if (L.r) L.r = sc.template alloc<VT>(k * L.N);
         }
+        h->chain_vectors();
         return h;
     }
 
@@ -867,9 +982,84 @@ template <class VT> struct DevHier : DevPrecBase {
         up<double, OP_CHEB2, 0>(L, C, x, g);
     }
 
+    // --- the chain (plan.h: ChainProgram) ---
+    template <class FT, int IN, class XT>
+    void chain_down(uint32_t j, const InSrc<VT, FT, IN>& f, XT* out, const uint32_t* g) const {
+        const uint32_t i0 = j - uint32_t(chain_a);
+        const uint32_t grid = chain_blk[i0] + cdiv(chain_npt[i0], kChainBS);
+        auto k = R > 1 ? k_chain_down<VT, double, FT, IN, XT, true> : k_chain_down<VT, double, FT, IN, XT, false>;
+        launch(k, grid, kChainBS, 0, cv, i0, f, out, lv[j].N, chain_blk[i0], g);
+    }
+    template <class XT>
+    void chain_up(uint32_t ke, XT* out, const uint32_t* g) const {
+        const uint32_t ie = ke - uint32_t(chain_a);
+        const uint32_t grid = chain_blk[ie] + cdiv(chain_npt[ie + 1], kChainBS);
+        if (lv[nlevels - 2].s1_skip) {
+            auto k = R > 1 ? k_chain_up<VT, double, XT, false, true> : k_chain_up<VT, double, XT, false, false>;
+            launch(k, grid, kChainBS, 0, cv, ie, out, lv[ke].N, chain_blk[ie], g);
+        } else {
+            auto k = R > 1 ? k_chain_up<VT, double, XT, true, true> : k_chain_up<VT, double, XT, true, false>;
+            launch(k, grid, kChainBS, 0, cv, ie, out, lv[ke].N, chain_blk[ie], g);
+        }
+    }
+    // The operations of Prec::solve on the chain levels, in order: D (L sweep of a level: its visit starts),
+    // the coarsest level's first and second visits, U (U sweep: a visit ends).
+    struct ChainOp {
+        int kind;                            // 0 D, 1 coarsest first visit, 2 coarsest second visit, 3 U
+        uint32_t k;
+        bool top, first;                     // the chain's own visit; a first visit (input F of the level above)
+    };
+    void chain_ops(uint32_t k, bool first, bool top, std::vector<ChainOp>& ops) const {
+        if (k + 1 == nlevels) {
+            ops.push_back({first ? 1 : 2, k, false, first});
+            return;
+        }
+        ops.push_back({0, k, top, first});
+        chain_ops(k + 1, true, false, ops);
+        chain_ops(k + 1, false, false, ops);
+        ops.push_back({3, k, top, first});
+    }
+    // The chain's visit: a descent runs the D operations down to the coarsest level's first visit, an ascent the
+    // coarsest level's second visit and the U operations that follow it.  A descent starts with the chain's own
+    // input or with the Chebyshev residual of a second visit; an ascent ends with the chain's own output or with
+    // a first visit's (into u1 of the level above).
+    template <class FT, int IN, class XT>
+    void run_chain(const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
+        std::vector<ChainOp> ops;
+        chain_ops(uint32_t(chain_a), true, true, ops);
+        for (size_t i = 0; i < ops.size();) {
+            const ChainOp& o = ops[i];
+            if (o.kind == 0) {
+                if (o.top) {
+                    chain_down(o.k, f, x, g);
+                } else {
+                    const DevLevel<VT>& P = lv[o.k - 1];
+                    if (P.s1_skip) chain_down(o.k, residual_src<IN_RES>(o.k - 1, P.u1, 1.0, nullptr), P.x2, g);
+                    else chain_down(o.k, residual_src<IN_RES_SCALE>(o.k - 1, P.u1, P.a1, P.u0), P.x2, g);
+                }
+                while (i < ops.size() && ops[i].kind == 0) i++;
+                if (i >= ops.size() || ops[i].kind != 1) throw Error("chain: unexpected operation order", 1);
+                i++;
+            } else {
+                if (o.kind != 2) throw Error("chain: unexpected operation order", 1);
+                size_t e = i + 1;
+                while (e < ops.size() && ops[e].kind == 3) e++;
+                if (e == i + 1) throw Error("chain: unexpected operation order", 1);
+                const ChainOp& u = ops[e - 1];
+                if (u.top) chain_up(u.k, x, g);
+                else chain_up(u.k, u.first ? lv[u.k - 1].u1 : lv[u.k - 1].x2, g);
+                i = e;
+            }
+        }
+    }
+
     template <class FT, int IN, class XT>
     void cheb(uint32_t l, const InSrc<VT, FT, IN>& f, XT* x, const uint32_t* g) const {
         const DevLevel<VT>& L = lv[l];
+        if (chain_a >= 0 && int(l) == chain_a) {
+            run_chain(f, x, g);
+            return;
+        }
         if (int(l) == pair_l) {
             pair_visit(L, lv[l + 1], f, x, g);
             return;

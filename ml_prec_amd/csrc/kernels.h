@@ -1107,6 +1107,372 @@ __global__ void __launch_bounds__(BS) k_pair_up(PairView<VT, CT> P, SellView<CT>
     }
 }
 
+// ---------------------------------------------------------------------------
+// chain of uh_cheb levels a..c (plan.h: ChainProgram): descent and ascent
+// launches of Prec::solve (cheb_solve.cc:13-123).  One wavefront per block;
+// thread t owns slot t of every level (node slots are the first ones, so the
+// owned-node sweeps above apply), slots beyond the wavefront (rows without L
+// entries of large components) are handled in a second pass.  Every row keeps
+// the arithmetic of k_level_down / k_level_up / k_pair_down / k_pair_up.
+// ---------------------------------------------------------------------------
+constexpr int kChainMax = 4;               // levels a..c-1 of one chain
+constexpr uint32_t kChainBS = 64;
+constexpr uint32_t kChainSlots = 128;
+constexpr uint32_t kChainHdrW = 8;         // header words per block and level
+
+// A block-uniform word of program data (never written by the kernel), read
+// through the scalar cache so that it occupies a scalar register.
+__device__ __forceinline__ uint32_t uload(const uint32_t* p) {
+    return *(const __attribute__((address_space(4))) uint32_t*)p;
+}
+
+template <class T> struct ChainLv {
+    const uint32_t *hdr = nullptr, *qrow = nullptr, *src = nullptr, *xl = nullptr;
+    const uint32_t *Ldep = nullptr, *Lptr = nullptr, *Lloc = nullptr;
+    const uint32_t *Udep = nullptr, *uloc = nullptr, *uxidx = nullptr, *Uptr = nullptr, *Ucol = nullptr;
+    const T *Lval = nullptr, *Uval = nullptr, *Udiag = nullptr, *dval = nullptr;
+    uint32_t nslots = 0, npt = 0;
+};
+// a chain level's vectors as the parent of the next level (cheb_prec.h:36), and
+// the combine of its second Chebyshev step (cheb_solve.cc:79-81)
+template <class VT> struct ChainVec {
+    VT *F = nullptr, *u1 = nullptr, *u0 = nullptr, *x2 = nullptr, *wt = nullptr;
+    const VT* u1s = nullptr;               // u1, or a1 u1 (in u0) when the first step scales
+    double ma2 = 0.0, opb2 = 0.0;
+    int s2_skip = 0, opb2_one = 0;
+    uint32_t n = 0, N = 0;                 // batched strides: the next level's size, this level's
+};
+template <class VT, class CT> struct ChainView {
+    ChainLv<VT> L[kChainMax];
+    ChainVec<VT> V[kChainMax];
+    ChainLv<CT> C;                         // the coarsest level
+    SellView<CT> A2;                       // its matrix, rows in its chain order (step 2's residual)
+    double a1c = 1.0;                      // level c-1's first-step scaling
+    uint32_t nv = 0;                       // levels a..c-1
+    uint32_t st = 0;                       // solve-state stride, in words (batched guards)
+};
+
+// Descent from chain level i0: level i0 reads its input from f, levels
+// i0+1..nv-1 (first visits) the F of the level above from LDS, then step 1 on
+// the coarsest level.  Writes F of every level below i0, wt of every level,
+// the isolated rows' x (level i0's into out0, the others' into the parent's
+// u1) and u1 of the coarsest level.
+template <class VT, class CT, class FT, int IN, class XT, bool BAT>
+__global__ void __launch_bounds__(kChainBS) k_chain_down(ChainView<VT, CT> V, uint32_t i0, InSrc<VT, FT, IN> f,
+                                                           XT* __restrict__ out0, uint32_t Ntop, uint32_t nblk,
+                                                           const uint32_t* guard) {
+    const size_t y = BAT ? blockIdx.y : 0;
+    if constexpr (BAT) {
+        f.f = shift(f.f, y * Ntop); f.u = shift(f.u, y * Ntop); f.F = shift(f.F, y * Ntop); f.us = shift(f.us, y * Ntop);
+        out0 = shift(out0, y * Ntop); guard = shift(guard, y * V.st);
+    }
+    const uint32_t gflag = guard ? *guard : 0u;
+    const uint32_t nv = V.nv, t = threadIdx.x;
+    // the last level's vectors (the coarsest level's parent)
+    VT *pF = nullptr, *pu1 = nullptr;
+#pragma unroll
+    for (int i = 0; i < kChainMax; i++)
+        if (uint32_t(i) + 1 == nv) { pF = V.V[i].F + y * V.V[i].n; pu1 = V.V[i].u1 + y * V.V[i].n; }
+    if (blockIdx.x < nblk) {
+        __shared__ VT sv[2][kChainSlots];
+        __shared__ CT cw[kChainSlots], cx[kChainSlots];
+        const uint32_t b = blockIdx.x;
+        // every level's per-slot loads first, so that they overlap
+        uint32_t base[kChainMax], S[kChainMax], nn[kChainMax], nkeep[kChainMax], nlL[kChainMax], qr[kChainMax],
+            sr[kChainMax];
+        VT dv[kChainMax];
+        OwnL<VT> ol[kChainMax];
+        VT acc = VT(0);
+#pragma unroll
+        for (int i = 0; i < kChainMax; i++) {
+            base[i] = S[i] = nn[i] = nkeep[i] = nlL[i] = qr[i] = sr[i] = 0;
+            dv[i] = VT(0);
+            ol[i] = OwnL<VT>{0u, 0u, 0u, t, t, VT(0), VT(0)};
+            if (uint32_t(i) < i0 || uint32_t(i) >= nv) continue;
+            const ChainLv<VT>& L = V.L[i];
+            const uint32_t* H = L.hdr + kChainHdrW * b;
+            base[i] = uload(H); S[i] = uload(H + 1); nn[i] = uload(H + 2); nkeep[i] = nn[i] + uload(H + 3);
+            nlL[i] = uload(H + 4);
+            ol[i] = own_L(nn[i], nlL[i], base[i], L.Ldep, L.Lptr, L.Lloc, L.Lval);
+            if (t < S[i]) {
+                qr[i] = L.qrow[base[i] + t];
+                if (uint32_t(i) > i0) sr[i] = L.src[base[i] + t];
+                if (t >= nkeep[i]) dv[i] = L.dval[base[i] + t];
+                if (uint32_t(i) == i0) acc = fetch(f, base[i] + t, qr[i]);
+            }
+        }
+        const ChainLv<CT>& C = V.C;
+        const uint32_t* Hc = C.hdr + kChainHdrW * b;
+        const uint32_t baseC = uload(Hc), SC = uload(Hc + 1), nnC = uload(Hc + 2), nlLC = uload(Hc + 4),
+                       ubC = uload(Hc + 5), nuC = uload(Hc + 6), nlUC = uload(Hc + 7);
+        const OwnL<CT> olC = own_L(nnC, nlLC, baseC, C.Ldep, C.Lptr, C.Lloc, C.Lval);
+        const OwnU<CT> ouC = own_U(nuC, nlUC, ubC, C.Udep, C.uloc, C.Udiag, C.Uptr, C.Ucol, C.Uval);
+        uint32_t qrC = 0, srC = 0, uxC = 0;
+        CT dvC = CT(0);
+        if (t < SC) {
+            qrC = C.qrow[baseC + t];
+            srC = C.src[baseC + t];
+            if (t >= nnC) dvC = C.dval[baseC + t];
+        }
+        if (t < nuC) uxC = C.uxidx[ubC + t];
+        if (gflag) return;
+        int cur = 0;
+#pragma unroll
+        for (int i = 0; i < kChainMax; i++) {
+            if (uint32_t(i) < i0 || uint32_t(i) >= nv) continue;
+            const ChainLv<VT>& L = V.L[i];
+            const ChainVec<VT>& P = V.V[i > 0 ? i - 1 : 0];     // the parent's vectors (levels below i0)
+            VT* pFi = P.F + y * P.n;
+            VT* pu1i = P.u1 + y * P.n;
+            VT* s = sv[cur];
+            const VT* p = sv[cur ^ 1];
+            VT v = VT(0);
+            if (t < S[i]) {
+                v = uint32_t(i) == i0 ? acc : p[sr[i]];
+                if (uint32_t(i) > i0) pFi[base[i] + t] = v;
+                if (t >= nkeep[i]) {
+                    if (uint32_t(i) == i0) out0[qr[i]] = XT(dv[i] * v);
+                    else pu1i[qr[i]] = VT(dv[i] * v);
+                }
+                if (t >= nn[i]) s[t] = v;
+            }
+            for (uint32_t q = t + kChainBS; q < S[i]; q += kChainBS) {   // slots beyond the wavefront: no nodes
+                const uint32_t j = L.qrow[base[i] + q];
+                const VT w = uint32_t(i) == i0 ? fetch(f, base[i] + q, j) : p[L.src[base[i] + q]];
+                if (uint32_t(i) > i0) pFi[base[i] + q] = w;
+                if (q >= nkeep[i]) {
+                    if (uint32_t(i) == i0) out0[j] = XT(L.dval[base[i] + q] * w);
+                    else pu1i[j] = VT(L.dval[base[i] + q] * w);
+                }
+                s[q] = w;
+            }
+            const VT w = sweep_L<kChainBS>(ol[i], nlL[i], L.Lloc, L.Lval, v, s);
+            if (t < nn[i]) V.V[i].wt[y * V.V[i].N + base[i] + t] = w;
+            level_sync<kChainBS>();
+            cur ^= 1;
+        }
+        // step 1 on the coarsest level from the F of the last chain level
+        const VT* p = sv[cur ^ 1];
+        VT v = VT(0);
+        if (t < SC) {
+            v = p[srC];
+            pF[baseC + t] = v;
+            if (t >= nnC) pu1[qrC] = VT(dvC * CT(v));
+        }
+        for (uint32_t q = t + kChainBS; q < SC; q += kChainBS) {
+            const VT w = p[C.src[baseC + q]];
+            pF[baseC + q] = w;
+            pu1[C.qrow[baseC + q]] = VT(C.dval[baseC + q] * CT(w));
+        }
+        const CT f2 = CT(v);
+        sweep_L<kChainBS>(olC, nlLC, C.Lloc, C.Lval, f2, cw);
+        level_sync<kChainBS>();
+        const CT z = t < nuC ? cw[ouC.loc] : CT(0);
+        const CT x = sweep_U<kChainBS, CT, false>(ouC, nlUC, C.Ucol, C.Uval, z, (const CT*)nullptr, cx);
+        if (t < nuC) pu1[uxC] = VT(x);
+    } else {
+        // passthrough rows: kept rows without L entries down to the level where they are isolated
+        const uint32_t tp = (blockIdx.x - nblk) * kChainBS + t;
+        bool live = true;
+        VT v = VT(0);
+#pragma unroll
+        for (int i = 0; i < kChainMax; i++) {
+            if (uint32_t(i) < i0 || uint32_t(i) >= nv || !live) continue;
+            const ChainLv<VT>& L = V.L[i];
+            if (tp >= L.npt) { live = false; continue; }
+            const uint32_t q = L.nslots + tp;
+            const ChainVec<VT>& P = V.V[i > 0 ? i - 1 : 0];
+            if (uint32_t(i) == i0) {
+                v = fetch(f, q, L.qrow[q]);
+                if (gflag) return;
+            } else {
+                P.F[y * P.n + q] = v;
+            }
+            const uint32_t nxt = uint32_t(i) + 1 < nv ? V.L[i + 1 < kChainMax ? i + 1 : i].npt : V.C.npt;
+            if (tp >= nxt) {                  // isolated here
+                if (uint32_t(i) == i0) out0[L.qrow[q]] = XT(L.dval[q] * v);
+                else P.u1[y * P.n + L.qrow[q]] = VT(L.dval[q] * v);
+                live = false;
+            }
+        }
+        if (live && tp < V.C.npt) {           // isolated on the coarsest level
+            const uint32_t q = V.C.nslots + tp;
+            pF[q] = v;
+            pu1[V.C.qrow[q]] = VT(V.C.dval[q] * CT(v));
+        }
+    }
+}
+
+// Ascent to chain level ie: step 2 on the coarsest level from the residual
+// A_c (a1 u1) - F, then for levels nv-1 .. ie the combine of the kept rows and
+// the U sweep.  Every level above the coarsest but ie completes its second
+// visit here, so its isolated rows' values come from x2 of its parent, where
+// the descent of that visit wrote them; level ie's rows are written to out.
+template <class VT, class CT, class XT, bool SCALE, bool BAT>
+__global__ void __launch_bounds__(kChainBS) k_chain_up(ChainView<VT, CT> V, uint32_t ie, XT* __restrict__ out,
+                                                         uint32_t Ntop, uint32_t nblk, const uint32_t* guard) {
+    const size_t y = BAT ? blockIdx.y : 0;
+    if constexpr (BAT) { out = shift(out, y * Ntop); guard = shift(guard, y * V.st); }
+    const uint32_t gflag = guard ? *guard : 0u;
+    const uint32_t nv = V.nv, t = threadIdx.x;
+    const VT *pF = nullptr, *pu1 = nullptr;
+#pragma unroll
+    for (int i = 0; i < kChainMax; i++)
+        if (uint32_t(i) + 1 == nv) { pF = V.V[i].F + y * V.V[i].n; pu1 = V.V[i].u1 + y * V.V[i].n; }
+    const ChainLv<CT>& C = V.C;
+    if (blockIdx.x < nblk) {
+        __shared__ VT sx[2][kChainSlots];
+        __shared__ CT cw[kChainSlots], cx[kChainSlots];
+        const uint32_t b = blockIdx.x;
+        const uint32_t* Hc = C.hdr + kChainHdrW * b;
+        const uint32_t baseC = uload(Hc), SC = uload(Hc + 1), nnC = uload(Hc + 2), nlLC = uload(Hc + 4),
+                       ubC = uload(Hc + 5), nuC = uload(Hc + 6), nlUC = uload(Hc + 7);
+        // every per-slot load first: the coarsest level's residual and sweeps, each level's U sweep with its
+        // w, the next level's rows with their combine operand, the isolated rows' values
+        const OwnL<CT> olC = own_L(nnC, nlLC, baseC, C.Ldep, C.Lptr, C.Lloc, C.Lval);
+        const OwnU<CT> ouC = own_U(nuC, nlUC, ubC, C.Udep, C.uloc, C.Udiag, C.Uptr, C.Ucol, C.Uval);
+        CT f2 = CT(0), dvC = CT(0);
+        if (t < SC) {
+            f2 = sell_row<CT, VT, SCALE>(V.A2, pu1, baseC + t, V.a1c) - CT(pF[baseC + t]);
+            if (t >= nnC) dvC = C.dval[baseC + t];
+        }
+        uint32_t base[kChainMax], S[kChainMax], nkeep[kChainMax], nu[kChainMax], nlU[kChainMax], ux[kChainMax],
+            Sn[kChainMax], srn[kChainMax], xln[kChainMax];
+        VT z[kChainMax], xiso[kChainMax], usv[kChainMax];
+        OwnU<VT> ou[kChainMax];
+        CT usC = CT(0);
+#pragma unroll
+        for (int i = 0; i < kChainMax; i++) {
+            base[i] = S[i] = nkeep[i] = nu[i] = nlU[i] = ux[i] = Sn[i] = srn[i] = xln[i] = 0;
+            z[i] = xiso[i] = usv[i] = VT(0);
+            ou[i] = OwnU<VT>{0u, 0u, 0u, 0u, 0u, 0u, VT(1), VT(0), VT(0)};
+            if (uint32_t(i) < ie || uint32_t(i) >= nv) continue;
+            const ChainLv<VT>& L = V.L[i];
+            const ChainVec<VT>& Vi = V.V[i];
+            const uint32_t* H = L.hdr + kChainHdrW * b;
+            const uint32_t ub = uload(H + 5);
+            base[i] = uload(H); S[i] = uload(H + 1); nkeep[i] = uload(H + 2) + uload(H + 3); nu[i] = uload(H + 6);
+            nlU[i] = uload(H + 7);
+            ou[i] = own_U(nu[i], nlU[i], ub, L.Udep, L.uloc, L.Udiag, L.Uptr, L.Ucol, L.Uval);
+            if (t < nu[i]) {
+                z[i] = Vi.wt[y * Vi.N + base[i] + ou[i].loc];
+                if (uint32_t(i) == ie) ux[i] = L.uxidx[ub + t];
+            }
+            if (uint32_t(i) > ie && t < S[i] && t >= nkeep[i]) {
+                const ChainVec<VT>& P = V.V[i > 0 ? i - 1 : 0];
+                xiso[i] = P.x2[y * P.n + L.qrow[base[i] + t]];
+            }
+            // the next level's rows (the kept rows of this level)
+            const bool last = uint32_t(i) + 1 == nv;
+            const ChainLv<VT>& Ln = V.L[i + 1 < kChainMax ? i + 1 : i];
+            const uint32_t* qn = last ? C.qrow : Ln.qrow;
+            const uint32_t* sn = last ? C.src : Ln.src;
+            const uint32_t* xn = last ? C.xl : Ln.xl;
+            const uint32_t* Hn = (last ? C.hdr : Ln.hdr) + kChainHdrW * b;
+            Sn[i] = uload(Hn + 1);
+            if (t < Sn[i]) {
+                const uint32_t q = uload(Hn) + t, r = qn[q];
+                srn[i] = sn[q];
+                if (uint32_t(i) == ie) xln[i] = xn[q];
+                if (last) usC = SCALE ? CT(double(pu1[r]) * V.a1c) : CT(pu1[r]);
+                else usv[i] = Vi.u1s[y * Vi.n + r];
+            }
+        }
+        if (gflag) return;
+        // step 2 on the coarsest level
+        sweep_L<kChainBS>(olC, nlLC, C.Lloc, C.Lval, f2, cw);
+        if (t < SC && t >= nnC) cx[t] = dvC * f2;
+        for (uint32_t q = t + kChainBS; q < SC; q += kChainBS) {
+            const CT g = sell_row<CT, VT, SCALE>(V.A2, pu1, baseC + q, V.a1c) - CT(pF[baseC + q]);
+            cx[q] = C.dval[baseC + q] * g;
+        }
+        level_sync<kChainBS>();
+        const CT zc = t < nuC ? cw[ouC.loc] : CT(0);
+        sweep_U<kChainBS, CT, false>(ouC, nlUC, C.Ucol, C.Uval, zc, (const CT*)nullptr, cx);
+        level_sync<kChainBS>();
+        int cur = 0;
+#pragma unroll
+        for (int i = kChainMax - 1; i >= 0; i--) {
+            if (uint32_t(i) < ie || uint32_t(i) >= nv) continue;
+            const ChainLv<VT>& L = V.L[i];
+            const ChainVec<VT>& Vi = V.V[i];
+            const bool last = uint32_t(i) + 1 == nv;
+            VT* s = sx[cur];
+            const VT* xp = sx[cur ^ 1];
+            // the combine of this level's kept rows (the next level's rows)
+            if (t < Sn[i]) {
+                const VT c = last ? VT(pair_cheb2(cx[t], usC, Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one))
+                                  : VT(pair_cheb2(xp[t], usv[i], Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
+                s[srn[i]] = c;
+                if (uint32_t(i) == ie) out[xln[i]] = XT(c);
+            }
+            if (Sn[i] > kChainBS) {
+                const ChainLv<VT>& Ln = V.L[i + 1 < kChainMax ? i + 1 : i];
+                const uint32_t* qn = last ? C.qrow : Ln.qrow;
+                const uint32_t* sn = last ? C.src : Ln.src;
+                const uint32_t* xn = last ? C.xl : Ln.xl;
+                const uint32_t b0 = uload((last ? C.hdr : Ln.hdr) + kChainHdrW * b);
+                for (uint32_t q = t + kChainBS; q < Sn[i]; q += kChainBS) {
+                    const uint32_t r = qn[b0 + q];
+                    VT c;
+                    if (last) {
+                        const CT us = SCALE ? CT(double(pu1[r]) * V.a1c) : CT(pu1[r]);
+                        c = VT(pair_cheb2(cx[q], us, Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
+                    } else {
+                        c = VT(pair_cheb2(xp[q], Vi.u1s[y * Vi.n + r], Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
+                    }
+                    s[sn[b0 + q]] = c;
+                    if (uint32_t(i) == ie) out[xn[b0 + q]] = XT(c);
+                }
+            }
+            // the isolated rows of a level completing its second visit
+            if (uint32_t(i) > ie) {
+                if (t < S[i] && t >= nkeep[i]) s[t] = xiso[i];
+                const ChainVec<VT>& P = V.V[i > 0 ? i - 1 : 0];
+                for (uint32_t q = t + kChainBS; q < S[i]; q += kChainBS)
+                    if (q >= nkeep[i]) s[q] = P.x2[y * P.n + L.qrow[base[i] + q]];
+            }
+            level_sync<kChainBS>();
+            const VT x = sweep_U<kChainBS, VT, false>(ou[i], nlU[i], L.Ucol, L.Uval, z[i], (const VT*)nullptr, s);
+            if (uint32_t(i) == ie && t < nu[i]) out[ux[i]] = XT(x);
+            level_sync<kChainBS>();
+            cur ^= 1;
+        }
+    } else {
+        // passthrough rows that reach level ie + 1: their combines from the deepest level up
+        const uint32_t tp = (blockIdx.x - nblk) * kChainBS + t;
+        bool have = false;
+        VT v = VT(0);
+        if (gflag) return;
+#pragma unroll
+        for (int i = kChainMax - 1; i >= 0; i--) {
+            if (uint32_t(i) < ie || uint32_t(i) >= nv) continue;
+            const ChainVec<VT>& Vi = V.V[i];
+            const bool last = uint32_t(i) + 1 == nv;
+            const ChainLv<VT>& Ln = V.L[i + 1 < kChainMax ? i + 1 : i];
+            const uint32_t npn = last ? C.npt : Ln.npt;
+            if (tp >= npn) continue;                  // the row does not reach level i+1
+            const uint32_t qn = (last ? C.nslots : Ln.nslots) + tp;
+            const uint32_t r = last ? C.qrow[qn] : Ln.qrow[qn];
+            if (last) {                               // isolated on the coarsest level
+                const CT g = sell_row<CT, VT, SCALE>(V.A2, pu1, qn, V.a1c) - CT(pF[qn]);
+                const CT xc = C.dval[qn] * g;
+                const CT us = SCALE ? CT(double(pu1[r]) * V.a1c) : CT(pu1[r]);
+                v = VT(pair_cheb2(xc, us, Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
+            } else {
+                if (!have) v = Vi.x2[y * Vi.n + r];   // isolated on level i+1
+                v = VT(pair_cheb2(v, Vi.u1s[y * Vi.n + r], Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
+            }
+            have = true;
+        }
+        if (have) {
+#pragma unroll
+            for (int i = 0; i < kChainMax; i++)
+                if (uint32_t(i) == ie) out[V.L[i].qrow[V.L[i].nslots + tp]] = XT(v);
+        }
+    }
+}
+
 // step 2: x2 = (-alpha) x2 ; x2 += (1+beta) u1   (cheb_solve.cc:79-81), then store or scatter
 template <class VT, class XT, bool STORE, bool BAT>
 __global__ void __launch_bounds__(kBS) k_cheb2(uint32_t n, const VT* __restrict__ x2, const VT* __restrict__ u1s,

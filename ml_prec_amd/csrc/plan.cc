@@ -573,4 +573,307 @@ void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double*
     });
 }
 
+bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_nodes, uint32_t max_slots,
+                ChainProgram& out) {
+    const uint32_t nl = uint32_t(L.size());
+    if (nl < 2) return false;
+    // the structure the kernels rely on (as plan_level checks it)
+    for (uint32_t k = 0; k < nl; k++) {
+        const LevelArrays& lv = L[k];
+        const uint32_t M = lv.M, K = lv.N - lv.Md;
+        if (lv.M + lv.Md > lv.N) throw Error("level sizes inconsistent", 1);
+        for (uint32_t i = 0; i < K; i++)
+            for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++)
+                if (lv.L_ja[p] >= i || lv.L_ja[p] >= M) throw Error("L row has a column outside the eliminated block", 1);
+        for (uint32_t i = 0; i < M; i++) {
+            if (lv.U_ia[i + 1] <= lv.U_ia[i] || lv.U_ja[lv.U_ia[i]] != i)
+                throw Error("U row does not start with its pivot", 1);
+            for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++)
+                if (lv.U_ja[p] <= i || lv.U_ja[p] >= K) throw Error("U row has a column outside (i, N-Md)", 1);
+        }
+        if (k + 1 < nl && L[k + 1].N != lv.N - lv.M - lv.Md) throw Error("level sizes do not chain", 1);
+    }
+    if (L[nl - 1].M + L[nl - 1].Md != L[nl - 1].N) return false;       // the last level keeps no rows
+    const uint32_t NA = L[0].N;
+    // identity of every row: its natural row on level a.  perm[k][i]: identity of permuted row i of level k
+    std::vector<std::vector<uint32_t>> perm(nl);
+    for (uint32_t k = 0; k < nl; k++) {
+        const LevelArrays& lv = L[k];
+        std::vector<uint32_t> nat(lv.N);
+        if (k == 0) std::iota(nat.begin(), nat.end(), 0u);
+        else {
+            const LevelArrays& up = L[k - 1];
+            for (uint32_t r = 0; r < lv.N; r++) nat[r] = perm[k - 1][up.M + r];
+        }
+        perm[k].resize(lv.N);
+        for (uint32_t i = 0; i < lv.N; i++) {
+            if (lv.map[i] >= lv.N) throw Error("level map out of range", 1);
+            perm[k][i] = nat[lv.map[i]];
+        }
+    }
+    // deepest level of every identity; components of the couplings of all levels
+    std::vector<uint8_t> deep(NA, 0);
+    for (uint32_t k = 1; k < nl; k++)
+        for (uint32_t i = 0; i < L[k].N; i++) deep[perm[k][i]] = uint8_t(k);
+    DSU dsu(NA);
+    std::vector<char> has_node(NA, 0);
+    for (uint32_t k = 0; k < nl; k++) {
+        const LevelArrays& lv = L[k];
+        const uint32_t K = lv.N - lv.Md;
+        for (uint32_t i = 0; i < K; i++) {
+            if (i < lv.M || lv.L_ia[i + 1] > lv.L_ia[i]) has_node[perm[k][i]] = 1;
+            for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) dsu.unite(perm[k][i], perm[k][lv.L_ja[p]]);
+        }
+        for (uint32_t i = 0; i < lv.M; i++)
+            for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) dsu.unite(perm[k][i], perm[k][lv.U_ja[p]]);
+    }
+    // components with a node: their rows by identity; the others are passthrough rows
+    std::vector<uint32_t> root(NA);
+    for (uint32_t x = 0; x < NA; x++) root[x] = dsu.find(x);
+    std::vector<char> comp_node(NA, 0);
+    for (uint32_t x = 0; x < NA; x++)
+        if (has_node[x]) comp_node[root[x]] = 1;
+    std::vector<uint32_t> comp_start(NA + 1, 0), comp_rows;
+    for (uint32_t x = 0; x < NA; x++)
+        if (comp_node[root[x]]) comp_start[root[x] + 1]++;
+    for (uint32_t x = 0; x < NA; x++) comp_start[x + 1] += comp_start[x];
+    comp_rows.resize(comp_start[NA]);
+    {
+        std::vector<uint32_t> fill(comp_start.begin(), comp_start.end() - 1);
+        for (uint32_t x = 0; x < NA; x++)
+            if (comp_node[root[x]]) comp_rows[fill[root[x]]++] = x;   // ascending identity within a component
+    }
+    std::vector<uint32_t> comps;                                        // component roots
+    std::vector<uint8_t> comp_deep(NA, 0);
+    for (uint32_t x = 0; x < NA; x++)
+        if (comp_node[root[x]]) comp_deep[root[x]] = std::max(comp_deep[root[x]], deep[x]);
+    for (uint32_t x = 0; x < NA; x++)
+        if (comp_node[x] && root[x] == x) comps.push_back(x);
+    // deepest level first, then by smallest identity (the root is the smallest identity of its component)
+    std::stable_sort(comps.begin(), comps.end(),
+                     [&](uint32_t u, uint32_t v) { return comp_deep[u] > comp_deep[v]; });
+    std::vector<uint32_t> pt;                                           // passthrough identities
+    for (uint32_t x = 0; x < NA; x++)
+        if (!comp_node[root[x]]) pt.push_back(x);
+    std::stable_sort(pt.begin(), pt.end(), [&](uint32_t u, uint32_t v) { return deep[u] > deep[v]; });
+    std::vector<uint32_t> at(NA, kNoNode);                             // scratch: identity -> permuted row (level k)
+    auto is_node = [&](uint32_t k, uint32_t i) {
+        const LevelArrays& lv = L[k];
+        return i < lv.M || (i < lv.N - lv.Md && lv.L_ia[i + 1] > lv.L_ia[i]);
+    };
+    // per-level node / slot counts of each component, then the blocks
+    std::vector<std::vector<uint32_t>> cnodes(nl, std::vector<uint32_t>(comps.size(), 0)),
+        cslots(nl, std::vector<uint32_t>(comps.size(), 0));
+    for (uint32_t k = 0; k < nl; k++) {
+        for (uint32_t i = 0; i < L[k].N; i++) at[perm[k][i]] = i;
+        for (size_t c = 0; c < comps.size(); c++) {
+            const uint32_t r = comps[c];
+            for (uint32_t e = comp_start[r]; e < comp_start[r + 1]; e++) {
+                const uint32_t x = comp_rows[e];
+                if (deep[x] < k) continue;
+                cslots[k][c]++;
+                if (is_node(k, at[x])) cnodes[k][c]++;
+            }
+        }
+        for (uint32_t i = 0; i < L[k].N; i++) at[perm[k][i]] = kNoNode;
+    }
+    std::vector<uint32_t> bstart;                                       // first component of each block
+    {
+        std::vector<uint32_t> bn(nl, 0), bsl(nl, 0);
+        for (size_t c = 0; c < comps.size(); c++) {
+            // components are packed while every level keeps at most max_nodes nodes and max_nodes slots (one
+            // slot per thread); a component with more slots than that gets a block of its own
+            bool fits = !bstart.empty();
+            for (uint32_t k = 0; k < nl && fits; k++)
+                fits = bn[k] + cnodes[k][c] <= max_nodes && bsl[k] + cslots[k][c] <= max_nodes;
+            if (!fits) {
+                for (uint32_t k = 0; k < nl; k++) {
+                    if (cnodes[k][c] > max_nodes || cslots[k][c] > max_slots) return false;
+                    bn[k] = bsl[k] = 0;
+                }
+                bstart.push_back(uint32_t(c));
+            }
+            for (uint32_t k = 0; k < nl; k++) { bn[k] += cnodes[k][c]; bsl[k] += cslots[k][c]; }
+        }
+        bstart.push_back(uint32_t(comps.size()));
+    }
+    const uint32_t nblk = uint32_t(bstart.size()) - 1;
+    ChainProgram P;
+    P.a = a;
+    P.nlev = nl;
+    P.nblk = nblk;
+    P.lv.resize(nl);
+    P.blk_from.assign(nl, 0);
+    for (uint32_t b = 0; b < nblk; b++) {
+        const uint32_t d = comp_deep[comps[bstart[b]]];                // the block's deepest level
+        for (uint32_t k = 0; k <= d; k++) P.blk_from[k] = b + 1;
+    }
+    // per level: slots of every block, then the passthrough rows
+    std::vector<uint32_t> rows, dL, dU;
+    std::vector<uint32_t> prev_slot(NA, kNoNode), cur_slot(NA, kNoNode);   // identity -> slot in its block
+    std::vector<uint32_t> prev_nat(NA, kNoNode);                            // identity -> natural row, level above
+    for (uint32_t k = 0; k < nl; k++) {
+        const LevelArrays& lv = L[k];
+        ChainLevelProgram& C = P.lv[k];
+        const uint32_t N = lv.N, M = lv.M, K = N - lv.Md;
+        C.N = N; C.M = M; C.Md = lv.Md;
+        for (uint32_t i = 0; i < N; i++) at[perm[k][i]] = i;
+        // depths (L: over eliminated columns; U: over eliminated columns, kept columns are known beforehand)
+        dL.assign(K, 0);
+        for (uint32_t i = 0; i < K; i++)
+            for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) dL[i] = std::max(dL[i], dL[lv.L_ja[p]] + 1);
+        dU.assign(M, 0);
+        for (uint32_t i = M; i-- > 0;)
+            for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++)
+                if (lv.U_ja[p] < M) dU[i] = std::max(dU[i], dU[lv.U_ja[p]] + 1);
+        C.hdr.assign(size_t(kChainHdr) * nblk, 0);
+        C.Lptr.push_back(0);
+        C.Uptr.push_back(0);
+        C.dval.assign(N, 0.0);
+        uint32_t pos = 0;
+        std::vector<uint32_t> nodes, flat, iso, ul;
+        for (uint32_t b = 0; b < nblk; b++) {
+            nodes.clear(); flat.clear(); iso.clear(); ul.clear();
+            for (uint32_t c = bstart[b]; c < bstart[b + 1]; c++) {
+                const uint32_t r = comps[c];
+                for (uint32_t e = comp_start[r]; e < comp_start[r + 1]; e++) {
+                    const uint32_t x = comp_rows[e];
+                    if (deep[x] < k) continue;
+                    const uint32_t i = at[x];
+                    if (is_node(k, i)) nodes.push_back(i);
+                    else if (i < K) flat.push_back(i);
+                    else iso.push_back(i);
+                    if (i < M) ul.push_back(i);
+                }
+            }
+            std::stable_sort(nodes.begin(), nodes.end(), [&](uint32_t u, uint32_t v) {
+                return dL[u] != dL[v] ? dL[u] < dL[v] : u < v;
+            });
+            std::sort(flat.begin(), flat.end());
+            std::sort(iso.begin(), iso.end());
+            std::stable_sort(ul.begin(), ul.end(), [&](uint32_t u, uint32_t v) {
+                return dU[u] != dU[v] ? dU[u] < dU[v] : u < v;
+            });
+            uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
+            const uint32_t S = uint32_t(nodes.size() + flat.size() + iso.size());
+            H[0] = pos;
+            H[1] = S;
+            H[2] = uint32_t(nodes.size());
+            H[3] = uint32_t(flat.size());
+            uint32_t s = 0;
+            for (auto* v : {&nodes, &flat, &iso})
+                for (uint32_t i : *v) cur_slot[perm[k][i]] = s++;
+            uint32_t nlv = 0;
+            for (uint32_t i : nodes) nlv = std::max(nlv, dL[i] + 1);
+            H[4] = nodes.empty() ? 0 : nlv;
+            s = 0;
+            for (auto* v : {&nodes, &flat, &iso})
+                for (uint32_t i : *v) {
+                    const uint32_t x = perm[k][i];
+                    C.qrow.push_back(lv.map[i]);
+                    if (k > 0) {
+                        if (prev_slot[x] == kNoNode) throw Error("chain: a row has no slot on the level above", 1);
+                        C.src.push_back(prev_slot[x]);
+                        C.xl.push_back(prev_nat[x]);
+                    }
+                    const bool node = s < nodes.size();
+                    C.Ldep.push_back(node ? dL[i] : 0u);
+                    if (node)
+                        for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
+                            C.Lloc.push_back(cur_slot[perm[k][lv.L_ja[p]]]);
+                            C.Lval.push_back(lv.L_a[p]);
+                        }
+                    C.Lptr.push_back(uint32_t(C.Lloc.size()));
+                    if (i >= K) C.dval[pos + s] = lv.dval[i - K];
+                    s++;
+                }
+            H[5] = uint32_t(C.Udep.size());
+            H[6] = uint32_t(ul.size());
+            uint32_t nuv = 0;
+            for (uint32_t i : ul) {
+                nuv = std::max(nuv, dU[i] + 1);
+                C.Udep.push_back(dU[i]);
+                C.uloc.push_back(cur_slot[perm[k][i]]);
+                C.uxidx.push_back(lv.map[i]);
+                C.Udiag.push_back(lv.U_a[lv.U_ia[i]]);
+                for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) {
+                    C.Ucol.push_back(cur_slot[perm[k][lv.U_ja[p]]]);
+                    C.Uval.push_back(lv.U_a[p]);
+                }
+                C.Uptr.push_back(uint32_t(C.Ucol.size()));
+            }
+            H[7] = ul.empty() ? 0 : nuv;
+            P.max_slots = std::max(P.max_slots, S);
+            P.max_nodes = std::max<uint32_t>(P.max_nodes, uint32_t(nodes.size()));
+            pos += S;
+        }
+        C.nslots = pos;
+        // passthrough rows on this level (a prefix of pt: deepest level descending)
+        for (uint32_t x : pt) {
+            if (deep[x] < k) break;
+            const uint32_t i = at[x];
+            if (i < M || (i < K && lv.L_ia[i + 1] > lv.L_ia[i])) throw Error("chain: a passthrough row is a node", 1);
+            if ((i >= K) != (deep[x] == k)) throw Error("chain: a passthrough row ends without being isolated", 1);
+            C.qrow.push_back(lv.map[i]);
+            if (i >= K) C.dval[pos] = lv.dval[i - K];
+            pos++;
+            C.npt++;
+        }
+        if (pos != N || C.qrow.size() != N) throw Error("chain: level rows are not covered exactly once", 1);
+        {
+            std::vector<char> seen(N, 0);
+            for (uint32_t j : C.qrow) {
+                if (j >= N || seen[j]) throw Error("chain: level rows are not covered exactly once", 1);
+                seen[j] = 1;
+            }
+        }
+        // this level's slots and natural rows, by identity, for the level below
+        for (uint32_t x = 0; x < NA; x++) { prev_slot[x] = kNoNode; prev_nat[x] = kNoNode; }
+        for (uint32_t i = 0; i < N; i++) {
+            const uint32_t x = perm[k][i];
+            prev_slot[x] = cur_slot[x];
+            prev_nat[x] = lv.map[i];
+            cur_slot[x] = kNoNode;
+        }
+        for (uint32_t i = 0; i < N; i++) at[perm[k][i]] = kNoNode;
+        // no empty device arrays
+        for (auto* v : {&C.src, &C.xl, &C.Lloc, &C.Udep, &C.uloc, &C.uxidx, &C.Ucol, &C.hdr})
+            if (v->empty()) v->push_back(0);
+        for (auto* v : {&C.Lval, &C.Uval, &C.Udiag})
+            if (v->empty()) v->push_back(0.0);
+    }
+    // index checks: every local index inside its block, every position inside its level
+    for (uint32_t k = 0; k < nl; k++) {
+        const ChainLevelProgram& C = P.lv[k];
+        for (uint32_t b = 0; b < nblk; b++) {
+            const uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
+            const uint32_t base = H[0], S = H[1], nn = H[2];
+            if (nn > S || H[2] + H[3] > S || base + S > C.nslots) throw Error("chain: block header out of range", 1);
+            for (uint32_t s = 0; s < nn; s++) {
+                if (C.Ldep[base + s] >= H[4]) throw Error("chain: L depth out of range", 1);
+                for (uint32_t p = C.Lptr[base + s]; p < C.Lptr[base + s + 1]; p++)
+                    if (C.Lloc[p] >= s) throw Error("chain: L column not earlier in its block", 1);
+            }
+            for (uint32_t s = nn; s < S; s++)
+                if (C.Lptr[base + s + 1] != C.Lptr[base + s]) throw Error("chain: a non-node slot has L entries", 1);
+            if (k > 0) {
+                const uint32_t* Hp = &P.lv[k - 1].hdr[size_t(kChainHdr) * b];
+                for (uint32_t s = 0; s < S; s++)
+                    if (C.src[base + s] >= Hp[1] || C.src[base + s] >= Hp[2] + Hp[3] ||
+                        C.xl[base + s] >= P.lv[k - 1].N)
+                        throw Error("chain: a row's source on the level above is not a kept slot", 1);
+            }
+            for (uint32_t u = H[5]; u < H[5] + H[6]; u++) {
+                if (C.uloc[u] >= nn || C.Udep[u] >= H[7]) throw Error("chain: U-list entry out of range", 1);
+                for (uint32_t p = C.Uptr[u]; p < C.Uptr[u + 1]; p++)
+                    if (C.Ucol[p] >= H[2] + H[3] || C.Ucol[p] == C.uloc[u]) throw Error("chain: U column out of range", 1);
+            }
+            if (S > max_slots || nn > max_nodes || H[6] > nn) throw Error("chain: block too large", 1);
+        }
+        if (k > 0 && C.npt > P.lv[k - 1].npt) throw Error("chain: passthrough rows do not nest", 1);
+    }
+    out = std::move(P);
+    return true;
+}
+
 }  // namespace mlp

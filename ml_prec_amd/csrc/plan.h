@@ -178,4 +178,61 @@ constexpr uint32_t kNoRef = 0xffffffffu;
 // level-l nodes, max_nodes level-(l+1) nodes or max_nodes U-list entries.
 bool plan_pair(const LevelArrays& la, const LevelArrays& lb, uint32_t max_nodes, PairProgram& P);
 
+// Chain program of the uh_cheb levels a..c (c the coarsest, which keeps no
+// rows; ncheb = 2 on a..c-1).  The recursion of Prec::solve
+// (cheb_solve.cc:13-123) then runs as alternating launches:
+//   descent from level j: the L sweep of level j from its input (the
+//     parent's F, or the Chebyshev residual of a second visit), then the L
+//     sweeps of the first visits of levels j+1..c-1, each from the F of the
+//     level above kept in LDS, then step 1 on level c (its L and U sweeps);
+//   ascent to level k: step 2 on level c from the residual A_c (a u1) - F,
+//     then for c-1, c-2, ..., k the Chebyshev combine of the kept rows and
+//     the U sweep, as long as the visit just completed was a second visit.
+// A level-(k+1) row is a kept row of level k, so every row has one identity
+// across the levels (its row on level a).  Blocks are formed from the
+// connected components of the L and U couplings of all levels a..c joined
+// over these identities; a block holds all levels' rows of its components.
+// Per level, a block's rows ("slots") are ordered: nodes (eliminated rows and
+// kept rows with L entries) by L depth, kept rows without L entries, isolated
+// rows.  Rows of no component with a node ("passthrough" rows: kept rows
+// without L entries down to the level where they are isolated) run outside
+// the blocks.
+//
+// Chain order of level k (the order its input vector F_{k-1} and the rows of
+// A_k are stored in): the slots of block 0, block 1, ..., then the
+// passthrough rows that exist on level k.  Blocks are sorted by their deepest
+// level (descending), so the blocks with rows on level >= k are a prefix
+// (blk_from); passthrough rows likewise (npt of each level).
+//
+// Block header (kChainHdr words per block and level): [0] chain position of
+// slot 0, [1] slots, [2] nodes, [3] kept rows without L entries, [4] L depth
+// levels, [5] first U-list entry, [6] U-list length, [7] U depth levels.
+constexpr uint32_t kChainHdr = 8;
+struct ChainLevelProgram {
+    uint32_t N = 0, M = 0, Md = 0;
+    uint32_t nslots = 0, npt = 0;        // block slots; passthrough rows at positions nslots + t
+    std::vector<uint32_t> hdr;           // kChainHdr per block
+    std::vector<uint32_t> qrow;          // N: natural row at chain position q
+    std::vector<uint32_t> src;           // nslots: slot of the same row on the level above (not on level a)
+    std::vector<uint32_t> xl;            // nslots: natural row on the level above (where its combine goes)
+    std::vector<uint32_t> Ldep, Lptr;    // nslots (+1): node slots only have entries
+    std::vector<uint32_t> Lloc;          // slot of the L column
+    std::vector<double> Lval;
+    std::vector<uint32_t> Udep, uloc, uxidx, Uptr, Ucol;   // U list; Ucol: slot (eliminated or kept row)
+    std::vector<double> Uval, Udiag;
+    std::vector<double> dval;            // N: isolated-row factor at its chain position (0 elsewhere)
+};
+struct ChainProgram {
+    uint32_t a = 0, nlev = 0, nblk = 0;
+    uint32_t max_slots = 0, max_nodes = 0;
+    std::vector<ChainLevelProgram> lv;   // levels a..c
+    std::vector<uint32_t> blk_from;      // nlev: blocks with rows on level a+i
+};
+// Plans the chain over levels L[0..] = a..c.  Components are packed into
+// blocks of at most max_nodes nodes and max_nodes slots per level; returns
+// false when one component alone has more than max_nodes nodes or max_slots
+// slots on some level.
+bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_nodes, uint32_t max_slots,
+                ChainProgram& P);
+
 }  // namespace mlp

@@ -51,7 +51,8 @@ def test_bf16_apply_error_vs_fp64_oracle(ctx, z85):
     # passes through it (2.9e-4 measured at this size)
     assert max(ebf) <= 2e-3
     assert min(ebf) > 10 * max(e32)
-    assert Bbf.launches_per_apply() == B32.launches_per_apply()
+    # the variant runs level 0 on its own two kernels (its BF16 vectors), the levels below as in FP32
+    assert Bbf.launches_per_apply() <= B32.launches_per_apply() + 2
     assert Bbf.bytes_per_apply() < B32.bytes_per_apply()
     Bbf.close()
     B32.close()

@@ -77,3 +77,42 @@ def test_matrix_file_formats(io_exe, tmp_path):
     want = [f"0 {n - 1} 0 {n - 1}"] + ["%d %d %s" % (i - 1, j - 1, _fmt_e(v)) for i, j, v in ent]
     assert open(prefix + ".hypre").read().splitlines() == want
     assert np.isfinite(a).all()
+
+
+@pytest.fixture(scope="module")
+def chain_exe(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("chain") / "chain_check")
+    src = os.path.join(HERE, "cpp", "chain_check.cc")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-pthread", "-o", out, src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return out
+
+
+def _write_levels(path, H):
+    """The hierarchy's level arrays (with each level's A_next) in the layout chain_check.cc reads."""
+    import numpy as np
+    with open(path, "wb") as fh:
+        np.array([H.nlevels], np.uint32).tofile(fh)
+        for l in range(H.nlevels):
+            info, arr = H.level_arrays(l)
+            np.array([info["N"], info["M"], info["Md"], info["nnz_L"], info["nnz_U"], info["n_next"],
+                      info["nnz_next"]], np.uint32).tofile(fh)
+            for k in ("map", "L_ia", "L_ja", "L_a", "U_ia", "U_ja", "U_a", "dval", "A_ia", "A_ja", "A_a"):
+                arr[k].tofile(fh)
+
+
+@pytest.mark.parametrize("name", ["spe10_z12", "ln5_tpfa_c1", "spe10_z40"])
+def test_chain_launches_equal_recursion(chain_exe, tmp_path, name):
+    """The chain program (plan.cc: plan_chain) and the descent/ascent launch sequence, executed on the host slot
+    by slot as the device kernels do, give Prec::solve's result (cheb_solve.cc:13-123, plain recursion over the
+    level arrays) bit for bit, for chains starting at every level."""
+    from conftest import system
+    from ml_prec_amd import native as N
+    s = system(name)
+    path = str(tmp_path / "levels.bin")
+    _write_levels(path, N.build_uh_cheb(s.ia, s.ja, s.a))
+    r = subprocess.run([chain_exe, path], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0 and r.stdout.rstrip().endswith("OK"), r.stdout + r.stderr
+    assert " 0 of " in r.stdout
