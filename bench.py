@@ -115,7 +115,7 @@ def ensemble_cpu_baseline(spec, eps, procs):
                       f"{wall:.1f} s", "iterations": [it for _, it in res]}
 
 
-def ensemble(N, S, args):
+def ensemble(N, S, args, emit=True):
     """Ensemble: COUNT SPE10-shaped systems of 60x220xnz cells, nz drawn from 5/10/20/40/85, each with its own
     field, uh_cheb hierarchy and context (stream).  A step solves all of them to 1e-8 from x0 = 0, concurrently
     through mlp_pcg_batch; the same systems solved one after another are timed beside it."""
@@ -185,13 +185,15 @@ def ensemble(N, S, args):
         "worst_true_rel_res": worst,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
     for ctx, A, B, b, x in jobs:
         B.close()
         A.close()
     del b, x
     for j in jobs:
         j[0].close()
+    return out
 
 
 def _rhs_cpu_job(item):
@@ -359,7 +361,7 @@ def adjoint_cpu_baseline(S, shape, theta, b, obs, d, eps=1e-10):
                       f"({st_f['niter']} and {st_a['niter']} iterations), numpy gradient formula"}, grad.ravel()
 
 
-def adjoint(N, S, args):
+def adjoint(N, S, args, emit=True):
     """Adjoint gradient: L = sum over 200 observed cells of (p_i - d_i)^2 with d = p(true field) + N(0, 0.01|p_i|);
     theta = ln kx of a second field; one gradient = forward solve + adjoint solve (1e-10) + gradient kernel.
     Checked against central differences (h = 1e-6) along --fd-dirs seeded directions."""
@@ -425,9 +427,11 @@ def adjoint(N, S, args):
                      "median_rel_diff": float(np.median(errs))},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
     model.close()
     ctx.close()
+    return out
 
 
 MSPLIT_CFG = dict(q=(0.8,), niters=(5,), order="simple_1", degree_max=2)   # project/msplit.cfg:7-77
@@ -762,6 +766,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the ensemble and adjoint summaries of the default run")
     ap.add_argument("--spmv-reps", type=int, default=200)
     ap.add_argument("--spmv-warm-reps", type=int, default=None, help="0 skips the cache-warm SpMV timing")
     ap.add_argument("--graph", type=int, default=1)
@@ -864,6 +870,34 @@ def main():
     log(f"mixed: {msm:.3f} ms/solve, {stm[-1]['niter']} iterations, true rel res {stm[-1]['true_rel_res']:.3e}, "
         f"rel err vs FP64 GPU {err_mixed:.3e}")
 
+    # the same solves to 1e-10, and the BF16 variant of the mixed solve (level 0's Chebyshev vectors in BF16)
+    steps10 = max(3, args.steps // 2)
+
+    def timed(fn):
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        st = [fn() for _ in range(steps10)]
+        ctx.sync()
+        return 1e3 * (time.perf_counter() - t0) / steps10, st[-1]
+
+    ms64_10, st64_10 = timed(lambda: N.pcg(ctx, A, B64, b, x, eps=1e-10, maxit=10000, use_graph=bool(args.graph),
+                                           check_every=8))
+    x64_10 = x.get()
+    msm_10, stm_10 = timed(lambda: N.pcg(ctx, A, B32, b, x, eps=1e-10, maxit=10000, flexible=True,
+                                         true_res_every=20, use_graph=bool(args.graph), check_every=8))
+    err_m10 = float(np.linalg.norm(x.get() - x64_10) / np.linalg.norm(x64_10))
+    log(f"to 1e-10: FP64 {ms64_10:.3f} ms ({st64_10['niter']} it), mixed {msm_10:.3f} ms ({stm_10['niter']} it), "
+        f"mixed/FP64 {ms64_10 / msm_10:.3f}x, rel err vs FP64 {err_m10:.2e}")
+    Bbf = N.DevPrec(ctx, A, H, precision=N.MLP_FP32_BF16)
+    msbf, stbf = timed(lambda: N.pcg(ctx, A, Bbf, b, x, eps=args.eps, maxit=10000, flexible=True, true_res_every=20,
+                                     use_graph=bool(args.graph), check_every=8))
+    err_bf = float(np.linalg.norm(x.get() - x64) / np.linalg.norm(x64))
+    applybf_ms = min(Bbf.time_apply(b, x, reps=20) for _ in range(3))
+    log(f"mixed with BF16 Chebyshev vectors on level 0: {msbf:.3f} ms ({stbf['niter']} it), apply "
+        f"{applybf_ms * 1e3:.1f} us, rel err vs FP64 {err_bf:.2e}")
+    Bbf.close()
+
     # SpMV roofline: algorithmic bytes (w+4)*nnz + 4(N+1) + 2wN, cold L2 and Infinity Cache
     bytes_spmv = 12.0 * s.nnz + 4.0 * (s.n + 1) + 16.0 * s.n
     xv = ctx.to_device(np.random.default_rng(0).standard_normal(s.n))
@@ -929,14 +963,24 @@ def main():
         "prec_apply_us": {"fp64": round(apply64_ms * 1e3, 2), "fp32": round(apply32_ms * 1e3, 2)},
         "prec_apply_gbs_algorithmic": {"fp64": round(by64 / apply64_ms / 1e6, 1),
                                        "fp32": round(by32 / apply32_ms / 1e6, 1)},
+        "eps_1e-10": {"fp64_ms_per_solve": round(ms64_10, 4), "mixed_ms_per_solve": round(msm_10, 4),
+                      "speedup_mixed_vs_fp64": round(ms64_10 / msm_10, 3), "niter_fp64": int(st64_10["niter"]),
+                      "niter_mixed": int(stm_10["niter"]), "true_rel_res_mixed": stm_10["true_rel_res"],
+                      "mixed_rel_err_vs_fp64": err_m10, "steps": steps10},
+        "mixed_bf16": {"ms_per_solve": round(msbf, 4), "niter": int(stbf["niter"]),
+                       "true_rel_res": stbf["true_rel_res"], "rel_err_vs_fp64": err_bf,
+                       "prec_apply_us": round(applybf_ms * 1e3, 2),
+                       "what": "FP32 hierarchy with level 0's Chebyshev vectors (F, u1, a1 u1, x2) in BF16, FP64 "
+                               "flexible PCG"},
         "t_const_host_s": round(t_const, 3),
         # the dominant work of the headline (mixed) solve: the FP32-hierarchy preconditioner apply, one "launch"
         # being one apply (its level kernels back to back, HIP events on the context stream)
         "roofline": {
-            "kernel": f"uh_cheb preconditioner apply, FP32 hierarchy: {nl32} level-kernel launches per apply "
-                      "(~80% of the mixed solve); achieved = algorithmic bytes per apply (sum over level visits of "
-                      "L/U sweeps, level vectors, index arrays and the fused Chebyshev-residual SpMVs) / apply time; "
-                      "bound in practice by launch and dependent-load latency, not by bandwidth",
+            "kernel": f"uh_cheb preconditioner apply, FP32 hierarchy: {nl32} launches per apply (the descent and "
+                      f"ascent kernels of the level chain; {100 * apply32_ms * int(stm[-1]['niter']) / msm:.0f}% of "
+                      "the mixed solve); achieved = algorithmic bytes per apply (sum over level visits of L/U sweeps, "
+                      "level vectors, index arrays and the fused Chebyshev-residual SpMVs) / apply time; bound in "
+                      "practice by dependent-load and sweep-depth latency, not by bandwidth",
             "bound": "hbm",
             "achieved": round(by32 / apply32_ms / 1e6, 1),
             "peak": HBM_PEAK_GBS,
@@ -974,11 +1018,28 @@ def main():
         },
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     for o in (B32, B64, A):
         o.close()
     del xv, yv, f, z, b, x
     ctx.close()
+    if not args.no_extras:
+        # the other workload configurations, summarised (their own modes print the full records)
+        import copy
+        a2 = copy.copy(args)
+        a2.no_cpu, a2.steps, a2.warmup = True, 3, 1
+        a2.ensemble = 64
+        e = ensemble(N, S, a2, emit=False)
+        out["ensemble_64"] = {"solves_per_s": e["value"], "ms_all_64": e["ms_per_step"],
+                              "ms_one_after_another": e["ms_one_after_another"],
+                              "speedup_concurrent": e["speedup_concurrent"],
+                              "iterations": [min(e["iterations"]), max(e["iterations"])],
+                              "worst_true_rel_res": e["worst_true_rel_res"], "workload": e["config"]["workload"]}
+        a2.steps, a2.fd_dirs = 3, 2
+        d = adjoint(N, S, a2, emit=False)
+        out["adjoint"] = {"ms_per_gradient": d["value"], "iterations": d["iterations"],
+                          "fd_max_rel_diff": d["fd_check"]["max_rel_diff"], "fd_directions": 2,
+                          "workload": d["config"]["workload"]}
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -82,6 +82,31 @@ def test_uh_cheb_apply_bitexact(ctx, name, sigmas, niters):
         assert np.array_equal(got, ref), (name, np.abs(got - ref).max())
 
 
+@pytest.mark.parametrize("name,sigmas,niters", CHEB_CASES[:2] + [CHEB_CASES[3], ("spe10_z40", (3.0,), (2,))])
+def test_uh_cheb_apply_without_chain_bitexact(ctx, name, sigmas, niters, monkeypatch):
+    """The same applies on the launch schedule without the chain (MLP_NO_CHAIN=1): the level kernels and the
+    fused last pair.  Both schedules give the oracle's bits; the chain needs fewer launches."""
+    s = system(name)
+    P = O.cheb_prec(s.ia, s.ja, s.a, sigmas=sigmas, niters=niters)
+    H = N.build_uh_cheb(s.ia, s.ja, s.a, sigmas=sigmas, niters=niters)
+    A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+    monkeypatch.setenv("MLP_NO_CHAIN", "1")
+    B0 = N.DevPrec(ctx, A, H)
+    monkeypatch.delenv("MLP_NO_CHAIN")
+    B1 = N.DevPrec(ctx, A, H)
+    f = rng(5).standard_normal(s.n)
+    for B in (B0, B1):
+        x = ctx.array(s.n)
+        B.apply(dev(ctx, f), x)
+        assert np.array_equal(x.get(), P.apply(f)), name
+    print(f"{name} ncheb {niters}: launches per apply {B0.launches_per_apply()} without the chain, "
+          f"{B1.launches_per_apply()} with it")
+    assert B1.launches_per_apply() < B0.launches_per_apply()
+    B1.close()
+    B0.close()
+    A.close()
+
+
 MS_CASES = [("spe10_z12", "simple_1", 2, (5,)), ("spe10_z12", "original", 3, (5,)),
             ("spe10_z12", "none", 3, (2, 3)), ("ln5_tpfa_c1", "original", 3, (3,)),
             ("small_spe10", "simple_1", 2, (4,))]
