@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <exception>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -649,13 +650,20 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
         if (comp_node[root[x]]) comp_deep[root[x]] = std::max(comp_deep[root[x]], deep[x]);
     for (uint32_t x = 0; x < NA; x++)
         if (comp_node[x] && root[x] == x) comps.push_back(x);
-    // deepest level first, then by smallest identity (the root is the smallest identity of its component)
-    std::stable_sort(comps.begin(), comps.end(),
-                     [&](uint32_t u, uint32_t v) { return comp_deep[u] > comp_deep[v]; });
+    // deepest level first, then by smallest identity (the root is the smallest identity of its component):
+    // a stable counting sort on the level
+    auto by_level_desc = [nl](std::vector<uint32_t>& v, auto level) {
+        std::vector<uint32_t> cnt(nl + 1, 0), out(v.size());
+        for (uint32_t x : v) cnt[nl - 1 - level(x) + 1]++;
+        for (uint32_t k = 0; k < nl; k++) cnt[k + 1] += cnt[k];
+        for (uint32_t x : v) out[cnt[nl - 1 - level(x)]++] = x;
+        v.swap(out);
+    };
+    by_level_desc(comps, [&](uint32_t r) { return uint32_t(comp_deep[r]); });
     std::vector<uint32_t> pt;                                           // passthrough identities
     for (uint32_t x = 0; x < NA; x++)
         if (!comp_node[root[x]]) pt.push_back(x);
-    std::stable_sort(pt.begin(), pt.end(), [&](uint32_t u, uint32_t v) { return deep[u] > deep[v]; });
+    by_level_desc(pt, [&](uint32_t x) { return uint32_t(deep[x]); });
     std::vector<uint32_t> at(NA, kNoNode);                             // scratch: identity -> permuted row (level k)
     auto is_node = [&](uint32_t k, uint32_t i) {
         const LevelArrays& lv = L[k];
@@ -708,140 +716,173 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
         const uint32_t d = comp_deep[comps[bstart[b]]];                // the block's deepest level
         for (uint32_t k = 0; k <= d; k++) P.blk_from[k] = b + 1;
     }
-    // per level: slots of every block, then the passthrough rows
-    std::vector<uint32_t> rows, dL, dU;
-    std::vector<uint32_t> prev_slot(NA, kNoNode), cur_slot(NA, kNoNode);   // identity -> slot in its block
-    std::vector<uint32_t> prev_nat(NA, kNoNode);                            // identity -> natural row, level above
-    for (uint32_t k = 0; k < nl; k++) {
-        const LevelArrays& lv = L[k];
-        ChainLevelProgram& C = P.lv[k];
-        const uint32_t N = lv.N, M = lv.M, K = N - lv.Md;
-        C.N = N; C.M = M; C.Md = lv.Md;
-        for (uint32_t i = 0; i < N; i++) at[perm[k][i]] = i;
-        // depths (L: over eliminated columns; U: over eliminated columns, kept columns are known beforehand)
-        dL.assign(K, 0);
-        for (uint32_t i = 0; i < K; i++)
-            for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) dL[i] = std::max(dL[i], dL[lv.L_ja[p]] + 1);
-        dU.assign(M, 0);
-        for (uint32_t i = M; i-- > 0;)
-            for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++)
-                if (lv.U_ja[p] < M) dU[i] = std::max(dU[i], dU[lv.U_ja[p]] + 1);
-        C.hdr.assign(size_t(kChainHdr) * nblk, 0);
-        C.Lptr.push_back(0);
-        C.Uptr.push_back(0);
-        C.dval.assign(N, 0.0);
-        uint32_t pos = 0;
-        std::vector<uint32_t> nodes, flat, iso, ul;
-        for (uint32_t b = 0; b < nblk; b++) {
-            nodes.clear(); flat.clear(); iso.clear(); ul.clear();
-            for (uint32_t c = bstart[b]; c < bstart[b + 1]; c++) {
-                const uint32_t r = comps[c];
-                for (uint32_t e = comp_start[r]; e < comp_start[r + 1]; e++) {
-                    const uint32_t x = comp_rows[e];
-                    if (deep[x] < k) continue;
-                    const uint32_t i = at[x];
-                    if (is_node(k, i)) nodes.push_back(i);
-                    else if (i < K) flat.push_back(i);
-                    else iso.push_back(i);
-                    if (i < M) ul.push_back(i);
-                }
-            }
-            std::stable_sort(nodes.begin(), nodes.end(), [&](uint32_t u, uint32_t v) {
-                return dL[u] != dL[v] ? dL[u] < dL[v] : u < v;
-            });
-            std::sort(flat.begin(), flat.end());
-            std::sort(iso.begin(), iso.end());
-            std::stable_sort(ul.begin(), ul.end(), [&](uint32_t u, uint32_t v) {
-                return dU[u] != dU[v] ? dU[u] < dU[v] : u < v;
-            });
-            uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
-            const uint32_t S = uint32_t(nodes.size() + flat.size() + iso.size());
-            H[0] = pos;
-            H[1] = S;
-            H[2] = uint32_t(nodes.size());
-            H[3] = uint32_t(flat.size());
-            uint32_t s = 0;
-            for (auto* v : {&nodes, &flat, &iso})
-                for (uint32_t i : *v) cur_slot[perm[k][i]] = s++;
-            uint32_t nlv = 0;
-            for (uint32_t i : nodes) nlv = std::max(nlv, dL[i] + 1);
-            H[4] = nodes.empty() ? 0 : nlv;
-            s = 0;
-            for (auto* v : {&nodes, &flat, &iso})
-                for (uint32_t i : *v) {
-                    const uint32_t x = perm[k][i];
-                    C.qrow.push_back(lv.map[i]);
-                    if (k > 0) {
-                        if (prev_slot[x] == kNoNode) throw Error("chain: a row has no slot on the level above", 1);
-                        C.src.push_back(prev_slot[x]);
-                        C.xl.push_back(prev_nat[x]);
-                    }
-                    const bool node = s < nodes.size();
-                    C.Ldep.push_back(node ? dL[i] : 0u);
-                    if (node)
-                        for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
-                            C.Lloc.push_back(cur_slot[perm[k][lv.L_ja[p]]]);
-                            C.Lval.push_back(lv.L_a[p]);
+    // per level: slots of every block, then the passthrough rows.  Pass 1 orders each level's slots and
+    // records every identity's slot; pass 2 writes the arrays (the source slots of a level are the slots of
+    // the level above).  Both run one thread per level.
+    std::vector<std::vector<uint32_t>> inv(nl), slot(nl), order(nl), uorder(nl), dLk(nl), dUk(nl);
+    std::vector<std::exception_ptr> err(nl);
+    parallel_for(nl, 1, [&](uint64_t k0, uint64_t k1) {
+        for (uint32_t k = uint32_t(k0); k < k1; k++) {
+            try {
+                const LevelArrays& lv = L[k];
+                ChainLevelProgram& C = P.lv[k];
+                const uint32_t N = lv.N, M = lv.M, K = N - lv.Md;
+                C.N = N; C.M = M; C.Md = lv.Md;
+                std::vector<uint32_t>& at = inv[k];
+                at.assign(NA, kNoNode);
+                for (uint32_t i = 0; i < N; i++) at[perm[k][i]] = i;
+                // depths (L: over eliminated columns; U: over eliminated columns, kept columns are known
+                // beforehand)
+                std::vector<uint32_t>& dL = dLk[k];
+                std::vector<uint32_t>& dU = dUk[k];
+                dL.assign(K, 0);
+                for (uint32_t i = 0; i < K; i++)
+                    for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) dL[i] = std::max(dL[i], dL[lv.L_ja[p]] + 1);
+                dU.assign(M, 0);
+                for (uint32_t i = M; i-- > 0;)
+                    for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++)
+                        if (lv.U_ja[p] < M) dU[i] = std::max(dU[i], dU[lv.U_ja[p]] + 1);
+                C.hdr.assign(size_t(kChainHdr) * nblk, 0);
+                slot[k].assign(NA, kNoNode);
+                uint32_t pos = 0;
+                std::vector<uint32_t> nodes, flat, iso, ul;
+                for (uint32_t b = 0; b < nblk; b++) {
+                    nodes.clear(); flat.clear(); iso.clear(); ul.clear();
+                    for (uint32_t c = bstart[b]; c < bstart[b + 1]; c++) {
+                        const uint32_t r = comps[c];
+                        for (uint32_t e = comp_start[r]; e < comp_start[r + 1]; e++) {
+                            const uint32_t x = comp_rows[e];
+                            if (deep[x] < k) continue;
+                            const uint32_t i = at[x];
+                            if (is_node(k, i)) nodes.push_back(i);
+                            else if (i < K) flat.push_back(i);
+                            else iso.push_back(i);
+                            if (i < M) ul.push_back(i);
                         }
-                    C.Lptr.push_back(uint32_t(C.Lloc.size()));
-                    if (i >= K) C.dval[pos + s] = lv.dval[i - K];
-                    s++;
+                    }
+                    // (total orders: by depth, then by row)
+                    std::sort(nodes.begin(), nodes.end(), [&](uint32_t u, uint32_t v) {
+                        return dL[u] != dL[v] ? dL[u] < dL[v] : u < v;
+                    });
+                    std::sort(flat.begin(), flat.end());
+                    std::sort(iso.begin(), iso.end());
+                    std::sort(ul.begin(), ul.end(), [&](uint32_t u, uint32_t v) {
+                        return dU[u] != dU[v] ? dU[u] < dU[v] : u < v;
+                    });
+                    uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
+                    H[0] = pos;
+                    H[1] = uint32_t(nodes.size() + flat.size() + iso.size());
+                    H[2] = uint32_t(nodes.size());
+                    H[3] = uint32_t(flat.size());
+                    uint32_t nlv = 0, nuv = 0;
+                    for (uint32_t i : nodes) nlv = std::max(nlv, dL[i] + 1);
+                    for (uint32_t i : ul) nuv = std::max(nuv, dU[i] + 1);
+                    H[4] = nlv;
+                    H[5] = uint32_t(uorder[k].size());
+                    H[6] = uint32_t(ul.size());
+                    H[7] = nuv;
+                    uint32_t sl = 0;
+                    for (auto* v : {&nodes, &flat, &iso})
+                        for (uint32_t i : *v) {
+                            slot[k][perm[k][i]] = sl++;
+                            order[k].push_back(i);
+                        }
+                    for (uint32_t i : ul) uorder[k].push_back(i);
+                    pos += H[1];
                 }
-            H[5] = uint32_t(C.Udep.size());
-            H[6] = uint32_t(ul.size());
-            uint32_t nuv = 0;
-            for (uint32_t i : ul) {
-                nuv = std::max(nuv, dU[i] + 1);
-                C.Udep.push_back(dU[i]);
-                C.uloc.push_back(cur_slot[perm[k][i]]);
-                C.uxidx.push_back(lv.map[i]);
-                C.Udiag.push_back(lv.U_a[lv.U_ia[i]]);
-                for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) {
-                    C.Ucol.push_back(cur_slot[perm[k][lv.U_ja[p]]]);
-                    C.Uval.push_back(lv.U_a[p]);
-                }
-                C.Uptr.push_back(uint32_t(C.Ucol.size()));
-            }
-            H[7] = ul.empty() ? 0 : nuv;
-            P.max_slots = std::max(P.max_slots, S);
-            P.max_nodes = std::max<uint32_t>(P.max_nodes, uint32_t(nodes.size()));
-            pos += S;
-        }
-        C.nslots = pos;
-        // passthrough rows on this level (a prefix of pt: deepest level descending)
-        for (uint32_t x : pt) {
-            if (deep[x] < k) break;
-            const uint32_t i = at[x];
-            if (i < M || (i < K && lv.L_ia[i + 1] > lv.L_ia[i])) throw Error("chain: a passthrough row is a node", 1);
-            if ((i >= K) != (deep[x] == k)) throw Error("chain: a passthrough row ends without being isolated", 1);
-            C.qrow.push_back(lv.map[i]);
-            if (i >= K) C.dval[pos] = lv.dval[i - K];
-            pos++;
-            C.npt++;
-        }
-        if (pos != N || C.qrow.size() != N) throw Error("chain: level rows are not covered exactly once", 1);
-        {
-            std::vector<char> seen(N, 0);
-            for (uint32_t j : C.qrow) {
-                if (j >= N || seen[j]) throw Error("chain: level rows are not covered exactly once", 1);
-                seen[j] = 1;
+                C.nslots = pos;
+            } catch (...) {
+                err[k] = std::current_exception();
             }
         }
-        // this level's slots and natural rows, by identity, for the level below
-        for (uint32_t x = 0; x < NA; x++) { prev_slot[x] = kNoNode; prev_nat[x] = kNoNode; }
-        for (uint32_t i = 0; i < N; i++) {
-            const uint32_t x = perm[k][i];
-            prev_slot[x] = cur_slot[x];
-            prev_nat[x] = lv.map[i];
-            cur_slot[x] = kNoNode;
+    });
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+    parallel_for(nl, 1, [&](uint64_t k0, uint64_t k1) {
+        for (uint32_t k = uint32_t(k0); k < k1; k++) {
+            try {
+                const LevelArrays& lv = L[k];
+                ChainLevelProgram& C = P.lv[k];
+                const uint32_t N = lv.N, M = lv.M, K = N - lv.Md;
+                const std::vector<uint32_t>& at = inv[k];
+                const std::vector<uint32_t>& sk = slot[k];
+                const std::vector<uint32_t>& dL = dLk[k];
+                const std::vector<uint32_t>& dU = dUk[k];
+                C.Lptr.push_back(0);
+                C.Uptr.push_back(0);
+                C.dval.assign(N, 0.0);
+                C.qrow.reserve(N);
+                for (uint32_t b = 0; b < nblk; b++) {
+                    const uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
+                    for (uint32_t s = 0; s < H[1]; s++) {
+                        const uint32_t q = H[0] + s, i = order[k][q], x = perm[k][i];
+                        C.qrow.push_back(lv.map[i]);
+                        if (k > 0) {
+                            const uint32_t ps = slot[k - 1][x];
+                            if (ps == kNoNode) throw Error("chain: a row has no slot on the level above", 1);
+                            C.src.push_back(ps);
+                            C.xl.push_back(L[k - 1].map[inv[k - 1][x]]);
+                        }
+                        const bool node = s < H[2];
+                        C.Ldep.push_back(node ? dL[i] : 0u);
+                        if (node)
+                            for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
+                                C.Lloc.push_back(sk[perm[k][lv.L_ja[p]]]);
+                                C.Lval.push_back(lv.L_a[p]);
+                            }
+                        C.Lptr.push_back(uint32_t(C.Lloc.size()));
+                        if (i >= K) C.dval[q] = lv.dval[i - K];
+                    }
+                    for (uint32_t u = H[5]; u < H[5] + H[6]; u++) {
+                        const uint32_t i = uorder[k][u];
+                        C.Udep.push_back(dU[i]);
+                        C.uloc.push_back(sk[perm[k][i]]);
+                        C.uxidx.push_back(lv.map[i]);
+                        C.Udiag.push_back(lv.U_a[lv.U_ia[i]]);
+                        for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) {
+                            C.Ucol.push_back(sk[perm[k][lv.U_ja[p]]]);
+                            C.Uval.push_back(lv.U_a[p]);
+                        }
+                        C.Uptr.push_back(uint32_t(C.Ucol.size()));
+                    }
+                }
+                // passthrough rows on this level (a prefix of pt: deepest level descending)
+                uint32_t pos = C.nslots;
+                for (uint32_t x : pt) {
+                    if (deep[x] < k) break;
+                    const uint32_t i = at[x];
+                    if (i < M || (i < K && lv.L_ia[i + 1] > lv.L_ia[i]))
+                        throw Error("chain: a passthrough row is a node", 1);
+                    if ((i >= K) != (deep[x] == k)) throw Error("chain: a passthrough row ends without being isolated", 1);
+                    C.qrow.push_back(lv.map[i]);
+                    if (i >= K) C.dval[pos] = lv.dval[i - K];
+                    pos++;
+                    C.npt++;
+                }
+                if (pos != N || C.qrow.size() != N) throw Error("chain: level rows are not covered exactly once", 1);
+                std::vector<char> seen(N, 0);
+                for (uint32_t j : C.qrow) {
+                    if (j >= N || seen[j]) throw Error("chain: level rows are not covered exactly once", 1);
+                    seen[j] = 1;
+                }
+                // no empty device arrays
+                for (auto* v : {&C.src, &C.xl, &C.Lloc, &C.Udep, &C.uloc, &C.uxidx, &C.Ucol, &C.hdr})
+                    if (v->empty()) v->push_back(0);
+                for (auto* v : {&C.Lval, &C.Uval, &C.Udiag})
+                    if (v->empty()) v->push_back(0.0);
+            } catch (...) {
+                err[k] = std::current_exception();
+            }
         }
-        for (uint32_t i = 0; i < N; i++) at[perm[k][i]] = kNoNode;
-        // no empty device arrays
-        for (auto* v : {&C.src, &C.xl, &C.Lloc, &C.Udep, &C.uloc, &C.uxidx, &C.Ucol, &C.hdr})
-            if (v->empty()) v->push_back(0);
-        for (auto* v : {&C.Lval, &C.Uval, &C.Udiag})
-            if (v->empty()) v->push_back(0.0);
-    }
+    });
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+    for (uint32_t k = 0; k < nl; k++)
+        for (uint32_t b = 0; b < nblk; b++) {
+            const uint32_t* H = &P.lv[k].hdr[size_t(kChainHdr) * b];
+            P.max_slots = std::max(P.max_slots, H[1]);
+            P.max_nodes = std::max(P.max_nodes, H[2]);
+        }
     // index checks: every local index inside its block, every position inside its level
     for (uint32_t k = 0; k < nl; k++) {
         const ChainLevelProgram& C = P.lv[k];
