@@ -381,6 +381,14 @@ template <class VT> struct DevHier : DevPrecBase {
     bool bf16 = false;                       // level 0 keeps its Chebyshev vectors in BF16 (FP32 uh_cheb)
     // chain of the levels chain_a..nlevels-1 (plan.h: ChainProgram); -1: none
     int chain_a = -1;
+    bool no_chain = false;                   // build without the chain (the batched copy, see widen)
+    // a chain hierarchy's level arrays, kept on the host for a batched copy built without the chain
+    struct LevelCopy {
+        std::vector<uint32_t> map, L_ia, L_ja, U_ia, U_ja, A_ia, A_ja;
+        std::vector<double> L_a, U_a, dval, A_a;
+    };
+    std::shared_ptr<std::vector<LevelCopy>> host_levels;
+    std::shared_ptr<std::vector<mlp_level_desc>> host_desc;
     ChainView<VT, double> cv;                // the chain program and the chain levels' vectors (device pointers)
     std::vector<uint32_t> chain_blk;         // per chain level: blocks with rows on it
     std::vector<uint32_t> chain_npt;         // per chain level and the coarsest: passthrough rows
@@ -607,7 +615,7 @@ template <class VT> struct DevHier : DevPrecBase {
         // (MLP_NO_CHAIN=1: level kernels and the fused last pair instead).  The BF16 variant keeps level 0 apart.
         ChainProgram CP;
         const char* nochain = getenv("MLP_NO_CHAIN");
-        if (kind == MLP_PREC_UH_CHEB && nlevels >= 2 && !(nochain && nochain[0] == '1')) {
+        if (kind == MLP_PREC_UH_CHEB && nlevels >= 2 && !no_chain && !(nochain && nochain[0] == '1')) {
             const uint32_t c = nlevels - 1;
             uint32_t a0 = c > uint32_t(kChainMax) ? c - uint32_t(kChainMax) : 0u;
             if (bf16) a0 = std::max(a0, 1u);
@@ -789,7 +797,41 @@ template <class VT> struct DevHier : DevPrecBase {
         if (kind == MLP_PREC_MULTI_SPLIT && nlevels && lv[0].nit > 1)
             throw Error("multi_split level 0 runs one iteration", 1);
         chain_vectors();
+        if (chain_a >= 0) keep_levels(d);
         MLP_HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    // a host copy of the level arrays (one thread per level), for widen
+    void keep_levels(const std::vector<mlp_level_desc>& d) {
+        auto copy = std::make_shared<std::vector<LevelCopy>>(d.size());
+        auto desc = std::make_shared<std::vector<mlp_level_desc>>(d);
+        parallel_for(d.size(), 1, [&](uint64_t l0, uint64_t l1) {
+            for (uint64_t l = l0; l < l1; l++) {
+                const mlp_level_desc& D = d[l];
+                LevelCopy& C = (*copy)[l];
+                mlp_level_desc& E = (*desc)[l];
+                const uint32_t K = D.N - D.Md;
+                auto take = [](auto& v, const auto* p, size_t n) {
+                    if (p) v.assign(p, p + n);
+                    return p ? v.data() : nullptr;
+                };
+                E.map = take(C.map, D.map, D.N);
+                E.L_ia = take(C.L_ia, D.L_ia, K + 1);
+                const size_t nl = D.L_ia ? D.L_ia[K] : 0, nu = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
+                E.L_ja = take(C.L_ja, D.L_ja, nl);
+                E.L_a = take(C.L_a, D.L_a, nl);
+                E.U_ia = take(C.U_ia, D.U_ia, D.U_ia ? size_t(D.M) + 1 : 0);
+                E.U_ja = take(C.U_ja, D.U_ja, nu);
+                E.U_a = take(C.U_a, D.U_a, nu);
+                E.dval = take(C.dval, D.dval, D.Md);
+                const size_t na = D.A_ia ? D.A_ia[D.N] : 0;
+                E.A_ia = take(C.A_ia, D.A_ia, D.A_ia ? size_t(D.N) + 1 : 0);
+                E.A_ja = take(C.A_ja, D.A_ja, na);
+                E.A_a = take(C.A_a, D.A_a, na);
+            }
+        });
+        host_levels = copy;
+        host_desc = desc;
     }
 
     // A preconditioner on context c that shares this one's device data and has
@@ -807,9 +849,12 @@ template <class VT> struct DevHier : DevPrecBase {
         h->pair_l = pair_l;
         h->bf16 = bf16;
         h->chain_a = chain_a;
+        h->no_chain = no_chain;
         h->cv = cv;
         h->chain_blk = chain_blk;
         h->chain_npt = chain_npt;
+        h->host_levels = host_levels;
+        h->host_desc = host_desc;
         h->env_lbs = env_lbs;
         h->env_cap = env_cap;
         h->lds_budget = lds_budget;
@@ -838,8 +883,21 @@ template <class VT> struct DevHier : DevPrecBase {
 
     // a copy on context c that applies to `rhs` right-hand sides at once
     // (uh_cheb; multi_split's dynamic inner loops are per right-hand side)
+    // With many right-hand sides per launch the apply is bound by throughput, where the level kernels (short,
+    // light waves) beat the chain's long ones: a chain hierarchy builds its batched copy without the chain.
     std::unique_ptr<DevPrecBase> widen(mlp_ctx c, uint32_t rhs) const override {
         if (kind != MLP_PREC_UH_CHEB) return nullptr;
+        if (chain_a >= 0 && rhs > 1 && host_desc) {
+            auto h = std::make_unique<DevHier<VT>>();
+            h->ctx = c;
+            h->bf16 = bf16;
+            h->no_chain = true;
+            h->env_lbs = env_lbs;
+            h->env_cap = env_cap;
+            h->lds_budget = lds_budget;
+            h->build(*host_desc, kind, A0);
+            return h->clone(c, rhs);
+        }
         return clone(c, rhs);
     }
 
