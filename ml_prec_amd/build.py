@@ -31,6 +31,7 @@ SOURCES = [
     (os.path.join(CSRC, "assembly.cc"), False),
     (os.path.join(CPP, "ml_prec.cc"), False),
     (os.path.join(CSRC, "capi.hip"), True),
+    (os.path.join(CSRC, "devbuild.hip"), True),
 ]
 
 

@@ -3,6 +3,7 @@
 #include "../../include/mlp.h"
 
 #include "assembly.h"
+#include "devbuild.h"
 #include "hierarchy.h"
 #include "kernels.h"
 #include "mlp_error.h"
@@ -1865,48 +1866,6 @@ int mlp_dscal(mlp_ctx ctx, uint32_t n, double alpha, double* x) {
 }
 
 // ---- host hierarchies ----
-namespace mlp {
-namespace {
-// The link phase of a uh_cheb level on the device (k_cheb_mark, k_cheb_counters).
-struct DevChebLinks : ChebLinkPhase {
-    mlp_ctx ctx;
-    explicit DevChebLinks(mlp_ctx c) : ctx(c) {}
-    void run(const HostCSR& A, double beta, HostVec<uint32_t>& cnt, std::vector<int>& deg,
-             std::vector<double>& aux) override {
-        const uint32_t n = A.nrow;
-        const size_t nnz = A.nnz();
-        hipStream_t s = ctx->stream;
-        DevMem mem;
-        const uint32_t* ia = mem.upload(A.ia, s);
-        const uint32_t* ja = mem.upload(A.ja, s);
-        const double* a = mem.upload(A.a, s);
-        double* daux = mem.alloc<double>(n);
-        uint8_t* marked = mem.alloc<uint8_t>(nnz);
-        uint32_t* dcnt = mem.alloc<uint32_t>(nnz);
-        int* ddeg = mem.alloc<int>(n);
-        uint32_t* asym = mem.alloc<uint32_t>(1);
-        MLP_HIP_CHECK(hipMemsetAsync(asym, 0, sizeof(uint32_t), s));
-        if (n) {
-            hipLaunchKernelGGL(k_cheb_mark, dim3(cdiv(n, kBS)), dim3(kBS), 0, s, n, ia, a, beta, daux, marked);
-            hipLaunchKernelGGL(k_cheb_counters, dim3(cdiv(n, kBS)), dim3(kBS), 0, s, n, ia, ja,
-                               (const uint8_t*)marked, dcnt, ddeg, asym);
-            MLP_HIP_CHECK(hipGetLastError());
-        }
-        cnt.resize(nnz);
-        deg.resize(n);
-        aux.resize(n);
-        uint32_t hasym = 0;
-        MLP_HIP_CHECK(hipMemcpyAsync(cnt.data(), dcnt, nnz * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        MLP_HIP_CHECK(hipMemcpyAsync(deg.data(), ddeg, size_t(n) * sizeof(int), hipMemcpyDeviceToHost, s));
-        MLP_HIP_CHECK(hipMemcpyAsync(aux.data(), daux, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, s));
-        MLP_HIP_CHECK(hipMemcpyAsync(&hasym, asym, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        MLP_HIP_CHECK(hipStreamSynchronize(s));
-        if (hasym) throw Error("uh_cheb needs a structurally symmetric matrix", MLP_ERR_INVALID);
-    }
-};
-}  // namespace
-}  // namespace mlp
-
 static int build_uh_cheb_entry(mlp_ctx ctx, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
                                const double* sigmas, uint32_t nsigmas, const uint32_t* niters, uint32_t nniters,
                                int use_tails, mlp_hier* out) {
@@ -1925,8 +1884,8 @@ static int build_uh_cheb_entry(mlp_ctx ctx, uint32_t n, const uint32_t* ia, cons
     p.use_tails = use_tails != 0;
     if (ctx) {
         use_device(ctx);
-        DevChebLinks dev(ctx);
-        build_uh_cheb(h->A0, p, h->H, &dev);
+        auto dev = device_cheb_builder(ctx->stream);
+        build_uh_cheb(h->A0, p, h->H, dev.get());
     } else {
         build_uh_cheb(h->A0, p, h->H);
     }

@@ -1,0 +1,205 @@
+// uh_cheb hierarchy construction on the device: the steps of cheb_steps.h
+// launched as kernels on one stream, every level's matrix kept in device
+// memory for the next level, the level arrays copied to the host Hierarchy
+// (which the planner and the tests read).
+#define MLP_HD __device__
+#include "cheb_steps.h"
+#include "devbuild.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace mlp {
+namespace {
+
+constexpr uint32_t kStepBS = 256;
+constexpr uint32_t kScanChunk = 2048;              // elements per block of the scan (8 per thread)
+constexpr uint32_t kSegBlocks = 1024;              // grid of the round kernels (a stride loop over the round)
+
+struct DevOps {
+    __device__ static uint32_t add(uint32_t* p, uint32_t v) { return atomicAdd(p, v); }
+    __device__ static int sub(int* p, int v) { return atomicSub(p, v); }
+    __device__ static int max(int* p, int v) { return atomicMax(p, v); }
+    __device__ static uint32_t maxu(uint32_t* p, uint32_t v) { return atomicMax(p, v); }
+    __device__ static void fence() { __threadfence(); }
+};
+
+template <class F>
+__global__ void __launch_bounds__(kStepBS) k_step(uint32_t n, F f) {
+    const uint32_t i = blockIdx.x * kStepBS + threadIdx.x;
+    if (i < n) f(i);
+}
+
+// f(q) for bnd[r] <= q < bnd[r + 1], the bounds written by the launch before
+template <class F>
+__global__ void __launch_bounds__(kStepBS) k_step_seg(const uint32_t* bnd, uint32_t r, F f) {
+    const uint32_t b = bnd[r], e = bnd[r + 1];
+    for (uint32_t q = b + blockIdx.x * kStepBS + threadIdx.x; q < e; q += gridDim.x * kStepBS) f(q);
+}
+
+// exclusive scan, pass 1: each block scans kScanChunk elements (8 consecutive per
+// thread), writes the block-local prefix and its total
+__global__ void __launch_bounds__(kStepBS) k_scan_part(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                       uint32_t n, uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t s[kStepBS];
+    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * 8;
+    uint32_t v[8], t = 0;
+    for (int k = 0; k < 8; k++) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        t += v[k];
+    }
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t off = 1; off < kStepBS; off <<= 1) {
+        const uint32_t add = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - t;
+    for (int k = 0; k < 8; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == kStepBS - 1) bsum[blockIdx.x] = s[kStepBS - 1];
+}
+
+// pass 2 (one block): the block totals' exclusive scan in place, the grand total at bsum[nb]
+__global__ void __launch_bounds__(kStepBS) k_scan_top(uint32_t* bsum, uint32_t nb) {
+    __shared__ uint32_t s[kStepBS];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += kStepBS) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t t = i < nb ? bsum[i] : 0u;
+        s[threadIdx.x] = t;
+        __syncthreads();
+        for (uint32_t off = 1; off < kStepBS; off <<= 1) {
+            const uint32_t add = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < nb) bsum[i] = carry + s[threadIdx.x] - t;
+        carry += s[kStepBS - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+// pass 3: add each block's offset; out[n] = the total
+__global__ void __launch_bounds__(kStepBS) k_scan_fix(uint32_t* __restrict__ out, uint32_t n,
+                                                      const uint32_t* __restrict__ bsum, uint32_t nb) {
+    const uint32_t off = bsum[blockIdx.x];
+    const uint32_t base = blockIdx.x * kScanChunk;
+    for (uint32_t k = threadIdx.x; k < kScanChunk; k += kStepBS)
+        if (base + k < n) out[base + k] += off;
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
+}
+
+struct SetZero {
+    uint32_t* p;
+    __device__ void operator()(uint32_t i) const { p[i] = 0u; }
+};
+
+#define DEVBUILD_CHECK(expr)                                                                     \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            throw ::mlp::Error(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #expr ")", 2); \
+    } while (0)
+
+// Runs the steps as kernels on one stream.  Device memory comes from large
+// blocks carved in order (released together when the build ends).
+struct DevExec {
+    hipStream_t s;
+    std::vector<void*> blocks;
+    char* cur = nullptr;
+    size_t left = 0;
+    uint32_t* bsum = nullptr;                  // scan block totals
+    uint32_t bsum_cap = 0;
+    explicit DevExec(hipStream_t st) : s(st) {}
+    ~DevExec() {
+        (void)hipStreamSynchronize(s);
+        for (void* p : blocks) (void)hipFree(p);
+    }
+    template <class T> T* alloc(size_t n) {
+        const size_t b = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~size_t(255);
+        if (b > left) {
+            const size_t sz = std::max<size_t>(b, size_t(64) << 20);
+            void* p = nullptr;
+            DEVBUILD_CHECK(hipMalloc(&p, sz));
+            blocks.push_back(p);
+            cur = static_cast<char*>(p);
+            left = sz;
+        }
+        T* r = reinterpret_cast<T*>(cur);
+        cur += b;
+        left -= b;
+        return r;
+    }
+    template <class T> void zero(T* p, size_t n) { DEVBUILD_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), s)); }
+    template <class F> void each(uint32_t n, const F& f) {
+        if (!n) return;
+        hipLaunchKernelGGL(k_step<F>, dim3((n + kStepBS - 1) / kStepBS), dim3(kStepBS), 0, s, n, f);
+        DEVBUILD_CHECK(hipGetLastError());
+    }
+    template <class F> void each_seg(const uint32_t* bnd, uint32_t r, const F& f) {
+        hipLaunchKernelGGL(k_step_seg<F>, dim3(kSegBlocks), dim3(kStepBS), 0, s, bnd, r, f);
+        DEVBUILD_CHECK(hipGetLastError());
+    }
+    void scan(const uint32_t* in, uint32_t* out, uint32_t n) {
+        if (n == 0) {
+            each(1, SetZero{out});
+            return;
+        }
+        const uint32_t nb = (n + kScanChunk - 1) / kScanChunk;
+        if (nb + 1 > bsum_cap) {
+            bsum_cap = std::max<uint32_t>(nb + 1, 1024);
+            bsum = alloc<uint32_t>(bsum_cap);
+        }
+        hipLaunchKernelGGL(k_scan_part, dim3(nb), dim3(kStepBS), 0, s, in, out, n, bsum);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kStepBS), 0, s, bsum, nb);
+        hipLaunchKernelGGL(k_scan_fix, dim3(nb), dim3(kStepBS), 0, s, out, n, (const uint32_t*)bsum, nb);
+        DEVBUILD_CHECK(hipGetLastError());
+    }
+    template <class T> void get(T* dst, const T* src, size_t n) {
+        DEVBUILD_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+        DEVBUILD_CHECK(hipStreamSynchronize(s));
+    }
+    template <class T> T get1(const T* src) {
+        T v;
+        get(&v, src, 1);
+        return v;
+    }
+    template <class T> T* put(const T* src, size_t n) {
+        T* d = alloc<T>(n);
+        if (n) DEVBUILD_CHECK(hipMemcpyAsync(d, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+        return d;
+    }
+};
+
+struct DeviceChebBuilder : ChebLevelBuilder {
+    hipStream_t s;
+    explicit DeviceChebBuilder(hipStream_t st) : s(st) {}
+    void run(const HostCSR& A, bool use_tails, uint32_t max_levels, Hierarchy& H) override {
+        DevExec x(s);
+        steps::LevelMatrix M0;
+        M0.n = A.nrow;
+        M0.nnz = A.nnz();
+        M0.ia = x.put(A.ia.data(), A.ia.size());
+        M0.ja = x.put(A.ja.data(), A.ja.size());
+        M0.a = x.put(A.a.data(), A.a.size());
+        steps::build_levels_steps<DevExec, DevOps>(x, M0, use_tails, max_levels, H);
+    }
+};
+
+}  // namespace
+
+std::unique_ptr<ChebLevelBuilder> device_cheb_builder(void* stream) {
+    return std::make_unique<DeviceChebBuilder>(static_cast<hipStream_t>(stream));
+}
+
+}  // namespace mlp

@@ -89,7 +89,7 @@ struct TwoSidedLinks {
         });
         if (lower != upper) throw Error("uh_cheb needs a structurally symmetric matrix", 1);
     }
-    // counters and twin positions computed elsewhere (ChebLinkPhase)
+    // counters and twin positions given (host checks of the tail walk)
     TwoSidedLinks(const HostCSR& M, HostVec<uint32_t>&& c) : A(&M), cnt(std::move(c)) {}
     uint32_t slot(uint32_t row, uint32_t p) const { return A->ja[p] < row ? cnt[p] : p; }
     bool present(uint32_t row, uint32_t p) const { return cnt[slot(row, p)] != 0; }
@@ -317,7 +317,6 @@ struct ChebBuilder {
     Hierarchy& H;
     bool use_tails;
     uint32_t max_levels;
-    ChebLinkPhase* dev = nullptr;
 
     // Tails first (chains of nodes left with one link), isolated nodes last
     // (cheb_mark.cc:8-92).
@@ -379,13 +378,6 @@ struct ChebBuilder {
         const uint32_t N = li.N;
         std::vector<int> deg(N);
         li.aux.resize(N);
-        if (dev) {
-            HostVec<uint32_t> cnt;
-            dev->run(A, li.beta, cnt, deg, li.aux);
-            TwoSidedLinks links(A, std::move(cnt));
-            finish(l, A, links, deg);
-            return;
-        }
         TwoSidedLinks links(A);
         // drop weak links against the row budget (cheb_constr.cc:62-103, to_remove cheb_prec.h:71-99).
         // A row's choice depends only on its own values, so the rows choose in
@@ -643,7 +635,7 @@ double cheb_poly(double x, uint32_t k) {
     }
 }
 
-void build_uh_cheb(const HostCSR& A, const ChebParams& p, Hierarchy& H, ChebLinkPhase* dev) {
+void build_uh_cheb(const HostCSR& A, const ChebParams& p, Hierarchy& H, ChebLevelBuilder* dev) {
     if (p.sigmas.empty() || p.niters.empty()) throw Error("uh_cheb needs at least one sigma and one niter");
     for (double s : p.sigmas)
         if (!(s > 1)) throw Error("All sigmas must be > 1");
@@ -657,8 +649,12 @@ void build_uh_cheb(const HostCSR& A, const ChebParams& p, Hierarchy& H, ChebLink
         H.levels[l].alpha = 1.0;
         H.levels[l].beta = l < p.sigmas.size() ? p.sigmas[l] : p.sigmas.back();
     }
-    ChebBuilder b{H, p.use_tails, kMaxLevels, dev};
-    b.level(0, A);
+    if (dev) {
+        dev->run(A, p.use_tails, kMaxLevels, H);
+    } else {
+        ChebBuilder b{H, p.use_tails, kMaxLevels};
+        b.level(0, A);
+    }
     H.levels.resize(H.nlevels + 1);    // keep one slot: the last level's (empty) successor
     H.levels.back() = HostLevel();
     for (uint32_t l = 0; l < H.nlevels; l++)

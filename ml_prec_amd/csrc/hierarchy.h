@@ -1,7 +1,8 @@
-// Host-side hierarchy construction for the uh_cheb and multi_split
-// preconditioners.  Construction stays on the host in FP64 and reproduces the
-// reference's integer decisions (dropped links, tails, permutations, sparsity)
-// and floating-point values exactly; the device only applies the result.
+// Hierarchy construction for the uh_cheb and multi_split preconditioners, in
+// FP64, reproducing the reference's integer decisions (dropped links, tails,
+// permutations, sparsity) and floating-point values exactly.  The host builder
+// is here; a uh_cheb hierarchy can also be built level by level on a device
+// (ChebLevelBuilder) into the same arrays.
 #pragma once
 
 #include <cstdint>
@@ -66,22 +67,19 @@ struct Hierarchy {
     const HostCSR* A0 = nullptr;     // level-0 matrix, kept by reference like level0_A
 };
 
-// The link phase of one uh_cheb level (cheb_constr.cc:62-103 with the two-sided
-// counters of cheb_misc.h:45-96), computed somewhere other than the host
-// threads (a device).  For the level matrix A and the level's beta it fills
-// aux[i] = the row sum of row i (storage order), deg[i] = the links row i keeps,
-// and cnt[p] per stored entry: 2 on the diagonal; 2 minus the number of end
-// points that asked to drop the link on an upper entry; the position of the
-// twin entry (j, i) on a lower entry.  Everything must equal the host's values;
-// a structurally asymmetric A throws the host's error.
-struct ChebLinkPhase {
-    virtual ~ChebLinkPhase() = default;
-    virtual void run(const HostCSR& A, double beta, HostVec<uint32_t>& cnt, std::vector<int>& deg,
-                     std::vector<double>& aux) = 0;
+// Every level of a uh_cheb hierarchy built somewhere other than the host
+// threads (a device): given the level-0 matrix and H with each level's alpha
+// and beta set, fill H.levels[l] (N, nnz, M, Md, map, rmap, aux, L, U, dval)
+// and H.levels[l + 1].A for every level, and H.nlevels, throwing the host
+// builder's errors ("Too many levels: l" past max_levels).  Every array must
+// equal the host builder's bit for bit.
+struct ChebLevelBuilder {
+    virtual ~ChebLevelBuilder() = default;
+    virtual void run(const HostCSR& A, bool use_tails, uint32_t max_levels, Hierarchy& H) = 0;
 };
 
 // Throws mlp::Error (derived from std::runtime_error) with the reference's messages.
-void build_uh_cheb(const HostCSR& A, const ChebParams& p, Hierarchy& H, ChebLinkPhase* dev = nullptr);
+void build_uh_cheb(const HostCSR& A, const ChebParams& p, Hierarchy& H, ChebLevelBuilder* dev = nullptr);
 void build_multi_split(const HostCSR& A, const MultiSplitParams& p, Hierarchy& H);
 
 // cheb(x,k): modules/prec/misc/misc.cc:232-242 (host libm for k >= 4)

@@ -2016,79 +2016,4 @@ __global__ void __launch_bounds__(kBS) k_maxpy(uint32_t n, const double* __restr
     }
 }
 
-// ---------------------------------------------------------------------------
-// uh_cheb construction, link phase (cheb_constr.cc:62-103, cheb_misc.h:45-96)
-// ---------------------------------------------------------------------------
-
-// Row i: the row sum c (storage order) into aux[i]; then the off-diagonal
-// links in ascending (|a|, position) order -- the order of the stable
-// insertion sort psort (cheb_constr.cc:26-36), found here one link at a time
-// by a scan for the smallest key after the previous one -- each asking to be
-// dropped (marked[p] = 1) while the shares -2 a / (c (beta - 1)) add up to at
-// most 1 (to_remove, cheb_prec.h:71-99).
-__global__ void __launch_bounds__(kBS) k_cheb_mark(uint32_t n, const uint32_t* __restrict__ ia,
-                                                   const double* __restrict__ a, double beta,
-                                                   double* __restrict__ aux, uint8_t* __restrict__ marked) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t b = ia[i], e = ia[i + 1];
-    double c = 0.0;
-    for (uint32_t p = b; p < e; p++) c += a[p];
-    aux[i] = c;
-    for (uint32_t p = b; p < e; p++) marked[p] = 0;
-    double s = 0.0, pv = -1.0;            // every |a| is after the start key
-    uint32_t pp = 0;
-    for (uint32_t t = b + 1; t < e; t++) {
-        uint32_t best = 0xffffffffu;
-        double bv = 0.0;
-        for (uint32_t p = b + 1; p < e; p++) {
-            const double v = fabs(a[p]);
-            const bool after = v > pv || (v == pv && p > pp);
-            if (after && (best == 0xffffffffu || v < bv)) {   // p ascends: the first of equal keys wins
-                best = p;
-                bv = v;
-            }
-        }
-        const double share = -2 * a[best] / (c * (beta - 1));
-        if (s + share > 1) break;
-        s += share;
-        marked[best] = 1;
-        pv = bv;
-        pp = best;
-    }
-}
-
-// Row i: for every off-diagonal entry p = (i, j), the twin (j, i) by binary
-// search in row j; m = the marks of both ends; a link marked from both ends is
-// removed and lowers deg[i].  cnt holds the diagonal's 2, the upper entries'
-// counters 2 - m and the lower entries' twin positions.  A missing twin (a
-// structurally asymmetric matrix) sets *asym.
-__global__ void __launch_bounds__(kBS) k_cheb_counters(uint32_t n, const uint32_t* __restrict__ ia,
-                                                       const uint32_t* __restrict__ ja,
-                                                       const uint8_t* __restrict__ marked,
-                                                       uint32_t* __restrict__ cnt, int* __restrict__ deg,
-                                                       uint32_t* asym) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t b = ia[i], e = ia[i + 1];
-    cnt[b] = 2u;
-    int d = int(e - b - 1);
-    for (uint32_t p = b + 1; p < e; p++) {
-        const uint32_t j = ja[p], je = ia[j + 1];
-        uint32_t lo = ia[j] + 1, hi = je;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (ja[mid] < i) lo = mid + 1;
-            else hi = mid;
-        }
-        uint32_t t = p;
-        if (lo < je && ja[lo] == i) t = lo;
-        else *asym = 1u;
-        const uint32_t m = uint32_t(marked[p]) + uint32_t(marked[t]);
-        if (m == 2) d--;
-        cnt[p] = j > i ? 2u - m : t;
-    }
-    deg[i] = d;
-}
-
 }  // namespace mlp

@@ -1,0 +1,291 @@
+// Host check of the step-wise uh_cheb construction (ml_prec_amd/csrc/cheb_steps.h,
+// which the device runs as kernels): the same steps run here as loops, each
+// over its indices in a shuffled order, and every level array must equal the
+// host builder's bit for bit.  Matrices: the CSR BINARY files named on the
+// command line, and seeded random seven-point and path matrices whose link
+// dropping leaves forests of every shape (free trees, paths, trees hanging off
+// the core).  Built and run by tests/test_host_checks.py.
+#define MLP_HD
+#include "../../ml_prec_amd/csrc/hierarchy.cc"
+#include "../../ml_prec_amd/csrc/cheb_steps.h"
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+
+namespace {
+
+using namespace mlp;
+
+struct HostOps {
+    static uint32_t add(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p += v; return o; }
+    static int sub(int* p, int v) { const int o = *p; *p -= v; return o; }
+    static int max(int* p, int v) { const int o = *p; if (v > o) *p = v; return o; }
+    static uint32_t maxu(uint32_t* p, uint32_t v) { const uint32_t o = *p; if (v > o) *p = v; return o; }
+    static void fence() {}
+};
+
+// Runs every step as a loop over its indices in a random order; buffers start
+// filled with garbage, as device allocations do.
+struct HostExec {
+    std::mt19937_64 rng;
+    std::vector<std::unique_ptr<unsigned char[]>> bufs;
+    std::vector<uint32_t> order;
+    explicit HostExec(uint64_t seed) : rng(seed) {}
+    template <class T> T* alloc(size_t n) {
+        const size_t b = std::max<size_t>(n, 1) * sizeof(T);
+        bufs.emplace_back(new unsigned char[b]);
+        std::memset(bufs.back().get(), 0xa5, b);
+        return reinterpret_cast<T*>(bufs.back().get());
+    }
+    template <class T> void zero(T* p, size_t n) { std::memset(p, 0, n * sizeof(T)); }
+    void shuffled(uint32_t b, uint32_t e) {
+        order.resize(e - b);
+        for (uint32_t i = b; i < e; i++) order[i - b] = i;
+        std::shuffle(order.begin(), order.end(), rng);
+    }
+    template <class F> void each(uint32_t n, const F& f) {
+        shuffled(0, n);
+        const std::vector<uint32_t> o = order;
+        for (uint32_t i : o) f(i);
+    }
+    template <class F> void each_seg(const uint32_t* bnd, uint32_t r, const F& f) {
+        shuffled(bnd[r], bnd[r + 1]);
+        const std::vector<uint32_t> o = order;
+        for (uint32_t i : o) f(i);
+    }
+    void scan(const uint32_t* in, uint32_t* out, uint32_t n) {
+        uint32_t s = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t v = in[i];
+            out[i] = s;
+            s += v;
+        }
+        out[n] = s;
+    }
+    template <class T> void get(T* dst, const T* src, size_t n) { std::memcpy(dst, src, n * sizeof(T)); }
+    template <class T> T get1(const T* src) { return *src; }
+};
+
+struct StepsOnHost : ChebLevelBuilder {
+    uint64_t seed;
+    explicit StepsOnHost(uint64_t s) : seed(s) {}
+    void run(const HostCSR& A, bool use_tails, uint32_t max_levels, Hierarchy& H) override {
+        HostExec x(seed);
+        steps::LevelMatrix M0;
+        M0.n = A.nrow;
+        M0.nnz = A.nnz();
+        M0.ia = A.ia.data();
+        M0.ja = A.ja.data();
+        M0.a = A.a.data();
+        steps::build_levels_steps<HostExec, HostOps>(x, M0, use_tails, max_levels, H);
+    }
+};
+
+template <class V> bool same(const V& a, const V& b) {
+    return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(a[0])) == 0);
+}
+
+bool same_csr(const HostCSR& a, const HostCSR& b) {
+    return a.nrow == b.nrow && a.ncol == b.ncol && same(a.ia, b.ia) && same(a.ja, b.ja) && same(a.a, b.a);
+}
+
+// "" when the hierarchies agree, else what differs first
+std::string compare(const Hierarchy& h, const Hierarchy& s) {
+    if (h.nlevels != s.nlevels) return "nlevels " + std::to_string(h.nlevels) + " vs " + std::to_string(s.nlevels);
+    for (uint32_t l = 0; l < h.nlevels; l++) {
+        const HostLevel &a = h.levels[l], &b = s.levels[l];
+        const std::string L = "level " + std::to_string(l) + ": ";
+        if (a.N != b.N || a.nnz != b.nnz || a.M != b.M || a.Md != b.Md)
+            return L + "sizes N " + std::to_string(a.N) + "/" + std::to_string(b.N) + " M " + std::to_string(a.M) +
+                   "/" + std::to_string(b.M) + " Md " + std::to_string(a.Md) + "/" + std::to_string(b.Md);
+        if (!same(a.map, b.map)) {
+            for (size_t i = 0; i < a.map.size(); i++)
+                if (a.map[i] != b.map[i])
+                    return L + "map[" + std::to_string(i) + "] " + std::to_string(a.map[i]) + " vs " +
+                           std::to_string(b.map[i]);
+            return L + "map";
+        }
+        if (!same(a.rmap, b.rmap)) return L + "rmap";
+        if (!same(a.aux, b.aux)) return L + "aux";
+        if (!same_csr(a.L, b.L)) return L + "L";
+        if (!same_csr(a.U, b.U)) return L + "U";
+        if (!same(a.dval, b.dval)) return L + "dval";
+        if (!same_csr(h.levels[l + 1 < h.levels.size() ? l + 1 : l].A, s.levels[l + 1 < s.levels.size() ? l + 1 : l].A))
+            return L + "next matrix";
+        if (a.lmin != b.lmin || a.lmax != b.lmax) return L + "spectral bounds";
+    }
+    return "";
+}
+
+bool read_binary(const char* path, HostCSR& A) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    uint32_t hdr[2];
+    bool ok = std::fread(hdr, 4, 2, f) == 2;
+    if (ok) {
+        A.nrow = hdr[0];
+        A.ncol = hdr[1];
+        A.ia.resize(size_t(hdr[0]) + 1);
+        ok = std::fread(A.ia.data(), 4, A.ia.size(), f) == A.ia.size();
+    }
+    if (ok) {
+        const size_t nnz = A.ia.back();
+        A.ja.resize(nnz);
+        A.a.resize(nnz);
+        ok = std::fread(A.ja.data(), 4, nnz, f) == nnz && std::fread(A.a.data(), 8, nnz, f) == nnz;
+    }
+    std::fclose(f);
+    return ok;
+}
+
+// Skyline rows of a symmetric matrix from its (i, j, t) links, t > 0: a_ij = -t,
+// a_ii = c_i + sum of t.
+HostCSR from_links(uint32_t n, const std::vector<std::array<double, 3>>& links, const std::vector<double>& c) {
+    std::vector<std::vector<std::pair<uint32_t, double>>> rows(n);
+    std::vector<double> diag(c);
+    for (const auto& e : links) {
+        const uint32_t i = uint32_t(e[0]), j = uint32_t(e[1]);
+        rows[i].push_back({j, -e[2]});
+        rows[j].push_back({i, -e[2]});
+        diag[i] += e[2];
+        diag[j] += e[2];
+    }
+    HostCSR A;
+    A.nrow = A.ncol = n;
+    A.ia.assign(1, 0);
+    for (uint32_t i = 0; i < n; i++) {
+        std::sort(rows[i].begin(), rows[i].end());
+        A.ja.push_back(i);
+        A.a.push_back(diag[i]);
+        for (const auto& e : rows[i]) {
+            A.ja.push_back(e.first);
+            A.a.push_back(e.second);
+        }
+        A.ia.push_back(uint32_t(A.ja.size()));
+    }
+    return A;
+}
+
+// seven-point grid, log-normal transmissibilities with a wide spread, reaction
+// terms that differ by row (some rows keep no budget at all)
+HostCSR random_grid(std::mt19937_64& g, uint32_t nx, uint32_t ny, uint32_t nz, double sigma) {
+    std::normal_distribution<double> nd(0.0, sigma);
+    std::uniform_real_distribution<double> ud(0.0, 1.0);
+    const uint32_t n = nx * ny * nz;
+    std::vector<std::array<double, 3>> links;
+    auto id = [&](uint32_t x, uint32_t y, uint32_t z) { return double((z * ny + y) * nx + x); };
+    for (uint32_t z = 0; z < nz; z++)
+        for (uint32_t y = 0; y < ny; y++)
+            for (uint32_t x = 0; x < nx; x++) {
+                if (x + 1 < nx) links.push_back({id(x, y, z), id(x + 1, y, z), std::exp(nd(g))});
+                if (y + 1 < ny) links.push_back({id(x, y, z), id(x, y + 1, z), std::exp(nd(g))});
+                if (z + 1 < nz) links.push_back({id(x, y, z), id(x, y, z + 1), std::exp(nd(g))});
+            }
+    std::vector<double> c(n);
+    for (auto& v : c) v = ud(g) < 0.1 ? 0.0 : std::exp(nd(g));
+    return from_links(n, links, c);
+}
+
+// random graph: a few long paths, stars and cycles joined by random links
+HostCSR random_graph(std::mt19937_64& g, uint32_t n) {
+    std::uniform_int_distribution<uint32_t> pick(0, n - 1);
+    std::normal_distribution<double> nd(0.0, 2.0);
+    std::uniform_real_distribution<double> ud(0.0, 1.0);
+    std::set<std::pair<uint32_t, uint32_t>> seen;
+    std::vector<std::array<double, 3>> links;
+    auto add = [&](uint32_t i, uint32_t j) {
+        if (i == j) return;
+        if (i > j) std::swap(i, j);
+        if (!seen.insert({i, j}).second) return;
+        links.push_back({double(i), double(j), std::exp(nd(g))});
+    };
+    std::vector<uint32_t> perm(n);
+    for (uint32_t i = 0; i < n; i++) perm[i] = i;
+    std::shuffle(perm.begin(), perm.end(), g);
+    uint32_t k = 0;
+    while (k < n) {
+        const double u = ud(g);
+        const uint32_t len = std::min<uint32_t>(n - k, 2 + uint32_t(ud(g) * 12));
+        if (u < 0.4) {                                      // path
+            for (uint32_t t = 1; t < len; t++) add(perm[k + t - 1], perm[k + t]);
+        } else if (u < 0.7) {                               // star
+            for (uint32_t t = 1; t < len; t++) add(perm[k], perm[k + t]);
+        } else if (u < 0.85) {                              // cycle with a tail
+            for (uint32_t t = 1; t < len; t++) add(perm[k + t - 1], perm[k + t]);
+            if (len > 2) add(perm[k], perm[k + len - 1]);
+        }
+        k += len;
+    }
+    for (uint32_t t = 0; t < n / 8; t++) add(pick(g), pick(g));
+    std::vector<double> c(n);
+    for (auto& v : c) v = ud(g) < 0.3 ? 0.0 : std::exp(nd(g));
+    return from_links(n, links, c);
+}
+
+int check(const std::string& name, const HostCSR& A, const ChebParams& p, uint64_t seed) {
+    Hierarchy h, s;
+    std::string eh, es;
+    try { build_uh_cheb(A, p, h); } catch (const std::exception& e) { eh = e.what(); }
+    StepsOnHost st(seed);
+    try { build_uh_cheb(A, p, s, &st); } catch (const std::exception& e) { es = e.what(); }
+    if (!eh.empty() || !es.empty()) {
+        if (eh == es) {
+            std::printf("OK %s error \"%s\"\n", name.c_str(), eh.c_str());
+            return 0;
+        }
+        std::printf("FAIL %s: host error \"%s\", steps error \"%s\"\n", name.c_str(), eh.c_str(), es.c_str());
+        return 1;
+    }
+    const std::string d = compare(h, s);
+    if (!d.empty()) {
+        std::printf("FAIL %s: %s\n", name.c_str(), d.c_str());
+        return 1;
+    }
+    uint32_t M = 0;
+    for (uint32_t l = 0; l < h.nlevels; l++) M += h.levels[l].M;
+    std::printf("OK %s levels %u tails %u\n", name.c_str(), h.nlevels, M);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    int fails = 0;
+    uint32_t nrand = 40;
+    for (int k = 1; k < argc; k++) {
+        if (!std::strcmp(argv[k], "--random")) {
+            nrand = uint32_t(std::atoi(argv[++k]));
+            continue;
+        }
+        HostCSR A;
+        if (!read_binary(argv[k], A)) {
+            std::printf("FAIL cannot read %s\n", argv[k]);
+            return 2;
+        }
+        ChebParams p;
+        fails += check(argv[k], A, p, 1000 + k);
+        p.use_tails = false;
+        fails += check(std::string(argv[k]) + " (no tails)", A, p, 2000 + k);
+    }
+    for (uint32_t t = 0; t < nrand; t++) {
+        std::mt19937_64 g(77 + t);
+        ChebParams p;
+        p.sigmas = {t % 3 == 0 ? 1.5 : (t % 3 == 1 ? 3.0 : 8.0)};
+        const HostCSR A = t % 2 ? random_graph(g, 200 + 97 * t)
+                                : random_grid(g, 3 + t % 7, 4 + t % 5, 2 + t % 4, 1.0 + 0.1 * t);
+        fails += check("random " + std::to_string(t), A, p, 3000 + t);
+    }
+    // one long path: every level a single free tree (the reference's "fully tridiagonal" case)
+    {
+        std::vector<std::array<double, 3>> links;
+        const uint32_t n = 1000;
+        for (uint32_t i = 0; i + 1 < n; i++) links.push_back({double(i), double(i + 1), 1.0 + (i % 7) * 0.25});
+        std::vector<double> c(n, 0.5);
+        ChebParams p;
+        fails += check("path", from_links(n, links, c), p, 4000);
+    }
+    std::printf("%s %d\n", fails ? "FAILED" : "ALL OK", fails);
+    return fails ? 1 : 0;
+}

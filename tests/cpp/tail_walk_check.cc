@@ -21,7 +21,7 @@ int main() {
         Hierarchy H;
         H.levels.assign(2, HostLevel());
         H.levels[0].N = 3;
-        ChebBuilder b{H, true, 30, nullptr};
+        ChebBuilder b{H, true, 30};
         TwoSidedLinks links(A, HostVec<uint32_t>(cnt));
         std::vector<int> deg = {1, 2, 1};
         b.permutation(A, links, deg, H.levels[0]);
@@ -38,7 +38,7 @@ int main() {
         Hierarchy H;
         H.levels.assign(2, HostLevel());
         H.levels[0].N = 3;
-        ChebBuilder b{H, true, 30, nullptr};
+        ChebBuilder b{H, true, 30};
         TwoSidedLinks links(A, HostVec<uint32_t>(cnt));
         std::vector<int> deg = {1, 2, 1};
         try {

@@ -116,3 +116,32 @@ def test_chain_launches_equal_recursion(chain_exe, tmp_path, name):
     print(r.stdout)
     assert r.returncode == 0 and r.stdout.rstrip().endswith("OK"), r.stdout + r.stderr
     assert " 0 of " in r.stdout
+
+
+@pytest.fixture(scope="module")
+def steps_exe(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("steps") / "steps_check")
+    src = os.path.join(HERE, "cpp", "steps_check.cc")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-pthread", "-include", "set", "-include",
+                        "array", "-o", out, src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return out
+
+
+def test_device_construction_steps_on_host(steps_exe, tmp_path):
+    """The device construction's steps (cheb_steps.h) run as host loops, every step over its indices in a
+    shuffled order: the hierarchy equals the host builder's bit for bit on the workload systems up to
+    60x220x85 (with and without tails), on 40 seeded random grids and forest-shaped graphs, and on a
+    1000-node path; the same errors where the host builder throws."""
+    from conftest import system
+    from ml_prec_amd import spe_synth as S
+    files = []
+    for name in ["tiny", "tiny_tpfa_c1", "small_spe10", "ln5_spe", "ln5_tpfa_c1", "spe10_z12", "spe10_z85"]:
+        s = system(name)
+        f = str(tmp_path / (name + ".bin"))
+        S.write_csr_binary(f, s.ia, s.ja, s.a)
+        files.append(f)
+    r = subprocess.run([steps_exe, "--random", "40"] + files, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.rstrip().endswith("ALL OK 0")
+    assert r.stdout.count("\nOK ") + r.stdout.startswith("OK ") == 2 * len(files) + 40 + 1
