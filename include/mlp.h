@@ -108,14 +108,20 @@ int mlp_hier_build_uh_cheb(uint32_t n, const uint32_t* ia, const uint32_t* ja, c
                            const double* sigmas, uint32_t nsigmas,
                            const uint32_t* niters, uint32_t nniters,
                            int use_tails, mlp_hier* out);
-/* The same construction with each level's link phase on ctx's device: row sums, the magnitude order of
- * the links (psort, cheb_constr.cc:26-36), the budget marks (cheb_constr.cc:62-103, to_remove
- * cheb_prec.h:71-99) and the two-sided counters and degrees (cheb_misc.h:45-96).  The tail walk and the
- * sparse LU stay on the host.  The hierarchy is identical to mlp_hier_build_uh_cheb's. */
+/* The same construction on ctx's device, every level as kernels with the level matrices kept in device
+ * memory: row sums, the magnitude order of the links (psort, cheb_constr.cc:26-36), the budget marks
+ * (cheb_constr.cc:62-103, to_remove cheb_prec.h:71-99), the two-sided counters (cheb_misc.h:45-96), the
+ * tail order (construct_permutation, cheb_mark.cc:8-92), the sparse LU and the next level
+ * (construct_sparse_lu, sparse_lu.cc:6-133).  The level arrays are copied to the host hierarchy, which is
+ * identical to mlp_hier_build_uh_cheb's. */
 int mlp_hier_build_uh_cheb_dev(mlp_ctx ctx, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
                                const double* sigmas, uint32_t nsigmas,
                                const uint32_t* niters, uint32_t nniters,
                                int use_tails, mlp_hier* out);
+/* The device construction reading level 0 from the matrix handle: from device memory when A was assembled
+ * there (mlp_spe_matrix_dev), else from A's host copy.  Prec(A, cfg), cheb_constr.cc:163-248. */
+int mlp_hier_build_uh_cheb_mat(mlp_ctx ctx, mlp_mat A, const double* sigmas, uint32_t nsigmas,
+                               const uint32_t* niters, uint32_t nniters, int use_tails, mlp_hier* out);
 
 typedef struct {
     const double* q;        uint32_t nq;          /* key q            */
@@ -296,6 +302,15 @@ int mlp_simple_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* d_b, dou
 uint64_t mlp_spe_nnz(uint32_t nx, uint32_t ny, uint32_t nz);
 int mlp_spe_assemble(int kind, uint32_t nx, uint32_t ny, uint32_t nz, const double* kx, double kz_ratio,
                      double c, double hx, double hy, double hz, uint32_t* ia, uint32_t* ja, double* a);
+/* The same matrix assembled on ctx's device from device-resident kx, as a matrix handle: its Skyline CSR
+ * stays in device memory (mlp_hier_build_uh_cheb_mat reads it), the solve layout (seven-point when every
+ * dimension is at least 2) is written by the assembly, and the host copy is fetched only when something
+ * needs it (DiagPrec, GSPrec, CompositePrec, SymPrec, mlp_mat_get_csr).  Bit for bit mlp_spe_assemble's
+ * matrix. */
+int mlp_spe_matrix_dev(mlp_ctx ctx, int kind, uint32_t nx, uint32_t ny, uint32_t nz, const double* d_kx,
+                       double kz_ratio, double c, double hx, double hy, double hz, mlp_mat* out);
+/* A's Skyline CSR into host arrays (n+1, nnz, nnz): CSRMatrix::get_ia/get_ja/get_a, matrix.h:204-210 */
+int mlp_mat_get_csr(mlp_mat A, uint32_t* ia, uint32_t* ja, double* a);
 
 /* ---- diagnostics (no counterpart in the reference; used for roofline reporting) ---- */
 /* algorithmic HBM bytes of one apply (sum over level visits) and the kernels the last apply launched

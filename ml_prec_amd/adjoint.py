@@ -8,7 +8,10 @@ theta = ln kx per cell.  A misfit L(p) is differentiated through the solve:
 
 so one gradient costs the forward solve plus one adjoint solve with the same
 hierarchy, and one pass of k_spe_grad.  The forward and adjoint solves are
-PCG with the uh_cheb preconditioner built for A(theta).
+PCG with the uh_cheb preconditioner built for A(theta).  By default the
+matrix is assembled on the device from kx = exp(theta) and the hierarchy is
+built there from it (device=False assembles and builds on the host; both give
+the same matrix and hierarchy bit for bit).
 
 `SpeAdjointModel` is the plain interface; `spe_solve_function()` wraps it as a
 torch.autograd.Function (CPU float64 tensors in and out; all solver work runs
@@ -31,8 +34,9 @@ class SpeAdjointModel:
     tolerance of both solves.
     """
 
-    def __init__(self, ctx: N.Context, shape, b, c=1.0, eps=1e-10, maxit=20000):
+    def __init__(self, ctx: N.Context, shape, b, c=1.0, eps=1e-10, maxit=20000, device=True):
         self.ctx = ctx
+        self.device = bool(device)
         self.shape = tuple(int(v) for v in shape)
         self.n = int(np.prod(self.shape))
         self.b = np.ascontiguousarray(b, dtype=np.float64)
@@ -62,14 +66,22 @@ class SpeAdjointModel:
     def forward(self, theta):
         """p(theta) as a host array; keeps the matrix and hierarchy for the adjoint."""
         self.close()
-        kx, ia, ja, a = self._assemble(theta)
-        A = N.DevMatrix(self.ctx, ia, ja, a)
-        B = N.DevPrec(self.ctx, A, N.build_uh_cheb(ia, ja, a))
+        if self.device:
+            kx = np.exp(np.asarray(theta, dtype=np.float64))
+            dkx = self.ctx.to_device(kx)
+            A = N.DevMatrix.spe_on_device(self.ctx, dkx, self.shape, kind="spe", c=self.c, kz_ratio=KZ_RATIO,
+                                          h=(S.HX, S.HY, S.HZ))
+            B = N.DevPrec(self.ctx, A, N.build_uh_cheb_on_device(self.ctx, A))
+        else:
+            kx, ia, ja, a = self._assemble(theta)
+            A = N.DevMatrix(self.ctx, ia, ja, a)
+            B = N.DevPrec(self.ctx, A, N.build_uh_cheb(ia, ja, a))
+            dkx = self.ctx.to_device(kx)
         p = self.ctx.array(self.n)
         st = N.pcg(self.ctx, A, B, self.db, p, eps=self.eps, maxit=self.maxit, x0_mode=N.X0_ZERO, use_graph=True,
                    check_every=8)
         self.stats.append(("forward", st))
-        self._state = (np.array(theta, dtype=np.float64), A, B, self.ctx.to_device(kx), p)
+        self._state = (np.array(theta, dtype=np.float64), A, B, dkx, p)
         return p.get()
 
     def vjp(self, dL_dp):

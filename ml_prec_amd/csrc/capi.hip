@@ -190,12 +190,32 @@ struct mlp_mat_s {
     mlp::HostVec<uint32_t> ia, ja;     // host copy for DiagPrec, GS, Composite and SymPrec
     mlp::HostVec<double> a;
     uint64_t bytes = 0;
+    // a matrix assembled on the device keeps its Skyline CSR there (the device
+    // construction reads it) and fetches the host copy when something needs it
+    uint32_t* d_ia = nullptr;
+    uint32_t* d_ja = nullptr;
+    double* d_a = nullptr;
+    bool host_ready = true;
 };
 
 struct mlp_hier_s {
     mlp::HostCSR A0;
     mlp::Hierarchy H;
 };
+
+// the host copy of a device-assembled matrix, fetched on first use
+static void ensure_host(mlp_mat m) {
+    if (m->host_ready) return;
+    hipStream_t s = m->ctx->stream;
+    m->ia.resize(size_t(m->n) + 1);
+    m->ja.resize(m->nnz);
+    m->a.resize(m->nnz);
+    MLP_HIP_CHECK(hipMemcpyAsync(m->ia.data(), m->d_ia, (size_t(m->n) + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MLP_HIP_CHECK(hipMemcpyAsync(m->ja.data(), m->d_ja, size_t(m->nnz) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MLP_HIP_CHECK(hipMemcpyAsync(m->a.data(), m->d_a, size_t(m->nnz) * sizeof(double), hipMemcpyDeviceToHost, s));
+    MLP_HIP_CHECK(hipStreamSynchronize(s));
+    m->host_ready = true;
+}
 
 namespace mlp {
 namespace {
@@ -1560,6 +1580,7 @@ struct DevSym : DevPrecBase {
     }
     void build(mlp_mat A, const double* sigmas, uint32_t nsig, const uint32_t* niters, uint32_t nnit, int use_tails) {
         n = A->n;
+        ensure_host(A);
         std::vector<double> a(A->a.begin(), A->a.end());
         const HostVec<uint32_t>&ia = A->ia, &ja = A->ja;
         auto at = [&](uint32_t r, uint32_t c) -> double {      // Asym(r, c) with the updates made so far
@@ -1789,6 +1810,70 @@ int mlp_mat_info(mlp_mat m, uint32_t* n, uint32_t* nnz, uint64_t* bytes) {
     MLP_API_END
 }
 
+int mlp_spe_matrix_dev(mlp_ctx ctx, int kind, uint32_t nx, uint32_t ny, uint32_t nz, const double* kx,
+                       double kz_ratio, double c, double hx, double hy, double hz, mlp_mat* out) {
+    MLP_API_BEGIN
+    require(ctx && kx && out, "null argument");
+    require(kind == MLP_ASM_SPE || kind == MLP_ASM_TPFA, "kind must be MLP_ASM_SPE or MLP_ASM_TPFA");
+    require(nx && ny && nz, "grid dimensions must be positive");
+    const uint64_t n64 = uint64_t(nx) * ny * nz, nnz64 = spe_nnz(nx, ny, nz);
+    require(n64 < (uint64_t(1) << 32) && nnz64 < (uint64_t(1) << 32), "grid too large for 32-bit indices");
+    use_device(ctx);
+    auto m = std::make_unique<mlp_mat_s>();
+    m->ctx = ctx;
+    m->id = mlp::new_handle_id();
+    m->n = uint32_t(n64);
+    m->nnz = uint32_t(nnz64);
+    m->d_ia = m->mem.alloc<uint32_t>(size_t(m->n) + 1);
+    m->d_ja = m->mem.alloc<uint32_t>(m->nnz);
+    m->d_a = m->mem.alloc<double>(m->nnz);
+    // a full grid (every dimension >= 2) is stored in the seven-point layout, written by the assembly itself
+    const char* off = getenv("MLP_NO_STENCIL");
+    const bool stencil = nx >= 2 && ny >= 2 && nz >= 2 && n64 >= 8 && !(off && off[0] == '1');
+    if (stencil) {
+        SpeStencilOut st;
+        st.pres = m->mem.alloc<uint8_t>(m->n);
+        st.d = m->mem.alloc<double>(m->n);
+        st.lx = m->mem.alloc<double>(m->n);
+        st.ly = m->mem.alloc<double>(m->n);
+        st.lz = m->mem.alloc<double>(m->n);
+        spe_assemble_device(ctx->stream, kind == MLP_ASM_SPE, nx, ny, nz, kx, kz_ratio, c, hx, hy, hz, m->d_ia,
+                            m->d_ja, m->d_a, &st);
+        m->A.v.n = m->n;
+        m->A.v.spres = st.pres;
+        m->A.v.sd = st.d;
+        m->A.v.slx = st.lx;
+        m->A.v.sly = st.ly;
+        m->A.v.slz = st.lz;
+        m->A.v.ssy = nx;
+        m->A.v.ssz = nx * ny;
+        m->A.stencil = true;
+        m->A.nent = 0;
+        m->A.nnz = m->nnz;
+        m->host_ready = false;
+    } else {
+        spe_assemble_device(ctx->stream, kind == MLP_ASM_SPE, nx, ny, nz, kx, kz_ratio, c, hx, hy, hz, m->d_ia,
+                            m->d_ja, m->d_a);
+        m->host_ready = false;
+        ensure_host(m.get());
+        upload_sell<double>(ctx, m->mem, m->n, m->ia.data(), m->ja.data(), m->a.data(), m->A);
+    }
+    m->bytes = m->A.stored_bytes();
+    *out = m.release();
+    MLP_API_END
+}
+
+int mlp_mat_get_csr(mlp_mat A, uint32_t* ia, uint32_t* ja, double* a) {
+    MLP_API_BEGIN
+    require(A && ia && (A->n == 0 || (ja && a)), "null argument");
+    use_device(A->ctx);
+    ensure_host(A);
+    std::memcpy(ia, A->ia.data(), (size_t(A->n) + 1) * sizeof(uint32_t));
+    std::memcpy(ja, A->ja.data(), size_t(A->nnz) * sizeof(uint32_t));
+    std::memcpy(a, A->a.data(), size_t(A->nnz) * sizeof(double));
+    MLP_API_END
+}
+
 int mlp_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y) {
     MLP_API_BEGIN
     require(ctx && A, "null argument");
@@ -1906,6 +1991,34 @@ int mlp_hier_build_uh_cheb_dev(mlp_ctx ctx, uint32_t n, const uint32_t* ia, cons
         return MLP_ERR_INVALID;
     }
     return build_uh_cheb_entry(ctx, n, ia, ja, a, sigmas, nsigmas, niters, nniters, use_tails, out);
+}
+
+int mlp_hier_build_uh_cheb_mat(mlp_ctx ctx, mlp_mat A, const double* sigmas, uint32_t nsigmas, const uint32_t* niters,
+                               uint32_t nniters, int use_tails, mlp_hier* out) {
+    MLP_API_BEGIN
+    require(ctx && A && out, "null argument");
+    require(sigmas && nsigmas && niters && nniters, "sigmas and niters are required");
+    use_device(ctx);
+    if (!A->d_ia) {                                      // an uploaded matrix: its host copy goes up again
+        ensure_host(A);
+        return build_uh_cheb_entry(ctx, A->n, A->ia.data(), A->ja.data(), A->a.data(), sigmas, nsigmas, niters,
+                                   nniters, use_tails, out);
+    }
+    auto h = std::make_unique<mlp_hier_s>();
+    h->A0.nrow = h->A0.ncol = A->n;                     // level 0 is read on the device
+    ChebParams p;
+    p.sigmas.assign(sigmas, sigmas + nsigmas);
+    p.niters.assign(niters, niters + nniters);
+    p.use_tails = use_tails != 0;
+    DevCSRView v;
+    v.ia = A->d_ia;
+    v.ja = A->d_ja;
+    v.a = A->d_a;
+    v.nnz = A->nnz;
+    auto dev = device_cheb_builder(ctx->stream, v);
+    build_uh_cheb(h->A0, p, h->H, dev.get());
+    *out = h.release();
+    MLP_API_END
 }
 
 int mlp_hier_build_multi_split(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
@@ -2074,6 +2187,7 @@ int mlp_prec_simple(mlp_ctx ctx, int kind, mlp_mat A0, mlp_prec* out) {
     sp->ctx = ctx;
     sp->n = A0->n;
     if (kind == MLP_PREC_DIAG) {
+        ensure_host(A0);
         std::vector<double> d(A0->n);
         for (uint32_t i = 0; i < A0->n; i++) {
             require(A0->a[A0->ia[i]] > 0, "DiagPrec needs a positive diagonal");
@@ -2096,6 +2210,7 @@ int mlp_prec_gs(mlp_ctx ctx, mlp_mat A, mlp_prec* out) {
     MLP_API_BEGIN
     require(ctx && A && out, "null argument");
     use_device(ctx);
+    ensure_host(A);
     auto g = std::make_unique<DevGS>();
     g->ctx = ctx;
     g->build(A->ia, A->ja, A->a);
@@ -2113,11 +2228,13 @@ int mlp_prec_composite(mlp_ctx ctx, mlp_mat A, mlp_hier ms, mlp_prec* out) {
     MLP_API_BEGIN
     require(ctx && A && ms && out, "null argument");
     require(ms->H.kind == PrecKind::MULTI_SPLIT, "the second stage must be a multi_split hierarchy");
+    ensure_host(A);
     require(ms->A0.nrow == A->n && std::equal(A->ia.begin(), A->ia.end(), ms->A0.ia.begin(), ms->A0.ia.end()) &&
                 std::equal(A->ja.begin(), A->ja.end(), ms->A0.ja.begin(), ms->A0.ja.end()) &&
                 std::equal(A->a.begin(), A->a.end(), ms->A0.a.begin(), ms->A0.a.end()),
             "the multi_split hierarchy must be built on A");
     use_device(ctx);
+    ensure_host(A);
     auto g = std::make_unique<DevGS>();
     g->ctx = ctx;
     g->build(A->ia, A->ja, A->a);

@@ -12,6 +12,8 @@
 #include <cstring>
 #include <vector>
 
+#include "par.h"
+
 namespace mlp {
 namespace {
 
@@ -120,10 +122,25 @@ struct DevExec {
     size_t left = 0;
     uint32_t* bsum = nullptr;                  // scan block totals
     uint32_t bsum_cap = 0;
+    char* pin = nullptr;                       // pinned bounce buffer for large copies
+    size_t pin_n = 0;
     explicit DevExec(hipStream_t st) : s(st) {}
     ~DevExec() {
         (void)hipStreamSynchronize(s);
         for (void* p : blocks) (void)hipFree(p);
+        if (pin) (void)hipHostFree(pin);
+    }
+    static constexpr size_t kPinChunk = size_t(32) << 20;
+    char* bounce(size_t bytes) {
+        const size_t want = std::min(bytes, kPinChunk);
+        if (pin_n < want) {
+            if (pin) DEVBUILD_CHECK(hipHostFree(pin));
+            pin = nullptr;
+            pin_n = 0;
+            DEVBUILD_CHECK(hipHostMalloc((void**)&pin, want, hipHostMallocDefault));
+            pin_n = want;
+        }
+        return pin;
     }
     template <class T> T* alloc(size_t n) {
         const size_t b = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~size_t(255);
@@ -165,9 +182,23 @@ struct DevExec {
         hipLaunchKernelGGL(k_scan_fix, dim3(nb), dim3(kStepBS), 0, s, out, n, (const uint32_t*)bsum, nb);
         DEVBUILD_CHECK(hipGetLastError());
     }
+    // copies of 64 KiB and more go through the pinned buffer in chunks, each spread over the host threads
     template <class T> void get(T* dst, const T* src, size_t n) {
-        DEVBUILD_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
-        DEVBUILD_CHECK(hipStreamSynchronize(s));
+        const size_t bytes = n * sizeof(T);
+        if (bytes < (size_t(1) << 16)) {
+            DEVBUILD_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+            DEVBUILD_CHECK(hipStreamSynchronize(s));
+            return;
+        }
+        char* b = bounce(bytes);
+        const char* from = reinterpret_cast<const char*>(src);
+        char* to = reinterpret_cast<char*>(dst);
+        for (size_t off = 0; off < bytes; off += kPinChunk) {
+            const size_t len = std::min(kPinChunk, bytes - off);
+            DEVBUILD_CHECK(hipMemcpyAsync(b, from + off, len, hipMemcpyDeviceToHost, s));
+            DEVBUILD_CHECK(hipStreamSynchronize(s));
+            parallel_for(len, size_t(1) << 20, [&](uint64_t x0, uint64_t x1) { std::memcpy(to + off + x0, b + x0, x1 - x0); });
+        }
     }
     template <class T> T get1(const T* src) {
         T v;
@@ -176,30 +207,141 @@ struct DevExec {
     }
     template <class T> T* put(const T* src, size_t n) {
         T* d = alloc<T>(n);
-        if (n) DEVBUILD_CHECK(hipMemcpyAsync(d, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+        const size_t bytes = n * sizeof(T);
+        char* b = bounce(bytes);
+        const char* from = reinterpret_cast<const char*>(src);
+        char* to = reinterpret_cast<char*>(d);
+        for (size_t off = 0; off < bytes; off += kPinChunk) {
+            const size_t len = std::min(kPinChunk, bytes - off);
+            parallel_for(len, size_t(1) << 20, [&](uint64_t x0, uint64_t x1) { std::memcpy(b + x0, from + off + x0, x1 - x0); });
+            DEVBUILD_CHECK(hipMemcpyAsync(to + off, b, len, hipMemcpyHostToDevice, s));
+            DEVBUILD_CHECK(hipStreamSynchronize(s));
+        }
         return d;
     }
 };
 
 struct DeviceChebBuilder : ChebLevelBuilder {
     hipStream_t s;
-    explicit DeviceChebBuilder(hipStream_t st) : s(st) {}
+    DevCSRView dA;                              // level 0 on the device, or all null (then A is copied up)
+    DeviceChebBuilder(hipStream_t st, const DevCSRView& d) : s(st), dA(d) {}
     void run(const HostCSR& A, bool use_tails, uint32_t max_levels, Hierarchy& H) override {
         DevExec x(s);
         steps::LevelMatrix M0;
         M0.n = A.nrow;
-        M0.nnz = A.nnz();
-        M0.ia = x.put(A.ia.data(), A.ia.size());
-        M0.ja = x.put(A.ja.data(), A.ja.size());
-        M0.a = x.put(A.a.data(), A.a.size());
+        if (dA.ia) {
+            M0.nnz = dA.nnz;
+            M0.ia = dA.ia;
+            M0.ja = dA.ja;
+            M0.a = dA.a;
+        } else {
+            M0.nnz = A.nnz();
+            M0.ia = x.put(A.ia.data(), A.ia.size());
+            M0.ja = x.put(A.ja.data(), A.ja.size());
+            M0.a = x.put(A.a.data(), A.a.size());
+        }
         steps::build_levels_steps<DevExec, DevOps>(x, M0, use_tails, max_levels, H);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// seven-point assembly (modules/mesh/spe.cc:69-131), the host's spe_assemble
+// row by row: same operations in the same order, so the same bits
+// ---------------------------------------------------------------------------
+struct SpeRows {
+    int spe;
+    uint32_t nx, ny, nz;
+    const double* kx;
+    double kz_ratio, c, hx, hy, hz;
+    __device__ double face(double ka, double kb, int axis) const {
+        const double m = 2 / (1 / ka + 1 / kb);
+        if (spe) return axis == 0 ? m / (hx * hx) : axis == 1 ? m / (hy * hy) : m / (hz * hz);
+        return axis == 0 ? m * (hy * hz / hx) : axis == 1 ? m * (hx * hz / hy) : m * (hx * hy / hz);
+    }
+};
+
+struct SpeCount {
+    SpeRows R;
+    uint32_t* cnt;
+    __device__ void operator()(uint32_t r) const {
+        const uint32_t i = r % R.nx, j = (r / R.nx) % R.ny, k = r / (R.nx * R.ny);
+        cnt[r] = 1 + (k > 0) + (j > 0) + (i > 0) + (i + 1 < R.nx) + (j + 1 < R.ny) + (k + 1 < R.nz);
+    }
+};
+
+// row r: diagonal first, then z-, y-, x-, x+, y+, z+; optionally the seven-point
+// layout's arrays (diagonal, the three lower values, presence bits 1 z- 2 y- 4 x-
+// 8 x+ 16 y+ 32 z+)
+struct SpeFill {
+    SpeRows R;
+    const uint32_t* ia;
+    uint32_t* ja;
+    double* a;
+    uint8_t* pres;
+    double *d, *lx, *ly, *lz;
+    __device__ void operator()(uint32_t r) const {
+        const uint32_t nx = R.nx, ny = R.ny, nz = R.nz, sy = nx, sz = nx * ny;
+        const uint32_t i = r % nx, j = (r / nx) % ny, k = r / sz;
+        const double* kx = R.kx;
+        uint32_t p = ia[r];
+        const uint32_t dpos = p;
+        double diag = R.c;
+        if (!R.spe) {
+            double tb = 0.0;
+            if (i == 0) tb += kx[r] * (R.hy * R.hz) / (R.hx / 2);
+            if (i + 1 == nx) {
+                if (nx > 1) tb += kx[r] * (R.hy * R.hz) / (R.hx / 2);
+                else tb *= 2.0;
+            }
+            diag = diag + tb;
+        }
+        ja[p] = r;
+        p++;
+        double low[3] = {0.0, 0.0, 0.0};                      // z-, y-, x-
+        uint32_t m = 0;
+        auto add = [&](uint32_t q, double v) {
+            ja[p] = q;
+            a[p] = -v;
+            p++;
+            diag = diag + v;
+        };
+        const double kzr = R.kz_ratio * kx[r];
+        if (k > 0) { const double v = R.face(kzr, R.kz_ratio * kx[r - sz], 2); add(r - sz, v); low[0] = -v; m |= 1; }
+        if (j > 0) { const double v = R.face(kx[r], kx[r - sy], 1); add(r - sy, v); low[1] = -v; m |= 2; }
+        if (i > 0) { const double v = R.face(kx[r], kx[r - 1], 0); add(r - 1, v); low[2] = -v; m |= 4; }
+        if (i + 1 < nx) { add(r + 1, R.face(kx[r], kx[r + 1], 0)); m |= 8; }
+        if (j + 1 < ny) { add(r + sy, R.face(kx[r], kx[r + sy], 1)); m |= 16; }
+        if (k + 1 < nz) { add(r + sz, R.face(kzr, R.kz_ratio * kx[r + sz], 2)); m |= 32; }
+        a[dpos] = diag;
+        if (pres) {
+            pres[r] = uint8_t(m);
+            d[r] = diag;
+            lz[r] = low[0];
+            ly[r] = low[1];
+            lx[r] = low[2];
+        }
     }
 };
 
 }  // namespace
 
-std::unique_ptr<ChebLevelBuilder> device_cheb_builder(void* stream) {
-    return std::make_unique<DeviceChebBuilder>(static_cast<hipStream_t>(stream));
+std::unique_ptr<ChebLevelBuilder> device_cheb_builder(void* stream, const DevCSRView& level0) {
+    return std::make_unique<DeviceChebBuilder>(static_cast<hipStream_t>(stream), level0);
+}
+
+void spe_assemble_device(void* stream, int spe, uint32_t nx, uint32_t ny, uint32_t nz, const double* kx,
+                         double kz_ratio, double c, double hx, double hy, double hz, uint32_t* ia, uint32_t* ja,
+                         double* a, const SpeStencilOut* st) {
+    const uint64_t n = uint64_t(nx) * ny * nz;
+    if (!n) throw Error("grid dimensions must be positive", 1);
+    DevExec x(static_cast<hipStream_t>(stream));
+    const SpeRows R{spe, nx, ny, nz, kx, kz_ratio, c, hx, hy, hz};
+    uint32_t* cnt = x.alloc<uint32_t>(n);
+    x.each(uint32_t(n), SpeCount{R, cnt});
+    x.scan(cnt, ia, uint32_t(n));
+    if (st) x.each(uint32_t(n), SpeFill{R, ia, ja, a, st->pres, st->d, st->lx, st->ly, st->lz});
+    else x.each(uint32_t(n), SpeFill{R, ia, ja, a, nullptr, nullptr, nullptr, nullptr, nullptr});
+    DEVBUILD_CHECK(hipStreamSynchronize(x.s));
 }
 
 }  // namespace mlp

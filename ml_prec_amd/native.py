@@ -129,6 +129,11 @@ EXPORTS = {
     "mlp_time_spmv": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.c_size_t, C.POINTER(C.c_double)]),
     "mlp_time_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(C.c_double)]),
     "mlp_spe_nnz": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    "mlp_spe_matrix_dev": (C.c_int, [_vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, _vp, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]),
+    "mlp_mat_get_csr": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mlp_hier_build_uh_cheb_mat": (C.c_int, [_vp, _vp, _f64p, C.c_uint32, _u32p, C.c_uint32, C.c_int,
+                                            C.POINTER(_vp)]),
     "mlp_spe_assemble": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, _vp, C.c_double, C.c_double,
                                    C.c_double, C.c_double, C.c_double, _vp, _vp, _vp]),
 }
@@ -224,6 +229,15 @@ def build_uh_cheb(ia, ja, a, sigmas=(3.0,), niters=(2,), use_tails=True, ctx=Non
     return Hierarchy(h, UH_CHEB, n)
 
 
+def build_uh_cheb_on_device(ctx, A: "DevMatrix", sigmas=(3.0,), niters=(2,), use_tails=True):
+    """Prec(A, cfg) built on the device from the matrix handle (mlp_hier_build_uh_cheb_mat): level 0 read in
+    device memory when A was assembled there."""
+    h = _vp()
+    check(lib().mlp_hier_build_uh_cheb_mat(ctx.h, A.h, _f64(sigmas), len(sigmas), _u32(niters), len(niters),
+                                           int(use_tails), C.byref(h)))
+    return Hierarchy(h, UH_CHEB, A.n)
+
+
 def build_multi_split(ia, ja, a, config_path=None, q=(0.8,), niters=(5,), eps=(1e-2,), inner="fixed",
                       order="original", use_tails=True, degree_max=3, max_levels=15, coarse_n=0):
     """MultiSplitPrec(A, cfg) construction; config_path reads a prec config file."""
@@ -302,6 +316,32 @@ class DevMatrix:
         self.n = len(ia) - 1
         self.h = _vp()
         check(lib().mlp_csr_upload(ctx.h, self.n, _u32(ia), _u32(ja), _f64(a), C.byref(self.h)))
+
+    @classmethod
+    def spe_on_device(cls, ctx, kx: "DevArray", shape, kind="spe", c=0.0, kz_ratio=0.1, h=(20.0, 10.0, 2.0)):
+        """The seven-point matrix of a (nz, ny, nx) field assembled on the device from device-resident kx
+        (mlp_spe_matrix_dev); the same matrix as spe_assemble's."""
+        nz, ny, nx = (int(v) for v in shape)
+        if kx.n != nx * ny * nz:
+            raise ValueError("kx has the wrong size")
+        k = {"spe": 0, "tpfa": 1}.get(kind)
+        if k is None:
+            raise ValueError(f"unknown kind {kind!r}")
+        m = cls.__new__(cls)
+        m.ctx = ctx
+        m.n = nx * ny * nz
+        m.h = _vp()
+        check(lib().mlp_spe_matrix_dev(ctx.h, k, nx, ny, nz, kx.ptr, kz_ratio, c, h[0], h[1], h[2], C.byref(m.h)))
+        return m
+
+    def csr(self):
+        """The matrix's Skyline CSR as host arrays (ia, ja, a)."""
+        n, nnz, _ = self.info()
+        ia = np.empty(n + 1, dtype=np.uint32)
+        ja = np.empty(max(nnz, 1), dtype=np.uint32)
+        a = np.empty(max(nnz, 1), dtype=np.float64)
+        check(lib().mlp_mat_get_csr(self.h, ia.ctypes.data_as(_vp), ja.ctypes.data_as(_vp), a.ctypes.data_as(_vp)))
+        return ia, ja[:nnz], a[:nnz]
 
     def close(self):
         # a matrix outlives nothing: destroy it only while its context is alive
