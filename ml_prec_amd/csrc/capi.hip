@@ -479,14 +479,17 @@ template <class VT> struct DevHier : DevPrecBase {
     }
 
     // the launch shape of one level (see env_lbs) and its program
-    uint32_t plan_shaped(const LevelArrays& la, LevelProgram& P) const {
+    // small64: every component has at most 64 nodes (else the 64-thread plan is not tried)
+    uint32_t plan_shaped(const LevelArrays& la, LevelProgram& P, bool small64 = true) const {
         if (env_lbs || env_cap) {
             const uint32_t lbs = env_lbs ? env_lbs : kLBS;
             plan_fitting(la, P, lbs, env_cap ? env_cap : kLBS);
             return lbs;
         }
-        plan_fitting(la, P, 64, 64);
-        if (P.max_nodes <= 64 && !P.has_global) return 64;
+        if (small64) {
+            plan_fitting(la, P, 64, 64);
+            if (P.max_nodes <= 64 && !P.has_global) return 64;
+        }
         plan_fitting(la, P, kLBS, kLBS);
         return kLBS;
     }
@@ -634,6 +637,13 @@ template <class VT> struct DevHier : DevPrecBase {
         // uh_cheb runs its last levels as a chain (descent and ascent launches) from the highest level a such that
         // levels a..c-1 have ncheb = 2, at most kChainMax of them, and every merged component fits a wavefront
         // (MLP_NO_CHAIN=1: level kernels and the fused last pair instead).  The BF16 variant keeps level 0 apart.
+        // every level's largest component (plan_chain, plan_pair and plan_level cannot fit a 64-thread block
+        // past 64 nodes, so those attempts are skipped)
+        std::vector<uint32_t> big(nlevels, 0);
+        if (kind == MLP_PREC_UH_CHEB || kind == MLP_PREC_MULTI_SPLIT)
+            parallel_for(nlevels, 1, [&](uint64_t k0, uint64_t k1) {
+                for (uint64_t k = k0; k < k1; k++) big[k] = max_level_component(arrays_of(d[k]));
+            });
         ChainProgram CP;
         const char* nochain = getenv("MLP_NO_CHAIN");
         if (kind == MLP_PREC_UH_CHEB && nlevels >= 2 && !no_chain && !(nochain && nochain[0] == '1')) {
@@ -642,6 +652,9 @@ template <class VT> struct DevHier : DevPrecBase {
             if (bf16) a0 = std::max(a0, 1u);
             for (uint32_t l = a0; l < c; l++)
                 if (d[l].nit != 2) a0 = l + 1;
+            // a level whose own largest component exceeds a block rules out every chain through it
+            for (uint32_t l = a0; l <= c && a0 < c; l++)
+                if (big[l] > kChainBS) a0 = l + 1;
             for (uint32_t a = a0; a < c; a++) {
                 std::vector<LevelArrays> la;
                 for (uint32_t l = a; l <= c; l++) la.push_back(arrays_of(d[l]));
@@ -654,44 +667,58 @@ template <class VT> struct DevHier : DevPrecBase {
             }
         }
         const uint32_t nplan = chain_a >= 0 ? uint32_t(chain_a) : nlevels;
-        // the levels are planned independently: one thread each
-        std::vector<std::exception_ptr> err(nlevels);
-        parallel_for(nplan, 1, [&](uint64_t l0, uint64_t l1) {
-            for (uint64_t l = l0; l < l1; l++) {
-                try {
-                    const mlp_level_desc& D = d[l];
-                    LevelArrays la;
-                    la.N = D.N; la.M = D.M; la.Md = D.Md; la.map = D.map;
-                    la.L_ia = D.L_ia; la.L_ja = D.L_ja; la.L_a = D.L_a;
-                    la.U_ia = D.U_ia; la.U_ja = D.U_ja; la.U_a = D.U_a; la.dval = D.dval;
-                    lv[l].lbs = plan_shaped(la, progs[l]);
-                } catch (...) {
-                    err[l] = std::current_exception();
-                }
-            }
-        });
-        for (auto& e : err)
-            if (e) std::rethrow_exception(e);
-        // the last two uh_cheb levels run fused when every merged component fits a block
-        // (MLP_NO_PAIR=1 keeps them separate)
+        // The last two uh_cheb levels run fused when every merged component fits a block (MLP_NO_PAIR=1 keeps
+        // them separate; the BF16 variant runs level 0 on its own kernels).  The pair and the levels it does
+        // not cover are planned independently, one thread each; the pair's levels get level programs only if
+        // the pair does not fit.
+        const char* nopair = getenv("MLP_NO_PAIR");
+        const bool try_pair = kind == MLP_PREC_UH_CHEB && chain_a < 0 && nlevels >= 2 && d[nlevels - 2].nit == 2 &&
+                              !(nopair && nopair[0] == '1') && !(bf16 && nlevels == 2);
         PairProgram PP;
         uint32_t pbs = 0;
-        const char* nopair = getenv("MLP_NO_PAIR");
-        // (the BF16 variant runs level 0 on its own kernels)
-        if (kind == MLP_PREC_UH_CHEB && chain_a < 0 && nlevels >= 2 && d[nlevels - 2].nit == 2 &&
-            !(nopair && nopair[0] == '1') && !(bf16 && nlevels == 2)) {
-            auto arrays = [&](uint32_t l) {
-                const mlp_level_desc& D = d[l];
-                LevelArrays la;
-                la.N = D.N; la.M = D.M; la.Md = D.Md; la.map = D.map;
-                la.L_ia = D.L_ia; la.L_ja = D.L_ja; la.L_a = D.L_a;
-                la.U_ia = D.U_ia; la.U_ja = D.U_ja; la.U_a = D.U_a; la.dval = D.dval;
-                return la;
-            };
-            const LevelArrays la = arrays(nlevels - 2), lb = arrays(nlevels - 1);
-            if (plan_pair(la, lb, 64, PP)) pbs = 64;
-            else if (plan_pair(la, lb, kLBS, PP)) pbs = kLBS;
-            if (pbs) pair_l = int(nlevels - 2);
+        auto arrays = [&](uint32_t l) {
+            const mlp_level_desc& D = d[l];
+            LevelArrays la;
+            la.N = D.N; la.M = D.M; la.Md = D.Md; la.map = D.map;
+            la.L_ia = D.L_ia; la.L_ja = D.L_ja; la.L_a = D.L_a;
+            la.U_ia = D.U_ia; la.U_ja = D.U_ja; la.U_a = D.U_a; la.dval = D.dval;
+            return la;
+        };
+        constexpr uint32_t kPairTask = 0xffffffffu;
+        auto plan_tasks = [&](const std::vector<uint32_t>& tasks) {
+            std::vector<std::exception_ptr> err(tasks.size());
+            parallel_for(tasks.size(), 1, [&](uint64_t t0, uint64_t t1) {
+                for (uint64_t t = t0; t < t1; t++) {
+                    try {
+                        const uint32_t l = tasks[t];
+                        if (l == kPairTask) {
+                            const LevelArrays la = arrays(nlevels - 2), lb = arrays(nlevels - 1);
+                            if (big[nlevels - 2] <= 64 && big[nlevels - 1] <= 64 && plan_pair(la, lb, 64, PP)) pbs = 64;
+                            else if (plan_pair(la, lb, kLBS, PP)) pbs = kLBS;
+                        } else {
+                            lv[l].lbs = plan_shaped(arrays(l), progs[l], big[l] <= 64);
+                        }
+                    } catch (...) {
+                        err[t] = std::current_exception();
+                    }
+                }
+            });
+            for (auto& e : err)
+                if (e) std::rethrow_exception(e);
+        };
+        const uint32_t lead = try_pair ? nlevels - 2 : nplan;     // levels planned alongside the pair
+        std::vector<uint32_t> tasks;
+        if (try_pair) tasks.push_back(kPairTask);
+        for (uint32_t l = 0; l < lead; l++) tasks.push_back(l);
+        plan_tasks(tasks);
+        if (try_pair) {
+            if (pbs) {
+                pair_l = int(nlevels - 2);
+            } else {
+                tasks.clear();
+                for (uint32_t l = lead; l < nplan; l++) tasks.push_back(l);
+                plan_tasks(tasks);
+            }
         }
         if (kind == MLP_PREC_UH_CHEB)
             for (uint32_t l = 0; l < nplan && l + 1 < nlevels; l++) {

@@ -574,6 +574,22 @@ void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double*
     });
 }
 
+uint32_t max_level_component(const LevelArrays& lv) {
+    const uint32_t K = lv.N - lv.Md, M = lv.M;
+    DSU dsu(K);
+    for (uint32_t i = 0; i < K; i++)
+        for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++)
+            if (lv.L_ja[p] < K) dsu.unite(i, lv.L_ja[p]);
+    for (uint32_t i = 0; i < M; i++)
+        for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++)
+            if (lv.U_ja[p] < K) dsu.unite(i, lv.U_ja[p]);
+    std::vector<uint32_t> cnt(K, 0);
+    uint32_t best = 0;
+    for (uint32_t i = 0; i < K; i++)
+        if (i < M || lv.L_ia[i + 1] > lv.L_ia[i]) best = std::max(best, ++cnt[dsu.find(i)]);
+    return best;
+}
+
 bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_nodes, uint32_t max_slots,
                 ChainProgram& out) {
     const uint32_t nl = uint32_t(L.size());

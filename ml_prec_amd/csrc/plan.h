@@ -232,6 +232,11 @@ struct ChainProgram {
 // blocks of at most max_nodes nodes and max_nodes slots per level; returns
 // false when one component alone has more than max_nodes nodes or max_slots
 // slots on some level.
+// The most nodes (eliminated rows and kept rows with L entries) in one connected
+// component of a single level's L and U couplings.  A chain merges components
+// across levels, so a chain through a level whose largest component exceeds
+// the chain's block size cannot be planned.
+uint32_t max_level_component(const LevelArrays& lv);
 bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_nodes, uint32_t max_slots,
                 ChainProgram& P);
 
