@@ -378,16 +378,19 @@ def adjoint(N, S, args, emit=True):
     obs = g.choice(n, 200, replace=False)
     p_true = model.forward(theta_true)
     d = p_true[obs] + g.normal(0.0, 1.0, obs.size) * 0.01 * np.abs(p_true[obs])
-    times = []
+    times, phases = [], []
     for _ in range(max(1, args.steps)):
         t0 = time.perf_counter()
         p = model.forward(theta)
         L, gp = misfit(p, obs, d)
         grad = model.vjp(gp)
         times.append(time.perf_counter() - t0)
+        phases.append(dict(model.phases))
     fw, adj = model.stats[-2][1], model.stats[-1][1]
-    log(f"adjoint gradient: {1e3 * np.median(times):.1f} ms (assembly + construction on the host, forward "
-        f"{fw['niter']} and adjoint {adj['niter']} PCG iterations, solve times {fw['t_solve_ms']:.2f} + "
+    ph = {k: round(float(np.median([q[k] for q in phases])), 2) for k in phases[0]}
+    log(f"adjoint gradient: {1e3 * np.median(times):.1f} ms (matrix assembled on the device {ph['matrix']:.1f} ms, "
+        f"hierarchy built on the device {ph['construction']:.1f} ms, host planning + upload {ph['plan_upload']:.1f} "
+        f"ms; forward {fw['niter']} and adjoint {adj['niter']} PCG iterations, solve times {fw['t_solve_ms']:.2f} + "
         f"{adj['t_solve_ms']:.2f} ms)")
     errs = []
     h = 1e-6
@@ -421,6 +424,9 @@ def adjoint(N, S, args, emit=True):
         "config": {"workload": f"SPE10-shaped 60x220x{nz} ({n} cells), theta = ln kx, 200 observations, "
                                "uh_cheb PCG forward and adjoint to 1e-10"},
         "device_solve_ms": {"forward": fw["t_solve_ms"], "adjoint": adj["t_solve_ms"]},
+        "phases_ms": dict(ph, **{"what": "host wall time per forward: device assembly of the matrix, device "
+                                         "construction of the hierarchy (level arrays copied to the host), host "
+                                         "planning of the apply programs and their upload"}),
         "iterations": {"forward": fw["niter"], "adjoint": adj["niter"]},
         "misfit": L,
         "fd_check": {"directions": args.fd_dirs, "h": h, "max_rel_diff": max(errs),
@@ -1037,6 +1043,8 @@ def main():
         a2.steps, a2.fd_dirs = 3, 2
         d = adjoint(N, S, a2, emit=False)
         out["adjoint"] = {"ms_per_gradient": d["value"], "iterations": d["iterations"],
+                          "phases_ms": {k: v for k, v in d["phases_ms"].items() if k != "what"},
+                          "device_solve_ms": d["device_solve_ms"],
                           "fd_max_rel_diff": d["fd_check"]["max_rel_diff"], "fd_directions": 2,
                           "workload": d["config"]["workload"]}
     print(json.dumps(out), flush=True)

@@ -19,6 +19,8 @@ on the device through the C ABI).
 """
 from __future__ import annotations
 
+import time
+
 import numpy as np
 
 from . import native as N
@@ -49,6 +51,7 @@ class SpeAdjointModel:
         self.db = ctx.to_device(self.b)
         self._state = None          # (theta, A, B, kx, p) of the last forward solve
         self.stats = []
+        self.phases = {}            # host wall time (ms) of the last forward's phases
 
     def _assemble(self, theta):
         kx = np.exp(np.asarray(theta, dtype=np.float64))
@@ -66,20 +69,27 @@ class SpeAdjointModel:
     def forward(self, theta):
         """p(theta) as a host array; keeps the matrix and hierarchy for the adjoint."""
         self.close()
+        t0 = time.perf_counter()
         if self.device:
             kx = np.exp(np.asarray(theta, dtype=np.float64))
             dkx = self.ctx.to_device(kx)
             A = N.DevMatrix.spe_on_device(self.ctx, dkx, self.shape, kind="spe", c=self.c, kz_ratio=KZ_RATIO,
                                           h=(S.HX, S.HY, S.HZ))
-            B = N.DevPrec(self.ctx, A, N.build_uh_cheb_on_device(self.ctx, A))
+            t1 = time.perf_counter()
+            H = N.build_uh_cheb_on_device(self.ctx, A)
         else:
             kx, ia, ja, a = self._assemble(theta)
             A = N.DevMatrix(self.ctx, ia, ja, a)
-            B = N.DevPrec(self.ctx, A, N.build_uh_cheb(ia, ja, a))
             dkx = self.ctx.to_device(kx)
+            t1 = time.perf_counter()
+            H = N.build_uh_cheb(ia, ja, a)
+        t2 = time.perf_counter()
+        B = N.DevPrec(self.ctx, A, H)
+        t3 = time.perf_counter()
         p = self.ctx.array(self.n)
         st = N.pcg(self.ctx, A, B, self.db, p, eps=self.eps, maxit=self.maxit, x0_mode=N.X0_ZERO, use_graph=True,
                    check_every=8)
+        self.phases = {"matrix": 1e3 * (t1 - t0), "construction": 1e3 * (t2 - t1), "plan_upload": 1e3 * (t3 - t2)}
         self.stats.append(("forward", st))
         self._state = (np.array(theta, dtype=np.float64), A, B, dkx, p)
         return p.get()

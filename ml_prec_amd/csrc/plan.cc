@@ -119,13 +119,19 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P, const Bloc
         comp_blk[c] = nblk - 1;
         fill += csize[c];
     }
-    std::vector<std::vector<uint32_t>> rows(nblk);
-    for (uint32_t i = 0; i < K; i++) if (is_node[i]) rows[comp_blk[comp_of[i]]].push_back(i);
+    // the blocks' rows in ascending order, as one array with block offsets
+    std::vector<uint32_t> roff(size_t(nblk) + 1, 0), rows_all;
+    for (uint32_t i = 0; i < K; i++) if (is_node[i]) roff[comp_blk[comp_of[i]] + 1]++;
+    for (uint32_t b = 0; b < nblk; b++) roff[b + 1] += roff[b];
+    rows_all.resize(roff[nblk]);
+    {
+        std::vector<uint32_t> at(roff.begin(), roff.end() - 1);
+        for (uint32_t i = 0; i < K; i++) if (is_node[i]) rows_all[at[comp_blk[comp_of[i]]]++] = i;
+    }
 
     P.nblk = nblk;
     P.blk.resize(kBlkFields * size_t(nblk));
-    uint32_t total = 0;
-    for (auto& R : rows) total += uint32_t(R.size());
+    const uint32_t total = roff[nblk];
     P.nodes = total;
     P.qrow.resize(total);
     P.out.resize(total);
@@ -140,26 +146,27 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P, const Bloc
     // the rows of `in` (and, for the U list, only those < M) ordered by key,
     // equal keys in their order in `in` (a stable sort, done by counting)
     std::vector<uint32_t> count;
-    auto by_depth = [&count](const std::vector<uint32_t>& in, const std::vector<uint32_t>& key, uint32_t below,
+    auto by_depth = [&count](const uint32_t* b, const uint32_t* e, const std::vector<uint32_t>& key, uint32_t below,
                              std::vector<uint32_t>& out) {
         uint32_t kmax = 0, m = 0;
-        for (uint32_t i : in)
-            if (i < below) { kmax = std::max(kmax, key[i]); m++; }
+        for (const uint32_t* r = b; r < e; r++)
+            if (*r < below) { kmax = std::max(kmax, key[*r]); m++; }
         count.assign(size_t(kmax) + 2, 0);
-        for (uint32_t i : in)
-            if (i < below) count[key[i] + 1]++;
+        for (const uint32_t* r = b; r < e; r++)
+            if (*r < below) count[key[*r] + 1]++;
         for (uint32_t k = 0; k <= kmax; k++) count[k + 1] += count[k];
         out.resize(m);
-        for (uint32_t i : in)
-            if (i < below) out[count[key[i]]++] = i;
+        for (const uint32_t* r = b; r < e; r++)
+            if (*r < below) out[count[key[*r]]++] = *r;
     };
     std::vector<uint32_t> lo, uo;
     uint32_t pos = 0, upos = 0;
     for (uint32_t b = 0; b < nblk; b++) {
-        auto& R = rows[b];
+        const uint32_t* Rb = rows_all.data() + roff[b];
+        const uint32_t* Re = rows_all.data() + roff[b + 1];
         const uint32_t base = pos;
-        by_depth(R, dL, K, lo);
-        by_depth(R, dU, M, uo);
+        by_depth(Rb, Re, dL, K, lo);
+        by_depth(Rb, Re, dU, M, uo);
         for (uint32_t k = 0; k < lo.size(); k++) local[lo[k]] = k;
         uint32_t* B = &P.blk[kBlkFields * b];
         B[0] = base;
@@ -319,9 +326,17 @@ bool plan_pair(const LevelArrays& la, const LevelArrays& lb, uint32_t cap, PairP
         comp_blk[c] = nblk - 1;
         fill.S += k.S; fill.nk += k.nk; fill.nu += k.nu; fill.nu2 += k.nu2;
     }
-    std::vector<std::vector<uint32_t>> rows(nblk);
+    // the blocks' rows in ascending order, as one array with block offsets
+    std::vector<uint32_t> roff(size_t(nblk) + 1, 0), rows_all;
     for (uint32_t i = 0; i < K; i++)
-        if (is_node[i]) rows[comp_blk[comp_of[i]]].push_back(i);
+        if (is_node[i]) roff[comp_blk[comp_of[i]] + 1]++;
+    for (uint32_t b = 0; b < nblk; b++) roff[b + 1] += roff[b];
+    rows_all.resize(roff[nblk]);
+    {
+        std::vector<uint32_t> at(roff.begin(), roff.end() - 1);
+        for (uint32_t i = 0; i < K; i++)
+            if (is_node[i]) rows_all[at[comp_blk[comp_of[i]]]++] = i;
+    }
 
     PairProgram P;
     P.N = N; P.M = M; P.Md = Md; P.n = n; P.nblk = nblk;
@@ -331,27 +346,36 @@ bool plan_pair(const LevelArrays& la, const LevelArrays& lb, uint32_t cap, PairP
     P.Lptr2.push_back(0);
     P.Uptr2.push_back(0);
     std::vector<uint32_t> local(K, kNoNode), local2(K, kNoNode);
-    // rows of `in` passing `keep`, ordered by key (stable)
-    auto by_key = [](const std::vector<uint32_t>& in, auto key, auto keep) {
-        std::vector<uint32_t> o;
-        for (uint32_t i : in) if (keep(i)) o.push_back(i);
-        std::stable_sort(o.begin(), o.end(), [&](uint32_t a, uint32_t b) { return key(a) < key(b); });
-        return o;
+    // rows of [b, e) passing `keep`, ordered by key, equal keys in their order (a counting sort into o)
+    std::vector<uint32_t> kcount;
+    auto by_key = [&kcount](const uint32_t* b, const uint32_t* e, auto key, auto keep, std::vector<uint32_t>& o) {
+        uint32_t kmax = 0, m = 0;
+        for (const uint32_t* r = b; r < e; r++)
+            if (keep(*r)) { kmax = std::max(kmax, key(*r)); m++; }
+        kcount.assign(size_t(kmax) + 2, 0);
+        for (const uint32_t* r = b; r < e; r++)
+            if (keep(*r)) kcount[key(*r) + 1]++;
+        for (uint32_t k = 0; k <= kmax; k++) kcount[k + 1] += kcount[k];
+        o.resize(m);
+        for (const uint32_t* r = b; r < e; r++)
+            if (keep(*r)) o[kcount[key(*r)]++] = *r;
     };
+    std::vector<uint32_t> lo, uo, ko, u2o;
     auto levels_of = [](const std::vector<uint32_t>& o, auto key) {
         uint32_t nl = 0;
         for (size_t k = 0; k < o.size(); k++) nl = std::max(nl, key(o[k]) + 1);
         return nl;
     };
     for (uint32_t b = 0; b < nblk; b++) {
-        const auto& R = rows[b];
-        auto lo = by_key(R, [&](uint32_t i) { return dL[i]; }, [](uint32_t) { return true; });
-        auto uo = by_key(R, [&](uint32_t i) { return dU[i]; }, [&](uint32_t i) { return i < M; });
+        const uint32_t* Rb = rows_all.data() + roff[b];
+        const uint32_t* Re = rows_all.data() + roff[b + 1];
+        by_key(Rb, Re, [&](uint32_t i) { return dL[i]; }, [](uint32_t) { return true; }, lo);
+        by_key(Rb, Re, [&](uint32_t i) { return dU[i]; }, [&](uint32_t i) { return i < M; }, uo);
         // kept nodes, by level-(l+1) L depth (rows isolated there at depth 0)
         auto key2 = [&](uint32_t i) { const uint32_t q = pos2[i - M]; return q < M2 ? dL2[q] : 0u; };
-        auto ko = by_key(R, key2, [&](uint32_t i) { return i >= M; });
+        by_key(Rb, Re, key2, [&](uint32_t i) { return i >= M; }, ko);
         auto keyU2 = [&](uint32_t i) { return dU2[pos2[i - M]]; };
-        auto u2o = by_key(R, keyU2, [&](uint32_t i) { return i >= M && pos2[i - M] < M2; });
+        by_key(Rb, Re, keyU2, [&](uint32_t i) { return i >= M && pos2[i - M] < M2; }, u2o);
         for (uint32_t k = 0; k < lo.size(); k++) local[lo[k]] = k;
         for (uint32_t k = 0; k < ko.size(); k++) local2[ko[k]] = k;
         uint32_t* B = &P.blk[size_t(kPairFields) * b];
