@@ -117,8 +117,12 @@ def ensemble_cpu_baseline(spec, eps, procs):
 
 def ensemble(N, S, args, emit=True):
     """Ensemble: COUNT SPE10-shaped systems of 60x220xnz cells, nz drawn from 5/10/20/40/85, each with its own
-    field, uh_cheb hierarchy and context (stream).  A step solves all of them to 1e-8 from x0 = 0, concurrently
-    through mlp_pcg_batch; the same systems solved one after another are timed beside it."""
+    field; matrices assembled and hierarchies built on the device.  A step solves all of them to eps from
+    x0 = 0.  Headline: the systems grouped by hierarchy depth, each group stacked into one block-diagonal
+    system with one hierarchy (FP32, FP64 flexible PCG with true residual every 20 iterations, as the
+    configuration asks) and solved by mlp_pcg_ensemble, every launch covering the whole group; the groups run
+    concurrently, one stream each.  Beside it: the same stacked groups in FP64, the systems as 64 concurrent
+    solves on 64 streams (mlp_pcg_batch, FP64), and one after another."""
     from concurrent.futures import ThreadPoolExecutor
     spec = S.ensemble_spec(args.ensemble)
     cpu = None
@@ -128,71 +132,145 @@ def ensemble(N, S, args, emit=True):
         log(f"CPU oracle: {cpu['value']:.1f} solves/s aggregate on {procs} processes ({cpu['sample']})")
     t0 = time.perf_counter()
     with ThreadPoolExecutor(8) as ex:
-        systems = list(ex.map(lambda p: S.ensemble_system(*p), spec))
-        hiers = list(ex.map(lambda s: N.build_uh_cheb(s.ia, s.ja, s.a), systems))
-    cells = sum(s.n for s in systems)
-    log(f"ensemble: {len(systems)} systems, {cells} cells, generated and built in {time.perf_counter() - t0:.1f}s")
-    jobs = []
-    for s, H in zip(systems, hiers):
+        fields = list(ex.map(lambda p: S.spe10_field(60, 220, p[0], p[1]).ravel(), spec))
+    rhs = [S.rhs(k.size, seed + 7919) for k, (_, seed) in zip(fields, spec)]
+    hh = (S.HX, S.HY, S.HZ)
+    # one context per system: its matrix and hierarchy (device assembly and construction)
+    members = []
+    for (nz, _), kx, b in zip(spec, fields, rhs):
         ctx = N.Context(0)
-        A = N.DevMatrix(ctx, s.ia, s.ja, s.a)
+        A = N.DevMatrix.spe_on_device(ctx, ctx.to_device(kx), (nz, 220, 60), kind="spe", c=1.0, kz_ratio=0.1, h=hh)
+        H = N.build_uh_cheb_on_device(ctx, A)
         B = N.DevPrec(ctx, A, H, precision=N.MLP_FP64)
-        jobs.append((ctx, A, B, ctx.to_device(s.b), ctx.array(s.n)))
+        members.append(dict(ctx=ctx, A=A, B=B, depth=H.nlevels, nz=nz, n=kx.size, b=ctx.to_device(b),
+                            x=ctx.array(kx.size)))
+    cells = sum(m["n"] for m in members)
+    groups = {}
+    for i, m in enumerate(members):
+        groups.setdefault(m["depth"], []).append(i)
+    # the stacked groups, each on a context of its own, in FP64 and FP32
+    stacks = []
+    for depth, idx in sorted(groups.items()):
+        ctx = N.Context(0)
+        kx = np.concatenate([fields[i] for i in idx])
+        A = N.DevMatrix.spe_ensemble_on_device(ctx, ctx.to_device(kx), 60, 220, [spec[i][0] for i in idx],
+                                               kind="spe", c=1.0, kz_ratio=0.1, h=hh)
+        H = N.build_uh_cheb_on_device(ctx, A)
+        assert H.nlevels == depth
+        off = np.concatenate([[0], np.cumsum([members[i]["n"] for i in idx])]).astype(np.uint32)
+        stacks.append(dict(ctx=ctx, A=A, B64=N.DevPrec(ctx, A, H, precision=N.MLP_FP64),
+                           B32=N.DevPrec(ctx, A, H, precision=N.MLP_FP32), idx=idx, off=off, depth=depth,
+                           b=ctx.to_device(np.concatenate([rhs[i] for i in idx])), x=ctx.array(int(off[-1]))))
+    log(f"ensemble: {len(members)} systems, {cells} cells, {len(stacks)} depth groups "
+        f"{[(g['depth'], len(g['idx'])) for g in stacks]}, set up in {time.perf_counter() - t0:.1f}s")
     kw = dict(eps=args.eps, maxit=10000, x0_mode=N.X0_ZERO, check_every=8, use_graph=bool(args.graph))
-    for _ in range(max(1, args.warmup)):
+    kw32 = dict(kw, flexible=True, true_res_every=20)
+
+    def stacked(mixed):
+        def one(g):
+            rc, st = N.pcg_ensemble(g["ctx"], g["A"], g["B32"] if mixed else g["B64"], g["off"], g["b"], g["x"],
+                                    **(kw32 if mixed else kw))
+            if rc:
+                raise RuntimeError(f"ensemble group solve failed: {rc}")
+            return st
+        with ThreadPoolExecutor(len(stacks)) as ex:
+            return list(ex.map(one, stacks))
+
+    def timed(fn, steps, warmup):
+        for _ in range(max(1, warmup)):
+            out = fn()
+        t = time.perf_counter()
+        for _ in range(steps):
+            out = fn()
+        return 1e3 * (time.perf_counter() - t) / steps, out
+
+    ms32, st32 = timed(lambda: stacked(True), args.steps, args.warmup)
+    x32 = [g["x"].get() for g in stacks]
+    ms64, st64 = timed(lambda: stacked(False), args.steps, args.warmup)
+    x64 = [g["x"].get() for g in stacks]
+    it32 = [d["niter"] for g in st32 for d in g]
+    worst32 = max(d["true_rel_res"] for g in st32 for d in g)
+    log(f"stacked groups, FP32 hierarchy: {ms32:.2f} ms for {len(members)} solves "
+        f"({len(members) / ms32 * 1e3:.0f} solves/s), iterations {min(it32)}..{max(it32)}, worst true rel res "
+        f"{worst32:.2e}")
+    log(f"stacked groups, FP64: {ms64:.2f} ms ({len(members) / ms64 * 1e3:.0f} solves/s)")
+    jobs = [(m["ctx"], m["A"], m["B"], m["b"], m["x"]) for m in members]
+
+    def batch():
         rc, st = N.pcg_batch(jobs, **kw)
         if rc:
             raise RuntimeError(f"ensemble solve failed: {rc}")
-    for j in jobs:
-        j[0].sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rc, st = N.pcg_batch(jobs, **kw)
-    for j in jobs:
-        j[0].sync()
-    ms_batch = 1e3 * (time.perf_counter() - t0) / args.steps
-    iters = [d["niter"] for d in st]
-    worst = max(d["true_rel_res"] for d in st)
-    log(f"concurrent: {ms_batch:.2f} ms for {len(jobs)} solves ({len(jobs) / ms_batch * 1e3:.0f} solves/s), "
-        f"iterations {min(iters)}..{max(iters)}, worst true rel res {worst:.2e}")
+        for j in jobs:
+            j[0].sync()
+        return st
+    ms_batch, stb = timed(batch, args.steps, args.warmup)
+    log(f"64 streams (mlp_pcg_batch), FP64: {ms_batch:.2f} ms ({len(jobs) / ms_batch * 1e3:.0f} solves/s)")
+    # each system's stacked FP64 solution against its own solve on its own stream (bitwise by construction)
+    same = True
+    err32 = 0.0
+    for g, xs64, xs32 in zip(stacks, x64, x32):
+        for k, i in enumerate(g["idx"]):
+            xi = members[i]["x"].get()
+            o0, o1 = int(g["off"][k]), int(g["off"][k + 1])
+            same = same and np.array_equal(xs64[o0:o1].view(np.uint64), xi.view(np.uint64))
+            err32 = max(err32, float(np.linalg.norm(xs32[o0:o1] - xi) / np.linalg.norm(xi)))
     t0 = time.perf_counter()
     for ctx, A, B, b, x in jobs:
         N.pcg(ctx, A, B, b, x, **kw)
     for j in jobs:
         j[0].sync()
     ms_seq = 1e3 * (time.perf_counter() - t0)
-    log(f"one after another: {ms_seq:.2f} ms ({len(jobs) / ms_seq * 1e3:.0f} solves/s)")
+    log(f"one after another: {ms_seq:.2f} ms ({len(jobs) / ms_seq * 1e3:.0f} solves/s); stacked FP64 solutions "
+        f"{'equal' if same else 'DIFFER FROM'} the single solves bit for bit; FP32 within {err32:.1e} of FP64")
     out = {
-        "metric": "ensemble PCG solves per second (SPE10-shaped 60x220xnz systems, 1e-8 rel-res, FP64)",
-        "value": round(len(jobs) / ms_batch * 1e3, 2),
+        "metric": "ensemble PCG solves per second (64 SPE10-shaped 60x220xnz systems, 1e-8 rel-res, FP64 outer, "
+                  "FP32 preconditioner)",
+        "value": round(len(members) / ms32 * 1e3, 2),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_batch, 3),
+        "ms_per_step": round(ms32, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32",
         "data": "synthetic",
-        "config": {"workload": f"{len(jobs)} SPE10-shaped 60x220xnz systems (nz from 5/10/20/40/85, {cells} cells "
+        "config": {"workload": f"{len(members)} SPE10-shaped 60x220xnz systems (nz from 5/10/20/40/85, {cells} cells "
                                "in total), reference spe.cc assembly c=1, uh_cheb sigma=3 ncheb=2, PCG x0=0 to "
-                               f"{args.eps:g}, all solved concurrently (one stream per system)",
-                   "nz": [nz for nz, _ in spec]},
+                               f"{args.eps:g}; systems grouped by hierarchy depth, each group one block-diagonal "
+                               "system solved by mlp_pcg_ensemble (every launch covers the group), groups on "
+                               "concurrent streams",
+                   "nz": [nz for nz, _ in spec],
+                   "groups": [{"depth": g["depth"], "systems": len(g["idx"]), "rows": int(g["off"][-1])}
+                              for g in stacks]},
+        "ms_stacked_fp64": round(ms64, 3),
+        "ms_streams_fp64": round(ms_batch, 3),
         "ms_one_after_another": round(ms_seq, 3),
-        "speedup_concurrent": round(ms_seq / ms_batch, 3),
-        "iterations": iters,
-        "worst_true_rel_res": worst,
+        "speedup_stacked_vs_streams_fp64": round(ms_batch / ms64, 3),
+        "speedup_concurrent": round(ms_seq / ms32, 3),
+        "stacked_fp64_bitwise_equal_single": bool(same),
+        "fp32_rel_err_vs_fp64": err32,
+        "iterations": it32,
+        "worst_true_rel_res": worst32,
         "cpu_baseline": cpu,
     }
     if emit:
         print(json.dumps(out), flush=True)
-    for ctx, A, B, b, x in jobs:
-        B.close()
-        A.close()
-    del b, x
-    for j in jobs:
-        j[0].close()
+    for g in stacks:
+        g["B32"].close()
+        g["B64"].close()
+        g["A"].close()
+    for m in members:
+        m["B"].close()
+        m["A"].close()
+    del jobs
+    for g in stacks:
+        del g["b"], g["x"]
+        g["ctx"].close()
+    for m in members:
+        del m["b"], m["x"]
+        m["ctx"].close()
     return out
 
 
@@ -1036,10 +1114,13 @@ def main():
         a2.ensemble = 64
         e = ensemble(N, S, a2, emit=False)
         out["ensemble_64"] = {"solves_per_s": e["value"], "ms_all_64": e["ms_per_step"],
+                              "ms_stacked_fp64": e["ms_stacked_fp64"], "ms_streams_fp64": e["ms_streams_fp64"],
                               "ms_one_after_another": e["ms_one_after_another"],
                               "speedup_concurrent": e["speedup_concurrent"],
+                              "stacked_fp64_bitwise_equal_single": e["stacked_fp64_bitwise_equal_single"],
                               "iterations": [min(e["iterations"]), max(e["iterations"])],
-                              "worst_true_rel_res": e["worst_true_rel_res"], "workload": e["config"]["workload"]}
+                              "worst_true_rel_res": e["worst_true_rel_res"], "groups": e["config"]["groups"],
+                              "workload": e["config"]["workload"]}
         a2.steps, a2.fd_dirs = 3, 2
         d = adjoint(N, S, a2, emit=False)
         out["adjoint"] = {"ms_per_gradient": d["value"], "iterations": d["iterations"],

@@ -269,6 +269,18 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* opts)
  * right-hand side's. */
 int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const double* d_b, double* d_x,
                   const mlp_solve_opts* opts, mlp_solve_stats* stats);
+/* Ensembles in one launch per step (no counterpart in the reference): nseg (1..256) independent systems held
+ * as one block-diagonal matrix A (segment y = rows seg_off[y] .. seg_off[y+1]-1, host array; no entry
+ * couples two segments, e.g. mlp_spe_ensemble_matrix_dev) with one preconditioner B built on A (a uh_cheb
+ * hierarchy of such a matrix is the union of the systems' own hierarchies; when the systems' hierarchies
+ * have equal depth it applies each system's own preconditioner exactly).  Each segment runs PCGSolver's
+ * steps (pcg.cc:8-102) with its own state, its own reductions (the grid and order of a solve of that system
+ * alone) and its own stop test, so its x and stats are bitwise those of mlp_pcg on that system with that
+ * preconditioner; every kernel launch of an iteration, the preconditioner's included, covers all segments.
+ * Flexible solves then take mlp_pcg's true-residual restarts (every segment that needs one in one batched
+ * phase per round).  stats has nseg entries; the return code is the first failing segment's. */
+int mlp_pcg_ensemble(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nseg, const uint32_t* seg_off, const double* d_b,
+                     double* d_x, const mlp_solve_opts* opts, mlp_solve_stats* stats);
 /* Adjoint gradient for the two-point-flux assembly of modules/mesh/spe.cc:69-131 (no counterpart in the
  * reference; for differentiating a misfit through the solve).  Parameters theta_i = ln kx_i per cell
  * (ky = kx, kz = kz_ratio kx); fx, fy, fz are the per-direction face factors (1/h^2 in spe.cc's scaling);
@@ -309,6 +321,12 @@ int mlp_spe_assemble(int kind, uint32_t nx, uint32_t ny, uint32_t nz, const doub
  * matrix. */
 int mlp_spe_matrix_dev(mlp_ctx ctx, int kind, uint32_t nx, uint32_t ny, uint32_t nz, const double* d_kx,
                        double kz_ratio, double c, double hx, double hy, double hz, mlp_mat* out);
+/* An ensemble of nsys such systems with one nx*ny footprint and depths nz[0..nsys-1] (host array), stacked
+ * along z into one block-diagonal matrix (no entry couples two systems; block s is bit for bit the matrix
+ * mlp_spe_matrix_dev assembles for system s alone).  d_kx holds the systems' fields one after another. */
+int mlp_spe_ensemble_matrix_dev(mlp_ctx ctx, int kind, uint32_t nx, uint32_t ny, uint32_t nsys, const uint32_t* nz,
+                                const double* d_kx, double kz_ratio, double c, double hx, double hy, double hz,
+                                mlp_mat* out);
 /* A's Skyline CSR into host arrays (n+1, nnz, nnz): CSRMatrix::get_ia/get_ja/get_a, matrix.h:204-210 */
 int mlp_mat_get_csr(mlp_mat A, uint32_t* ia, uint32_t* ja, double* a);
 

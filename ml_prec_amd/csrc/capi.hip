@@ -126,6 +126,16 @@ struct mlp_ctx_s {
     double *mr = nullptr, *mz = nullptr, *mp = nullptr, *mAp = nullptr, *mzp = nullptr, *mpart = nullptr;
     mlp::SolveState* mst = nullptr;
     mlp::SolveState* mst_host = nullptr;   // pinned, R states
+    // ragged ensembles (mlp_pcg_ensemble): vectors of the block-diagonal system, per segment a state,
+    // a partial-sum buffer and a result slot, the segment offsets on the device
+    mlp::DevMem ens_mem;
+    uint32_t ens_S = 0, ens_n = 0;
+    double *er = nullptr, *ez = nullptr, *ep = nullptr, *eAp = nullptr, *ezp = nullptr, *epart = nullptr;
+    double* eout = nullptr;
+    uint32_t* esoff = nullptr;
+    mlp::SolveState* est = nullptr;
+    mlp::SolveState* est_host = nullptr;   // pinned, S states
+    double* eout_host = nullptr;           // pinned, S doubles
     // solver workspace, grown on demand and kept for the next solve
     uint32_t ws_n = 0;
     mlp::DevMem ws_mem;
@@ -134,8 +144,9 @@ struct mlp_ctx_s {
     // instantiated PCG iteration graphs, reused while the same matrix,
     // preconditioner and vectors are solved again
     struct GraphEntry {
-        int solver = 0;                // 0 PCG, 1 Chebyshev, 2 simple, 3 batched PCG
-        uint32_t R = 1;                // right-hand sides per launch
+        int solver = 0;                // 0 PCG, 1 Chebyshev, 2 simple, 3 batched PCG, 4 ensemble PCG
+        uint32_t R = 1;                // right-hand sides (segments) per launch
+        uint32_t gmax = 0;             // ensemble: the largest segment grid
         uint64_t A, B;
         const void *b, *x;
         int flex;
@@ -1553,7 +1564,12 @@ void context_release(mlp_ctx ctx) {
     ctx->drop_graphs(0);
     ctx->ws_mem.release();
     ctx->multi_mem.release();
+    ctx->ens_mem.release();
     ctx->mem.release();
+    if (ctx->est_host) (void)hipHostFree(ctx->est_host);
+    ctx->est_host = nullptr;
+    if (ctx->eout_host) (void)hipHostFree(ctx->eout_host);
+    ctx->eout_host = nullptr;
     if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
     ctx->host_pinned = nullptr;
     if (ctx->mst_host) (void)hipHostFree(ctx->mst_host);
@@ -1837,13 +1853,23 @@ int mlp_mat_info(mlp_mat m, uint32_t* n, uint32_t* nnz, uint64_t* bytes) {
     MLP_API_END
 }
 
-int mlp_spe_matrix_dev(mlp_ctx ctx, int kind, uint32_t nx, uint32_t ny, uint32_t nz, const double* kx,
-                       double kz_ratio, double c, double hx, double hy, double hz, mlp_mat* out) {
+// The device assembly behind mlp_spe_matrix_dev and mlp_spe_ensemble_matrix_dev: nsys systems of nx*ny*nzs[s]
+// cells stacked along z (one system: nsys = 1).
+static int spe_matrix_dev(mlp_ctx ctx, int kind, uint32_t nx, uint32_t ny, uint32_t nsys, const uint32_t* nzs,
+                          const double* kx, double kz_ratio, double c, double hx, double hy, double hz, mlp_mat* out) {
     MLP_API_BEGIN
-    require(ctx && kx && out, "null argument");
+    require(ctx && kx && out && nzs, "null argument");
     require(kind == MLP_ASM_SPE || kind == MLP_ASM_TPFA, "kind must be MLP_ASM_SPE or MLP_ASM_TPFA");
-    require(nx && ny && nz, "grid dimensions must be positive");
-    const uint64_t n64 = uint64_t(nx) * ny * nz, nnz64 = spe_nnz(nx, ny, nz);
+    require(nx && ny && nsys, "grid dimensions must be positive");
+    uint64_t nz64 = 0, nnz64 = 0;
+    for (uint32_t k = 0; k < nsys; k++) {
+        require(nzs[k] > 0, "grid dimensions must be positive");
+        nz64 += nzs[k];
+        nnz64 += spe_nnz(nx, ny, nzs[k]);
+    }
+    require(nz64 < (uint64_t(1) << 32), "grid too large for 32-bit indices");
+    const uint32_t nz = uint32_t(nz64);
+    const uint64_t n64 = uint64_t(nx) * ny * nz;
     require(n64 < (uint64_t(1) << 32) && nnz64 < (uint64_t(1) << 32), "grid too large for 32-bit indices");
     use_device(ctx);
     auto m = std::make_unique<mlp_mat_s>();
@@ -1854,9 +1880,25 @@ int mlp_spe_matrix_dev(mlp_ctx ctx, int kind, uint32_t nx, uint32_t ny, uint32_t
     m->d_ia = m->mem.alloc<uint32_t>(size_t(m->n) + 1);
     m->d_ja = m->mem.alloc<uint32_t>(m->nnz);
     m->d_a = m->mem.alloc<double>(m->nnz);
-    // a full grid (every dimension >= 2) is stored in the seven-point layout, written by the assembly itself
+    // the stacked systems' plane ranges
+    uint32_t* zfirst = nullptr;
+    uint32_t* zlast = nullptr;
+    DevMem ztab;
+    if (nsys > 1) {
+        std::vector<uint32_t> zf(nz), zl(nz);
+        for (uint32_t k = 0, z = 0; k < nsys; z += nzs[k], k++)
+            for (uint32_t q = z; q < z + nzs[k]; q++) {
+                zf[q] = z;
+                zl[q] = z + nzs[k] - 1;
+            }
+        zfirst = ztab.upload(zf, ctx->stream);
+        zlast = ztab.upload(zl, ctx->stream);
+    }
+    // a full grid (every dimension >= 2), or a stack of them, is stored in the seven-point layout (presence bits
+    // per row), written by the assembly itself
     const char* off = getenv("MLP_NO_STENCIL");
-    const bool stencil = nx >= 2 && ny >= 2 && nz >= 2 && n64 >= 8 && !(off && off[0] == '1');
+    bool stencil = nx >= 2 && ny >= 2 && n64 >= 8 && !(off && off[0] == '1');
+    for (uint32_t k = 0; k < nsys; k++) stencil = stencil && nzs[k] >= 2;
     if (stencil) {
         SpeStencilOut st;
         st.pres = m->mem.alloc<uint8_t>(m->n);
@@ -1865,7 +1907,7 @@ int mlp_spe_matrix_dev(mlp_ctx ctx, int kind, uint32_t nx, uint32_t ny, uint32_t
         st.ly = m->mem.alloc<double>(m->n);
         st.lz = m->mem.alloc<double>(m->n);
         spe_assemble_device(ctx->stream, kind == MLP_ASM_SPE, nx, ny, nz, kx, kz_ratio, c, hx, hy, hz, m->d_ia,
-                            m->d_ja, m->d_a, &st);
+                            m->d_ja, m->d_a, &st, zfirst, zlast);
         m->A.v.n = m->n;
         m->A.v.spres = st.pres;
         m->A.v.sd = st.d;
@@ -1880,7 +1922,7 @@ int mlp_spe_matrix_dev(mlp_ctx ctx, int kind, uint32_t nx, uint32_t ny, uint32_t
         m->host_ready = false;
     } else {
         spe_assemble_device(ctx->stream, kind == MLP_ASM_SPE, nx, ny, nz, kx, kz_ratio, c, hx, hy, hz, m->d_ia,
-                            m->d_ja, m->d_a);
+                            m->d_ja, m->d_a, nullptr, zfirst, zlast);
         m->host_ready = false;
         ensure_host(m.get());
         upload_sell<double>(ctx, m->mem, m->n, m->ia.data(), m->ja.data(), m->a.data(), m->A);
@@ -1888,6 +1930,17 @@ int mlp_spe_matrix_dev(mlp_ctx ctx, int kind, uint32_t nx, uint32_t ny, uint32_t
     m->bytes = m->A.stored_bytes();
     *out = m.release();
     MLP_API_END
+}
+
+int mlp_spe_matrix_dev(mlp_ctx ctx, int kind, uint32_t nx, uint32_t ny, uint32_t nz, const double* kx,
+                       double kz_ratio, double c, double hx, double hy, double hz, mlp_mat* out) {
+    return spe_matrix_dev(ctx, kind, nx, ny, 1, &nz, kx, kz_ratio, c, hx, hy, hz, out);
+}
+
+int mlp_spe_ensemble_matrix_dev(mlp_ctx ctx, int kind, uint32_t nx, uint32_t ny, uint32_t nsys, const uint32_t* nz,
+                                const double* kx, double kz_ratio, double c, double hx, double hy, double hz,
+                                mlp_mat* out) {
+    return spe_matrix_dev(ctx, kind, nx, ny, nsys, nz, kx, kz_ratio, c, hx, hy, hz, out);
 }
 
 int mlp_mat_get_csr(mlp_mat A, uint32_t* ia, uint32_t* ja, double* a) {
@@ -1924,8 +1977,8 @@ int mlp_residual(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, doubl
 // ---- BLAS-1 ----
 // Streaming reductions run a grid-stride loop over at most kRedGrid blocks
 // (8 per CU), which keeps the one-block finalize short.
-constexpr uint32_t kRedGrid = 1024;
 constexpr uint32_t kMaxRhs = 64;          // right-hand sides of one mlp_pcg_multi call
+constexpr uint32_t kMaxSegments = 256;    // systems of one mlp_pcg_ensemble call
 static_assert(kRedGrid <= kRedBlocks, "partials buffer too small");
 static_assert(2 * kRedGrid <= kPartials / 2, "each half of the partials buffer must hold two values per block");
 static uint32_t red_grid(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n, kBS), kRedGrid)); }
@@ -2600,13 +2653,13 @@ void enqueue_pcg_iteration(mlp_ctx ctx, mlp_mat A, DevPrecBase* B, const double*
     auto fin = [&](int op, const double* part) {
         hipLaunchKernelGGL(k_finalize, one, dim3(kBS), 0, s, op, rg, part, V.st, (double*)nullptr, V.bt);
     };
-    hipLaunchKernelGGL(k_pcg_spmv_dot, grid, dim3(kBS), 0, s, A->A.v, (const double*)V.p, V.Ap,
+    hipLaunchKernelGGL(k_pcg_spmv_dot<false>, grid, dim3(kBS), 0, s, A->A.v, (const double*)V.p, V.Ap,
                        (const SolveState*)V.st, P0, V.bt);
-    hipLaunchKernelGGL(k_pcg_xr, grid, dim3(kBS), 0, s, n, (const double*)V.p, (const double*)V.Ap, x, V.r, V.st,
+    hipLaunchKernelGGL(k_pcg_xr<false>, grid, dim3(kBS), 0, s, n, (const double*)V.p, (const double*)V.Ap, x, V.r, V.st,
                        (const double*)P0, rg, P1, V.bt);
     if (replace) {
         fin(FIN_XR_NOTEST, P1);
-        hipLaunchKernelGGL(k_pcg_true_res, grid, dim3(kBS), 0, s, A->A.v, b, (const double*)x, V.r,
+        hipLaunchKernelGGL(k_pcg_true_res<false>, grid, dim3(kBS), 0, s, A->A.v, b, (const double*)x, V.r,
                            (const SolveState*)V.st, P0, V.bt);
         fin(FIN_TRUE_RES, P0);
     } else {
@@ -2732,6 +2785,133 @@ mlp_ctx_s::GraphEntry* multi_graph(mlp_ctx ctx, mlp_mat A, mlp_prec B, DevPrecBa
     for (int v = 0; v < (tre ? 2 : 1); v++) {
         MLP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         enqueue_pcg_iteration(ctx, A, W, b, x, flex, v == 1, V);
+        MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[v]));
+        MLP_HIP_CHECK(hipGraphInstantiate(&e.e[v], e.g[v], nullptr, nullptr, 0));
+    }
+    return ctx->add_graph(e);
+}
+
+// Workspace of an ensemble solve on the context: the block-diagonal system's n rows, S segments with the
+// offsets seg_off (uploaded); growing it drops the ensemble graphs, which hold its addresses.
+PcgVecs ensure_ens(mlp_ctx ctx, uint32_t n, uint32_t S, const uint32_t* seg_off) {
+    if (ctx->ens_S < S || ctx->ens_n < n) {
+        MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (auto it = ctx->graphs.begin(); it != ctx->graphs.end();) {
+            if (it->solver != 4) { ++it; continue; }
+            mlp_ctx_s::destroy_graph(*it);
+            it = ctx->graphs.erase(it);
+        }
+        ctx->ens_mem.release();
+        if (ctx->est_host) (void)hipHostFree(ctx->est_host);
+        if (ctx->eout_host) (void)hipHostFree(ctx->eout_host);
+        ctx->est_host = nullptr;
+        ctx->eout_host = nullptr;
+        const uint32_t r = std::max(S, ctx->ens_S), m = std::max(n, ctx->ens_n);
+        ctx->er = ctx->ens_mem.alloc<double>(m);
+        ctx->ez = ctx->ens_mem.alloc<double>(m);
+        ctx->ep = ctx->ens_mem.alloc<double>(m);
+        ctx->eAp = ctx->ens_mem.alloc<double>(m);
+        ctx->ezp = ctx->ens_mem.alloc<double>(m);
+        ctx->epart = ctx->ens_mem.alloc<double>(size_t(r) * kPartials);
+        ctx->eout = ctx->ens_mem.alloc<double>(r);
+        ctx->esoff = ctx->ens_mem.alloc<uint32_t>(size_t(r) + 1);
+        ctx->est = ctx->ens_mem.alloc<SolveState>(r);
+        MLP_HIP_CHECK(hipHostMalloc((void**)&ctx->est_host, size_t(r) * sizeof(SolveState), hipHostMallocDefault));
+        MLP_HIP_CHECK(hipHostMalloc((void**)&ctx->eout_host, size_t(r) * sizeof(double), hipHostMallocDefault));
+        ctx->ens_S = r;
+        ctx->ens_n = m;
+    }
+    MLP_HIP_CHECK(hipMemcpyAsync(ctx->esoff, seg_off, (size_t(S) + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    PcgVecs V;
+    V.R = S;
+    V.r = ctx->er; V.z = ctx->ez; V.p = ctx->ep; V.Ap = ctx->eAp; V.zp = ctx->ezp;
+    V.st = ctx->est;
+    V.partials = ctx->epart;
+    V.bt.n = 0;
+    V.bt.part = uint32_t(kPartials);
+    V.bt.soff = ctx->esoff;
+    return V;
+}
+
+// One PCG iteration of every segment of an ensemble (enqueue_pcg_iteration with the segments' own grids); the
+// preconditioner applies once to the whole block-diagonal system.  gmax: the largest segment grid.
+void enqueue_ens_iteration(mlp_ctx ctx, mlp_mat A, DevPrecBase* B, const double* b, double* x, bool flex,
+                           bool replace, const PcgVecs& V, uint32_t gmax) {
+    hipStream_t s = ctx->stream;
+    const dim3 grid(gmax, V.R), one(1, V.R);
+    double* P0 = V.partials;
+    double* P1 = V.partials + kPartials / 2;
+    auto fin = [&](int op, const double* part) {
+        hipLaunchKernelGGL(k_finalize, one, dim3(kBS), 0, s, op, 0u, part, V.st, (double*)nullptr, V.bt);
+    };
+    hipLaunchKernelGGL(k_pcg_spmv_dot<true>, grid, dim3(kBS), 0, s, A->A.v, (const double*)V.p, V.Ap,
+                       (const SolveState*)V.st, P0, V.bt);
+    hipLaunchKernelGGL(k_pcg_xr<true>, grid, dim3(kBS), 0, s, 0u, (const double*)V.p, (const double*)V.Ap, x, V.r,
+                       V.st, (const double*)P0, 0u, P1, V.bt);
+    if (replace) {
+        fin(FIN_XR_NOTEST, P1);
+        hipLaunchKernelGGL(k_pcg_true_res<true>, grid, dim3(kBS), 0, s, A->A.v, b, (const double*)x, V.r,
+                           (const SolveState*)V.st, P0, V.bt);
+        fin(FIN_TRUE_RES, P0);
+    } else {
+        fin(FIN_XR, P1);
+    }
+    B->apply(V.r, V.z, nullptr);
+    if (flex) {
+        hipLaunchKernelGGL((k_pcg_rz<true, true>), grid, dim3(kBS), 0, s, 0u, (const double*)V.r, (const double*)V.z,
+                           (const double*)V.zp, (const SolveState*)V.st, P0, V.bt);
+        hipLaunchKernelGGL((k_pcg_p<true, true>), grid, dim3(kBS), 0, s, 0u, (const double*)V.z, V.p, V.zp, V.st,
+                           (const double*)P0, 0u, V.bt);
+    } else {
+        hipLaunchKernelGGL((k_pcg_rz<false, true>), grid, dim3(kBS), 0, s, 0u, (const double*)V.r,
+                           (const double*)V.z, (const double*)V.zp, (const SolveState*)V.st, P0, V.bt);
+        hipLaunchKernelGGL((k_pcg_p<false, true>), grid, dim3(kBS), 0, s, 0u, (const double*)V.z, V.p, V.zp, V.st,
+                           (const double*)P0, 0u, V.bt);
+    }
+}
+
+// p = z, rz = (r, z) after z = B r for every segment not done
+void enqueue_ens_start(mlp_ctx ctx, DevPrecBase* B, bool flex, const PcgVecs& V, uint32_t gmax) {
+    hipStream_t s = ctx->stream;
+    const dim3 grid(gmax, V.R), one(1, V.R);
+    B->apply(V.r, V.z, nullptr);
+    if (flex)
+        hipLaunchKernelGGL((k_pcg_init<true, true>), grid, dim3(kBS), 0, s, 0u, (const double*)V.r, (const double*)V.z,
+                           V.p, V.zp, (const SolveState*)V.st, V.partials, V.bt);
+    else
+        hipLaunchKernelGGL((k_pcg_init<false, true>), grid, dim3(kBS), 0, s, 0u, (const double*)V.r,
+                           (const double*)V.z, V.p, V.zp, (const SolveState*)V.st, V.partials, V.bt);
+    hipLaunchKernelGGL(k_finalize, one, dim3(kBS), 0, s, int(FIN_INIT), 0u, (const double*)V.partials, V.st,
+                       (double*)nullptr, V.bt);
+}
+
+// (r, r) of every segment r = b - A x (k_dot_batch with the segments' grids) into the partials, then op
+void enqueue_ens_norms(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, const PcgVecs& V, uint32_t gmax,
+                       int op, double* out) {
+    hipStream_t s = ctx->stream;
+    launch_residual(ctx, A, b, x, V.r);
+    hipLaunchKernelGGL(k_dot_batch<true>, dim3(gmax, V.R), dim3(kBS), 0, s, 0u, (const double*)V.r, V.partials, V.bt);
+    hipLaunchKernelGGL(k_finalize, dim3(1, V.R), dim3(kBS), 0, s, op, 0u, (const double*)V.partials, V.st, out, V.bt);
+}
+
+// the cached graphs of one ensemble iteration (plain and residual-replacing)
+mlp_ctx_s::GraphEntry* ens_graph(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, bool flex,
+                                 uint32_t tre, const PcgVecs& V, uint32_t gmax) {
+    for (auto& e : ctx->graphs)
+        if (e.solver == 4 && e.A == A->id && e.B == B->id && e.b == b && e.x == x && e.R == V.R &&
+            e.flex == int(flex) && e.tre == tre && e.ws_n == ctx->ens_n && e.gmax == gmax)
+            return &e;
+    hipStream_t s = ctx->stream;
+    MLP_HIP_CHECK(hipStreamSynchronize(s));
+    mlp_ctx_s::GraphEntry e;
+    e.solver = 4;
+    e.A = A->id; e.B = B->id; e.b = b; e.x = x; e.flex = int(flex); e.tre = tre; e.ws_n = ctx->ens_n; e.R = V.R;
+    e.gmax = gmax;
+    for (int v = 0; v < (tre ? 2 : 1); v++) {
+        MLP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        enqueue_ens_iteration(ctx, A, B->impl.get(), b, x, flex, v == 1, V, gmax);
         MLP_HIP_CHECK(hipStreamEndCapture(s, &e.g[v]));
         MLP_HIP_CHECK(hipGraphInstantiate(&e.e[v], e.g[v], nullptr, nullptr, 0));
     }
@@ -2954,7 +3134,7 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
     MLP_HIP_CHECK(hipMemcpyAsync(V.st, ctx->mst_host, R * sizeof(SolveState), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_residual_batch, dim3(cdiv(n, kBS), R), dim3(kBS), 0, s, A->A.v, b, (const double*)x, V.r,
                        V.bt);
-    hipLaunchKernelGGL(k_dot_batch, dim3(rg, R), dim3(kBS), 0, s, n, (const double*)V.r, V.partials, V.bt);
+    hipLaunchKernelGGL(k_dot_batch<false>, dim3(rg, R), dim3(kBS), 0, s, n, (const double*)V.r, V.partials, V.bt);
     hipLaunchKernelGGL(k_finalize, dim3(1, R), dim3(kBS), 0, s, int(FIN_START), rg, (const double*)V.partials, V.st,
                        (double*)nullptr, V.bt);
     enqueue_pcg_start(ctx, W, n, flex, V);
@@ -3035,6 +3215,172 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
         if (code && first == MLP_OK) {
             first = code;
             msg = "right-hand side " + std::to_string(k) + ": " + what;
+        }
+    }
+    if (first != MLP_OK) {
+        set_last_error(msg.c_str());
+        return first;
+    }
+    MLP_API_END
+}
+
+// An ensemble of independent systems given as one block-diagonal system (segment y = rows
+// [seg_off[y], seg_off[y+1]), no entries coupling two segments), with one preconditioner for the whole of it.
+// Every segment runs PCGSolver's steps (pcg.cc:8-102) on its own rows with its own state, reductions of its
+// own grid and its own stop test, as mlp_pcg on that system alone would; every launch covers all segments,
+// the preconditioner's launches included.  Flexible solves then take mlp_pcg's true-residual restarts, all
+// segments that need one in one batched phase per round.
+int mlp_pcg_ensemble(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nseg, const uint32_t* seg_off, const double* b,
+                     double* x, const mlp_solve_opts* o, mlp_solve_stats* stats) {
+    MLP_API_BEGIN
+    require(ctx && A && B && b && x && o && stats && seg_off, "null argument");
+    require(nseg >= 1 && nseg <= kMaxSegments, "nseg must lie in [1, 256]");
+    require(seg_off[0] == 0 && seg_off[nseg] == A->n, "segments must cover the matrix rows");
+    for (uint32_t y = 0; y < nseg; y++) require(seg_off[y + 1] > seg_off[y], "segments must be non-empty and ascending");
+    require(B->n == A->n, "preconditioner size differs from A");
+    require(B->ctx == ctx && A->ctx->device == ctx->device, "the preconditioner must live on the context");
+    use_device(ctx);
+    const bool flex = o->flexible != 0;
+    const uint32_t tre = o->true_res_every, S = nseg;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = ctx->stream;
+    const PcgVecs V = ensure_ens(ctx, A->n, S, seg_off);
+    uint32_t gmax = 1;
+    for (uint32_t y = 0; y < S; y++) gmax = std::max(gmax, red_grid(seg_off[y + 1] - seg_off[y]));
+    for (uint32_t y = 0; y < S; y++) fill_x0(ctx, seg_off[y + 1] - seg_off[y], o->x0_mode, x + seg_off[y]);
+    for (uint32_t y = 0; y < S; y++) {
+        SolveState& st0 = ctx->est_host[y];
+        std::memset(&st0, 0, sizeof(st0));
+        st0.eps = o->eps;
+        st0.maxit = o->maxit;
+    }
+    MLP_HIP_CHECK(hipMemcpyAsync(V.st, ctx->est_host, S * sizeof(SolveState), hipMemcpyHostToDevice, s));
+    enqueue_ens_norms(ctx, A, b, x, V, gmax, FIN_START, nullptr);
+    enqueue_ens_start(ctx, B->impl.get(), flex, V, gmax);
+    const bool graphs = o->use_graph && B->impl->graph_safe() && graphs_allowed();
+    mlp_ctx_s::GraphEntry* ge = graphs ? ens_graph(ctx, A, B, b, x, flex, tre, V, gmax) : nullptr;
+    const uint32_t chunk = o->check_every ? o->check_every : 16;
+    // iterations until every segment is done
+    auto run = [&]() {
+        uint32_t c = chunk, k = 0;
+        while (true) {
+            for (uint32_t i = 0; i < c; i++) {
+                k++;
+                const bool replace = tre && (k % tre == 0);
+                if (ge) MLP_HIP_CHECK(hipGraphLaunch(ge->e[replace ? 1 : 0], s));
+                else enqueue_ens_iteration(ctx, A, B->impl.get(), b, x, flex, replace, V, gmax);
+            }
+            MLP_HIP_CHECK(hipGetLastError());
+            MLP_HIP_CHECK(hipMemcpyAsync(ctx->est_host, V.st, S * sizeof(SolveState), hipMemcpyDeviceToHost, s));
+            MLP_HIP_CHECK(hipStreamSynchronize(s));
+            bool all = true;
+            uint32_t next = 1;
+            for (uint32_t y = 0; y < S; y++) {
+                const SolveState& st = ctx->est_host[y];
+                if (st.done) continue;
+                all = false;
+                next = std::max(next, next_chunk(st, chunk));
+            }
+            if (all) return;
+            c = next;
+        }
+    };
+    MLP_HIP_CHECK(hipMemcpyAsync(ctx->est_host, V.st, S * sizeof(SolveState), hipMemcpyDeviceToHost, s));
+    MLP_HIP_CHECK(hipStreamSynchronize(s));
+    bool any = false;
+    for (uint32_t y = 0; y < S; y++) any = any || !ctx->est_host[y].done;
+    if (any) run();
+    std::vector<SolveState> fin(ctx->est_host, ctx->est_host + S);
+    // FP64 true relative residuals ||b - A x|| / init_norm of every segment
+    auto true_rel = [&](std::vector<double>& tr) {
+        enqueue_ens_norms(ctx, A, b, x, V, gmax, FIN_DOT, ctx->eout);
+        MLP_HIP_CHECK(hipMemcpyAsync(ctx->eout_host, ctx->eout, S * sizeof(double), hipMemcpyDeviceToHost, s));
+        MLP_HIP_CHECK(hipStreamSynchronize(s));
+        tr.resize(S);
+        for (uint32_t y = 0; y < S; y++) {
+            const double nr = std::sqrt(ctx->eout_host[y]);
+            tr[y] = fin[y].init_norm > 0 ? nr / fin[y].init_norm : nr;
+        }
+    };
+    std::vector<double> tr;
+    if (flex) {
+        // mlp_pcg's restarts from the FP64 true residual (pcg_phase), every segment that needs one at once
+        std::vector<char> stop(S, 0);
+        for (uint32_t y = 0; y < S; y++) stop[y] = fin[y].status != 0;
+        for (uint32_t restart = 0; restart < kFlexRestarts; restart++) {
+            true_rel(tr);
+            bool need = false;
+            for (uint32_t y = 0; y < S; y++) {
+                if (!stop[y] && tr[y] <= fin[y].eps) stop[y] = 1;
+                SolveState& st0 = ctx->est_host[y];              // pcg_phase's starting state
+                std::memset(&st0, 0, sizeof(st0));
+                st0.init_norm = fin[y].init_norm;
+                st0.eps = fin[y].eps;
+                st0.maxit = o->maxit;
+                st0.niter = fin[y].niter;
+                st0.done = stop[y] ? 1u : 0u;
+                need = need || !stop[y];
+            }
+            if (!need) break;
+            MLP_HIP_CHECK(hipMemcpyAsync(V.st, ctx->est_host, S * sizeof(SolveState), hipMemcpyHostToDevice, s));
+            enqueue_ens_norms(ctx, A, b, x, V, gmax, FIN_RESTART, nullptr);
+            // a segment whose restarted residual already meets eps stops there (no iteration, no more restarts)
+            MLP_HIP_CHECK(hipMemcpyAsync(ctx->est_host, V.st, S * sizeof(SolveState), hipMemcpyDeviceToHost, s));
+            MLP_HIP_CHECK(hipStreamSynchronize(s));
+            bool go = false;
+            for (uint32_t y = 0; y < S; y++) {
+                if (stop[y]) continue;
+                if (ctx->est_host[y].done) {
+                    fin[y] = ctx->est_host[y];
+                    fin[y].done = 1;
+                    stop[y] = 1;
+                } else {
+                    go = true;
+                }
+            }
+            if (!go) break;
+            enqueue_ens_start(ctx, B->impl.get(), flex, V, gmax);
+            run();
+            for (uint32_t y = 0; y < S; y++)
+                if (!stop[y]) {
+                    fin[y] = ctx->est_host[y];
+                    if (fin[y].status != 0) stop[y] = 1;
+                }
+        }
+    }
+    true_rel(tr);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    int first = MLP_OK;
+    std::string msg;
+    for (uint32_t y = 0; y < S; y++) {
+        const SolveState& st = fin[y];
+        mlp_solve_stats& St = stats[y];
+        std::memset(&St, 0, sizeof(St));
+        St.niter = st.niter;
+        St.init_norm = st.init_norm;
+        St.final_norm = st.norm;
+        St.eps_used = st.eps;
+        St.t_solve_ms = ms;
+        St.true_rel_res = tr[y];
+        int code = MLP_OK;
+        const char* what = "";
+        if (st.status == 4) {
+            code = MLP_ERR_NO_ITER;
+            what = "No need to iterate";
+        } else if (st.status == 3) {
+            code = MLP_ERR_NOT_SPD;
+            what = "<Ap, p> <= 0";
+        } else if (st.status == 5) {
+            code = MLP_ERR_MAXIT;
+            what = "maximum number of iterations reached";
+        } else if (flex && St.true_rel_res > st.eps) {
+            code = MLP_ERR_NO_CONV;
+            what = "flexible PCG: the FP64 true residual is above eps after the restarts";
+        }
+        St.status = code;
+        if (code && first == MLP_OK) {
+            first = code;
+            msg = "segment " + std::to_string(y) + ": " + what;
         }
     }
     if (first != MLP_OK) {
@@ -3370,7 +3716,7 @@ int mlp_cheb_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, double lmin, double lmax
             B->impl->apply(ctx->r, X, &ctx->st->done);
             hipLaunchKernelGGL(k_cheb_outer, dim3(rg), dim3(kBS), 0, s, n, X, (const double*)U1, (const double*)U0,
                                (const double*)ctx->coef, (const SolveState*)ctx->st);
-            hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)X, ctx->r,
+            hipLaunchKernelGGL(k_pcg_true_res<false>, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)X, ctx->r,
                                (const SolveState*)ctx->st, ctx->partials, Batch{});
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, int(FIN_STEP), rg, (const double*)ctx->partials,
                                ctx->st, (double*)nullptr, Batch{});
@@ -3427,7 +3773,7 @@ int mlp_simple_solver(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, doubl
         auto iteration = [&](int) {
             B->impl->apply(ctx->r, ctx->z, &ctx->st->done);
             hipLaunchKernelGGL(k_add, dim3(rg), dim3(kBS), 0, s, n, (const double*)ctx->z, x, (const SolveState*)ctx->st);
-            hipLaunchKernelGGL(k_pcg_true_res, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)x, ctx->r,
+            hipLaunchKernelGGL(k_pcg_true_res<false>, dim3(rg), dim3(kBS), 0, s, A->A.v, b, (const double*)x, ctx->r,
                                (const SolveState*)ctx->st, ctx->partials, Batch{});
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, int(FIN_STEP), rg, (const double*)ctx->partials,
                                ctx->st, (double*)nullptr, Batch{});

@@ -31,8 +31,13 @@ struct SpeStencilOut {
 
 // spe_assemble (assembly.h) on the device: kx and the outputs ia (n+1), ja, a
 // (spe_nnz) in device memory; spe = 1 for the reference's assembly, 0 for TPFA.
+// With zfirst/zlast (device arrays of nz entries) the planes form a stack of
+// independent systems: plane k's system spans planes zfirst[k] .. zlast[k],
+// and the matrix has no z-link between systems (block diagonal, each block the
+// matrix of that system alone, bit for bit).
 void spe_assemble_device(void* stream, int spe, uint32_t nx, uint32_t ny, uint32_t nz, const double* kx,
                          double kz_ratio, double c, double hx, double hy, double hz, uint32_t* ia, uint32_t* ja,
-                         double* a, const SpeStencilOut* st = nullptr);
+                         double* a, const SpeStencilOut* st = nullptr, const uint32_t* zfirst = nullptr,
+                         const uint32_t* zlast = nullptr);
 
 }  // namespace mlp

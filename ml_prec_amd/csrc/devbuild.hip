@@ -253,6 +253,12 @@ struct SpeRows {
     uint32_t nx, ny, nz;
     const double* kx;
     double kz_ratio, c, hx, hy, hz;
+    // an ensemble stacked along z: plane k belongs to the system of planes zfirst[k] .. zlast[k], and no
+    // z-link crosses from one system into the next (null: one system of nz planes)
+    const uint32_t* zfirst;
+    const uint32_t* zlast;
+    __device__ bool zlow(uint32_t k) const { return zfirst ? k > zfirst[k] : k > 0; }
+    __device__ bool zhigh(uint32_t k) const { return zlast ? k < zlast[k] : k + 1 < nz; }
     __device__ double face(double ka, double kb, int axis) const {
         const double m = 2 / (1 / ka + 1 / kb);
         if (spe) return axis == 0 ? m / (hx * hx) : axis == 1 ? m / (hy * hy) : m / (hz * hz);
@@ -265,7 +271,7 @@ struct SpeCount {
     uint32_t* cnt;
     __device__ void operator()(uint32_t r) const {
         const uint32_t i = r % R.nx, j = (r / R.nx) % R.ny, k = r / (R.nx * R.ny);
-        cnt[r] = 1 + (k > 0) + (j > 0) + (i > 0) + (i + 1 < R.nx) + (j + 1 < R.ny) + (k + 1 < R.nz);
+        cnt[r] = 1 + R.zlow(k) + (j > 0) + (i > 0) + (i + 1 < R.nx) + (j + 1 < R.ny) + R.zhigh(k);
     }
 };
 
@@ -306,12 +312,12 @@ struct SpeFill {
             diag = diag + v;
         };
         const double kzr = R.kz_ratio * kx[r];
-        if (k > 0) { const double v = R.face(kzr, R.kz_ratio * kx[r - sz], 2); add(r - sz, v); low[0] = -v; m |= 1; }
+        if (R.zlow(k)) { const double v = R.face(kzr, R.kz_ratio * kx[r - sz], 2); add(r - sz, v); low[0] = -v; m |= 1; }
         if (j > 0) { const double v = R.face(kx[r], kx[r - sy], 1); add(r - sy, v); low[1] = -v; m |= 2; }
         if (i > 0) { const double v = R.face(kx[r], kx[r - 1], 0); add(r - 1, v); low[2] = -v; m |= 4; }
         if (i + 1 < nx) { add(r + 1, R.face(kx[r], kx[r + 1], 0)); m |= 8; }
         if (j + 1 < ny) { add(r + sy, R.face(kx[r], kx[r + sy], 1)); m |= 16; }
-        if (k + 1 < nz) { add(r + sz, R.face(kzr, R.kz_ratio * kx[r + sz], 2)); m |= 32; }
+        if (R.zhigh(k)) { add(r + sz, R.face(kzr, R.kz_ratio * kx[r + sz], 2)); m |= 32; }
         a[dpos] = diag;
         if (pres) {
             pres[r] = uint8_t(m);
@@ -331,11 +337,11 @@ std::unique_ptr<ChebLevelBuilder> device_cheb_builder(void* stream, const DevCSR
 
 void spe_assemble_device(void* stream, int spe, uint32_t nx, uint32_t ny, uint32_t nz, const double* kx,
                          double kz_ratio, double c, double hx, double hy, double hz, uint32_t* ia, uint32_t* ja,
-                         double* a, const SpeStencilOut* st) {
+                         double* a, const SpeStencilOut* st, const uint32_t* zfirst, const uint32_t* zlast) {
     const uint64_t n = uint64_t(nx) * ny * nz;
     if (!n) throw Error("grid dimensions must be positive", 1);
     DevExec x(static_cast<hipStream_t>(stream));
-    const SpeRows R{spe, nx, ny, nz, kx, kz_ratio, c, hx, hy, hz};
+    const SpeRows R{spe, nx, ny, nz, kx, kz_ratio, c, hx, hy, hz, zfirst, zlast};
     uint32_t* cnt = x.alloc<uint32_t>(n);
     x.each(uint32_t(n), SpeCount{R, cnt});
     x.scan(cnt, ia, uint32_t(n));

@@ -19,6 +19,7 @@ constexpr int kBS = 256;                 // threads per block for flat kernels
 constexpr int kRedBlocks = 16384;        // max blocks of a one- or two-value reduction grid
 constexpr int kMdotBlocks = 1024;        // max blocks of the Gram-Schmidt multi-dot (32 partials each)
 constexpr size_t kPartials = 32768;      // doubles of partial-sum scratch (2 * kRedBlocks = 32 * kMdotBlocks)
+constexpr uint32_t kRedGrid = 1024;      // blocks of a streaming reduction over n rows: clamp(ceil(n / kBS), 1, kRedGrid)
 
 // Sliced ELL (64-row slices, column-major inside a slice).  Column indices
 // are either uint32 (col) or, when every |column - row| fits, int16 deltas
@@ -57,6 +58,9 @@ struct SolveState {
 struct Batch {
     uint32_t n = 0;        // the length-n solve vectors
     uint32_t part = 0;     // partial-sum buffers
+    // ragged batches (an ensemble of systems in one block-diagonal matrix): solve y owns rows
+    // [soff[y], soff[y+1]) of one vector; null for batches of equal-length vectors
+    const uint32_t* soff = nullptr;
 };
 struct LevelBatch {
     uint32_t N = 0;        // level vectors: the input f, the output x, the parent's u, F and us
@@ -134,6 +138,26 @@ __device__ __forceinline__ bool block_done(const SolveState* st) {
     const bool r = d != 0;
     __syncthreads();
     return r;
+}
+
+// Solve y of a batch: its first row, its length and the reduction grid that a
+// solve of that length alone launches.  Batches of equal-length vectors: solve
+// y at y * n with the launch's own grid.  Ragged batches (SEG): segment
+// [soff[y], soff[y+1]) with grid clamp(ceil(len / kBS), 1, kRedGrid), so that
+// a segment's rows go to the same threads and blocks, and its partial sums are
+// added in the same order, as in a solve of that system alone.
+struct Span {
+    size_t base;
+    uint32_t len, rg;
+};
+template <bool SEG> __device__ __forceinline__ Span span_of(const Batch& bt, uint32_t y) {
+    if constexpr (SEG) {
+        const uint32_t b = bt.soff[y], len = bt.soff[y + 1] - b;
+        const uint32_t t = (len + kBS - 1) / kBS;
+        return {b, len, t < 1 ? 1u : (t > kRedGrid ? kRedGrid : t)};
+    } else {
+        return {size_t(y) * bt.n, bt.n, gridDim.x};
+    }
 }
 
 // sum_partials, returned to every thread of the block
@@ -1530,12 +1554,17 @@ __global__ void __launch_bounds__(kBS) k_diag_apply(uint32_t n, const double* __
 // The streaming kernels store per-block partial sums; k_finalize adds them
 // and updates the scalar recurrences of the device-resident SolveState.
 // ---------------------------------------------------------------------------
-enum { FIN_DOT = 0, FIN_XR = 2, FIN_XR_NOTEST = 3, FIN_INIT = 6, FIN_TRUE_RES = 7, FIN_STEP = 8, FIN_START = 9 };
+enum { FIN_DOT = 0, FIN_XR = 2, FIN_XR_NOTEST = 3, FIN_INIT = 6, FIN_TRUE_RES = 7, FIN_STEP = 8, FIN_START = 9,
+       FIN_RESTART = 10 };
 
 __global__ void __launch_bounds__(kBS) k_finalize(int op, uint32_t nb, const double* __restrict__ partials,
                                                   SolveState* st, double* out, Batch bt) {
     partials += size_t(blockIdx.y) * bt.part;
     if (st) st += blockIdx.y;
+    if (bt.soff) {                                            // a ragged batch: this segment's own grid
+        const uint32_t len = bt.soff[blockIdx.y + 1] - bt.soff[blockIdx.y], t = (len + kBS - 1) / kBS;
+        nb = t < 1 ? 1u : (t > kRedGrid ? kRedGrid : t);
+    }
     if (op != FIN_DOT && op != FIN_INIT && op != FIN_START && st->done) return;
     __shared__ double sh[kBS / 64];
     const int NV = 1;
@@ -1543,7 +1572,12 @@ __global__ void __launch_bounds__(kBS) k_finalize(int op, uint32_t nb, const dou
     const double t1 = NV == 2 ? sum_partials<kBS>(partials, nb, NV, 1, sh) : 0.0;
     if (threadIdx.x != 0) return;
     switch (op) {
-        case FIN_DOT: *out = t0; break;
+        case FIN_DOT: out[blockIdx.y] = t0; break;
+        case FIN_RESTART:                                    // a flexible restart's first norm (pcg_phase)
+            st->rr = t0;
+            st->norm = sqrt(t0);
+            if (!(st->norm / st->init_norm > st->eps)) st->done = 1;
+            break;
         case FIN_INIT: st->rzs[st->niter & 1] = t0; break;
         case FIN_XR:                                         // norm, niter, stop test   pcg.cc:42, 61, 75
         case FIN_XR_NOTEST:
@@ -1598,12 +1632,16 @@ __global__ void __launch_bounds__(kBS) k_residual_batch(SellView<double> A, cons
     if (r >= A.n) return;
     rr[r] = b[r] - mat_row<double, double>(A, x, r);
 }
+template <bool SEG = false>
 __global__ void __launch_bounds__(kBS) k_dot_batch(uint32_t n, const double* __restrict__ x, double* partials,
                                                    Batch bt) {
     const size_t y = blockIdx.y;
-    x += y * bt.n; partials += y * bt.part;
+    const Span sp = span_of<SEG>(bt, uint32_t(y));
+    if (SEG && blockIdx.x >= sp.rg) return;
+    x += sp.base; partials += y * bt.part;
+    if (SEG) n = sp.len;
     double s = 0.0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += x[i] * x[i];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += sp.rg * blockDim.x) s += x[i] * x[i];
     double part[1] = {s};
     block_partials<kBS, 1>(part, partials);
 }
@@ -1624,22 +1662,28 @@ __global__ void __launch_bounds__(kBS) k_copy(uint32_t n, const double* __restri
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = x[i];
 }
 
-// Ap = A p ; partials of (Ap, p)
+// Ap = A p ; partials of (Ap, p).  A ragged batch multiplies its segment's
+// rows of the block-diagonal matrix (global rows and columns).
+template <bool SEG = false>
 __global__ void __launch_bounds__(kBS) k_pcg_spmv_dot(SellView<double> A, const double* __restrict__ p,
                                                       double* __restrict__ Ap, const SolveState* st, double* partials,
                                                       Batch bt) {
     const uint32_t y = blockIdx.y;
-    p += size_t(y) * bt.n; Ap += size_t(y) * bt.n; st += y; partials += size_t(y) * bt.part;
+    const Span sp = span_of<SEG>(bt, y);
+    if (SEG && blockIdx.x >= sp.rg) return;
+    if (!SEG) { p += sp.base; Ap += sp.base; }
+    st += y; partials += size_t(y) * bt.part;
     if (st->done) return;
     double s = 0.0;
     // grid-stride over 256-row tiles, XCD-grouped within each sweep of the grid
-    const uint32_t tiles = (A.n + blockDim.x - 1) / blockDim.x;
-    for (uint32_t t = xcd_block(blockIdx.x, gridDim.x); t < tiles; t += gridDim.x) {
+    const uint32_t len = SEG ? sp.len : A.n, r0 = SEG ? uint32_t(sp.base) : 0u;
+    const uint32_t tiles = (len + blockDim.x - 1) / blockDim.x;
+    for (uint32_t t = xcd_block(blockIdx.x, sp.rg); t < tiles; t += sp.rg) {
         const uint32_t r = t * blockDim.x + threadIdx.x;
-        if (r < A.n) {
-            const double v = mat_row<double, double>(A, p, r);
-            Ap[r] = v;
-            s += v * p[r];
+        if (r < len) {
+            const double v = mat_row<double, double>(A, p, r0 + r);
+            Ap[r0 + r] = v;
+            s += v * p[r0 + r];
         }
     }
     double part[1] = {s};
@@ -1650,12 +1694,16 @@ __global__ void __launch_bounds__(kBS) k_pcg_spmv_dot(SellView<double> A, const 
 // partials of k_pcg_spmv_dot in the same fixed order; then x += alpha p,
 // r -= alpha Ap and partials of (r, r) (pcg.cc:58-61).  Block 0 records alpha
 // (and "<Ap, p> <= 0", which stops the solve) in the state.
+template <bool SEG = false>
 __global__ void __launch_bounds__(kBS) k_pcg_xr(uint32_t n, const double* __restrict__ p, const double* __restrict__ Ap,
                                                 double* __restrict__ x, double* __restrict__ r, SolveState* st,
                                                 const double* __restrict__ app_partials, uint32_t nb,
                                                 double* partials, Batch bt) {
     const uint32_t y = blockIdx.y;
-    p += size_t(y) * bt.n; Ap += size_t(y) * bt.n; x += size_t(y) * bt.n; r += size_t(y) * bt.n; st += y;
+    const Span sp = span_of<SEG>(bt, y);
+    if (SEG && blockIdx.x >= sp.rg) return;
+    if (SEG) { n = sp.len; nb = sp.rg; }
+    p += sp.base; Ap += sp.base; x += sp.base; r += sp.base; st += y;
     app_partials += size_t(y) * bt.part; partials += size_t(y) * bt.part;
     if (block_done(st)) return;
     __shared__ double sh[kBS / 64];
@@ -1678,7 +1726,7 @@ __global__ void __launch_bounds__(kBS) k_pcg_xr(uint32_t n, const double* __rest
         st->malpha_one = mone;
     }
     double s = 0.0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += sp.rg * blockDim.x) {
         x[i] = one ? x[i] + p[i] : x[i] + a * p[i];
         const double ri = mone ? r[i] + Ap[i] : r[i] + ma * Ap[i];
         r[i] = ri;
@@ -1689,15 +1737,18 @@ __global__ void __launch_bounds__(kBS) k_pcg_xr(uint32_t n, const double* __rest
 }
 
 // partials of (r, z) and, flexible, (r, z - z_prev)   (pcg.cc:68)
-template <bool FLEX>
+template <bool FLEX, bool SEG = false>
 __global__ void __launch_bounds__(kBS) k_pcg_rz(uint32_t n, const double* __restrict__ r, const double* __restrict__ z,
                                                 const double* __restrict__ zp, const SolveState* st, double* partials,
                                                 Batch bt) {
     const uint32_t y = blockIdx.y;
-    r += size_t(y) * bt.n; z += size_t(y) * bt.n; zp += size_t(y) * bt.n; st += y; partials += size_t(y) * bt.part;
+    const Span sp = span_of<SEG>(bt, y);
+    if (SEG && blockIdx.x >= sp.rg) return;
+    if (SEG) n = sp.len;
+    r += sp.base; z += sp.base; zp += sp.base; st += y; partials += size_t(y) * bt.part;
     if (st->done) return;
     double s0 = 0.0, s1 = 0.0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += sp.rg * blockDim.x) {
         s0 += r[i] * z[i];
         if (FLEX) s1 += r[i] * (z[i] - zp[i]);
     }
@@ -1715,12 +1766,15 @@ __global__ void __launch_bounds__(kBS) k_pcg_rz(uint32_t n, const double* __rest
 // daxpy(1, z, p), pcg.cc:71-72); flexible keeps z for the next beta.  The new
 // rz goes to the state slot of this iteration's parity, so no block reads a
 // slot another block writes.
-template <bool FLEX>
+template <bool FLEX, bool SEG = false>
 __global__ void __launch_bounds__(kBS) k_pcg_p(uint32_t n, const double* __restrict__ z, double* __restrict__ p,
                                                double* __restrict__ zp, SolveState* st,
                                                const double* __restrict__ rz_partials, uint32_t nb, Batch bt) {
     const uint32_t y = blockIdx.y;
-    z += size_t(y) * bt.n; p += size_t(y) * bt.n; zp += size_t(y) * bt.n; st += y;
+    const Span sp = span_of<SEG>(bt, y);
+    if (SEG && blockIdx.x >= sp.rg) return;
+    if (SEG) { n = sp.len; nb = sp.rg; }
+    z += sp.base; p += sp.base; zp += sp.base; st += y;
     rz_partials += size_t(y) * bt.part;
     if (block_done(st)) return;
     __shared__ double sh[kBS / 64];
@@ -1734,7 +1788,7 @@ __global__ void __launch_bounds__(kBS) k_pcg_p(uint32_t n, const double* __restr
         st->beta = b;
         st->beta_one = one;
     }
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += sp.rg * blockDim.x) {
         const double pi = one ? p[i] : p[i] * b;
         p[i] = pi + z[i];
         if (FLEX) zp[i] = z[i];
@@ -1742,16 +1796,19 @@ __global__ void __launch_bounds__(kBS) k_pcg_p(uint32_t n, const double* __restr
 }
 
 // p = z ; partials of (r, z)   (pcg.cc:32-33)
-template <bool FLEX>
+template <bool FLEX, bool SEG = false>
 __global__ void __launch_bounds__(kBS) k_pcg_init(uint32_t n, const double* __restrict__ r, const double* __restrict__ z,
                                                   double* __restrict__ p, double* __restrict__ zp, const SolveState* st,
                                                   double* partials, Batch bt) {
     const uint32_t y = blockIdx.y;
-    r += size_t(y) * bt.n; z += size_t(y) * bt.n; p += size_t(y) * bt.n; zp += size_t(y) * bt.n; st += y;
+    const Span sp = span_of<SEG>(bt, y);
+    if (SEG && blockIdx.x >= sp.rg) return;
+    if (SEG) n = sp.len;
+    r += sp.base; z += sp.base; p += sp.base; zp += sp.base; st += y;
     partials += size_t(y) * bt.part;
     if (st->done) return;
     double s = 0.0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += sp.rg * blockDim.x) {
         p[i] = z[i];
         if (FLEX) zp[i] = z[i];
         s += r[i] * z[i];
@@ -1761,19 +1818,24 @@ __global__ void __launch_bounds__(kBS) k_pcg_init(uint32_t n, const double* __re
 }
 
 // true residual replacement: r = b - A x ; partials of (r, r)
+template <bool SEG = false>
 __global__ void __launch_bounds__(kBS) k_pcg_true_res(SellView<double> A, const double* __restrict__ b,
                                                       const double* __restrict__ x, double* __restrict__ r,
                                                       const SolveState* st, double* partials, Batch bt) {
     const uint32_t y = blockIdx.y;
-    b += size_t(y) * bt.n; x += size_t(y) * bt.n; r += size_t(y) * bt.n; st += y; partials += size_t(y) * bt.part;
+    const Span sp = span_of<SEG>(bt, y);
+    if (SEG && blockIdx.x >= sp.rg) return;
+    if (!SEG) { b += sp.base; x += sp.base; r += sp.base; }
+    st += y; partials += size_t(y) * bt.part;
     if (st->done) return;
     double s = 0.0;
-    const uint32_t tiles = (A.n + blockDim.x - 1) / blockDim.x;
-    for (uint32_t t = xcd_block(blockIdx.x, gridDim.x); t < tiles; t += gridDim.x) {
+    const uint32_t len = SEG ? sp.len : A.n, r0 = SEG ? uint32_t(sp.base) : 0u;
+    const uint32_t tiles = (len + blockDim.x - 1) / blockDim.x;
+    for (uint32_t t = xcd_block(blockIdx.x, sp.rg); t < tiles; t += sp.rg) {
         const uint32_t i = t * blockDim.x + threadIdx.x;
-        if (i < A.n) {
-            const double v = b[i] - mat_row<double, double>(A, x, i);
-            r[i] = v;
+        if (i < len) {
+            const double v = b[r0 + i] - mat_row<double, double>(A, x, r0 + i);
+            r[r0 + i] = v;
             s += v * v;
         }
     }

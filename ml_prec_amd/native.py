@@ -132,6 +132,11 @@ EXPORTS = {
     "mlp_spe_matrix_dev": (C.c_int, [_vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, _vp, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]),
     "mlp_mat_get_csr": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mlp_spe_ensemble_matrix_dev": (C.c_int, [_vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, _u32p, _vp,
+                                              C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                              C.POINTER(_vp)]),
+    "mlp_pcg_ensemble": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _u32p, _vp, _vp, C.POINTER(SolveOpts),
+                                   C.POINTER(SolveStats)]),
     "mlp_hier_build_uh_cheb_mat": (C.c_int, [_vp, _vp, _f64p, C.c_uint32, _u32p, C.c_uint32, C.c_int,
                                             C.POINTER(_vp)]),
     "mlp_spe_assemble": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, _vp, C.c_double, C.c_double,
@@ -334,6 +339,25 @@ class DevMatrix:
         check(lib().mlp_spe_matrix_dev(ctx.h, k, nx, ny, nz, kx.ptr, kz_ratio, c, h[0], h[1], h[2], C.byref(m.h)))
         return m
 
+    @classmethod
+    def spe_ensemble_on_device(cls, ctx, kx: "DevArray", nx, ny, nzs, kind="spe", c=0.0, kz_ratio=0.1,
+                               h=(20.0, 10.0, 2.0)):
+        """Systems of nx*ny*nz cells (nz from nzs) stacked along z into one block-diagonal matrix assembled on
+        the device (mlp_spe_ensemble_matrix_dev); kx holds their fields one after another."""
+        nzs = np.ascontiguousarray(nzs, dtype=np.uint32)
+        if kx.n != nx * ny * int(nzs.sum()):
+            raise ValueError("kx has the wrong size")
+        k = {"spe": 0, "tpfa": 1}.get(kind)
+        if k is None:
+            raise ValueError(f"unknown kind {kind!r}")
+        m = cls.__new__(cls)
+        m.ctx = ctx
+        m.n = kx.n
+        m.h = _vp()
+        check(lib().mlp_spe_ensemble_matrix_dev(ctx.h, k, nx, ny, len(nzs), nzs, kx.ptr, kz_ratio, c, h[0], h[1],
+                                                h[2], C.byref(m.h)))
+        return m
+
     def csr(self):
         """The matrix's Skyline CSR as host arrays (ia, ja, a)."""
         n, nnz, _ = self.info()
@@ -511,6 +535,18 @@ def pcg_multi(ctx, A, B, nrhs, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_m
     arr = (SolveStats * nrhs)()
     rc = lib().mlp_pcg_multi(ctx.h, A.h, B.h, nrhs, b.ptr, x.ptr, C.byref(o), arr)
     return rc, [st.as_dict() for st in arr]
+
+
+def pcg_ensemble(ctx, A, B, seg_off, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO, flexible=False,
+                 true_res_every=0, check_every=16, use_graph=True):
+    """mlp_pcg_ensemble: every segment [seg_off[y], seg_off[y+1]) of the block-diagonal system solved as its own
+    PCG, all in the same launches.  Returns (rc, [stats per segment])."""
+    off = np.ascontiguousarray(seg_off, dtype=np.uint32)
+    S = len(off) - 1
+    st = (SolveStats * S)()
+    o = _opts(eps, maxit, x0_mode, flexible, true_res_every, check_every, use_graph)
+    rc = lib().mlp_pcg_ensemble(ctx.h, A.h, B.h, S, off, b.ptr, x.ptr, C.byref(o), st)
+    return rc, [s.as_dict() for s in st]
 
 
 def gmres(ctx, A, B, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO, raise_on_error=True):
