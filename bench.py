@@ -158,8 +158,10 @@ def ensemble(N, S, args, emit=True):
         H = N.build_uh_cheb_on_device(ctx, A)
         assert H.nlevels == depth
         off = np.concatenate([[0], np.cumsum([members[i]["n"] for i in idx])]).astype(np.uint32)
-        stacks.append(dict(ctx=ctx, A=A, B64=N.DevPrec(ctx, A, H, precision=N.MLP_FP64),
-                           B32=N.DevPrec(ctx, A, H, precision=N.MLP_FP32), idx=idx, off=off, depth=depth,
+        # each group's launch schedule picked by the autotuner (candidate schedules timed on the group)
+        B64 = N.DevPrec.tuned(ctx, A, H, precision=N.MLP_FP64, reps=5)
+        B32 = N.DevPrec.tuned(ctx, A, H, precision=N.MLP_FP32, reps=5)
+        stacks.append(dict(ctx=ctx, A=A, B64=B64, B32=B32, idx=idx, off=off, depth=depth,
                            b=ctx.to_device(np.concatenate([rhs[i] for i in idx])), x=ctx.array(int(off[-1]))))
     log(f"ensemble: {len(members)} systems, {cells} cells, {len(stacks)} depth groups "
         f"{[(g['depth'], len(g['idx'])) for g in stacks]}, set up in {time.perf_counter() - t0:.1f}s")
@@ -242,8 +244,8 @@ def ensemble(N, S, args, emit=True):
                                "system solved by mlp_pcg_ensemble (every launch covers the group), groups on "
                                "concurrent streams",
                    "nz": [nz for nz, _ in spec],
-                   "groups": [{"depth": g["depth"], "systems": len(g["idx"]), "rows": int(g["off"][-1])}
-                              for g in stacks]},
+                   "groups": [{"depth": g["depth"], "systems": len(g["idx"]), "rows": int(g["off"][-1]),
+                               "tuned_fp32": g["B32"].tune, "tuned_fp64": g["B64"].tune} for g in stacks]},
         "ms_stacked_fp64": round(ms64, 3),
         "ms_streams_fp64": round(ms_batch, 3),
         "ms_one_after_another": round(ms_seq, 3),

@@ -175,6 +175,22 @@ typedef struct {
 int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_desc* levels,
                     mlp_mat A0, int precision, mlp_prec* out);
 int mlp_prec_from_hier(mlp_ctx ctx, mlp_hier h, mlp_mat A0, int precision, mlp_prec* out);
+/* Autotuned launch schedule (no counterpart in the reference, which has one CPU schedule): the hierarchy's
+ * device programs built with each candidate schedule -- the default (uh_cheb: the level chain where every
+ * merged component fits a wavefront), level kernels with and without the fused last pair, every level
+ * program in 64- or 256-thread blocks -- each timed over `reps` applies on ctx's stream (HIP events, after
+ * one untimed apply), the fastest kept.  FP64 applies are bitwise the same under every schedule.  report
+ * (optional) receives each distinct candidate's schedule and time. */
+#define MLP_SCHED_CHAIN 1u    /* the level chain runs the last levels */
+#define MLP_SCHED_PAIR  2u    /* the last two levels run fused */
+typedef struct {
+    uint32_t ncand;           /* distinct candidates timed (at most 8 reported) */
+    uint32_t chosen;          /* index of the one kept */
+    uint32_t sched[8];        /* MLP_SCHED_* bits | forced level block size << 8 (0: chosen per level) */
+    double us[8];             /* microseconds per apply */
+} mlp_tune_report;
+int mlp_prec_from_hier_tuned(mlp_ctx ctx, mlp_hier h, mlp_mat A0, int precision, uint32_t reps,
+                             mlp_tune_report* report, mlp_prec* out);
 int mlp_prec_simple(mlp_ctx ctx, int kind, mlp_mat A0, mlp_prec* out);   /* DIAG or ID */
 /* GSPrec(const SkylineMatrix& A)  gs_prec.cc:26-65: one forward Gauss-Seidel sweep (lower triangle
  * with the diagonal; a non-positive diagonal is an error as in the reference). */

@@ -274,6 +274,8 @@ struct DevPrecBase {
     // a copy that applies to rhs right-hand sides per call, vectors stored one
     // after another (nullptr: not supported)
     virtual std::unique_ptr<DevPrecBase> widen(mlp_ctx, uint32_t) const { return nullptr; }
+    // the launch schedule in use (MLP_SCHED_* bits, level block size << 8)
+    virtual uint32_t schedule() const { return 0; }
 };
 
 constexpr uint32_t kLdsBudget = 60 * 1024;
@@ -414,6 +416,8 @@ template <class VT> struct DevHier : DevPrecBase {
     // chain of the levels chain_a..nlevels-1 (plan.h: ChainProgram); -1: none
     int chain_a = -1;
     bool no_chain = false;                   // build without the chain (the batched copy, see widen)
+    bool no_pair = false;                    // keep the last two levels separate (a tuner candidate)
+    uint32_t sched_lbs = 0;                  // level blocks of this many threads, every level (a tuner candidate)
     // a chain hierarchy's level arrays, kept on the host for a batched copy built without the chain
     struct LevelCopy {
         std::vector<uint32_t> map, L_ia, L_ja, U_ia, U_ja, A_ia, A_ja;
@@ -632,6 +636,7 @@ template <class VT> struct DevHier : DevPrecBase {
         nlevels = uint32_t(d.size());
         lv.resize(nlevels);
         read_launch_shape();
+        if (sched_lbs) env_lbs = env_cap = sched_lbs;
         hipStream_t s = stream();
         // plan every level first: uh_cheb links each level's F to the next
         // level's consumption order
@@ -684,7 +689,7 @@ template <class VT> struct DevHier : DevPrecBase {
         // the pair does not fit.
         const char* nopair = getenv("MLP_NO_PAIR");
         const bool try_pair = kind == MLP_PREC_UH_CHEB && chain_a < 0 && nlevels >= 2 && d[nlevels - 2].nit == 2 &&
-                              !(nopair && nopair[0] == '1') && !(bf16 && nlevels == 2);
+                              !(nopair && nopair[0] == '1') && !no_pair && !(bf16 && nlevels == 2);
         PairProgram PP;
         uint32_t pbs = 0;
         auto arrays = [&](uint32_t l) {
@@ -1331,6 +1336,10 @@ template <class VT> struct DevHier : DevPrecBase {
         return tot;
     }
     // kernels the last apply launched (0 before the first apply)
+    uint32_t schedule() const override {
+        uint32_t b = (chain_a >= 0 ? uint32_t(MLP_SCHED_CHAIN) : 0u) | (pair_l >= 0 ? uint32_t(MLP_SCHED_PAIR) : 0u);
+        return b | (sched_lbs << 8);
+    }
     uint32_t launches_per_apply() const override { return last_launches; }
 };
 
@@ -2196,12 +2205,21 @@ int mlp_hier_level_array(mlp_hier h, uint32_t l, int which, void* out) {
 
 // ---- device preconditioners ----
 }  // extern "C"
+// A launch schedule to build with (the autotuner's candidates); the default picks per hierarchy.
+struct Schedule {
+    bool no_chain = false, no_pair = false;
+    uint32_t lbs = 0;
+};
+
 template <class VT>
 static DevPrecBase* make_hier(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_desc* levels, mlp_mat A0,
-                              bool bf16 = false) {
+                              bool bf16 = false, const Schedule& sc = Schedule()) {
     auto* h = new DevHier<VT>();
     h->ctx = ctx;
     h->bf16 = bf16;
+    h->no_chain = sc.no_chain;
+    h->no_pair = sc.no_pair;
+    h->sched_lbs = sc.lbs;
     try {
         std::vector<mlp_level_desc> d(levels, levels + nlevels);
         h->build(d, kind, A0);
@@ -2213,8 +2231,8 @@ static DevPrecBase* make_hier(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp
 }
 extern "C" {
 
-int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_desc* levels, mlp_mat A0, int precision,
-                    mlp_prec* out) {
+static int prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_desc* levels, mlp_mat A0,
+                       int precision, mlp_prec* out, const Schedule& sc) {
     MLP_API_BEGIN
     require(ctx && out && levels && nlevels && A0, "null argument");
     require(kind == MLP_PREC_UH_CHEB || kind == MLP_PREC_MULTI_SPLIT, "unknown hierarchy kind");
@@ -2227,17 +2245,18 @@ int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_des
     p->kind = kind;
     p->precision = precision;
     p->n = A0->n;
-    if (precision == MLP_FP64) p->impl.reset(make_hier<double>(ctx, kind, nlevels, levels, A0));
-    else p->impl.reset(make_hier<float>(ctx, kind, nlevels, levels, A0, precision == MLP_FP32_BF16));
+    if (precision == MLP_FP64) p->impl.reset(make_hier<double>(ctx, kind, nlevels, levels, A0, false, sc));
+    else p->impl.reset(make_hier<float>(ctx, kind, nlevels, levels, A0, precision == MLP_FP32_BF16, sc));
     *out = p.release();
     MLP_API_END
 }
 
-int mlp_prec_from_hier(mlp_ctx ctx, mlp_hier h, mlp_mat A0, int precision, mlp_prec* out) {
-    MLP_API_BEGIN
-    require(ctx && h && A0 && out, "null argument");
-    use_device(ctx);
-    const Hierarchy& H = h->H;
+int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_desc* levels, mlp_mat A0, int precision,
+                    mlp_prec* out) {
+    return prec_upload(ctx, kind, nlevels, levels, A0, precision, out, Schedule());
+}
+
+static std::vector<mlp_level_desc> level_descs(const Hierarchy& H) {
     std::vector<mlp_level_desc> d(H.nlevels);
     for (uint32_t l = 0; l < H.nlevels; l++) {
         const HostLevel& L = H.levels[l];
@@ -2252,9 +2271,95 @@ int mlp_prec_from_hier(mlp_ctx ctx, mlp_hier h, mlp_mat A0, int precision, mlp_p
         D.dval = L.dval.data();
         if (l > 0) { D.A_ia = L.A.ia.data(); D.A_ja = L.A.ja.data(); D.A_a = L.A.a.data(); }
     }
+    return d;
+}
+
+int mlp_prec_from_hier(mlp_ctx ctx, mlp_hier h, mlp_mat A0, int precision, mlp_prec* out) {
+    MLP_API_BEGIN
+    require(ctx && h && A0 && out, "null argument");
+    use_device(ctx);
+    const Hierarchy& H = h->H;
+    const std::vector<mlp_level_desc> d = level_descs(H);
     int kind = H.kind == PrecKind::UH_CHEB ? MLP_PREC_UH_CHEB : MLP_PREC_MULTI_SPLIT;
     int rc = mlp_prec_upload(ctx, kind, H.nlevels, d.data(), A0, precision, out);
     if (rc) return rc;
+    MLP_API_END
+}
+
+int mlp_prec_from_hier_tuned(mlp_ctx ctx, mlp_hier h, mlp_mat A0, int precision, uint32_t reps,
+                             mlp_tune_report* report, mlp_prec* out) {
+    MLP_API_BEGIN
+    require(ctx && h && A0 && out, "null argument");
+    require(reps >= 1 && reps <= 1000, "reps must lie in [1, 1000]");
+    use_device(ctx);
+    const Hierarchy& H = h->H;
+    const std::vector<mlp_level_desc> d = level_descs(H);
+    const int kind = H.kind == PrecKind::UH_CHEB ? MLP_PREC_UH_CHEB : MLP_PREC_MULTI_SPLIT;
+    std::vector<Schedule> cand;
+    cand.push_back(Schedule());                                  // the default: chain where it fits
+    if (kind == MLP_PREC_UH_CHEB) {
+        cand.push_back({true, false, 0});                        // level kernels, fused last pair
+        cand.push_back({true, true, 0});                         // level kernels only
+    }
+    cand.push_back({true, false, 64});                           // every level program in 64-thread blocks
+    cand.push_back({true, false, 256});                          // ... in 256-thread blocks
+    // time `reps` applies of each candidate on one random right-hand side (after one untimed apply)
+    const uint32_t n = A0->n;
+    DevMem tmp;
+    double* f = tmp.alloc<double>(n);
+    double* x = tmp.alloc<double>(n);
+    {
+        std::vector<double> hf(n);
+        uint64_t z = 0x9e3779b97f4a7c15ull;
+        for (uint32_t i = 0; i < n; i++) {
+            z = z * 6364136223846793005ull + 1442695040888963407ull;
+            hf[i] = double(int64_t(z >> 11) - (int64_t(1) << 52)) / double(int64_t(1) << 52);
+        }
+        MLP_HIP_CHECK(hipMemcpyAsync(f, hf.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    mlp_prec best = nullptr;
+    double best_ms = 0.0;
+    mlp_tune_report rp;
+    std::memset(&rp, 0, sizeof(rp));
+    std::vector<uint32_t> seen;
+    for (const Schedule& sc : cand) {
+        mlp_prec p = nullptr;
+        int rc = prec_upload(ctx, kind, H.nlevels, d.data(), A0, precision, &p, sc);
+        if (rc) {
+            if (best) mlp_prec_destroy(best);
+            return rc;
+        }
+        const uint32_t sb = p->impl->schedule();
+        if (std::find(seen.begin(), seen.end(), sb) != seen.end()) {   // the same schedule as an earlier one
+            mlp_prec_destroy(p);
+            continue;
+        }
+        seen.push_back(sb);
+        double ms = 0.0;
+        p->impl->apply(f, x, nullptr);
+        rc = mlp_time_prec_apply(ctx, p, f, x, reps, &ms);
+        if (rc) {
+            mlp_prec_destroy(p);
+            if (best) mlp_prec_destroy(best);
+            return rc;
+        }
+        if (rp.ncand < 8) {
+            rp.sched[rp.ncand] = sb;
+            rp.us[rp.ncand] = 1e3 * ms;
+        }
+        if (!best || ms < best_ms) {
+            if (best) mlp_prec_destroy(best);
+            best = p;
+            best_ms = ms;
+            rp.chosen = rp.ncand;
+        } else {
+            mlp_prec_destroy(p);
+        }
+        rp.ncand++;
+    }
+    if (report) *report = rp;
+    *out = best;
     MLP_API_END
 }
 

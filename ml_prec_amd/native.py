@@ -71,6 +71,10 @@ class LevelDesc(C.Structure):
                 ("A_ia", _vp), ("A_ja", _vp), ("A_a", _vp)]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("ncand", C.c_uint32), ("chosen", C.c_uint32), ("sched", C.c_uint32 * 8), ("us", C.c_double * 8)]
+
+
 class PcgJob(C.Structure):
     _fields_ = [("ctx", _vp), ("A", _vp), ("B", _vp), ("d_b", _vp), ("d_x", _vp), ("stats", SolveStats),
                 ("status", C.c_int)]
@@ -137,6 +141,8 @@ EXPORTS = {
                                               C.POINTER(_vp)]),
     "mlp_pcg_ensemble": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _u32p, _vp, _vp, C.POINTER(SolveOpts),
                                    C.POINTER(SolveStats)]),
+    "mlp_prec_from_hier_tuned": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_uint32, C.POINTER(TuneReport),
+                                           C.POINTER(_vp)]),
     "mlp_hier_build_uh_cheb_mat": (C.c_int, [_vp, _vp, _f64p, C.c_uint32, _u32p, C.c_uint32, C.c_int,
                                             C.POINTER(_vp)]),
     "mlp_spe_assemble": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, _vp, C.c_double, C.c_double,
@@ -416,6 +422,19 @@ class DevPrec:
             check(lib().mlp_prec_simple(ctx.h, kind, A.h, C.byref(self.h)))
             self.kind = kind
         self.precision = precision
+
+    @classmethod
+    def tuned(cls, ctx, A: DevMatrix, hier: Hierarchy, precision=MLP_FP64, reps=10):
+        """The hierarchy with the fastest of the candidate launch schedules (mlp_prec_from_hier_tuned); the
+        report is kept in .tune: [(schedule bits, us per apply)] and the index chosen."""
+        p = cls.__new__(cls)
+        p.ctx, p.A, p.hier, p.kind, p.precision = ctx, A, hier, hier.kind, precision
+        p.h = _vp()
+        rep = TuneReport()
+        check(lib().mlp_prec_from_hier_tuned(ctx.h, hier.h, A.h, precision, reps, C.byref(rep), C.byref(p.h)))
+        p.tune = {"candidates": [(int(rep.sched[k]), float(rep.us[k])) for k in range(min(rep.ncand, 8))],
+                  "chosen": int(rep.chosen)}
+        return p
 
     def close(self):
         if getattr(self, "h", None) and _lib is not None and self.ctx.h:
