@@ -297,6 +297,22 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
  * phase per round).  stats has nseg entries; the return code is the first failing segment's. */
 int mlp_pcg_ensemble(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nseg, const uint32_t* seg_off, const double* d_b,
                      double* d_x, const mlp_solve_opts* opts, mlp_solve_stats* stats);
+/* Several such ensembles at once, each on its own context (e.g. the groups of an ensemble whose systems'
+ * hierarchies have different depths), driven from the calling thread: their iteration chunks are enqueued
+ * round-robin and polled through events.  Every job's x, stats and status are those of mlp_pcg_ensemble on
+ * it alone; returns MLP_OK or the first failing job's code. */
+typedef struct {
+    mlp_ctx ctx;
+    mlp_mat A;
+    mlp_prec B;
+    uint32_t nseg;
+    const uint32_t* seg_off;  /* host, nseg + 1 */
+    const double* d_b;
+    double* d_x;
+    mlp_solve_stats* stats;   /* nseg entries (out) */
+    int status;               /* out */
+} mlp_ens_job;
+int mlp_pcg_ensemble_batch(uint32_t count, mlp_ens_job* jobs, const mlp_solve_opts* opts);
 /* Adjoint gradient for the two-point-flux assembly of modules/mesh/spe.cc:69-131 (no counterpart in the
  * reference; for differentiating a misfit through the solve).  Parameters theta_i = ln kx_i per cell
  * (ky = kx, kz = kz_ratio kx); fx, fy, fz are the per-direction face factors (1/h^2 in spe.cc's scaling);

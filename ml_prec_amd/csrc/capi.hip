@@ -3329,6 +3329,213 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
     MLP_API_END
 }
 
+}  // extern "C"
+
+namespace mlp {
+namespace {
+
+// One ensemble solve (mlp_pcg_ensemble) advanced in steps, so that the solves of several contexts can be driven
+// round-robin from one host thread.
+struct EnsRun {
+    mlp_ctx ctx = nullptr;
+    mlp_mat A = nullptr;
+    mlp_prec B = nullptr;
+    uint32_t S = 0;
+    const uint32_t* seg_off = nullptr;
+    const double* b = nullptr;
+    double* x = nullptr;
+    const mlp_solve_opts* o = nullptr;
+    bool flex = false;
+    uint32_t tre = 0, chunk = 16, gmax = 1, k = 0;
+    PcgVecs V;
+    mlp_ctx_s::GraphEntry* ge = nullptr;
+    hipEvent_t ev = nullptr;
+    std::chrono::steady_clock::time_point t0;
+
+    ~EnsRun() {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+    void check() const {
+        require(ctx && A && B && b && x && o && seg_off, "null argument");
+        require(S >= 1 && S <= kMaxSegments, "nseg must lie in [1, 256]");
+        require(seg_off[0] == 0 && seg_off[S] == A->n, "segments must cover the matrix rows");
+        for (uint32_t y = 0; y < S; y++)
+            require(seg_off[y + 1] > seg_off[y], "segments must be non-empty and ascending");
+        require(B->n == A->n, "preconditioner size differs from A");
+        require(B->ctx == ctx && A->ctx->device == ctx->device, "the preconditioner must live on the context");
+    }
+    // x0, the initial norms and eps floors, z = B r and p = z for every segment; the iteration graphs
+    void start() {
+        use_device(ctx);
+        t0 = std::chrono::steady_clock::now();
+        flex = o->flexible != 0;
+        tre = o->true_res_every;
+        chunk = o->check_every ? o->check_every : 16;
+        hipStream_t s = ctx->stream;
+        V = ensure_ens(ctx, A->n, S, seg_off);
+        gmax = 1;
+        for (uint32_t y = 0; y < S; y++) gmax = std::max(gmax, red_grid(seg_off[y + 1] - seg_off[y]));
+        for (uint32_t y = 0; y < S; y++) fill_x0(ctx, seg_off[y + 1] - seg_off[y], o->x0_mode, x + seg_off[y]);
+        for (uint32_t y = 0; y < S; y++) {
+            SolveState& st0 = ctx->est_host[y];
+            std::memset(&st0, 0, sizeof(st0));
+            st0.eps = o->eps;
+            st0.maxit = o->maxit;
+        }
+        MLP_HIP_CHECK(hipMemcpyAsync(V.st, ctx->est_host, S * sizeof(SolveState), hipMemcpyHostToDevice, s));
+        enqueue_ens_norms(ctx, A, b, x, V, gmax, FIN_START, nullptr);
+        enqueue_ens_start(ctx, B->impl.get(), flex, V, gmax);
+        const bool graphs = o->use_graph && B->impl->graph_safe() && graphs_allowed();
+        ge = graphs ? ens_graph(ctx, A, B, b, x, flex, tre, V, gmax) : nullptr;
+        if (!ev) MLP_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        k = 0;
+    }
+    // c iterations, then the states copied to the host behind an event
+    void enqueue(uint32_t c) {
+        hipStream_t s = ctx->stream;
+        for (uint32_t i = 0; i < c; i++) {
+            k++;
+            const bool replace = tre && (k % tre == 0);
+            if (ge) MLP_HIP_CHECK(hipGraphLaunch(ge->e[replace ? 1 : 0], s));
+            else enqueue_ens_iteration(ctx, A, B->impl.get(), b, x, flex, replace, V, gmax);
+        }
+        MLP_HIP_CHECK(hipGetLastError());
+        MLP_HIP_CHECK(hipMemcpyAsync(ctx->est_host, V.st, S * sizeof(SolveState), hipMemcpyDeviceToHost, s));
+        MLP_HIP_CHECK(hipEventRecord(ev, s));
+    }
+    // after the event: true when every segment is done, else the next chunk is enqueued
+    bool advance() {
+        bool all = true;
+        uint32_t next = 1;
+        for (uint32_t y = 0; y < S; y++) {
+            const SolveState& st = ctx->est_host[y];
+            if (st.done) continue;
+            all = false;
+            next = std::max(next, next_chunk(st, chunk));
+        }
+        if (!all) enqueue(next);
+        return all;
+    }
+    // true when the enqueued work has finished and every segment is done
+    bool poll() {
+        const hipError_t q = hipEventQuery(ev);
+        if (q == hipErrorNotReady) return false;
+        MLP_HIP_CHECK(q);
+        return advance();
+    }
+    void run_sync() {
+        enqueue(chunk);
+        while (true) {
+            MLP_HIP_CHECK(hipEventSynchronize(ev));
+            if (advance()) return;
+        }
+    }
+    // FP64 true relative residuals ||b - A x|| / init_norm of every segment
+    void true_rel(const std::vector<SolveState>& fin, std::vector<double>& tr) {
+        hipStream_t s = ctx->stream;
+        enqueue_ens_norms(ctx, A, b, x, V, gmax, FIN_DOT, ctx->eout);
+        MLP_HIP_CHECK(hipMemcpyAsync(ctx->eout_host, ctx->eout, S * sizeof(double), hipMemcpyDeviceToHost, s));
+        MLP_HIP_CHECK(hipStreamSynchronize(s));
+        tr.resize(S);
+        for (uint32_t y = 0; y < S; y++) {
+            const double nr = std::sqrt(ctx->eout_host[y]);
+            tr[y] = fin[y].init_norm > 0 ? nr / fin[y].init_norm : nr;
+        }
+    }
+    // flexible restarts, the stats; returns the first failing segment's code (message in *msg)
+    int finish(mlp_solve_stats* stats, std::string* msg) {
+        use_device(ctx);
+        hipStream_t s = ctx->stream;
+        std::vector<SolveState> fin(ctx->est_host, ctx->est_host + S);
+        std::vector<double> tr;
+        if (flex) {
+            // mlp_pcg's restarts from the FP64 true residual (pcg_phase), every segment that needs one at once
+            std::vector<char> stop(S, 0);
+            for (uint32_t y = 0; y < S; y++) stop[y] = fin[y].status != 0;
+            for (uint32_t restart = 0; restart < kFlexRestarts; restart++) {
+                true_rel(fin, tr);
+                bool need = false;
+                for (uint32_t y = 0; y < S; y++) {
+                    if (!stop[y] && tr[y] <= fin[y].eps) stop[y] = 1;
+                    SolveState& st0 = ctx->est_host[y];              // pcg_phase's starting state
+                    std::memset(&st0, 0, sizeof(st0));
+                    st0.init_norm = fin[y].init_norm;
+                    st0.eps = fin[y].eps;
+                    st0.maxit = o->maxit;
+                    st0.niter = fin[y].niter;
+                    st0.done = stop[y] ? 1u : 0u;
+                    need = need || !stop[y];
+                }
+                if (!need) break;
+                MLP_HIP_CHECK(hipMemcpyAsync(V.st, ctx->est_host, S * sizeof(SolveState), hipMemcpyHostToDevice, s));
+                enqueue_ens_norms(ctx, A, b, x, V, gmax, FIN_RESTART, nullptr);
+                // a segment whose restarted residual already meets eps stops there (no iteration, no more restarts)
+                MLP_HIP_CHECK(hipMemcpyAsync(ctx->est_host, V.st, S * sizeof(SolveState), hipMemcpyDeviceToHost, s));
+                MLP_HIP_CHECK(hipStreamSynchronize(s));
+                bool go = false;
+                for (uint32_t y = 0; y < S; y++) {
+                    if (stop[y]) continue;
+                    if (ctx->est_host[y].done) {
+                        fin[y] = ctx->est_host[y];
+                        fin[y].done = 1;
+                        stop[y] = 1;
+                    } else {
+                        go = true;
+                    }
+                }
+                if (!go) break;
+                enqueue_ens_start(ctx, B->impl.get(), flex, V, gmax);
+                run_sync();
+                for (uint32_t y = 0; y < S; y++)
+                    if (!stop[y]) {
+                        fin[y] = ctx->est_host[y];
+                        if (fin[y].status != 0) stop[y] = 1;
+                    }
+            }
+        }
+        true_rel(fin, tr);
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        int first = MLP_OK;
+        for (uint32_t y = 0; y < S; y++) {
+            const SolveState& st = fin[y];
+            mlp_solve_stats& St = stats[y];
+            std::memset(&St, 0, sizeof(St));
+            St.niter = st.niter;
+            St.init_norm = st.init_norm;
+            St.final_norm = st.norm;
+            St.eps_used = st.eps;
+            St.t_solve_ms = ms;
+            St.true_rel_res = tr[y];
+            int code = MLP_OK;
+            const char* what = "";
+            if (st.status == 4) {
+                code = MLP_ERR_NO_ITER;
+                what = "No need to iterate";
+            } else if (st.status == 3) {
+                code = MLP_ERR_NOT_SPD;
+                what = "<Ap, p> <= 0";
+            } else if (st.status == 5) {
+                code = MLP_ERR_MAXIT;
+                what = "maximum number of iterations reached";
+            } else if (flex && St.true_rel_res > st.eps) {
+                code = MLP_ERR_NO_CONV;
+                what = "flexible PCG: the FP64 true residual is above eps after the restarts";
+            }
+            St.status = code;
+            if (code && first == MLP_OK) {
+                first = code;
+                *msg = "segment " + std::to_string(y) + ": " + what;
+            }
+        }
+        return first;
+    }
+};
+
+}  // namespace
+}  // namespace mlp
+
+extern "C" {
+
 // An ensemble of independent systems given as one block-diagonal system (segment y = rows
 // [seg_off[y], seg_off[y+1]), no entries coupling two segments), with one preconditioner for the whole of it.
 // Every segment runs PCGSolver's steps (pcg.cc:8-102) on its own rows with its own state, reductions of its
@@ -3338,160 +3545,72 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
 int mlp_pcg_ensemble(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nseg, const uint32_t* seg_off, const double* b,
                      double* x, const mlp_solve_opts* o, mlp_solve_stats* stats) {
     MLP_API_BEGIN
-    require(ctx && A && B && b && x && o && stats && seg_off, "null argument");
-    require(nseg >= 1 && nseg <= kMaxSegments, "nseg must lie in [1, 256]");
-    require(seg_off[0] == 0 && seg_off[nseg] == A->n, "segments must cover the matrix rows");
-    for (uint32_t y = 0; y < nseg; y++) require(seg_off[y + 1] > seg_off[y], "segments must be non-empty and ascending");
-    require(B->n == A->n, "preconditioner size differs from A");
-    require(B->ctx == ctx && A->ctx->device == ctx->device, "the preconditioner must live on the context");
-    use_device(ctx);
-    const bool flex = o->flexible != 0;
-    const uint32_t tre = o->true_res_every, S = nseg;
-    const auto t0 = std::chrono::steady_clock::now();
-    hipStream_t s = ctx->stream;
-    const PcgVecs V = ensure_ens(ctx, A->n, S, seg_off);
-    uint32_t gmax = 1;
-    for (uint32_t y = 0; y < S; y++) gmax = std::max(gmax, red_grid(seg_off[y + 1] - seg_off[y]));
-    for (uint32_t y = 0; y < S; y++) fill_x0(ctx, seg_off[y + 1] - seg_off[y], o->x0_mode, x + seg_off[y]);
-    for (uint32_t y = 0; y < S; y++) {
-        SolveState& st0 = ctx->est_host[y];
-        std::memset(&st0, 0, sizeof(st0));
-        st0.eps = o->eps;
-        st0.maxit = o->maxit;
-    }
-    MLP_HIP_CHECK(hipMemcpyAsync(V.st, ctx->est_host, S * sizeof(SolveState), hipMemcpyHostToDevice, s));
-    enqueue_ens_norms(ctx, A, b, x, V, gmax, FIN_START, nullptr);
-    enqueue_ens_start(ctx, B->impl.get(), flex, V, gmax);
-    const bool graphs = o->use_graph && B->impl->graph_safe() && graphs_allowed();
-    mlp_ctx_s::GraphEntry* ge = graphs ? ens_graph(ctx, A, B, b, x, flex, tre, V, gmax) : nullptr;
-    const uint32_t chunk = o->check_every ? o->check_every : 16;
-    // iterations until every segment is done
-    auto run = [&]() {
-        uint32_t c = chunk, k = 0;
-        while (true) {
-            for (uint32_t i = 0; i < c; i++) {
-                k++;
-                const bool replace = tre && (k % tre == 0);
-                if (ge) MLP_HIP_CHECK(hipGraphLaunch(ge->e[replace ? 1 : 0], s));
-                else enqueue_ens_iteration(ctx, A, B->impl.get(), b, x, flex, replace, V, gmax);
-            }
-            MLP_HIP_CHECK(hipGetLastError());
-            MLP_HIP_CHECK(hipMemcpyAsync(ctx->est_host, V.st, S * sizeof(SolveState), hipMemcpyDeviceToHost, s));
-            MLP_HIP_CHECK(hipStreamSynchronize(s));
-            bool all = true;
-            uint32_t next = 1;
-            for (uint32_t y = 0; y < S; y++) {
-                const SolveState& st = ctx->est_host[y];
-                if (st.done) continue;
-                all = false;
-                next = std::max(next, next_chunk(st, chunk));
-            }
-            if (all) return;
-            c = next;
-        }
-    };
-    MLP_HIP_CHECK(hipMemcpyAsync(ctx->est_host, V.st, S * sizeof(SolveState), hipMemcpyDeviceToHost, s));
-    MLP_HIP_CHECK(hipStreamSynchronize(s));
-    bool any = false;
-    for (uint32_t y = 0; y < S; y++) any = any || !ctx->est_host[y].done;
-    if (any) run();
-    std::vector<SolveState> fin(ctx->est_host, ctx->est_host + S);
-    // FP64 true relative residuals ||b - A x|| / init_norm of every segment
-    auto true_rel = [&](std::vector<double>& tr) {
-        enqueue_ens_norms(ctx, A, b, x, V, gmax, FIN_DOT, ctx->eout);
-        MLP_HIP_CHECK(hipMemcpyAsync(ctx->eout_host, ctx->eout, S * sizeof(double), hipMemcpyDeviceToHost, s));
-        MLP_HIP_CHECK(hipStreamSynchronize(s));
-        tr.resize(S);
-        for (uint32_t y = 0; y < S; y++) {
-            const double nr = std::sqrt(ctx->eout_host[y]);
-            tr[y] = fin[y].init_norm > 0 ? nr / fin[y].init_norm : nr;
-        }
-    };
-    std::vector<double> tr;
-    if (flex) {
-        // mlp_pcg's restarts from the FP64 true residual (pcg_phase), every segment that needs one at once
-        std::vector<char> stop(S, 0);
-        for (uint32_t y = 0; y < S; y++) stop[y] = fin[y].status != 0;
-        for (uint32_t restart = 0; restart < kFlexRestarts; restart++) {
-            true_rel(tr);
-            bool need = false;
-            for (uint32_t y = 0; y < S; y++) {
-                if (!stop[y] && tr[y] <= fin[y].eps) stop[y] = 1;
-                SolveState& st0 = ctx->est_host[y];              // pcg_phase's starting state
-                std::memset(&st0, 0, sizeof(st0));
-                st0.init_norm = fin[y].init_norm;
-                st0.eps = fin[y].eps;
-                st0.maxit = o->maxit;
-                st0.niter = fin[y].niter;
-                st0.done = stop[y] ? 1u : 0u;
-                need = need || !stop[y];
-            }
-            if (!need) break;
-            MLP_HIP_CHECK(hipMemcpyAsync(V.st, ctx->est_host, S * sizeof(SolveState), hipMemcpyHostToDevice, s));
-            enqueue_ens_norms(ctx, A, b, x, V, gmax, FIN_RESTART, nullptr);
-            // a segment whose restarted residual already meets eps stops there (no iteration, no more restarts)
-            MLP_HIP_CHECK(hipMemcpyAsync(ctx->est_host, V.st, S * sizeof(SolveState), hipMemcpyDeviceToHost, s));
-            MLP_HIP_CHECK(hipStreamSynchronize(s));
-            bool go = false;
-            for (uint32_t y = 0; y < S; y++) {
-                if (stop[y]) continue;
-                if (ctx->est_host[y].done) {
-                    fin[y] = ctx->est_host[y];
-                    fin[y].done = 1;
-                    stop[y] = 1;
-                } else {
-                    go = true;
-                }
-            }
-            if (!go) break;
-            enqueue_ens_start(ctx, B->impl.get(), flex, V, gmax);
-            run();
-            for (uint32_t y = 0; y < S; y++)
-                if (!stop[y]) {
-                    fin[y] = ctx->est_host[y];
-                    if (fin[y].status != 0) stop[y] = 1;
-                }
-        }
-    }
-    true_rel(tr);
-    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    int first = MLP_OK;
+    require(stats, "null argument");
+    EnsRun r;
+    r.ctx = ctx; r.A = A; r.B = B; r.S = nseg; r.seg_off = seg_off; r.b = b; r.x = x; r.o = o;
+    r.check();
+    r.start();
+    r.run_sync();
     std::string msg;
-    for (uint32_t y = 0; y < S; y++) {
-        const SolveState& st = fin[y];
-        mlp_solve_stats& St = stats[y];
-        std::memset(&St, 0, sizeof(St));
-        St.niter = st.niter;
-        St.init_norm = st.init_norm;
-        St.final_norm = st.norm;
-        St.eps_used = st.eps;
-        St.t_solve_ms = ms;
-        St.true_rel_res = tr[y];
-        int code = MLP_OK;
-        const char* what = "";
-        if (st.status == 4) {
-            code = MLP_ERR_NO_ITER;
-            what = "No need to iterate";
-        } else if (st.status == 3) {
-            code = MLP_ERR_NOT_SPD;
-            what = "<Ap, p> <= 0";
-        } else if (st.status == 5) {
-            code = MLP_ERR_MAXIT;
-            what = "maximum number of iterations reached";
-        } else if (flex && St.true_rel_res > st.eps) {
-            code = MLP_ERR_NO_CONV;
-            what = "flexible PCG: the FP64 true residual is above eps after the restarts";
+    const int rc = r.finish(stats, &msg);
+    if (rc != MLP_OK) {
+        set_last_error(msg);
+        return rc;
+    }
+    MLP_API_END
+}
+
+// Several ensembles (each its own context and stream) driven from this thread: their iteration chunks are
+// enqueued round-robin and each one's states polled through an event.  Every job's results are those of
+// mlp_pcg_ensemble on it alone.
+int mlp_pcg_ensemble_batch(uint32_t count, mlp_ens_job* jobs, const mlp_solve_opts* o) {
+    MLP_API_BEGIN
+    require(o && (jobs || !count), "null argument");
+    std::vector<EnsRun> runs(count);
+    std::vector<mlp_ctx> seen;
+    for (uint32_t i = 0; i < count; i++) {
+        mlp_ens_job& J = jobs[i];
+        require(J.stats, "null job argument");
+        EnsRun& r = runs[i];
+        r.ctx = J.ctx; r.A = J.A; r.B = J.B; r.S = J.nseg; r.seg_off = J.seg_off; r.b = J.d_b; r.x = J.d_x; r.o = o;
+        r.check();
+        for (mlp_ctx c : seen) require(c != J.ctx, "two jobs share a context");
+        seen.push_back(J.ctx);
+        J.status = MLP_OK;
+    }
+    for (EnsRun& r : runs) {
+        r.start();
+        r.enqueue(r.chunk);
+    }
+    std::vector<char> active(count, 1);
+    uint32_t left = count;
+    while (left) {
+        bool progressed = false;
+        for (uint32_t i = 0; i < count; i++) {
+            if (!active[i]) continue;
+            use_device(runs[i].ctx);
+            const hipError_t q = hipEventQuery(runs[i].ev);
+            if (q == hipErrorNotReady) continue;
+            MLP_HIP_CHECK(q);
+            progressed = true;
+            if (runs[i].advance()) {
+                active[i] = 0;
+                left--;
+            }
         }
-        St.status = code;
-        if (code && first == MLP_OK) {
-            first = code;
-            msg = "segment " + std::to_string(y) + ": " + what;
+        if (!progressed) std::this_thread::yield();
+    }
+    int first = MLP_OK;
+    for (uint32_t i = 0; i < count; i++) {
+        std::string msg;
+        const int rc = runs[i].finish(jobs[i].stats, &msg);
+        jobs[i].status = rc;
+        if (rc != MLP_OK && first == MLP_OK) {
+            first = rc;
+            set_last_error("job " + std::to_string(i) + ", " + msg);
         }
     }
-    if (first != MLP_OK) {
-        set_last_error(msg.c_str());
-        return first;
-    }
+    return first;
     MLP_API_END
 }
 

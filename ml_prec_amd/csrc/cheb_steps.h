@@ -560,7 +560,8 @@ struct IsoDiag {
 // provides alloc<T>(n), zero(p, n), each(n, f) (f(i) for i < n), each_seg(bnd,
 // r, f) (f(q) for bnd[r] <= q < bnd[r+1], bounds read where f runs), scan(in,
 // out, n) (exclusive, out[n] = total), get(dst, src, n) and get1(src) (copies
-// to the host, waiting for the steps before them).
+// to the host, waiting for the steps before them) and get_many(items) (several
+// copies, one wait).
 // ---------------------------------------------------------------------------
 struct LevelMatrix {               // a level matrix on the executor (Skyline CSR)
     uint32_t n = 0, nnz = 0;
@@ -569,13 +570,14 @@ struct LevelMatrix {               // a level matrix on the executor (Skyline CS
     const double* a = nullptr;
 };
 
-template <class T, class X> void fetch(X& x, HostVec<T>& dst, const T* src, size_t n) {
+struct Fetch {                     // one copy of get_many: bytes from the executor's src to the host's dst
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+template <class V, class T> void fetch(std::vector<Fetch>& f, V& dst, const T* src, size_t n) {
     dst.resize(n);
-    if (n) x.get(dst.data(), src, n);
-}
-template <class T, class X> void fetch(X& x, std::vector<T>& dst, const T* src, size_t n) {
-    dst.resize(n);
-    if (n) x.get(dst.data(), src, n);
+    if (n) f.push_back({dst.data(), src, n * sizeof(T)});
 }
 
 // One level: fills li (N, nnz, M, Md, map, rmap, aux, L, U, dval) and returns
@@ -695,21 +697,22 @@ LevelMatrix level_steps(X& x, const LevelMatrix& A, double beta, bool tails, Hos
     double* dval = x.template alloc<double>(Md);
     x.each(Md, IsoDiag{map, aux, nkeep, dval});
 
-    fetch(x, li.map, map, N);
-    fetch(x, li.rmap, rmap, N);
-    fetch(x, li.aux, aux, N);
+    std::vector<Fetch> out;
+    fetch(out, li.map, map, N);
+    fetch(out, li.rmap, rmap, N);
+    fetch(out, li.aux, aux, N);
     li.L = HostCSR();
     li.L.nrow = li.L.ncol = nkeep;
-    fetch(x, li.L.ia, L_ia, size_t(nkeep) + 1);
-    fetch(x, li.L.ja, L_ja, nnzL);
-    fetch(x, li.L.a, L_a, nnzL);
+    fetch(out, li.L.ia, L_ia, size_t(nkeep) + 1);
+    fetch(out, li.L.ja, L_ja, nnzL);
+    fetch(out, li.L.a, L_a, nnzL);
     li.U = HostCSR();
     li.U.nrow = M;
     li.U.ncol = nkeep;
-    fetch(x, li.U.ia, U_ia, size_t(M) + 1);
-    fetch(x, li.U.ja, U_ja, nnzU);
-    fetch(x, li.U.a, U_a, nnzU);
-    fetch(x, li.dval, dval, Md);
+    fetch(out, li.U.ia, U_ia, size_t(M) + 1);
+    fetch(out, li.U.ja, U_ja, nnzU);
+    fetch(out, li.U.a, U_a, nnzU);
+    fetch(out, li.dval, dval, Md);
     LevelMatrix nx;
     nx.n = n;
     nx.nnz = nnzN;
@@ -720,11 +723,12 @@ LevelMatrix level_steps(X& x, const LevelMatrix& A, double beta, bool tails, Hos
         *next_host = HostCSR();
         next_host->nrow = next_host->ncol = n;
         if (n) {
-            fetch(x, next_host->ia, N_ia, size_t(n) + 1);
-            fetch(x, next_host->ja, N_ja, nnzN);
-            fetch(x, next_host->a, N_a, nnzN);
+            fetch(out, next_host->ia, N_ia, size_t(n) + 1);
+            fetch(out, next_host->ja, N_ja, nnzN);
+            fetch(out, next_host->a, N_a, nnzN);
         }
     }
+    x.get_many(out);
     return nx;
 }
 

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "par.h"
@@ -113,44 +114,68 @@ struct SetZero {
             throw ::mlp::Error(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #expr ")", 2); \
     } while (0)
 
-// Runs the steps as kernels on one stream.  Device memory comes from large
-// blocks carved in order (released together when the build ends).
+// Memory a device keeps for its constructions (until the process ends): device
+// blocks carved in order by each build, and the pinned staging buffers.  One
+// build at a time uses them (the lock is held for the whole build).
+struct BuildMemory {
+    std::mutex m;
+    std::vector<std::pair<void*, size_t>> blocks;
+    char* pin = nullptr;                       // bounce buffer of large single copies
+    size_t pin_n = 0;
+    char* gpin = nullptr;                      // staging of a level's batched copies
+    size_t gpin_n = 0;
+};
+BuildMemory& build_memory(int device) {
+    static std::mutex g;
+    static std::vector<std::unique_ptr<BuildMemory>> per;
+    std::lock_guard<std::mutex> hold(g);
+    if (device < 0) device = 0;
+    if (size_t(device) >= per.size()) per.resize(size_t(device) + 1);
+    if (!per[device]) per[device] = std::make_unique<BuildMemory>();
+    return *per[device];
+}
+char* grow_pinned(char*& p, size_t& have, size_t want) {
+    if (have < want) {
+        if (p) DEVBUILD_CHECK(hipHostFree(p));
+        p = nullptr;
+        have = 0;
+        DEVBUILD_CHECK(hipHostMalloc((void**)&p, want, hipHostMallocDefault));
+        have = want;
+    }
+    return p;
+}
+
+// Runs the steps as kernels on one stream, with the device's build memory.
 struct DevExec {
     hipStream_t s;
-    std::vector<void*> blocks;
+    BuildMemory& bm;
+    std::unique_lock<std::mutex> hold;
+    size_t blk = 0;                            // block being carved
     char* cur = nullptr;
     size_t left = 0;
     uint32_t* bsum = nullptr;                  // scan block totals
     uint32_t bsum_cap = 0;
-    char* pin = nullptr;                       // pinned bounce buffer for large copies
-    size_t pin_n = 0;
-    explicit DevExec(hipStream_t st) : s(st) {}
-    ~DevExec() {
-        (void)hipStreamSynchronize(s);
-        for (void* p : blocks) (void)hipFree(p);
-        if (pin) (void)hipHostFree(pin);
+    static int device_of() {
+        int d = 0;
+        DEVBUILD_CHECK(hipGetDevice(&d));
+        return d;
     }
+    explicit DevExec(hipStream_t st) : s(st), bm(build_memory(device_of())), hold(bm.m) {}
+    ~DevExec() { (void)hipStreamSynchronize(s); }
     static constexpr size_t kPinChunk = size_t(32) << 20;
-    char* bounce(size_t bytes) {
-        const size_t want = std::min(bytes, kPinChunk);
-        if (pin_n < want) {
-            if (pin) DEVBUILD_CHECK(hipHostFree(pin));
-            pin = nullptr;
-            pin_n = 0;
-            DEVBUILD_CHECK(hipHostMalloc((void**)&pin, want, hipHostMallocDefault));
-            pin_n = want;
-        }
-        return pin;
-    }
+    char* bounce(size_t bytes) { return grow_pinned(bm.pin, bm.pin_n, std::min(bytes, kPinChunk)); }
     template <class T> T* alloc(size_t n) {
         const size_t b = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~size_t(255);
-        if (b > left) {
-            const size_t sz = std::max<size_t>(b, size_t(64) << 20);
-            void* p = nullptr;
-            DEVBUILD_CHECK(hipMalloc(&p, sz));
-            blocks.push_back(p);
-            cur = static_cast<char*>(p);
-            left = sz;
+        while (b > left) {
+            if (cur) blk++;                                   // the next kept block, or a new one
+            if (blk == bm.blocks.size()) {
+                const size_t sz = std::max<size_t>(b, size_t(64) << 20);
+                void* p = nullptr;
+                DEVBUILD_CHECK(hipMalloc(&p, sz));
+                bm.blocks.push_back({p, sz});
+            }
+            cur = static_cast<char*>(bm.blocks[blk].first);
+            left = bm.blocks[blk].second;
         }
         T* r = reinterpret_cast<T*>(cur);
         cur += b;
@@ -198,6 +223,27 @@ struct DevExec {
             DEVBUILD_CHECK(hipMemcpyAsync(b, from + off, len, hipMemcpyDeviceToHost, s));
             DEVBUILD_CHECK(hipStreamSynchronize(s));
             parallel_for(len, size_t(1) << 20, [&](uint64_t x0, uint64_t x1) { std::memcpy(to + off + x0, b + x0, x1 - x0); });
+        }
+    }
+    // every copy into one pinned buffer behind one wait, then spread over the host threads into place
+    void get_many(const std::vector<steps::Fetch>& items) {
+        size_t total = 0;
+        for (const auto& f : items) total += (f.bytes + 255) & ~size_t(255);
+        if (!total) return;
+        char* gpin = grow_pinned(bm.gpin, bm.gpin_n, total);
+        std::vector<size_t> off(items.size());
+        size_t o = 0;
+        for (size_t k = 0; k < items.size(); k++) {
+            off[k] = o;
+            DEVBUILD_CHECK(hipMemcpyAsync(gpin + o, items[k].src, items[k].bytes, hipMemcpyDeviceToHost, s));
+            o += (items[k].bytes + 255) & ~size_t(255);
+        }
+        DEVBUILD_CHECK(hipStreamSynchronize(s));
+        for (size_t k = 0; k < items.size(); k++) {
+            char* to = static_cast<char*>(items[k].dst);
+            const char* from = gpin + off[k];
+            parallel_for(items[k].bytes, size_t(1) << 20,
+                         [&](uint64_t x0, uint64_t x1) { std::memcpy(to + x0, from + x0, x1 - x0); });
         }
     }
     template <class T> T get1(const T* src) {

@@ -71,6 +71,11 @@ class LevelDesc(C.Structure):
                 ("A_ia", _vp), ("A_ja", _vp), ("A_a", _vp)]
 
 
+class EnsJob(C.Structure):
+    _fields_ = [("ctx", _vp), ("A", _vp), ("B", _vp), ("nseg", C.c_uint32), ("seg_off", _vp), ("d_b", _vp),
+                ("d_x", _vp), ("stats", C.POINTER(SolveStats)), ("status", C.c_int)]
+
+
 class TuneReport(C.Structure):
     _fields_ = [("ncand", C.c_uint32), ("chosen", C.c_uint32), ("sched", C.c_uint32 * 8), ("us", C.c_double * 8)]
 
@@ -141,6 +146,7 @@ EXPORTS = {
                                               C.POINTER(_vp)]),
     "mlp_pcg_ensemble": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _u32p, _vp, _vp, C.POINTER(SolveOpts),
                                    C.POINTER(SolveStats)]),
+    "mlp_pcg_ensemble_batch": (C.c_int, [C.c_uint32, _vp, C.POINTER(SolveOpts)]),
     "mlp_prec_from_hier_tuned": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_uint32, C.POINTER(TuneReport),
                                            C.POINTER(_vp)]),
     "mlp_hier_build_uh_cheb_mat": (C.c_int, [_vp, _vp, _f64p, C.c_uint32, _u32p, C.c_uint32, C.c_int,
@@ -566,6 +572,24 @@ def pcg_ensemble(ctx, A, B, seg_off, b: DevArray, x: DevArray, eps=1e-8, maxit=0
     o = _opts(eps, maxit, x0_mode, flexible, true_res_every, check_every, use_graph)
     rc = lib().mlp_pcg_ensemble(ctx.h, A.h, B.h, S, off, b.ptr, x.ptr, C.byref(o), st)
     return rc, [s.as_dict() for s in st]
+
+
+def pcg_ensemble_batch(jobs, eps=1e-8, maxit=0, x0_mode=X0_ZERO, flexible=False, true_res_every=0, check_every=16,
+                       use_graph=True):
+    """mlp_pcg_ensemble_batch over jobs = [(ctx, A, B, seg_off, b, x)], each context used once.  Returns
+    (rc, [(status, [stats per segment]) per job])."""
+    n = len(jobs)
+    arr = (EnsJob * n)()
+    keep = []
+    for k, (ctx, A, B, off, b, x) in enumerate(jobs):
+        off = np.ascontiguousarray(off, dtype=np.uint32)
+        st = (SolveStats * (len(off) - 1))()
+        keep.append((off, st))
+        arr[k] = EnsJob(ctx.h, A.h, B.h, len(off) - 1, off.ctypes.data_as(_vp), b.ptr, x.ptr,
+                        C.cast(st, C.POINTER(SolveStats)), 0)
+    o = _opts(eps, maxit, x0_mode, flexible, true_res_every, check_every, use_graph)
+    rc = lib().mlp_pcg_ensemble_batch(n, C.cast(arr, _vp), C.byref(o))
+    return rc, [(arr[k].status, [s.as_dict() for s in keep[k][1]]) for k in range(n)]
 
 
 def gmres(ctx, A, B, b: DevArray, x: DevArray, eps=1e-8, maxit=0, x0_mode=X0_ZERO, raise_on_error=True):

@@ -66,6 +66,9 @@ struct HostExec {
     }
     template <class T> void get(T* dst, const T* src, size_t n) { std::memcpy(dst, src, n * sizeof(T)); }
     template <class T> T get1(const T* src) { return *src; }
+    void get_many(const std::vector<steps::Fetch>& items) {
+        for (const auto& f : items) std::memcpy(f.dst, f.src, f.bytes);
+    }
 };
 
 struct StepsOnHost : ChebLevelBuilder {

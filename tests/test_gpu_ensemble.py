@@ -174,3 +174,38 @@ def test_ensemble_errors(ctx):
     A.close()
     for m in ms:
         m.close()
+
+
+def test_ensemble_batch_of_groups_equals_single_solves():
+    """Every depth group on its own context, all driven from one thread (mlp_pcg_ensemble_batch): each
+    system's solution and iteration count are its single solve's, bit for bit."""
+    ctx0 = N.Context(0)
+    ms = _members(ctx0)
+    groups = list(_groups(ms).values())
+    kw = dict(eps=1e-8, maxit=5000, x0_mode=N.X0_ZERO, check_every=4, use_graph=True)
+    jobs, keep = [], []
+    for g in groups:
+        c = N.Context(0)
+        A, Hg, B, off = _stack(c, g)
+        b = c.to_device(np.concatenate([m.b for m in g]))
+        x = c.array(int(off[-1]))
+        jobs.append((c, A, B, off, b, x))
+        keep.append(g)
+    rc, res = N.pcg_ensemble_batch(jobs, **kw)
+    assert rc == 0
+    for (c, A, B, off, b, x), g, (status, stats) in zip(jobs, keep, res):
+        assert status == 0
+        xs = x.get()
+        for m, st, o in zip(g, stats, off[:-1]):
+            st1, x1 = _single(ctx0, m, **kw)
+            assert st["niter"] == st1["niter"]
+            assert np.array_equal(xs[o:o + m.n].view(np.uint64), x1.view(np.uint64))
+    for c, A, B, off, b, x in jobs:
+        B.close()
+        A.close()
+    del b, x
+    for c, *_ in jobs:
+        c.close()
+    for m in ms:
+        m.close()
+    ctx0.close()
