@@ -77,6 +77,9 @@ int mlp_ctx_sync(mlp_ctx ctx);
 int mlp_ctx_stream(mlp_ctx ctx, void** hip_stream_out);
 
 /* ---- device memory helpers (for callers without their own allocator) ---- */
+/* Device memory on ctx's device.  Blocks the library releases (these, matrices', preconditioners' and
+ * contexts' own) are kept per device and size class, up to 8 GiB per device, and handed out again; a release
+ * first waits for the device, as hipFree does.  mlp_dev_free takes blocks of mlp_dev_alloc only. */
 int mlp_dev_alloc(mlp_ctx ctx, size_t bytes, void** d_out);
 int mlp_dev_free(mlp_ctx ctx, void* d_ptr);
 int mlp_copy_h2d(mlp_ctx ctx, void* d_dst, const void* src, size_t bytes);

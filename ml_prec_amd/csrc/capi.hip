@@ -26,6 +26,7 @@
 #include <mutex>
 #include <type_traits>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 namespace mlp {
@@ -41,20 +42,109 @@ inline bool host_is_one(double a) { return std::fabs(a - 1.0) <= 2.2204460492503
 // ---------------------------------------------------------------------------
 // device memory
 // ---------------------------------------------------------------------------
+// Device memory released by the library is kept per device and size class and handed out again, so that
+// repeated set-ups (the solves of an adjoint loop, finite-difference checks, ensembles) do not pay hipMalloc
+// and hipFree (each hipFree waits for the whole device).  A release waits for its devices once, which is what
+// hipFree did, so no kernel still running can see its memory reused.  At most kCap bytes per device are kept.
+struct MemCache {
+    static constexpr size_t kCap = size_t(8) << 30;
+    std::mutex m;
+    std::unordered_map<uint64_t, std::vector<void*>> free;   // key: device << 56 | size class
+    std::unordered_map<void*, std::pair<size_t, int>> live;  // blocks handed out by mlp_dev_alloc
+    size_t cached[64] = {};
+    static size_t size_class(size_t bytes) {
+        if (bytes <= 4096) return 4096;
+        size_t e = 63 - size_t(__builtin_clzll(bytes));
+        const size_t step = size_t(1) << (e - 2);                 // four classes per octave (at most 25% waste)
+        return (bytes + step - 1) & ~(step - 1);
+    }
+    static uint64_t key(int dev, size_t cls) { return (uint64_t(dev) << 56) | cls; }
+};
+inline MemCache& mem_cache() {
+    static MemCache c;
+    return c;
+}
+inline int current_device() {
+    int d = 0;
+    MLP_HIP_CHECK(hipGetDevice(&d));
+    return d;
+}
+// a block of at least `bytes` on the current device
+inline void* cache_alloc(size_t bytes, size_t* cls_out) {
+    MemCache& c = mem_cache();
+    const int dev = current_device();
+    const size_t cls = MemCache::size_class(std::max<size_t>(bytes, 1));
+    *cls_out = cls;
+    {
+        std::lock_guard<std::mutex> hold(c.m);
+        auto it = c.free.find(MemCache::key(dev, cls));
+        if (it != c.free.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            if (dev < 64) c.cached[dev] -= cls;
+            return p;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, cls);
+    if (e == hipErrorOutOfMemory) {                     // give the kept blocks of this device back, then retry
+        (void)hipGetLastError();
+        std::lock_guard<std::mutex> hold(c.m);
+        for (auto& kv : c.free)
+            if (int(kv.first >> 56) == dev) {
+                for (void* q : kv.second) (void)hipFree(q);
+                kv.second.clear();
+            }
+        if (dev < 64) c.cached[dev] = 0;
+        e = hipMalloc(&p, cls);
+    }
+    MLP_HIP_CHECK(e);
+    return p;
+}
+struct CachedBlock {
+    void* p;
+    size_t cls;
+    int dev;
+};
+// blocks back into the cache, after the work on their devices has finished
+inline void cache_release(std::vector<CachedBlock>& blocks) {
+    if (blocks.empty()) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    uint64_t synced = 0;
+    for (const CachedBlock& b : blocks)
+        if (b.dev < 64 && !(synced >> b.dev & 1)) {
+            if (b.dev != cur) (void)hipSetDevice(b.dev);
+            (void)hipDeviceSynchronize();
+            synced |= uint64_t(1) << b.dev;
+        }
+    (void)hipSetDevice(cur);
+    MemCache& c = mem_cache();
+    std::lock_guard<std::mutex> hold(c.m);
+    for (const CachedBlock& b : blocks) {
+        if (b.dev >= 64 || c.cached[b.dev] + b.cls > MemCache::kCap) {
+            if (b.dev != cur) (void)hipSetDevice(b.dev);
+            (void)hipFree(b.p);
+            if (b.dev != cur) (void)hipSetDevice(cur);
+            continue;
+        }
+        c.free[MemCache::key(b.dev, b.cls)].push_back(b.p);
+        c.cached[b.dev] += b.cls;
+    }
+    blocks.clear();
+}
+
 struct DevMem {
-    std::vector<void*> ptrs;
+    std::vector<CachedBlock> ptrs;
     DevMem() = default;
     DevMem(const DevMem&) = delete;
     DevMem& operator=(const DevMem&) = delete;
     ~DevMem() { release(); }
-    void release() {
-        for (void* p : ptrs) (void)hipFree(p);
-        ptrs.clear();
-    }
+    void release() { cache_release(ptrs); }
     template <class T> T* alloc(size_t n) {
-        void* p = nullptr;
-        MLP_HIP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
-        ptrs.push_back(p);
+        size_t cls = 0;
+        void* p = cache_alloc(std::max<size_t>(n, 1) * sizeof(T), &cls);
+        ptrs.push_back({p, cls, current_device()});
         return static_cast<T*>(p);
     }
     // synchronous: the host vectors are usually temporaries
@@ -1772,7 +1862,14 @@ int mlp_dev_alloc(mlp_ctx ctx, size_t bytes, void** d_out) {
     MLP_API_BEGIN
     require(ctx && d_out, "null argument");
     use_device(ctx);
-    MLP_HIP_CHECK(hipMalloc(d_out, std::max<size_t>(bytes, 1)));
+    size_t cls = 0;
+    void* p = cache_alloc(std::max<size_t>(bytes, 1), &cls);
+    {
+        MemCache& c = mem_cache();
+        std::lock_guard<std::mutex> hold(c.m);
+        c.live[p] = {cls, ctx->device};
+    }
+    *d_out = p;
     MLP_API_END
 }
 
@@ -1780,18 +1877,71 @@ int mlp_dev_free(mlp_ctx ctx, void* p) {
     MLP_API_BEGIN
     require(ctx, "null context");
     use_device(ctx);
-    if (p) MLP_HIP_CHECK(hipFree(p));
+    if (p) {
+        MemCache& c = mem_cache();
+        std::vector<CachedBlock> one;
+        {
+            std::lock_guard<std::mutex> hold(c.m);
+            auto it = c.live.find(p);
+            require(it != c.live.end(), "not a block of mlp_dev_alloc");
+            one.push_back({p, it->second.first, it->second.second});
+            c.live.erase(it);
+        }
+        cache_release(one);
+    }
     MLP_API_END
 }
+
+}  // extern "C"
+
+namespace mlp {
+namespace {
+// A copy between pageable host memory and the device: small ones directly, large ones in chunks through the
+// device's pinned staging buffer, the host side of each chunk spread over the host threads.
+void staged_copy(mlp_ctx ctx, void* dst, const void* src, size_t bytes, bool to_device) {
+    constexpr size_t kDirect = size_t(1) << 20, kChunk = size_t(32) << 20;
+    hipStream_t s = ctx->stream;
+    if (bytes < kDirect) {
+        MLP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, s));
+        MLP_HIP_CHECK(hipStreamSynchronize(s));
+        return;
+    }
+    PinnedStage& st = pinned_stage(ctx->device);
+    std::lock_guard<std::mutex> hold(st.m);
+    const size_t want = std::min(bytes, kChunk);
+    if (st.n < want) {
+        if (st.p) MLP_HIP_CHECK(hipHostFree(st.p));
+        st.p = nullptr;
+        st.n = 0;
+        MLP_HIP_CHECK(hipHostMalloc((void**)&st.p, want, hipHostMallocDefault));
+        st.n = want;
+    }
+    char* pin = st.p;
+    for (size_t off = 0; off < bytes; off += kChunk) {
+        const size_t len = std::min(kChunk, bytes - off);
+        if (to_device) {
+            const char* from = static_cast<const char*>(src) + off;
+            parallel_for(len, size_t(1) << 20, [&](uint64_t a, uint64_t b) { std::memcpy(pin + a, from + a, b - a); });
+            MLP_HIP_CHECK(hipMemcpyAsync(static_cast<char*>(dst) + off, pin, len, hipMemcpyHostToDevice, s));
+            MLP_HIP_CHECK(hipStreamSynchronize(s));
+        } else {
+            MLP_HIP_CHECK(hipMemcpyAsync(pin, static_cast<const char*>(src) + off, len, hipMemcpyDeviceToHost, s));
+            MLP_HIP_CHECK(hipStreamSynchronize(s));
+            char* to = static_cast<char*>(dst) + off;
+            parallel_for(len, size_t(1) << 20, [&](uint64_t a, uint64_t b) { std::memcpy(to + a, pin + a, b - a); });
+        }
+    }
+}
+}  // namespace
+}  // namespace mlp
+
+extern "C" {
 
 int mlp_copy_h2d(mlp_ctx ctx, void* dst, const void* src, size_t bytes) {
     MLP_API_BEGIN
     require(ctx, "null context");
     use_device(ctx);
-    if (bytes) {
-        MLP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-        MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    }
+    if (bytes) staged_copy(ctx, dst, src, bytes, true);
     MLP_API_END
 }
 
@@ -1799,10 +1949,7 @@ int mlp_copy_d2h(mlp_ctx ctx, void* dst, const void* src, size_t bytes) {
     MLP_API_BEGIN
     require(ctx, "null context");
     use_device(ctx);
-    if (bytes) {
-        MLP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-        MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    }
+    if (bytes) staged_copy(ctx, dst, src, bytes, false);
     MLP_API_END
 }
 

@@ -143,11 +143,9 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P, const Bloc
     P.Udiag.resize(M);
     P.Uptr.assign(M + 1, 0);
     std::vector<uint32_t> local(K, kNoNode);
-    // the rows of `in` (and, for the U list, only those < M) ordered by key,
-    // equal keys in their order in `in` (a stable sort, done by counting)
-    std::vector<uint32_t> count;
-    auto by_depth = [&count](const uint32_t* b, const uint32_t* e, const std::vector<uint32_t>& key, uint32_t below,
-                             std::vector<uint32_t>& out) {
+    // rows [b, e) below `below` ordered by key, equal keys in row order (a counting sort into out)
+    auto by_depth = [](const uint32_t* b, const uint32_t* e, const std::vector<uint32_t>& key, uint32_t below,
+                       std::vector<uint32_t>& out, std::vector<uint32_t>& count) {
         uint32_t kmax = 0, m = 0;
         for (const uint32_t* r = b; r < e; r++)
             if (*r < below) { kmax = std::max(kmax, key[*r]); m++; }
@@ -159,67 +157,117 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P, const Bloc
         for (const uint32_t* r = b; r < e; r++)
             if (*r < below) out[count[key[*r]]++] = *r;
     };
-    std::vector<uint32_t> lo, uo;
-    uint32_t pos = 0, upos = 0;
+    // The blocks are independent: pass 1 counts each block's depth levels and entries, pass 2 writes every
+    // block's ranges at the offsets those counts give (the arrays the sequential append would build).
+    struct Counts { uint32_t lvL = 0, entL = 0, nU = 0, lvU = 0, entU = 0; };
+    std::vector<Counts> bc(nblk);
+    parallel_for(nblk, 256, [&](uint64_t b0, uint64_t b1) {
+        std::vector<uint32_t> lo, uo, count;
+        for (uint64_t b = b0; b < b1; b++) {
+            const uint32_t* Rb = rows_all.data() + roff[b];
+            const uint32_t* Re = rows_all.data() + roff[b + 1];
+            by_depth(Rb, Re, dL, K, lo, count);
+            by_depth(Rb, Re, dU, M, uo, count);
+            Counts& c = bc[b];
+            for (size_t k = 0; k < lo.size(); k++) {
+                if (k + 1 == lo.size() || dL[lo[k + 1]] != dL[lo[k]]) c.lvL++;
+                c.entL += lv.L_ia[lo[k] + 1] - lv.L_ia[lo[k]];
+            }
+            c.nU = uint32_t(uo.size());
+            for (size_t k = 0; k < uo.size(); k++) {
+                if (k + 1 == uo.size() || dU[uo[k + 1]] != dU[uo[k]]) c.lvU++;
+                c.entU += lv.U_ia[uo[k] + 1] - lv.U_ia[uo[k]] - 1;
+            }
+        }
+    });
+    std::vector<uint32_t> oLv(size_t(nblk) + 1, 0), oLe(size_t(nblk) + 1, 0), oU(size_t(nblk) + 1, 0),
+        oUv(size_t(nblk) + 1, 0), oUe(size_t(nblk) + 1, 0);
     for (uint32_t b = 0; b < nblk; b++) {
-        const uint32_t* Rb = rows_all.data() + roff[b];
-        const uint32_t* Re = rows_all.data() + roff[b + 1];
-        const uint32_t base = pos;
-        by_depth(Rb, Re, dL, K, lo);
-        by_depth(Rb, Re, dU, M, uo);
-        for (uint32_t k = 0; k < lo.size(); k++) local[lo[k]] = k;
-        uint32_t* B = &P.blk[kBlkFields * b];
-        B[0] = base;
-        B[1] = uint32_t(lo.size());
-        B[2] = uint32_t(P.lvlL.size());
-        B[6] = uint32_t(P.Lloc.size());
-        for (uint32_t k = 0; k < lo.size(); k++) {
-            const uint32_t i = lo[k];
-            P.Ldep[base + k] = uint32_t(P.lvlL.size()) - B[2];
-            if (k + 1 == lo.size() || dL[lo[k + 1]] != dL[i]) P.lvlL.push_back(k + 1);
-            P.qrow[base + k] = lv.map[i];
-            P.out[base + k] = i >= M ? (kGlobalRef | (i - M)) : k;
-            for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
-                P.Lloc.push_back(local[lv.L_ja[p]]);
-                P.Lval.push_back(lv.L_a[p]);
+        oLv[b + 1] = oLv[b] + bc[b].lvL;
+        oLe[b + 1] = oLe[b] + bc[b].entL;
+        oU[b + 1] = oU[b] + bc[b].nU;
+        oUv[b + 1] = oUv[b] + bc[b].lvU;
+        oUe[b + 1] = oUe[b] + bc[b].entU;
+    }
+    P.lvlL.resize(oLv[nblk]);
+    P.Lloc.resize(oLe[nblk]);
+    P.Lval.resize(oLe[nblk]);
+    P.lvlU.resize(oUv[nblk]);
+    P.Ucol.resize(oUe[nblk]);
+    P.Uval.resize(oUe[nblk]);
+    std::vector<char> glob(nblk, 0);
+    std::vector<uint32_t> nlev(nblk, 0);
+    parallel_for(nblk, 256, [&](uint64_t b0, uint64_t b1) {
+        std::vector<uint32_t> lo, uo, count;
+        for (uint64_t b = b0; b < b1; b++) {
+            const uint32_t* Rb = rows_all.data() + roff[b];
+            const uint32_t* Re = rows_all.data() + roff[b + 1];
+            const uint32_t base = roff[b];
+            by_depth(Rb, Re, dL, K, lo, count);
+            by_depth(Rb, Re, dU, M, uo, count);
+            for (uint32_t k = 0; k < lo.size(); k++) local[lo[k]] = k;
+            uint32_t* B = &P.blk[kBlkFields * b];
+            B[0] = base;
+            B[1] = uint32_t(lo.size());
+            B[2] = oLv[b];
+            B[6] = oLe[b];
+            uint32_t lvl = oLv[b], le = oLe[b];
+            for (uint32_t k = 0; k < lo.size(); k++) {
+                const uint32_t i = lo[k];
+                P.Ldep[base + k] = lvl - B[2];
+                if (k + 1 == lo.size() || dL[lo[k + 1]] != dL[i]) P.lvlL[lvl++] = k + 1;
+                P.qrow[base + k] = lv.map[i];
+                P.out[base + k] = i >= M ? (kGlobalRef | (i - M)) : k;
+                for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
+                    P.Lloc[le] = local[lv.L_ja[p]];
+                    P.Lval[le] = lv.L_a[p];
+                    le++;
+                }
+                P.Lptr[base + k + 1] = le;
             }
-            P.Lptr[base + k + 1] = uint32_t(P.Lloc.size());
-        }
-        B[3] = uint32_t(P.lvlL.size());
-        B[7] = uint32_t(P.Lloc.size());
-        B[4] = uint32_t(P.lvlU.size());
-        B[8] = uint32_t(P.Ucol.size());
-        B[10] = upos;
-        B[11] = uint32_t(uo.size());
-        for (uint32_t k = 0; k < uo.size(); k++) {
-            const uint32_t i = uo[k];
-            P.Udep[upos + k] = uint32_t(P.lvlU.size()) - B[4];
-            if (k + 1 == uo.size() || dU[uo[k + 1]] != dU[i]) P.lvlU.push_back(k + 1);
-            P.uloc[upos + k] = local[i];
-            P.uxidx[upos + k] = lv.map[i];
-            P.Udiag[upos + k] = lv.U_a[lv.U_ia[i]];
-            for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) {
-                uint32_t j = lv.U_ja[p];
-                P.Ucol.push_back(j < M ? local[j] : (kGlobalRef | (j - M)));
-                P.Uval.push_back(lv.U_a[p]);
+            B[3] = lvl;
+            B[7] = le;
+            B[4] = oUv[b];
+            B[8] = oUe[b];
+            B[10] = oU[b];
+            B[11] = uint32_t(uo.size());
+            uint32_t ulv = oUv[b], ue = oUe[b];
+            const uint32_t upos = oU[b];
+            for (uint32_t k = 0; k < uo.size(); k++) {
+                const uint32_t i = uo[k];
+                P.Udep[upos + k] = ulv - B[4];
+                if (k + 1 == uo.size() || dU[uo[k + 1]] != dU[i]) P.lvlU[ulv++] = k + 1;
+                P.uloc[upos + k] = local[i];
+                P.uxidx[upos + k] = lv.map[i];
+                P.Udiag[upos + k] = lv.U_a[lv.U_ia[i]];
+                for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) {
+                    const uint32_t j = lv.U_ja[p];
+                    P.Ucol[ue] = j < M ? local[j] : (kGlobalRef | (j - M));
+                    P.Uval[ue] = lv.U_a[p];
+                    ue++;
+                }
+                P.Uptr[upos + k + 1] = ue;
             }
-            P.Uptr[upos + k + 1] = uint32_t(P.Ucol.size());
+            B[5] = ulv;
+            B[9] = ue;
+            nlev[b] = std::max(B[3] - B[2], B[5] - B[4]);
+            if (fits && !fits(uint32_t(lo.size()), B[7] - B[6], B[9] - B[8], uint32_t(uo.size()), nlev[b])) {
+                B[1] |= kGlobalBlock;
+                glob[b] = 1;
+            }
         }
-        B[5] = uint32_t(P.lvlU.size());
-        B[9] = uint32_t(P.Ucol.size());
-        const uint32_t nlev = std::max(B[3] - B[2], B[5] - B[4]);
-        if (fits && !fits(uint32_t(lo.size()), B[7] - B[6], B[9] - B[8], uint32_t(uo.size()), nlev)) {
-            B[1] |= kGlobalBlock;
+    });
+    for (uint32_t b = 0; b < nblk; b++) {
+        const uint32_t* B = &P.blk[kBlkFields * b];
+        if (glob[b]) {
             P.has_global = true;
-        } else {
-            P.max_nodes = std::max<uint32_t>(P.max_nodes, uint32_t(lo.size()));
-            P.max_Unodes = std::max<uint32_t>(P.max_Unodes, uint32_t(uo.size()));
-            P.max_Lent = std::max<uint32_t>(P.max_Lent, B[7] - B[6]);
-            P.max_Uent = std::max<uint32_t>(P.max_Uent, B[9] - B[8]);
-            P.max_levels = std::max<uint32_t>(P.max_levels, nlev);
+            continue;
         }
-        pos += uint32_t(lo.size());
-        upos += uint32_t(uo.size());
+        P.max_nodes = std::max<uint32_t>(P.max_nodes, B[1]);
+        P.max_Unodes = std::max<uint32_t>(P.max_Unodes, B[11]);
+        P.max_Lent = std::max<uint32_t>(P.max_Lent, B[7] - B[6]);
+        P.max_Uent = std::max<uint32_t>(P.max_Uent, B[9] - B[8]);
+        P.max_levels = std::max<uint32_t>(P.max_levels, nlev[b]);
     }
     finish_order(lv, P);
     if (P.Lloc.empty()) { P.Lloc.push_back(0); P.Lval.push_back(0.0); }
@@ -341,14 +389,10 @@ bool plan_pair(const LevelArrays& la, const LevelArrays& lb, uint32_t cap, PairP
     PairProgram P;
     P.N = N; P.M = M; P.Md = Md; P.n = n; P.nblk = nblk;
     P.blk.assign(size_t(kPairFields) * nblk, 0);
-    P.Lptr.push_back(0);
-    P.Uptr.push_back(0);
-    P.Lptr2.push_back(0);
-    P.Uptr2.push_back(0);
     std::vector<uint32_t> local(K, kNoNode), local2(K, kNoNode);
     // rows of [b, e) passing `keep`, ordered by key, equal keys in their order (a counting sort into o)
-    std::vector<uint32_t> kcount;
-    auto by_key = [&kcount](const uint32_t* b, const uint32_t* e, auto key, auto keep, std::vector<uint32_t>& o) {
+    auto by_key = [](const uint32_t* b, const uint32_t* e, auto key, auto keep, std::vector<uint32_t>& o,
+                     std::vector<uint32_t>& kcount) {
         uint32_t kmax = 0, m = 0;
         for (const uint32_t* r = b; r < e; r++)
             if (keep(*r)) { kmax = std::max(kmax, key(*r)); m++; }
@@ -360,85 +404,175 @@ bool plan_pair(const LevelArrays& la, const LevelArrays& lb, uint32_t cap, PairP
         for (const uint32_t* r = b; r < e; r++)
             if (keep(*r)) o[kcount[key(*r)]++] = *r;
     };
-    std::vector<uint32_t> lo, uo, ko, u2o;
     auto levels_of = [](const std::vector<uint32_t>& o, auto key) {
         uint32_t nl = 0;
         for (size_t k = 0; k < o.size(); k++) nl = std::max(nl, key(o[k]) + 1);
         return nl;
     };
+    auto kL = [&](uint32_t i) { return dL[i]; };
+    auto kU = [&](uint32_t i) { return dU[i]; };
+    // kept nodes, by level-(l+1) L depth (rows isolated there at depth 0)
+    auto key2 = [&](uint32_t i) { const uint32_t q = pos2[i - M]; return q < M2 ? dL2[q] : 0u; };
+    auto keyU2 = [&](uint32_t i) { return dU2[pos2[i - M]]; };
+    auto all = [](uint32_t) { return true; };
+    auto elim = [&](uint32_t i) { return i < M; };
+    auto kept = [&](uint32_t i) { return i >= M; };
+    auto kept_elim2 = [&](uint32_t i) { return i >= M && pos2[i - M] < M2; };
+    // The blocks are independent: pass 1 counts each block's rows and entries of every list, pass 2 writes every
+    // block's ranges at the offsets those counts give (the arrays the sequential append would build).
+    struct Counts { uint32_t nU = 0, nK = 0, nU2 = 0, eL = 0, eU = 0, eL2 = 0, eU2 = 0; };
+    std::vector<Counts> bc(nblk);
+    parallel_for(nblk, 256, [&](uint64_t b0, uint64_t b1) {
+        std::vector<uint32_t> uo, ko, u2o, kc;
+        for (uint64_t b = b0; b < b1; b++) {
+            const uint32_t* Rb = rows_all.data() + roff[b];
+            const uint32_t* Re = rows_all.data() + roff[b + 1];
+            Counts& c = bc[b];
+            for (const uint32_t* r = Rb; r < Re; r++) c.eL += la.L_ia[*r + 1] - la.L_ia[*r];
+            by_key(Rb, Re, kU, elim, uo, kc);
+            by_key(Rb, Re, key2, kept, ko, kc);
+            by_key(Rb, Re, keyU2, kept_elim2, u2o, kc);
+            c.nU = uint32_t(uo.size());
+            c.nK = uint32_t(ko.size());
+            c.nU2 = uint32_t(u2o.size());
+            for (uint32_t i : uo) c.eU += la.U_ia[i + 1] - la.U_ia[i] - 1;
+            for (uint32_t i : ko) {
+                const uint32_t q = pos2[i - M];
+                if (q < M2) c.eL2 += lb.L_ia[q + 1] - lb.L_ia[q];
+            }
+            for (uint32_t i : u2o) {
+                const uint32_t q = pos2[i - M];
+                c.eU2 += lb.U_ia[q + 1] - lb.U_ia[q] - 1;
+            }
+        }
+    });
+    std::vector<uint32_t> oU(size_t(nblk) + 1, 0), oK(size_t(nblk) + 1, 0), oU2(size_t(nblk) + 1, 0),
+        oeL(size_t(nblk) + 1, 0), oeU(size_t(nblk) + 1, 0), oeL2(size_t(nblk) + 1, 0), oeU2(size_t(nblk) + 1, 0);
     for (uint32_t b = 0; b < nblk; b++) {
-        const uint32_t* Rb = rows_all.data() + roff[b];
-        const uint32_t* Re = rows_all.data() + roff[b + 1];
-        by_key(Rb, Re, [&](uint32_t i) { return dL[i]; }, [](uint32_t) { return true; }, lo);
-        by_key(Rb, Re, [&](uint32_t i) { return dU[i]; }, [&](uint32_t i) { return i < M; }, uo);
-        // kept nodes, by level-(l+1) L depth (rows isolated there at depth 0)
-        auto key2 = [&](uint32_t i) { const uint32_t q = pos2[i - M]; return q < M2 ? dL2[q] : 0u; };
-        by_key(Rb, Re, key2, [&](uint32_t i) { return i >= M; }, ko);
-        auto keyU2 = [&](uint32_t i) { return dU2[pos2[i - M]]; };
-        by_key(Rb, Re, keyU2, [&](uint32_t i) { return i >= M && pos2[i - M] < M2; }, u2o);
-        for (uint32_t k = 0; k < lo.size(); k++) local[lo[k]] = k;
-        for (uint32_t k = 0; k < ko.size(); k++) local2[ko[k]] = k;
-        uint32_t* B = &P.blk[size_t(kPairFields) * b];
-        B[0] = uint32_t(P.Ldep.size());
-        B[1] = uint32_t(lo.size());
-        B[2] = levels_of(lo, [&](uint32_t i) { return dL[i]; });
-        B[3] = uint32_t(P.Udep.size());
-        B[4] = uint32_t(uo.size());
-        B[5] = levels_of(uo, [&](uint32_t i) { return dU[i]; });
-        B[6] = uint32_t(P.src2.size());
-        B[7] = uint32_t(ko.size());
-        B[8] = levels_of(ko, key2);
-        B[9] = uint32_t(P.Udep2.size());
-        B[10] = uint32_t(u2o.size());
-        B[11] = levels_of(u2o, keyU2);
-        for (uint32_t i : lo) {
-            P.qrow.push_back(la.map[i]);
-            P.Ldep.push_back(dL[i]);
-            P.kept2.push_back(i >= M ? local2[i] : kNoRef);
-            for (uint32_t p = la.L_ia[i]; p < la.L_ia[i + 1]; p++) {
-                P.Lloc.push_back(local[la.L_ja[p]]);
-                P.Lval.push_back(la.L_a[p]);
-            }
-            P.Lptr.push_back(uint32_t(P.Lloc.size()));
-        }
-        for (uint32_t i : uo) {
-            P.Udep.push_back(dU[i]);
-            P.uloc.push_back(local[i]);
-            P.uxidx.push_back(la.map[i]);
-            P.Udiag.push_back(la.U_a[la.U_ia[i]]);
-            for (uint32_t p = la.U_ia[i] + 1; p < la.U_ia[i + 1]; p++) {
-                const uint32_t j = la.U_ja[p];
-                P.Ucol.push_back(j < M ? local[j] : (kGlobalRef | local2[j]));
-                P.Uval.push_back(la.U_a[p]);
-            }
-            P.Uptr.push_back(uint32_t(P.Ucol.size()));
-        }
-        for (uint32_t i : ko) {
-            const uint32_t r = i - M, q = pos2[r];
-            P.qrow2.push_back(r);
-            P.src2.push_back(local[i]);
-            P.row2.push_back(q < M2 ? r : (r | kIso2));
-            P.xl2.push_back(la.map[i]);
-            P.Ldep2.push_back(q < M2 ? dL2[q] : 0u);
-            P.d2.push_back(q < M2 ? 0.0 : lb.dval[q - M2]);
-            if (q < M2)
-                for (uint32_t p = lb.L_ia[q]; p < lb.L_ia[q + 1]; p++) {
-                    P.Lloc2.push_back(local2[M + lb.map[lb.L_ja[p]]]);
-                    P.Lval2.push_back(lb.L_a[p]);
+        oU[b + 1] = oU[b] + bc[b].nU;
+        oK[b + 1] = oK[b] + bc[b].nK;
+        oU2[b + 1] = oU2[b] + bc[b].nU2;
+        oeL[b + 1] = oeL[b] + bc[b].eL;
+        oeU[b + 1] = oeU[b] + bc[b].eU;
+        oeL2[b + 1] = oeL2[b] + bc[b].eL2;
+        oeU2[b + 1] = oeU2[b] + bc[b].eU2;
+    }
+    const uint32_t nodes = roff[nblk];
+    P.qrow.resize(nodes);
+    P.Ldep.resize(nodes);
+    P.kept2.resize(nodes);
+    P.Lptr.resize(size_t(nodes) + 1);
+    P.Lloc.resize(oeL[nblk]);
+    P.Lval.resize(oeL[nblk]);
+    P.Udep.resize(oU[nblk]);
+    P.uloc.resize(oU[nblk]);
+    P.uxidx.resize(oU[nblk]);
+    P.Udiag.resize(oU[nblk]);
+    P.Uptr.resize(size_t(oU[nblk]) + 1);
+    P.Ucol.resize(oeU[nblk]);
+    P.Uval.resize(oeU[nblk]);
+    P.qrow2.resize(oK[nblk]);
+    P.src2.resize(oK[nblk]);
+    P.row2.resize(oK[nblk]);
+    P.xl2.resize(oK[nblk]);
+    P.Ldep2.resize(oK[nblk]);
+    P.d2.resize(oK[nblk]);
+    P.Lptr2.resize(size_t(oK[nblk]) + 1);
+    P.Lloc2.resize(oeL2[nblk]);
+    P.Lval2.resize(oeL2[nblk]);
+    P.Udep2.resize(oU2[nblk]);
+    P.uloc2.resize(oU2[nblk]);
+    P.Udiag2.resize(oU2[nblk]);
+    P.Uptr2.resize(size_t(oU2[nblk]) + 1);
+    P.Ucol2.resize(oeU2[nblk]);
+    P.Uval2.resize(oeU2[nblk]);
+    parallel_for(nblk, 256, [&](uint64_t b0, uint64_t b1) {
+        std::vector<uint32_t> lo, uo, ko, u2o, kc;
+        for (uint64_t b = b0; b < b1; b++) {
+            const uint32_t* Rb = rows_all.data() + roff[b];
+            const uint32_t* Re = rows_all.data() + roff[b + 1];
+            by_key(Rb, Re, kL, all, lo, kc);
+            by_key(Rb, Re, kU, elim, uo, kc);
+            by_key(Rb, Re, key2, kept, ko, kc);
+            by_key(Rb, Re, keyU2, kept_elim2, u2o, kc);
+            for (uint32_t k = 0; k < lo.size(); k++) local[lo[k]] = k;
+            for (uint32_t k = 0; k < ko.size(); k++) local2[ko[k]] = k;
+            uint32_t* B = &P.blk[size_t(kPairFields) * b];
+            B[0] = roff[b];
+            B[1] = uint32_t(lo.size());
+            B[2] = levels_of(lo, kL);
+            B[3] = oU[b];
+            B[4] = uint32_t(uo.size());
+            B[5] = levels_of(uo, kU);
+            B[6] = oK[b];
+            B[7] = uint32_t(ko.size());
+            B[8] = levels_of(ko, key2);
+            B[9] = oU2[b];
+            B[10] = uint32_t(u2o.size());
+            B[11] = levels_of(u2o, keyU2);
+            uint32_t q0 = roff[b], e = oeL[b];
+            for (uint32_t i : lo) {
+                P.qrow[q0] = la.map[i];
+                P.Ldep[q0] = dL[i];
+                P.kept2[q0] = i >= M ? local2[i] : kNoRef;
+                for (uint32_t p = la.L_ia[i]; p < la.L_ia[i + 1]; p++) {
+                    P.Lloc[e] = local[la.L_ja[p]];
+                    P.Lval[e] = la.L_a[p];
+                    e++;
                 }
-            P.Lptr2.push_back(uint32_t(P.Lloc2.size()));
-        }
-        for (uint32_t i : u2o) {
-            const uint32_t q = pos2[i - M];
-            P.Udep2.push_back(dU2[q]);
-            P.uloc2.push_back(local2[i]);
-            P.Udiag2.push_back(lb.U_a[lb.U_ia[q]]);
-            for (uint32_t p = lb.U_ia[q] + 1; p < lb.U_ia[q + 1]; p++) {
-                P.Ucol2.push_back(local2[M + lb.map[lb.U_ja[p]]]);
-                P.Uval2.push_back(lb.U_a[p]);
+                P.Lptr[++q0] = e;
             }
-            P.Uptr2.push_back(uint32_t(P.Ucol2.size()));
+            q0 = oU[b];
+            e = oeU[b];
+            for (uint32_t i : uo) {
+                P.Udep[q0] = dU[i];
+                P.uloc[q0] = local[i];
+                P.uxidx[q0] = la.map[i];
+                P.Udiag[q0] = la.U_a[la.U_ia[i]];
+                for (uint32_t p = la.U_ia[i] + 1; p < la.U_ia[i + 1]; p++) {
+                    const uint32_t j = la.U_ja[p];
+                    P.Ucol[e] = j < M ? local[j] : (kGlobalRef | local2[j]);
+                    P.Uval[e] = la.U_a[p];
+                    e++;
+                }
+                P.Uptr[++q0] = e;
+            }
+            q0 = oK[b];
+            e = oeL2[b];
+            for (uint32_t i : ko) {
+                const uint32_t r = i - M, q = pos2[r];
+                P.qrow2[q0] = r;
+                P.src2[q0] = local[i];
+                P.row2[q0] = q < M2 ? r : (r | kIso2);
+                P.xl2[q0] = la.map[i];
+                P.Ldep2[q0] = q < M2 ? dL2[q] : 0u;
+                P.d2[q0] = q < M2 ? 0.0 : lb.dval[q - M2];
+                if (q < M2)
+                    for (uint32_t p = lb.L_ia[q]; p < lb.L_ia[q + 1]; p++) {
+                        P.Lloc2[e] = local2[M + lb.map[lb.L_ja[p]]];
+                        P.Lval2[e] = lb.L_a[p];
+                        e++;
+                    }
+                P.Lptr2[++q0] = e;
+            }
+            q0 = oU2[b];
+            e = oeU2[b];
+            for (uint32_t i : u2o) {
+                const uint32_t q = pos2[i - M];
+                P.Udep2[q0] = dU2[q];
+                P.uloc2[q0] = local2[i];
+                P.Udiag2[q0] = lb.U_a[lb.U_ia[q]];
+                for (uint32_t p = lb.U_ia[q] + 1; p < lb.U_ia[q + 1]; p++) {
+                    P.Ucol2[e] = local2[M + lb.map[lb.U_ja[p]]];
+                    P.Uval2[e] = lb.U_a[p];
+                    e++;
+                }
+                P.Uptr2[++q0] = e;
+            }
         }
+    });
+    for (uint32_t b = 0; b < nblk; b++) {
+        const uint32_t* B = &P.blk[size_t(kPairFields) * b];
         P.max_nodes = std::max<uint32_t>(P.max_nodes, B[1]);
         P.max_U = std::max<uint32_t>(P.max_U, B[4]);
         P.max_nodes2 = std::max<uint32_t>(P.max_nodes2, B[7]);
