@@ -1089,9 +1089,9 @@ def main():
             "us_per_apply": round(apply64_ms * 1e3, 2),
         },
         "roofline_spmv": {
-            "kernel": "k_spmv, FP64, 1.12M rows in the symmetric seven-point layout (diagonal and three lower values "
-                      "per row; upper values read from the neighbouring rows), launches rotating over 1 GiB of matrix "
-                      "copies so that the matrix streams from HBM",
+            "kernel": "k_spmv_pairs, FP64, 1.12M rows in the symmetric seven-point layout (diagonal and three lower "
+                      "values per row; upper values read from the neighbouring rows), two rows per thread with 16-byte "
+                      "loads, launches rotating over 1 GiB of matrix copies so that the matrix streams from HBM",
             "bound": "hbm",
             "achieved_dram": round((load_traffic() or 0) / (spmv_cold_ms * 1e-3) / 1e9, 1) if load_traffic() else None,
             "frac_dram": round((load_traffic() or 0) / (spmv_cold_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)

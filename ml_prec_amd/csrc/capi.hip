@@ -1812,6 +1812,24 @@ static void require(bool c, const char* msg) {
     if (!c) throw Error(msg, MLP_ERR_INVALID);
 }
 
+// y = A x (b null) or y = b - A x on a top-level FP64 matrix: two rows per
+// thread on the seven-point layout, one on slices.
+static void launch_rows(const SellView<double>& A, const double* b, const double* x, double* y, hipStream_t s) {
+    if (!A.n) return;
+    if (A.sd) {
+        const dim3 g(cdiv(A.n, 2 * kBS)), t(kBS);
+        const bool vec = pair_vec(A, uintptr_t(x) | uintptr_t(y) | uintptr_t(b));
+        if (b && vec) hipLaunchKernelGGL((k_spmv_pairs<true, true>), g, t, 0, s, A, b, x, y);
+        else if (b) hipLaunchKernelGGL((k_spmv_pairs<true, false>), g, t, 0, s, A, b, x, y);
+        else if (vec) hipLaunchKernelGGL((k_spmv_pairs<false, true>), g, t, 0, s, A, b, x, y);
+        else hipLaunchKernelGGL((k_spmv_pairs<false, false>), g, t, 0, s, A, b, x, y);
+        return;
+    }
+    const uint32_t* g = nullptr;
+    if (b) hipLaunchKernelGGL((k_residual<double, double>), dim3(cdiv(A.n, kBS)), dim3(kBS), 0, s, A, b, x, y, g);
+    else hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(A.n, kBS)), dim3(kBS), 0, s, A, x, y, g);
+}
+
 extern "C" {
 
 const char* mlp_last_error(void) { return mlp::g_last_error.c_str(); }
@@ -2114,8 +2132,7 @@ int mlp_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y) {
     MLP_API_BEGIN
     require(ctx && A, "null argument");
     use_device(ctx);
-    hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream, A->A.v, x, y,
-                       (const uint32_t*)nullptr);
+    launch_rows(A->A.v, nullptr, x, y, ctx->stream);
     MLP_HIP_CHECK(hipGetLastError());
     MLP_API_END
 }
@@ -2124,8 +2141,7 @@ int mlp_residual(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, doubl
     MLP_API_BEGIN
     require(ctx && A, "null argument");
     use_device(ctx);
-    hipLaunchKernelGGL((k_residual<double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream, A->A.v, b, x, r,
-                       (const uint32_t*)nullptr);
+    launch_rows(A->A.v, b, x, r, ctx->stream);
     MLP_HIP_CHECK(hipGetLastError());
     MLP_API_END
 }
@@ -2705,10 +2721,7 @@ int mlp_time_spmv(mlp_ctx ctx, mlp_mat A, const double* x, double* y, uint32_t r
             views.push_back(v);
         }
     }
-    auto run = [&](uint32_t r) {
-        hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream,
-                           views[r % views.size()], x, y, (const uint32_t*)nullptr);
-    };
+    auto run = [&](uint32_t r) { launch_rows(views[r % views.size()], nullptr, x, y, ctx->stream); };
     for (uint32_t r = 0; r < views.size(); r++) run(r);   // untimed pass over every copy
     hipEvent_t e0, e1;
     MLP_HIP_CHECK(hipEventCreate(&e0));
@@ -2829,8 +2842,7 @@ void write_state(mlp_ctx ctx, SolveState* st, const SolveState& s) {
 void write_state(mlp_ctx ctx, const SolveState& s) { write_state(ctx, ctx->st, s); }
 
 void launch_residual(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, double* r) {
-    hipLaunchKernelGGL((k_residual<double, double>), dim3(cdiv(A->n, kBS)), dim3(kBS), 0, ctx->stream, A->A.v, b, x, r,
-                       (const uint32_t*)nullptr);
+    launch_rows(A->A.v, b, x, r, ctx->stream);
 }
 
 double true_rel_res(mlp_ctx ctx, mlp_mat A, const double* b, const double* x, double* r, double init_norm) {
@@ -3981,8 +3993,7 @@ int mlp_gmres(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double* x, co
         uint32_t i = 0;
         for (; i < m - 1;) {
             B->impl->apply(v(i), z, nullptr);
-            hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(cdiv(n, kBS)), dim3(kBS), 0, s, A->A.v,
-                               (const double*)z, v(i + 1), (const uint32_t*)nullptr);
+            launch_rows(A->A.v, nullptr, z, v(i + 1), s);
             // Gram-Schmidt column and the new vector's norm stay on the device until
             // one copy brings h_0..h_i and (v_{i+1}, v_{i+1}) to the host
             hipLaunchKernelGGL(k_mdot, dim3(mdot_grid(n)), dim3(kBS), 0, s, n, i + 1, (const double*)ctx->V, ld,

@@ -222,6 +222,77 @@ __device__ __forceinline__ VT stencil_row(const SellView<VT>& A, const XT* __res
     return acc;
 }
 
+// Rows r and r+1 of the FP64 seven-point layout, each summed with stencil_row's
+// operations in its order.  VEC: each pair of neighbouring values is one
+// 16-byte load; that needs r, n, ssy and ssz even and 16-byte aligned arrays
+// (pair_vec), and then every load is in bounds for both rows.  Otherwise the
+// rows are summed one by one, row r+1 only when two.  x0, x1: x[r], x[r+1].
+struct RowPair {
+    double y0, y1, x0, x1;
+};
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+__host__ __device__ __forceinline__ bool pair_vec(const SellView<double>& A, uintptr_t vecs) {
+    const uintptr_t a = uintptr_t(A.sd) | uintptr_t(A.slx) | uintptr_t(A.sly) | uintptr_t(A.slz) | vecs;
+    return ((A.n | A.ssy | A.ssz) & 1u) == 0 && (a & 15u) == 0 && (uintptr_t(A.spres) & 1u) == 0;
+}
+
+template <bool VEC>
+__device__ __forceinline__ RowPair stencil_pair(const SellView<double>& A, const double* __restrict__ x, uint32_t r,
+                                                bool two) {
+    RowPair o{0.0, 0.0, 0.0, 0.0};
+    if constexpr (!VEC) {
+        o.y0 = stencil_row<double, double, false>(A, x, r, 1.0);
+        o.x0 = x[r];
+        if (two) {
+            o.y1 = stencil_row<double, double, false>(A, x, r + 1, 1.0);
+            o.x1 = x[r + 1];
+        }
+    } else {
+        const uint32_t n = A.n, sy = A.ssy, sz = A.ssz;
+        const dbl2 z2{0.0, 0.0};
+        auto l2 = [](const double* p) { return *(const dbl2*)p; };
+        const uint32_t m2 = *(const uint16_t*)(A.spres + r);
+        const dbl2 dd = l2(A.sd + r), vz = l2(A.slz + r), vy = l2(A.sly + r), vx = l2(A.slx + r);
+        const double vx2 = r + 2 < n ? A.slx[r + 2] : 0.0;
+        const dbl2 vyp = r + sy < n ? l2(A.sly + r + sy) : z2;
+        const dbl2 vzp = r + sz < n ? l2(A.slz + r + sz) : z2;
+        const dbl2 xc = l2(x + r);
+        const double xm1 = r >= 1 ? x[r - 1] : 0.0;
+        const double xp2 = r + 2 < n ? x[r + 2] : 0.0;
+        const dbl2 xmy = r >= sy ? l2(x + r - sy) : z2;
+        const dbl2 xpy = r + sy < n ? l2(x + r + sy) : z2;
+        const dbl2 xmz = r >= sz ? l2(x + r - sz) : z2;
+        const dbl2 xpz = r + sz < n ? l2(x + r + sz) : z2;
+        // row r: diagonal, then z-, y-, x-, x+, y+, z+
+        {
+            const uint32_t m = m2 & 0xffu;
+            const double v[6] = {vz.x, vy.x, vx.x, vx.y, vyp.x, vzp.x};
+            const double xv[6] = {xmz.x, xmy.x, xm1, xc.y, xpy.x, xpz.x};
+            double acc = 0.0;
+            acc += dd.x * xc.x;
+#pragma unroll
+            for (int t = 0; t < 6; t++)
+                if ((m >> t) & 1u) acc += v[t] * xv[t];
+            o.y0 = acc;
+        }
+        {
+            const uint32_t m = m2 >> 8;
+            const double v[6] = {vz.y, vy.y, vx.y, vx2, vyp.y, vzp.y};
+            const double xv[6] = {xmz.y, xmy.y, xc.x, xp2, xpy.y, xpz.y};
+            double acc = 0.0;
+            acc += dd.y * xc.y;
+#pragma unroll
+            for (int t = 0; t < 6; t++)
+                if ((m >> t) & 1u) acc += v[t] * xv[t];
+            o.y1 = acc;
+        }
+        o.x0 = xc.x;
+        o.x1 = xc.y;
+    }
+    return o;
+}
+
 // SCALE: the row of A (s x).  An FP64 row scales every entry first (the
 // reference's dscal then multiply, bit for bit); an FP32 row scales its sum.
 template <class VT, class XT, bool SCALE>
@@ -285,6 +356,27 @@ __global__ void __launch_bounds__(kBS) k_spmv(SellView<VT> A, const XT* __restri
     const uint32_t r = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (r >= A.n) return;
     y[r] = YT(mat_row<VT, XT>(A, x, r));
+}
+
+// y = A x and r = b - A x on the FP64 seven-point layout, two rows per thread
+// (grid: ceil(n / (2 kBS))); every row as k_spmv / k_residual compute it.
+// VEC: pair_vec holds for A, x, y and b (the host checks it).
+template <bool RES, bool VEC>
+__global__ void __launch_bounds__(kBS) k_spmv_pairs(SellView<double> A, const double* __restrict__ b,
+                                                    const double* __restrict__ x, double* __restrict__ y) {
+    const uint32_t r = 2 * (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    if (r >= A.n) return;
+    if constexpr (VEC) {
+        const RowPair o = stencil_pair<true>(A, x, r, true);       // n even: both rows exist
+        dbl2 v{o.y0, o.y1};
+        if (RES) v = *(const dbl2*)(b + r) - v;
+        *(dbl2*)(y + r) = v;
+    } else {
+        const bool two = r + 1 < A.n;
+        const RowPair o = stencil_pair<false>(A, x, r, two);
+        y[r] = RES ? b[r] - o.y0 : o.y0;
+        if (two) y[r + 1] = RES ? b[r + 1] - o.y1 : o.y1;
+    }
 }
 
 // r = b - A x   (residual, csrmatrix.cc:293-300)
@@ -1675,16 +1767,37 @@ __global__ void __launch_bounds__(kBS) k_pcg_spmv_dot(SellView<double> A, const 
     st += y; partials += size_t(y) * bt.part;
     if (st->done) return;
     double s = 0.0;
-    // grid-stride over 256-row tiles, XCD-grouped within each sweep of the grid
+    // grid-stride over 512-row tiles, XCD-grouped within each sweep of the grid;
+    // two rows per thread, each thread adding its rows' products in row order
+    // (the same order in both matrix layouts)
     const uint32_t len = SEG ? sp.len : A.n, r0 = SEG ? uint32_t(sp.base) : 0u;
-    const uint32_t tiles = (len + blockDim.x - 1) / blockDim.x;
+    const uint32_t tiles = (len + 2 * blockDim.x - 1) / (2 * blockDim.x);
+    const bool vec = A.sd && (r0 & 1u) == 0 && pair_vec(A, uintptr_t(p) | uintptr_t(Ap));
     for (uint32_t t = xcd_block(blockIdx.x, sp.rg); t < tiles; t += sp.rg) {
-        const uint32_t r = t * blockDim.x + threadIdx.x;
-        if (r < len) {
-            const double v = mat_row<double, double>(A, p, r0 + r);
-            Ap[r0 + r] = v;
-            s += v * p[r0 + r];
+        const uint32_t q = 2 * (t * blockDim.x + threadIdx.x);
+        if (q >= len) continue;
+        const bool two = q + 1 < len;
+        const uint32_t r = r0 + q;
+        RowPair o;
+        if (vec) {
+            o = stencil_pair<true>(A, p, r, two);
+        } else if (A.sd) {
+            o = stencil_pair<false>(A, p, r, two);
+        } else {
+            o = RowPair{sell_row<double, double, false>(A, p, r, 1.0), 0.0, p[r], 0.0};
+            if (two) {
+                o.y1 = sell_row<double, double, false>(A, p, r + 1, 1.0);
+                o.x1 = p[r + 1];
+            }
         }
+        if (vec && two) {
+            *(dbl2*)(Ap + r) = dbl2{o.y0, o.y1};
+        } else {
+            Ap[r] = o.y0;
+            if (two) Ap[r + 1] = o.y1;
+        }
+        s += o.y0 * o.x0;
+        if (two) s += o.y1 * o.x1;
     }
     double part[1] = {s};
     block_partials<kBS, 1>(part, partials);

@@ -63,6 +63,39 @@ def test_stencil_pcg_equals_sliced(ctx, monkeypatch):
     assert np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("dims", [(5, 7, 3), (7, 4, 2), (6, 5, 3), (25, 37, 9), (30, 21, 8)])
+def test_stencil_row_pairs_any_shape(ctx, monkeypatch, dims):
+    """The seven-point kernels take two rows per thread: one 16-byte load per
+    pair of values when n, nx and nx*ny are even ((6,5,3), (30,21,8)), the rows
+    one by one otherwise (odd n or odd nx).  Both equal the oracle's SpMV, and
+    a Jacobi PCG on either layout gives the same iterates bit for bit (the fused
+    SpMV + (Ap, p) kernel adds its products in the same order for both)."""
+    nx, ny, nz = dims
+    s = S.make_system(nx, ny, nz, 31, "lognormal", "spe", 1.0, sigma=1.5)
+    A1, A0 = _pair(ctx, monkeypatch, s)
+    n = s.n
+    assert A1.info()[2] == n * 33 and A0.info()[2] > n * 33
+    x = rng(5).standard_normal(n)
+    y1, y0, r1 = ctx.array(n), ctx.array(n), ctx.array(n)
+    A1.spmv(dev(ctx, x), y1)
+    A0.spmv(dev(ctx, x), y0)
+    ref = O.spmv(s.ia, s.ja, s.a, x)
+    assert np.array_equal(y1.get(), ref) and np.array_equal(y0.get(), ref)
+    A1.residual(dev(ctx, s.b), dev(ctx, x), r1)
+    assert np.array_equal(r1.get(), s.b - ref)
+    out = []
+    for A in (A1, A0):
+        B = N.DevPrec(ctx, A, kind=N.DIAG)
+        xs = ctx.array(n)
+        st = N.pcg(ctx, A, B, dev(ctx, s.b), xs, eps=1e-10, maxit=5000, x0_mode=N.X0_ZERO, use_graph=False)
+        out.append((st["niter"], xs.get()))
+        B.close()
+    assert out[0][0] == out[1][0] and out[0][0] > 5
+    assert np.array_equal(out[0][1], out[1][1])
+    A1.close()
+    A0.close()
+
+
 def test_non_stencil_matrices_keep_slices(ctx):
     s = system("spe10_z12")
     n = s.n
