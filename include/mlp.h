@@ -179,13 +179,15 @@ int mlp_prec_upload(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp_level_des
                     mlp_mat A0, int precision, mlp_prec* out);
 int mlp_prec_from_hier(mlp_ctx ctx, mlp_hier h, mlp_mat A0, int precision, mlp_prec* out);
 /* Autotuned launch schedule (no counterpart in the reference, which has one CPU schedule): the hierarchy's
- * device programs built with each candidate schedule -- the default (uh_cheb: the level chain where every
- * merged component fits a wavefront), level kernels with and without the fused last pair, every level
- * program in 64- or 256-thread blocks -- each timed over `reps` applies on ctx's stream (HIP events, after
- * one untimed apply), the fastest kept.  FP64 applies are bitwise the same under every schedule.  report
- * (optional) receives each distinct candidate's schedule and time. */
-#define MLP_SCHED_CHAIN 1u    /* the level chain runs the last levels */
-#define MLP_SCHED_PAIR  2u    /* the last two levels run fused */
+ * device programs built with each candidate schedule -- the default (uh_cheb: the longest level chain whose
+ * merged components fit one-wavefront blocks of 64 nodes), the chain in two-wavefront blocks of up to 128
+ * nodes (also what MLP_CHAIN_BS=128 asks of every build), level kernels with and without the fused last
+ * pair, every level program in 64- or 256-thread blocks -- each timed over `reps` applies on ctx's stream
+ * (HIP events, after one untimed apply), the fastest kept.  FP64 applies are bitwise the same under every
+ * schedule.  report (optional) receives each distinct candidate's schedule and time (at most 8). */
+#define MLP_SCHED_CHAIN 1u        /* the level chain runs the last levels */
+#define MLP_SCHED_PAIR  2u        /* the last two levels run fused */
+#define MLP_SCHED_WIDE_CHAIN 4u   /* the chain runs in two-wavefront blocks (components of up to 128 nodes) */
 typedef struct {
     uint32_t ncand;           /* distinct candidates timed (at most 8 reported) */
     uint32_t chosen;          /* index of the one kept */
@@ -369,6 +371,9 @@ int mlp_mat_get_csr(mlp_mat A, uint32_t* ia, uint32_t* ja, double* a);
 /* algorithmic HBM bytes of one apply (sum over level visits) and the kernels the last apply launched
  * (0 before the first apply) */
 int mlp_prec_bytes(mlp_prec p, double* bytes_per_apply, uint32_t* launches_per_apply);
+/* The launch schedule a preconditioner was built with (no counterpart in the reference): MLP_SCHED_* bits |
+ * forced level block size << 8 (0 for preconditioners other than uh_cheb / multi_split hierarchies). */
+int mlp_prec_schedule(mlp_prec p, uint32_t* sched);
 /* Average device time of `reps` back-to-back SpMV launches y = A x (two HIP
  * events on the context stream around the whole run).  flush_bytes > 0 makes
  * the launches rotate over device copies of A totalling at least flush_bytes,

@@ -505,6 +505,8 @@ template <class VT> struct DevHier : DevPrecBase {
     bool bf16 = false;                       // level 0 keeps its Chebyshev vectors in BF16 (FP32 uh_cheb)
     // chain of the levels chain_a..nlevels-1 (plan.h: ChainProgram); -1: none
     int chain_a = -1;
+    uint32_t chain_bs = kChainBS;            // threads per chain block: kChainBS, or 2 kChainBS (the wide chain)
+    uint32_t want_chain_bs = 0;              // a tuner candidate's chain block size (0: as the build chooses)
     bool no_chain = false;                   // build without the chain (the batched copy, see widen)
     bool no_pair = false;                    // keep the last two levels separate (a tuner candidate)
     uint32_t sched_lbs = 0;                  // level blocks of this many threads, every level (a tuner candidate)
@@ -750,25 +752,40 @@ template <class VT> struct DevHier : DevPrecBase {
             parallel_for(nlevels, 1, [&](uint64_t k0, uint64_t k1) {
                 for (uint64_t k = k0; k < k1; k++) big[k] = max_level_component(arrays_of(d[k]));
             });
+        // The chain runs in one-wavefront blocks (components of at most kChainBS nodes).  Two-wavefront blocks
+        // (at most 2 kChainBS nodes: most 60x220x85 SPE10 fields have a larger component on some level) are
+        // used when asked for, by MLP_CHAIN_BS=128 or a tuner candidate: their plan costs about as much host
+        // time as the level programs and the pair together, which pays only over many solves.
         ChainProgram CP;
         const char* nochain = getenv("MLP_NO_CHAIN");
         if (kind == MLP_PREC_UH_CHEB && nlevels >= 2 && !no_chain && !(nochain && nochain[0] == '1')) {
             const uint32_t c = nlevels - 1;
-            uint32_t a0 = c > uint32_t(kChainMax) ? c - uint32_t(kChainMax) : 0u;
-            if (bf16) a0 = std::max(a0, 1u);
-            for (uint32_t l = a0; l < c; l++)
-                if (d[l].nit != 2) a0 = l + 1;
-            // a level whose own largest component exceeds a block rules out every chain through it
-            for (uint32_t l = a0; l <= c && a0 < c; l++)
-                if (big[l] > kChainBS) a0 = l + 1;
-            for (uint32_t a = a0; a < c; a++) {
-                std::vector<LevelArrays> la;
-                for (uint32_t l = a; l <= c; l++) la.push_back(arrays_of(d[l]));
-                if (plan_chain(la, a, kChainBS, kChainSlots, CP)) {
-                    chain_a = int(a);
-                    chain_blk = CP.blk_from;
-                    chain_npt.assign(CP.nlev, 0);
-                    break;
+            uint32_t want = want_chain_bs;
+            if (const char* e = getenv("MLP_CHAIN_BS")) want = uint32_t(std::atoi(e));
+            if (want && want != kChainBS && want != 2 * kChainBS) throw Error("MLP_CHAIN_BS must be 64 or 128", 1);
+            uint32_t top = c > uint32_t(kChainMax) ? c - uint32_t(kChainMax) : 0u;
+            if (bf16) top = std::max(top, 1u);
+            for (uint32_t l = top; l < c; l++)
+                if (d[l].nit != 2) top = l + 1;
+            for (const uint32_t bs : {kChainBS, 2 * kChainBS}) {
+                if (bs != (want ? want : kChainBS)) continue;
+                // a level whose own largest component exceeds a block rules out every chain through it
+                uint32_t a0 = top;
+                for (uint32_t l = a0; l <= c && a0 < c; l++)
+                    if (big[l] > bs) a0 = l + 1;
+                const uint32_t a1 = chain_a >= 0 ? uint32_t(chain_a) : c;   // only a longer chain replaces one
+                for (uint32_t a = a0; a < a1; a++) {
+                    std::vector<LevelArrays> la;
+                    for (uint32_t l = a; l <= c; l++) la.push_back(arrays_of(d[l]));
+                    ChainProgram P;
+                    if (plan_chain(la, a, bs, 2 * bs, P)) {
+                        CP = std::move(P);
+                        chain_a = int(a);
+                        chain_bs = bs;
+                        chain_blk = CP.blk_from;
+                        chain_npt.assign(CP.nlev, 0);
+                        break;
+                    }
                 }
             }
         }
@@ -829,8 +846,8 @@ template <class VT> struct DevHier : DevPrecBase {
         if (kind == MLP_PREC_UH_CHEB)
             for (uint32_t l = 0; l < nplan && l + 1 < nlevels; l++) {
                 if (int(l) == pair_l) continue;
-                if (int(l + 1) == pair_l) link_levels(progs[l], PP.qrow);
-                else if (int(l + 1) == chain_a) link_levels(progs[l], CP.lv[0].qrow);
+                if (int(l + 1) == pair_l) link_levels(progs[l], PP.qrow.data(), PP.qrow.size());
+                else if (int(l + 1) == chain_a) link_levels(progs[l], CP.lv[0].qrow.data(), CP.lv[0].qrow.size());
                 else link_levels(progs[l], progs[l + 1]);
             }
         for (uint32_t l = 0; l < nlevels; l++) {
@@ -1003,6 +1020,8 @@ template <class VT> struct DevHier : DevPrecBase {
         h->pair_l = pair_l;
         h->bf16 = bf16;
         h->chain_a = chain_a;
+        h->chain_bs = chain_bs;
+        h->want_chain_bs = want_chain_bs;
         h->no_chain = no_chain;
         h->cv = cv;
         h->chain_blk = chain_blk;
@@ -1198,20 +1217,30 @@ template <class VT> struct DevHier : DevPrecBase {
     template <class FT, int IN, class XT>
     void chain_down(uint32_t j, const InSrc<VT, FT, IN>& f, XT* out, const uint32_t* g) const {
         const uint32_t i0 = j - uint32_t(chain_a);
-        const uint32_t grid = chain_blk[i0] + cdiv(chain_npt[i0], kChainBS);
-        auto k = R > 1 ? k_chain_down<VT, double, FT, IN, XT, true> : k_chain_down<VT, double, FT, IN, XT, false>;
-        launch(k, grid, kChainBS, 0, cv, i0, f, out, lv[j].N, chain_blk[i0], g);
+        const uint32_t grid = chain_blk[i0] + cdiv(chain_npt[i0], chain_bs);
+        constexpr int W = int(2 * kChainBS);
+        auto k = chain_bs == kChainBS
+                     ? (R > 1 ? k_chain_down<VT, double, FT, IN, XT, true, kChainBS>
+                              : k_chain_down<VT, double, FT, IN, XT, false, kChainBS>)
+                     : (R > 1 ? k_chain_down<VT, double, FT, IN, XT, true, W> : k_chain_down<VT, double, FT, IN, XT, false, W>);
+        launch(k, grid, chain_bs, 0, cv, i0, f, out, lv[j].N, chain_blk[i0], g);
     }
     template <class XT>
     void chain_up(uint32_t ke, XT* out, const uint32_t* g) const {
         const uint32_t ie = ke - uint32_t(chain_a);
-        const uint32_t grid = chain_blk[ie] + cdiv(chain_npt[ie + 1], kChainBS);
+        const uint32_t grid = chain_blk[ie] + cdiv(chain_npt[ie + 1], chain_bs);
+        constexpr int W = int(2 * kChainBS);
+        const bool narrow = chain_bs == kChainBS;
         if (lv[nlevels - 2].s1_skip) {
-            auto k = R > 1 ? k_chain_up<VT, double, XT, false, true> : k_chain_up<VT, double, XT, false, false>;
-            launch(k, grid, kChainBS, 0, cv, ie, out, lv[ke].N, chain_blk[ie], g);
+            auto k = narrow ? (R > 1 ? k_chain_up<VT, double, XT, false, true, kChainBS>
+                                     : k_chain_up<VT, double, XT, false, false, kChainBS>)
+                            : (R > 1 ? k_chain_up<VT, double, XT, false, true, W> : k_chain_up<VT, double, XT, false, false, W>);
+            launch(k, grid, chain_bs, 0, cv, ie, out, lv[ke].N, chain_blk[ie], g);
         } else {
-            auto k = R > 1 ? k_chain_up<VT, double, XT, true, true> : k_chain_up<VT, double, XT, true, false>;
-            launch(k, grid, kChainBS, 0, cv, ie, out, lv[ke].N, chain_blk[ie], g);
+            auto k = narrow ? (R > 1 ? k_chain_up<VT, double, XT, true, true, kChainBS>
+                                     : k_chain_up<VT, double, XT, true, false, kChainBS>)
+                            : (R > 1 ? k_chain_up<VT, double, XT, true, true, W> : k_chain_up<VT, double, XT, true, false, W>);
+            launch(k, grid, chain_bs, 0, cv, ie, out, lv[ke].N, chain_blk[ie], g);
         }
     }
     // The operations of Prec::solve on the chain levels, in order: D (L sweep of a level: its visit starts),
@@ -1428,6 +1457,7 @@ template <class VT> struct DevHier : DevPrecBase {
     // kernels the last apply launched (0 before the first apply)
     uint32_t schedule() const override {
         uint32_t b = (chain_a >= 0 ? uint32_t(MLP_SCHED_CHAIN) : 0u) | (pair_l >= 0 ? uint32_t(MLP_SCHED_PAIR) : 0u);
+        if (chain_a >= 0 && chain_bs != kChainBS) b |= MLP_SCHED_WIDE_CHAIN;
         return b | (sched_lbs << 8);
     }
     uint32_t launches_per_apply() const override { return last_launches; }
@@ -2372,6 +2402,7 @@ int mlp_hier_level_array(mlp_hier h, uint32_t l, int which, void* out) {
 struct Schedule {
     bool no_chain = false, no_pair = false;
     uint32_t lbs = 0;
+    uint32_t chain_bs = 0;
 };
 
 template <class VT>
@@ -2383,6 +2414,7 @@ static DevPrecBase* make_hier(mlp_ctx ctx, int kind, uint32_t nlevels, const mlp
     h->no_chain = sc.no_chain;
     h->no_pair = sc.no_pair;
     h->sched_lbs = sc.lbs;
+    h->want_chain_bs = sc.chain_bs;
     try {
         std::vector<mlp_level_desc> d(levels, levels + nlevels);
         h->build(d, kind, A0);
@@ -2461,6 +2493,7 @@ int mlp_prec_from_hier_tuned(mlp_ctx ctx, mlp_hier h, mlp_mat A0, int precision,
     std::vector<Schedule> cand;
     cand.push_back(Schedule());                                  // the default: chain where it fits
     if (kind == MLP_PREC_UH_CHEB) {
+        cand.push_back({false, false, 0, 2 * kChainBS});         // the chain in two-wavefront blocks
         cand.push_back({true, false, 0});                        // level kernels, fused last pair
         cand.push_back({true, true, 0});                         // level kernels only
     }
@@ -2775,6 +2808,13 @@ int mlp_prec_bytes(mlp_prec p, double* bytes, uint32_t* launches) {
     require(p, "null argument");
     if (bytes) *bytes = p->impl->bytes_per_apply();
     if (launches) *launches = p->impl->launches_per_apply();
+    MLP_API_END
+}
+
+int mlp_prec_schedule(mlp_prec p, uint32_t* sched) {
+    MLP_API_BEGIN
+    require(p && sched, "null argument");
+    *sched = p->impl->schedule();
     MLP_API_END
 }
 

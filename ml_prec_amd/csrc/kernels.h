@@ -1225,15 +1225,17 @@ __global__ void __launch_bounds__(BS) k_pair_up(PairView<VT, CT> P, SellView<CT>
 
 // ---------------------------------------------------------------------------
 // chain of uh_cheb levels a..c (plan.h: ChainProgram): descent and ascent
-// launches of Prec::solve (cheb_solve.cc:13-123).  One wavefront per block;
-// thread t owns slot t of every level (node slots are the first ones, so the
-// owned-node sweeps above apply), slots beyond the wavefront (rows without L
-// entries of large components) are handled in a second pass.  Every row keeps
-// the arithmetic of k_level_down / k_level_up / k_pair_down / k_pair_up.
+// launches of Prec::solve (cheb_solve.cc:13-123).  BS threads per block: one
+// wavefront (64; depth levels end with a wave barrier) or two (128, for
+// hierarchies with components of 65..128 nodes; depth levels end with a
+// workgroup barrier).  Thread t owns slot t of every level (node slots are the
+// first ones, so the owned-node sweeps above apply); slots t + BS (rows without
+// L entries of large components, at most 2 BS slots per level) are handled in a
+// second pass.  Every row keeps the arithmetic of k_level_down / k_level_up /
+// k_pair_down / k_pair_up.
 // ---------------------------------------------------------------------------
 constexpr int kChainMax = 4;               // levels a..c-1 of one chain
-constexpr uint32_t kChainBS = 64;
-constexpr uint32_t kChainSlots = 128;
+constexpr uint32_t kChainBS = 64;          // block size of the one-wavefront chain; the wide chain has 2 kChainBS
 constexpr uint32_t kChainHdrW = 8;         // header words per block and level
 
 // A block-uniform word of program data (never written by the kernel), read
@@ -1273,8 +1275,8 @@ template <class VT, class CT> struct ChainView {
 // the coarsest level.  Writes F of every level below i0, wt of every level,
 // the isolated rows' x (level i0's into out0, the others' into the parent's
 // u1) and u1 of the coarsest level.
-template <class VT, class CT, class FT, int IN, class XT, bool BAT>
-__global__ void __launch_bounds__(kChainBS) k_chain_down(ChainView<VT, CT> V, uint32_t i0, InSrc<VT, FT, IN> f,
+template <class VT, class CT, class FT, int IN, class XT, bool BAT, int BS>
+__global__ void __launch_bounds__(BS) k_chain_down(ChainView<VT, CT> V, uint32_t i0, InSrc<VT, FT, IN> f,
                                                            XT* __restrict__ out0, uint32_t Ntop, uint32_t nblk,
                                                            const uint32_t* guard) {
     const size_t y = BAT ? blockIdx.y : 0;
@@ -1290,8 +1292,8 @@ __global__ void __launch_bounds__(kChainBS) k_chain_down(ChainView<VT, CT> V, ui
     for (int i = 0; i < kChainMax; i++)
         if (uint32_t(i) + 1 == nv) { pF = V.V[i].F + y * V.V[i].n; pu1 = V.V[i].u1 + y * V.V[i].n; }
     if (blockIdx.x < nblk) {
-        __shared__ VT sv[2][kChainSlots];
-        __shared__ CT cw[kChainSlots], cx[kChainSlots];
+        __shared__ VT sv[2][2 * BS];
+        __shared__ CT cw[2 * BS], cx[2 * BS];
         const uint32_t b = blockIdx.x;
         // every level's per-slot loads first, so that they overlap
         uint32_t base[kChainMax], S[kChainMax], nn[kChainMax], nkeep[kChainMax], nlL[kChainMax], qr[kChainMax],
@@ -1352,7 +1354,7 @@ __global__ void __launch_bounds__(kChainBS) k_chain_down(ChainView<VT, CT> V, ui
                 }
                 if (t >= nn[i]) s[t] = v;
             }
-            for (uint32_t q = t + kChainBS; q < S[i]; q += kChainBS) {   // slots beyond the wavefront: no nodes
+            for (uint32_t q = t + BS; q < S[i]; q += BS) {   // slots beyond the wavefront: no nodes
                 const uint32_t j = L.qrow[base[i] + q];
                 const VT w = uint32_t(i) == i0 ? fetch(f, base[i] + q, j) : p[L.src[base[i] + q]];
                 if (uint32_t(i) > i0) pFi[base[i] + q] = w;
@@ -1362,9 +1364,9 @@ __global__ void __launch_bounds__(kChainBS) k_chain_down(ChainView<VT, CT> V, ui
                 }
                 s[q] = w;
             }
-            const VT w = sweep_L<kChainBS>(ol[i], nlL[i], L.Lloc, L.Lval, v, s);
+            const VT w = sweep_L<BS>(ol[i], nlL[i], L.Lloc, L.Lval, v, s);
             if (t < nn[i]) V.V[i].wt[y * V.V[i].N + base[i] + t] = w;
-            level_sync<kChainBS>();
+            level_sync<BS>();
             cur ^= 1;
         }
         // step 1 on the coarsest level from the F of the last chain level
@@ -1375,20 +1377,20 @@ __global__ void __launch_bounds__(kChainBS) k_chain_down(ChainView<VT, CT> V, ui
             pF[baseC + t] = v;
             if (t >= nnC) pu1[qrC] = VT(dvC * CT(v));
         }
-        for (uint32_t q = t + kChainBS; q < SC; q += kChainBS) {
+        for (uint32_t q = t + BS; q < SC; q += BS) {
             const VT w = p[C.src[baseC + q]];
             pF[baseC + q] = w;
             pu1[C.qrow[baseC + q]] = VT(C.dval[baseC + q] * CT(w));
         }
         const CT f2 = CT(v);
-        sweep_L<kChainBS>(olC, nlLC, C.Lloc, C.Lval, f2, cw);
-        level_sync<kChainBS>();
+        sweep_L<BS>(olC, nlLC, C.Lloc, C.Lval, f2, cw);
+        level_sync<BS>();
         const CT z = t < nuC ? cw[ouC.loc] : CT(0);
-        const CT x = sweep_U<kChainBS, CT, false>(ouC, nlUC, C.Ucol, C.Uval, z, (const CT*)nullptr, cx);
+        const CT x = sweep_U<BS, CT, false>(ouC, nlUC, C.Ucol, C.Uval, z, (const CT*)nullptr, cx);
         if (t < nuC) pu1[uxC] = VT(x);
     } else {
         // passthrough rows: kept rows without L entries down to the level where they are isolated
-        const uint32_t tp = (blockIdx.x - nblk) * kChainBS + t;
+        const uint32_t tp = (blockIdx.x - nblk) * BS + t;
         bool live = true;
         VT v = VT(0);
 #pragma unroll
@@ -1424,8 +1426,8 @@ __global__ void __launch_bounds__(kChainBS) k_chain_down(ChainView<VT, CT> V, ui
 // the U sweep.  Every level above the coarsest but ie completes its second
 // visit here, so its isolated rows' values come from x2 of its parent, where
 // the descent of that visit wrote them; level ie's rows are written to out.
-template <class VT, class CT, class XT, bool SCALE, bool BAT>
-__global__ void __launch_bounds__(kChainBS) k_chain_up(ChainView<VT, CT> V, uint32_t ie, XT* __restrict__ out,
+template <class VT, class CT, class XT, bool SCALE, bool BAT, int BS>
+__global__ void __launch_bounds__(BS) k_chain_up(ChainView<VT, CT> V, uint32_t ie, XT* __restrict__ out,
                                                          uint32_t Ntop, uint32_t nblk, const uint32_t* guard) {
     const size_t y = BAT ? blockIdx.y : 0;
     if constexpr (BAT) { out = shift(out, y * Ntop); guard = shift(guard, y * V.st); }
@@ -1437,8 +1439,8 @@ __global__ void __launch_bounds__(kChainBS) k_chain_up(ChainView<VT, CT> V, uint
         if (uint32_t(i) + 1 == nv) { pF = V.V[i].F + y * V.V[i].n; pu1 = V.V[i].u1 + y * V.V[i].n; }
     const ChainLv<CT>& C = V.C;
     if (blockIdx.x < nblk) {
-        __shared__ VT sx[2][kChainSlots];
-        __shared__ CT cw[kChainSlots], cx[kChainSlots];
+        __shared__ VT sx[2][2 * BS];
+        __shared__ CT cw[2 * BS], cx[2 * BS];
         const uint32_t b = blockIdx.x;
         const uint32_t* Hc = C.hdr + kChainHdrW * b;
         const uint32_t baseC = uload(Hc), SC = uload(Hc + 1), nnC = uload(Hc + 2), nlLC = uload(Hc + 4),
@@ -1496,16 +1498,16 @@ __global__ void __launch_bounds__(kChainBS) k_chain_up(ChainView<VT, CT> V, uint
         }
         if (gflag) return;
         // step 2 on the coarsest level
-        sweep_L<kChainBS>(olC, nlLC, C.Lloc, C.Lval, f2, cw);
+        sweep_L<BS>(olC, nlLC, C.Lloc, C.Lval, f2, cw);
         if (t < SC && t >= nnC) cx[t] = dvC * f2;
-        for (uint32_t q = t + kChainBS; q < SC; q += kChainBS) {
+        for (uint32_t q = t + BS; q < SC; q += BS) {
             const CT g = sell_row<CT, VT, SCALE>(V.A2, pu1, baseC + q, V.a1c) - CT(pF[baseC + q]);
             cx[q] = C.dval[baseC + q] * g;
         }
-        level_sync<kChainBS>();
+        level_sync<BS>();
         const CT zc = t < nuC ? cw[ouC.loc] : CT(0);
-        sweep_U<kChainBS, CT, false>(ouC, nlUC, C.Ucol, C.Uval, zc, (const CT*)nullptr, cx);
-        level_sync<kChainBS>();
+        sweep_U<BS, CT, false>(ouC, nlUC, C.Ucol, C.Uval, zc, (const CT*)nullptr, cx);
+        level_sync<BS>();
         int cur = 0;
 #pragma unroll
         for (int i = kChainMax - 1; i >= 0; i--) {
@@ -1522,13 +1524,13 @@ __global__ void __launch_bounds__(kChainBS) k_chain_up(ChainView<VT, CT> V, uint
                 s[srn[i]] = c;
                 if (uint32_t(i) == ie) out[xln[i]] = XT(c);
             }
-            if (Sn[i] > kChainBS) {
+            if (Sn[i] > BS) {
                 const ChainLv<VT>& Ln = V.L[i + 1 < kChainMax ? i + 1 : i];
                 const uint32_t* qn = last ? C.qrow : Ln.qrow;
                 const uint32_t* sn = last ? C.src : Ln.src;
                 const uint32_t* xn = last ? C.xl : Ln.xl;
                 const uint32_t b0 = uload((last ? C.hdr : Ln.hdr) + kChainHdrW * b);
-                for (uint32_t q = t + kChainBS; q < Sn[i]; q += kChainBS) {
+                for (uint32_t q = t + BS; q < Sn[i]; q += BS) {
                     const uint32_t r = qn[b0 + q];
                     VT c;
                     if (last) {
@@ -1545,18 +1547,18 @@ __global__ void __launch_bounds__(kChainBS) k_chain_up(ChainView<VT, CT> V, uint
             if (uint32_t(i) > ie) {
                 if (t < S[i] && t >= nkeep[i]) s[t] = xiso[i];
                 const ChainVec<VT>& P = V.V[i > 0 ? i - 1 : 0];
-                for (uint32_t q = t + kChainBS; q < S[i]; q += kChainBS)
+                for (uint32_t q = t + BS; q < S[i]; q += BS)
                     if (q >= nkeep[i]) s[q] = P.x2[y * P.n + L.qrow[base[i] + q]];
             }
-            level_sync<kChainBS>();
-            const VT x = sweep_U<kChainBS, VT, false>(ou[i], nlU[i], L.Ucol, L.Uval, z[i], (const VT*)nullptr, s);
+            level_sync<BS>();
+            const VT x = sweep_U<BS, VT, false>(ou[i], nlU[i], L.Ucol, L.Uval, z[i], (const VT*)nullptr, s);
             if (uint32_t(i) == ie && t < nu[i]) out[ux[i]] = XT(x);
-            level_sync<kChainBS>();
+            level_sync<BS>();
             cur ^= 1;
         }
     } else {
         // passthrough rows that reach level ie + 1: their combines from the deepest level up
-        const uint32_t tp = (blockIdx.x - nblk) * kChainBS + t;
+        const uint32_t tp = (blockIdx.x - nblk) * BS + t;
         bool have = false;
         VT v = VT(0);
         if (gflag) return;

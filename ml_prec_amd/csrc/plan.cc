@@ -275,8 +275,8 @@ void plan_level(const LevelArrays& lv, uint32_t cap, LevelProgram& P, const Bloc
     if (P.lvlU.empty()) P.lvlU.push_back(0);
 }
 
-void link_levels(LevelProgram& P, const std::vector<uint32_t>& next_qrow) {
-    if (next_qrow.size() != P.n) throw Error("level sizes do not chain", 1);
+void link_levels(LevelProgram& P, const uint32_t* next_qrow, size_t n) {
+    if (n != P.n) throw Error("level sizes do not chain", 1);
     std::vector<uint32_t> qpos(P.n);
     for (uint32_t q = 0; q < P.n; q++) qpos[next_qrow[q]] = q;
     for (uint32_t& o : P.out)
@@ -286,7 +286,7 @@ void link_levels(LevelProgram& P, const std::vector<uint32_t>& next_qrow) {
 
 void link_levels(LevelProgram& P, const LevelProgram& next) {
     if (next.N != P.n) throw Error("level sizes do not chain", 1);
-    link_levels(P, next.qrow);
+    link_levels(P, next.qrow.data(), next.qrow.size());
 }
 
 bool plan_pair(const LevelArrays& la, const LevelArrays& lb, uint32_t cap, PairProgram& out) {
@@ -748,24 +748,90 @@ uint32_t max_level_component(const LevelArrays& lv) {
     return best;
 }
 
+namespace {
+// The first error of a parallel loop: the one a sequential loop over the same
+// positions would have met first (smallest position), raised after the loop.
+struct FirstError {
+    std::mutex m;
+    uint64_t at = ~uint64_t(0);
+    const char* msg = nullptr;
+    void put(uint64_t i, const char* s) {
+        std::lock_guard<std::mutex> g(m);
+        if (i < at) {
+            at = i;
+            msg = s;
+        }
+    }
+    void raise() const {
+        if (msg) throw Error(msg, 1);
+    }
+};
+
+// The x in [0, n) with keep(x), in ascending order within buckets key(x) in [0, nb), bucket 0 first (a stable
+// counting sort, chunks of x counted and placed in parallel).
+template <class Keep, class Key>
+std::vector<uint32_t> bucket_ids(uint32_t n, uint32_t nb, const Keep& keep, const Key& key) {
+    const uint64_t chunks = std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(host_threads()) * 4, (n + 65535) / 65536));
+    const uint64_t step = (n + chunks - 1) / chunks;
+    std::vector<uint32_t> cnt(chunks * nb, 0);
+    parallel_for(chunks, 1, [&](uint64_t c0, uint64_t c1) {
+        for (uint64_t c = c0; c < c1; c++)
+            for (uint64_t x = c * step; x < std::min<uint64_t>(n, (c + 1) * step); x++)
+                if (keep(uint32_t(x))) cnt[c * nb + key(uint32_t(x))]++;
+    });
+    uint32_t tot = 0;
+    for (uint32_t b = 0; b < nb; b++)
+        for (uint64_t c = 0; c < chunks; c++) {
+            const uint32_t v = cnt[c * nb + b];
+            cnt[c * nb + b] = tot;
+            tot += v;
+        }
+    std::vector<uint32_t> out(tot);
+    parallel_for(chunks, 1, [&](uint64_t c0, uint64_t c1) {
+        for (uint64_t c = c0; c < c1; c++) {
+            uint32_t* o = &cnt[c * nb];
+            for (uint64_t x = c * step; x < std::min<uint64_t>(n, (c + 1) * step); x++)
+                if (keep(uint32_t(x))) out[o[key(uint32_t(x))]++] = uint32_t(x);
+        }
+    });
+    return out;
+}
+}  // namespace
+
 bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_nodes, uint32_t max_slots,
                 ChainProgram& out) {
     const uint32_t nl = uint32_t(L.size());
     if (nl < 2) return false;
+    constexpr uint64_t kGrain = 1 << 14;
     // the structure the kernels rely on (as plan_level checks it)
     for (uint32_t k = 0; k < nl; k++) {
         const LevelArrays& lv = L[k];
         const uint32_t M = lv.M, K = lv.N - lv.Md;
         if (lv.M + lv.Md > lv.N) throw Error("level sizes inconsistent", 1);
-        for (uint32_t i = 0; i < K; i++)
-            for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++)
-                if (lv.L_ja[p] >= i || lv.L_ja[p] >= M) throw Error("L row has a column outside the eliminated block", 1);
-        for (uint32_t i = 0; i < M; i++) {
-            if (lv.U_ia[i + 1] <= lv.U_ia[i] || lv.U_ja[lv.U_ia[i]] != i)
-                throw Error("U row does not start with its pivot", 1);
-            for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++)
-                if (lv.U_ja[p] <= i || lv.U_ja[p] >= K) throw Error("U row has a column outside (i, N-Md)", 1);
-        }
+        FirstError fl, fu;
+        parallel_for(K, kGrain, [&](uint64_t i0, uint64_t i1) {
+            for (uint64_t i = i0; i < i1; i++)
+                for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++)
+                    if (lv.L_ja[p] >= i || lv.L_ja[p] >= M) {
+                        fl.put(i, "L row has a column outside the eliminated block");
+                        return;
+                    }
+        });
+        fl.raise();
+        parallel_for(M, kGrain, [&](uint64_t i0, uint64_t i1) {
+            for (uint64_t i = i0; i < i1; i++) {
+                if (lv.U_ia[i + 1] <= lv.U_ia[i] || lv.U_ja[lv.U_ia[i]] != i) {
+                    fu.put(i, "U row does not start with its pivot");
+                    return;
+                }
+                for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++)
+                    if (lv.U_ja[p] <= i || lv.U_ja[p] >= K) {
+                        fu.put(i, "U row has a column outside (i, N-Md)");
+                        return;
+                    }
+            }
+        });
+        fu.raise();
         if (k + 1 < nl && L[k + 1].N != lv.N - lv.M - lv.Md) throw Error("level sizes do not chain", 1);
     }
     if (L[nl - 1].M + L[nl - 1].Md != L[nl - 1].N) return false;       // the last level keeps no rows
@@ -774,37 +840,50 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
     std::vector<std::vector<uint32_t>> perm(nl);
     for (uint32_t k = 0; k < nl; k++) {
         const LevelArrays& lv = L[k];
-        std::vector<uint32_t> nat(lv.N);
-        if (k == 0) std::iota(nat.begin(), nat.end(), 0u);
-        else {
-            const LevelArrays& up = L[k - 1];
-            for (uint32_t r = 0; r < lv.N; r++) nat[r] = perm[k - 1][up.M + r];
-        }
+        const uint32_t* nat = k ? perm[k - 1].data() + L[k - 1].M : nullptr;   // identity of level-k row r
         perm[k].resize(lv.N);
-        for (uint32_t i = 0; i < lv.N; i++) {
-            if (lv.map[i] >= lv.N) throw Error("level map out of range", 1);
-            perm[k][i] = nat[lv.map[i]];
-        }
+        uint32_t* pk = perm[k].data();
+        FirstError fe;
+        parallel_for(lv.N, kGrain, [&](uint64_t i0, uint64_t i1) {
+            for (uint64_t i = i0; i < i1; i++) {
+                const uint32_t m = lv.map[i];
+                if (m >= lv.N) {
+                    fe.put(i, "level map out of range");
+                    return;
+                }
+                pk[i] = nat ? nat[m] : m;
+            }
+        });
+        fe.raise();
     }
     // deepest level of every identity; components of the couplings of all levels
     std::vector<uint8_t> deep(NA, 0);
-    for (uint32_t k = 1; k < nl; k++)
-        for (uint32_t i = 0; i < L[k].N; i++) deep[perm[k][i]] = uint8_t(k);
-    DSU dsu(NA);
     std::vector<char> has_node(NA, 0);
     for (uint32_t k = 0; k < nl; k++) {
         const LevelArrays& lv = L[k];
         const uint32_t K = lv.N - lv.Md;
-        for (uint32_t i = 0; i < K; i++) {
-            if (i < lv.M || lv.L_ia[i + 1] > lv.L_ia[i]) has_node[perm[k][i]] = 1;
-            for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) dsu.unite(perm[k][i], perm[k][lv.L_ja[p]]);
-        }
-        for (uint32_t i = 0; i < lv.M; i++)
-            for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) dsu.unite(perm[k][i], perm[k][lv.U_ja[p]]);
+        const uint32_t* pk = perm[k].data();
+        parallel_for(lv.N, kGrain, [&](uint64_t i0, uint64_t i1) {
+            for (uint64_t i = i0; i < i1; i++) {
+                if (k) deep[pk[i]] = uint8_t(k);
+                if (i < lv.M || (i < K && lv.L_ia[i + 1] > lv.L_ia[i])) has_node[pk[i]] = 1;
+            }
+        });
     }
-    // components with a node: their rows by identity; the others are passthrough rows
+    DSU dsu(NA);
+    for (uint32_t k = 0; k < nl; k++) {
+        const LevelArrays& lv = L[k];
+        const uint32_t K = lv.N - lv.Md;
+        const uint32_t* pk = perm[k].data();
+        for (uint32_t i = 0; i < K; i++)
+            for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) dsu.unite(pk[i], pk[lv.L_ja[p]]);
+        for (uint32_t i = 0; i < lv.M; i++)
+            for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) dsu.unite(pk[i], pk[lv.U_ja[p]]);
+    }
+    // components with a node: their rows by identity; the others are passthrough rows.  Every parent is a
+    // smaller identity, so one ascending pass resolves every root (the smallest identity of its component).
     std::vector<uint32_t> root(NA);
-    for (uint32_t x = 0; x < NA; x++) root[x] = dsu.find(x);
+    for (uint32_t x = 0; x < NA; x++) root[x] = dsu.p[x] == x ? x : root[dsu.p[x]];
     std::vector<char> comp_node(NA, 0);
     for (uint32_t x = 0; x < NA; x++)
         if (has_node[x]) comp_node[root[x]] = 1;
@@ -818,51 +897,59 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
         for (uint32_t x = 0; x < NA; x++)
             if (comp_node[root[x]]) comp_rows[fill[root[x]]++] = x;   // ascending identity within a component
     }
-    std::vector<uint32_t> comps;                                        // component roots
+    // each component's deepest level (its rows are comp_rows[comp_start[r] ..])
     std::vector<uint8_t> comp_deep(NA, 0);
-    for (uint32_t x = 0; x < NA; x++)
-        if (comp_node[root[x]]) comp_deep[root[x]] = std::max(comp_deep[root[x]], deep[x]);
-    for (uint32_t x = 0; x < NA; x++)
-        if (comp_node[x] && root[x] == x) comps.push_back(x);
-    // deepest level first, then by smallest identity (the root is the smallest identity of its component):
-    // a stable counting sort on the level
-    auto by_level_desc = [nl](std::vector<uint32_t>& v, auto level) {
-        std::vector<uint32_t> cnt(nl + 1, 0), out(v.size());
-        for (uint32_t x : v) cnt[nl - 1 - level(x) + 1]++;
-        for (uint32_t k = 0; k < nl; k++) cnt[k + 1] += cnt[k];
-        for (uint32_t x : v) out[cnt[nl - 1 - level(x)]++] = x;
-        v.swap(out);
-    };
-    by_level_desc(comps, [&](uint32_t r) { return uint32_t(comp_deep[r]); });
-    std::vector<uint32_t> pt;                                           // passthrough identities
-    for (uint32_t x = 0; x < NA; x++)
-        if (!comp_node[root[x]]) pt.push_back(x);
-    by_level_desc(pt, [&](uint32_t x) { return uint32_t(deep[x]); });
-    std::vector<uint32_t> at(NA, kNoNode);                             // scratch: identity -> permuted row (level k)
+    parallel_for(NA, kGrain, [&](uint64_t r0, uint64_t r1) {
+        for (uint64_t r = r0; r < r1; r++) {
+            uint8_t d = 0;
+            for (uint32_t e = comp_start[r]; e < comp_start[r + 1]; e++) d = std::max(d, deep[comp_rows[e]]);
+            comp_deep[r] = d;
+        }
+    });
+    // component roots and passthrough identities: deepest level first, then by smallest identity (the root
+    // is the smallest identity of its component)
+    const std::vector<uint32_t> comps = bucket_ids(
+        NA, nl, [&](uint32_t x) { return comp_node[x] && root[x] == x; },
+        [&](uint32_t x) { return nl - 1 - uint32_t(comp_deep[x]); });
+    const std::vector<uint32_t> pt = bucket_ids(
+        NA, nl, [&](uint32_t x) { return !comp_node[root[x]]; }, [&](uint32_t x) { return nl - 1 - uint32_t(deep[x]); });
     auto is_node = [&](uint32_t k, uint32_t i) {
         const LevelArrays& lv = L[k];
         return i < lv.M || (i < lv.N - lv.Md && lv.L_ia[i + 1] > lv.L_ia[i]);
     };
+    const uint64_t NC = comps.size();
     // per-level node / slot counts of each component, then the blocks
-    std::vector<std::vector<uint32_t>> cnodes(nl, std::vector<uint32_t>(comps.size(), 0)),
-        cslots(nl, std::vector<uint32_t>(comps.size(), 0));
-    for (uint32_t k = 0; k < nl; k++) {
-        for (uint32_t i = 0; i < L[k].N; i++) at[perm[k][i]] = i;
-        for (size_t c = 0; c < comps.size(); c++) {
-            const uint32_t r = comps[c];
-            for (uint32_t e = comp_start[r]; e < comp_start[r + 1]; e++) {
-                const uint32_t x = comp_rows[e];
-                if (deep[x] < k) continue;
-                cslots[k][c]++;
-                if (is_node(k, at[x])) cnodes[k][c]++;
-            }
+    std::vector<std::vector<uint32_t>> cnodes(nl, std::vector<uint32_t>(NC, 0)), cslots(nl, std::vector<uint32_t>(NC, 0));
+    {
+        std::vector<uint32_t> at(NA, kNoNode);                          // identity -> permuted row (level k)
+        for (uint32_t k = 0; k < nl; k++) {
+            const uint32_t* pk = perm[k].data();
+            parallel_for(L[k].N, kGrain, [&](uint64_t i0, uint64_t i1) {
+                for (uint64_t i = i0; i < i1; i++) at[pk[i]] = uint32_t(i);
+            });
+            parallel_for(NC, 1024, [&](uint64_t c0, uint64_t c1) {
+                for (uint64_t c = c0; c < c1; c++) {
+                    const uint32_t r = comps[c];
+                    uint32_t nn = 0, ns = 0;
+                    for (uint32_t e = comp_start[r]; e < comp_start[r + 1]; e++) {
+                        const uint32_t x = comp_rows[e];
+                        if (deep[x] < k) continue;
+                        ns++;
+                        if (is_node(k, at[x])) nn++;
+                    }
+                    cnodes[k][c] = nn;
+                    cslots[k][c] = ns;
+                }
+            });
+            parallel_for(L[k].N, kGrain, [&](uint64_t i0, uint64_t i1) {
+                for (uint64_t i = i0; i < i1; i++) at[pk[i]] = kNoNode;
+            });
         }
-        for (uint32_t i = 0; i < L[k].N; i++) at[perm[k][i]] = kNoNode;
     }
     std::vector<uint32_t> bstart;                                       // first component of each block
     {
         std::vector<uint32_t> bn(nl, 0), bsl(nl, 0);
-        for (size_t c = 0; c < comps.size(); c++) {
+        for (size_t c = 0; c < NC; c++) {
             // components are packed while every level keeps at most max_nodes nodes and max_nodes slots (one
             // slot per thread); a component with more slots than that gets a block of its own
             bool fits = !bstart.empty();
@@ -877,7 +964,7 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
             }
             for (uint32_t k = 0; k < nl; k++) { bn[k] += cnodes[k][c]; bsl[k] += cslots[k][c]; }
         }
-        bstart.push_back(uint32_t(comps.size()));
+        bstart.push_back(uint32_t(NC));
     }
     const uint32_t nblk = uint32_t(bstart.size()) - 1;
     ChainProgram P;
@@ -890,167 +977,257 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
         const uint32_t d = comp_deep[comps[bstart[b]]];                // the block's deepest level
         for (uint32_t k = 0; k <= d; k++) P.blk_from[k] = b + 1;
     }
-    // per level: slots of every block, then the passthrough rows.  Pass 1 orders each level's slots and
-    // records every identity's slot; pass 2 writes the arrays (the source slots of a level are the slots of
-    // the level above).  Both run one thread per level.
-    std::vector<std::vector<uint32_t>> inv(nl), slot(nl), order(nl), uorder(nl), dLk(nl), dUk(nl);
-    std::vector<std::exception_ptr> err(nl);
-    parallel_for(nl, 1, [&](uint64_t k0, uint64_t k1) {
-        for (uint32_t k = uint32_t(k0); k < k1; k++) {
-            try {
-                const LevelArrays& lv = L[k];
-                ChainLevelProgram& C = P.lv[k];
-                const uint32_t N = lv.N, M = lv.M, K = N - lv.Md;
-                C.N = N; C.M = M; C.Md = lv.Md;
-                std::vector<uint32_t>& at = inv[k];
-                at.assign(NA, kNoNode);
-                for (uint32_t i = 0; i < N; i++) at[perm[k][i]] = i;
-                // depths (L: over eliminated columns; U: over eliminated columns, kept columns are known
-                // beforehand)
-                std::vector<uint32_t>& dL = dLk[k];
-                std::vector<uint32_t>& dU = dUk[k];
-                dL.assign(K, 0);
-                for (uint32_t i = 0; i < K; i++)
-                    for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) dL[i] = std::max(dL[i], dL[lv.L_ja[p]] + 1);
-                dU.assign(M, 0);
-                for (uint32_t i = M; i-- > 0;)
-                    for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++)
-                        if (lv.U_ja[p] < M) dU[i] = std::max(dU[i], dU[lv.U_ja[p]] + 1);
-                C.hdr.assign(size_t(kChainHdr) * nblk, 0);
-                slot[k].assign(NA, kNoNode);
-                uint32_t pos = 0;
-                std::vector<uint32_t> nodes, flat, iso, ul;
-                for (uint32_t b = 0; b < nblk; b++) {
-                    nodes.clear(); flat.clear(); iso.clear(); ul.clear();
-                    for (uint32_t c = bstart[b]; c < bstart[b + 1]; c++) {
-                        const uint32_t r = comps[c];
-                        for (uint32_t e = comp_start[r]; e < comp_start[r + 1]; e++) {
-                            const uint32_t x = comp_rows[e];
-                            if (deep[x] < k) continue;
-                            const uint32_t i = at[x];
-                            if (is_node(k, i)) nodes.push_back(i);
-                            else if (i < K) flat.push_back(i);
-                            else iso.push_back(i);
-                            if (i < M) ul.push_back(i);
-                        }
+    // Per level: slots of every block, then the passthrough rows.  Pass 1 orders each block's slots and U list
+    // (blocks in parallel), places the blocks one after another and records every identity's slot; pass 2
+    // writes the arrays (the source slots of a level are the slots of the level above), blocks in parallel
+    // from their entry offsets.  The levels run in turn.
+    // slot[k][i]: the block slot of level-k row i (kNoNode: a passthrough row).  Level k's rows are the kept
+    // rows of level k-1: level-k row i is row M_{k-1} + map_k[i] of level k-1.
+    std::vector<std::vector<uint32_t>> slot(nl), order(nl), uorder(nl), dLk(nl), dUk(nl);
+    HostVec<uint32_t> at(NA);                                           // identity -> row of the current level
+    for (uint32_t k = 0; k < nl; k++) {
+        const LevelArrays& lv = L[k];
+        ChainLevelProgram& C = P.lv[k];
+        const uint32_t N = lv.N, M = lv.M, K = N - lv.Md;
+        const uint32_t* pk = perm[k].data();
+        C.N = N; C.M = M; C.Md = lv.Md;
+        // (only identities on level k are looked up)
+        parallel_for(N, kGrain, [&](uint64_t i0, uint64_t i1) {
+            for (uint64_t i = i0; i < i1; i++) at[pk[i]] = uint32_t(i);
+        });
+        // depths (L: over eliminated columns; U: over eliminated columns, kept columns are known beforehand)
+        std::vector<uint32_t>& dL = dLk[k];
+        std::vector<uint32_t>& dU = dUk[k];
+        dL.assign(K, 0);
+        for (uint32_t i = 0; i < K; i++)
+            for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) dL[i] = std::max(dL[i], dL[lv.L_ja[p]] + 1);
+        dU.assign(M, 0);
+        for (uint32_t i = M; i-- > 0;)
+            for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++)
+                if (lv.U_ja[p] < M) dU[i] = std::max(dU[i], dU[lv.U_ja[p]] + 1);
+        C.hdr.assign(size_t(kChainHdr) * nblk, 0);
+        slot[k].assign(N, kNoNode);
+        std::vector<std::vector<uint32_t>> bsl(nblk), bul(nblk);     // each block's slot order and U list
+        parallel_for(nblk, 16, [&](uint64_t b0, uint64_t b1) {
+            std::vector<uint32_t> nodes, flat, iso, ul;
+            for (uint64_t b = b0; b < b1; b++) {
+                nodes.clear(); flat.clear(); iso.clear(); ul.clear();
+                for (uint32_t c = bstart[b]; c < bstart[b + 1]; c++) {
+                    const uint32_t r = comps[c];
+                    for (uint32_t e = comp_start[r]; e < comp_start[r + 1]; e++) {
+                        const uint32_t x = comp_rows[e];
+                        if (deep[x] < k) continue;
+                        const uint32_t i = at[x];
+                        if (is_node(k, i)) nodes.push_back(i);
+                        else if (i < K) flat.push_back(i);
+                        else iso.push_back(i);
+                        if (i < M) ul.push_back(i);
                     }
-                    // (total orders: by depth, then by row)
-                    std::sort(nodes.begin(), nodes.end(), [&](uint32_t u, uint32_t v) {
-                        return dL[u] != dL[v] ? dL[u] < dL[v] : u < v;
-                    });
-                    std::sort(flat.begin(), flat.end());
-                    std::sort(iso.begin(), iso.end());
-                    std::sort(ul.begin(), ul.end(), [&](uint32_t u, uint32_t v) {
-                        return dU[u] != dU[v] ? dU[u] < dU[v] : u < v;
-                    });
-                    uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
-                    H[0] = pos;
-                    H[1] = uint32_t(nodes.size() + flat.size() + iso.size());
-                    H[2] = uint32_t(nodes.size());
-                    H[3] = uint32_t(flat.size());
-                    uint32_t nlv = 0, nuv = 0;
-                    for (uint32_t i : nodes) nlv = std::max(nlv, dL[i] + 1);
-                    for (uint32_t i : ul) nuv = std::max(nuv, dU[i] + 1);
-                    H[4] = nlv;
-                    H[5] = uint32_t(uorder[k].size());
-                    H[6] = uint32_t(ul.size());
-                    H[7] = nuv;
-                    uint32_t sl = 0;
-                    for (auto* v : {&nodes, &flat, &iso})
-                        for (uint32_t i : *v) {
-                            slot[k][perm[k][i]] = sl++;
-                            order[k].push_back(i);
-                        }
-                    for (uint32_t i : ul) uorder[k].push_back(i);
-                    pos += H[1];
                 }
-                C.nslots = pos;
-            } catch (...) {
-                err[k] = std::current_exception();
+                // (total orders: by depth, then by row)
+                std::sort(nodes.begin(), nodes.end(), [&](uint32_t u, uint32_t v) {
+                    return dL[u] != dL[v] ? dL[u] < dL[v] : u < v;
+                });
+                std::sort(flat.begin(), flat.end());
+                std::sort(iso.begin(), iso.end());
+                std::sort(ul.begin(), ul.end(), [&](uint32_t u, uint32_t v) {
+                    return dU[u] != dU[v] ? dU[u] < dU[v] : u < v;
+                });
+                uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
+                H[1] = uint32_t(nodes.size() + flat.size() + iso.size());
+                H[2] = uint32_t(nodes.size());
+                H[3] = uint32_t(flat.size());
+                uint32_t nlv = 0, nuv = 0;
+                for (uint32_t i : nodes) nlv = std::max(nlv, dL[i] + 1);
+                for (uint32_t i : ul) nuv = std::max(nuv, dU[i] + 1);
+                H[4] = nlv;
+                H[6] = uint32_t(ul.size());
+                H[7] = nuv;
+                std::vector<uint32_t>& o = bsl[b];
+                o.reserve(H[1]);
+                o.insert(o.end(), nodes.begin(), nodes.end());
+                o.insert(o.end(), flat.begin(), flat.end());
+                o.insert(o.end(), iso.begin(), iso.end());
+                bul[b] = ul;
             }
+        });
+        uint32_t pos = 0, upos = 0;
+        for (uint32_t b = 0; b < nblk; b++) {
+            uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
+            H[0] = pos;
+            H[5] = upos;
+            pos += H[1];
+            upos += H[6];
         }
-    });
-    for (auto& e : err)
-        if (e) std::rethrow_exception(e);
-    parallel_for(nl, 1, [&](uint64_t k0, uint64_t k1) {
-        for (uint32_t k = uint32_t(k0); k < k1; k++) {
-            try {
-                const LevelArrays& lv = L[k];
-                ChainLevelProgram& C = P.lv[k];
-                const uint32_t N = lv.N, M = lv.M, K = N - lv.Md;
-                const std::vector<uint32_t>& at = inv[k];
-                const std::vector<uint32_t>& sk = slot[k];
-                const std::vector<uint32_t>& dL = dLk[k];
-                const std::vector<uint32_t>& dU = dUk[k];
-                C.Lptr.push_back(0);
-                C.Uptr.push_back(0);
-                C.dval.assign(N, 0.0);
-                C.qrow.reserve(N);
-                for (uint32_t b = 0; b < nblk; b++) {
-                    const uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
-                    for (uint32_t s = 0; s < H[1]; s++) {
-                        const uint32_t q = H[0] + s, i = order[k][q], x = perm[k][i];
-                        C.qrow.push_back(lv.map[i]);
-                        if (k > 0) {
-                            const uint32_t ps = slot[k - 1][x];
-                            if (ps == kNoNode) throw Error("chain: a row has no slot on the level above", 1);
-                            C.src.push_back(ps);
-                            C.xl.push_back(L[k - 1].map[inv[k - 1][x]]);
-                        }
-                        const bool node = s < H[2];
-                        C.Ldep.push_back(node ? dL[i] : 0u);
-                        if (node)
-                            for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
-                                C.Lloc.push_back(sk[perm[k][lv.L_ja[p]]]);
-                                C.Lval.push_back(lv.L_a[p]);
-                            }
-                        C.Lptr.push_back(uint32_t(C.Lloc.size()));
-                        if (i >= K) C.dval[q] = lv.dval[i - K];
-                    }
-                    for (uint32_t u = H[5]; u < H[5] + H[6]; u++) {
-                        const uint32_t i = uorder[k][u];
-                        C.Udep.push_back(dU[i]);
-                        C.uloc.push_back(sk[perm[k][i]]);
-                        C.uxidx.push_back(lv.map[i]);
-                        C.Udiag.push_back(lv.U_a[lv.U_ia[i]]);
-                        for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) {
-                            C.Ucol.push_back(sk[perm[k][lv.U_ja[p]]]);
-                            C.Uval.push_back(lv.U_a[p]);
-                        }
-                        C.Uptr.push_back(uint32_t(C.Ucol.size()));
-                    }
+        order[k].resize(pos);
+        uorder[k].resize(upos);
+        parallel_for(nblk, 16, [&](uint64_t b0, uint64_t b1) {
+            for (uint64_t b = b0; b < b1; b++) {
+                const uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
+                for (uint32_t sl = 0; sl < H[1]; sl++) {
+                    const uint32_t i = bsl[b][sl];
+                    slot[k][i] = sl;
+                    order[k][H[0] + sl] = i;
                 }
-                // passthrough rows on this level (a prefix of pt: deepest level descending)
-                uint32_t pos = C.nslots;
-                for (uint32_t x : pt) {
-                    if (deep[x] < k) break;
-                    const uint32_t i = at[x];
-                    if (i < M || (i < K && lv.L_ia[i + 1] > lv.L_ia[i]))
-                        throw Error("chain: a passthrough row is a node", 1);
-                    if ((i >= K) != (deep[x] == k)) throw Error("chain: a passthrough row ends without being isolated", 1);
-                    C.qrow.push_back(lv.map[i]);
-                    if (i >= K) C.dval[pos] = lv.dval[i - K];
-                    pos++;
-                    C.npt++;
-                }
-                if (pos != N || C.qrow.size() != N) throw Error("chain: level rows are not covered exactly once", 1);
-                std::vector<char> seen(N, 0);
-                for (uint32_t j : C.qrow) {
-                    if (j >= N || seen[j]) throw Error("chain: level rows are not covered exactly once", 1);
-                    seen[j] = 1;
-                }
-                // no empty device arrays
-                for (auto* v : {&C.src, &C.xl, &C.Lloc, &C.Udep, &C.uloc, &C.uxidx, &C.Ucol, &C.hdr})
-                    if (v->empty()) v->push_back(0);
-                for (auto* v : {&C.Lval, &C.Uval, &C.Udiag})
-                    if (v->empty()) v->push_back(0.0);
-            } catch (...) {
-                err[k] = std::current_exception();
+                std::copy(bul[b].begin(), bul[b].end(), uorder[k].begin() + H[5]);
             }
+        });
+        C.nslots = pos;
+    }
+    for (uint32_t k = 0; k < nl; k++) {
+        const LevelArrays& lv = L[k];
+        ChainLevelProgram& C = P.lv[k];
+        const uint32_t N = lv.N, M = lv.M, K = N - lv.Md;
+        const uint32_t* pk = perm[k].data();
+        parallel_for(N, kGrain, [&](uint64_t i0, uint64_t i1) {
+            for (uint64_t i = i0; i < i1; i++) at[pk[i]] = uint32_t(i);
+        });
+        const std::vector<uint32_t>& sk = slot[k];
+        const std::vector<uint32_t>& dL = dLk[k];
+        const std::vector<uint32_t>& dU = dUk[k];
+        // passthrough rows on this level: a prefix of pt (deepest level descending)
+        uint32_t npt = 0;
+        while (npt < pt.size() && deep[pt[npt]] >= k) npt++;
+        {
+            FirstError fe;
+            parallel_for(npt, kGrain, [&](uint64_t t0, uint64_t t1) {
+                for (uint64_t t = t0; t < t1; t++) {
+                    const uint32_t x = pt[t], i = at[x];
+                    if (i < M || (i < K && lv.L_ia[i + 1] > lv.L_ia[i])) {
+                        fe.put(t, "chain: a passthrough row is a node");
+                        return;
+                    }
+                    if ((i >= K) != (deep[x] == k)) {
+                        fe.put(t, "chain: a passthrough row ends without being isolated");
+                        return;
+                    }
+                }
+            });
+            if (C.nslots + uint64_t(npt) != N) {
+                // the blocks' rows are checked in pass 2 of a consistent plan; an inconsistent one stops here
+                fe.raise();
+                throw Error("chain: level rows are not covered exactly once", 1);
+            }
+            fe.raise();
         }
-    });
-    for (auto& e : err)
-        if (e) std::rethrow_exception(e);
+        // every block's L and U entry counts, then their offsets
+        const uint32_t NU = uint32_t(uorder[k].size());
+        std::vector<uint32_t> offL(nblk + 1, 0), offU(nblk + 1, 0);
+        parallel_for(nblk, 64, [&](uint64_t b0, uint64_t b1) {
+            for (uint64_t b = b0; b < b1; b++) {
+                const uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
+                uint32_t nLe = 0, nUe = 0;
+                for (uint32_t s = 0; s < H[2]; s++) {
+                    const uint32_t i = order[k][H[0] + s];
+                    nLe += lv.L_ia[i + 1] - lv.L_ia[i];
+                }
+                for (uint32_t u = H[5]; u < H[5] + H[6]; u++) {
+                    const uint32_t i = uorder[k][u];
+                    nUe += lv.U_ia[i + 1] - lv.U_ia[i] - 1;
+                }
+                offL[b + 1] = nLe;
+                offU[b + 1] = nUe;
+            }
+        });
+        for (uint32_t b = 0; b < nblk; b++) {
+            offL[b + 1] += offL[b];
+            offU[b + 1] += offU[b];
+        }
+        const uint32_t ns = C.nslots;
+        C.qrow.resize(N);
+        C.dval.resize(N);                                               // every position written below
+        if (k > 0) {
+            C.src.resize(ns);
+            C.xl.resize(ns);
+        }
+        C.Ldep.resize(ns);
+        C.Lptr.resize(size_t(ns) + 1);
+        C.Lptr[0] = 0;
+        C.Lloc.resize(offL[nblk]);
+        C.Lval.resize(offL[nblk]);
+        C.Udep.resize(NU);
+        C.uloc.resize(NU);
+        C.uxidx.resize(NU);
+        C.Udiag.resize(NU);
+        C.Uptr.resize(size_t(NU) + 1);
+        C.Uptr[0] = 0;
+        C.Ucol.resize(offU[nblk]);
+        C.Uval.resize(offU[nblk]);
+        FirstError fe;
+        parallel_for(nblk, 16, [&](uint64_t b0, uint64_t b1) {
+            for (uint64_t b = b0; b < b1; b++) {
+                const uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
+                uint32_t cl = offL[b];
+                for (uint32_t s = 0; s < H[1]; s++) {
+                    const uint32_t q = H[0] + s, i = order[k][q];
+                    C.qrow[q] = lv.map[i];
+                    if (k > 0) {
+                        const uint32_t iu = L[k - 1].M + lv.map[i];        // the row on the level above
+                        const uint32_t ps = slot[k - 1][iu];
+                        if (ps == kNoNode) {
+                            fe.put((b << 32) | s, "chain: a row has no slot on the level above");
+                            return;
+                        }
+                        C.src[q] = ps;
+                        C.xl[q] = L[k - 1].map[iu];
+                    }
+                    const bool node = s < H[2];
+                    C.Ldep[q] = node ? dL[i] : 0u;
+                    if (node)
+                        for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
+                            C.Lloc[cl] = sk[lv.L_ja[p]];
+                            C.Lval[cl] = lv.L_a[p];
+                            cl++;
+                        }
+                    C.Lptr[q + 1] = cl;
+                    C.dval[q] = i >= K ? lv.dval[i - K] : 0.0;
+                }
+                uint32_t cu = offU[b];
+                for (uint32_t u = H[5]; u < H[5] + H[6]; u++) {
+                    const uint32_t i = uorder[k][u];
+                    C.Udep[u] = dU[i];
+                    C.uloc[u] = sk[i];
+                    C.uxidx[u] = lv.map[i];
+                    C.Udiag[u] = lv.U_a[lv.U_ia[i]];
+                    for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) {
+                        C.Ucol[cu] = sk[lv.U_ja[p]];
+                        C.Uval[cu] = lv.U_a[p];
+                        cu++;
+                    }
+                    C.Uptr[u + 1] = cu;
+                }
+            }
+        });
+        fe.raise();
+        parallel_for(npt, kGrain, [&](uint64_t t0, uint64_t t1) {
+            for (uint64_t t = t0; t < t1; t++) {
+                const uint32_t q = ns + uint32_t(t), i = at[pt[t]];
+                C.qrow[q] = lv.map[i];
+                C.dval[q] = i >= K ? lv.dval[i - K] : 0.0;
+            }
+        });
+        C.npt = npt;
+        {
+            HostVec<uint8_t> seen(N);
+            parallel_for(N, kGrain, [&](uint64_t q0, uint64_t q1) { std::fill(&seen[q0], &seen[q0] + (q1 - q0), 0); });
+            std::atomic<bool> bad{false};
+            parallel_for(N, kGrain, [&](uint64_t q0, uint64_t q1) {
+                for (uint64_t q = q0; q < q1; q++) {
+                    const uint32_t j = C.qrow[q];
+                    if (j >= N || __atomic_exchange_n(&seen[j], uint8_t(1), __ATOMIC_RELAXED)) {
+                        bad = true;
+                        return;
+                    }
+                }
+            });
+            if (bad) throw Error("chain: level rows are not covered exactly once", 1);
+        }
+        // no empty device arrays
+        for (auto* v : {&C.src, &C.xl, &C.Lloc, &C.Udep, &C.uloc, &C.uxidx, &C.Ucol, &C.hdr})
+            if (v->empty()) v->push_back(0);
+        for (auto* v : {&C.Lval, &C.Uval, &C.Udiag})
+            if (v->empty()) v->push_back(0.0);
+    }
     for (uint32_t k = 0; k < nl; k++)
         for (uint32_t b = 0; b < nblk; b++) {
             const uint32_t* H = &P.lv[k].hdr[size_t(kChainHdr) * b];
@@ -1060,31 +1237,36 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
     // index checks: every local index inside its block, every position inside its level
     for (uint32_t k = 0; k < nl; k++) {
         const ChainLevelProgram& C = P.lv[k];
-        for (uint32_t b = 0; b < nblk; b++) {
-            const uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
-            const uint32_t base = H[0], S = H[1], nn = H[2];
-            if (nn > S || H[2] + H[3] > S || base + S > C.nslots) throw Error("chain: block header out of range", 1);
-            for (uint32_t s = 0; s < nn; s++) {
-                if (C.Ldep[base + s] >= H[4]) throw Error("chain: L depth out of range", 1);
-                for (uint32_t p = C.Lptr[base + s]; p < C.Lptr[base + s + 1]; p++)
-                    if (C.Lloc[p] >= s) throw Error("chain: L column not earlier in its block", 1);
+        FirstError fe;
+        parallel_for(nblk, 64, [&](uint64_t b0, uint64_t b1) {
+            for (uint64_t b = b0; b < b1; b++) {
+                auto bad = [&](const char* m) { fe.put(b, m); };
+                const uint32_t* H = &C.hdr[size_t(kChainHdr) * b];
+                const uint32_t base = H[0], S = H[1], nn = H[2];
+                if (nn > S || H[2] + H[3] > S || base + S > C.nslots) return bad("chain: block header out of range");
+                for (uint32_t s = 0; s < nn; s++) {
+                    if (C.Ldep[base + s] >= H[4]) return bad("chain: L depth out of range");
+                    for (uint32_t p = C.Lptr[base + s]; p < C.Lptr[base + s + 1]; p++)
+                        if (C.Lloc[p] >= s) return bad("chain: L column not earlier in its block");
+                }
+                for (uint32_t s = nn; s < S; s++)
+                    if (C.Lptr[base + s + 1] != C.Lptr[base + s]) return bad("chain: a non-node slot has L entries");
+                if (k > 0) {
+                    const uint32_t* Hp = &P.lv[k - 1].hdr[size_t(kChainHdr) * b];
+                    for (uint32_t s = 0; s < S; s++)
+                        if (C.src[base + s] >= Hp[1] || C.src[base + s] >= Hp[2] + Hp[3] ||
+                            C.xl[base + s] >= P.lv[k - 1].N)
+                            return bad("chain: a row's source on the level above is not a kept slot");
+                }
+                for (uint32_t u = H[5]; u < H[5] + H[6]; u++) {
+                    if (C.uloc[u] >= nn || C.Udep[u] >= H[7]) return bad("chain: U-list entry out of range");
+                    for (uint32_t p = C.Uptr[u]; p < C.Uptr[u + 1]; p++)
+                        if (C.Ucol[p] >= H[2] + H[3] || C.Ucol[p] == C.uloc[u]) return bad("chain: U column out of range");
+                }
+                if (S > max_slots || nn > max_nodes || H[6] > nn) return bad("chain: block too large");
             }
-            for (uint32_t s = nn; s < S; s++)
-                if (C.Lptr[base + s + 1] != C.Lptr[base + s]) throw Error("chain: a non-node slot has L entries", 1);
-            if (k > 0) {
-                const uint32_t* Hp = &P.lv[k - 1].hdr[size_t(kChainHdr) * b];
-                for (uint32_t s = 0; s < S; s++)
-                    if (C.src[base + s] >= Hp[1] || C.src[base + s] >= Hp[2] + Hp[3] ||
-                        C.xl[base + s] >= P.lv[k - 1].N)
-                        throw Error("chain: a row's source on the level above is not a kept slot", 1);
-            }
-            for (uint32_t u = H[5]; u < H[5] + H[6]; u++) {
-                if (C.uloc[u] >= nn || C.Udep[u] >= H[7]) throw Error("chain: U-list entry out of range", 1);
-                for (uint32_t p = C.Uptr[u]; p < C.Uptr[u + 1]; p++)
-                    if (C.Ucol[p] >= H[2] + H[3] || C.Ucol[p] == C.uloc[u]) throw Error("chain: U column out of range", 1);
-            }
-            if (S > max_slots || nn > max_nodes || H[6] > nn) throw Error("chain: block too large", 1);
-        }
+        });
+        fe.raise();
         if (k > 0 && C.npt > P.lv[k - 1].npt) throw Error("chain: passthrough rows do not nest", 1);
     }
     out = std::move(P);

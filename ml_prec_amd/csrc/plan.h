@@ -122,8 +122,8 @@ void plan_sell(uint32_t n, const uint32_t* ia, const uint32_t* ja, const double*
 // of level l+1 (F slot of kept row r = position of row r in next.qrow), so
 // that level l+1 reads it with plain coalesced loads.
 void link_levels(LevelProgram& P, const LevelProgram& next);
-// The same for a next level whose consumption order is next_qrow.
-void link_levels(LevelProgram& P, const std::vector<uint32_t>& next_qrow);
+// The same for a next level whose consumption order is next_qrow[0..n).
+void link_levels(LevelProgram& P, const uint32_t* next_qrow, size_t n);
 
 // Fused program of a uh_cheb level l and the coarsest level l+1 (which keeps
 // no rows).  A visit of level l with two Chebyshev steps then takes two
@@ -208,19 +208,20 @@ bool plan_pair(const LevelArrays& la, const LevelArrays& lb, uint32_t max_nodes,
 // slot 0, [1] slots, [2] nodes, [3] kept rows without L entries, [4] L depth
 // levels, [5] first U-list entry, [6] U-list length, [7] U depth levels.
 constexpr uint32_t kChainHdr = 8;
+// (HostVec: the planner's threads write, and so first touch, every element)
 struct ChainLevelProgram {
     uint32_t N = 0, M = 0, Md = 0;
     uint32_t nslots = 0, npt = 0;        // block slots; passthrough rows at positions nslots + t
-    std::vector<uint32_t> hdr;           // kChainHdr per block
-    std::vector<uint32_t> qrow;          // N: natural row at chain position q
-    std::vector<uint32_t> src;           // nslots: slot of the same row on the level above (not on level a)
-    std::vector<uint32_t> xl;            // nslots: natural row on the level above (where its combine goes)
-    std::vector<uint32_t> Ldep, Lptr;    // nslots (+1): node slots only have entries
-    std::vector<uint32_t> Lloc;          // slot of the L column
-    std::vector<double> Lval;
-    std::vector<uint32_t> Udep, uloc, uxidx, Uptr, Ucol;   // U list; Ucol: slot (eliminated or kept row)
-    std::vector<double> Uval, Udiag;
-    std::vector<double> dval;            // N: isolated-row factor at its chain position (0 elsewhere)
+    HostVec<uint32_t> hdr;               // kChainHdr per block
+    HostVec<uint32_t> qrow;              // N: natural row at chain position q
+    HostVec<uint32_t> src;               // nslots: slot of the same row on the level above (not on level a)
+    HostVec<uint32_t> xl;                // nslots: natural row on the level above (where its combine goes)
+    HostVec<uint32_t> Ldep, Lptr;        // nslots (+1): node slots only have entries
+    HostVec<uint32_t> Lloc;              // slot of the L column
+    HostVec<double> Lval;
+    HostVec<uint32_t> Udep, uloc, uxidx, Uptr, Ucol;   // U list; Ucol: slot (eliminated or kept row)
+    HostVec<double> Uval, Udiag;
+    HostVec<double> dval;                // N: isolated-row factor at its chain position (0 elsewhere)
 };
 struct ChainProgram {
     uint32_t a = 0, nlev = 0, nblk = 0;

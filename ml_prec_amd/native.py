@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmlp.so")
 
 MLP_FP64, MLP_FP32, MLP_FP32_BF16 = 0, 1, 2
 UH_CHEB, MULTI_SPLIT, DIAG, ID, GS, COMPOSITE, SYM = 1, 2, 3, 4, 5, 6, 7
+SCHED_CHAIN, SCHED_PAIR, SCHED_WIDE_CHAIN = 1, 2, 4        # mlp_prec_schedule bits
 X0_ZERO, X0_GIVEN, X0_REFERENCE = 0, 1, 2
 ORDERS = {"none": 0, "original": 1, "block": 2, "simple_1": 3}
 STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NOT_SPD", 4: "NO_ITER", 5: "MAXIT", 6: "CONSTRUCT", 7: "CONFIG",
@@ -135,6 +136,7 @@ EXPORTS = {
                                   C.POINTER(SolveStats)]),
     "mlp_simple_solver": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "mlp_prec_bytes": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
+    "mlp_prec_schedule": (C.c_int, [_vp, C.POINTER(C.c_uint32)]),
     "mlp_time_spmv": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.c_size_t, C.POINTER(C.c_double)]),
     "mlp_time_prec_apply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(C.c_double)]),
     "mlp_spe_nnz": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32]),
@@ -503,6 +505,12 @@ class DevPrec:
         n = C.c_uint32()
         check(lib().mlp_prec_bytes(self.h, C.byref(b), C.byref(n)))
         return n.value
+
+    def schedule(self):
+        """The launch schedule built: SCHED_* bits | forced level block size << 8."""
+        v = C.c_uint32()
+        check(lib().mlp_prec_schedule(self.h, C.byref(v)))
+        return v.value
 
 
 def _opts(eps, maxit, x0_mode, flexible, true_res_every, check_every, use_graph):
