@@ -667,8 +667,9 @@ template <class VT> struct DevHier : DevPrecBase {
         std::vector<double> urcp;
         up.add(D.map, D.N, L.map);
         up.add(C.hdr, V.hdr); up.add(C.qrow, V.qrow); up.add(C.src, V.src); up.add(C.xl, V.xl);
-        up.add(C.Ldep, V.Ldep); up.add(C.Lptr, V.Lptr); up.add(C.Lloc, V.Lloc); up.add(C.Lval, V.Lval);
-        up.add(C.Udep, V.Udep); up.add(C.uloc, V.uloc); up.add(C.uxidx, V.uxidx); up.add(C.Uptr, V.Uptr);
+        up.add(C.Lmeta, V.Lmeta); up.add(C.Lv2, V.Lv2); up.add(C.Lptr, V.Lptr); up.add(C.Lloc, V.Lloc);
+        up.add(C.Lval, V.Lval);
+        up.add(C.Umeta, V.Umeta); up.add(C.Uv1, V.Uv1); up.add(C.uxidx, V.uxidx); up.add(C.Uptr, V.Uptr);
         up.add(C.Ucol, V.Ucol); up.add(C.Uval, V.Uval); up.add(C.dval, V.dval);
         if constexpr (std::is_same_v<T, float>) {               // udiv: FP32 multiplies by 1 / u_ii
             urcp.resize(C.Udiag.size());

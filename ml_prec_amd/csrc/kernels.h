@@ -1246,11 +1246,98 @@ __device__ __forceinline__ uint32_t uload(const uint32_t* p) {
 
 template <class T> struct ChainLv {
     const uint32_t *hdr = nullptr, *qrow = nullptr, *src = nullptr, *xl = nullptr;
-    const uint32_t *Ldep = nullptr, *Lptr = nullptr, *Lloc = nullptr;
-    const uint32_t *Udep = nullptr, *uloc = nullptr, *uxidx = nullptr, *Uptr = nullptr, *Ucol = nullptr;
-    const T *Lval = nullptr, *Uval = nullptr, *Udiag = nullptr, *dval = nullptr;
+    const uint32_t *Lmeta = nullptr, *Lptr = nullptr, *Lloc = nullptr;
+    const uint32_t *Umeta = nullptr, *uxidx = nullptr, *Uptr = nullptr, *Ucol = nullptr;
+    const T *Lv2 = nullptr, *Lval = nullptr, *Uv1 = nullptr, *Uval = nullptr, *Udiag = nullptr, *dval = nullptr;
     uint32_t nslots = 0, npt = 0;
 };
+
+// A chain node's sweep data from its packed word (plan.h: ChainLevelProgram::Lmeta / Umeta) and inline
+// values: one round of loads with no dependence between them.  Entries past the first two (L) or the first
+// (U) are read through Lptr / Uptr, which only such rows load.
+// both inline values of a node in one load (the arrays are 256-byte aligned, a node's pair at 2 x its slot)
+__device__ __forceinline__ void load_pair(const double* p, double& a, double& b) {
+    const dbl2 v = *reinterpret_cast<const dbl2*>(p);
+    a = v.x;
+    b = v.y;
+}
+__device__ __forceinline__ void load_pair(const float* p, float& a, float& b) {
+    typedef float flt2 __attribute__((ext_vector_type(2)));
+    const flt2 v = *reinterpret_cast<const flt2*>(p);
+    a = v.x;
+    b = v.y;
+}
+// A chain node's L row: entries 0 and 1 from the packed word, entry 2 (rows of three entries occur on every
+// level of the SPE10 hierarchies) loaded with them, so that the sweep reads no global memory for rows of up to
+// three entries; entries past the third through Lloc / Lval at the sweep.
+template <class VT> struct ChainOwnL {
+    OwnL<VT> o;
+    uint32_t l2;
+    VT v2;
+};
+// (the values load with the word, unconditionally: an absent entry's value is stored as 0 and selected away by
+// the sweep's entry count, as in own_L / own_U)
+template <class VT>
+__device__ __forceinline__ ChainOwnL<VT> chain_own_L(uint32_t n, uint32_t nl, uint32_t base, const ChainLv<VT>& L) {
+    const uint32_t p = threadIdx.x;
+    ChainOwnL<VT> e{OwnL<VT>{nl, 0u, 0u, p, p, VT(0), VT(0)}, p, VT(0)};
+    OwnL<VT>& o = e.o;
+    if (p < n) {
+        const uint32_t m = L.Lmeta[base + p], q0 = L.Lptr[base + p];
+        load_pair(L.Lv2 + 2 * size_t(base + p), o.v0, o.v1);
+        const uint32_t c = (m >> 8) & 0xffu;
+        o.my = m & 0xffu;
+        o.q0 = q0;
+        o.q1 = q0 + c;
+        if (c > 0) o.l0 = (m >> 16) & 0xffu;
+        if (c > 1) o.l1 = m >> 24;
+        if (c > 2) {
+            e.l2 = L.Lloc[q0 + 2];
+            e.v2 = L.Lval[q0 + 2];
+        }
+    }
+    return e;
+}
+// sweep_L for a chain node: its entries in storage order, the first three from registers
+template <int BS, class VT>
+__device__ __forceinline__ VT chain_sweep_L(const ChainOwnL<VT>& e, uint32_t nl, const uint32_t* __restrict__ Lloc,
+                                            const VT* __restrict__ Lval, VT acc, VT* sw) {
+    const OwnL<VT>& o = e.o;
+    const bool h0 = o.q1 > o.q0, h1 = o.q1 > o.q0 + 1, h2 = o.q1 > o.q0 + 2, more = o.q1 > o.q0 + 3;
+    for (uint32_t li = 0; li < nl; li++) {
+        if (li == o.my) {
+            const VT x0 = sw[o.l0], x1 = sw[o.l1];
+            acc = h0 ? msub(acc, o.v0, x0) : acc;
+            acc = h1 ? msub(acc, o.v1, x1) : acc;
+            if (h2) {
+                acc = msub(acc, e.v2, sw[e.l2]);
+                if (more)
+                    for (uint32_t q = o.q0 + 3; q < o.q1; q++) acc = msub(acc, Lval[q], sw[Lloc[q]]);
+            }
+            sw[threadIdx.x] = acc;
+        }
+        level_sync<BS>();
+    }
+    return acc;
+}
+template <class VT>
+__device__ __forceinline__ OwnU<VT> chain_own_U(uint32_t nu, uint32_t nl, uint32_t ub, const ChainLv<VT>& L) {
+    const uint32_t u = threadIdx.x;
+    OwnU<VT> o{nl, 0u, 0u, 0u, 0u, 0u, VT(1), VT(0), VT(0)};
+    if (u < nu) {
+        const uint32_t m = L.Umeta[ub + u];
+        o.d = L.Udiag[ub + u];
+        o.v0 = L.Uv1[ub + u];
+        const uint32_t c = (m >> 8) & 0xffu;
+        o.my = m & 0xffu;
+        o.loc = (m >> 16) & 0xffu;
+        o.q0 = c > 1 ? L.Uptr[ub + u] : 0u;
+        o.q1 = o.q0 + c;
+        if (c > 0) o.c0 = m >> 24;
+        if (c > 1) { o.c1 = L.Ucol[o.q0 + 1]; o.v1 = L.Uval[o.q0 + 1]; }
+    }
+    return o;
+}
 // a chain level's vectors as the parent of the next level (cheb_prec.h:36), and
 // the combine of its second Chebyshev step (cheb_solve.cc:79-81)
 template <class VT> struct ChainVec {
@@ -1299,19 +1386,19 @@ __global__ void __launch_bounds__(BS) k_chain_down(ChainView<VT, CT> V, uint32_t
         uint32_t base[kChainMax], S[kChainMax], nn[kChainMax], nkeep[kChainMax], nlL[kChainMax], qr[kChainMax],
             sr[kChainMax];
         VT dv[kChainMax];
-        OwnL<VT> ol[kChainMax];
+        ChainOwnL<VT> ol[kChainMax];
         VT acc = VT(0);
 #pragma unroll
         for (int i = 0; i < kChainMax; i++) {
             base[i] = S[i] = nn[i] = nkeep[i] = nlL[i] = qr[i] = sr[i] = 0;
             dv[i] = VT(0);
-            ol[i] = OwnL<VT>{0u, 0u, 0u, t, t, VT(0), VT(0)};
+            ol[i] = ChainOwnL<VT>{OwnL<VT>{0u, 0u, 0u, t, t, VT(0), VT(0)}, t, VT(0)};
             if (uint32_t(i) < i0 || uint32_t(i) >= nv) continue;
             const ChainLv<VT>& L = V.L[i];
             const uint32_t* H = L.hdr + kChainHdrW * b;
             base[i] = uload(H); S[i] = uload(H + 1); nn[i] = uload(H + 2); nkeep[i] = nn[i] + uload(H + 3);
             nlL[i] = uload(H + 4);
-            ol[i] = own_L(nn[i], nlL[i], base[i], L.Ldep, L.Lptr, L.Lloc, L.Lval);
+            ol[i] = chain_own_L(nn[i], nlL[i], base[i], L);
             if (t < S[i]) {
                 qr[i] = L.qrow[base[i] + t];
                 if (uint32_t(i) > i0) sr[i] = L.src[base[i] + t];
@@ -1323,8 +1410,8 @@ __global__ void __launch_bounds__(BS) k_chain_down(ChainView<VT, CT> V, uint32_t
         const uint32_t* Hc = C.hdr + kChainHdrW * b;
         const uint32_t baseC = uload(Hc), SC = uload(Hc + 1), nnC = uload(Hc + 2), nlLC = uload(Hc + 4),
                        ubC = uload(Hc + 5), nuC = uload(Hc + 6), nlUC = uload(Hc + 7);
-        const OwnL<CT> olC = own_L(nnC, nlLC, baseC, C.Ldep, C.Lptr, C.Lloc, C.Lval);
-        const OwnU<CT> ouC = own_U(nuC, nlUC, ubC, C.Udep, C.uloc, C.Udiag, C.Uptr, C.Ucol, C.Uval);
+        const ChainOwnL<CT> olC = chain_own_L(nnC, nlLC, baseC, C);
+        const OwnU<CT> ouC = chain_own_U(nuC, nlUC, ubC, C);
         uint32_t qrC = 0, srC = 0, uxC = 0;
         CT dvC = CT(0);
         if (t < SC) {
@@ -1364,7 +1451,7 @@ __global__ void __launch_bounds__(BS) k_chain_down(ChainView<VT, CT> V, uint32_t
                 }
                 s[q] = w;
             }
-            const VT w = sweep_L<BS>(ol[i], nlL[i], L.Lloc, L.Lval, v, s);
+            const VT w = chain_sweep_L<BS>(ol[i], nlL[i], L.Lloc, L.Lval, v, s);
             if (t < nn[i]) V.V[i].wt[y * V.V[i].N + base[i] + t] = w;
             level_sync<BS>();
             cur ^= 1;
@@ -1383,7 +1470,7 @@ __global__ void __launch_bounds__(BS) k_chain_down(ChainView<VT, CT> V, uint32_t
             pu1[C.qrow[baseC + q]] = VT(C.dval[baseC + q] * CT(w));
         }
         const CT f2 = CT(v);
-        sweep_L<BS>(olC, nlLC, C.Lloc, C.Lval, f2, cw);
+        chain_sweep_L<BS>(olC, nlLC, C.Lloc, C.Lval, f2, cw);
         level_sync<BS>();
         const CT z = t < nuC ? cw[ouC.loc] : CT(0);
         const CT x = sweep_U<BS, CT, false>(ouC, nlUC, C.Ucol, C.Uval, z, (const CT*)nullptr, cx);
@@ -1447,8 +1534,8 @@ __global__ void __launch_bounds__(BS) k_chain_up(ChainView<VT, CT> V, uint32_t i
                        ubC = uload(Hc + 5), nuC = uload(Hc + 6), nlUC = uload(Hc + 7);
         // every per-slot load first: the coarsest level's residual and sweeps, each level's U sweep with its
         // w, the next level's rows with their combine operand, the isolated rows' values
-        const OwnL<CT> olC = own_L(nnC, nlLC, baseC, C.Ldep, C.Lptr, C.Lloc, C.Lval);
-        const OwnU<CT> ouC = own_U(nuC, nlUC, ubC, C.Udep, C.uloc, C.Udiag, C.Uptr, C.Ucol, C.Uval);
+        const ChainOwnL<CT> olC = chain_own_L(nnC, nlLC, baseC, C);
+        const OwnU<CT> ouC = chain_own_U(nuC, nlUC, ubC, C);
         CT f2 = CT(0), dvC = CT(0);
         if (t < SC) {
             f2 = sell_row<CT, VT, SCALE>(V.A2, pu1, baseC + t, V.a1c) - CT(pF[baseC + t]);
@@ -1471,7 +1558,7 @@ __global__ void __launch_bounds__(BS) k_chain_up(ChainView<VT, CT> V, uint32_t i
             const uint32_t ub = uload(H + 5);
             base[i] = uload(H); S[i] = uload(H + 1); nkeep[i] = uload(H + 2) + uload(H + 3); nu[i] = uload(H + 6);
             nlU[i] = uload(H + 7);
-            ou[i] = own_U(nu[i], nlU[i], ub, L.Udep, L.uloc, L.Udiag, L.Uptr, L.Ucol, L.Uval);
+            ou[i] = chain_own_U(nu[i], nlU[i], ub, L);
             if (t < nu[i]) {
                 z[i] = Vi.wt[y * Vi.N + base[i] + ou[i].loc];
                 if (uint32_t(i) == ie) ux[i] = L.uxidx[ub + t];
@@ -1498,7 +1585,7 @@ __global__ void __launch_bounds__(BS) k_chain_up(ChainView<VT, CT> V, uint32_t i
         }
         if (gflag) return;
         // step 2 on the coarsest level
-        sweep_L<BS>(olC, nlLC, C.Lloc, C.Lval, f2, cw);
+        chain_sweep_L<BS>(olC, nlLC, C.Lloc, C.Lval, f2, cw);
         if (t < SC && t >= nnC) cx[t] = dvC * f2;
         for (uint32_t q = t + BS; q < SC; q += BS) {
             const CT g = sell_row<CT, VT, SCALE>(V.A2, pu1, baseC + q, V.a1c) - CT(pF[baseC + q]);

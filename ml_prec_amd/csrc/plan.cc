@@ -802,6 +802,7 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
                 ChainProgram& out) {
     const uint32_t nl = uint32_t(L.size());
     if (nl < 2) return false;
+    if (max_slots > 256 || max_nodes > max_slots) throw Error("chain blocks hold at most 256 slots", 1);
     constexpr uint64_t kGrain = 1 << 14;
     // the structure the kernels rely on (as plan_level checks it)
     for (uint32_t k = 0; k < nl; k++) {
@@ -1153,6 +1154,10 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
         C.Uptr[0] = 0;
         C.Ucol.resize(offU[nblk]);
         C.Uval.resize(offU[nblk]);
+        C.Lmeta.resize(ns);
+        C.Lv2.resize(size_t(2) * ns);
+        C.Umeta.resize(NU);
+        C.Uv1.resize(NU);
         FirstError fe;
         parallel_for(nblk, 16, [&](uint64_t b0, uint64_t b1) {
             for (uint64_t b = b0; b < b1; b++) {
@@ -1173,6 +1178,7 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
                     }
                     const bool node = s < H[2];
                     C.Ldep[q] = node ? dL[i] : 0u;
+                    const uint32_t l0 = cl;
                     if (node)
                         for (uint32_t p = lv.L_ia[i]; p < lv.L_ia[i + 1]; p++) {
                             C.Lloc[cl] = sk[lv.L_ja[p]];
@@ -1180,6 +1186,14 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
                             cl++;
                         }
                     C.Lptr[q + 1] = cl;
+                    const uint32_t ne = cl - l0;
+                    if (C.Ldep[q] > 0xffu || ne > 0xffu) {
+                        fe.put((b << 32) | s, "chain: an L depth or entry count does not fit 8 bits");
+                        return;
+                    }
+                    C.Lmeta[q] = C.Ldep[q] | ne << 8 | (ne > 0 ? C.Lloc[l0] : 0u) << 16 | (ne > 1 ? C.Lloc[l0 + 1] : 0u) << 24;
+                    C.Lv2[2 * size_t(q)] = ne > 0 ? C.Lval[l0] : 0.0;
+                    C.Lv2[2 * size_t(q) + 1] = ne > 1 ? C.Lval[l0 + 1] : 0.0;
                     C.dval[q] = i >= K ? lv.dval[i - K] : 0.0;
                 }
                 uint32_t cu = offU[b];
@@ -1189,12 +1203,20 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
                     C.uloc[u] = sk[i];
                     C.uxidx[u] = lv.map[i];
                     C.Udiag[u] = lv.U_a[lv.U_ia[i]];
+                    const uint32_t u0 = cu;
                     for (uint32_t p = lv.U_ia[i] + 1; p < lv.U_ia[i + 1]; p++) {
                         C.Ucol[cu] = sk[lv.U_ja[p]];
                         C.Uval[cu] = lv.U_a[p];
                         cu++;
                     }
                     C.Uptr[u + 1] = cu;
+                    const uint32_t ne = cu - u0;
+                    if (C.Udep[u] > 0xffu || ne > 0xffu || C.uloc[u] > 0xffu) {
+                        fe.put((b << 32) | 0xffffffffu, "chain: a U depth, entry count or slot does not fit 8 bits");
+                        return;
+                    }
+                    C.Umeta[u] = C.Udep[u] | ne << 8 | C.uloc[u] << 16 | (ne > 0 ? C.Ucol[u0] : 0u) << 24;
+                    C.Uv1[u] = ne > 0 ? C.Uval[u0] : 0.0;
                 }
             }
         });
@@ -1223,9 +1245,9 @@ bool plan_chain(const std::vector<LevelArrays>& L, uint32_t a, uint32_t max_node
             if (bad) throw Error("chain: level rows are not covered exactly once", 1);
         }
         // no empty device arrays
-        for (auto* v : {&C.src, &C.xl, &C.Lloc, &C.Udep, &C.uloc, &C.uxidx, &C.Ucol, &C.hdr})
+        for (auto* v : {&C.src, &C.xl, &C.Lloc, &C.Udep, &C.uloc, &C.uxidx, &C.Ucol, &C.hdr, &C.Lmeta, &C.Umeta})
             if (v->empty()) v->push_back(0);
-        for (auto* v : {&C.Lval, &C.Uval, &C.Udiag})
+        for (auto* v : {&C.Lval, &C.Uval, &C.Udiag, &C.Lv2, &C.Uv1})
             if (v->empty()) v->push_back(0.0);
     }
     for (uint32_t k = 0; k < nl; k++)

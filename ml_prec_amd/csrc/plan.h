@@ -222,6 +222,12 @@ struct ChainLevelProgram {
     HostVec<uint32_t> Udep, uloc, uxidx, Uptr, Ucol;   // U list; Ucol: slot (eliminated or kept row)
     HostVec<double> Uval, Udiag;
     HostVec<double> dval;                // N: isolated-row factor at its chain position (0 elsewhere)
+    // The same node data packed for the kernels (8-bit fields: depths, counts and slots are below 256 in
+    // blocks of at most 256 slots): Lmeta per slot = L depth | entries << 8 | slot of entry 0 << 16 | slot of
+    // entry 1 << 24, Lv2 its first two values; Umeta per U-list entry = U depth | off-diagonal entries << 8 |
+    // loc << 16 | slot of entry 0 << 24, Uv1 that entry's value.  Entries past those run through Lptr / Uptr.
+    HostVec<uint32_t> Lmeta, Umeta;
+    HostVec<double> Lv2, Uv1;
 };
 struct ChainProgram {
     uint32_t a = 0, nlev = 0, nblk = 0;

@@ -235,8 +235,9 @@ class Hierarchy:
 
 
 def build_uh_cheb(ia, ja, a, sigmas=(3.0,), niters=(2,), use_tails=True, ctx=None):
-    """Prec(A, cfg) construction (defaults: sigma 3, ncheb 2, tails on).  With a Context, each level's link
-    phase runs on its device (mlp_hier_build_uh_cheb_dev); the hierarchy is the same."""
+    """Prec(A, cfg) construction (defaults: sigma 3, ncheb 2, tails on).  With a Context, every level is built
+    on its device (mlp_hier_build_uh_cheb_dev: link dropping, tail order, elimination); the hierarchy is the
+    same bit for bit."""
     h = _vp()
     n = len(ia) - 1
     args = (n, _u32(ia), _u32(ja), _f64(a), _f64(sigmas), len(sigmas), _u32(niters), len(niters), int(use_tails),

@@ -49,6 +49,34 @@ static std::vector<FileLevel> lv;
 static std::vector<Coef> co;           // per level l: the iteration on level l+1
 static uint32_t nlev;
 
+static int packed_bad = 0;              // packed node words that disagree with the full arrays
+
+// A node's L entries as the kernels read them: the first two from its packed word and inline values
+// (ChainLevelProgram::Lmeta, Lv2), the rest through Lptr; the word's depth and count must be Ldep's and Lptr's.
+template <class Vec> static double l_sub(const ChainLevelProgram& C, uint32_t q, double acc, const Vec& x) {
+    const uint32_t m = C.Lmeta[q], c = (m >> 8) & 0xffu, p0 = C.Lptr[q];
+    if ((m & 0xffu) != C.Ldep[q] || c != C.Lptr[q + 1] - p0 ||
+        (c > 0 && (((m >> 16) & 0xffu) != C.Lloc[p0] || C.Lv2[2 * size_t(q)] != C.Lval[p0])) ||
+        (c > 1 && ((m >> 24) != C.Lloc[p0 + 1] || C.Lv2[2 * size_t(q) + 1] != C.Lval[p0 + 1])))
+        packed_bad++;
+    if (c > 0) acc -= C.Lv2[2 * size_t(q)] * x[(m >> 16) & 0xffu];
+    if (c > 1) acc -= C.Lv2[2 * size_t(q) + 1] * x[m >> 24];
+    for (uint32_t p = p0 + 2; p < p0 + c; p++) acc -= C.Lval[p] * x[C.Lloc[p]];
+    return acc;
+}
+// A U-list entry's solve as the kernels read it: its first off-diagonal entry from the packed word and Uv1,
+// the rest through Uptr; writes x[loc] and returns loc.
+template <class Vec> static uint32_t u_solve(const ChainLevelProgram& C, uint32_t u, double z, Vec& x) {
+    const uint32_t m = C.Umeta[u], c = (m >> 8) & 0xffu, p0 = C.Uptr[u], loc = (m >> 16) & 0xffu;
+    if ((m & 0xffu) != C.Udep[u] || c != C.Uptr[u + 1] - p0 || loc != C.uloc[u] ||
+        (c > 0 && ((m >> 24) != C.Ucol[p0] || C.Uv1[u] != C.Uval[p0])))
+        packed_bad++;
+    if (c > 0) z -= C.Uv1[u] * x[m >> 24];
+    for (uint32_t p = p0 + 1; p < p0 + c; p++) z -= C.Uval[p] * x[C.Ucol[p]];
+    x[loc] = z / C.Udiag[u];
+    return loc;
+}
+
 // A_{l+1} row r times (s u), entries in storage order (sym_multiply's full-row order)
 static double arow(uint32_t l, uint32_t r, const std::vector<double>& u, double s, bool scale) {
     const FileLevel& L = lv[l];
@@ -142,18 +170,15 @@ static void descent(uint32_t j, const Input& in, std::vector<double>& out_j) {
                 if (s >= nkeep) out_of(k)[C.qrow[base + s]] = C.dval[base + s] * v;
             }
             for (uint32_t s = 0; s < nn; s++) {          // slot order is depth order
-                double acc = V[k][s];
-                for (uint32_t p = C.Lptr[base + s]; p < C.Lptr[base + s + 1]; p++) acc -= C.Lval[p] * V[k][C.Lloc[p]];
+                const double acc = l_sub(C, base + s, V[k][s], V[k]);
                 V[k][s] = acc;
                 if (k < c) B.wt[k][base + s] = acc;
             }
             if (k == c) {                                // step 1 on the coarsest level
                 std::vector<double> X(S, 0.0);
                 for (uint32_t u = H[5]; u < H[5] + H[6]; u++) {
-                    double z = V[k][C.uloc[u]];
-                    for (uint32_t p = C.Uptr[u]; p < C.Uptr[u + 1]; p++) z -= C.Uval[p] * X[C.Ucol[p]];
-                    X[C.uloc[u]] = z / C.Udiag[u];
-                    B.u1[c - 1][C.uxidx[u]] = X[C.uloc[u]];
+                    const uint32_t loc = u_solve(C, u, V[k][C.uloc[u]], X);
+                    B.u1[c - 1][C.uxidx[u]] = X[loc];
                 }
             }
             (void)L;
@@ -197,16 +222,8 @@ static void ascent(uint32_t ke, std::vector<double>& out_top) {
                 w[s] = arow(c - 1, C.qrow[base + s], B.u1[c - 1], cc.a1, !cc.s1_skip) - B.F[c - 1][base + s];
             X[c].assign(S, 0.0);
             for (uint32_t s = nn; s < S; s++) X[c][s] = C.dval[base + s] * w[s];
-            for (uint32_t s = 0; s < nn; s++) {
-                double acc = w[s];
-                for (uint32_t p = C.Lptr[base + s]; p < C.Lptr[base + s + 1]; p++) acc -= C.Lval[p] * w[C.Lloc[p]];
-                w[s] = acc;
-            }
-            for (uint32_t u = H[5]; u < H[5] + H[6]; u++) {
-                double z = w[C.uloc[u]];
-                for (uint32_t p = C.Uptr[u]; p < C.Uptr[u + 1]; p++) z -= C.Uval[p] * X[c][C.Ucol[p]];
-                X[c][C.uloc[u]] = z / C.Udiag[u];
-            }
+            for (uint32_t s = 0; s < nn; s++) w[s] = l_sub(C, base + s, w[s], w);
+            for (uint32_t u = H[5]; u < H[5] + H[6]; u++) u_solve(C, u, w[C.uloc[u]], X[c]);
         }
         for (uint32_t k = c; k-- > ke;) {
             const ChainLevelProgram& C = CP.lv[k - a];
@@ -225,10 +242,8 @@ static void ascent(uint32_t ke, std::vector<double>& out_top) {
             if (k > ke)
                 for (uint32_t s = nkeep; s < S; s++) X[k][s] = B.x2[k - 1][C.qrow[base + s]];
             for (uint32_t u = H[5]; u < H[5] + H[6]; u++) {
-                double z = B.wt[k][base + C.uloc[u]];
-                for (uint32_t p = C.Uptr[u]; p < C.Uptr[u + 1]; p++) z -= C.Uval[p] * X[k][C.Ucol[p]];
-                X[k][C.uloc[u]] = z / C.Udiag[u];
-                if (k == ke) out[C.uxidx[u]] = X[k][C.uloc[u]];
+                const uint32_t loc = u_solve(C, u, B.wt[k][base + C.uloc[u]], X[k]);
+                if (k == ke) out[C.uxidx[u]] = X[k][loc];
             }
         }
     }
@@ -321,12 +336,13 @@ int main(int argc, char** argv) {
     std::normal_distribution<double> nd;
     const uint32_t c = nlev - 1;
     int bad = 0;
+    for (const uint32_t bs : {64u, 128u})       // one- and two-wavefront blocks
     for (uint32_t a = 0; a < c; a++) {
         std::vector<LevelArrays> la;
         for (uint32_t l = a; l < nlev; l++) la.push_back(lv[l].arrays());
         ChainProgram P;
-        if (!plan_chain(la, a, 64, 128, P)) {
-            std::printf("chain %u..%u: does not fit\n", a, c);
+        if (!plan_chain(la, a, bs, 2 * bs, P)) {
+            std::printf("chain %u..%u, %u-thread blocks: does not fit\n", a, c, bs);
             continue;
         }
         CP = std::move(P);
@@ -345,9 +361,13 @@ int main(int argc, char** argv) {
         size_t diff = 0;
         for (uint32_t i = 0; i < lv[a].N; i++)
             if (std::memcmp(&xr[i], &xc[i], sizeof(double)) != 0) diff++;
-        std::printf("chain %u..%u: %u blocks, max slots %u, max nodes %u, %u launches, %zu of %u rows differ\n", a,
-                    c, CP.nblk, CP.max_slots, CP.max_nodes, launches, diff, lv[a].N);
+        std::printf("chain %u..%u, %u-thread blocks: %u blocks, max slots %u, max nodes %u, %u launches, %zu of %u rows "
+                    "differ\n", a, c, bs, CP.nblk, CP.max_slots, CP.max_nodes, launches, diff, lv[a].N);
         if (diff) bad++;
+    }
+    if (packed_bad) {
+        std::printf("%d packed node words disagree with the full arrays\n", packed_bad);
+        bad++;
     }
     std::printf(bad ? "FAIL %d\n" : "OK\n", bad);
     return bad ? 1 : 0;

@@ -105,8 +105,9 @@ def _write_levels(path, H):
 @pytest.mark.parametrize("name", ["spe10_z12", "ln5_tpfa_c1", "spe10_z40"])
 def test_chain_launches_equal_recursion(chain_exe, tmp_path, name):
     """The chain program (plan.cc: plan_chain) and the descent/ascent launch sequence, executed on the host slot
-    by slot as the device kernels do, give Prec::solve's result (cheb_solve.cc:13-123, plain recursion over the
-    level arrays) bit for bit, for chains starting at every level."""
+    by slot as the device kernels do (node data from the packed words), give Prec::solve's result
+    (cheb_solve.cc:13-123, plain recursion over the level arrays) bit for bit, for chains starting at every level,
+    in one- and two-wavefront blocks; every packed word agrees with the full arrays."""
     from conftest import system
     from ml_prec_amd import native as N
     s = system(name)
