@@ -3470,6 +3470,7 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
         c = next;
     }
     std::vector<SolveState> fin(ctx->mst_host, ctx->mst_host + R);
+    std::vector<double> trr(R, -1.0);          // a final x's true residual, when the restart test has it
     if (flex) {
         // mlp_pcg's restarts from the FP64 true residual, per right-hand side
         for (uint32_t q = 0; q < R; q++) {
@@ -3477,7 +3478,11 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
             const double* bq = b + size_t(q) * n;
             double* xq = x + size_t(q) * n;
             for (uint32_t restart = 0; restart < kFlexRestarts && st.status == 0; restart++) {
-                if (true_rel_res(ctx, A, bq, xq, ctx->r, st.init_norm) <= st.eps) break;
+                const double t = true_rel_res(ctx, A, bq, xq, ctx->r, st.init_norm);
+                if (t <= st.eps) {
+                    trr[q] = t;
+                    break;
+                }
                 bool again = false;
                 const double init_norm = st.init_norm, eps = st.eps;
                 st = pcg_phase(ctx, A, B, bq, xq, o, init_norm, eps, st.niter, &again);
@@ -3500,7 +3505,8 @@ int mlp_pcg_multi(mlp_ctx ctx, mlp_mat A, mlp_prec B, uint32_t nrhs, const doubl
         S.final_norm = st.norm;
         S.eps_used = st.eps;
         S.t_solve_ms = ms;
-        S.true_rel_res = true_rel_res(ctx, A, b + size_t(k) * n, x + size_t(k) * n, ctx->r, st.init_norm);
+        S.true_rel_res = trr[k] >= 0.0 ? trr[k]
+                                       : true_rel_res(ctx, A, b + size_t(k) * n, x + size_t(k) * n, ctx->r, st.init_norm);
         int code = MLP_OK;
         const char* what = "";
         if (st.status == 4) {
@@ -3918,9 +3924,14 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
                 J.stats.status = MLP_ERR_NO_ITER;
                 throw Error("No need to iterate", MLP_ERR_NO_ITER);
             }
+            double trr = -1.0;                    // the final x's true residual, when the restart test has it
             if (flex) {
                 for (uint32_t restart = 0; restart < kFlexRestarts && st.status == 0; restart++) {
-                    if (true_rel_res(ctx, J.A, J.d_b, J.d_x, ctx->r, init_norm) <= eps) break;
+                    const double t = true_rel_res(ctx, J.A, J.d_b, J.d_x, ctx->r, init_norm);
+                    if (t <= eps) {
+                        trr = t;
+                        break;
+                    }
                     bool again = false;
                     st = pcg_phase(ctx, J.A, J.B, J.d_b, J.d_x, o, init_norm, eps, st.niter, &again);
                     if (again) break;
@@ -3933,7 +3944,7 @@ int mlp_pcg_batch(uint32_t count, mlp_pcg_job* jobs, const mlp_solve_opts* o) {
             J.stats.eps_used = eps;
             J.stats.t_solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - R.t0).count();
             J.stats.status = int(st.status);
-            J.stats.true_rel_res = true_rel_res(ctx, J.A, J.d_b, J.d_x, ctx->r, init_norm);
+            J.stats.true_rel_res = trr >= 0.0 ? trr : true_rel_res(ctx, J.A, J.d_b, J.d_x, ctx->r, init_norm);
             if (st.status == 3) {
                 char buf[160];
                 std::snprintf(buf, sizeof(buf), "<Ap, p> = %g", st.app);
@@ -4337,13 +4348,18 @@ int mlp_pcg_noalloc(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double*
     const double init_norm = st.init_norm, eps = st.eps;
     int code = MLP_OK;
     std::string msg;
+    double known = -1.0;                           // the final x's true residual, when the restart test has it
     if (st.status == 4) {
         code = MLP_ERR_NO_ITER;
         msg = "No need to iterate";
     } else {
         if (flex) {
             for (uint32_t restart = 0; restart < kFlexRestarts && st.status == 0; restart++) {
-                if (true_rel_res(ctx, A, b, x, V, init_norm) <= eps) break;
+                const double t = true_rel_res(ctx, A, b, x, V, init_norm);
+                if (t <= eps) {
+                    known = t;
+                    break;
+                }
                 bool again = false;
                 st = pcg_phase(ctx, A, B, b, x, o, init_norm, eps, st.niter, &again, &V);
                 if (again) break;
@@ -4351,7 +4367,7 @@ int mlp_pcg_noalloc(mlp_ctx ctx, mlp_mat A, mlp_prec B, const double* b, double*
         }
         MLP_HIP_CHECK(hipStreamSynchronize(s));
     }
-    const double trr = code == MLP_OK ? true_rel_res(ctx, A, b, x, V, init_norm) : 0.0;
+    const double trr = code != MLP_OK ? 0.0 : known >= 0.0 ? known : true_rel_res(ctx, A, b, x, V, init_norm);
     if (code == MLP_OK) {
         if (st.status == 3) {
             code = MLP_ERR_NOT_SPD;
