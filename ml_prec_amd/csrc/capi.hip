@@ -389,41 +389,72 @@ inline PinnedStage& pinned_stage(int device) {
 // them (converting element types where asked) into the device's pinned
 // staging buffer (a block over max_pinned() goes through a pageable buffer
 // instead), one copy moves the block, and flush() then points every
-// registered field at its array.  The host arrays must live until flush().
+// registered field at its array and runs the actions queued with after().
+// The host arrays must live until flush(); own() keeps a temporary alive
+// that long.  Packing runs as one parallel loop over pieces of at most 1 MB
+// of every array.
 struct Stager {
     mlp_ctx ctx;
     DevMem& mem;
     struct Item {
-        size_t off;
-        std::function<void(char*)> write, set;
+        size_t off, bytes;
+        std::function<void(char*, size_t, size_t)> write;   // elements [b, e) into the block at this item
+        std::function<void(char*)> set;
+        size_t elem;
     };
     std::vector<Item> items;
+    std::vector<std::shared_ptr<void>> owned;
+    std::vector<std::function<void()>> actions;
     size_t total = 0;
     Stager(mlp_ctx c, DevMem& m) : ctx(c), mem(m) {}
     template <class P, class S> void add(const S* src, size_t n, P& field) {
         using T = std::remove_const_t<std::remove_pointer_t<P>>;
         const size_t off = total;
         total += (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~size_t(255);
-        items.push_back({off,
-                         [src, n](char* dst) {
+        items.push_back({off, n * sizeof(T),
+                         [src](char* dst, size_t b, size_t e) {
                              T* d = reinterpret_cast<T*>(dst);
-                             parallel_for(n, 1 << 16, [&](uint64_t b, uint64_t e) {
-                                 if constexpr (std::is_same_v<T, S>) std::memcpy(d + b, src + b, (e - b) * sizeof(T));
-                                 else for (uint64_t i = b; i < e; i++) d[i] = T(src[i]);
-                             });
+                             if constexpr (std::is_same_v<T, S>) std::memcpy(d + b, src + b, (e - b) * sizeof(T));
+                             else for (size_t i = b; i < e; i++) d[i] = T(src[i]);
                          },
-                         [&field](char* base) { field = reinterpret_cast<P>(base); }});
+                         [&field](char* base) { field = reinterpret_cast<P>(base); }, sizeof(T)});
     }
     template <class P, class S, class Al> void add(const std::vector<S, Al>& v, P& field) {
         add(v.data(), v.size(), field);
     }
+    // a temporary that lives until flush()
+    template <class V> V& own(V&& v) {
+        auto p = std::make_shared<std::decay_t<V>>(std::move(v));
+        owned.push_back(p);
+        return *p;
+    }
+    void after(std::function<void()> f) { actions.push_back(std::move(f)); }
     // blocks up to this size use the pinned buffer: 256 MB, or MLP_STAGE_MAX bytes
     static size_t max_pinned() {
         const char* e = getenv("MLP_STAGE_MAX");
         return e ? size_t(std::strtoull(e, nullptr, 10)) : size_t(256) << 20;
     }
+    void pack(char* host) {
+        struct Piece {
+            const Item* it;
+            size_t b, e;
+        };
+        std::vector<Piece> pieces;
+        for (const Item& it : items) {
+            const size_t n = it.bytes / it.elem, step = std::max<size_t>(1, (size_t(1) << 20) / it.elem);
+            for (size_t b = 0; b < n; b += step) pieces.push_back({&it, b, std::min(n, b + step)});
+        }
+        parallel_for(pieces.size(), 1, [&](uint64_t p0, uint64_t p1) {
+            for (uint64_t p = p0; p < p1; p++) pieces[p].it->write(host + pieces[p].it->off, pieces[p].b, pieces[p].e);
+        });
+    }
     void flush() {
-        if (items.empty()) return;
+        if (items.empty()) {
+            for (auto& f : actions) f();
+            actions.clear();
+            owned.clear();
+            return;
+        }
         char* dev = mem.alloc<char>(total);
         if (total <= max_pinned()) {
             PinnedStage& st = pinned_stage(ctx->device);
@@ -435,58 +466,66 @@ struct Stager {
                 MLP_HIP_CHECK(hipHostMalloc((void**)&st.p, total, hipHostMallocDefault));
                 st.n = total;
             }
-            for (Item& it : items) it.write(st.p + it.off);
+            pack(st.p);
             MLP_HIP_CHECK(hipMemcpyAsync(dev, st.p, total, hipMemcpyHostToDevice, ctx->stream));
             MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         } else {
             HostVec<char> buf(total);
-            for (Item& it : items) it.write(buf.data() + it.off);
+            pack(buf.data());
             MLP_HIP_CHECK(hipMemcpyAsync(dev, buf.data(), total, hipMemcpyHostToDevice, ctx->stream));
             MLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         }
         for (Item& it : items) it.set(dev + it.off);
         items.clear();
         total = 0;
+        for (auto& f : actions) f();
+        actions.clear();
+        owned.clear();
     }
 };
 
 // A matrix in its own row order is stored in the seven-point layout when it is
 // a symmetric seven-point grid operator (MLP_NO_STENCIL=1 keeps the slices),
 // else as sliced ELL.
+// (staged on `up`; its device pointers are set at up.flush())
 template <class VT>
-void upload_sell(mlp_ctx ctx, DevMem& mem, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
-                 DevSell<VT>& out, const uint32_t* rows = nullptr) {
+void upload_sell(Stager& up, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a, DevSell<VT>& out,
+                 const uint32_t* rows = nullptr) {
     const char* off = getenv("MLP_NO_STENCIL");
     if (!rows && !(off && off[0] == '1')) {
         StencilLayout T;
         if (plan_stencil(n, ia, ja, a, T)) {
+            StencilLayout& K = up.own(std::move(T));
             out.v.n = n;
-            Stager up(ctx, mem);
-            up.add(T.pres, out.v.spres);
-            up.add(T.d, out.v.sd);
-            up.add(T.lx, out.v.slx);
-            up.add(T.ly, out.v.sly);
-            up.add(T.lz, out.v.slz);
-            up.flush();
-            out.v.ssy = T.sy;
-            out.v.ssz = T.sz;
+            up.add(K.pres, out.v.spres);
+            up.add(K.d, out.v.sd);
+            up.add(K.lx, out.v.slx);
+            up.add(K.ly, out.v.sly);
+            up.add(K.lz, out.v.slz);
+            out.v.ssy = K.sy;
+            out.v.ssz = K.sz;
             out.stencil = true;
             out.nent = 0;
             out.nnz = ia[n];
             return;
         }
     }
-    SellLayout L;
+    SellLayout& L = up.own(SellLayout());
     plan_sell(n, ia, ja, a, L, rows);
     out.v.n = n;
-    Stager up(ctx, mem);
     up.add(L.off64, out.v.off64);
     if (!L.dcol.empty()) up.add(L.dcol, out.v.dcol);
     else up.add(L.col, out.v.col);
     up.add(L.val, out.v.val);
-    up.flush();
     out.nent = L.nent;
     out.nnz = n ? ia[n] : 0;
+}
+template <class VT>
+void upload_sell(mlp_ctx ctx, DevMem& mem, uint32_t n, const uint32_t* ia, const uint32_t* ja, const double* a,
+                 DevSell<VT>& out, const uint32_t* rows = nullptr) {
+    Stager up(ctx, mem);
+    upload_sell<VT>(up, n, ia, ja, a, out, rows);
+    up.flush();
 }
 
 template <class VT> struct DevHier : DevPrecBase {
@@ -602,15 +641,14 @@ template <class VT> struct DevHier : DevPrecBase {
     }
 
     // the device arrays of the fused pair program on level L (see plan.h: PairProgram)
-    void upload_pair(uint32_t l, const mlp_level_desc& D, const PairProgram& PP, uint32_t pbs, DevLevel<VT>& L) {
+    void upload_pair(Stager& up, uint32_t l, const mlp_level_desc& D, const PairProgram& PP, uint32_t pbs,
+                     DevLevel<VT>& L) {
         L.fused = true;
         L.lbs = pbs;
         L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
         L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
         PairView<VT, double>& V = L.pair;
         V.nblk = PP.nblk; V.nodes = PP.nodes; V.nodes2 = PP.nodes2; V.nflat2 = PP.nflat2; V.Md = D.Md;
-        Stager up(ctx, *data);
-        std::vector<double> r1, r2;
         up.add(D.map, D.N, L.map);
         up.add(PP.blk, V.blk);
         up.add(PP.qrow, V.qrow);
@@ -624,7 +662,7 @@ template <class VT> struct DevHier : DevPrecBase {
         up.add(PP.Uval2, V.Uval2);
         up.add(PP.frow2, V.frow2); up.add(PP.fxl, V.fxl); up.add(PP.fd2, V.fd2);
         if constexpr (std::is_same_v<VT, float>) {             // udiv: FP32 multiplies by 1 / u_ii
-            r1.resize(PP.Udiag.size());
+            std::vector<double>& r1 = up.own(std::vector<double>(PP.Udiag.size()));
             for (size_t i = 0; i < r1.size(); i++) r1[i] = 1.0 / PP.Udiag[i];
             up.add(r1, V.Udiag);
         } else {
@@ -632,14 +670,13 @@ template <class VT> struct DevHier : DevPrecBase {
         }
         up.add(PP.Udiag2, V.Udiag2);                            // FP64: the coarsest level divides
         up.add(D.dval, D.Md, L.dval);
-        up.flush();
-        V.dval = L.dval;
+        up.after([&V, &L] { V.dval = L.dval; });
         // clones size the per-instance w of the block nodes from prog.nodes
         L.prog = ProgView<VT>();
         L.prog.nodes = PP.nodes;
         L.wt = scratch.alloc<VT>(std::max<uint32_t>(PP.nodes, 1));
         // level l's own matrix (l >= 1), read by its parent's fused Chebyshev residual in its consumption order
-        if (l > 0) upload_sell<VT>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A, PP.qrow.data());
+        if (l > 0) upload_sell<VT>(up, D.N, D.A_ia, D.A_ja, D.A_a, L.A, PP.qrow.data());
         if (L.n) {
             L.F = scratch.alloc<VT>(L.n);
             L.u1 = scratch.alloc<VT>(L.n);
@@ -658,13 +695,11 @@ template <class VT> struct DevHier : DevPrecBase {
 
     // the device arrays of chain level l (plan.h: ChainLevelProgram) and its vectors
     template <class T>
-    void upload_chain_level(uint32_t l, const mlp_level_desc& D, const ChainLevelProgram& C, ChainLv<T>& V,
-                            DevLevel<VT>& L) {
+    void upload_chain_level(Stager& up, uint32_t l, const mlp_level_desc& D, const ChainLevelProgram& C,
+                            ChainLv<T>& V, DevLevel<VT>& L) {
         const uint32_t c = nlevels - 1;
         L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
         L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
-        Stager up(ctx, *data);
-        std::vector<double> urcp;
         up.add(D.map, D.N, L.map);
         up.add(C.hdr, V.hdr); up.add(C.qrow, V.qrow); up.add(C.src, V.src); up.add(C.xl, V.xl);
         up.add(C.Lmeta, V.Lmeta); up.add(C.Lv2, V.Lv2); up.add(C.Lptr, V.Lptr); up.add(C.Lloc, V.Lloc);
@@ -672,23 +707,22 @@ template <class VT> struct DevHier : DevPrecBase {
         up.add(C.Umeta, V.Umeta); up.add(C.Uv1, V.Uv1); up.add(C.uxidx, V.uxidx); up.add(C.Uptr, V.Uptr);
         up.add(C.Ucol, V.Ucol); up.add(C.Uval, V.Uval); up.add(C.dval, V.dval);
         if constexpr (std::is_same_v<T, float>) {               // udiv: FP32 multiplies by 1 / u_ii
-            urcp.resize(C.Udiag.size());
+            std::vector<double>& urcp = up.own(std::vector<double>(C.Udiag.size()));
             for (size_t i = 0; i < urcp.size(); i++) urcp[i] = 1.0 / C.Udiag[i];
             up.add(urcp, V.Udiag);
         } else {
             up.add(C.Udiag, V.Udiag);
         }
-        up.flush();
         V.nslots = C.nslots;
         V.npt = C.npt;
         chain_npt[l - chain_a] = C.npt;
         // this level's matrix in its chain order: read by the Chebyshev residual of its second visits (the
         // coarsest level's in FP64, by step 2)
         if (l == c) {
-            upload_sell<double>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A64, C.qrow.data());
+            upload_sell<double>(up, D.N, D.A_ia, D.A_ja, D.A_a, L.A64, C.qrow.data());
             L.A.nnz = L.A64.nnz;
         } else if (l > 0) {
-            upload_sell<VT>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A, C.qrow.data());
+            upload_sell<VT>(up, D.N, D.A_ia, D.A_ja, D.A_a, L.A, C.qrow.data());
         }
         if (l < c) {
             L.prog = ProgView<VT>();
@@ -851,6 +885,8 @@ template <class VT> struct DevHier : DevPrecBase {
                 else if (int(l + 1) == chain_a) link_levels(progs[l], CP.lv[0].qrow.data(), CP.lv[0].qrow.size());
                 else link_levels(progs[l], progs[l + 1]);
             }
+        // every level's arrays go up in one staged copy (device pointers are set when it is flushed)
+        Stager up(ctx, *data);
         for (uint32_t l = 0; l < nlevels; l++) {
             const mlp_level_desc& D = d[l];
             const LevelProgram& P = progs[l];
@@ -859,25 +895,23 @@ template <class VT> struct DevHier : DevPrecBase {
             L.n = L.N - L.M - L.Md;
             if (L.eps != 0.0) dynamic = true;
             if (chain_a >= 0 && l >= uint32_t(chain_a)) {
-                if (l + 1 == nlevels) upload_chain_level(l, D, CP.lv[l - chain_a], cv.C, L);
-                else upload_chain_level(l, D, CP.lv[l - chain_a], cv.L[l - chain_a], L);
+                if (l + 1 == nlevels) upload_chain_level(up, l, D, CP.lv[l - chain_a], cv.C, L);
+                else upload_chain_level(up, l, D, CP.lv[l - chain_a], cv.L[l - chain_a], L);
                 continue;
             }
             if (int(l) == pair_l) {
-                upload_pair(l, D, PP, pbs, L);
+                upload_pair(up, l, D, PP, pbs, L);
                 continue;
             }
             if (pair_l >= 0 && int(l) == pair_l + 1) {
                 // the coarsest level runs inside the pair kernels; only its matrix is read here (FP64),
                 // by step 2's residual, rows in the pair's level-(l+1) consumption order
-                upload_sell<double>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A64, PP.qrow2.data());
+                upload_sell<double>(up, D.N, D.A_ia, D.A_ja, D.A_a, L.A64, PP.qrow2.data());
                 L.A.nnz = L.A64.nnz;                            // for the byte model
                 L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
                 L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
                 continue;
             }
-            Stager up(ctx, *data);
-            std::vector<double> urcp;
             up.add(D.map, D.N, L.map);
             L.nnzL = D.L_ia ? D.L_ia[D.N - D.Md] : 0;
             L.nnzU = (D.U_ia && D.M) ? D.U_ia[D.M] : 0;
@@ -905,7 +939,7 @@ template <class VT> struct DevHier : DevPrecBase {
                 up.add(P.Ucol, G.Ucol);
                 up.add(P.Uval, G.Uval);
                 if constexpr (std::is_same_v<VT, float>) {         // udiv: FP32 multiplies by 1 / u_ii
-                    urcp.resize(P.Udiag.size());
+                    std::vector<double>& urcp = up.own(std::vector<double>(P.Udiag.size()));
                     for (size_t i = 0; i < urcp.size(); i++) urcp[i] = 1.0 / P.Udiag[i];
                     up.add(urcp, G.Udiag);
                 } else {
@@ -925,13 +959,12 @@ template <class VT> struct DevHier : DevPrecBase {
                 L.lds_solve = lds_plan<VT>(P.max_nodes, P.max_Lent, P.max_Uent, P.max_Unodes, P.max_levels, true, true).total;
             }
             up.add(D.dval, D.Md, L.dval);
-            up.flush();
             if (l > 0) {
                 // uh_cheb reads A_l only through the fused Chebyshev residual, row by row in consumption order
                 if (kind == MLP_PREC_UH_CHEB)
-                    upload_sell<VT>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A, P.qrow.data());
+                    upload_sell<VT>(up, D.N, D.A_ia, D.A_ja, D.A_a, L.A, P.qrow.data());
                 else
-                    upload_sell<VT>(ctx, *data, D.N, D.A_ia, D.A_ja, D.A_a, L.A);
+                    upload_sell<VT>(up, D.N, D.A_ia, D.A_ja, D.A_a, L.A);
             }
             if (L.n && bf16 && l == 0) {
                 L.bF = scratch.alloc<hip_bfloat16>(L.n);
@@ -946,6 +979,7 @@ template <class VT> struct DevHier : DevPrecBase {
             }
             if (kind == MLP_PREC_MULTI_SPLIT && l > 0) L.r = scratch.alloc<VT>(L.N);
         }
+        up.flush();
         if (kind == MLP_PREC_UH_CHEB) {
             for (uint32_t l = 0; l + 1 < nlevels; l++) {
                 DevLevel<VT>& L = lv[l];
