@@ -6,8 +6,11 @@
 * FP32 hierarchy inside FP64 flexible PCG at 60x220x85 to 1e-8 and 1e-10: the FP64 true residual meets the
   tolerance and the solution is within a bound of the FP64 oracle solution.
 * 64 systems with depths drawn from {5, 10, 20, 40, 85}: the concurrent solves are bitwise the single solves,
-  and three of them (FP64 hierarchy) agree with the oracle; 32 right-hand sides of the 60x220x85 system in one
-  batched solve, each bitwise its single solve, one against the oracle.
+  and three of them (FP64 hierarchy) agree with the oracle; the same 64 stacked into one block-diagonal matrix per
+  hierarchy depth, each group's schedule autotuned, solved one launch per step per group
+  (mlp_pcg_ensemble_batch): FP64 bitwise the single solves, the FP32 hierarchy in flexible PCG within the FP64
+  true-residual bar; 32 right-hand sides of the 60x220x85 system in one batched solve, each bitwise its single
+  solve, one against the oracle.
 * Adjoint gradient at 1,122,000 cells: 200 observations, 20 seeded directions, central differences
   (h = 1e-6) within 1e-5 relative, and the device gradient against the FP64 gradient from the oracle's solves.
 """
@@ -149,6 +152,71 @@ def test_ensemble_64_systems():
         tol = 1e-10 if st["niter"] == so["niter"] else 1e-6
         assert np.linalg.norm(job[4].get() - xo) <= tol * np.linalg.norm(xo)
     _close(jobs)
+
+
+def test_ensemble_64_systems_stacked_and_tuned():
+    spec = S.ensemble_spec(64)
+    hh = (S.HX, S.HY, S.HZ)
+    fields = [S.spe10_field(60, 220, nz, seed).ravel() for nz, seed in spec]
+    rhs = [S.rhs(k.size, seed + 7919) for k, (_, seed) in zip(fields, spec)]
+    kw = dict(eps=1e-8, maxit=5000, x0_mode=N.X0_ZERO, check_every=8, use_graph=True)
+    # every system alone: its own matrix and hierarchy built on the device, FP64 PCG
+    ctx0 = N.Context(0)
+    single, depth = [], []
+    for (nz, _), kx, b in zip(spec, fields, rhs):
+        A = N.DevMatrix.spe_on_device(ctx0, ctx0.to_device(kx), (nz, 220, 60), kind="spe", c=1.0, kz_ratio=0.1,
+                                      h=hh)
+        H = N.build_uh_cheb_on_device(ctx0, A)
+        B = N.DevPrec(ctx0, A, H)
+        x = ctx0.array(kx.size)
+        st = N.pcg(ctx0, A, B, ctx0.to_device(b), x, **kw)
+        single.append((st["niter"], x.get()))
+        depth.append(H.nlevels)
+        B.close()
+        A.close()
+    groups = {}
+    for i, d in enumerate(depth):
+        groups.setdefault(d, []).append(i)
+    assert len(groups) >= 3
+    # one stacked matrix and tuned hierarchy per depth, each group on a context of its own
+    stacks = []
+    for d, idx in sorted(groups.items()):
+        c = N.Context(0)
+        A = N.DevMatrix.spe_ensemble_on_device(c, c.to_device(np.concatenate([fields[i] for i in idx])), 60, 220,
+                                               [spec[i][0] for i in idx], kind="spe", c=1.0, kz_ratio=0.1, h=hh)
+        H = N.build_uh_cheb_on_device(c, A)
+        assert H.nlevels == d
+        off = np.concatenate([[0], np.cumsum([fields[i].size for i in idx])]).astype(np.uint32)
+        B64 = N.DevPrec.tuned(c, A, H, precision=N.MLP_FP64, reps=3)
+        B32 = N.DevPrec.tuned(c, A, H, precision=N.MLP_FP32, reps=3)
+        stacks.append((c, A, B64, B32, idx, off, c.to_device(np.concatenate([rhs[i] for i in idx])),
+                       c.array(int(off[-1]))))
+    for mixed in (False, True):
+        jobs = [(c, A, B32 if mixed else B64, off, b, x) for c, A, B64, B32, idx, off, b, x in stacks]
+        rc, res = N.pcg_ensemble_batch(jobs, **(dict(kw, flexible=True, true_res_every=20) if mixed else kw))
+        assert rc == 0
+        for (c, A, B64, B32, idx, off, b, x), (status, stats) in zip(stacks, res):
+            assert status == 0
+            xs = x.get()
+            for i, st, o in zip(idx, stats, off[:-1]):
+                niter, x1 = single[i]
+                xi = xs[o:o + x1.size]
+                if mixed:
+                    assert st["true_rel_res"] <= 1e-8 and st["niter"] <= niter + 2
+                    assert np.linalg.norm(xi - x1) <= 1e-5 * np.linalg.norm(x1)
+                else:
+                    assert st["niter"] == niter
+                    assert np.array_equal(xi.view(np.uint64), x1.view(np.uint64))
+    ctxs = [st[0] for st in stacks]
+    for c, A, B64, B32, idx, off, b, x in stacks:
+        B64.close()
+        B32.close()
+        A.close()
+    del jobs, res, b, x
+    stacks.clear()                                  # the group vectors are freed before their contexts close
+    for c in ctxs:
+        c.close()
+    ctx0.close()
 
 
 def test_32_rhs_full_size(ctx):
