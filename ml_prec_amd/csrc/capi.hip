@@ -748,7 +748,7 @@ template <class VT> struct DevHier : DevPrecBase {
             const DevLevel<VT>& L = lv[l];
             ChainVec<VT>& V = cv.V[l - chain_a];
             V.F = L.F; V.u1 = L.u1; V.u0 = L.u0; V.x2 = L.x2; V.wt = L.wt;
-            V.u1s = L.s1_skip ? L.u1 : L.u0;
+            V.a1 = L.a1; V.s1_skip = L.s1_skip;
             V.ma2 = L.ma2; V.s2_skip = L.s2_skip; V.opb2 = L.opb2; V.opb2_one = L.opb2_one;
             V.n = L.n;
             V.N = L.N;
@@ -1249,34 +1249,47 @@ template <class VT> struct DevHier : DevPrecBase {
     }
 
     // --- the chain (plan.h: ChainProgram) ---
+    // The chain kernels come in two register budgets: NL = 1 for launches that run only the last chain level
+    // (with the coarsest), NL = kChainMax for any other.
+    template <class FT, int IN, class XT, int NL>
+    static auto chain_down_kernel(bool batched, bool narrow) {
+        constexpr int W = int(2 * kChainBS);
+        return narrow ? (batched ? k_chain_down<VT, double, FT, IN, XT, true, kChainBS, NL>
+                                 : k_chain_down<VT, double, FT, IN, XT, false, kChainBS, NL>)
+                      : (batched ? k_chain_down<VT, double, FT, IN, XT, true, W, NL>
+                                 : k_chain_down<VT, double, FT, IN, XT, false, W, NL>);
+    }
     template <class FT, int IN, class XT>
     void chain_down(uint32_t j, const InSrc<VT, FT, IN>& f, XT* out, const uint32_t* g) const {
         const uint32_t i0 = j - uint32_t(chain_a);
         const uint32_t grid = chain_blk[i0] + cdiv(chain_npt[i0], chain_bs);
-        constexpr int W = int(2 * kChainBS);
-        auto k = chain_bs == kChainBS
-                     ? (R > 1 ? k_chain_down<VT, double, FT, IN, XT, true, kChainBS>
-                              : k_chain_down<VT, double, FT, IN, XT, false, kChainBS>)
-                     : (R > 1 ? k_chain_down<VT, double, FT, IN, XT, true, W> : k_chain_down<VT, double, FT, IN, XT, false, W>);
+        const bool narrow = chain_bs == kChainBS;
+        auto k = i0 + 1 == cv.nv ? chain_down_kernel<FT, IN, XT, 1>(R > 1, narrow)
+                                 : chain_down_kernel<FT, IN, XT, kChainMax>(R > 1, narrow);
         launch(k, grid, chain_bs, 0, cv, i0, f, out, lv[j].N, chain_blk[i0], g);
+    }
+    template <class XT, bool SCALE, int NL>
+    static auto chain_up_kernel(bool batched, bool narrow) {
+        constexpr int W = int(2 * kChainBS);
+        return narrow ? (batched ? k_chain_up<VT, double, XT, SCALE, true, kChainBS, NL>
+                                 : k_chain_up<VT, double, XT, SCALE, false, kChainBS, NL>)
+                      : (batched ? k_chain_up<VT, double, XT, SCALE, true, W, NL>
+                                 : k_chain_up<VT, double, XT, SCALE, false, W, NL>);
+    }
+    template <class XT, bool SCALE>
+    auto chain_up_pick(uint32_t ie) const {
+        const bool narrow = chain_bs == kChainBS;
+        return ie + 1 == cv.nv ? chain_up_kernel<XT, SCALE, 1>(R > 1, narrow)
+                               : chain_up_kernel<XT, SCALE, kChainMax>(R > 1, narrow);
     }
     template <class XT>
     void chain_up(uint32_t ke, XT* out, const uint32_t* g) const {
         const uint32_t ie = ke - uint32_t(chain_a);
         const uint32_t grid = chain_blk[ie] + cdiv(chain_npt[ie + 1], chain_bs);
-        constexpr int W = int(2 * kChainBS);
-        const bool narrow = chain_bs == kChainBS;
-        if (lv[nlevels - 2].s1_skip) {
-            auto k = narrow ? (R > 1 ? k_chain_up<VT, double, XT, false, true, kChainBS>
-                                     : k_chain_up<VT, double, XT, false, false, kChainBS>)
-                            : (R > 1 ? k_chain_up<VT, double, XT, false, true, W> : k_chain_up<VT, double, XT, false, false, W>);
-            launch(k, grid, chain_bs, 0, cv, ie, out, lv[ke].N, chain_blk[ie], g);
-        } else {
-            auto k = narrow ? (R > 1 ? k_chain_up<VT, double, XT, true, true, kChainBS>
-                                     : k_chain_up<VT, double, XT, true, false, kChainBS>)
-                            : (R > 1 ? k_chain_up<VT, double, XT, true, true, W> : k_chain_up<VT, double, XT, true, false, W>);
-            launch(k, grid, chain_bs, 0, cv, ie, out, lv[ke].N, chain_blk[ie], g);
-        }
+        if (lv[nlevels - 2].s1_skip)
+            launch(chain_up_pick<XT, false>(ie), grid, chain_bs, 0, cv, ie, out, lv[ke].N, chain_blk[ie], g);
+        else
+            launch(chain_up_pick<XT, true>(ie), grid, chain_bs, 0, cv, ie, out, lv[ke].N, chain_blk[ie], g);
     }
     // The operations of Prec::solve on the chain levels, in order: D (L sweep of a level: its visit starts),
     // the coarsest level's first and second visits, U (U sweep: a visit ends).
@@ -1310,8 +1323,9 @@ template <class VT> struct DevHier : DevPrecBase {
                     chain_down(o.k, f, x, g);
                 } else {
                     const DevLevel<VT>& P = lv[o.k - 1];
+                    // a1 u1 is formed again where the ascent's combine reads it (chain_u1s): not stored
                     if (P.s1_skip) chain_down(o.k, residual_src<IN_RES>(o.k - 1, P.u1, 1.0, nullptr), P.x2, g);
-                    else chain_down(o.k, residual_src<IN_RES_SCALE>(o.k - 1, P.u1, P.a1, P.u0), P.x2, g);
+                    else chain_down(o.k, residual_src<IN_RES_SCALE>(o.k - 1, P.u1, P.a1, nullptr), P.x2, g);
                 }
                 while (i < ops.size() && ops[i].kind == 0) i++;
                 if (i >= ops.size() || ops[i].kind != 1) throw Error("chain: unexpected operation order", 1);

@@ -463,7 +463,8 @@ __device__ __forceinline__ VT combine(const Combine<VT, CV>& C, uint32_t r) {
 //  IN_RES, IN_RES_SCALE: the Chebyshev residual of the parent level fused into
 //            the read, f = (A (s u))_j - F[q], the row taken from a copy of A
 //            stored in consumption order; IN_RES_SCALE also stores
-//            us[j] = s u[j] (cheb_solve.cc:61-78, 96-97).
+//            us[j] = s u[j] when us is given (cheb_solve.cc:61-78, 96-97; the
+//            chain recomputes s u[j] where its combine reads it instead).
 // The kernels read every position exactly once, so each residual row is
 // computed once, with the same operations as a separate SpMV.
 enum { IN_PLAIN = 0, IN_PERM = 1, IN_RES = 2, IN_RES_SCALE = 3 };
@@ -483,7 +484,7 @@ __device__ __forceinline__ VT fetch(const InSrc<VT, FT, IN>& S, uint32_t q, uint
     if (IN == IN_PLAIN) return VT(S.f[j]);
     if (IN == IN_PERM) return VT(S.f[q]);
     const VT acc = sell_row<VT, FT, IN == IN_RES_SCALE>(S.A, S.u, q, S.s);
-    if (IN == IN_RES_SCALE) S.us[j] = FT(double(S.u[j]) * S.s);
+    if (IN == IN_RES_SCALE && S.us) S.us[j] = FT(double(S.u[j]) * S.s);
     return acc - VT(S.F[q]);
 }
 
@@ -1342,11 +1343,20 @@ __device__ __forceinline__ OwnU<VT> chain_own_U(uint32_t nu, uint32_t nl, uint32
 // the combine of its second Chebyshev step (cheb_solve.cc:79-81)
 template <class VT> struct ChainVec {
     VT *F = nullptr, *u1 = nullptr, *u0 = nullptr, *x2 = nullptr, *wt = nullptr;
-    const VT* u1s = nullptr;               // u1, or a1 u1 (in u0) when the first step scales
+    double a1 = 1.0;                       // the first step's scaling of u1 (cheb_solve.cc:67-68)
+    int s1_skip = 1;                       // a1 == 1: u1 unscaled
     double ma2 = 0.0, opb2 = 0.0;
     int s2_skip = 0, opb2_one = 0;
     uint32_t n = 0, N = 0;                 // batched strides: the next level's size, this level's
 };
+// u1 as step 2's combine reads it: a1 u1 when the first step scales, formed here with the same single product
+// (rounded to VT) that IN_RES_SCALE stores, so that no pass writes it out
+template <class VT>
+__device__ __forceinline__ VT chain_u1s(const ChainVec<VT>& V, size_t k) {
+    const VT u = V.u1[k];
+    return V.s1_skip ? u : VT(double(u) * V.a1);
+}
+
 template <class VT, class CT> struct ChainView {
     ChainLv<VT> L[kChainMax];
     ChainVec<VT> V[kChainMax];
@@ -1357,12 +1367,34 @@ template <class VT, class CT> struct ChainView {
     uint32_t st = 0;                       // solve-state stride, in words (batched guards)
 };
 
+// The j-th level one chain launch runs (j < NL): with NL = kChainMax the loops
+// run over every chain level with compile-time indices and skip those outside
+// i0..nv-1; with NL = 1 the launch runs level i0 = nv-1 alone (run-time index).
+// chain_top: level i0.
+template <int NL>
+__device__ __forceinline__ uint32_t chain_level(int j, uint32_t i0) {
+    static_assert(NL == 1 || NL == kChainMax, "a chain launch runs the last level alone or any levels");
+    return NL == kChainMax ? uint32_t(j) : i0 + uint32_t(j);
+}
+template <int NL>
+__device__ __forceinline__ bool chain_skip(uint32_t i, uint32_t i0, uint32_t nv) {
+    return (NL == kChainMax && i < i0) || i >= nv;
+}
+template <int NL>
+__device__ __forceinline__ bool chain_top(uint32_t i, uint32_t i0, int j) {
+    return NL == kChainMax ? i == i0 : j == 0;
+}
+__device__ __forceinline__ uint32_t chain_next(uint32_t i) { return i + 1 < uint32_t(kChainMax) ? i + 1 : i; }
+
 // Descent from chain level i0: level i0 reads its input from f, levels
 // i0+1..nv-1 (first visits) the F of the level above from LDS, then step 1 on
 // the coarsest level.  Writes F of every level below i0, wt of every level,
 // the isolated rows' x (level i0's into out0, the others' into the parent's
-// u1) and u1 of the coarsest level.
-template <class VT, class CT, class FT, int IN, class XT, bool BAT, int BS>
+// u1) and u1 of the coarsest level.  NL >= nv - i0 bounds the chain levels
+// one launch runs: its per-level registers are sized for NL levels (a descent
+// from the last chain level, NL = 1, keeps one level's node data live beside
+// the residual row instead of kChainMax levels').
+template <class VT, class CT, class FT, int IN, class XT, bool BAT, int BS, int NL>
 __global__ void __launch_bounds__(BS) k_chain_down(ChainView<VT, CT> V, uint32_t i0, InSrc<VT, FT, IN> f,
                                                            XT* __restrict__ out0, uint32_t Ntop, uint32_t nblk,
                                                            const uint32_t* guard) {
@@ -1383,27 +1415,27 @@ __global__ void __launch_bounds__(BS) k_chain_down(ChainView<VT, CT> V, uint32_t
         __shared__ CT cw[2 * BS], cx[2 * BS];
         const uint32_t b = blockIdx.x;
         // every level's per-slot loads first, so that they overlap
-        uint32_t base[kChainMax], S[kChainMax], nn[kChainMax], nkeep[kChainMax], nlL[kChainMax], qr[kChainMax],
-            sr[kChainMax];
-        VT dv[kChainMax];
-        ChainOwnL<VT> ol[kChainMax];
+        uint32_t base[NL], S[NL], nn[NL], nkeep[NL], nlL[NL], qr[NL], sr[NL];
+        VT dv[NL];
+        ChainOwnL<VT> ol[NL];
         VT acc = VT(0);
 #pragma unroll
-        for (int i = 0; i < kChainMax; i++) {
-            base[i] = S[i] = nn[i] = nkeep[i] = nlL[i] = qr[i] = sr[i] = 0;
-            dv[i] = VT(0);
-            ol[i] = ChainOwnL<VT>{OwnL<VT>{0u, 0u, 0u, t, t, VT(0), VT(0)}, t, VT(0)};
-            if (uint32_t(i) < i0 || uint32_t(i) >= nv) continue;
+        for (int j = 0; j < NL; j++) {
+            const uint32_t i = chain_level<NL>(j, i0);
+            base[j] = S[j] = nn[j] = nkeep[j] = nlL[j] = qr[j] = sr[j] = 0;
+            dv[j] = VT(0);
+            ol[j] = ChainOwnL<VT>{OwnL<VT>{0u, 0u, 0u, t, t, VT(0), VT(0)}, t, VT(0)};
+            if (chain_skip<NL>(i, i0, nv)) continue;
             const ChainLv<VT>& L = V.L[i];
             const uint32_t* H = L.hdr + kChainHdrW * b;
-            base[i] = uload(H); S[i] = uload(H + 1); nn[i] = uload(H + 2); nkeep[i] = nn[i] + uload(H + 3);
-            nlL[i] = uload(H + 4);
-            ol[i] = chain_own_L(nn[i], nlL[i], base[i], L);
-            if (t < S[i]) {
-                qr[i] = L.qrow[base[i] + t];
-                if (uint32_t(i) > i0) sr[i] = L.src[base[i] + t];
-                if (t >= nkeep[i]) dv[i] = L.dval[base[i] + t];
-                if (uint32_t(i) == i0) acc = fetch(f, base[i] + t, qr[i]);
+            base[j] = uload(H); S[j] = uload(H + 1); nn[j] = uload(H + 2); nkeep[j] = nn[j] + uload(H + 3);
+            nlL[j] = uload(H + 4);
+            ol[j] = chain_own_L(nn[j], nlL[j], base[j], L);
+            if (t < S[j]) {
+                qr[j] = L.qrow[base[j] + t];
+                if (!chain_top<NL>(i, i0, j)) sr[j] = L.src[base[j] + t];
+                if (t >= nkeep[j]) dv[j] = L.dval[base[j] + t];
+                if (chain_top<NL>(i, i0, j)) acc = fetch(f, base[j] + t, qr[j]);
             }
         }
         const ChainLv<CT>& C = V.C;
@@ -1423,8 +1455,9 @@ __global__ void __launch_bounds__(BS) k_chain_down(ChainView<VT, CT> V, uint32_t
         if (gflag) return;
         int cur = 0;
 #pragma unroll
-        for (int i = 0; i < kChainMax; i++) {
-            if (uint32_t(i) < i0 || uint32_t(i) >= nv) continue;
+        for (int j = 0; j < NL; j++) {
+            const uint32_t i = chain_level<NL>(j, i0);
+            if (chain_skip<NL>(i, i0, nv)) continue;
             const ChainLv<VT>& L = V.L[i];
             const ChainVec<VT>& P = V.V[i > 0 ? i - 1 : 0];     // the parent's vectors (levels below i0)
             VT* pFi = P.F + y * P.n;
@@ -1432,27 +1465,27 @@ __global__ void __launch_bounds__(BS) k_chain_down(ChainView<VT, CT> V, uint32_t
             VT* s = sv[cur];
             const VT* p = sv[cur ^ 1];
             VT v = VT(0);
-            if (t < S[i]) {
-                v = uint32_t(i) == i0 ? acc : p[sr[i]];
-                if (uint32_t(i) > i0) pFi[base[i] + t] = v;
-                if (t >= nkeep[i]) {
-                    if (uint32_t(i) == i0) out0[qr[i]] = XT(dv[i] * v);
-                    else pu1i[qr[i]] = VT(dv[i] * v);
+            if (t < S[j]) {
+                v = chain_top<NL>(i, i0, j) ? acc : p[sr[j]];
+                if (!chain_top<NL>(i, i0, j)) pFi[base[j] + t] = v;
+                if (t >= nkeep[j]) {
+                    if (chain_top<NL>(i, i0, j)) out0[qr[j]] = XT(dv[j] * v);
+                    else pu1i[qr[j]] = VT(dv[j] * v);
                 }
-                if (t >= nn[i]) s[t] = v;
+                if (t >= nn[j]) s[t] = v;
             }
-            for (uint32_t q = t + BS; q < S[i]; q += BS) {   // slots beyond the wavefront: no nodes
-                const uint32_t j = L.qrow[base[i] + q];
-                const VT w = uint32_t(i) == i0 ? fetch(f, base[i] + q, j) : p[L.src[base[i] + q]];
-                if (uint32_t(i) > i0) pFi[base[i] + q] = w;
-                if (q >= nkeep[i]) {
-                    if (uint32_t(i) == i0) out0[j] = XT(L.dval[base[i] + q] * w);
-                    else pu1i[j] = VT(L.dval[base[i] + q] * w);
+            for (uint32_t q = t + BS; q < S[j]; q += BS) {   // slots beyond the wavefront: no nodes
+                const uint32_t r = L.qrow[base[j] + q];
+                const VT w = chain_top<NL>(i, i0, j) ? fetch(f, base[j] + q, r) : p[L.src[base[j] + q]];
+                if (!chain_top<NL>(i, i0, j)) pFi[base[j] + q] = w;
+                if (q >= nkeep[j]) {
+                    if (chain_top<NL>(i, i0, j)) out0[r] = XT(L.dval[base[j] + q] * w);
+                    else pu1i[r] = VT(L.dval[base[j] + q] * w);
                 }
                 s[q] = w;
             }
-            const VT w = chain_sweep_L<BS>(ol[i], nlL[i], L.Lloc, L.Lval, v, s);
-            if (t < nn[i]) V.V[i].wt[y * V.V[i].N + base[i] + t] = w;
+            const VT w = chain_sweep_L<BS>(ol[j], nlL[j], L.Lloc, L.Lval, v, s);
+            if (t < nn[j]) V.V[i].wt[y * V.V[i].N + base[j] + t] = w;
             level_sync<BS>();
             cur ^= 1;
         }
@@ -1481,21 +1514,22 @@ __global__ void __launch_bounds__(BS) k_chain_down(ChainView<VT, CT> V, uint32_t
         bool live = true;
         VT v = VT(0);
 #pragma unroll
-        for (int i = 0; i < kChainMax; i++) {
-            if (uint32_t(i) < i0 || uint32_t(i) >= nv || !live) continue;
+        for (int j = 0; j < NL; j++) {
+            const uint32_t i = chain_level<NL>(j, i0);
+            if (chain_skip<NL>(i, i0, nv) || !live) continue;
             const ChainLv<VT>& L = V.L[i];
             if (tp >= L.npt) { live = false; continue; }
             const uint32_t q = L.nslots + tp;
             const ChainVec<VT>& P = V.V[i > 0 ? i - 1 : 0];
-            if (uint32_t(i) == i0) {
+            if (chain_top<NL>(i, i0, j)) {
                 v = fetch(f, q, L.qrow[q]);
                 if (gflag) return;
             } else {
                 P.F[y * P.n + q] = v;
             }
-            const uint32_t nxt = uint32_t(i) + 1 < nv ? V.L[i + 1 < kChainMax ? i + 1 : i].npt : V.C.npt;
+            const uint32_t nxt = NL != 1 && i + 1 < nv ? V.L[chain_next(i)].npt : V.C.npt;
             if (tp >= nxt) {                  // isolated here
-                if (uint32_t(i) == i0) out0[L.qrow[q]] = XT(L.dval[q] * v);
+                if (chain_top<NL>(i, i0, j)) out0[L.qrow[q]] = XT(L.dval[q] * v);
                 else P.u1[y * P.n + L.qrow[q]] = VT(L.dval[q] * v);
                 live = false;
             }
@@ -1513,7 +1547,8 @@ __global__ void __launch_bounds__(BS) k_chain_down(ChainView<VT, CT> V, uint32_t
 // the U sweep.  Every level above the coarsest but ie completes its second
 // visit here, so its isolated rows' values come from x2 of its parent, where
 // the descent of that visit wrote them; level ie's rows are written to out.
-template <class VT, class CT, class XT, bool SCALE, bool BAT, int BS>
+// NL >= nv - ie, as for the descent.
+template <class VT, class CT, class XT, bool SCALE, bool BAT, int BS, int NL>
 __global__ void __launch_bounds__(BS) k_chain_up(ChainView<VT, CT> V, uint32_t ie, XT* __restrict__ out,
                                                          uint32_t Ntop, uint32_t nblk, const uint32_t* guard) {
     const size_t y = BAT ? blockIdx.y : 0;
@@ -1541,46 +1576,46 @@ __global__ void __launch_bounds__(BS) k_chain_up(ChainView<VT, CT> V, uint32_t i
             f2 = sell_row<CT, VT, SCALE>(V.A2, pu1, baseC + t, V.a1c) - CT(pF[baseC + t]);
             if (t >= nnC) dvC = C.dval[baseC + t];
         }
-        uint32_t base[kChainMax], S[kChainMax], nkeep[kChainMax], nu[kChainMax], nlU[kChainMax], ux[kChainMax],
-            Sn[kChainMax], srn[kChainMax], xln[kChainMax];
-        VT z[kChainMax], xiso[kChainMax], usv[kChainMax];
-        OwnU<VT> ou[kChainMax];
+        uint32_t base[NL], S[NL], nkeep[NL], nu[NL], nlU[NL], ux[NL], Sn[NL], srn[NL], xln[NL];
+        VT z[NL], xiso[NL], usv[NL];
+        OwnU<VT> ou[NL];
         CT usC = CT(0);
 #pragma unroll
-        for (int i = 0; i < kChainMax; i++) {
-            base[i] = S[i] = nkeep[i] = nu[i] = nlU[i] = ux[i] = Sn[i] = srn[i] = xln[i] = 0;
-            z[i] = xiso[i] = usv[i] = VT(0);
-            ou[i] = OwnU<VT>{0u, 0u, 0u, 0u, 0u, 0u, VT(1), VT(0), VT(0)};
-            if (uint32_t(i) < ie || uint32_t(i) >= nv) continue;
+        for (int j = 0; j < NL; j++) {
+            const uint32_t i = chain_level<NL>(j, ie);
+            base[j] = S[j] = nkeep[j] = nu[j] = nlU[j] = ux[j] = Sn[j] = srn[j] = xln[j] = 0;
+            z[j] = xiso[j] = usv[j] = VT(0);
+            ou[j] = OwnU<VT>{0u, 0u, 0u, 0u, 0u, 0u, VT(1), VT(0), VT(0)};
+            if (chain_skip<NL>(i, ie, nv)) continue;
             const ChainLv<VT>& L = V.L[i];
             const ChainVec<VT>& Vi = V.V[i];
             const uint32_t* H = L.hdr + kChainHdrW * b;
             const uint32_t ub = uload(H + 5);
-            base[i] = uload(H); S[i] = uload(H + 1); nkeep[i] = uload(H + 2) + uload(H + 3); nu[i] = uload(H + 6);
-            nlU[i] = uload(H + 7);
-            ou[i] = chain_own_U(nu[i], nlU[i], ub, L);
-            if (t < nu[i]) {
-                z[i] = Vi.wt[y * Vi.N + base[i] + ou[i].loc];
-                if (uint32_t(i) == ie) ux[i] = L.uxidx[ub + t];
+            base[j] = uload(H); S[j] = uload(H + 1); nkeep[j] = uload(H + 2) + uload(H + 3); nu[j] = uload(H + 6);
+            nlU[j] = uload(H + 7);
+            ou[j] = chain_own_U(nu[j], nlU[j], ub, L);
+            if (t < nu[j]) {
+                z[j] = Vi.wt[y * Vi.N + base[j] + ou[j].loc];
+                if (chain_top<NL>(i, ie, j)) ux[j] = L.uxidx[ub + t];
             }
-            if (uint32_t(i) > ie && t < S[i] && t >= nkeep[i]) {
+            if (!chain_top<NL>(i, ie, j) && t < S[j] && t >= nkeep[j]) {
                 const ChainVec<VT>& P = V.V[i > 0 ? i - 1 : 0];
-                xiso[i] = P.x2[y * P.n + L.qrow[base[i] + t]];
+                xiso[j] = P.x2[y * P.n + L.qrow[base[j] + t]];
             }
             // the next level's rows (the kept rows of this level)
-            const bool last = uint32_t(i) + 1 == nv;
-            const ChainLv<VT>& Ln = V.L[i + 1 < kChainMax ? i + 1 : i];
+            const bool last = NL == 1 || i + 1 == nv;   // NL = 1: level nv-1 only
+            const ChainLv<VT>& Ln = V.L[chain_next(i)];
             const uint32_t* qn = last ? C.qrow : Ln.qrow;
             const uint32_t* sn = last ? C.src : Ln.src;
             const uint32_t* xn = last ? C.xl : Ln.xl;
             const uint32_t* Hn = (last ? C.hdr : Ln.hdr) + kChainHdrW * b;
-            Sn[i] = uload(Hn + 1);
-            if (t < Sn[i]) {
+            Sn[j] = uload(Hn + 1);
+            if (t < Sn[j]) {
                 const uint32_t q = uload(Hn) + t, r = qn[q];
-                srn[i] = sn[q];
-                if (uint32_t(i) == ie) xln[i] = xn[q];
+                srn[j] = sn[q];
+                if (chain_top<NL>(i, ie, j)) xln[j] = xn[q];
                 if (last) usC = SCALE ? CT(double(pu1[r]) * V.a1c) : CT(pu1[r]);
-                else usv[i] = Vi.u1s[y * Vi.n + r];
+                else usv[j] = chain_u1s(Vi, y * Vi.n + r);
             }
         }
         if (gflag) return;
@@ -1597,49 +1632,50 @@ __global__ void __launch_bounds__(BS) k_chain_up(ChainView<VT, CT> V, uint32_t i
         level_sync<BS>();
         int cur = 0;
 #pragma unroll
-        for (int i = kChainMax - 1; i >= 0; i--) {
-            if (uint32_t(i) < ie || uint32_t(i) >= nv) continue;
+        for (int j = NL - 1; j >= 0; j--) {
+            const uint32_t i = chain_level<NL>(j, ie);
+            if (chain_skip<NL>(i, ie, nv)) continue;
             const ChainLv<VT>& L = V.L[i];
             const ChainVec<VT>& Vi = V.V[i];
-            const bool last = uint32_t(i) + 1 == nv;
+            const bool last = NL == 1 || i + 1 == nv;   // NL = 1: level nv-1 only
             VT* s = sx[cur];
             const VT* xp = sx[cur ^ 1];
             // the combine of this level's kept rows (the next level's rows)
-            if (t < Sn[i]) {
+            if (t < Sn[j]) {
                 const VT c = last ? VT(pair_cheb2(cx[t], usC, Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one))
-                                  : VT(pair_cheb2(xp[t], usv[i], Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
-                s[srn[i]] = c;
-                if (uint32_t(i) == ie) out[xln[i]] = XT(c);
+                                  : VT(pair_cheb2(xp[t], usv[j], Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
+                s[srn[j]] = c;
+                if (chain_top<NL>(i, ie, j)) out[xln[j]] = XT(c);
             }
-            if (Sn[i] > BS) {
-                const ChainLv<VT>& Ln = V.L[i + 1 < kChainMax ? i + 1 : i];
+            if (Sn[j] > BS) {
+                const ChainLv<VT>& Ln = V.L[chain_next(i)];
                 const uint32_t* qn = last ? C.qrow : Ln.qrow;
                 const uint32_t* sn = last ? C.src : Ln.src;
                 const uint32_t* xn = last ? C.xl : Ln.xl;
                 const uint32_t b0 = uload((last ? C.hdr : Ln.hdr) + kChainHdrW * b);
-                for (uint32_t q = t + BS; q < Sn[i]; q += BS) {
+                for (uint32_t q = t + BS; q < Sn[j]; q += BS) {
                     const uint32_t r = qn[b0 + q];
                     VT c;
                     if (last) {
                         const CT us = SCALE ? CT(double(pu1[r]) * V.a1c) : CT(pu1[r]);
                         c = VT(pair_cheb2(cx[q], us, Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
                     } else {
-                        c = VT(pair_cheb2(xp[q], Vi.u1s[y * Vi.n + r], Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
+                        c = VT(pair_cheb2(xp[q], chain_u1s(Vi, y * Vi.n + r), Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
                     }
                     s[sn[b0 + q]] = c;
-                    if (uint32_t(i) == ie) out[xn[b0 + q]] = XT(c);
+                    if (chain_top<NL>(i, ie, j)) out[xn[b0 + q]] = XT(c);
                 }
             }
             // the isolated rows of a level completing its second visit
-            if (uint32_t(i) > ie) {
-                if (t < S[i] && t >= nkeep[i]) s[t] = xiso[i];
+            if (!chain_top<NL>(i, ie, j)) {
+                if (t < S[j] && t >= nkeep[j]) s[t] = xiso[j];
                 const ChainVec<VT>& P = V.V[i > 0 ? i - 1 : 0];
-                for (uint32_t q = t + BS; q < S[i]; q += BS)
-                    if (q >= nkeep[i]) s[q] = P.x2[y * P.n + L.qrow[base[i] + q]];
+                for (uint32_t q = t + BS; q < S[j]; q += BS)
+                    if (q >= nkeep[j]) s[q] = P.x2[y * P.n + L.qrow[base[j] + q]];
             }
             level_sync<BS>();
-            const VT x = sweep_U<BS, VT, false>(ou[i], nlU[i], L.Ucol, L.Uval, z[i], (const VT*)nullptr, s);
-            if (uint32_t(i) == ie && t < nu[i]) out[ux[i]] = XT(x);
+            const VT x = sweep_U<BS, VT, false>(ou[j], nlU[j], L.Ucol, L.Uval, z[j], (const VT*)nullptr, s);
+            if (chain_top<NL>(i, ie, j) && t < nu[j]) out[ux[j]] = XT(x);
             level_sync<BS>();
             cur ^= 1;
         }
@@ -1650,11 +1686,12 @@ __global__ void __launch_bounds__(BS) k_chain_up(ChainView<VT, CT> V, uint32_t i
         VT v = VT(0);
         if (gflag) return;
 #pragma unroll
-        for (int i = kChainMax - 1; i >= 0; i--) {
-            if (uint32_t(i) < ie || uint32_t(i) >= nv) continue;
+        for (int j = NL - 1; j >= 0; j--) {
+            const uint32_t i = chain_level<NL>(j, ie);
+            if (chain_skip<NL>(i, ie, nv)) continue;
             const ChainVec<VT>& Vi = V.V[i];
-            const bool last = uint32_t(i) + 1 == nv;
-            const ChainLv<VT>& Ln = V.L[i + 1 < kChainMax ? i + 1 : i];
+            const bool last = NL == 1 || i + 1 == nv;   // NL = 1: level nv-1 only
+            const ChainLv<VT>& Ln = V.L[chain_next(i)];
             const uint32_t npn = last ? C.npt : Ln.npt;
             if (tp >= npn) continue;                  // the row does not reach level i+1
             const uint32_t qn = (last ? C.nslots : Ln.nslots) + tp;
@@ -1666,15 +1703,11 @@ __global__ void __launch_bounds__(BS) k_chain_up(ChainView<VT, CT> V, uint32_t i
                 v = VT(pair_cheb2(xc, us, Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
             } else {
                 if (!have) v = Vi.x2[y * Vi.n + r];   // isolated on level i+1
-                v = VT(pair_cheb2(v, Vi.u1s[y * Vi.n + r], Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
+                v = VT(pair_cheb2(v, chain_u1s(Vi, y * Vi.n + r), Vi.ma2, Vi.s2_skip, Vi.opb2, Vi.opb2_one));
             }
             have = true;
         }
-        if (have) {
-#pragma unroll
-            for (int i = 0; i < kChainMax; i++)
-                if (uint32_t(i) == ie) out[V.L[i].qrow[V.L[i].nslots + tp]] = XT(v);
-        }
+        if (have) out[V.L[ie].qrow[V.L[ie].nslots + tp]] = XT(v);
     }
 }
 
