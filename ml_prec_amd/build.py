@@ -10,6 +10,7 @@ import os
 import subprocess
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -44,19 +45,22 @@ def _run(cmd):
 
 
 def build(verbose: bool = False) -> str:
-    """Compile every source (always) and link libmlp.so."""
+    """Compile every source (always; the sources at once, each in its own hipcc process) and link libmlp.so."""
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
     t0 = time.time()
+    jobs = []
     for src, is_hip in SOURCES:
         if not os.path.exists(src):
             continue
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
         cmd = [HIPCC] + COMMON + (DEVICE + ["-x", "hip"] if is_hip else ["-x", "c++"]) + ["-c", src, "-o", obj]
-        out = _run(cmd)
+        jobs.append((obj, cmd))
+    with ThreadPoolExecutor(max_workers=len(jobs)) as pool:
+        outs = list(pool.map(lambda j: _run(j[1]), jobs))     # a failed step raises here
+    for out in outs:
         if verbose and out.strip():
             print(out)
-        objs.append(obj)
+    objs = [obj for obj, _ in jobs]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
