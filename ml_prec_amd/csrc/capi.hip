@@ -728,10 +728,9 @@ template <class VT> struct DevHier : DevPrecBase {
             L.prog = ProgView<VT>();
             L.prog.nodes = L.N;                                  // wt: one value per chain position
             L.wt = scratch.alloc<VT>(std::max<uint32_t>(L.N, 1));
-            if (L.n) {
+            if (L.n) {                                           // (no u0: the chain forms a1 u1 where it reads it)
                 L.F = scratch.alloc<VT>(L.n);
                 L.u1 = scratch.alloc<VT>(L.n);
-                L.u0 = scratch.alloc<VT>(L.n);
                 L.x2 = scratch.alloc<VT>(L.n);
             }
         }
@@ -747,7 +746,7 @@ template <class VT> struct DevHier : DevPrecBase {
         for (uint32_t l = uint32_t(chain_a); l < c; l++) {
             const DevLevel<VT>& L = lv[l];
             ChainVec<VT>& V = cv.V[l - chain_a];
-            V.F = L.F; V.u1 = L.u1; V.u0 = L.u0; V.x2 = L.x2; V.wt = L.wt;
+            V.F = L.F; V.u1 = L.u1; V.x2 = L.x2; V.wt = L.wt;
             V.a1 = L.a1; V.s1_skip = L.s1_skip;
             V.ma2 = L.ma2; V.s2_skip = L.s2_skip; V.opb2 = L.opb2; V.opb2_one = L.opb2_one;
             V.n = L.n;

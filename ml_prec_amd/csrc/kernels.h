@@ -1342,7 +1342,7 @@ __device__ __forceinline__ OwnU<VT> chain_own_U(uint32_t nu, uint32_t nl, uint32
 // a chain level's vectors as the parent of the next level (cheb_prec.h:36), and
 // the combine of its second Chebyshev step (cheb_solve.cc:79-81)
 template <class VT> struct ChainVec {
-    VT *F = nullptr, *u1 = nullptr, *u0 = nullptr, *x2 = nullptr, *wt = nullptr;
+    VT *F = nullptr, *u1 = nullptr, *x2 = nullptr, *wt = nullptr;
     double a1 = 1.0;                       // the first step's scaling of u1 (cheb_solve.cc:67-68)
     int s1_skip = 1;                       // a1 == 1: u1 unscaled
     double ma2 = 0.0, opb2 = 0.0;
